@@ -1,0 +1,234 @@
+"""Benchmark of the FP64 fvm_eqs! right-hand side on one MI355X.
+
+Workload (the reference's throughput configuration): a 2000 x 2000-point grid on
+[0, 1]^2 (4,000,000 nodes), interior points jittered by U(-h/4, h/4) per axis
+(SplitMix64, seed 20261202) and Delaunay re-triangulated (7,992,002 triangles),
+homogeneous Dirichlet boundary, D(u) = 0.1 + u^2, S(u) = u (1 - u), u ~ U[0, 1)
+i.i.d.  One step = one fvm_eqs!(du, u, p, 0) over the whole mesh with u and du
+resident in HBM.  Throughput = node updates per second.
+
+Printed JSON (one line): the metric, the roofline of the RHS kernels from HIP
+events on the launch stream, the VJP/RHS cost ratio on the 1000 x 1000 nodal-
+log-D mesh, a copy-kernel bandwidth, and the CPU baseline: the oracle's OpenMP
+restatement of parallel_fvm_eqs! timed on a bounded number of evaluations.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "finitevolumemethod.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "FP64 fvm_eqs! Gnode-updates/s @4M nodes (%HBM peak); VJP/RHS ratio"
+HBM_PEAK = 8.0e12  # B/s, MI355X
+SEED = 20261202
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def canonical_bytes(n_nodes, n_tri, n_bedges, per_tri=204, per_node=25, per_bedge=44):
+    """Algorithmic bytes of one RHS: every array of the reference's data model read
+    once and du written once (triangle record 204 B, node 25 B, boundary edge 44 B)."""
+    return per_tri * n_tri + per_node * n_nodes + per_bedge * n_bedges
+
+
+def build_throughput_problem(n_side):
+    import finitevolumemethod as FVM
+
+    t0 = time.time()
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n_side, n_side, jitter=0.25, seed=SEED)
+    log(f"mesh {tri.num_points()} nodes {len(tri.triangles)} triangles ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    mesh = FVM.FVMGeometry(tri)
+    log(f"FVMGeometry on device: {mesh.info.n_tiles} tiles, {mesh.info.n_fix_nodes} fix nodes "
+        f"({time.time() - t0:.1f}s)")
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    u = np.random.default_rng(SEED + 1).random(tri.num_points())
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 0.1 + u ** 2,
+                          source_function=lambda x, y, t, u, p: u * (1 - u),
+                          initial_condition=u, final_time=1.0)
+    prob.device_context()
+    return tri, mesh, prob, u
+
+
+def rhs_timed(ctx, d_u, d_du):
+    from finitevolumemethod import _lib
+
+    a, b = ctypes.c_float(), ctypes.c_float()
+    _lib.check(_lib.hip_lib().fvm_rhs_timed(ctx.handle, d_u.ptr, d_du.ptr, 0.0, None,
+                                            ctypes.byref(a), ctypes.byref(b)), "fvm_rhs_timed")
+    return a.value, b.value
+
+
+def copy_bandwidth(n=1 << 27, reps=20):
+    """A plain copy kernel (out = 1.0 * x) of n doubles: bytes moved / time."""
+    from finitevolumemethod import _lib
+
+    x = _lib.DeviceArray(n)
+    y = _lib.DeviceArray(n)
+    x.fill_zero()
+    _lib.lincomb(y, (1.0,), (x,))
+    tm = _lib.Timer()
+    tm.start()
+    for _ in range(reps):
+        _lib.lincomb(y, (1.0,), (x,))
+    ms = tm.stop()
+    return 2 * 8 * n * reps / (ms * 1e-3)
+
+
+def vjp_ratio(n_side, reps=20):
+    """Time fvm_rhs and fvm_vjp (u and log D) on the nodal-log-D workload."""
+    import finitevolumemethod as FVM
+    from finitevolumemethod import _lib
+
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n_side, n_side, jitter=0.25, seed=SEED + 7)
+    mesh = FVM.FVMGeometry(tri)
+    rng = np.random.default_rng(SEED + 8)
+    logd = np.log(0.1) + 0.3 * rng.standard_normal(tri.num_points())
+    u = rng.random(tri.num_points())
+    w = rng.standard_normal(tri.num_points())
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=FVM.NodalDiffusion(logd),
+                          initial_condition=u, final_time=0.1)
+    ctx = prob.device_context()
+    d_u, d_w = _lib.DeviceArray.from_host(u), _lib.DeviceArray.from_host(w)
+    d_du, d_ub, d_lb = (_lib.DeviceArray(len(u)) for _ in range(3))
+    for _ in range(3):
+        ctx.rhs(d_u, d_du)
+        ctx.vjp(d_u, d_w, d_ub, d_lb)
+    tm = _lib.Timer()
+    tm.start()
+    for _ in range(reps):
+        ctx.rhs(d_u, d_du)
+    t_rhs = tm.stop() / reps
+    tm.start()
+    for _ in range(reps):
+        ctx.vjp(d_u, d_w, d_ub, d_lb)
+    t_vjp = tm.stop() / reps
+    return dict(nodes=int(tri.num_points()), triangles=int(len(tri.triangles)),
+                rhs_ms=round(t_rhs, 4), vjp_ms=round(t_vjp, 4), ratio=round(t_vjp / t_rhs, 3))
+
+
+def cpu_baseline(tri, u, budget_s, nthreads):
+    """The oracle's restatement of parallel_fvm_eqs! (per-thread du columns, serial
+    combine) on the same mesh and state, for about budget_s seconds."""
+    from oracle import OracleProblem
+
+    from finitevolumemethod.conditions import Conditions  # noqa: F401
+
+    N = tri.num_points()
+    kind = np.zeros(N, np.int8)
+    fidx = np.full(N, -1, np.int32)
+    bn = tri.boundary_nodes()
+    kind[bn], fidx[bn] = 1, 0
+    bneu = np.full(len(tri.boundary_edges), -1, np.int32)
+    t0 = time.time()
+    orc = OracleProblem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                        kind, fidx, bneu,
+                        dict(builtin=dict(flux_kind=0, dpoly=[0.1, 0, 1, 0], spoly=[0, 1, -1, 0],
+                                          fn=([0.0], [0.0], [0.0]))))
+    log(f"oracle geometry {time.time() - t0:.1f}s")
+    orc.rhs(u, 0.0, nthreads=nthreads)  # warm-up
+    reps, t_total = 0, 0.0
+    while t_total < budget_s and reps < 200:
+        t0 = time.perf_counter()
+        du = orc.rhs(u, 0.0, nthreads=nthreads)
+        t_total += time.perf_counter() - t0
+        reps += 1
+    return du, dict(value=round(N * reps / t_total / 1e9, 5), unit="Gnode-updates/s",
+                    cores=nthreads, kind="port",
+                    sample=f"{reps} full evaluations of the same {N}-node mesh "
+                           f"({t_total:.1f} s, OpenMP restatement of parallel_fvm_eqs!, "
+                           f"{nthreads} threads)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n-side", type=int, default=2000)
+    ap.add_argument("--vjp-side", type=int, default=1000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the finite-volume path runs on a single MI355X here "
+              "(multi-GPU sharding is design-only); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import finitevolumemethod as FVM
+    from finitevolumemethod import _lib
+
+    _lib.require_gpu()
+    tri, mesh, prob, u = build_throughput_problem(args.n_side)
+    ctx = prob.device_context()
+    N, T, B = tri.num_points(), len(tri.triangles), len(tri.boundary_edges)
+    d_u = _lib.DeviceArray.from_host(u)
+    d_du = _lib.DeviceArray(N)
+    for _ in range(args.warmup):
+        ctx.rhs(d_u, d_du)
+    _lib.synchronize()
+    tm = _lib.Timer()
+    tm.start()
+    for _ in range(args.steps):
+        ctx.rhs(d_u, d_du)
+    ms_total = tm.stop()
+    _lib.synchronize()
+    ms_step = ms_total / args.steps
+    value = N * args.steps / (ms_total * 1e-3) / 1e9
+    # per-kernel durations (HIP events on the launch stream)
+    tiles, fix = [], []
+    for _ in range(max(5, min(50, args.steps))):
+        a, b = rhs_timed(ctx, d_u, d_du)
+        tiles.append(a)
+        fix.append(b)
+    ms_tiles, ms_fix = float(np.mean(tiles)), float(np.mean(fix))
+    alg = canonical_bytes(N, T, B)
+    achieved = alg / ((ms_tiles + ms_fix) * 1e-3)
+    du_gpu = d_du.to_host()
+    log(f"RHS {ms_step * 1e3:.1f} us/step ({value:.3f} Gnode/s); tiles {ms_tiles * 1e3:.1f} us, "
+        f"fix-up {ms_fix * 1e3:.1f} us; {achieved / 1e9:.0f} GB/s algorithmic")
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "Gnode-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "fvm_eqs! nonlinear diffusion D(u)=0.1+u^2, S=u(1-u), "
+                               f"{args.n_side}x{args.n_side} jittered Delaunay mesh on [0,1]^2, "
+                               "homogeneous Dirichlet",
+                   "nodes": N, "triangles": T, "boundary_edges": B,
+                   "tiles": int(mesh.info.n_tiles), "fix_nodes": int(mesh.info.n_fix_nodes)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "kernels": {"tile_kernel_ms": round(ms_tiles, 5),
+                                 "fix_kernel_ms": round(ms_fix, 5)},
+                     "algorithmic_bytes_per_rhs": int(alg)},
+    }
+    if not args.no_extras:
+        out["copy_bandwidth_GBps"] = round(copy_bandwidth() / 1e9, 1)
+        log(f"copy kernel {out['copy_bandwidth_GBps']} GB/s")
+        out["vjp_rhs"] = vjp_ratio(args.vjp_side)
+        log(f"VJP/RHS {out['vjp_rhs']}")
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        du_cpu, cb = cpu_baseline(tri, u, args.cpu_seconds, threads)
+        out["cpu_baseline"] = cb
+        err = float(np.max(np.abs(du_gpu - du_cpu)) / np.max(np.abs(du_cpu)))
+        out["parity_max_rel_err_vs_cpu"] = err
+        log(f"CPU baseline {cb['value']} Gnode/s on {threads} threads; max rel err {err:.2e}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

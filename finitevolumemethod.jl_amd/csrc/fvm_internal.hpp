@@ -1,0 +1,135 @@
+// Internal definitions shared by the libfvmhip.so translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fvm_hip.h"
+#include "fvm_host.h"
+
+namespace fvm {
+
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define FVM_HIP(call)                                        \
+  do {                                                       \
+    hipError_t e_ = (call);                                  \
+    if (e_ != hipSuccess) return ::fvm::hip_fail(e_, #call); \
+  } while (0)
+
+#define FVM_CHECK(cond, msg)          \
+  do {                                \
+    if (!(cond)) {                    \
+      ::fvm::set_error(msg);          \
+      return FVM_ERR_ARG;             \
+    }                                 \
+  } while (0)
+
+// Number of XCDs of an MI355X; consecutive workgroups are dispatched round-robin
+// over them, each XCD with its own L2.
+constexpr int kXCD = 8;
+
+// Maps workgroup id to a work index so that each XCD processes one contiguous
+// range of work items (neighbouring tiles share nodes, hence L2 lines).  The
+// grid must hold kXCD * ceil(n / kXCD) workgroups; ids >= n are idle.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  int per = nblocks / kXCD;
+  return (bid % kXCD) * per + bid / kXCD;
+}
+inline int xcd_grid(int n) { return kXCD * ((n + kXCD - 1) / kXCD); }
+
+// Per-triangle record in the device SoA: field f of sorted triangle s lives at
+// props[f * T + s].  Fields follow TriangleProperties (geometry.jl:21-26).
+enum : int {
+  P_S1 = 0,   // s1..s9 at 0..8
+  P_MX = 9,   // CV-edge midpoints m1x m1y m2x m2y m3x m3y at 9..14
+  P_NX = 15,  // normals n1x n1y n2x n2y n3x n3y at 15..20
+  P_L = 21,   // lengths l1 l2 l3 at 21..23
+  P_COUNT = 24
+};
+
+// Condition functions a(x,y,t,u) = c0 + cu u + ct t + cx x + cy y + g.
+struct CondTable {
+  int32_t n;
+  const double* c0;
+  const double* cu;
+  const double* ct;
+  const double* cx;
+  const double* cy;
+  const double* node_g;   // may be null
+  const double* bedge_g;  // may be null
+};
+
+__device__ __forceinline__ double eval_cond(const CondTable& C, int f, double x, double y,
+                                            double t, double u, double g) {
+  double v = C.c0[f] + C.cu[f] * u;
+  v = v + C.ct[f] * t;
+  v = v + C.cx[f] * x;
+  v = v + C.cy[f] * y;
+  return v + g;
+}
+
+}  // namespace fvm
+
+// Device-resident mesh: geometry SoA in tile (sorted) triangle order plus the
+// tile / fix-up structures from fvmh_tiles_build.
+struct fvm_mesh {
+  int64_t N = 0, T = 0, B = 0;
+  int32_t n_tiles = 0, n_fix = 0, n_parts = 0, tile_tris = 0, max_nodes = 0;
+  int64_t n_local = 0;
+  // device arrays
+  double *x = nullptr, *y = nullptr, *cv = nullptr, *props = nullptr;
+  int32_t* tri = nullptr;  // [3][T] sorted order
+  int32_t *tile_tri_start = nullptr, *tile_node_start = nullptr, *tile_nodes = nullptr;
+  uint32_t* loc = nullptr;
+  int32_t* inc_start = nullptr;
+  uint16_t* inc = nullptr;
+  int32_t *part_idx = nullptr, *fix_nodes = nullptr, *fix_part_start = nullptr;
+  int32_t *fix_bedge_start = nullptr, *fix_bedge = nullptr;
+  int32_t *fix_bvert_start = nullptr, *fix_bvert = nullptr;
+  int32_t *node_tri_start = nullptr, *node_tri = nullptr;
+  int32_t *bedges = nullptr, *badj = nullptr;  // bedges [2][B]
+  int32_t* perm = nullptr;                      // sorted -> input triangle
+  uint8_t* is_vertex = nullptr;
+  // host copies needed to build contexts
+  std::vector<int32_t> h_perm;
+  std::vector<int32_t> h_node_tri_start;
+};
+
+struct fvm_ctx {
+  const fvm_mesh* m = nullptr;
+  int32_t flux_kind = 0;
+  double dpoly[4] = {0, 0, 0, 0};
+  double spoly[4] = {0, 0, 0, 0};
+  double st = 0;
+  double* d_edge = nullptr;   // [3][T] sorted order
+  double* d_bedge = nullptr;  // [2][B]
+  double* log_d_own = nullptr;
+  const double* log_d = nullptr;
+  double* s_node = nullptr;
+  int8_t* kind = nullptr;     // 0 none, 1 Dirichlet, 2 Dudt, 3 not a vertex
+  int32_t* fidx = nullptr;
+  int32_t* bneu = nullptr;    // [B]
+  double* fn = nullptr;       // 5 * n_functions: c0, cu, ct, cx, cy
+  double* node_g = nullptr;
+  double* bedge_g = nullptr;
+  int32_t n_fn = 0;
+  int32_t* dir_nodes = nullptr;
+  int32_t n_dir = 0;
+  double* partial = nullptr;   // [n_parts] (x2 for the VJP)
+  fvm::CondTable cond() const {
+    fvm::CondTable C;
+    C.n = n_fn;
+    C.c0 = fn;
+    C.cu = fn + n_fn;
+    C.ct = fn + 2 * n_fn;
+    C.cx = fn + 3 * n_fn;
+    C.cy = fn + 4 * n_fn;
+    C.node_g = node_g;
+    C.bedge_g = bedge_g;
+    return C;
+  }
+};

@@ -1,0 +1,861 @@
+// fvm_eqs! on the device: RHS, Jacobian-vector and vector-Jacobian products.
+//
+// Reference: src/equations/*.jl (main_equations.jl:28-81 is the entry point).
+// Layout: triangles are stored in tiles (fvm_host.h).  One workgroup per tile:
+//   1. issue the loads of its triangles' geometry (SoA, coalesced) into registers
+//      and gather u (and the nodal field) of its local nodes into LDS;
+//   2. compute the three CV-edge fluxes of every triangle into LDS;
+//   3. every local node sums its incident triangles' contributions in a fixed
+//      order; a node owned by this tile alone is finished here (divide by the
+//      control volume, add the source, apply the node's condition) and written
+//      once; a node shared with other tiles or the boundary writes a partial.
+// A second, small kernel finishes the shared nodes: partials in tile order, then
+// the boundary-edge fluxes (boundary_edge_contributions.jl), then the node rule
+// (source_contributions.jl).  Every output is written by exactly one thread,
+// with no atomics, so results are bitwise reproducible run to run.
+//
+// Arithmetic follows the reference's operation order with FMA contraction off
+// (-ffp-contract=off), so each triangle's fluxes equal the CPU restatement's;
+// only the order of the per-node sums differs.
+#include <vector>
+
+#include "fvm_internal.hpp"
+
+using namespace fvm;
+
+namespace {
+
+enum { M_RHS = 0, M_JVP = 1 };
+constexpr int kBlock = 256;
+constexpr int kMaxNodes = 1024;
+
+struct KArgs {
+  int32_t n_tiles, n_fix;
+  int64_t T, B;
+  const int32_t* tile_tri_start;
+  const int32_t* tile_node_start;
+  const int32_t* tile_nodes;
+  const uint32_t* loc;
+  const int32_t* inc_start;
+  const uint16_t* inc;
+  const int32_t* part_idx;
+  const int32_t* fix_nodes;
+  const int32_t* fix_part_start;
+  const int32_t* fix_bedge_start;
+  const int32_t* fix_bedge;
+  const int32_t* fix_bvert_start;
+  const int32_t* fix_bvert;
+  const double* props;
+  const int32_t* tri;
+  const double* x;
+  const double* y;
+  const double* cv;
+  const int32_t* bedges;
+  const int32_t* badj;
+  const int8_t* code;  // 0 none, 1 Dirichlet, 2 Dudt, 3 not a vertex
+  const int32_t* fidx;
+  const int32_t* bneu;
+  double d0, d1, d2, d3;
+  const double* d_edge;   // [3][T] sorted
+  const double* d_bedge;  // [2*B] interleaved
+  const double* log_d;
+  double s0, s1, s2, s3, st;
+  const double* s_node;
+  CondTable C;
+  const double* u;
+  const double* v;  // JVP direction, or VJP cotangent
+  double* out;
+  double* out2;
+  double* partial;
+  double* partial2;
+  double t;
+};
+
+// Fields of the triangle record a flux form reads (FVM_FLUX_D_EDGE needs neither
+// gamma nor the CV-edge midpoints).
+__host__ __device__ constexpr bool needs_field(int fk, int f) {
+  return fk != FVM_FLUX_D_EDGE || (f < 6 || f >= P_NX);
+}
+
+__device__ __forceinline__ double dpoly(const KArgs& a, double u) {
+  return a.d0 + u * (a.d1 + u * (a.d2 + u * a.d3));
+}
+__device__ __forceinline__ double dpoly_du(const KArgs& a, double u) {
+  return a.d1 + u * (2.0 * a.d2 + 3.0 * a.d3 * u);
+}
+
+// S(x, y, t, u) of the source term (source_contributions.jl:2-14)
+__device__ __forceinline__ double source(const KArgs& a, int g, double u) {
+  double S = a.s0 + u * (a.s1 + u * (a.s2 + u * a.s3));
+  if (a.s_node) S = a.s_node[g] + S;
+  if (a.st != 0.0) S = S + a.st * a.t;
+  return S;
+}
+__device__ __forceinline__ double source_du(const KArgs& a, double u) {
+  return a.s1 + u * (2.0 * a.s2 + 3.0 * a.s3 * u);
+}
+
+// fvm_eqs_single_source_contribution! (source_contributions.jl:33-47) for the RHS
+// and its derivative for the JVP.
+template <int MODE>
+__device__ __forceinline__ double finish(const KArgs& a, int g, double acc, double ug, double vg) {
+  const int c = a.code[g];
+  if (c == 0) {
+    if (MODE == M_RHS) return acc / a.cv[g] + source(a, g, ug);
+    return acc / a.cv[g] + source_du(a, ug) * vg;
+  }
+  if (c == 2) {
+    const int f = a.fidx[g];
+    if (MODE == M_RHS)
+      return eval_cond(a.C, f, a.x[g], a.y[g], a.t, ug, a.C.node_g ? a.C.node_g[g] : 0.0);
+    return a.C.cu[f] * vg;
+  }
+  return 0.0;
+}
+
+// The three CV-edge fluxes Q_e = (q . n) l of one triangle (get_fluxes,
+// triangle_contributions.jl:2-7), or their directional derivatives.
+template <int FK, int MODE>
+__device__ __forceinline__ void tri_fluxes(const KArgs& a, const double* P, double ua, double ub,
+                                           double uc, double va, double vb, double vc, double Da,
+                                           double Db, double Dc, const double* Ds, double* Q) {
+  const double al = P[0] * ua + P[1] * ub + P[2] * uc;
+  const double be = P[3] * ua + P[4] * ub + P[5] * uc;
+  double ga = 0.0, dal = 0.0, dbe = 0.0, dga = 0.0, da = 0.0, db = 0.0, dg = 0.0;
+  if (FK == FVM_FLUX_D_POLY_U) ga = P[6] * ua + P[7] * ub + P[8] * uc;
+  if (MODE == M_JVP) {
+    dal = P[0] * va + P[1] * vb + P[2] * vc;
+    dbe = P[3] * va + P[4] * vb + P[5] * vc;
+    if (FK == FVM_FLUX_D_POLY_U) dga = P[6] * va + P[7] * vb + P[8] * vc;
+  }
+  if (FK == FVM_FLUX_D_NODAL) {
+    da = P[0] * Da + P[1] * Db + P[2] * Dc;
+    db = P[3] * Da + P[4] * Db + P[5] * Dc;
+    dg = P[6] * Da + P[7] * Db + P[8] * Dc;
+  }
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const double nx = P[P_NX + 2 * e], ny = P[P_NX + 1 + 2 * e], l = P[P_L + e];
+    double D, Dd = 0.0;
+    if (FK == FVM_FLUX_D_POLY_U) {
+      const double x = P[P_MX + 2 * e], y = P[P_MX + 1 + 2 * e];
+      const double us = al * x + be * y + ga;
+      D = dpoly(a, us);
+      if (MODE == M_JVP) Dd = dpoly_du(a, us) * (dal * x + dbe * y + dga);
+    } else if (FK == FVM_FLUX_D_NODAL) {
+      const double x = P[P_MX + 2 * e], y = P[P_MX + 1 + 2 * e];
+      D = da * x + db * y + dg;
+    } else {
+      D = Ds[e];
+    }
+    if (MODE == M_RHS) {
+      const double qx = -D * al, qy = -D * be;
+      const double qn = qx * nx + qy * ny;
+      Q[e] = qn * l;
+    } else {
+      const double qx = -(Dd * al + D * dal), qy = -(Dd * be + D * dbe);
+      Q[e] = (qx * nx + qy * ny) * l;
+    }
+  }
+}
+
+struct BGeo {
+  double nx, ny, mx[2], my[2], ell;
+};
+
+// get_boundary_cv_components (control_volumes.jl:41-56)
+__device__ __forceinline__ BGeo bgeo(const KArgs& a, int e) {
+  const int i = a.bedges[e], j = a.bedges[a.B + e];
+  const double px = a.x[i], py = a.y[i], qx = a.x[j], qy = a.y[j];
+  BGeo G;
+  const double lij = sqrt((qx - px) * (qx - px) + (qy - py) * (qy - py));
+  G.nx = (qy - py) / lij;
+  G.ny = -(qx - px) / lij;
+  const double mijx = (px + qx) / 2, mijy = (py + qy) / 2;
+  G.mx[0] = (px + mijx) / 2;
+  G.my[0] = (py + mijy) / 2;
+  G.mx[1] = (qx + mijx) / 2;
+  G.my[1] = (qy + mijy) / 2;
+  G.ell = sqrt((mijx - px) * (mijx - px) + (mijy - py) * (mijy - py));
+  return G;
+}
+
+// Flux through the boundary CV edge of endpoint ep of boundary edge e, times its
+// length (get_boundary_fluxes, boundary_edge_contributions.jl:2-48), or its
+// directional derivative.
+template <int FK, int MODE>
+__device__ double bedge_flux(const KArgs& a, int e, int ep) {
+  const BGeo G = bgeo(a, e);
+  const int64_t Tt = a.badj[e];
+  const int va = a.tri[Tt], vb = a.tri[a.T + Tt], vc = a.tri[2 * a.T + Tt];
+  double s[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) s[f] = a.props[f * a.T + Tt];
+  const double ua = a.u[va], ub = a.u[vb], uc = a.u[vc];
+  const double al = s[0] * ua + s[1] * ub + s[2] * uc;
+  const double be = s[3] * ua + s[4] * ub + s[5] * uc;
+  const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
+  double dal = 0, dbe = 0, dga = 0;
+  if (MODE == M_JVP) {
+    const double wa = a.v[va], wb = a.v[vb], wc = a.v[vc];
+    dal = s[0] * wa + s[1] * wb + s[2] * wc;
+    dbe = s[3] * wa + s[4] * wb + s[5] * wc;
+    dga = s[6] * wa + s[7] * wb + s[8] * wc;
+  }
+  const double mx = G.mx[ep], my = G.my[ep];
+  const int nf = a.bneu[e];
+  double qn;
+  if (nf < 0) {
+    double D, Dd = 0.0;
+    if (FK == FVM_FLUX_D_POLY_U) {
+      const double um = al * mx + be * my + ga;
+      D = dpoly(a, um);
+      if (MODE == M_JVP) Dd = dpoly_du(a, um) * (dal * mx + dbe * my + dga);
+    } else if (FK == FVM_FLUX_D_NODAL) {
+      const double Da = exp(a.log_d[va]), Db = exp(a.log_d[vb]), Dc = exp(a.log_d[vc]);
+      const double da = s[0] * Da + s[1] * Db + s[2] * Dc;
+      const double db = s[3] * Da + s[4] * Db + s[5] * Dc;
+      const double dg = s[6] * Da + s[7] * Db + s[8] * Dc;
+      D = da * mx + db * my + dg;
+    } else {
+      D = a.d_bedge[2 * e + ep];
+    }
+    if (MODE == M_RHS) {
+      const double qx = -D * al, qy = -D * be;
+      qn = qx * G.nx + qy * G.ny;
+    } else {
+      const double qx = -(Dd * al + D * dal), qy = -(Dd * be + D * dbe);
+      qn = qx * G.nx + qy * G.ny;
+    }
+  } else {
+    if (MODE == M_RHS)
+      qn = eval_cond(a.C, nf, mx, my, a.t, al * mx + be * my + ga,
+                     a.C.bedge_g ? a.C.bedge_g[2 * e + ep] : 0.0);
+    else
+      qn = a.C.cu[nf] * (dal * mx + dbe * my + dga);
+  }
+  return qn * G.ell;
+}
+
+// ---------------------------------------------------------------------------
+// RHS / JVP: tile kernel and fix-up kernel
+// ---------------------------------------------------------------------------
+template <int FK, int MODE, int TPT>
+__global__ __launch_bounds__(kBlock) void tile_kernel(KArgs a) {
+  constexpr int TILE = kBlock * TPT;
+  __shared__ double s_u[kMaxNodes];
+  __shared__ double s_v[MODE == M_JVP ? kMaxNodes : 1];
+  __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? kMaxNodes : 1];
+  __shared__ double s_q[3 * TILE];
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  if (tile >= a.n_tiles) return;
+  const int tid = threadIdx.x;
+  const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
+  const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
+
+  double P[TPT][P_COUNT];
+  double Ds[TPT][3];
+  uint32_t L[TPT];
+#pragma unroll
+  for (int r = 0; r < TPT; ++r) {
+    const int lt = tid + r * kBlock;
+    L[r] = 0;
+    if (lt < nt) {
+      const int64_t s = t0 + lt;
+      L[r] = a.loc[s];
+#pragma unroll
+      for (int f = 0; f < P_COUNT; ++f)
+        if (needs_field(FK, f)) P[r][f] = __builtin_nontemporal_load(a.props + f * a.T + s);
+      if (FK == FVM_FLUX_D_EDGE) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) Ds[r][e] = __builtin_nontemporal_load(a.d_edge + e * a.T + s);
+      }
+    }
+  }
+  for (int l = tid; l < nn; l += kBlock) {
+    const int g = a.tile_nodes[n0 + l];
+    s_u[l] = a.u[g];
+    if (MODE == M_JVP) s_v[l] = a.v[g];
+    if (FK == FVM_FLUX_D_NODAL) s_D[l] = exp(a.log_d[g]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < TPT; ++r) {
+    const int lt = tid + r * kBlock;
+    if (lt < nt) {
+      const int ia = L[r] & 1023u, ib = (L[r] >> 10) & 1023u, ic = (L[r] >> 20) & 1023u;
+      double Q[3];
+      tri_fluxes<FK, MODE>(a, P[r], s_u[ia], s_u[ib], s_u[ic],
+                           MODE == M_JVP ? s_v[ia] : 0.0, MODE == M_JVP ? s_v[ib] : 0.0,
+                           MODE == M_JVP ? s_v[ic] : 0.0,
+                           FK == FVM_FLUX_D_NODAL ? s_D[ia] : 0.0,
+                           FK == FVM_FLUX_D_NODAL ? s_D[ib] : 0.0,
+                           FK == FVM_FLUX_D_NODAL ? s_D[ic] : 0.0, Ds[r], Q);
+      s_q[3 * lt] = Q[0];
+      s_q[3 * lt + 1] = Q[1];
+      s_q[3 * lt + 2] = Q[2];
+    }
+  }
+  __syncthreads();
+  for (int l = tid; l < nn; l += kBlock) {
+    double acc = 0.0;
+    const int k0 = a.inc_start[n0 + l], k1 = a.inc_start[n0 + l + 1];
+    for (int k = k0; k < k1; ++k) {
+      const int w = a.inc[k];
+      const int lt = w >> 2, sl = w & 3;
+      // update_du! (triangle_contributions.jl:10-15): i += Q3 - Q1, j += Q1 - Q2, k += Q2 - Q3
+      acc = acc + s_q[3 * lt + (sl == 0 ? 2 : sl - 1)] - s_q[3 * lt + sl];
+    }
+    const int pi = a.part_idx[n0 + l];
+    if (pi >= 0) {
+      a.partial[pi] = acc;
+    } else {
+      const int g = a.tile_nodes[n0 + l];
+      a.out[g] = finish<MODE>(a, g, acc, s_u[l], MODE == M_JVP ? s_v[l] : 0.0);
+    }
+  }
+}
+
+template <int FK, int MODE>
+__global__ __launch_bounds__(256) void fix_kernel(KArgs a) {
+  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fi >= a.n_fix) return;
+  const int g = a.fix_nodes[fi];
+  double acc = 0.0;
+  for (int p = a.fix_part_start[fi]; p < a.fix_part_start[fi + 1]; ++p) acc = acc + a.partial[p];
+  for (int k = a.fix_bedge_start[fi]; k < a.fix_bedge_start[fi + 1]; ++k) {
+    const int w = a.fix_bedge[k];
+    acc = acc - bedge_flux<FK, MODE>(a, w >> 1, w & 1);
+  }
+  a.out[g] = finish<MODE>(a, g, acc, a.u[g], MODE == M_JVP ? a.v[g] : 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// VJP
+// ---------------------------------------------------------------------------
+// w / V at nodes whose du carries the fluxes, 0 elsewhere.
+__device__ __forceinline__ double wt_of(const KArgs& a, int g) {
+  return a.code[g] == 0 ? a.v[g] / a.cv[g] : 0.0;
+}
+// Adjoint of the node rule: w_g dS/du (free nodes) or w_g da/du (Dudt nodes).
+__device__ __forceinline__ double node_adj(const KArgs& a, int g, double ug) {
+  const int c = a.code[g];
+  if (c == 0) return a.v[g] * source_du(a, ug);
+  if (c == 2) return a.v[g] * a.C.cu[a.fidx[g]];
+  return 0.0;
+}
+
+// Adjoint of one CV-edge flux Q = -l D (alpha nx + beta ny) with upstream Qbar:
+// accumulates alpha/beta/gamma adjoints and the adjoint of D.
+template <int FK>
+__device__ __forceinline__ void edge_adj(const KArgs& a, double Qbar, double x, double y, double nx,
+                                         double ny, double l, double al, double be, double ga,
+                                         double D_given, double& abar, double& bbar, double& gbar,
+                                         double& Dbar) {
+  const double gn = al * nx + be * ny;
+  double D, Du = 0.0;
+  if (FK == FVM_FLUX_D_POLY_U) {
+    const double us = al * x + be * y + ga;
+    D = dpoly(a, us);
+    Du = dpoly_du(a, us);
+  } else {
+    D = D_given;
+  }
+  abar += Qbar * (-l * (D * nx + Du * x * gn));
+  bbar += Qbar * (-l * (D * ny + Du * y * gn));
+  gbar += Qbar * (-l * Du * gn);
+  Dbar += Qbar * (-l * gn);
+}
+
+template <int FK, int TPT>
+__global__ __launch_bounds__(kBlock) void vjp_tile_kernel(KArgs a) {
+  constexpr int TILE = kBlock * TPT;
+  __shared__ double s_u[kMaxNodes];
+  __shared__ double s_w[kMaxNodes];
+  __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? kMaxNodes : 1];
+  __shared__ double s_c[6 * TILE];
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  if (tile >= a.n_tiles) return;
+  const int tid = threadIdx.x;
+  const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
+  const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
+  double P[TPT][P_COUNT];
+  double Ds[TPT][3];
+  uint32_t L[TPT];
+#pragma unroll
+  for (int r = 0; r < TPT; ++r) {
+    const int lt = tid + r * kBlock;
+    L[r] = 0;
+    if (lt < nt) {
+      const int64_t s = t0 + lt;
+      L[r] = a.loc[s];
+#pragma unroll
+      for (int f = 0; f < P_COUNT; ++f)
+        if (needs_field(FK, f)) P[r][f] = __builtin_nontemporal_load(a.props + f * a.T + s);
+      if (FK == FVM_FLUX_D_EDGE) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) Ds[r][e] = __builtin_nontemporal_load(a.d_edge + e * a.T + s);
+      }
+    }
+  }
+  for (int l = tid; l < nn; l += kBlock) {
+    const int g = a.tile_nodes[n0 + l];
+    s_u[l] = a.u[g];
+    s_w[l] = wt_of(a, g);
+    if (FK == FVM_FLUX_D_NODAL) s_D[l] = exp(a.log_d[g]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < TPT; ++r) {
+    const int lt = tid + r * kBlock;
+    if (lt < nt) {
+      const double* p = P[r];
+      const int id[3] = {(int)(L[r] & 1023u), (int)((L[r] >> 10) & 1023u), (int)((L[r] >> 20) & 1023u)};
+      const double ua = s_u[id[0]], ub = s_u[id[1]], uc = s_u[id[2]];
+      const double al = p[0] * ua + p[1] * ub + p[2] * uc;
+      const double be = p[3] * ua + p[4] * ub + p[5] * uc;
+      const double ga = FK == FVM_FLUX_D_POLY_U ? p[6] * ua + p[7] * ub + p[8] * uc : 0.0;
+      double Dn[3] = {0, 0, 0}, da = 0, db = 0, dg = 0;
+      if (FK == FVM_FLUX_D_NODAL) {
+        Dn[0] = s_D[id[0]];
+        Dn[1] = s_D[id[1]];
+        Dn[2] = s_D[id[2]];
+        da = p[0] * Dn[0] + p[1] * Dn[1] + p[2] * Dn[2];
+        db = p[3] * Dn[0] + p[4] * Dn[1] + p[5] * Dn[2];
+        dg = p[6] * Dn[0] + p[7] * Dn[1] + p[8] * Dn[2];
+      }
+      double abar = 0, bbar = 0, gbar = 0, lbar[3] = {0, 0, 0};
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        // du_e1 -= Q_e, du_e2 += Q_e  ->  Qbar = w~_e2 - w~_e1
+        const double Qbar = s_w[id[(e + 1) % 3]] - s_w[id[e]];
+        const double nx = p[P_NX + 2 * e], ny = p[P_NX + 1 + 2 * e], l = p[P_L + e];
+        double x = 0, y = 0, Dg = 0;
+        if (FK != FVM_FLUX_D_EDGE) {
+          x = p[P_MX + 2 * e];
+          y = p[P_MX + 1 + 2 * e];
+        }
+        if (FK == FVM_FLUX_D_NODAL) Dg = da * x + db * y + dg;
+        if (FK == FVM_FLUX_D_EDGE) Dg = Ds[r][e];
+        double Dbar = 0;
+        edge_adj<FK>(a, Qbar, x, y, nx, ny, l, al, be, ga, Dg, abar, bbar, gbar, Dbar);
+        if (FK == FVM_FLUX_D_NODAL) {
+#pragma unroll
+          for (int v = 0; v < 3; ++v) lbar[v] += Dbar * (p[v] * x + p[3 + v] * y + p[6 + v]);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 3; ++v) {
+        double ub = p[v] * abar + p[3 + v] * bbar;
+        if (FK == FVM_FLUX_D_POLY_U) ub = ub + p[6 + v] * gbar;  // gamma only enters D(u)
+        s_c[6 * lt + v] = ub;
+        s_c[6 * lt + 3 + v] = lbar[v] * Dn[v];
+      }
+    }
+  }
+  __syncthreads();
+  for (int l = tid; l < nn; l += kBlock) {
+    double acc = 0.0, accl = 0.0;
+    const int k0 = a.inc_start[n0 + l], k1 = a.inc_start[n0 + l + 1];
+    for (int k = k0; k < k1; ++k) {
+      const int w = a.inc[k];
+      acc += s_c[6 * (w >> 2) + (w & 3)];
+      accl += s_c[6 * (w >> 2) + 3 + (w & 3)];
+    }
+    const int pi = a.part_idx[n0 + l];
+    if (pi >= 0) {
+      a.partial[pi] = acc;
+      a.partial2[pi] = accl;
+    } else {
+      const int g = a.tile_nodes[n0 + l];
+      a.out[g] = acc + node_adj(a, g, s_u[l]);
+      if (a.out2) a.out2[g] = accl;
+    }
+  }
+}
+
+// Adjoint of both boundary CV-edge fluxes of boundary edge e with respect to the
+// vertex in slot `slot` of its triangle.
+template <int FK>
+__device__ void bedge_adj(const KArgs& a, int e, int slot, double& ubar, double& lbar) {
+  const BGeo G = bgeo(a, e);
+  const int64_t Tt = a.badj[e];
+  const int vtx[3] = {a.tri[Tt], a.tri[a.T + Tt], a.tri[2 * a.T + Tt]};
+  double s[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) s[f] = a.props[f * a.T + Tt];
+  const double ua = a.u[vtx[0]], ub = a.u[vtx[1]], uc = a.u[vtx[2]];
+  const double al = s[0] * ua + s[1] * ub + s[2] * uc;
+  const double be = s[3] * ua + s[4] * ub + s[5] * uc;
+  const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
+  double Dn[3] = {0, 0, 0}, da = 0, db = 0, dg = 0;
+  if (FK == FVM_FLUX_D_NODAL) {
+    for (int v = 0; v < 3; ++v) Dn[v] = exp(a.log_d[vtx[v]]);
+    da = s[0] * Dn[0] + s[1] * Dn[1] + s[2] * Dn[2];
+    db = s[3] * Dn[0] + s[4] * Dn[1] + s[5] * Dn[2];
+    dg = s[6] * Dn[0] + s[7] * Dn[1] + s[8] * Dn[2];
+  }
+  const int nf = a.bneu[e];
+  double abar = 0, bbar = 0, gbar = 0, lb = 0;
+  for (int ep = 0; ep < 2; ++ep) {
+    const int node = a.bedges[ep * a.B + e];
+    const double Qbar = -wt_of(a, node);  // du_node -= Q
+    const double mx = G.mx[ep], my = G.my[ep];
+    if (nf < 0) {
+      double Dg = 0, Dbar = 0;
+      if (FK == FVM_FLUX_D_NODAL) Dg = da * mx + db * my + dg;
+      if (FK == FVM_FLUX_D_EDGE) Dg = a.d_bedge[2 * e + ep];
+      edge_adj<FK>(a, Qbar, mx, my, G.nx, G.ny, G.ell, al, be, ga, Dg, abar, bbar, gbar, Dbar);
+      if (FK == FVM_FLUX_D_NODAL) lb += Dbar * (s[slot] * mx + s[3 + slot] * my + s[6 + slot]);
+    } else {
+      const double c = Qbar * G.ell * a.C.cu[nf];
+      abar += c * mx;
+      bbar += c * my;
+      gbar += c;
+    }
+  }
+  ubar += s[slot] * abar + s[3 + slot] * bbar + s[6 + slot] * gbar;
+  lbar += lb * Dn[slot];
+}
+
+template <int FK>
+__global__ __launch_bounds__(256) void vjp_fix_kernel(KArgs a) {
+  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fi >= a.n_fix) return;
+  const int g = a.fix_nodes[fi];
+  double acc = 0.0, accl = 0.0;
+  for (int p = a.fix_part_start[fi]; p < a.fix_part_start[fi + 1]; ++p) {
+    acc += a.partial[p];
+    accl += a.partial2[p];
+  }
+  for (int k = a.fix_bvert_start[fi]; k < a.fix_bvert_start[fi + 1]; ++k) {
+    const int w = a.fix_bvert[k];
+    bedge_adj<FK>(a, w >> 2, w & 3, acc, accl);
+  }
+  a.out[g] = acc + node_adj(a, g, a.u[g]);
+  if (a.out2) a.out2[g] = accl;
+}
+
+__global__ __launch_bounds__(256) void dirichlet_kernel(int n, const int32_t* __restrict__ nodes,
+                                                        const int32_t* __restrict__ fidx,
+                                                        const double* __restrict__ X,
+                                                        const double* __restrict__ Y, CondTable C,
+                                                        double t, double* __restrict__ u) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int g = nodes[k];
+  u[g] = eval_cond(C, fidx[g], X[g], Y[g], t, u[g], C.node_g ? C.node_g[g] : 0.0);
+}
+
+KArgs make_args(const fvm_ctx* c) {
+  const fvm_mesh* m = c->m;
+  KArgs a;
+  a.n_tiles = m->n_tiles;
+  a.n_fix = m->n_fix;
+  a.T = m->T;
+  a.B = m->B;
+  a.tile_tri_start = m->tile_tri_start;
+  a.tile_node_start = m->tile_node_start;
+  a.tile_nodes = m->tile_nodes;
+  a.loc = m->loc;
+  a.inc_start = m->inc_start;
+  a.inc = m->inc;
+  a.part_idx = m->part_idx;
+  a.fix_nodes = m->fix_nodes;
+  a.fix_part_start = m->fix_part_start;
+  a.fix_bedge_start = m->fix_bedge_start;
+  a.fix_bedge = m->fix_bedge;
+  a.fix_bvert_start = m->fix_bvert_start;
+  a.fix_bvert = m->fix_bvert;
+  a.props = m->props;
+  a.tri = m->tri;
+  a.x = m->x;
+  a.y = m->y;
+  a.cv = m->cv;
+  a.bedges = m->bedges;
+  a.badj = m->badj;
+  a.code = c->kind;
+  a.fidx = c->fidx;
+  a.bneu = c->bneu;
+  a.d0 = c->dpoly[0];
+  a.d1 = c->dpoly[1];
+  a.d2 = c->dpoly[2];
+  a.d3 = c->dpoly[3];
+  a.d_edge = c->d_edge;
+  a.d_bedge = c->d_bedge;
+  a.log_d = c->log_d;
+  a.s0 = c->spoly[0];
+  a.s1 = c->spoly[1];
+  a.s2 = c->spoly[2];
+  a.s3 = c->spoly[3];
+  a.st = c->st;
+  a.s_node = c->s_node;
+  a.C = c->cond();
+  a.u = a.v = nullptr;
+  a.out = a.out2 = nullptr;
+  a.partial = c->partial;
+  a.partial2 = c->partial + (m->n_parts > 0 ? m->n_parts : 1);
+  a.t = 0;
+  return a;
+}
+
+template <int FK, int MODE>
+int launch_fk(const KArgs& a, int tile_tris, hipStream_t s) {
+  const dim3 grid((unsigned)xcd_grid(a.n_tiles));
+  if (tile_tris <= kBlock)
+    hipLaunchKernelGGL((tile_kernel<FK, MODE, 1>), grid, dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((tile_kernel<FK, MODE, 2>), grid, dim3(kBlock), 0, s, a);
+  if (a.n_fix > 0)
+    hipLaunchKernelGGL((fix_kernel<FK, MODE>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0,
+                       s, a);
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+template <int MODE>
+int launch(const fvm_ctx* c, const KArgs& a, hipStream_t s) {
+  switch (c->flux_kind) {
+    case FVM_FLUX_D_POLY_U: return launch_fk<FVM_FLUX_D_POLY_U, MODE>(a, c->m->tile_tris, s);
+    case FVM_FLUX_D_NODAL: return launch_fk<FVM_FLUX_D_NODAL, MODE>(a, c->m->tile_tris, s);
+    default: return launch_fk<FVM_FLUX_D_EDGE, MODE>(a, c->m->tile_tris, s);
+  }
+}
+
+template <int FK>
+int launch_vjp_fk(const KArgs& a, int tile_tris, hipStream_t s) {
+  const dim3 grid((unsigned)xcd_grid(a.n_tiles));
+  if (tile_tris <= kBlock)
+    hipLaunchKernelGGL((vjp_tile_kernel<FK, 1>), grid, dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((vjp_tile_kernel<FK, 2>), grid, dim3(kBlock), 0, s, a);
+  if (a.n_fix > 0)
+    hipLaunchKernelGGL((vjp_fix_kernel<FK>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0, s,
+                       a);
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+template <class T>
+int dev_copy(T** dst, const T* src, int64_t n) {
+  *dst = nullptr;
+  FVM_HIP(hipMalloc((void**)dst, sizeof(T) * (size_t)(n > 0 ? n : 1)));
+  if (n > 0 && src) FVM_HIP(hipMemcpy(*dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice));
+  return FVM_OK;
+}
+
+void free_ctx(fvm_ctx* c) {
+  if (!c) return;
+  void* ptrs[] = {c->d_edge, c->d_bedge, c->log_d_own, c->s_node, c->kind, c->fidx,
+                  c->bneu, c->fn, c->node_g, c->bedge_g, c->dir_nodes, c->partial};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete c;
+}
+
+int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
+  const int64_t N = m->N, T = m->T, B = m->B;
+  FVM_CHECK(ph->flux_kind >= FVM_FLUX_D_POLY_U && ph->flux_kind <= FVM_FLUX_D_EDGE,
+            "fvm_ctx_create: unknown flux_kind");
+  FVM_CHECK(ph->node_kind, "fvm_ctx_create: node_kind is required");
+  FVM_CHECK(ph->node_fidx, "fvm_ctx_create: node_fidx is required");
+  FVM_CHECK(B == 0 || ph->bedge_neumann_fidx, "fvm_ctx_create: bedge_neumann_fidx is required");
+  FVM_CHECK(ph->n_functions >= 0, "fvm_ctx_create: negative n_functions");
+  FVM_CHECK(ph->n_functions == 0 || (ph->fn_c0 && ph->fn_cu && ph->fn_ct),
+            "fvm_ctx_create: condition function coefficients missing");
+  if (ph->flux_kind == FVM_FLUX_D_EDGE)
+    FVM_CHECK(ph->d_edge && (B == 0 || ph->d_bedge), "fvm_ctx_create: d_edge / d_bedge required");
+  if (ph->flux_kind == FVM_FLUX_D_NODAL) FVM_CHECK(ph->log_d, "fvm_ctx_create: log_d required");
+  c->m = m;
+  c->flux_kind = ph->flux_kind;
+  for (int k = 0; k < 4; ++k) {
+    c->dpoly[k] = ph->dpoly[k];
+    c->spoly[k] = ph->spoly[k];
+  }
+  c->st = ph->st;
+  c->n_fn = ph->n_functions;
+  // node codes: 3 marks points that are not vertices (du = 0, source_contributions.jl:36-37)
+  std::vector<int8_t> code(N);
+  std::vector<int32_t> dir;
+  for (int64_t g = 0; g < N; ++g) {
+    const int k = ph->node_kind[g];
+    FVM_CHECK(k >= 0 && k <= 2, "fvm_ctx_create: node_kind entries must be 0, 1 or 2");
+    if (k != 0) {
+      const int f = ph->node_fidx[g];
+      FVM_CHECK(f >= 0 && f < ph->n_functions, "fvm_ctx_create: node_fidx out of range");
+    }
+    const bool vert = m->h_node_tri_start[g + 1] > m->h_node_tri_start[g];
+    code[g] = vert ? (int8_t)k : (int8_t)3;
+    if (k == FVM_NODE_DIRICHLET) dir.push_back((int32_t)g);
+  }
+  for (int64_t e = 0; e < B; ++e) {
+    const int f = ph->bedge_neumann_fidx[e];
+    FVM_CHECK(f == -1 || (f >= 0 && f < ph->n_functions), "fvm_ctx_create: bedge_neumann_fidx out of range");
+  }
+  int rc;
+#define DC(dst, src, n) \
+  if ((rc = dev_copy(&(dst), src, n)) != FVM_OK) return rc
+  DC(c->kind, code.data(), N);
+  DC(c->fidx, ph->node_fidx, N);
+  DC(c->bneu, ph->bedge_neumann_fidx, B);
+  {
+    const int n = ph->n_functions;
+    std::vector<double> fn(5 * (size_t)(n > 0 ? n : 1), 0.0);
+    for (int f = 0; f < n; ++f) {
+      fn[f] = ph->fn_c0[f];
+      fn[n + f] = ph->fn_cu[f];
+      fn[2 * n + f] = ph->fn_ct[f];
+      fn[3 * n + f] = ph->fn_cx ? ph->fn_cx[f] : 0.0;
+      fn[4 * n + f] = ph->fn_cy ? ph->fn_cy[f] : 0.0;
+    }
+    DC(c->fn, fn.data(), (int64_t)fn.size());
+  }
+  if (ph->node_g) DC(c->node_g, ph->node_g, N);
+  if (ph->bedge_g) DC(c->bedge_g, ph->bedge_g, 2 * B);
+  if (ph->s_node) DC(c->s_node, ph->s_node, N);
+  if (ph->flux_kind == FVM_FLUX_D_EDGE) {
+    std::vector<double> de(3 * (size_t)T);
+    for (int64_t s = 0; s < T; ++s)
+      for (int e = 0; e < 3; ++e) de[(size_t)e * T + s] = ph->d_edge[3 * (int64_t)m->h_perm[s] + e];
+    DC(c->d_edge, de.data(), 3 * T);
+    DC(c->d_bedge, ph->d_bedge, 2 * B);
+  }
+  if (ph->log_d) {
+    DC(c->log_d_own, ph->log_d, N);
+    c->log_d = c->log_d_own;
+  }
+  c->n_dir = (int32_t)dir.size();
+  DC(c->dir_nodes, dir.data(), (int64_t)dir.size());
+  DC(c->partial, (const double*)nullptr, 2 * (int64_t)(m->n_parts > 0 ? m->n_parts : 1));
+#undef DC
+  return FVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvm_ctx_create(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx** out) {
+  FVM_CHECK(m && ph && out, "fvm_ctx_create: null argument");
+  *out = nullptr;
+  fvm_ctx* c = new fvm_ctx();
+  int rc = build_ctx(m, ph, c);
+  if (rc != FVM_OK) {
+    free_ctx(c);
+    return rc;
+  }
+  *out = c;
+  return FVM_OK;
+}
+
+int fvm_ctx_destroy(fvm_ctx* c) {
+  free_ctx(c);
+  return FVM_OK;
+}
+
+int fvm_ctx_set_log_d(fvm_ctx* c, const double* d_log_d) {
+  FVM_CHECK(c && d_log_d, "fvm_ctx_set_log_d: null argument");
+  FVM_CHECK(c->flux_kind == FVM_FLUX_D_NODAL, "fvm_ctx_set_log_d: context has no nodal field");
+  c->log_d = d_log_d;
+  return FVM_OK;
+}
+
+int fvm_rhs(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_stream s) {
+  FVM_CHECK(c && d_u && d_du, "fvm_rhs: null argument");
+  FVM_CHECK(d_u != d_du, "fvm_rhs: du must not alias u");
+  KArgs a = make_args(c);
+  a.u = d_u;
+  a.out = d_du;
+  a.t = t;
+  return launch<M_RHS>(c, a, (hipStream_t)s);
+}
+
+int fvm_rhs_timed(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_stream s,
+                  float* ms_tiles, float* ms_fixup) {
+  FVM_CHECK(c && d_u && d_du && ms_tiles && ms_fixup, "fvm_rhs_timed: null argument");
+  FVM_CHECK(d_u != d_du, "fvm_rhs_timed: du must not alias u");
+  KArgs a = make_args(c);
+  a.u = d_u;
+  a.out = d_du;
+  a.t = t;
+  hipStream_t st = (hipStream_t)s;
+  hipEvent_t ev[3];
+  for (int k = 0; k < 3; ++k) FVM_HIP(hipEventCreate(&ev[k]));
+  FVM_HIP(hipEventRecord(ev[0], st));
+  int rc = FVM_OK;
+  const int tt = c->m->tile_tris;
+  const dim3 grid((unsigned)xcd_grid(a.n_tiles));
+  switch (c->flux_kind) {
+    case FVM_FLUX_D_POLY_U:
+      if (tt <= kBlock) hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_POLY_U, M_RHS, 1>), grid, dim3(kBlock), 0, st, a);
+      else hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_POLY_U, M_RHS, 2>), grid, dim3(kBlock), 0, st, a);
+      break;
+    case FVM_FLUX_D_NODAL:
+      if (tt <= kBlock) hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_NODAL, M_RHS, 1>), grid, dim3(kBlock), 0, st, a);
+      else hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_NODAL, M_RHS, 2>), grid, dim3(kBlock), 0, st, a);
+      break;
+    default:
+      if (tt <= kBlock) hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_EDGE, M_RHS, 1>), grid, dim3(kBlock), 0, st, a);
+      else hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_EDGE, M_RHS, 2>), grid, dim3(kBlock), 0, st, a);
+  }
+  FVM_HIP(hipEventRecord(ev[1], st));
+  if (a.n_fix > 0) {
+    const dim3 fg((unsigned)((a.n_fix + 255) / 256));
+    switch (c->flux_kind) {
+      case FVM_FLUX_D_POLY_U: hipLaunchKernelGGL((fix_kernel<FVM_FLUX_D_POLY_U, M_RHS>), fg, dim3(256), 0, st, a); break;
+      case FVM_FLUX_D_NODAL: hipLaunchKernelGGL((fix_kernel<FVM_FLUX_D_NODAL, M_RHS>), fg, dim3(256), 0, st, a); break;
+      default: hipLaunchKernelGGL((fix_kernel<FVM_FLUX_D_EDGE, M_RHS>), fg, dim3(256), 0, st, a);
+    }
+  }
+  FVM_HIP(hipEventRecord(ev[2], st));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
+  if (e == hipSuccess) e = hipEventElapsedTime(ms_tiles, ev[0], ev[1]);
+  if (e == hipSuccess) e = hipEventElapsedTime(ms_fixup, ev[1], ev[2]);
+  for (int k = 0; k < 3; ++k) (void)hipEventDestroy(ev[k]);
+  if (e != hipSuccess) rc = hip_fail(e, "fvm_rhs_timed");
+  return rc;
+}
+
+int fvm_jvp(fvm_ctx* c, const double* d_u, const double* d_v, double* d_jv, double t, fvm_stream s) {
+  FVM_CHECK(c && d_u && d_v && d_jv, "fvm_jvp: null argument");
+  FVM_CHECK(d_jv != d_u && d_jv != d_v, "fvm_jvp: output must not alias inputs");
+  KArgs a = make_args(c);
+  a.u = d_u;
+  a.v = d_v;
+  a.out = d_jv;
+  a.t = t;
+  return launch<M_JVP>(c, a, (hipStream_t)s);
+}
+
+int fvm_vjp(fvm_ctx* c, const double* d_u, const double* d_w, double* d_ubar, double* d_logd_bar,
+            double t, fvm_stream s) {
+  FVM_CHECK(c && d_u && d_w && d_ubar, "fvm_vjp: null argument");
+  FVM_CHECK(d_ubar != d_u && d_ubar != d_w, "fvm_vjp: output must not alias inputs");
+  FVM_CHECK(d_logd_bar == nullptr || c->flux_kind == FVM_FLUX_D_NODAL,
+            "fvm_vjp: d_logd_bar needs a FVM_FLUX_D_NODAL context");
+  KArgs a = make_args(c);
+  a.u = d_u;
+  a.v = d_w;
+  a.out = d_ubar;
+  a.out2 = d_logd_bar;
+  a.t = t;
+  hipStream_t st = (hipStream_t)s;
+  switch (c->flux_kind) {
+    case FVM_FLUX_D_POLY_U: return launch_vjp_fk<FVM_FLUX_D_POLY_U>(a, c->m->tile_tris, st);
+    case FVM_FLUX_D_NODAL: return launch_vjp_fk<FVM_FLUX_D_NODAL>(a, c->m->tile_tris, st);
+    default: return launch_vjp_fk<FVM_FLUX_D_EDGE>(a, c->m->tile_tris, st);
+  }
+}
+
+int fvm_dirichlet_apply(fvm_ctx* c, double* d_u, double t, fvm_stream s) {
+  FVM_CHECK(c && d_u, "fvm_dirichlet_apply: null argument");
+  if (c->n_dir == 0) return FVM_OK;
+  hipLaunchKernelGGL(dirichlet_kernel, dim3((unsigned)((c->n_dir + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)s, c->n_dir, c->dir_nodes, c->fidx, c->m->x, c->m->y, c->cond(), t,
+                     d_u);
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// Device vector operations used by the host-side integrators and solvers: fused
+// linear combinations and deterministic reductions (fixed grid, fixed order).
+#include <mutex>
+
+#include "fvm_internal.hpp"
+
+using namespace fvm;
+
+namespace {
+
+constexpr int kRedBlocks = 512;  // fixed: the summation order never depends on timing
+constexpr int kRedThreads = 256;
+constexpr int kMaxTerms = 8;
+
+struct Terms {
+  int n;
+  double c[kMaxTerms];
+  const double* x[kMaxTerms];
+};
+
+__global__ __launch_bounds__(256) void lincomb_kernel(int64_t n, double* __restrict__ out, Terms t) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = t.c[0] * t.x[0][i];
+    for (int k = 1; k < t.n; ++k) v = v + t.c[k] * t.x[k][i];
+    out[i] = v;
+  }
+}
+
+// kind 0: sum x*y;  kind 1: sum (e / (abstol + reltol*max(|a|,|b|)))^2
+__device__ __forceinline__ double red_term(int kind, const double* x, const double* y,
+                                           const double* a, const double* b, double abstol,
+                                           double reltol, int64_t i) {
+  if (kind == 0) return x[i] * y[i];
+  const double sc = abstol + reltol * fmax(fabs(a[i]), fabs(b[i]));
+  const double r = x[i] / sc;
+  return r * r;
+}
+
+__global__ __launch_bounds__(kRedThreads) void reduce_stage1(int kind, int64_t n, const double* x,
+                                                             const double* y, const double* a,
+                                                             const double* b, double abstol,
+                                                             double reltol, double* part) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    acc += red_term(kind, x, y, a, b, abstol, reltol, i);
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(kRedThreads) void reduce_stage2(const double* part, double* res) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) acc += part[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *res = sh[0];
+}
+
+std::mutex g_mu;
+double* g_scratch = nullptr;  // kRedBlocks partials + 1 result
+
+int scratch(double** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_scratch) FVM_HIP(hipMalloc((void**)&g_scratch, sizeof(double) * (kRedBlocks + 8)));
+  *out = g_scratch;
+  return FVM_OK;
+}
+
+int reduce(int kind, int64_t n, const double* x, const double* y, const double* a, const double* b,
+           double abstol, double reltol, double* result, hipStream_t s) {
+  double* buf;
+  int rc = scratch(&buf);
+  if (rc) return rc;
+  hipLaunchKernelGGL(reduce_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, s, kind, n, x, y, a, b,
+                     abstol, reltol, buf);
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kRedThreads), 0, s, buf, buf + kRedBlocks);
+  FVM_HIP(hipGetLastError());
+  FVM_HIP(hipMemcpyAsync(result, buf + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+  FVM_HIP(hipStreamSynchronize(s));
+  return FVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvm_vec_lincomb(int64_t n, double* d_out, int nterms, const double* coefs,
+                    const double* const* d_xs, fvm_stream s) {
+  FVM_CHECK(d_out && coefs && d_xs, "fvm_vec_lincomb: null argument");
+  FVM_CHECK(nterms >= 1 && nterms <= kMaxTerms, "fvm_vec_lincomb: 1..8 terms");
+  if (n <= 0) return FVM_OK;
+  Terms t;
+  t.n = nterms;
+  for (int k = 0; k < nterms; ++k) {
+    FVM_CHECK(d_xs[k], "fvm_vec_lincomb: null input vector");
+    t.c[k] = coefs[k];
+    t.x[k] = d_xs[k];
+  }
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(lincomb_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s, n, d_out, t);
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_vec_dot(int64_t n, const double* d_x, const double* d_y, double* result, fvm_stream s) {
+  FVM_CHECK(d_x && d_y && result, "fvm_vec_dot: null argument");
+  return reduce(0, n, d_x, d_y, nullptr, nullptr, 0, 0, result, (hipStream_t)s);
+}
+
+int fvm_vec_err_norm(int64_t n, const double* d_err, const double* d_u0, const double* d_u1,
+                     double abstol, double reltol, double* result, fvm_stream s) {
+  FVM_CHECK(d_err && d_u0 && d_u1 && result, "fvm_vec_err_norm: null argument");
+  double sum = 0;
+  int rc = reduce(1, n, d_err, nullptr, d_u0, d_u1, abstol, reltol, &sum, (hipStream_t)s);
+  if (rc) return rc;
+  *result = n > 0 ? sqrt(sum / (double)n) : 0.0;
+  return FVM_OK;
+}
+
+}  // extern "C"

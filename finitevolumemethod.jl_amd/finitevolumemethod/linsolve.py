@@ -1,0 +1,62 @@
+"""Krylov solvers over device vectors (matrix-free operators)."""
+import math
+
+import numpy as np
+
+from . import _lib
+
+
+def gmres_device(apply_A, b, x, rtol=1e-6, restart=50, maxiter=1000):
+    """Restarted GMRES(m) (Saad & Schultz 1986) for A x = b with x0 = 0; the
+    operator is a callable apply_A(v, out) on device vectors.  Returns the
+    number of iterations."""
+    n = b.n
+    V = [_lib.DeviceArray(n) for _ in range(restart + 1)]
+    w = _lib.DeviceArray(n)
+    x.fill_zero()
+    bnorm = math.sqrt(_lib.dot(b, b))
+    if bnorm == 0:
+        return 0
+    total = 0
+    r = _lib.DeviceArray(n)
+    while total < maxiter:
+        apply_A(x, w)
+        _lib.lincomb(r, (1.0, -1.0), (b, w))
+        beta = math.sqrt(_lib.dot(r, r))
+        if beta <= rtol * bnorm:
+            break
+        _lib.lincomb(V[0], (1.0 / beta,), (r,))
+        H = np.zeros((restart + 1, restart))
+        g = np.zeros(restart + 1)
+        g[0] = beta
+        cs = np.zeros(restart)
+        sn = np.zeros(restart)
+        k_done = 0
+        for j in range(restart):
+            apply_A(V[j], w)
+            for i in range(j + 1):  # modified Gram-Schmidt
+                H[i, j] = _lib.dot(w, V[i])
+                _lib.lincomb(w, (1.0, -H[i, j]), (w, V[i]))
+            H[j + 1, j] = math.sqrt(_lib.dot(w, w))
+            if H[j + 1, j] > 0:
+                _lib.lincomb(V[j + 1], (1.0 / H[j + 1, j],), (w,))
+            for i in range(j):  # apply previous Givens rotations
+                a, c = H[i, j], H[i + 1, j]
+                H[i, j], H[i + 1, j] = cs[i] * a + sn[i] * c, -sn[i] * a + cs[i] * c
+            d = math.hypot(H[j, j], H[j + 1, j])
+            cs[j], sn[j] = (1.0, 0.0) if d == 0 else (H[j, j] / d, H[j + 1, j] / d)
+            H[j, j] = d
+            H[j + 1, j] = 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            total += 1
+            k_done = j + 1
+            if abs(g[j + 1]) <= rtol * bnorm or total >= maxiter:
+                break
+        y = np.linalg.solve(np.triu(H[:k_done, :k_done]), g[:k_done]) if k_done else np.zeros(0)
+        for i in range(0, k_done, 7):
+            idx = list(range(i, min(k_done, i + 7)))
+            _lib.lincomb(x, [1.0] + [float(y[q]) for q in idx], [x] + [V[q] for q in idx])
+        if abs(g[k_done]) <= rtol * bnorm:
+            break
+    return total

@@ -1,0 +1,298 @@
+"""solve(prob, alg; saveat, ...) with the state resident on the GPU
+(reference src/solve.jl:133-220).
+
+The reference hands fvm_eqs! to OrdinaryDiffEq.  This package carries its own
+explicit integrators over device vectors (every stage is a fused linear
+combination kernel plus one fvm_rhs launch) and applies the Dirichlet callback
+(solve.jl:133-165, dirichlet.jl:78-86) after every accepted step, like the
+reference's DiscreteCallback(true, update_dirichlet_nodes!).  Saved states at
+`saveat` times come from cubic Hermite interpolation of the accepted steps.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .problem import FVMProblem, SteadyFVMProblem
+
+# Tsitouras (2011) 5(4) pair: nodes, stage weights (the 7th stage is the
+# solution, first-same-as-last) and the error weights b - bhat.
+_TSIT5_C = (0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0)
+_TSIT5_A = (
+    (),
+    (0.161,),
+    (-0.008480655492356989, 0.335480655492357),
+    (2.897153057105493, -6.359448489975075, 4.3622954328695815),
+    (5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525),
+    (5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401,
+     -0.028269050394068383),
+    (0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,
+     2.324710524099774),
+)
+_TSIT5_BTILDE = (-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                 -0.1447110071732629, 0.5823571654525552, -0.45808210592918697,
+                 0.015151515151515152)
+
+
+@dataclass
+class Tsit5:
+    """Adaptive Tsitouras 5(4) with a PI step controller (beta1 = 7/50,
+    beta2 = 2/25, safety 0.9, step ratio in [1/10, 5])."""
+    pass
+
+
+@dataclass
+class Euler:
+    """Forward Euler with a fixed step dt."""
+    dt: float = None
+
+
+@dataclass
+class SSPRK3:
+    """Shu-Osher three-stage strong-stability-preserving Runge-Kutta, fixed dt."""
+    dt: float = None
+
+
+@dataclass
+class ODESolution:
+    t: list = field(default_factory=list)
+    u: list = field(default_factory=list)
+    retcode: str = "Default"
+    stats: dict = field(default_factory=dict)
+
+    def __getitem__(self, k):
+        return self.u[k]
+
+    def __len__(self):
+        return len(self.u)
+
+
+class _Saver:
+    def __init__(self, saveat, t0, tf):
+        if saveat is None or (np.ndim(saveat) == 0 and float(saveat) == 0):
+            self.times = None
+        elif np.ndim(saveat) == 0:
+            h = float(saveat)
+            n = int(math.floor((tf - t0) / h + 1e-9))
+            ts = [t0 + k * h for k in range(n + 1)]
+            if ts[-1] < tf - 1e-12 * max(1.0, abs(tf)):
+                ts.append(tf)
+            self.times = ts
+        else:
+            self.times = sorted(float(v) for v in saveat)
+        self.k = 0
+
+
+def _hermite(out, u0, f0, u1, f1, dt, theta):
+    h00 = 2 * theta ** 3 - 3 * theta ** 2 + 1
+    h10 = theta ** 3 - 2 * theta ** 2 + theta
+    h01 = -2 * theta ** 3 + 3 * theta ** 2
+    h11 = theta ** 3 - theta ** 2
+    _lib.lincomb(out, (h00, h10 * dt, h01, h11 * dt), (u0, f0, u1, f1))
+
+
+def solve(prob, alg=None, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None, maxiters=10**7,
+          save_start=True, callback=None, parallel=True, **kwargs):
+    """solve(prob, alg; saveat, abstol, reltol, dt, ...).
+
+    FVMProblem: integrates on the device; returns ODESolution (t, u host arrays).
+    SteadyFVMProblem: see solve_steady.  Templates forward to their own solver."""
+    from .templates import AbstractFVMTemplate
+
+    if isinstance(prob, AbstractFVMTemplate):
+        return prob.solve(alg, saveat=saveat, abstol=abstol, reltol=reltol, dt=dt, **kwargs)
+    if isinstance(prob, SteadyFVMProblem):
+        return solve_steady(prob, alg, abstol=abstol, reltol=reltol, **kwargs)
+    if not isinstance(prob, FVMProblem):
+        raise TypeError(f"solve: unsupported problem type {type(prob).__name__}")
+    ctx = prob.device_context()
+    rhs = lambda u, du, t: ctx.rhs(u, du, t)  # noqa: E731
+    has_dir = prob.conditions.has_dirichlet_nodes()
+    cb = (lambda u, t: ctx.apply_dirichlet(u, t)) if has_dir else None
+    return integrate(rhs, prob.initial_condition, prob.initial_time, prob.final_time,
+                     alg if alg is not None else Tsit5(), saveat=saveat, abstol=abstol,
+                     reltol=reltol, dt=dt, maxiters=maxiters, save_start=save_start,
+                     step_callback=cb, user_callback=callback)
+
+
+def integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None,
+              maxiters=10**7, save_start=True, step_callback=None, user_callback=None):
+    """Integrate du/dt = rhs(u, t) on the device from t0 to tf."""
+    n = len(u0)
+    u = _lib.DeviceArray.from_host(np.asarray(u0, np.float64))
+    sol = ODESolution()
+    saver = _Saver(saveat, t0, tf)
+    nf = [0]
+
+    def f(uu, out, tt):
+        nf[0] += 1
+        rhs(uu, out, tt)
+
+    def save(tt, arr):
+        sol.t.append(tt)
+        sol.u.append(arr.to_host())
+
+    if saver.times is None:
+        if save_start:
+            save(t0, u)
+    t = t0
+    naccept = nreject = 0
+    if isinstance(alg, Tsit5):
+        k = [_lib.DeviceArray(n) for _ in range(7)]
+        utmp = _lib.DeviceArray(n)
+        unew = _lib.DeviceArray(n)
+        err = _lib.DeviceArray(n)
+        interp = _lib.DeviceArray(n)
+        f(u, k[0], t)
+        if dt is None:
+            dt = _initial_dt(f, u, k[0], t, tf, abstol, reltol, n)
+        qold = 1e-4
+        it = 0
+        if saver.times is not None:
+            while saver.k < len(saver.times) and saver.times[saver.k] <= t + 1e-14 * max(1, abs(t)):
+                save(saver.times[saver.k], u)
+                saver.k += 1
+        while t < tf and it < maxiters:
+            it += 1
+            dt = min(dt, tf - t)
+            for s in range(1, 7):
+                coefs = [1.0] + [dt * a for a in _TSIT5_A[s]]
+                xs = [u] + k[:s]
+                dst = unew if s == 6 else utmp
+                _lib.lincomb(dst, coefs, xs)
+                f(dst, k[s], t + _TSIT5_C[s] * dt)
+            _lib.lincomb(err, [dt * b for b in _TSIT5_BTILDE], k)
+            EEst = _lib.err_norm(err, u, unew, abstol, reltol)
+            if not np.isfinite(EEst):
+                sol.retcode = "Unstable"
+                break
+            if EEst <= 1.0:
+                # dense output for saveat inside (t, t + dt]
+                tn = t + dt
+                if saver.times is not None:
+                    while saver.k < len(saver.times) and saver.times[saver.k] <= tn + 1e-12 * max(1, abs(tn)):
+                        ts = saver.times[saver.k]
+                        theta = min(1.0, max(0.0, (ts - t) / dt))
+                        _hermite(interp, u, k[0], unew, k[6], dt, theta)
+                        save(ts, interp)
+                        saver.k += 1
+                u, unew = unew, u
+                t = tn
+                naccept += 1
+                if step_callback is not None:
+                    step_callback(u, t)
+                if user_callback is not None:
+                    user_callback(u, t)
+                if saver.times is None:
+                    save(t, u)
+                f(u, k[0], t)
+                q = max(EEst, 1e-16) ** (7 / 50) / qold ** (2 / 25)
+                q = min(5.0, max(0.1, q / 0.9))
+                dt = dt / q
+                qold = max(EEst, 1e-4)
+            else:
+                nreject += 1
+                q = min(5.0, max(EEst, 1e-16) ** (7 / 50) / 0.9)
+                dt = dt / q
+        sol.retcode = "Success" if t >= tf else (sol.retcode if sol.retcode != "Default" else "MaxIters")
+    else:
+        if dt is None and getattr(alg, "dt", None) is None:
+            raise ValueError("fixed-step integrators need dt")
+        h = float(dt if dt is not None else alg.dt)
+        nsteps = int(math.ceil((tf - t0) / h - 1e-9))
+        k1 = _lib.DeviceArray(n)
+        u1 = _lib.DeviceArray(n)
+        u2 = _lib.DeviceArray(n)
+        if saver.times is not None:
+            while saver.k < len(saver.times) and saver.times[saver.k] <= t + 1e-14:
+                save(saver.times[saver.k], u)
+                saver.k += 1
+        for step in range(nsteps):
+            hh = min(h, tf - t)
+            if isinstance(alg, Euler):
+                f(u, k1, t)
+                _lib.lincomb(u1, (1.0, hh), (u, k1))
+            elif isinstance(alg, SSPRK3):
+                f(u, k1, t)
+                _lib.lincomb(u1, (1.0, hh), (u, k1))
+                f(u1, k1, t + hh)
+                _lib.lincomb(u2, (0.75, 0.25, 0.25 * hh), (u, u1, k1))
+                f(u2, k1, t + 0.5 * hh)
+                _lib.lincomb(u1, (1 / 3, 2 / 3, 2 / 3 * hh), (u, u2, k1))
+            else:
+                raise TypeError(f"unsupported algorithm {type(alg).__name__}")
+            tn = t + hh
+            if saver.times is not None:
+                while saver.k < len(saver.times) and saver.times[saver.k] <= tn + 1e-12:
+                    ts = saver.times[saver.k]
+                    theta = (ts - t) / hh
+                    _lib.lincomb(u2, (1 - theta, theta), (u, u1))
+                    save(ts, u2)
+                    saver.k += 1
+            u, u1 = u1, u
+            t = tn
+            naccept += 1
+            if step_callback is not None:
+                step_callback(u, t)
+            if user_callback is not None:
+                user_callback(u, t)
+            if saver.times is None:
+                save(t, u)
+        sol.retcode = "Success"
+    sol.stats = dict(nf=nf[0], naccept=naccept, nreject=nreject)
+    sol.final_device = u
+    return sol
+
+
+def _initial_dt(f, u0, f0, t0, tf, abstol, reltol, n):
+    """Hairer, Norsett & Wanner (1993), II.4: starting step size for order 5."""
+    tmp = _lib.DeviceArray(n)
+    f1 = _lib.DeviceArray(n)
+    d0 = _lib.err_norm(u0, u0, u0, abstol, reltol)
+    d1 = _lib.err_norm(f0, u0, u0, abstol, reltol)
+    dt0 = 1e-6 if (d0 < 1e-5 or d1 < 1e-5) else 0.01 * d0 / d1
+    dt0 = min(dt0, tf - t0)
+    _lib.lincomb(tmp, (1.0, dt0), (u0, f0))
+    f(tmp, f1, t0 + dt0)
+    _lib.lincomb(tmp, (1.0 / dt0, -1.0 / dt0), (f1, f0))
+    d2 = _lib.err_norm(tmp, u0, u0, abstol, reltol)
+    m = max(d1, d2)
+    dt1 = max(1e-6, dt0 * 1e-3) if m <= 1e-15 else (0.01 / m) ** (1 / 5)
+    return min(100 * dt0, dt1, tf - t0)
+
+
+def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kwargs):
+    """SteadyFVMProblem: Newton's method on fvm_eqs!(du, u) = 0 with the device
+    Jacobian-vector product (fvm_jvp) inside restarted GMRES.  Dirichlet nodes are
+    set by the Dirichlet functions first and stay fixed (their du is 0)."""
+    from .linsolve import gmres_device
+
+    p = prob.problem
+    ctx = p.device_context()
+    n = len(p.initial_condition)
+    u = _lib.DeviceArray.from_host(p.initial_condition)
+    if p.conditions.has_dirichlet_nodes():
+        ctx.apply_dirichlet(u, p.final_time if np.isfinite(p.final_time) else 0.0)
+    t = 0.0
+    F = _lib.DeviceArray(n)
+    du = _lib.DeviceArray(n)
+    sol = ODESolution()
+    ctx.rhs(u, F, t)
+    r0 = math.sqrt(_lib.dot(F, F))
+    it = 0
+    rn = r0
+    while it < maxiters:
+        if rn <= abstol + reltol * r0:
+            break
+        it += 1
+        jv = lambda v, out: ctx.jvp(u, v, out, t)  # noqa: E731
+        gmres_device(jv, F, du, rtol=1e-3, restart=50, maxiter=2000)
+        _lib.lincomb(u, (1.0, -1.0), (u, du))
+        ctx.rhs(u, F, t)
+        rn = math.sqrt(_lib.dot(F, F))
+    sol.t = [math.inf]
+    sol.u = [u.to_host()]
+    sol.retcode = "Success" if rn <= abstol + reltol * r0 else "MaxIters"
+    sol.stats = dict(newton_iters=it, residual=rn)
+    return sol

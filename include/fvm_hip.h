@@ -1,0 +1,170 @@
+/* C ABI of libfvmhip.so: the MI355X (gfx950) implementation of the vertex-centred
+ * finite-volume hot path of FiniteVolumeMethod.jl v1.2.3.
+ *
+ * Conventions
+ *  - Every function returns 0 (FVM_OK) or a negative status; fvm_last_error()
+ *    returns the message of the calling thread's last failure.  No C++ exception
+ *    crosses this boundary.
+ *  - Pointers named d_* are device pointers (from fvm_alloc); all others are host
+ *    pointers.  Kernels run asynchronously on the given stream (NULL = the null
+ *    stream); only *_synchronize and fvm_memcpy_d2h block.
+ *  - Indices are 0-based int32: reference node k (1-based Julia) is node k-1.
+ *  - A context (fvm_ctx) holds scratch buffers: calls on one context must not
+ *    overlap on different streams.
+ */
+#ifndef FVM_HIP_H
+#define FVM_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVM_OK 0
+#define FVM_ERR_ARG -1
+#define FVM_ERR_HIP -2
+#define FVM_ERR_ALLOC -3
+#define FVM_ERR_STATE -4
+
+typedef void* fvm_stream; /* hipStream_t */
+typedef void* fvm_event;  /* hipEvent_t */
+typedef struct fvm_mesh fvm_mesh;
+typedef struct fvm_ctx fvm_ctx;
+
+/* ---- runtime ------------------------------------------------------------- */
+/* Error text of the calling thread's last failure (copied into buf, truncated). */
+int fvm_last_error(char* buf, size_t n);
+int fvm_device_count(int* n);
+int fvm_device_synchronize(void);
+/* Device memory (what Julia would wrap as a device vector with a finalizer). */
+int fvm_alloc(void** d_ptr, size_t bytes);
+int fvm_free(void* d_ptr);
+int fvm_memcpy_h2d(void* d_dst, const void* src, size_t bytes, fvm_stream s);
+int fvm_memcpy_d2h(void* dst, const void* d_src, size_t bytes, fvm_stream s); /* blocks */
+int fvm_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, fvm_stream s);
+int fvm_memset(void* d_dst, int value, size_t bytes, fvm_stream s);
+int fvm_stream_create(fvm_stream* s);
+int fvm_stream_destroy(fvm_stream s);
+int fvm_stream_synchronize(fvm_stream s);
+int fvm_event_create(fvm_event* e);
+int fvm_event_destroy(fvm_event e);
+int fvm_event_record(fvm_event e, fvm_stream s);
+int fvm_event_elapsed_ms(float* ms, fvm_event start, fvm_event stop); /* waits for stop */
+
+/* ---- geometry (FVMGeometry, reference src/geometry.jl:99-169) -------------- */
+typedef struct fvm_mesh_desc {
+  int64_t n_points;          /* DelaunayTriangulation.num_points (vertices or not) */
+  int64_t n_triangles;       /* solid triangles */
+  int64_t n_boundary_edges;
+  const double* x;           /* [n_points] */
+  const double* y;           /* [n_points] */
+  const int32_t* triangles;  /* [3*n_triangles], counter-clockwise (each_solid_triangle) */
+  const int32_t* boundary_edges; /* [2*n_boundary_edges], counter-clockwise, interior on the left
+                                    (keys of get_boundary_edge_map) */
+  int32_t tile_triangles;    /* 0: default (512) */
+  int32_t tile_max_nodes;    /* 0: default (640) */
+} fvm_mesh_desc;
+
+/* Replaces FVMGeometry(tri) (geometry.jl:99): uploads the mesh, builds the
+ * triangle tiles, and computes the control-volume geometry on the device. */
+int fvm_mesh_create(const fvm_mesh_desc* desc, fvm_mesh** out);
+int fvm_mesh_destroy(fvm_mesh* m);
+
+typedef struct fvm_mesh_info {
+  int64_t n_points, n_triangles, n_boundary_edges;
+  int32_t n_tiles, n_fix_nodes, n_partials;
+  int64_t n_local_nodes;
+  const double* d_cv_volumes;     /* [n_points] device */
+  const double* d_x;              /* [n_points] device */
+  const double* d_y;              /* [n_points] device */
+} fvm_mesh_info;
+int fvm_mesh_get_info(const fvm_mesh* m, fvm_mesh_info* info);
+
+/* Copies the geometry back in the input triangle order: cv_volumes [n_points]
+ * (FVMGeometry.cv_volumes) and props [24*n_triangles], per triangle
+ * s1..s9, CV-edge midpoints (m1x m1y m2x m2y m3x m3y), unit normals (n1x n1y
+ * n2x n2y n3x n3y), lengths (l1 l2 l3) -- TriangleProperties, geometry.jl:21-26. */
+int fvm_mesh_download_geometry(const fvm_mesh* m, double* cv_volumes, double* props);
+
+/* ---- problem (FVMProblem + Conditions, problem.jl:118-163, conditions.jl:486-552) ---- */
+enum {
+  FVM_FLUX_D_POLY_U = 0, /* q = -D(u_s)(alpha, beta), D = d0 + u(d1 + u(d2 + u d3)),
+                            u_s = alpha x_s + beta y_s + gamma (construct_flux_function) */
+  FVM_FLUX_D_NODAL = 1,  /* q = -D(x_s)(alpha, beta), D the piecewise-linear interpolant of
+                            exp(logD) (pl_interpolate of a nodal field) */
+  FVM_FLUX_D_EDGE = 2    /* q = -D_s (alpha, beta) with D_s given at every CV-edge midpoint and
+                            boundary half-edge point (D(x, y) independent of t and u) */
+};
+enum { FVM_NODE_NONE = 0, FVM_NODE_DIRICHLET = 1, FVM_NODE_DUDT = 2 };
+
+typedef struct fvm_physics_desc {
+  int32_t flux_kind;
+  double dpoly[4];              /* FVM_FLUX_D_POLY_U */
+  const double* d_edge;         /* [3*n_triangles] input triangle order, CV edge (i,j),(j,k),(k,i) */
+  const double* d_bedge;        /* [2*n_boundary_edges] at the half-edge points m_i, m_j */
+  const double* log_d;          /* [n_points] FVM_FLUX_D_NODAL (replaceable by fvm_set_log_d) */
+  double spoly[4];              /* S(x,y,t,u) = s_node[i] + s0 + u(s1 + u(s2 + u s3)) + st t */
+  double st;
+  const double* s_node;         /* [n_points] or NULL */
+  const int8_t* node_kind;      /* [n_points] FVM_NODE_*; a node in both Dicts is Dirichlet */
+  const int32_t* node_fidx;     /* [n_points] condition function of Dirichlet / Dudt nodes */
+  const int32_t* bedge_neumann_fidx; /* [n_boundary_edges] function of a Neumann edge, -1 otherwise */
+  int32_t n_functions;          /* condition functions a(x,y,t,u) = c0 + cu u + ct t + cx x + cy y + g */
+  const double* fn_c0;
+  const double* fn_cu;
+  const double* fn_ct;
+  const double* fn_cx;          /* may be NULL (0) */
+  const double* fn_cy;          /* may be NULL (0) */
+  const double* node_g;         /* [n_points] g at Dirichlet / Dudt nodes, or NULL (0) */
+  const double* bedge_g;        /* [2*n_boundary_edges] g at Neumann half-edge points, or NULL */
+} fvm_physics_desc;
+
+int fvm_ctx_create(const fvm_mesh* m, const fvm_physics_desc* phys, fvm_ctx** out);
+int fvm_ctx_destroy(fvm_ctx* c);
+/* FVM_FLUX_D_NODAL: use the device field d_log_d [n_points] from now on. */
+int fvm_ctx_set_log_d(fvm_ctx* c, const double* d_log_d);
+
+/* fvm_eqs!(du, u, p, t) (main_equations.jl:28-81): triangle fluxes, boundary-edge
+ * fluxes and source / condition terms, du = 0 at Dirichlet nodes and at points
+ * that are not vertices. */
+int fvm_rhs(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_stream s);
+
+/* fvm_rhs with HIP events around its two kernels (triangle tiles, then the
+ * shared-node fix-up); waits for completion and returns both durations.  No
+ * reference counterpart: a measurement hook for benchmarks. */
+int fvm_rhs_timed(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_stream s,
+                  float* ms_tiles, float* ms_fixup);
+
+/* update_dirichlet_nodes! (equations/dirichlet.jl:38-86): u_i = a(x_i, y_i, t, u_i)
+ * at Dirichlet nodes (the Dirichlet callback of solve.jl:133-165). */
+int fvm_dirichlet_apply(fvm_ctx* c, double* d_u, double t, fvm_stream s);
+
+/* Jacobian-vector product J_u v of fvm_eqs! (what ForwardDiff through DiffCache
+ * computes for implicit steppers, main_equations.jl:56). */
+int fvm_jvp(fvm_ctx* c, const double* d_u, const double* d_v, double* d_jv, double t, fvm_stream s);
+
+/* Vector-Jacobian products of fvm_eqs!: d_ubar = J_u^T w and, for
+ * FVM_FLUX_D_NODAL, d_logd_bar = J_logD^T w (d_logd_bar may be NULL).  No
+ * reference counterpart (the reference differentiates only forward, by
+ * ForwardDiff); it is the adjoint of fvm_rhs. */
+int fvm_vjp(fvm_ctx* c, const double* d_u, const double* d_w, double* d_ubar,
+            double* d_logd_bar, double t, fvm_stream s);
+
+/* ---- device vector operations (used by the integrators and solvers) ------
+ * The reference broadcasts over Vector{Float64} inside OrdinaryDiffEq; device-
+ * resident state needs these as kernels.  Reductions use a fixed grid and order,
+ * so they are bitwise reproducible. */
+/* d_out = sum_k coefs[k] * d_xs[k], 1 <= nterms <= 8 (d_out may alias an input). */
+int fvm_vec_lincomb(int64_t n, double* d_out, int nterms, const double* coefs,
+                    const double* const* d_xs, fvm_stream s);
+int fvm_vec_dot(int64_t n, const double* d_x, const double* d_y, double* result, fvm_stream s);
+/* sqrt(mean((err_i / (abstol + reltol * max(|u0_i|, |u1_i|)))^2)): the
+ * error norm of OrdinaryDiffEq's adaptive step control. */
+int fvm_vec_err_norm(int64_t n, const double* d_err, const double* d_u0, const double* d_u1,
+                     double abstol, double reltol, double* result, fvm_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,147 @@
+"""Problem builders shared by the tests: each returns the product problem and the
+oracle problem with the same mesh, conditions and closures.
+
+The closures mirror the reference's own test problems (test/test_functions.jl:
+example_diffusion_problem, example_heat_convection_problem, example_problem) and
+the workloads the benchmark uses (nonlinear D(u) = 0.1 + u^2 with S = u(1 - u),
+a nodal log-diffusivity field, D(x, y) = exp(0.5 sin 4 pi x cos 4 pi y))."""
+import numpy as np
+
+import finitevolumemethod as FVM
+from oracle import OracleProblem
+
+
+def oracle_for(prob, physics, tri=None):
+    tri = tri or prob.mesh.triangulation
+    kind, fidx, bneu = prob.conditions.flatten(tri)
+    return OracleProblem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                         kind, fidx, bneu, physics)
+
+
+def nonlinear_problem(nx=30, ny=26, seed=3, jitter=0.25):
+    """Throughput workload: jittered unit square, homogeneous Dirichlet,
+    D(u) = 0.1 + u^2, S = u (1 - u), u ~ U[0, 1)."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, nx, ny, jitter=jitter, seed=seed)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    u = np.random.default_rng(seed + 1).random(tri.num_points())
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 0.1 + u ** 2,
+                          source_function=lambda x, y, t, u, p: u * (1 - u),
+                          initial_condition=u, final_time=1.0)
+    orc = oracle_for(prob, dict(builtin=dict(flux_kind=0, dpoly=[0.1, 0, 1, 0],
+                                             spoly=[0, 1, -1, 0], fn=([0.0], [0.0], [0.0]))))
+    return prob, orc, u
+
+
+def diffusion_problem(n=25):
+    """example_diffusion_problem (test_functions.jl:315-328): [0,2]^2, D = 1/9,
+    homogeneous Dirichlet, u0 = 50 for y <= 1."""
+    tri = FVM.triangulate_rectangle(0.0, 2.0, 0.0, 2.0, n, n, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    u0 = np.where(tri.y <= 1.0, 50.0, 0.0)
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 1 / 9,
+                          initial_condition=u0, final_time=0.5)
+    orc = oracle_for(prob, dict(builtin=dict(flux_kind=0, dpoly=[1 / 9, 0, 0, 0],
+                                             fn=([0.0], [0.0], [0.0]))))
+    return prob, orc, u0
+
+
+HEAT = dict(L=1.0, k=237.0, T0=10.0, Tinf=10.0, alpha=80.0e-6, q=10.0, h=25.0)
+
+
+def heat_convection_problem(n=200):
+    """example_heat_convection_problem (test_functions.jl:346-374): four Neumann
+    walls, flux q = -alpha grad u given as a flux function."""
+    P = HEAT
+    tri = FVM.triangulate_rectangle(0, P["L"], 0, P["L"], n, n, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    bot = lambda x, y, t, T, p: -p["alpha"] * p["q"] / p["k"]  # noqa: E731
+    right = lambda x, y, t, T, p: 0.0 * T  # noqa: E731
+    top = lambda x, y, t, T, p: -p["alpha"] * p["h"] / p["k"] * (p["Tinf"] - T)  # noqa: E731
+    left = lambda x, y, t, T, p: 0.0 * T  # noqa: E731
+    BCs = FVM.BoundaryConditions(mesh, (bot, right, top, left), (FVM.Neumann,) * 4,
+                                 parameters=(P, None, P, None))
+    flux = lambda x, y, t, a, b, g, p: (-p["alpha"] * a, -p["alpha"] * b)  # noqa: E731
+    u0 = np.full(tri.num_points(), P["T0"])
+    prob = FVM.FVMProblem(mesh, BCs, flux_function=flux, flux_parameters=dict(alpha=P["alpha"]),
+                          initial_condition=u0, final_time=2000.0)
+    c_top = -P["alpha"] * P["h"] / P["k"]
+    fn = ([-P["alpha"] * P["q"] / P["k"], 0.0, c_top * P["Tinf"], 0.0],
+          [0.0, 0.0, -c_top, 0.0], [0.0] * 4)
+    orc = oracle_for(prob, dict(builtin=dict(flux_kind=0, dpoly=[P["alpha"], 0, 0, 0], fn=fn)))
+    return prob, orc, u0
+
+
+def mixed_problem(nx=12, ny=19, seed=5):
+    """example_problem-like (test_functions.jl:86-135) on [0,2]x[0,5] with four
+    segments (Dirichlet, Neumann, Dudt, Neumann), internal Dirichlet and Dudt nodes,
+    D(u) = 1 + 0.5 u, S = u + 1.5, all condition functions affine."""
+    tri = FVM.triangulate_rectangle(0.0, 2.0, 0.0, 5.0, nx, ny, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    f1 = lambda x, y, t, u, p: x * y + u - p  # noqa: E731  Dirichlet (bottom)
+    f2 = lambda x, y, t, u, p: u + p - t  # noqa: E731       Neumann (right)
+    f3 = lambda x, y, t, u, p: 2 * u + 0.25 * t - p  # noqa: E731  Dudt (top)
+    f4 = lambda x, y, t, u, p: y - x  # noqa: E731           Neumann (left)
+    BCs = FVM.BoundaryConditions(mesh, (f1, f2, f3, f4),
+                                 (FVM.Dirichlet, FVM.Neumann, FVM.Dudt, FVM.Neumann),
+                                 parameters=(0.5, 0.2, 0.3, 0.4))
+    idir = {6 + k * nx: 0 for k in range(2, ny - 2)}
+    idudt = {3 + 3 * nx: 1, 4 + 5 * nx: 1}
+    ICs = FVM.InternalConditions((lambda x, y, t, u, p: x + y + t + u + p,
+                                  lambda x, y, t, u, p: -u + 3 * t),
+                                 dirichlet_nodes=idir, dudt_nodes=idudt, parameters=(0.29, None))
+    u0 = np.random.default_rng(seed).random(tri.num_points())
+    prob = FVM.FVMProblem(mesh, BCs, ICs, diffusion_function=lambda x, y, t, u, p: 1 + 0.5 * u,
+                          source_function=lambda x, y, t, u, p: u + p, source_parameters=1.5,
+                          initial_condition=u0, initial_time=2.0, final_time=5.0)
+    # merged function list: (internal 0, internal 1, f1, f2, f3, f4)
+    c0 = [0.29, 0.0, -0.5, 0.2, -0.3, 0.0]
+    cu = [1.0, -1.0, 1.0, 1.0, 2.0, 0.0]
+    ct = [1.0, 3.0, 0.0, -1.0, 0.25, 0.0]
+    cx = [1.0, 0.0, 0.0, 0.0, 0.0, -1.0]
+    cy = [1.0, 0.0, 0.0, 0.0, 0.0, 1.0]
+    orc = oracle_for(prob, dict(builtin=dict(flux_kind=0, dpoly=[1.0, 0.5, 0, 0],
+                                             spoly=[1.5, 1.0, 0, 0], fn=(c0, cu, ct, cx, cy))))
+    # f1 = x*y + u - p has an x*y term: give the oracle the exact closure instead
+    orc_cb = oracle_for(prob, dict(
+        flux=lambda x, y, t, a, b, g, T: (-(1 + 0.5 * (a * x + b * y + g)) * a,
+                                          -(1 + 0.5 * (a * x + b * y + g)) * b),
+        source=lambda x, y, t, u, n: u + 1.5,
+        cond=lambda f, x, y, t, u: prob.conditions.functions[f](x, y, t, u)))
+    return prob, orc_cb, u0
+
+
+def nodal_problem(nx=28, ny=24, seed=11):
+    """Per-node log-diffusivity field (the differentiation workload), D on the
+    boundary and a homogeneous Dirichlet boundary."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, nx, ny, jitter=0.25, seed=seed)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    rng = np.random.default_rng(seed)
+    logd = np.log(0.1) + 0.3 * rng.standard_normal(tri.num_points())
+    u = rng.random(tri.num_points())
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=FVM.NodalDiffusion(logd),
+                          initial_condition=u, final_time=0.1)
+    orc = oracle_for(prob, dict(builtin=dict(flux_kind=1, logD=logd, fn=([0.0], [0.0], [0.0]))))
+    return prob, orc, u, logd
+
+
+def dxy_problem(n=30, seed=13):
+    """D(x, y) = exp(0.5 sin 4 pi x cos 4 pi y) on a jittered square with a Neumann
+    bottom / right and Dirichlet top / left boundary, S = 0.3 - u^2."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n, n, jitter=0.2, seed=seed,
+                                 single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    D = lambda x, y, t, u, p: np.exp(0.5 * np.sin(4 * np.pi * x) * np.cos(4 * np.pi * y))  # noqa: E731
+    BCs = FVM.BoundaryConditions(
+        mesh, (lambda x, y, t, u, p: 0.5 + 0.0 * u, lambda x, y, t, u, p: -0.25 * u,
+               lambda x, y, t, u, p: 0.0 * u, lambda x, y, t, u, p: 1.0 + 0.0 * u),
+        (FVM.Neumann, FVM.Neumann, FVM.Dirichlet, FVM.Dirichlet))
+    u = np.random.default_rng(seed).random(tri.num_points())
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=D,
+                          source_function=lambda x, y, t, u, p: 0.3 - u ** 2,
+                          initial_condition=u, final_time=1.0)
+    fn = ([0.5, 0.0, 0.0, 1.0], [0.0, -0.25, 0.0, 0.0], [0.0] * 4)
+    orc = oracle_for(prob, dict(builtin=dict(flux_kind=2, dscale=1.0, spoly=[0.3, 0, -1, 0], fn=fn)))
+    return prob, orc, u

@@ -1,0 +1,139 @@
+"""GPU parity of the device geometry and fvm_eqs! against the CPU oracle.
+
+Bars: connectivity and indices bit-exact; geometry fields that need no square
+root (s1..s9, CV-edge midpoints) bit-exact, normals and lengths within a few
+ulp, control volumes within 1e-14 (different summation order); du within
+1e-12 of the oracle in the max norm and per node relative to
+sum_sigma |Q_sigma| / V_i + |S_i| (the reference's own threaded and serial forms
+differ by rounding of the same size)."""
+import numpy as np
+import pytest
+
+import finitevolumemethod as FVM
+from oracle import geometry as oracle_geometry
+from problems import (diffusion_problem, dxy_problem, heat_convection_problem, mixed_problem,
+                      nodal_problem, nonlinear_problem)
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def assert_parity(du, ref, scale, tol=TOL):
+    du, ref = np.asarray(du), np.asarray(ref)
+    assert du.shape == ref.shape
+    assert np.all(np.isfinite(du))
+    nref = np.max(np.abs(ref))
+    assert np.max(np.abs(du - ref)) <= tol * max(nref, 1e-300), np.max(np.abs(du - ref)) / nref
+    bound = tol * (scale + 1e-300) + 1e-300
+    bad = np.abs(du - ref) > bound
+    assert not bad.any(), (np.nonzero(bad)[0][:10], (np.abs(du - ref) / (scale + 1e-300)).max())
+
+
+@pytest.mark.parametrize("nx,ny,jitter", [(9, 7, 0.0), (40, 33, 0.25), (129, 130, 0.25)])
+def test_geometry_matches_oracle(gpu, nx, ny, jitter):
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, nx, ny, jitter=jitter, seed=nx)
+    mesh = FVM.FVMGeometry(tri)
+    cv_ref, props_ref = oracle_geometry(tri.x, tri.y, tri.triangles)
+    P = mesh.props_array
+    assert np.array_equal(P[:, :15], props_ref[:, :15])  # s1..s9, CV-edge midpoints
+    np.testing.assert_allclose(P[:, 15:], props_ref[:, 15:], rtol=4e-16, atol=4e-16)
+    np.testing.assert_allclose(mesh.cv_volumes, cv_ref, rtol=1e-14, atol=0)
+    assert abs(mesh.cv_volumes.sum() - 1.0) < 1e-12
+
+
+def test_tiles_cover_every_node(gpu):
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 70, 61, jitter=0.25, seed=2)
+    for tt in (64, 256, 512):
+        mesh = FVM.FVMGeometry(tri, tile_triangles=tt)
+        assert mesh.info.n_tiles >= len(tri.triangles) // tt
+
+
+@pytest.mark.parametrize("builder", [nonlinear_problem, diffusion_problem, mixed_problem,
+                                     dxy_problem])
+def test_fvm_eqs_matches_oracle(gpu, builder):
+    prob, orc, u = builder()[:3]
+    for t in (prob.initial_time, prob.initial_time + 0.37):
+        du = FVM.fvm_eqs(np.zeros_like(u), u, prob, t)
+        ref = orc.rhs(u, t)
+        assert_parity(du, ref, orc.scale(u, t))
+
+
+def test_fvm_eqs_heat_convection_neumann(gpu):
+    prob, orc, u0 = heat_convection_problem(60)
+    u = u0 + np.random.default_rng(0).random(len(u0))
+    du = FVM.fvm_eqs(np.zeros_like(u), u, prob, 0.0)
+    assert_parity(du, orc.rhs(u, 0.0), orc.scale(u, 0.0))
+
+
+def test_fvm_eqs_nodal_field(gpu):
+    prob, orc, u, logd = nodal_problem()
+    du = FVM.fvm_eqs(np.zeros_like(u), u, prob, 0.0)
+    assert_parity(du, orc.rhs(u, 0.0), orc.scale(u, 0.0))
+
+
+def test_fvm_eqs_threaded_oracle_agrees(gpu):
+    prob, orc, u = nonlinear_problem(60, 50)
+    du = FVM.fvm_eqs(np.zeros_like(u), u, prob, 0.0)
+    assert_parity(du, orc.rhs(u, 0.0, nthreads=4), orc.scale(u, 0.0))
+
+
+def test_points_that_are_not_vertices(gpu):
+    base = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 20, 17, jitter=0.25, seed=4)
+    x = np.concatenate([base.x, [0.5, 0.25]])
+    y = np.concatenate([base.y, [0.5, 0.75]])
+    tri = FVM.Triangulation(x, y, base.triangles, base.boundary_edges, base.boundary_segments)
+    mesh = FVM.FVMGeometry(tri)
+    assert mesh.cv_volumes[-1] == 0 and mesh.cv_volumes[-2] == 0
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    u = np.random.default_rng(1).random(len(x))
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 0.1 + u ** 2,
+                          source_function=lambda x, y, t, u, p: u * (1 - u), initial_condition=u,
+                          final_time=1.0)
+    from problems import oracle_for
+    orc = oracle_for(prob, dict(builtin=dict(flux_kind=0, dpoly=[0.1, 0, 1, 0], spoly=[0, 1, -1, 0],
+                                             fn=([0.0], [0.0], [0.0]))))
+    du = FVM.fvm_eqs(np.zeros_like(u), u, prob, 0.0)
+    assert du[-1] == 0 and du[-2] == 0
+    assert_parity(du, orc.rhs(u, 0.0), orc.scale(u, 0.0))
+
+
+@pytest.mark.parametrize("tile", [32, 128, 512])
+def test_tile_size_does_not_change_result_beyond_rounding(gpu, tile):
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 50, 45, jitter=0.25, seed=9)
+    mesh = FVM.FVMGeometry(tri, tile_triangles=tile)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    u = np.random.default_rng(2).random(tri.num_points())
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 0.1 + u ** 2,
+                          source_function=lambda x, y, t, u, p: u * (1 - u), initial_condition=u,
+                          final_time=1.0)
+    from problems import oracle_for
+    orc = oracle_for(prob, dict(builtin=dict(flux_kind=0, dpoly=[0.1, 0, 1, 0], spoly=[0, 1, -1, 0],
+                                             fn=([0.0], [0.0], [0.0]))))
+    du = FVM.fvm_eqs(np.zeros_like(u), u, prob, 0.0)
+    assert_parity(du, orc.rhs(u, 0.0), orc.scale(u, 0.0))
+
+
+def test_bitwise_deterministic(gpu):
+    prob, orc, u = nonlinear_problem(200, 190)
+    d_u = FVM.DeviceArray.from_host(u)
+    outs = []
+    for _ in range(3):
+        d_du = FVM.DeviceArray(len(u))
+        FVM.fvm_eqs(d_du, d_u, prob, 0.0)
+        outs.append(d_du.to_host())
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+def test_dirichlet_update_matches_oracle(gpu):
+    prob, orc, u = mixed_problem()
+    for t in (2.0, 3.5):
+        got = FVM.update_dirichlet_nodes(u.copy(), t, prob)
+        ref = orc.update_dirichlet(u, t)
+        np.testing.assert_allclose(got, ref, rtol=1e-15, atol=1e-15)
+
+
+def test_invalid_arguments_raise(gpu):
+    prob, orc, u = nonlinear_problem(10, 9)
+    d = FVM.DeviceArray.from_host(u)
+    with pytest.raises(FVM.FVMError):
+        prob.device_context().rhs(d, d, 0.0)  # du aliasing u is rejected by the C ABI

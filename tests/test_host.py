@@ -1,0 +1,222 @@
+"""Host-side logic (no GPU): tile invariants, mesh generators, condition merging,
+closure tracing, and the C ABI export table."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import finitevolumemethod as FVM
+from finitevolumemethod import _lib, forms
+from oracle import merge_conditions
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def tiles(tri, tile_tris=512, max_nodes=640):
+    L = _lib.host_lib()
+    out = ctypes.POINTER(_lib._Tiles)()
+    st = L.fvmh_tiles_build(len(tri.x), _lib.ptr(tri.x), _lib.ptr(tri.y), len(tri.triangles),
+                            _lib.ptr(tri.triangles, _lib._i32p), len(tri.boundary_edges),
+                            _lib.ptr(tri.boundary_edges, _lib._i32p), tile_tris, max_nodes,
+                            ctypes.byref(out))
+    assert st == 0, L.fvmh_last_error()
+    t = out.contents
+
+    def arr(name, n, dt):
+        p = getattr(t, name)
+        return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                     shape=(n,)).copy() if n > 0 else np.zeros(0, dt)
+
+    T, N = t.n_tri, t.n_points
+    d = dict(n_tiles=t.n_tiles, n_local=t.n_local, n_fix=t.n_fix, n_parts=t.n_parts)
+    d["perm"] = arr("perm", T, np.int32)
+    d["tri_sorted"] = arr("tri_sorted", 3 * T, np.int32).reshape(3, T)
+    d["tile_tri_start"] = arr("tile_tri_start", t.n_tiles + 1, np.int32)
+    d["tile_node_start"] = arr("tile_node_start", t.n_tiles + 1, np.int32)
+    d["tile_nodes"] = arr("tile_nodes", t.n_local, np.int32)
+    d["loc"] = arr("loc", T, np.uint32)
+    d["inc_start"] = arr("inc_start", t.n_local + 1, np.int32)
+    d["inc"] = arr("inc", 3 * T, np.uint16)
+    d["part_idx"] = arr("part_idx", t.n_local, np.int32)
+    d["fix_nodes"] = arr("fix_nodes", t.n_fix, np.int32)
+    d["fix_part_start"] = arr("fix_part_start", t.n_fix + 1, np.int32)
+    d["fix_bedge_start"] = arr("fix_bedge_start", t.n_fix + 1, np.int32)
+    d["fix_bedge"] = arr("fix_bedge", d["fix_bedge_start"][-1], np.int32)
+    d["fix_bvert_start"] = arr("fix_bvert_start", t.n_fix + 1, np.int32)
+    d["fix_bvert"] = arr("fix_bvert", d["fix_bvert_start"][-1], np.int32)
+    d["badj_sorted"] = arr("badj_sorted", t.n_bedges, np.int32)
+    L.fvmh_tiles_free(out)
+    return d
+
+
+@pytest.mark.parametrize("tile_tris,max_nodes", [(512, 640), (256, 300), (64, 80), (7, 20)])
+def test_tile_invariants(tile_tris, max_nodes):
+    base = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 41, 37, jitter=0.25, seed=3)
+    # two points that are not vertices
+    tri = FVM.Triangulation(np.r_[base.x, 0.5, 0.3], np.r_[base.y, 0.5, 0.2], base.triangles,
+                            base.boundary_edges)
+    d = tiles(tri, tile_tris, max_nodes)
+    N, T = tri.num_points(), len(tri.triangles)
+    assert sorted(d["perm"]) == list(range(T))
+    assert np.array_equal(d["tri_sorted"].T, tri.triangles[d["perm"]])
+    written = np.zeros(N, int)  # exclusive writes + fix-node writes: exactly one per node
+    for k in range(d["n_tiles"]):
+        t0, t1 = d["tile_tri_start"][k:k + 2]
+        n0, n1 = d["tile_node_start"][k:k + 2]
+        nt, nn = t1 - t0, n1 - n0
+        assert 0 < nt <= tile_tris and 0 < nn <= max_nodes <= 1024
+        nodes = d["tile_nodes"][n0:n1]
+        assert np.all(np.diff(nodes) > 0)
+        loc = d["loc"][t0:t1]
+        ids = np.stack([loc & 1023, (loc >> 10) & 1023, (loc >> 20) & 1023], 1)
+        assert ids.max() < nn
+        assert np.array_equal(nodes[ids], d["tri_sorted"][:, t0:t1].T)
+        for ln in range(nn):
+            a, b = d["inc_start"][n0 + ln], d["inc_start"][n0 + ln + 1]
+            inc = d["inc"][a:b].astype(int)
+            assert np.all((inc >> 2) < nt) and np.all((inc & 3) < 3)
+            assert np.all(ids[inc >> 2, inc & 3] == ln)
+            if d["part_idx"][n0 + ln] < 0:
+                written[nodes[ln]] += 1
+            else:
+                assert d["part_idx"][n0 + ln] < d["n_parts"]
+    assert d["inc_start"][-1] == 3 * T
+    written[d["fix_nodes"]] += 1
+    assert np.all(written == 1)
+    fixed = set(d["fix_nodes"].tolist())
+    assert {N - 1, N - 2} <= fixed  # points that are not vertices are finished by the fix-up
+    assert set(tri.boundary_nodes().tolist()) <= fixed
+    assert np.array_equal(np.sort(d["part_idx"][d["part_idx"] >= 0]), np.arange(d["n_parts"]))
+    assert d["fix_part_start"][-1] == d["n_parts"]
+    be = d["fix_bedge"]
+    assert np.all((be >> 1) < len(tri.boundary_edges))
+    assert np.all(tri.boundary_edges[be >> 1, be & 1] ==
+                  np.repeat(d["fix_nodes"], np.diff(d["fix_bedge_start"])))
+    bv = d["fix_bvert"]
+    badj = tri.boundary_adjacent()
+    assert np.all(tri.triangles[badj[bv >> 2], bv & 3] ==
+                  np.repeat(d["fix_nodes"], np.diff(d["fix_bvert_start"])))
+    assert np.array_equal(d["perm"][d["badj_sorted"]], badj)
+
+
+def test_rectangle_numbering_matches_reference_layout():
+    # test/equations.jl:42-56: node 201 = (0, h) for nx = 200 and triangle (1, 2, 201)
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 200, 200, single_boundary=False)
+    assert tri.get_point(200) == (0.0, 1.0 / 199)
+    assert tuple(tri.triangles[0]) == (0, 1, 200)
+    assert tri.num_points() == 40000 and len(tri.triangles) == 2 * 199 * 199
+    # counter-clockwise triangles
+    p = tri.points[tri.triangles]
+    area2 = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    assert np.all(area2 > 0)
+    # boundary edges: interior on the left; segments bottom, right, top, left
+    e = tri.boundary_edges
+    assert np.all(tri.boundary_segments[tri.y[e[:, 0]] + tri.y[e[:, 1]] == 0] == 0)
+    assert np.all(tri.boundary_segments[(tri.x[e[:, 0]] == 1) & (tri.x[e[:, 1]] == 1)] == 1)
+    assert tri.num_boundary_segments() == 4
+    badj = tri.boundary_adjacent()
+    assert np.all((tri.triangles[badj] == e[:, :1]).any(1) & (tri.triangles[badj] == e[:, 1:]).any(1))
+
+
+@pytest.mark.parametrize("n,seed", [(12, 1), (33, 2), (60, 3)])
+def test_jittered_mesh_is_the_delaunay_triangulation(n, seed):
+    from scipy.spatial import Delaunay
+
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n, n + 3, jitter=0.25, seed=seed)
+    assert tri.count_non_delaunay() == 0
+    N, B = tri.num_points(), len(tri.boundary_edges)
+    assert len(tri.triangles) == 2 * N - 2 - B
+    # same triangle set as Qhull (interior points are in general position;
+    # boundary cells with collinear boundary points keep a valid Delaunay choice)
+    ours = {tuple(sorted(T)) for T in tri.triangles.tolist()}
+    qh = Delaunay(tri.points)
+    area = np.abs(np.cross(tri.points[qh.simplices[:, 1]] - tri.points[qh.simplices[:, 0]],
+                           tri.points[qh.simplices[:, 2]] - tri.points[qh.simplices[:, 0]]))
+    theirs = {tuple(sorted(T)) for T, a in zip(qh.simplices.tolist(), area) if a > 1e-14}
+    assert ours == theirs
+
+
+def test_jitter_bounds_and_seed():
+    a = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 30, 30, jitter=0.25, seed=11)
+    b = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 30, 30, jitter=0.25, seed=11)
+    g = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 30, 30)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.triangles, b.triangles)
+    h = 1.0 / 29
+    assert np.max(np.abs(a.x - g.x)) <= 0.25 * h and np.max(np.abs(a.y - g.y)) <= 0.25 * h
+    bn = g.boundary_nodes()
+    assert np.array_equal(a.x[bn], g.x[bn]) and np.array_equal(a.y[bn], g.y[bn])
+
+
+def test_csr_pattern_matches_brute_force():
+    # test_jacobian_sparsity (test_functions.jl:749-763)
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 11, 9, jitter=0.25, seed=1)
+    N = tri.num_points()
+    A = np.zeros((N, N))
+    for i in range(N):
+        A[i, i] = 1
+        for j in tri.neighbours(i):
+            A[i, j] = 1
+    J = FVM.jacobian_sparsity(tri)
+    assert np.array_equal(J.toarray(), A)
+
+
+def test_condition_merging_matches_restatement():
+    class _Mesh:
+        def __init__(self, tri):
+            self.triangulation = tri
+
+    tri = FVM.triangulate_rectangle(0.0, 2.0, 0.0, 5.0, 12, 19, single_boundary=False)
+    mesh = _Mesh(tri)
+    f = lambda x, y, t, u, p: u  # noqa: E731
+    types = (FVM.Dirichlet, FVM.Dudt, FVM.Neumann, FVM.Constrained)
+    BCs = FVM.BoundaryConditions(mesh, (f,) * 4, types)
+    idir = {6 + 12 * k: 0 for k in range(1, 10)}
+    idudt = {37 + k: 1 for k in range(8)}
+    ICs = FVM.InternalConditions((f, f), dirichlet_nodes=idir, dudt_nodes=idudt)
+    conds = FVM.Conditions(mesh, BCs, ICs)
+    ne, ce, dn, dd = merge_conditions(tri.boundary_edges, tri.boundary_segments,
+                                      [t.name for t in types], 2, idir, idudt)
+    assert conds.neumann_edges == ne and conds.constrained_edges == ce
+    assert conds.dirichlet_nodes == dn and conds.dudt_nodes == dd
+    kind, fidx, bn = conds.flatten(tri)
+    for i in range(tri.num_points()):
+        if i in dn:
+            assert kind[i] == 1 and fidx[i] == dn[i]
+        elif i in dd:
+            assert kind[i] == 2 and fidx[i] == dd[i]
+        else:
+            assert kind[i] == 0
+    with pytest.raises(AssertionError):
+        FVM.BoundaryConditions(mesh, (f,) * 3, types[:3])
+
+
+def test_closure_tracing_forms():
+    assert forms.classify_diffusion(lambda x, y, t, u: 0.1 + u ** 2) == ("poly", [0.1, 0.0, 1.0, 0.0])
+    assert forms.classify_diffusion(lambda x, y, t, u: 1 / 9) == ("poly", [1 / 9, 0.0, 0.0, 0.0])
+    assert forms.classify_diffusion(lambda x, y, t, u: np.exp(x * y))[0] == "xy"
+    with pytest.raises(forms.UnsupportedClosure):
+        forms.classify_diffusion(lambda x, y, t, u: np.exp(u))
+    with pytest.raises(forms.UnsupportedClosure):
+        forms.classify_diffusion(lambda x, y, t, u: 1.0 if u > 0 else 2.0)
+    s_node, poly, st = forms.classify_source(lambda x, y, t, u: u * (1 - u), np.zeros(3), np.zeros(3))
+    assert s_node is None and poly == [0.0, 1.0, -1.0, 0.0] and st == 0.0
+    x = np.array([0.0, 1.0, 2.0])
+    g, cu, ct = forms.classify_condition(lambda x, y, t, u: x * y + u - 0.5, x, x + 1)
+    np.testing.assert_array_equal(g, x * (x + 1) - 0.5)
+    assert (cu, ct) == (1.0, 0.0)
+    with pytest.raises(forms.UnsupportedClosure):
+        forms.classify_condition(lambda x, y, t, u: x * t * u, x, x)
+    D = forms.classify_flux(lambda x, y, t, a, b, g: (-80e-6 * a, -80e-6 * b))
+    np.testing.assert_array_equal(D(x, x), np.full(3, 80e-6))
+
+
+def test_c_abi_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "fvm_hip.h")).read()
+    declared = set(re.findall(r"^int\s+(fvm_\w+)\s*\(", header, re.M))
+    assert declared, "no declarations parsed"
+    lib = ctypes.CDLL(os.path.join(_lib.LIB_DIR, "libfvmhip.so"))
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert declared <= set(_lib.HIP_API), sorted(declared - set(_lib.HIP_API))
