@@ -41,14 +41,14 @@ def canonical_bytes(n_nodes, n_tri, n_bedges, per_tri=204, per_node=25, per_bedg
     return per_tri * n_tri + per_node * n_nodes + per_bedge * n_bedges
 
 
-def build_throughput_problem(n_side):
+def build_throughput_problem(n_side, tile=0):
     import finitevolumemethod as FVM
 
     t0 = time.time()
     tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n_side, n_side, jitter=0.25, seed=SEED)
     log(f"mesh {tri.num_points()} nodes {len(tri.triangles)} triangles ({time.time() - t0:.1f}s)")
     t0 = time.time()
-    mesh = FVM.FVMGeometry(tri)
+    mesh = FVM.FVMGeometry(tri, tile_triangles=tile)
     log(f"FVMGeometry on device: {mesh.info.n_tiles} tiles, {mesh.info.n_fix_nodes} fix nodes "
         f"({time.time() - t0:.1f}s)")
     BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
@@ -159,6 +159,7 @@ def main():
     ap.add_argument("--n-side", type=int, default=2000)
     ap.add_argument("--vjp-side", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--tile", type=int, default=0, help="triangles per tile (0: default 512)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     args = ap.parse_args()
@@ -171,7 +172,7 @@ def main():
     from finitevolumemethod import _lib
 
     _lib.require_gpu()
-    tri, mesh, prob, u = build_throughput_problem(args.n_side)
+    tri, mesh, prob, u = build_throughput_problem(args.n_side, args.tile)
     ctx = prob.device_context()
     N, T, B = tri.num_points(), len(tri.triangles), len(tri.boundary_edges)
     d_u = _lib.DeviceArray.from_host(u)

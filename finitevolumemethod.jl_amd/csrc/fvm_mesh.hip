@@ -115,10 +115,14 @@ void free_mesh(fvm_mesh* m) {
 }
 
 int build(const fvm_mesh_desc* d, fvm_mesh* m) {
+  // Tiles of up to 512 triangles run on 256-thread workgroups with room for 512
+  // local nodes; tiles of up to 1024 triangles on 512-thread workgroups with 1024.
   const int tile_tris = d->tile_triangles > 0 ? d->tile_triangles : 512;
-  const int max_nodes = d->tile_max_nodes > 0 ? d->tile_max_nodes : 640;
-  FVM_CHECK(tile_tris <= 512, "fvm_mesh_create: tile_triangles must be <= 512");
-  FVM_CHECK(max_nodes <= 1024, "fvm_mesh_create: tile_max_nodes must be <= 1024");
+  FVM_CHECK(tile_tris <= 1024, "fvm_mesh_create: tile_triangles must be <= 1024");
+  const int node_limit = tile_tris <= 512 ? 512 : 1024;
+  const int max_nodes = d->tile_max_nodes > 0 ? d->tile_max_nodes : node_limit;
+  FVM_CHECK(max_nodes <= node_limit,
+            "fvm_mesh_create: tile_max_nodes must be <= 512 (<= 1024 for tiles above 512 triangles)");
   fvmh_tiles* tl = nullptr;
   if (fvmh_tiles_build(d->n_points, d->x, d->y, d->n_triangles, d->triangles, d->n_boundary_edges,
                        d->boundary_edges, tile_tris, max_nodes, &tl) != 0) {

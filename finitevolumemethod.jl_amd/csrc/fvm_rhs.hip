@@ -26,8 +26,6 @@ using namespace fvm;
 namespace {
 
 enum { M_RHS = 0, M_JVP = 1 };
-constexpr int kBlock = 256;
-constexpr int kMaxNodes = 1024;
 
 struct KArgs {
   int32_t n_tiles, n_fix;
@@ -240,25 +238,74 @@ __device__ double bedge_flux(const KArgs& a, int e, int ep) {
 // ---------------------------------------------------------------------------
 // RHS / JVP: tile kernel and fix-up kernel
 // ---------------------------------------------------------------------------
-template <int FK, int MODE, int TPT>
-__global__ __launch_bounds__(kBlock) void tile_kernel(KArgs a) {
-  constexpr int TILE = kBlock * TPT;
-  __shared__ double s_u[kMaxNodes];
-  __shared__ double s_v[MODE == M_JVP ? kMaxNodes : 1];
-  __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? kMaxNodes : 1];
+// Local-node capacity of a tile processed by BLOCK threads: two node slots per
+// thread, at most 1024 (local ids are packed in 10 bits).
+__host__ __device__ constexpr int node_cap(int block) { return 2 * block < 1024 ? 2 * block : 1024; }
+
+// finish() with the node's code and control volume already in registers.
+template <int MODE>
+__device__ __forceinline__ double finish_pre(const KArgs& a, int g, double acc, double ug, double vg,
+                                             int c, double V) {
+  if (c == 0) {
+    if (MODE == M_RHS) return acc / V + source(a, g, ug);
+    return acc / V + source_du(a, ug) * vg;
+  }
+  if (c == 2) {
+    const int f = a.fidx[g];
+    if (MODE == M_RHS)
+      return eval_cond(a.C, f, a.x[g], a.y[g], a.t, ug, a.C.node_g ? a.C.node_g[g] : 0.0);
+    return a.C.cu[f] * vg;
+  }
+  return 0.0;
+}
+
+// One workgroup per tile.  All global loads of the tile (node ids, incidence
+// block, partial slots, the triangle records, then the per-node gathers that
+// depend on the node ids) are issued before the first wait, so the workgroup
+// exposes one memory latency; the rest works out of LDS and registers.
+template <int FK, int MODE, int BLOCK, int TPT>
+__global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
+  constexpr int TILE = BLOCK * TPT;
+  constexpr int MAXN = node_cap(BLOCK);
+  constexpr int NS = MAXN / BLOCK;
+  __shared__ double s_u[MAXN];
+  __shared__ double s_v[MODE == M_JVP ? MAXN : 1];
+  __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
   __shared__ double s_q[3 * TILE];
+  __shared__ uint16_t s_inc[3 * TILE];
+  __shared__ int s_is[MAXN + 1];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   if (tile >= a.n_tiles) return;
   const int tid = threadIdx.x;
   const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
   const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
+  const int ib0 = 3 * t0;  // a tile's incidence block is [3 t0, 3 (t0 + nt))
 
+  int g[NS], pi[NS], is[NS];
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const int l = tid + r * BLOCK;
+    g[r] = 0;
+    pi[r] = -1;
+    is[r] = 0;
+    if (l < nn) {
+      g[r] = a.tile_nodes[n0 + l];
+      pi[r] = a.part_idx[n0 + l];
+      is[r] = a.inc_start[n0 + l] - ib0;
+    }
+  }
+  uint16_t iv[3 * TPT];
+#pragma unroll
+  for (int k = 0; k < 3 * TPT; ++k) {
+    const int idx = tid + k * BLOCK;
+    iv[k] = idx < 3 * nt ? a.inc[ib0 + idx] : (uint16_t)0;
+  }
   double P[TPT][P_COUNT];
   double Ds[TPT][3];
   uint32_t L[TPT];
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * kBlock;
+    const int lt = tid + r * BLOCK;
     L[r] = 0;
     if (lt < nt) {
       const int64_t s = t0 + lt;
@@ -272,16 +319,43 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(KArgs a) {
       }
     }
   }
-  for (int l = tid; l < nn; l += kBlock) {
-    const int g = a.tile_nodes[n0 + l];
-    s_u[l] = a.u[g];
-    if (MODE == M_JVP) s_v[l] = a.v[g];
-    if (FK == FVM_FLUX_D_NODAL) s_D[l] = exp(a.log_d[g]);
+  double ug[NS], vg[NS], Vg[NS], lg[NS];
+  int cg[NS];
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const int l = tid + r * BLOCK;
+    ug[r] = vg[r] = Vg[r] = lg[r] = 0.0;
+    cg[r] = 3;
+    if (l < nn) {
+      ug[r] = a.u[g[r]];
+      if (MODE == M_JVP) vg[r] = a.v[g[r]];
+      if (FK == FVM_FLUX_D_NODAL) lg[r] = a.log_d[g[r]];
+      if (pi[r] < 0) {
+        cg[r] = a.code[g[r]];
+        Vg[r] = a.cv[g[r]];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const int l = tid + r * BLOCK;
+    if (l < nn) {
+      s_u[l] = ug[r];
+      if (MODE == M_JVP) s_v[l] = vg[r];
+      if (FK == FVM_FLUX_D_NODAL) s_D[l] = exp(lg[r]);
+      s_is[l] = is[r];
+    }
+  }
+  if (tid == 0) s_is[nn] = 3 * nt;
+#pragma unroll
+  for (int k = 0; k < 3 * TPT; ++k) {
+    const int idx = tid + k * BLOCK;
+    if (idx < 3 * nt) s_inc[idx] = iv[k];
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * kBlock;
+    const int lt = tid + r * BLOCK;
     if (lt < nt) {
       const int ia = L[r] & 1023u, ib = (L[r] >> 10) & 1023u, ic = (L[r] >> 20) & 1023u;
       double Q[3];
@@ -297,37 +371,53 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(KArgs a) {
     }
   }
   __syncthreads();
-  for (int l = tid; l < nn; l += kBlock) {
-    double acc = 0.0;
-    const int k0 = a.inc_start[n0 + l], k1 = a.inc_start[n0 + l + 1];
-    for (int k = k0; k < k1; ++k) {
-      const int w = a.inc[k];
-      const int lt = w >> 2, sl = w & 3;
-      // update_du! (triangle_contributions.jl:10-15): i += Q3 - Q1, j += Q1 - Q2, k += Q2 - Q3
-      acc = acc + s_q[3 * lt + (sl == 0 ? 2 : sl - 1)] - s_q[3 * lt + sl];
-    }
-    const int pi = a.part_idx[n0 + l];
-    if (pi >= 0) {
-      a.partial[pi] = acc;
-    } else {
-      const int g = a.tile_nodes[n0 + l];
-      a.out[g] = finish<MODE>(a, g, acc, s_u[l], MODE == M_JVP ? s_v[l] : 0.0);
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const int l = tid + r * BLOCK;
+    if (l < nn) {
+      double acc = 0.0;
+      const int k1 = s_is[l + 1];
+      for (int k = s_is[l]; k < k1; ++k) {
+        const int w = s_inc[k];
+        const int lt = w >> 2, sl = w & 3;
+        // update_du! (triangle_contributions.jl:10-15): i += Q3 - Q1, j += Q1 - Q2, k += Q2 - Q3
+        acc = acc + s_q[3 * lt + (sl == 0 ? 2 : sl - 1)] - s_q[3 * lt + sl];
+      }
+      if (pi[r] >= 0)
+        a.partial[pi[r]] = acc;
+      else
+        a.out[g[r]] = finish_pre<MODE>(a, g[r], acc, ug[r], vg[r], cg[r], Vg[r]);
     }
   }
 }
 
+// Shared nodes: partial sums in tile order, then the boundary-edge fluxes, then
+// the node rule.  The first four slots are loaded without a loop-carried
+// dependency (nearly every node has at most four).
 template <int FK, int MODE>
 __global__ __launch_bounds__(256) void fix_kernel(KArgs a) {
   const int fi = blockIdx.x * blockDim.x + threadIdx.x;
   if (fi >= a.n_fix) return;
   const int g = a.fix_nodes[fi];
+  const int p0 = a.fix_part_start[fi], p1 = a.fix_part_start[fi + 1];
+  const int b0 = a.fix_bedge_start[fi], b1 = a.fix_bedge_start[fi + 1];
+  const double ug = a.u[g];
+  const double vg = MODE == M_JVP ? a.v[g] : 0.0;
+  const int c = a.code[g];
+  const double V = a.cv[g];
+  double pv[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) pv[k] = p0 + k < p1 ? a.partial[p0 + k] : 0.0;
   double acc = 0.0;
-  for (int p = a.fix_part_start[fi]; p < a.fix_part_start[fi + 1]; ++p) acc = acc + a.partial[p];
-  for (int k = a.fix_bedge_start[fi]; k < a.fix_bedge_start[fi + 1]; ++k) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (p0 + k < p1) acc = acc + pv[k];
+  for (int p = p0 + 4; p < p1; ++p) acc = acc + a.partial[p];
+  for (int k = b0; k < b1; ++k) {
     const int w = a.fix_bedge[k];
     acc = acc - bedge_flux<FK, MODE>(a, w >> 1, w & 1);
   }
-  a.out[g] = finish<MODE>(a, g, acc, a.u[g], MODE == M_JVP ? a.v[g] : 0.0);
+  a.out[g] = finish_pre<MODE>(a, g, acc, ug, vg, c, V);
 }
 
 // ---------------------------------------------------------------------------
@@ -367,13 +457,15 @@ __device__ __forceinline__ void edge_adj(const KArgs& a, double Qbar, double x, 
   Dbar += Qbar * (-l * gn);
 }
 
-template <int FK, int TPT>
-__global__ __launch_bounds__(kBlock) void vjp_tile_kernel(KArgs a) {
-  constexpr int TILE = kBlock * TPT;
-  __shared__ double s_u[kMaxNodes];
-  __shared__ double s_w[kMaxNodes];
-  __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? kMaxNodes : 1];
-  __shared__ double s_c[6 * TILE];
+template <int FK, int BLOCK, int TPT>
+__global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
+  constexpr int TILE = BLOCK * TPT;
+  constexpr int MAXN = node_cap(BLOCK);
+  constexpr int CW = FK == FVM_FLUX_D_NODAL ? 6 : 3;  // u-adjoint (+ log D adjoint) per vertex
+  __shared__ double s_u[MAXN];
+  __shared__ double s_w[MAXN];
+  __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
+  __shared__ double s_c[CW * TILE];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   if (tile >= a.n_tiles) return;
   const int tid = threadIdx.x;
@@ -384,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void vjp_tile_kernel(KArgs a) {
   uint32_t L[TPT];
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * kBlock;
+    const int lt = tid + r * BLOCK;
     L[r] = 0;
     if (lt < nt) {
       const int64_t s = t0 + lt;
@@ -398,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void vjp_tile_kernel(KArgs a) {
       }
     }
   }
-  for (int l = tid; l < nn; l += kBlock) {
+  for (int l = tid; l < nn; l += BLOCK) {
     const int g = a.tile_nodes[n0 + l];
     s_u[l] = a.u[g];
     s_w[l] = wt_of(a, g);
@@ -407,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void vjp_tile_kernel(KArgs a) {
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * kBlock;
+    const int lt = tid + r * BLOCK;
     if (lt < nt) {
       const double* p = P[r];
       const int id[3] = {(int)(L[r] & 1023u), (int)((L[r] >> 10) & 1023u), (int)((L[r] >> 20) & 1023u)};
@@ -448,19 +540,19 @@ __global__ __launch_bounds__(kBlock) void vjp_tile_kernel(KArgs a) {
       for (int v = 0; v < 3; ++v) {
         double ub = p[v] * abar + p[3 + v] * bbar;
         if (FK == FVM_FLUX_D_POLY_U) ub = ub + p[6 + v] * gbar;  // gamma only enters D(u)
-        s_c[6 * lt + v] = ub;
-        s_c[6 * lt + 3 + v] = lbar[v] * Dn[v];
+        s_c[CW * lt + v] = ub;
+        if (FK == FVM_FLUX_D_NODAL) s_c[CW * lt + 3 + v] = lbar[v] * Dn[v];
       }
     }
   }
   __syncthreads();
-  for (int l = tid; l < nn; l += kBlock) {
+  for (int l = tid; l < nn; l += BLOCK) {
     double acc = 0.0, accl = 0.0;
     const int k0 = a.inc_start[n0 + l], k1 = a.inc_start[n0 + l + 1];
     for (int k = k0; k < k1; ++k) {
       const int w = a.inc[k];
-      acc += s_c[6 * (w >> 2) + (w & 3)];
-      accl += s_c[6 * (w >> 2) + 3 + (w & 3)];
+      acc += s_c[CW * (w >> 2) + (w & 3)];
+      if (FK == FVM_FLUX_D_NODAL) accl += s_c[CW * (w >> 2) + 3 + (w & 3)];
     }
     const int pi = a.part_idx[n0 + l];
     if (pi >= 0) {
@@ -600,15 +692,27 @@ KArgs make_args(const fvm_ctx* c) {
 }
 
 template <int FK, int MODE>
-int launch_fk(const KArgs& a, int tile_tris, hipStream_t s) {
+void launch_tiles(const KArgs& a, int tile_tris, hipStream_t s) {
   const dim3 grid((unsigned)xcd_grid(a.n_tiles));
-  if (tile_tris <= kBlock)
-    hipLaunchKernelGGL((tile_kernel<FK, MODE, 1>), grid, dim3(kBlock), 0, s, a);
+  if (tile_tris <= 256)
+    hipLaunchKernelGGL((tile_kernel<FK, MODE, 256, 1>), grid, dim3(256), 0, s, a);
+  else if (tile_tris <= 512)
+    hipLaunchKernelGGL((tile_kernel<FK, MODE, 256, 2>), grid, dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((tile_kernel<FK, MODE, 2>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((tile_kernel<FK, MODE, 512, 2>), grid, dim3(512), 0, s, a);
+}
+
+template <int FK, int MODE>
+void launch_fix(const KArgs& a, hipStream_t s) {
   if (a.n_fix > 0)
     hipLaunchKernelGGL((fix_kernel<FK, MODE>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0,
                        s, a);
+}
+
+template <int FK, int MODE>
+int launch_fk(const KArgs& a, int tile_tris, hipStream_t s) {
+  launch_tiles<FK, MODE>(a, tile_tris, s);
+  launch_fix<FK, MODE>(a, s);
   FVM_HIP(hipGetLastError());
   return FVM_OK;
 }
@@ -625,10 +729,12 @@ int launch(const fvm_ctx* c, const KArgs& a, hipStream_t s) {
 template <int FK>
 int launch_vjp_fk(const KArgs& a, int tile_tris, hipStream_t s) {
   const dim3 grid((unsigned)xcd_grid(a.n_tiles));
-  if (tile_tris <= kBlock)
-    hipLaunchKernelGGL((vjp_tile_kernel<FK, 1>), grid, dim3(kBlock), 0, s, a);
+  if (tile_tris <= 256)
+    hipLaunchKernelGGL((vjp_tile_kernel<FK, 256, 1>), grid, dim3(256), 0, s, a);
+  else if (tile_tris <= 512)
+    hipLaunchKernelGGL((vjp_tile_kernel<FK, 256, 2>), grid, dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((vjp_tile_kernel<FK, 2>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((vjp_tile_kernel<FK, 512, 2>), grid, dim3(512), 0, s, a);
   if (a.n_fix > 0)
     hipLaunchKernelGGL((vjp_fix_kernel<FK>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0, s,
                        a);
@@ -784,28 +890,16 @@ int fvm_rhs_timed(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_str
   FVM_HIP(hipEventRecord(ev[0], st));
   int rc = FVM_OK;
   const int tt = c->m->tile_tris;
-  const dim3 grid((unsigned)xcd_grid(a.n_tiles));
   switch (c->flux_kind) {
-    case FVM_FLUX_D_POLY_U:
-      if (tt <= kBlock) hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_POLY_U, M_RHS, 1>), grid, dim3(kBlock), 0, st, a);
-      else hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_POLY_U, M_RHS, 2>), grid, dim3(kBlock), 0, st, a);
-      break;
-    case FVM_FLUX_D_NODAL:
-      if (tt <= kBlock) hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_NODAL, M_RHS, 1>), grid, dim3(kBlock), 0, st, a);
-      else hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_NODAL, M_RHS, 2>), grid, dim3(kBlock), 0, st, a);
-      break;
-    default:
-      if (tt <= kBlock) hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_EDGE, M_RHS, 1>), grid, dim3(kBlock), 0, st, a);
-      else hipLaunchKernelGGL((tile_kernel<FVM_FLUX_D_EDGE, M_RHS, 2>), grid, dim3(kBlock), 0, st, a);
+    case FVM_FLUX_D_POLY_U: launch_tiles<FVM_FLUX_D_POLY_U, M_RHS>(a, tt, st); break;
+    case FVM_FLUX_D_NODAL: launch_tiles<FVM_FLUX_D_NODAL, M_RHS>(a, tt, st); break;
+    default: launch_tiles<FVM_FLUX_D_EDGE, M_RHS>(a, tt, st);
   }
   FVM_HIP(hipEventRecord(ev[1], st));
-  if (a.n_fix > 0) {
-    const dim3 fg((unsigned)((a.n_fix + 255) / 256));
-    switch (c->flux_kind) {
-      case FVM_FLUX_D_POLY_U: hipLaunchKernelGGL((fix_kernel<FVM_FLUX_D_POLY_U, M_RHS>), fg, dim3(256), 0, st, a); break;
-      case FVM_FLUX_D_NODAL: hipLaunchKernelGGL((fix_kernel<FVM_FLUX_D_NODAL, M_RHS>), fg, dim3(256), 0, st, a); break;
-      default: hipLaunchKernelGGL((fix_kernel<FVM_FLUX_D_EDGE, M_RHS>), fg, dim3(256), 0, st, a);
-    }
+  switch (c->flux_kind) {
+    case FVM_FLUX_D_POLY_U: launch_fix<FVM_FLUX_D_POLY_U, M_RHS>(a, st); break;
+    case FVM_FLUX_D_NODAL: launch_fix<FVM_FLUX_D_NODAL, M_RHS>(a, st); break;
+    default: launch_fix<FVM_FLUX_D_EDGE, M_RHS>(a, st);
   }
   FVM_HIP(hipEventRecord(ev[2], st));
   hipError_t e = hipGetLastError();

@@ -62,8 +62,8 @@ typedef struct fvm_mesh_desc {
   const int32_t* triangles;  /* [3*n_triangles], counter-clockwise (each_solid_triangle) */
   const int32_t* boundary_edges; /* [2*n_boundary_edges], counter-clockwise, interior on the left
                                     (keys of get_boundary_edge_map) */
-  int32_t tile_triangles;    /* 0: default (512) */
-  int32_t tile_max_nodes;    /* 0: default (640) */
+  int32_t tile_triangles;    /* triangles per tile, <= 1024; 0: default (512) */
+  int32_t tile_max_nodes;    /* vertices per tile, <= 512 (<= 1024 above 512 triangles); 0: the limit */
 } fvm_mesh_desc;
 
 /* Replaces FVMGeometry(tri) (geometry.jl:99): uploads the mesh, builds the
