@@ -94,9 +94,14 @@ struct fvm_mesh {
   int32_t *bedges = nullptr, *badj = nullptr;  // bedges [2][B]
   int32_t* perm = nullptr;                      // sorted -> input triangle
   uint8_t* is_vertex = nullptr;
+  double* lcv = nullptr;            // [n_local] control volume of each local-node entry
+  int32_t n_bedge_items = 0;        // entries of fix_bedge (boundary edge endpoints)
+  int32_t n_bvert_items = 0;        // entries of fix_bvert (boundary triangle vertices)
   // host copies needed to build contexts
   std::vector<int32_t> h_perm;
   std::vector<int32_t> h_node_tri_start;
+  std::vector<int32_t> h_part_idx;    // [n_local]
+  std::vector<int32_t> h_tile_nodes;  // [n_local]
 };
 
 struct fvm_ctx {
@@ -120,6 +125,8 @@ struct fvm_ctx {
   int32_t* dir_nodes = nullptr;
   int32_t n_dir = 0;
   double* partial = nullptr;   // [n_parts] (x2 for the VJP)
+  int32_t* lpart = nullptr;    // [n_local] partial slot, or -1 - node code for exclusive entries
+  double* bflux = nullptr;     // boundary terms per fix_bedge item (RHS/JVP) or 2 per fix_bvert item (VJP)
   fvm::CondTable cond() const {
     fvm::CondTable C;
     C.n = n_fn;

@@ -102,12 +102,20 @@ __global__ __launch_bounds__(256) void volume_kernel(int64_t N, int64_t T,
   is_vertex[g] = k1 > k0;
 }
 
+// Control volume of every local-node entry, read contiguously by the tile kernels.
+__global__ __launch_bounds__(256) void local_cv_kernel(int64_t n, const int32_t* __restrict__ nodes,
+                                                       const double* __restrict__ cv,
+                                                       double* __restrict__ lcv) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) lcv[e] = cv[nodes[e]];
+}
+
 void free_mesh(fvm_mesh* m) {
   if (!m) return;
   void* ptrs[] = {m->x, m->y, m->cv, m->props, m->tri, m->tile_tri_start, m->tile_node_start,
                   m->tile_nodes, m->loc, m->inc_start, m->inc, m->part_idx, m->fix_nodes,
                   m->fix_part_start, m->fix_bedge_start, m->fix_bedge, m->fix_bvert_start,
-                  m->fix_bvert, m->node_tri_start,
+                  m->fix_bvert, m->lcv, m->node_tri_start,
                   m->node_tri, m->bedges, m->badj, m->perm, m->is_vertex};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -177,7 +185,12 @@ int build(const fvm_mesh_desc* d, fvm_mesh* m) {
   UP(m->props, (const double*)nullptr, P_COUNT * T);
   UP(m->cv, (const double*)nullptr, N);
   UP(m->is_vertex, (const uint8_t*)nullptr, N);
+  UP(m->lcv, (const double*)nullptr, m->n_local);
 #undef UP
+  m->n_bedge_items = tl->fix_bedge_start[m->n_fix];
+  m->n_bvert_items = tl->fix_bvert_start[m->n_fix];
+  m->h_part_idx.assign(tl->part_idx, tl->part_idx + m->n_local);
+  m->h_tile_nodes.assign(tl->tile_nodes, tl->tile_nodes + m->n_local);
   m->h_perm.assign(tl->perm, tl->perm + T);
   m->h_node_tri_start.assign(tl->node_tri_start, tl->node_tri_start + N + 1);
   double* sub = nullptr;
@@ -188,6 +201,9 @@ int build(const fvm_mesh_desc* d, fvm_mesh* m) {
   if (N > 0)
     hipLaunchKernelGGL(volume_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, 0, N, T,
                        m->node_tri_start, m->node_tri, sub, m->cv, m->is_vertex);
+  if (m->n_local > 0)
+    hipLaunchKernelGGL(local_cv_kernel, dim3((unsigned)((m->n_local + 255) / 256)), dim3(256), 0, 0,
+                       m->n_local, m->tile_nodes, m->cv, m->lcv);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipDeviceSynchronize();
   (void)hipFree(sub);

@@ -17,6 +17,7 @@
 // Arithmetic follows the reference's operation order with FMA contraction off
 // (-ffp-contract=off), so each triangle's fluxes equal the CPU restatement's;
 // only the order of the per-node sums differs.
+#include <algorithm>
 #include <vector>
 
 #include "fvm_internal.hpp"
@@ -29,6 +30,7 @@ enum { M_RHS = 0, M_JVP = 1 };
 
 struct KArgs {
   int32_t n_tiles, n_fix;
+  int32_t n_bitems;  // boundary work items computed by the tile kernel's leading workgroups
   int64_t T, B;
   const int32_t* tile_tri_start;
   const int32_t* tile_node_start;
@@ -66,6 +68,9 @@ struct KArgs {
   double* out2;
   double* partial;
   double* partial2;
+  const int32_t* lpart;  // per local entry: partial slot, or -1 - code (exclusive node)
+  const double* lcv;     // per local entry: control volume
+  double* bflux;         // boundary terms per work item
   double t;
 };
 
@@ -274,9 +279,20 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
   __shared__ double s_q[3 * TILE];
   __shared__ uint16_t s_inc[3 * TILE];
   __shared__ int s_is[MAXN + 1];
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  if (tile >= a.n_tiles) return;
+  const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
+  const int wid = xcd_remap(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
+  if (wid < nbw) {
+    // leading workgroups: the boundary-edge fluxes, one thread per (edge, endpoint)
+    const int k = wid * BLOCK + tid;
+    if (k < a.n_bitems) {
+      const int w = a.fix_bedge[k];
+      a.bflux[k] = bedge_flux<FK, MODE>(a, w >> 1, w & 1);
+    }
+    return;
+  }
+  const int tile = wid - nbw;
+  if (tile >= a.n_tiles) return;
   const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
   const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
   const int ib0 = 3 * t0;  // a tile's incidence block is [3 t0, 3 (t0 + nt))
@@ -290,7 +306,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
     is[r] = 0;
     if (l < nn) {
       g[r] = a.tile_nodes[n0 + l];
-      pi[r] = a.part_idx[n0 + l];
+      pi[r] = a.lpart[n0 + l];
       is[r] = a.inc_start[n0 + l] - ib0;
     }
   }
@@ -331,8 +347,8 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
       if (MODE == M_JVP) vg[r] = a.v[g[r]];
       if (FK == FVM_FLUX_D_NODAL) lg[r] = a.log_d[g[r]];
       if (pi[r] < 0) {
-        cg[r] = a.code[g[r]];
-        Vg[r] = a.cv[g[r]];
+        cg[r] = -1 - pi[r];
+        Vg[r] = a.lcv[n0 + l];
       }
     }
   }
@@ -391,8 +407,8 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
   }
 }
 
-// Shared nodes: partial sums in tile order, then the boundary-edge fluxes, then
-// the node rule.  The first four slots are loaded without a loop-carried
+// Shared nodes: partial sums in tile order, then the boundary-edge fluxes (computed
+// by the tile kernel's leading workgroups), then the node rule.  The first four slots are loaded without a loop-carried
 // dependency (nearly every node has at most four).
 template <int FK, int MODE>
 __global__ __launch_bounds__(256) void fix_kernel(KArgs a) {
@@ -413,10 +429,7 @@ __global__ __launch_bounds__(256) void fix_kernel(KArgs a) {
   for (int k = 0; k < 4; ++k)
     if (p0 + k < p1) acc = acc + pv[k];
   for (int p = p0 + 4; p < p1; ++p) acc = acc + a.partial[p];
-  for (int k = b0; k < b1; ++k) {
-    const int w = a.fix_bedge[k];
-    acc = acc - bedge_flux<FK, MODE>(a, w >> 1, w & 1);
-  }
+  for (int k = b0; k < b1; ++k) acc = acc - a.bflux[k];
   a.out[g] = finish_pre<MODE>(a, g, acc, ug, vg, c, V);
 }
 
@@ -457,6 +470,50 @@ __device__ __forceinline__ void edge_adj(const KArgs& a, double Qbar, double x, 
   Dbar += Qbar * (-l * gn);
 }
 
+// Adjoint of both boundary CV-edge fluxes of boundary edge e with respect to the
+// vertex in slot `slot` of its triangle.
+template <int FK>
+__device__ void bedge_adj(const KArgs& a, int e, int slot, double& ubar, double& lbar) {
+  const BGeo G = bgeo(a, e);
+  const int64_t Tt = a.badj[e];
+  const int vtx[3] = {a.tri[Tt], a.tri[a.T + Tt], a.tri[2 * a.T + Tt]};
+  double s[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) s[f] = a.props[f * a.T + Tt];
+  const double ua = a.u[vtx[0]], ub = a.u[vtx[1]], uc = a.u[vtx[2]];
+  const double al = s[0] * ua + s[1] * ub + s[2] * uc;
+  const double be = s[3] * ua + s[4] * ub + s[5] * uc;
+  const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
+  double Dn[3] = {0, 0, 0}, da = 0, db = 0, dg = 0;
+  if (FK == FVM_FLUX_D_NODAL) {
+    for (int v = 0; v < 3; ++v) Dn[v] = exp(a.log_d[vtx[v]]);
+    da = s[0] * Dn[0] + s[1] * Dn[1] + s[2] * Dn[2];
+    db = s[3] * Dn[0] + s[4] * Dn[1] + s[5] * Dn[2];
+    dg = s[6] * Dn[0] + s[7] * Dn[1] + s[8] * Dn[2];
+  }
+  const int nf = a.bneu[e];
+  double abar = 0, bbar = 0, gbar = 0, lb = 0;
+  for (int ep = 0; ep < 2; ++ep) {
+    const int node = a.bedges[ep * a.B + e];
+    const double Qbar = -wt_of(a, node);  // du_node -= Q
+    const double mx = G.mx[ep], my = G.my[ep];
+    if (nf < 0) {
+      double Dg = 0, Dbar = 0;
+      if (FK == FVM_FLUX_D_NODAL) Dg = da * mx + db * my + dg;
+      if (FK == FVM_FLUX_D_EDGE) Dg = a.d_bedge[2 * e + ep];
+      edge_adj<FK>(a, Qbar, mx, my, G.nx, G.ny, G.ell, al, be, ga, Dg, abar, bbar, gbar, Dbar);
+      if (FK == FVM_FLUX_D_NODAL) lb += Dbar * (s[slot] * mx + s[3 + slot] * my + s[6 + slot]);
+    } else {
+      const double c = Qbar * G.ell * a.C.cu[nf];
+      abar += c * mx;
+      bbar += c * my;
+      gbar += c;
+    }
+  }
+  ubar += s[slot] * abar + s[3 + slot] * bbar + s[6 + slot] * gbar;
+  lbar += lb * Dn[slot];
+}
+
 template <int FK, int BLOCK, int TPT>
 __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
   constexpr int TILE = BLOCK * TPT;
@@ -466,9 +523,23 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
   __shared__ double s_w[MAXN];
   __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
   __shared__ double s_c[CW * TILE];
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  if (tile >= a.n_tiles) return;
+  const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
+  const int wid = xcd_remap(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
+  if (wid < nbw) {
+    // leading workgroups: adjoints of the boundary-edge fluxes per (edge, triangle vertex)
+    const int k = wid * BLOCK + tid;
+    if (k < a.n_bitems) {
+      const int w = a.fix_bvert[k];
+      double ub = 0.0, lb = 0.0;
+      bedge_adj<FK>(a, w >> 2, w & 3, ub, lb);
+      a.bflux[2 * k] = ub;
+      a.bflux[2 * k + 1] = lb;
+    }
+    return;
+  }
+  const int tile = wid - nbw;
+  if (tile >= a.n_tiles) return;
   const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
   const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
   double P[TPT][P_COUNT];
@@ -566,50 +637,6 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
   }
 }
 
-// Adjoint of both boundary CV-edge fluxes of boundary edge e with respect to the
-// vertex in slot `slot` of its triangle.
-template <int FK>
-__device__ void bedge_adj(const KArgs& a, int e, int slot, double& ubar, double& lbar) {
-  const BGeo G = bgeo(a, e);
-  const int64_t Tt = a.badj[e];
-  const int vtx[3] = {a.tri[Tt], a.tri[a.T + Tt], a.tri[2 * a.T + Tt]};
-  double s[9];
-#pragma unroll
-  for (int f = 0; f < 9; ++f) s[f] = a.props[f * a.T + Tt];
-  const double ua = a.u[vtx[0]], ub = a.u[vtx[1]], uc = a.u[vtx[2]];
-  const double al = s[0] * ua + s[1] * ub + s[2] * uc;
-  const double be = s[3] * ua + s[4] * ub + s[5] * uc;
-  const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
-  double Dn[3] = {0, 0, 0}, da = 0, db = 0, dg = 0;
-  if (FK == FVM_FLUX_D_NODAL) {
-    for (int v = 0; v < 3; ++v) Dn[v] = exp(a.log_d[vtx[v]]);
-    da = s[0] * Dn[0] + s[1] * Dn[1] + s[2] * Dn[2];
-    db = s[3] * Dn[0] + s[4] * Dn[1] + s[5] * Dn[2];
-    dg = s[6] * Dn[0] + s[7] * Dn[1] + s[8] * Dn[2];
-  }
-  const int nf = a.bneu[e];
-  double abar = 0, bbar = 0, gbar = 0, lb = 0;
-  for (int ep = 0; ep < 2; ++ep) {
-    const int node = a.bedges[ep * a.B + e];
-    const double Qbar = -wt_of(a, node);  // du_node -= Q
-    const double mx = G.mx[ep], my = G.my[ep];
-    if (nf < 0) {
-      double Dg = 0, Dbar = 0;
-      if (FK == FVM_FLUX_D_NODAL) Dg = da * mx + db * my + dg;
-      if (FK == FVM_FLUX_D_EDGE) Dg = a.d_bedge[2 * e + ep];
-      edge_adj<FK>(a, Qbar, mx, my, G.nx, G.ny, G.ell, al, be, ga, Dg, abar, bbar, gbar, Dbar);
-      if (FK == FVM_FLUX_D_NODAL) lb += Dbar * (s[slot] * mx + s[3 + slot] * my + s[6 + slot]);
-    } else {
-      const double c = Qbar * G.ell * a.C.cu[nf];
-      abar += c * mx;
-      bbar += c * my;
-      gbar += c;
-    }
-  }
-  ubar += s[slot] * abar + s[3 + slot] * bbar + s[6 + slot] * gbar;
-  lbar += lb * Dn[slot];
-}
-
 template <int FK>
 __global__ __launch_bounds__(256) void vjp_fix_kernel(KArgs a) {
   const int fi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -621,8 +648,8 @@ __global__ __launch_bounds__(256) void vjp_fix_kernel(KArgs a) {
     accl += a.partial2[p];
   }
   for (int k = a.fix_bvert_start[fi]; k < a.fix_bvert_start[fi + 1]; ++k) {
-    const int w = a.fix_bvert[k];
-    bedge_adj<FK>(a, w >> 2, w & 3, acc, accl);
+    acc += a.bflux[2 * k];
+    accl += a.bflux[2 * k + 1];
   }
   a.out[g] = acc + node_adj(a, g, a.u[g]);
   if (a.out2) a.out2[g] = accl;
@@ -687,19 +714,27 @@ KArgs make_args(const fvm_ctx* c) {
   a.out = a.out2 = nullptr;
   a.partial = c->partial;
   a.partial2 = c->partial + (m->n_parts > 0 ? m->n_parts : 1);
+  a.lpart = c->lpart;
+  a.lcv = m->lcv;
+  a.bflux = c->bflux;
+  a.n_bitems = m->n_bedge_items;
   a.t = 0;
   return a;
 }
 
+// Grid: the boundary workgroups followed by one workgroup per tile.
+inline dim3 tile_grid(const KArgs& a, int block) {
+  return dim3((unsigned)xcd_grid(a.n_tiles + (a.n_bitems + block - 1) / block));
+}
+
 template <int FK, int MODE>
 void launch_tiles(const KArgs& a, int tile_tris, hipStream_t s) {
-  const dim3 grid((unsigned)xcd_grid(a.n_tiles));
   if (tile_tris <= 256)
-    hipLaunchKernelGGL((tile_kernel<FK, MODE, 256, 1>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((tile_kernel<FK, MODE, 256, 1>), tile_grid(a, 256), dim3(256), 0, s, a);
   else if (tile_tris <= 512)
-    hipLaunchKernelGGL((tile_kernel<FK, MODE, 256, 2>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((tile_kernel<FK, MODE, 256, 2>), tile_grid(a, 256), dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((tile_kernel<FK, MODE, 512, 2>), grid, dim3(512), 0, s, a);
+    hipLaunchKernelGGL((tile_kernel<FK, MODE, 512, 2>), tile_grid(a, 512), dim3(512), 0, s, a);
 }
 
 template <int FK, int MODE>
@@ -728,13 +763,12 @@ int launch(const fvm_ctx* c, const KArgs& a, hipStream_t s) {
 
 template <int FK>
 int launch_vjp_fk(const KArgs& a, int tile_tris, hipStream_t s) {
-  const dim3 grid((unsigned)xcd_grid(a.n_tiles));
   if (tile_tris <= 256)
-    hipLaunchKernelGGL((vjp_tile_kernel<FK, 256, 1>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((vjp_tile_kernel<FK, 256, 1>), tile_grid(a, 256), dim3(256), 0, s, a);
   else if (tile_tris <= 512)
-    hipLaunchKernelGGL((vjp_tile_kernel<FK, 256, 2>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((vjp_tile_kernel<FK, 256, 2>), tile_grid(a, 256), dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((vjp_tile_kernel<FK, 512, 2>), grid, dim3(512), 0, s, a);
+    hipLaunchKernelGGL((vjp_tile_kernel<FK, 512, 2>), tile_grid(a, 512), dim3(512), 0, s, a);
   if (a.n_fix > 0)
     hipLaunchKernelGGL((vjp_fix_kernel<FK>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0, s,
                        a);
@@ -753,7 +787,8 @@ int dev_copy(T** dst, const T* src, int64_t n) {
 void free_ctx(fvm_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->d_edge, c->d_bedge, c->log_d_own, c->s_node, c->kind, c->fidx,
-                  c->bneu, c->fn, c->node_g, c->bedge_g, c->dir_nodes, c->partial};
+                  c->bneu, c->fn, c->node_g, c->bedge_g, c->dir_nodes, c->partial,
+                  c->lpart, c->bflux};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete c;
@@ -833,6 +868,19 @@ int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
   c->n_dir = (int32_t)dir.size();
   DC(c->dir_nodes, dir.data(), (int64_t)dir.size());
   DC(c->partial, (const double*)nullptr, 2 * (int64_t)(m->n_parts > 0 ? m->n_parts : 1));
+  {
+    // exclusive entries carry their node's code: -1 - code
+    std::vector<int32_t> lp(m->n_local);
+    for (int64_t e = 0; e < m->n_local; ++e) {
+      const int32_t p = m->h_part_idx[e];
+      lp[e] = p >= 0 ? p : -1 - (int32_t)code[m->h_tile_nodes[e]];
+    }
+    DC(c->lpart, lp.data(), m->n_local);
+  }
+  {
+    const int64_t nb = std::max<int64_t>(m->n_bedge_items, 2 * (int64_t)m->n_bvert_items);
+    DC(c->bflux, (const double*)nullptr, nb);
+  }
 #undef DC
   return FVM_OK;
 }
@@ -934,6 +982,7 @@ int fvm_vjp(fvm_ctx* c, const double* d_u, const double* d_w, double* d_ubar, do
   a.out = d_ubar;
   a.out2 = d_logd_bar;
   a.t = t;
+  a.n_bitems = c->m->n_bvert_items;
   hipStream_t st = (hipStream_t)s;
   switch (c->flux_kind) {
     case FVM_FLUX_D_POLY_U: return launch_vjp_fk<FVM_FLUX_D_POLY_U>(a, c->m->tile_tris, st);

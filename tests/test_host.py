@@ -109,7 +109,8 @@ def test_rectangle_numbering_matches_reference_layout():
     assert tri.num_points() == 40000 and len(tri.triangles) == 2 * 199 * 199
     # counter-clockwise triangles
     p = tri.points[tri.triangles]
-    area2 = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    d1, d2 = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]
+    area2 = d1[:, 0] * d2[:, 1] - d1[:, 1] * d2[:, 0]
     assert np.all(area2 > 0)
     # boundary edges: interior on the left; segments bottom, right, top, left
     e = tri.boundary_edges
@@ -132,8 +133,9 @@ def test_jittered_mesh_is_the_delaunay_triangulation(n, seed):
     # boundary cells with collinear boundary points keep a valid Delaunay choice)
     ours = {tuple(sorted(T)) for T in tri.triangles.tolist()}
     qh = Delaunay(tri.points)
-    area = np.abs(np.cross(tri.points[qh.simplices[:, 1]] - tri.points[qh.simplices[:, 0]],
-                           tri.points[qh.simplices[:, 2]] - tri.points[qh.simplices[:, 0]]))
+    d1 = tri.points[qh.simplices[:, 1]] - tri.points[qh.simplices[:, 0]]
+    d2 = tri.points[qh.simplices[:, 2]] - tri.points[qh.simplices[:, 0]]
+    area = np.abs(d1[:, 0] * d2[:, 1] - d1[:, 1] * d2[:, 0])
     theirs = {tuple(sorted(T)) for T, a in zip(qh.simplices.tolist(), area) if a > 1e-14}
     assert ours == theirs
 
