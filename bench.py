@@ -118,6 +118,59 @@ def vjp_ratio(n_side, reps=20):
                 rhs_ms=round(t_rhs, 4), vjp_ms=round(t_vjp, 4), ratio=round(t_vjp / t_rhs, 3))
 
 
+def template_solve(n_side, rtol=1e-10):
+    """The steady template workload: PoissonsEquation on an n_side x n_side jittered
+    mesh of [0, 1]^2, homogeneous Dirichlet, D = exp(0.5 sin 4 pi x cos 4 pi y),
+    f ~ U[-1, 1].  Time to |b - A x| <= rtol |b| for each device solver (after one
+    warm-up solve), and the SpMV rates of the solver operator."""
+    import finitevolumemethod as FVM
+    from finitevolumemethod import _lib
+    from finitevolumemethod import templates as TM
+
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n_side, n_side, jitter=0.25, seed=SEED + 3)
+    mesh = FVM.FVMGeometry(tri)
+    N = tri.num_points()
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    f = np.random.default_rng(SEED + 4).uniform(-1.0, 1.0, N)
+    t0 = time.time()
+    prob = FVM.PoissonsEquation(
+        mesh, BCs, diffusion_function=lambda x, y, p: np.exp(0.5 * np.sin(4 * np.pi * x) * np.cos(4 * np.pi * y)),
+        source_function=lambda x, y, p: p, source_parameters=f)
+    t_build = time.time() - t0
+    dt = prob.problem
+    out = dict(workload=f"PoissonsEquation {n_side}x{n_side} jittered, D=exp(0.5 sin4pix cos4piy), "
+                        "f~U[-1,1], homogeneous Dirichlet", nodes=int(N), nnz=int(dt.nnz),
+               rtol=rtol, build_seconds=round(t_build, 3))
+    sols = {}
+    for name, alg in (("cg_fp64", TM.CG()), ("mixed_fp32_fp64", TM.MixedCG()),
+                      ("bicgstab_fp64", TM.BiCGStab())):
+        dt.solve(alg, rtol=rtol)
+        x, st = dt.solve(alg, rtol=rtol)
+        sols[name] = x.to_host()
+        out[name] = dict(seconds=round(st["seconds"], 4), iterations=st["iterations"],
+                         outer=st["outer"], residual=st["residual"], converged=st["converged"])
+        log(f"template {name}: {st}")
+    ref = sols["cg_fp64"]
+    out["max_rel_diff_vs_cg"] = {k: float(np.linalg.norm(v - ref) / np.linalg.norm(ref))
+                                 for k, v in sols.items() if k != "cg_fp64"}
+    out["mixed_speedup_vs_cg"] = round(out["cg_fp64"]["seconds"] / out["mixed_fp32_fp64"]["seconds"], 3)
+    x = _lib.DeviceArray.from_host(ref)
+    y = _lib.DeviceArray(N)
+    rows = N
+    for which, key, per_nnz, per_row in ((1, "spmv_fp64", 12, 20), (2, "spmv_fp32", 8, 12)):
+        for _ in range(3):
+            dt.spmv(which, x, y)
+        tm = _lib.Timer()
+        tm.start()
+        reps = 50
+        for _ in range(reps):
+            dt.spmv(which, x, y)
+        ms = tm.stop() / reps
+        out[key] = dict(us=round(ms * 1e3, 2),
+                        GBps=round((per_nnz * dt.nnz + per_row * rows) / (ms * 1e-3) / 1e9, 1))
+    return out
+
+
 def cpu_baseline(tri, u, budget_s, nthreads):
     """The oracle's restatement of parallel_fvm_eqs! (per-thread du columns, serial
     combine) on the same mesh and state, for about budget_s seconds."""
@@ -162,6 +215,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="triangles per tile (0: default 512)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--template-side", type=int, default=1000,
+                    help="PoissonsEquation solve workload side (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the finite-volume path runs on a single MI355X here "
@@ -227,6 +282,8 @@ def main():
         log(f"copy kernel {out['copy_bandwidth_GBps']} GB/s")
         out["vjp_rhs"] = vjp_ratio(args.vjp_side)
         log(f"VJP/RHS {out['vjp_rhs']}")
+        if args.template_side > 0:
+            out["template_solve"] = template_solve(args.template_side)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         du_cpu, cb = cpu_baseline(tri, u, args.cpu_seconds, threads)

@@ -1,0 +1,967 @@
+// Linear templates on the device: CSR assembly of the template operators, SpMV,
+// and the steady solvers that replace the reference's sparse direct solve.
+//
+// Reference: src/specific_problems/abstract_templates.jl:73-325 (triangle and
+// boundary-edge contributions, right-hand sides, Dirichlet rows, missing
+// vertices), poissons_equation.jl:58-88, laplaces_equation.jl:51-78,
+// mean_exit_time.jl:57-94, diffusion_equation.jl:69-101,
+// linear_reaction_diffusion_equations.jl:76-125.  The reference fills a dense
+// n x n matrix and converts it with sparse(); here each row is assembled by one
+// thread into the CSR pattern of jacobian_sparsity (the node and its mesh
+// neighbours, solve.jl:50-77), in a fixed order, so the result is deterministic.
+//
+// Steady solves.  A (rows scaled by 1/V_i, Dirichlet and missing-vertex rows =
+// identity) is not symmetric.  M = -diag(V) A restricted to the free rows and
+// columns (identity on the fixed rows) is the P1 stiffness matrix when D is
+// constant per triangle and close to symmetric otherwise, so Jacobi-preconditioned
+// CG on M d = -V (b - A x) converges; the stopping test is always the FP64 true
+// residual |b - A x|_2 <= rtol |b|_2 on A itself.
+//   FVM_SOLVE_CG        FP64 CG; a restart from the true residual if the
+//                       recursive residual has drifted.
+//   FVM_SOLVE_CG_MIXED  iterative refinement: the correction comes from CG with
+//                       M, vectors and arithmetic in FP32 (dot products summed in
+//                       FP64), stopped at a relative drop of inner_rtol; the
+//                       residual and the solution are FP64.
+//   FVM_SOLVE_BICGSTAB  FP64 BiCGStab on M with Jacobi right preconditioning.
+// Every reduction finishes in the last workgroup to arrive, in a fixed order, so
+// solver scalars stay on the device and results are bitwise reproducible; the
+// host reads the scalars only every check_every iterations.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "fvm_internal.hpp"
+
+using namespace fvm;
+
+namespace {
+
+constexpr int kRT = 256;     // threads per workgroup of every solver kernel
+constexpr int kMaxG = 2048;  // reduction grid (fixed per template)
+constexpr int kLPR = 8;      // lanes per CSR row in the SpMV kernels
+constexpr int kRowsPerBlock = kRT / kLPR;
+
+// solver scalars (device doubles)
+enum {
+  S_RZ = 0, S_ALPHA, S_BETA, S_WR, S_WR0, S_TOL2, S_ITER, S_DONE, S_RES2, S_B2,
+  S_RHO, S_OMEGA, S_FLAG, S_COUNT = 16
+};
+
+}  // namespace
+
+struct fvm_template {
+  const fvm_mesh* m = nullptr;
+  int32_t kind = 0;
+  int64_t n = 0, nnz = 0;
+  int32_t* row_ptr = nullptr;
+  int32_t* col = nullptr;
+  double* val = nullptr;     // A
+  double* valM = nullptr;    // M, FP64
+  float* valM32 = nullptr;   // M, FP32
+  double* b = nullptr;
+  uint8_t* fixed = nullptr;  // identity rows of A
+  double* vol = nullptr;     // V_i
+  double* dinv = nullptr;    // 1 / M_ii
+  double* wv = nullptr;      // 1 / V_i on free rows, 0 on fixed rows
+  float* dinv32 = nullptr;
+  float* wv32 = nullptr;
+  double* work = nullptr;    // 10 n doubles
+  double* part = nullptr;    // [4 * kMaxG] reduction partials
+  double* sc = nullptr;      // [S_COUNT] solver scalars
+  unsigned* cnt = nullptr;   // last-workgroup counter
+  int G = 1;
+};
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// assembly
+// ----------------------------------------------------------------------------
+struct AsmArgs {
+  int64_t N, T, B;
+  int32_t kind;
+  const int32_t* row_ptr;
+  const int32_t* col;
+  double* val;
+  double* b;
+  const int32_t* node_tri_start;
+  const int32_t* node_tri;
+  const int32_t* tri;       // [3][T] sorted
+  const double* props;      // [24][T] sorted
+  const double* cv;
+  const double* x;
+  const double* y;
+  const uint8_t* is_vertex;
+  const double* d_edge;     // [3][T] sorted
+  const double* d_bedge;    // [2B] interleaved
+  const int32_t* bedges;    // [2][B]
+  const int32_t* badj;      // [B] sorted triangle
+  const int32_t* nb_start;  // [N+1] boundary-edge endpoints per node
+  const int32_t* nb;        // (edge << 1) | endpoint
+  const int8_t* kindn;      // 0 none, 1 Dirichlet, 2 Dudt
+  const int8_t* neu;        // [B] 1 for a Neumann edge
+  const double* f_node;     // may be null
+  const double* r_node;     // may be null
+  const double* dir_val;    // may be null
+  const double* dudt_val;   // may be null
+  const double* neu_a;      // [2B] may be null
+  uint8_t* fixed;
+};
+
+__device__ __forceinline__ void add_at(const AsmArgs& a, int k0, int k1, int c, double v) {
+  for (int k = k0; k < k1; ++k)
+    if (a.col[k] == c) {
+      a.val[k] = a.val[k] + v;
+      return;
+    }
+}
+
+__device__ __forceinline__ void set_at(const AsmArgs& a, int k0, int k1, int c, double v) {
+  for (int k = k0; k < k1; ++k)
+    if (a.col[k] == c) {
+      a.val[k] = v;
+      return;
+    }
+}
+
+// One thread per row r: the row's share of triangle_contributions!
+// (abstract_templates.jl:73-99), of the boundary-edge contributions (:149-267, the
+// j row divided by V_i as at :262; Neumann terms into b, :175-191), of
+// linear_source_contributions! (linear_reaction_diffusion_equations.jl:115-125),
+// then b and the identity rows: create_rhs_b / apply_steady_dirichlet_conditions!
+// (:278-309), create_met_b! (mean_exit_time.jl:84-94), apply_dudt_conditions!
+// (:127-135) and fix_missing_vertices! (:317-325).  Rows with a Dirichlet or Dudt
+// condition receive no flux terms.
+__global__ __launch_bounds__(256) void assemble_kernel(AsmArgs a) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.N) return;
+  const int k0 = a.row_ptr[r], k1 = a.row_ptr[r + 1];
+  for (int k = k0; k < k1; ++k) a.val[k] = 0.0;
+  const int kd = a.kindn[r];
+  const bool hascond = kd != 0;
+  const bool vertex = a.is_vertex[r] != 0;
+  double bneu = 0.0;
+  if (vertex && !hascond) {
+    const double Vr = a.cv[r];
+    for (int q = a.node_tri_start[r]; q < a.node_tri_start[r + 1]; ++q) {
+      const int w = a.node_tri[q];
+      const int64_t s = w >> 2;
+      const int v = w & 3;
+      const int ijk[3] = {a.tri[s], a.tri[a.T + s], a.tri[2 * a.T + s]};
+      double sc[6];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) sc[f] = a.props[f * a.T + s];
+      // CV edge e joins (ijk[e], ijk[e+1]): r is its first endpoint when e == v
+      // and its second endpoint when e == v + 2 (mod 3).
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        const bool first = e == v, second = e == (v + 2) % 3;
+        if (!first && !second) continue;
+        const double nx = a.props[(P_NX + 2 * e) * a.T + s], ny = a.props[(P_NX + 1 + 2 * e) * a.T + s];
+        const double l = a.props[(P_L + e) * a.T + s];
+        const double Dl = a.d_edge[e * a.T + s] * l;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double av = Dl * (sc[c] * nx + sc[3 + c] * ny);
+          add_at(a, k0, k1, ijk[c], first ? av / Vr : -(av / Vr));
+        }
+      }
+    }
+    if (a.kind != 1) {
+      for (int q = a.nb_start[r]; q < a.nb_start[r + 1]; ++q) {
+        const int w = a.nb[q];
+        const int e = w >> 1, ep = w & 1;
+        const int i = a.bedges[e], j = a.bedges[a.B + e];
+        const double px = a.x[i], py = a.y[i], qx = a.x[j], qy = a.y[j];
+        const double lij = sqrt((qx - px) * (qx - px) + (qy - py) * (qy - py));
+        const double nx = (qy - py) / lij, ny = -(qx - px) / lij;
+        const double mijx = (px + qx) / 2, mijy = (py + qy) / 2;
+        const double ell = sqrt((mijx - px) * (mijx - px) + (mijy - py) * (mijy - py));
+        const double De = a.d_bedge[2 * e + ep];
+        if (!a.neu[e]) {
+          const int64_t s = a.badj[e];
+          const int ijk[3] = {a.tri[s], a.tri[a.T + s], a.tri[2 * a.T + s]};
+          const double Vi = a.cv[i];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double s1 = a.props[c * a.T + s], s2 = a.props[(3 + c) * a.T + s];
+            add_at(a, k0, k1, ijk[c], De * ell * (s1 * nx + s2 * ny) / Vi);
+          }
+        } else if (a.neu_a) {
+          bneu = bneu + De * a.neu_a[2 * e + ep] * ell / Vr;
+        }
+      }
+    }
+    if (a.r_node) add_at(a, k0, k1, (int)r, a.r_node[r]);
+  }
+  double bv = 0.0;
+  bool fix = false;
+  if (!vertex) {
+    fix = true;
+  } else if (a.kind == 0) {
+    if (kd == 1) {
+      bv = a.dir_val ? a.dir_val[r] : 0.0;
+      fix = true;
+    } else {
+      bv = (a.f_node ? a.f_node[r] : 0.0) + bneu;
+    }
+  } else if (a.kind == 1) {
+    if (kd == 1) fix = true;
+    else bv = -1.0;
+  } else {
+    if (kd == 2) bv = a.dudt_val ? a.dudt_val[r] : 0.0;
+    else if (kd == 0) bv = bneu;
+  }
+  if (fix) set_at(a, k0, k1, (int)r, 1.0);
+  a.b[r] = bv;
+  a.fixed[r] = fix;
+}
+
+// M = -V A on free rows and columns, identity on fixed rows; Jacobi inverse of M
+// and the weights that turn a residual of M back into one of A.
+__global__ __launch_bounds__(256) void scaled_kernel(int64_t N, const int32_t* __restrict__ row_ptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const double* __restrict__ valA,
+                                                     const uint8_t* __restrict__ fixed,
+                                                     const double* __restrict__ vol,
+                                                     double* __restrict__ valM, float* __restrict__ valM32,
+                                                     double* __restrict__ dinv, float* __restrict__ dinv32,
+                                                     double* __restrict__ wv, float* __restrict__ wv32) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  double dM = 1.0;
+  const bool fr = fixed[r] != 0;
+  for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+    const int c = col[k];
+    double v;
+    if (fr) v = c == r ? 1.0 : 0.0;
+    else v = fixed[c] ? 0.0 : -vol[r] * valA[k];
+    valM[k] = v;
+    valM32[k] = (float)v;
+    if (c == r) dM = v;
+  }
+  const double di = dM != 0.0 ? 1.0 / dM : 1.0;
+  dinv[r] = di;
+  dinv32[r] = (float)di;
+  const double w = fr || vol[r] == 0.0 ? 0.0 : 1.0 / vol[r];
+  wv[r] = w;
+  wv32[r] = (float)w;
+}
+
+// ----------------------------------------------------------------------------
+// reductions that finish in the last workgroup
+// ----------------------------------------------------------------------------
+// Sums K per-thread values over the grid.  Returns true in the workgroup that
+// arrives last; there every thread gets the totals, added in an order that does
+// not depend on which workgroup was last.
+template <int K>
+__device__ __forceinline__ bool grid_sum(double (&v)[K], double* part, unsigned* cnt) {
+  __shared__ double sh[K][kRT];
+  __shared__ int last;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < K; ++k) sh[k][tid] = v[k];
+  __syncthreads();
+  for (int s = kRT / 2; s > 0; s >>= 1) {
+    if (tid < s)
+#pragma unroll
+      for (int k = 0; k < K; ++k) sh[k][tid] += sh[k][tid + s];
+    __syncthreads();
+  }
+  const int G = gridDim.x;
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      __hip_atomic_store(&part[k * G + blockIdx.x], sh[k][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned prev = atomicAdd(cnt, 1u);
+    last = prev == (unsigned)G - 1;
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int i = tid; i < G; i += kRT)
+      s += __hip_atomic_load(&part[k * G + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh[k][tid] = s;
+  }
+  __syncthreads();
+  for (int s = kRT / 2; s > 0; s >>= 1) {
+    if (tid < s)
+#pragma unroll
+      for (int k = 0; k < K; ++k) sh[k][tid] += sh[k][tid + s];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = sh[k][0];
+  if (tid == 0) __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// Sum over the kLPR lanes of a row group.
+template <class V>
+__device__ __forceinline__ V group_sum(V a) {
+  a += __shfl_xor(a, 4, kLPR);
+  a += __shfl_xor(a, 2, kLPR);
+  a += __shfl_xor(a, 1, kLPR);
+  return a;
+}
+
+// sum_k val[k] * g(col[k]) over row i, kLPR lanes per row; the total is valid in
+// every lane of the group.
+template <class V, class Gather>
+__device__ __forceinline__ V row_product(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                         const V* __restrict__ val, int64_t i, int lane, Gather g) {
+  V acc = 0;
+  const int k1 = rp[i + 1];
+  for (int k = rp[i] + lane; k < k1; k += kLPR) acc += val[k] * g(col[k]);
+  return group_sum(acc);
+}
+
+// ----------------------------------------------------------------------------
+// CG on M (precision V for the matrix, vectors and arithmetic)
+// ----------------------------------------------------------------------------
+// start from d = 0 with residual r: z = D^-1 r, p = 0; scalars rz, |r|_A^2, tolerance
+template <class V>
+__global__ __launch_bounds__(kRT) void cg_init(int64_t n, const V* __restrict__ r, const V* __restrict__ dinv,
+                                               const V* __restrict__ wv, V* __restrict__ d, V* __restrict__ z,
+                                               V* __restrict__ p, double irt2, double tabs2, double* sc,
+                                               double* part, unsigned* cnt) {
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+    const V ri = r[i];
+    const V zi = dinv[i] * ri;
+    z[i] = zi;
+    d[i] = 0;
+    p[i] = 0;
+    acc[0] += (double)ri * (double)zi;
+    const double w = (double)ri * (double)wv[i];
+    acc[1] += w * w;
+  }
+  if (grid_sum<2>(acc, part, cnt) && threadIdx.x == 0) {
+    const double tol2 = fmax(irt2 * acc[1], tabs2);
+    sc[S_RZ] = acc[0];
+    sc[S_BETA] = 0.0;
+    sc[S_WR] = acc[1];
+    sc[S_WR0] = acc[1];
+    sc[S_TOL2] = tol2;
+    sc[S_ITER] = 0.0;
+    sc[S_FLAG] = 0.0;
+    sc[S_DONE] = acc[1] <= tol2 ? 1.0 : 0.0;
+  }
+}
+
+// p_new = z + beta p_old; q = M p_new (p_new of the neighbours formed on the fly
+// from z and p_old, so no separate direction update pass); alpha = rz / (p.q)
+template <class V>
+__global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ col, const V* __restrict__ val,
+                                                   const V* __restrict__ z, const V* __restrict__ pold,
+                                                   V* __restrict__ pnew, V* __restrict__ q, double* sc,
+                                                   double* part, unsigned* cnt) {
+  if (sc[S_DONE] != 0.0) return;
+  const V beta = (V)sc[S_BETA];
+  const int lane = threadIdx.x % kLPR;
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLPR; i < n;
+       i += (int64_t)gridDim.x * kRowsPerBlock) {
+    const V qi = row_product(rp, col, val, i, lane, [&](int c) { return z[c] + beta * pold[c]; });
+    if (lane == 0) {
+      const V pi = z[i] + beta * pold[i];
+      pnew[i] = pi;
+      q[i] = qi;
+      acc[0] += (double)pi * (double)qi;
+    }
+  }
+  if (grid_sum<1>(acc, part, cnt) && threadIdx.x == 0) {
+    const double pq = acc[0];
+    if (pq > 0.0) {
+      sc[S_ALPHA] = sc[S_RZ] / pq;
+    } else {  // M is not positive definite on this direction: stop
+      sc[S_ALPHA] = 0.0;
+      sc[S_FLAG] = 2.0;
+      sc[S_DONE] = 1.0;
+    }
+  }
+}
+
+// d += alpha p; r -= alpha q; z = D^-1 r; beta = rz_new / rz; stop test on |r|_A
+template <class V>
+__global__ __launch_bounds__(kRT) void cg_update(int64_t n, V* __restrict__ d, V* __restrict__ r,
+                                                 const V* __restrict__ p, const V* __restrict__ q,
+                                                 const V* __restrict__ dinv, const V* __restrict__ wv,
+                                                 V* __restrict__ z, double* sc, double* part, unsigned* cnt) {
+  if (sc[S_DONE] != 0.0) return;
+  const V alpha = (V)sc[S_ALPHA];
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+    d[i] = d[i] + alpha * p[i];
+    const V ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const V zi = dinv[i] * ri;
+    z[i] = zi;
+    acc[0] += (double)ri * (double)zi;
+    const double w = (double)ri * (double)wv[i];
+    acc[1] += w * w;
+  }
+  if (grid_sum<2>(acc, part, cnt) && threadIdx.x == 0) {
+    const double rz = acc[0];
+    sc[S_BETA] = sc[S_RZ] != 0.0 ? rz / sc[S_RZ] : 0.0;
+    sc[S_RZ] = rz;
+    sc[S_WR] = acc[1];
+    sc[S_ITER] = sc[S_ITER] + 1.0;
+    if (!(acc[1] > sc[S_TOL2])) sc[S_DONE] = 1.0;  // also stops on NaN
+  }
+}
+
+// ----------------------------------------------------------------------------
+// BiCGStab on M, FP64, Jacobi right preconditioning
+// ----------------------------------------------------------------------------
+// d = 0, rhat = r, p = v = 0; rho = rhat.r, alpha = omega = 1
+__global__ __launch_bounds__(kRT) void bi_init(int64_t n, const double* __restrict__ r,
+                                               const double* __restrict__ wv, double* __restrict__ d,
+                                               double* __restrict__ rhat, double* __restrict__ p,
+                                               double* __restrict__ v, double tabs2, double* sc, double* part,
+                                               unsigned* cnt) {
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+    const double ri = r[i];
+    rhat[i] = ri;
+    d[i] = 0.0;
+    p[i] = 0.0;
+    v[i] = 0.0;
+    acc[0] += ri * ri;
+    const double w = ri * wv[i];
+    acc[1] += w * w;
+  }
+  if (grid_sum<2>(acc, part, cnt) && threadIdx.x == 0) {
+    sc[S_RHO] = acc[0];
+    sc[S_BETA] = 0.0;
+    sc[S_ALPHA] = 1.0;
+    sc[S_OMEGA] = 1.0;
+    sc[S_WR] = acc[1];
+    sc[S_WR0] = acc[1];
+    sc[S_TOL2] = tabs2;
+    sc[S_ITER] = 0.0;
+    sc[S_FLAG] = 0.0;
+    sc[S_DONE] = acc[1] <= tabs2 ? 1.0 : 0.0;
+  }
+}
+
+// p = r + beta (p - omega v); phat = D^-1 p
+__global__ __launch_bounds__(kRT) void bi_dir(int64_t n, const double* __restrict__ r, double* __restrict__ p,
+                                              const double* __restrict__ v, const double* __restrict__ dinv,
+                                              double* __restrict__ phat, const double* sc) {
+  if (sc[S_DONE] != 0.0) return;
+  const double beta = sc[S_BETA], omega = sc[S_OMEGA];
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+    const double pi = r[i] + beta * (p[i] - omega * v[i]);
+    p[i] = pi;
+    phat[i] = dinv[i] * pi;
+  }
+}
+
+// y = M x; MODE 0: alpha = rho / (rhat.y); MODE 1: omega = (y.s) / (y.y)
+template <int MODE>
+__global__ __launch_bounds__(kRT) void bi_spmv(int64_t n, const int32_t* __restrict__ rp,
+                                               const int32_t* __restrict__ col, const double* __restrict__ val,
+                                               const double* __restrict__ x, double* __restrict__ y,
+                                               const double* __restrict__ a, double* sc, double* part,
+                                               unsigned* cnt) {
+  if (sc[S_DONE] != 0.0) return;
+  const int lane = threadIdx.x % kLPR;
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLPR; i < n;
+       i += (int64_t)gridDim.x * kRowsPerBlock) {
+    const double yi = row_product(rp, col, val, i, lane, [&](int c) { return x[c]; });
+    if (lane == 0) {
+      y[i] = yi;
+      acc[0] += a[i] * yi;
+      if (MODE == 1) acc[1] += yi * yi;
+    }
+  }
+  if (grid_sum<2>(acc, part, cnt) && threadIdx.x == 0) {
+    if (MODE == 0) {
+      if (acc[0] != 0.0 && acc[0] == acc[0]) {
+        sc[S_ALPHA] = sc[S_RHO] / acc[0];
+      } else {
+        sc[S_FLAG] = 2.0;
+        sc[S_DONE] = 1.0;
+      }
+    } else {
+      sc[S_OMEGA] = acc[1] != 0.0 ? acc[0] / acc[1] : 0.0;
+    }
+  }
+}
+
+// s = r - alpha v (into r); shat = D^-1 s
+__global__ __launch_bounds__(kRT) void bi_s(int64_t n, double* __restrict__ r, const double* __restrict__ v,
+                                            const double* __restrict__ dinv, double* __restrict__ shat,
+                                            const double* sc) {
+  if (sc[S_DONE] != 0.0) return;
+  const double alpha = sc[S_ALPHA];
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+    const double si = r[i] - alpha * v[i];
+    r[i] = si;
+    shat[i] = dinv[i] * si;
+  }
+}
+
+// d += alpha phat + omega shat; r = s - omega t; rho_new = rhat.r, beta, stop test
+__global__ __launch_bounds__(kRT) void bi_update(int64_t n, double* __restrict__ d, double* __restrict__ r,
+                                                 const double* __restrict__ phat,
+                                                 const double* __restrict__ shat, const double* __restrict__ t,
+                                                 const double* __restrict__ rhat, const double* __restrict__ wv,
+                                                 double* sc, double* part, unsigned* cnt) {
+  if (sc[S_DONE] != 0.0) return;
+  const double alpha = sc[S_ALPHA], omega = sc[S_OMEGA];
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+    d[i] = d[i] + alpha * phat[i] + omega * shat[i];
+    const double ri = r[i] - omega * t[i];
+    r[i] = ri;
+    acc[0] += rhat[i] * ri;
+    const double w = ri * wv[i];
+    acc[1] += w * w;
+  }
+  if (grid_sum<2>(acc, part, cnt) && threadIdx.x == 0) {
+    const double rho = acc[0];
+    sc[S_BETA] = (sc[S_RHO] != 0.0 && omega != 0.0) ? (rho / sc[S_RHO]) * (alpha / omega) : 0.0;
+    sc[S_RHO] = rho;
+    sc[S_WR] = acc[1];
+    sc[S_ITER] = sc[S_ITER] + 1.0;
+    if (!(acc[1] > sc[S_TOL2])) {
+      sc[S_DONE] = 1.0;
+    } else if (omega == 0.0 || rho == 0.0) {
+      sc[S_FLAG] = 2.0;
+      sc[S_DONE] = 1.0;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// residuals and small vector kernels
+// ----------------------------------------------------------------------------
+// ra = b - A x (FP64); r = -V ra on free rows, 0 on fixed rows (precision V);
+// sc[S_RES2] = |ra|^2
+template <class V>
+__global__ __launch_bounds__(kRT) void true_residual(int64_t n, const int32_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ col,
+                                                     const double* __restrict__ val, const double* __restrict__ b,
+                                                     const double* __restrict__ x, const uint8_t* __restrict__ fixed,
+                                                     const double* __restrict__ vol, V* __restrict__ r, double* sc,
+                                                     double* part, unsigned* cnt) {
+  const int lane = threadIdx.x % kLPR;
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLPR; i < n;
+       i += (int64_t)gridDim.x * kRowsPerBlock) {
+    const double ax = row_product(rp, col, val, i, lane, [&](int c) { return x[c]; });
+    if (lane == 0) {
+      const double ra = b[i] - ax;
+      if (r) r[i] = fixed[i] ? (V)0 : (V)(-vol[i] * ra);
+      acc[0] += ra * ra;
+    }
+  }
+  if (grid_sum<1>(acc, part, cnt) && threadIdx.x == 0) sc[S_RES2] = acc[0];
+}
+
+__global__ __launch_bounds__(kRT) void norm2_kernel(int64_t n, const double* __restrict__ x, double* sc,
+                                                    int slot, double* part, unsigned* cnt) {
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT)
+    acc[0] += x[i] * x[i];
+  if (grid_sum<1>(acc, part, cnt) && threadIdx.x == 0) sc[slot] = acc[0];
+}
+
+// x = 0 (unless keep) with x_i = b_i on fixed rows
+__global__ __launch_bounds__(kRT) void init_x(int64_t n, const uint8_t* __restrict__ fixed,
+                                              const double* __restrict__ b, double* __restrict__ x, int keep) {
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+    if (fixed[i]) x[i] = b[i];
+    else if (!keep) x[i] = 0.0;
+  }
+}
+
+template <class V>
+__global__ __launch_bounds__(kRT) void add_correction(int64_t n, const V* __restrict__ d, double* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT)
+    x[i] = x[i] + (double)d[i];
+}
+
+// y = A x (+ b) in precision V, one row group per row
+template <class V>
+__global__ __launch_bounds__(kRT) void spmv_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ col, const V* __restrict__ val,
+                                                   const double* __restrict__ x, const double* __restrict__ b,
+                                                   double* __restrict__ y) {
+  const int lane = threadIdx.x % kLPR;
+  for (int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLPR; i < n;
+       i += (int64_t)gridDim.x * kRowsPerBlock) {
+    const V yi = row_product(rp, col, val, i, lane, [&](int c) { return (V)x[c]; });
+    if (lane == 0) y[i] = b ? (double)yi + b[i] : (double)yi;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// host helpers
+// ----------------------------------------------------------------------------
+template <class T>
+int dalloc(T** dst, int64_t n) {
+  *dst = nullptr;
+  FVM_HIP(hipMalloc((void**)dst, sizeof(T) * (size_t)(n > 0 ? n : 1)));
+  return FVM_OK;
+}
+
+template <class T>
+int dupload(T** dst, const T* src, int64_t n) {
+  int rc = dalloc(dst, n);
+  if (rc) return rc;
+  if (n > 0) FVM_HIP(hipMemcpy(*dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice));
+  return FVM_OK;
+}
+
+void free_tmpl(fvm_template* t) {
+  if (!t) return;
+  void* p[] = {t->row_ptr, t->col, t->val, t->valM, t->valM32, t->b, t->fixed, t->vol, t->dinv,
+               t->wv, t->dinv32, t->wv32, t->work, t->part, t->sc, t->cnt};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  delete t;
+}
+
+int read_scalars(const fvm_template* t, double* h, hipStream_t s) {
+  FVM_HIP(hipGetLastError());
+  FVM_HIP(hipMemcpyAsync(h, t->sc, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
+  FVM_HIP(hipStreamSynchronize(s));
+  return FVM_OK;
+}
+
+inline dim3 grid(const fvm_template* t) { return dim3((unsigned)t->G); }
+
+template <class V>
+int true_res(fvm_template* t, const double* x, V* r, hipStream_t s) {
+  hipLaunchKernelGGL((true_residual<V>), grid(t), dim3(kRT), 0, s, t->n, t->row_ptr, t->col, t->val, t->b, x,
+                     t->fixed, t->vol, r, t->sc, t->part, t->cnt);
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+// Inner CG on M d = r from d = 0, until |r|_A^2 <= max(irt2 |r0|_A^2, tabs2) or
+// maxit iterations; returns the iterations taken and the breakdown flag.
+template <class V>
+int inner_cg(fvm_template* t, const V* valM, const V* dinv, const V* wv, V* d, V* r, V* z, V* p0, V* p1, V* q,
+             double irt2, double tabs2, int maxit, int check, hipStream_t s, int* iters, int* flag) {
+  const int64_t n = t->n;
+  hipLaunchKernelGGL((cg_init<V>), grid(t), dim3(kRT), 0, s, n, r, dinv, wv, d, z, p0, irt2, tabs2, t->sc,
+                     t->part, t->cnt);
+  double h[S_COUNT];
+  int launched = 0;
+  V* pp[2] = {p0, p1};
+  for (;;) {
+    const int batch = std::min(check, maxit - launched);
+    for (int k = 0; k < batch; ++k) {
+      V* pold = pp[launched & 1];
+      V* pnew = pp[(launched + 1) & 1];
+      hipLaunchKernelGGL((cg_dir_spmv<V>), grid(t), dim3(kRT), 0, s, n, t->row_ptr, t->col, valM, z, pold,
+                         pnew, q, t->sc, t->part, t->cnt);
+      hipLaunchKernelGGL((cg_update<V>), grid(t), dim3(kRT), 0, s, n, d, r, pnew, q, dinv, wv, z, t->sc,
+                         t->part, t->cnt);
+      ++launched;
+    }
+    int rc = read_scalars(t, h, s);
+    if (rc) return rc;
+    if (h[S_DONE] != 0.0 || launched >= maxit) break;
+  }
+  *iters = (int)h[S_ITER];
+  *flag = (int)h[S_FLAG];
+  return FVM_OK;
+}
+
+int inner_bicgstab(fvm_template* t, double* d, double* r, double tabs2, int maxit, int check, hipStream_t s,
+                   int* iters, int* flag) {
+  const int64_t n = t->n;
+  double* w = t->work;
+  double *rhat = w + 3 * n, *p = w + 4 * n, *v = w + 5 * n, *phat = w + 6 * n, *shat = w + 7 * n,
+         *tt = w + 8 * n;
+  const dim3 g = grid(t), b(kRT);
+  hipLaunchKernelGGL(bi_init, g, b, 0, s, n, r, t->wv, d, rhat, p, v, tabs2, t->sc, t->part, t->cnt);
+  double h[S_COUNT];
+  int launched = 0;
+  for (;;) {
+    const int batch = std::min(check, maxit - launched);
+    for (int k = 0; k < batch; ++k) {
+      hipLaunchKernelGGL(bi_dir, g, b, 0, s, n, r, p, v, t->dinv, phat, t->sc);
+      hipLaunchKernelGGL(bi_spmv<0>, g, b, 0, s, n, t->row_ptr, t->col, t->valM, phat, v, rhat, t->sc, t->part,
+                         t->cnt);
+      hipLaunchKernelGGL(bi_s, g, b, 0, s, n, r, v, t->dinv, shat, t->sc);
+      hipLaunchKernelGGL(bi_spmv<1>, g, b, 0, s, n, t->row_ptr, t->col, t->valM, shat, tt, r, t->sc, t->part,
+                         t->cnt);
+      hipLaunchKernelGGL(bi_update, g, b, 0, s, n, d, r, phat, shat, tt, rhat, t->wv, t->sc, t->part, t->cnt);
+      ++launched;
+    }
+    int rc = read_scalars(t, h, s);
+    if (rc) return rc;
+    if (h[S_DONE] != 0.0 || launched >= maxit) break;
+  }
+  *iters = (int)h[S_ITER];
+  *flag = (int)h[S_FLAG];
+  return FVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_template** out) {
+  FVM_CHECK(m && d && out, "fvm_template_create: null argument");
+  *out = nullptr;
+  FVM_CHECK(d->kind >= 0 && d->kind <= 2, "fvm_template_create: kind must be 0, 1 or 2");
+  FVM_CHECK(d->d_edge, "fvm_template_create: d_edge required");
+  FVM_CHECK(m->B == 0 || d->d_bedge, "fvm_template_create: d_bedge required");
+  FVM_CHECK(d->node_kind, "fvm_template_create: node_kind required");
+  FVM_CHECK(m->B == 0 || d->bedge_neumann_fidx, "fvm_template_create: bedge_neumann_fidx required");
+  const int64_t N = m->N, T = m->T, B = m->B;
+  for (int64_t g = 0; g < N; ++g)
+    FVM_CHECK(d->node_kind[g] >= 0 && d->node_kind[g] <= 2, "fvm_template_create: node_kind must be 0, 1 or 2");
+  // jacobian_sparsity's pattern from the input-order triangles
+  std::vector<int32_t> hs(3 * T), tri_in(3 * T);
+  FVM_HIP(hipMemcpy(hs.data(), m->tri, sizeof(int32_t) * 3 * T, hipMemcpyDeviceToHost));
+  for (int64_t s = 0; s < T; ++s)
+    for (int v = 0; v < 3; ++v) tri_in[3 * (int64_t)m->h_perm[s] + v] = hs[v * T + s];
+  int64_t nnz = 0;
+  FVM_CHECK(fvmh_csr_pattern(N, T, tri_in.data(), nullptr, nullptr, &nnz) == 0, "fvm_template_create: pattern");
+  FVM_CHECK(nnz < INT32_MAX, "fvm_template_create: more than 2^31 nonzeros");
+  std::vector<int64_t> rp64(N + 1);
+  std::vector<int32_t> col(nnz);
+  fvmh_csr_pattern(N, T, tri_in.data(), rp64.data(), col.data(), &nnz);
+  std::vector<int32_t> rp(rp64.begin(), rp64.end());
+  // boundary-edge endpoints per node
+  std::vector<int32_t> hbe(2 * B);
+  if (B) FVM_HIP(hipMemcpy(hbe.data(), m->bedges, sizeof(int32_t) * 2 * B, hipMemcpyDeviceToHost));
+  std::vector<int32_t> nbs(N + 1, 0), nb(2 * B);
+  for (int64_t e = 0; e < B; ++e)
+    for (int ep = 0; ep < 2; ++ep) ++nbs[hbe[ep * B + e] + 1];
+  for (int64_t i = 0; i < N; ++i) nbs[i + 1] += nbs[i];
+  {
+    std::vector<int32_t> f(nbs.begin(), nbs.end() - 1);
+    for (int64_t e = 0; e < B; ++e)
+      for (int ep = 0; ep < 2; ++ep) nb[f[hbe[ep * B + e]]++] = (int32_t)((e << 1) | ep);
+  }
+  std::vector<int8_t> neu(B);
+  for (int64_t e = 0; e < B; ++e) neu[e] = d->bedge_neumann_fidx[e] >= 0;
+  std::vector<double> de(3 * T);
+  for (int64_t s = 0; s < T; ++s)
+    for (int e = 0; e < 3; ++e) de[(size_t)e * T + s] = d->d_edge[3 * (int64_t)m->h_perm[s] + e];
+
+  fvm_template* t = new fvm_template();
+  t->m = m;
+  t->kind = d->kind;
+  t->n = N;
+  t->nnz = nnz;
+  t->G = (int)std::max<int64_t>(1, std::min<int64_t>((N + kRT - 1) / kRT, kMaxG));
+  int rc = FVM_OK;
+  int32_t *d_nbs = nullptr, *d_nb = nullptr;
+  int8_t *d_kind = nullptr, *d_neu = nullptr;
+  double *d_de = nullptr, *d_dbe = nullptr, *d_f = nullptr, *d_r = nullptr, *d_dir = nullptr,
+         *d_dudt = nullptr, *d_na = nullptr;
+  auto opt = [&](double** dst, const double* src, int64_t n) -> int {
+    *dst = nullptr;
+    return src ? dupload(dst, src, n) : FVM_OK;
+  };
+  do {
+    if ((rc = dupload(&t->row_ptr, rp.data(), N + 1))) break;
+    if ((rc = dupload(&t->col, col.data(), nnz))) break;
+    if ((rc = dalloc(&t->val, nnz)) || (rc = dalloc(&t->valM, nnz)) || (rc = dalloc(&t->valM32, nnz))) break;
+    if ((rc = dalloc(&t->b, N)) || (rc = dalloc(&t->fixed, N)) || (rc = dalloc(&t->vol, N))) break;
+    if ((rc = dalloc(&t->dinv, N)) || (rc = dalloc(&t->wv, N)) || (rc = dalloc(&t->dinv32, N)) ||
+        (rc = dalloc(&t->wv32, N)))
+      break;
+    if ((rc = dalloc(&t->work, 10 * N)) || (rc = dalloc(&t->part, 4 * kMaxG)) ||
+        (rc = dalloc(&t->sc, S_COUNT)) || (rc = dalloc(&t->cnt, 1)))
+      break;
+    if ((rc = dupload(&d_nbs, nbs.data(), N + 1)) || (rc = dupload(&d_nb, nb.data(), 2 * B))) break;
+    if ((rc = dupload(&d_kind, d->node_kind, N)) || (rc = dupload(&d_neu, neu.data(), B))) break;
+    if ((rc = dupload(&d_de, de.data(), 3 * T)) || (rc = opt(&d_dbe, d->d_bedge, 2 * B))) break;
+    if ((rc = opt(&d_f, d->f_node, N)) || (rc = opt(&d_r, d->r_node, N)) || (rc = opt(&d_dir, d->dir_val, N)) ||
+        (rc = opt(&d_dudt, d->dudt_val, N)) || (rc = opt(&d_na, d->neu_a, 2 * B)))
+      break;
+    hipError_t e = hipMemset(t->cnt, 0, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(t->sc, 0, sizeof(double) * S_COUNT);
+    if (e == hipSuccess) e = hipMemcpy(t->vol, m->cv, sizeof(double) * N, hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "fvm_template_create: initialise");
+      break;
+    }
+    AsmArgs a;
+    a.N = N;
+    a.T = T;
+    a.B = B;
+    a.kind = d->kind;
+    a.row_ptr = t->row_ptr;
+    a.col = t->col;
+    a.val = t->val;
+    a.b = t->b;
+    a.node_tri_start = m->node_tri_start;
+    a.node_tri = m->node_tri;
+    a.tri = m->tri;
+    a.props = m->props;
+    a.cv = m->cv;
+    a.x = m->x;
+    a.y = m->y;
+    a.is_vertex = m->is_vertex;
+    a.d_edge = d_de;
+    a.d_bedge = d_dbe;
+    a.bedges = m->bedges;
+    a.badj = m->badj;
+    a.nb_start = d_nbs;
+    a.nb = d_nb;
+    a.kindn = d_kind;
+    a.neu = d_neu;
+    a.f_node = d_f;
+    a.r_node = d_r;
+    a.dir_val = d_dir;
+    a.dudt_val = d_dudt;
+    a.neu_a = d_na;
+    a.fixed = t->fixed;
+    const dim3 g((unsigned)((N + 255) / 256));
+    hipLaunchKernelGGL(assemble_kernel, g, dim3(256), 0, 0, a);
+    hipLaunchKernelGGL(scaled_kernel, g, dim3(256), 0, 0, N, t->row_ptr, t->col, t->val, t->fixed, t->vol,
+                       t->valM, t->valM32, t->dinv, t->dinv32, t->wv, t->wv32);
+    hipLaunchKernelGGL(norm2_kernel, grid(t), dim3(kRT), 0, 0, N, t->b, t->sc, (int)S_B2, t->part, t->cnt);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) rc = hip_fail(e, "fvm_template_create: assembly");
+  } while (false);
+  void* tmp[] = {d_nbs, d_nb, d_kind, d_neu, d_de, d_dbe, d_f, d_r, d_dir, d_dudt, d_na};
+  for (void* p : tmp)
+    if (p) (void)hipFree(p);
+  if (rc != FVM_OK) {
+    free_tmpl(t);
+    return rc;
+  }
+  *out = t;
+  return FVM_OK;
+}
+
+int fvm_template_destroy(fvm_template* t) {
+  free_tmpl(t);
+  return FVM_OK;
+}
+
+int fvm_template_info(const fvm_template* t, int64_t* n, int64_t* nnz) {
+  FVM_CHECK(t, "fvm_template_info: null template");
+  if (n) *n = t->n;
+  if (nnz) *nnz = t->nnz;
+  return FVM_OK;
+}
+
+int fvm_template_download(const fvm_template* t, int32_t* row_ptr, int32_t* col, double* val, double* b) {
+  FVM_CHECK(t, "fvm_template_download: null template");
+  if (row_ptr) FVM_HIP(hipMemcpy(row_ptr, t->row_ptr, sizeof(int32_t) * (t->n + 1), hipMemcpyDeviceToHost));
+  if (col) FVM_HIP(hipMemcpy(col, t->col, sizeof(int32_t) * t->nnz, hipMemcpyDeviceToHost));
+  if (val) FVM_HIP(hipMemcpy(val, t->val, sizeof(double) * t->nnz, hipMemcpyDeviceToHost));
+  if (b) FVM_HIP(hipMemcpy(b, t->b, sizeof(double) * t->n, hipMemcpyDeviceToHost));
+  return FVM_OK;
+}
+
+int fvm_template_apply(const fvm_template* t, const double* d_u, double* d_du, fvm_stream s) {
+  FVM_CHECK(t && d_u && d_du, "fvm_template_apply: null argument");
+  FVM_CHECK(d_u != d_du, "fvm_template_apply: du must not alias u");
+  hipLaunchKernelGGL((spmv_kernel<double>), grid(t), dim3(kRT), 0, (hipStream_t)s, t->n, t->row_ptr, t->col,
+                     t->val, d_u, t->b, d_du);
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_template_spmv(const fvm_template* t, int32_t which, const double* d_x, double* d_y, fvm_stream s) {
+  FVM_CHECK(t && d_x && d_y, "fvm_template_spmv: null argument");
+  FVM_CHECK(d_x != d_y, "fvm_template_spmv: y must not alias x");
+  FVM_CHECK(which >= 0 && which <= 2, "fvm_template_spmv: which must be 0, 1 or 2");
+  hipStream_t st = (hipStream_t)s;
+  if (which == 0)
+    hipLaunchKernelGGL((spmv_kernel<double>), grid(t), dim3(kRT), 0, st, t->n, t->row_ptr, t->col, t->val, d_x,
+                       nullptr, d_y);
+  else if (which == 1)
+    hipLaunchKernelGGL((spmv_kernel<double>), grid(t), dim3(kRT), 0, st, t->n, t->row_ptr, t->col, t->valM, d_x,
+                       nullptr, d_y);
+  else
+    hipLaunchKernelGGL((spmv_kernel<float>), grid(t), dim3(kRT), 0, st, t->n, t->row_ptr, t->col, t->valM32,
+                       d_x, nullptr, d_y);
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_template_residual(fvm_template* t, const double* d_x, double* rel, fvm_stream s) {
+  FVM_CHECK(t && d_x && rel, "fvm_template_residual: null argument");
+  hipStream_t st = (hipStream_t)s;
+  int rc = true_res<double>(t, d_x, nullptr, st);
+  if (rc) return rc;
+  double h[S_COUNT];
+  if ((rc = read_scalars(t, h, st))) return rc;
+  const double bn = std::sqrt(h[S_B2]);
+  *rel = std::sqrt(h[S_RES2]) / (bn > 0 ? bn : 1.0);
+  return FVM_OK;
+}
+
+int fvm_template_solve(fvm_template* t, const fvm_solve_opts* o, double* d_x, fvm_solve_stats* st,
+                       fvm_stream s) {
+  FVM_CHECK(t && o && d_x && st, "fvm_template_solve: null argument");
+  FVM_CHECK(o->method >= 0 && o->method <= 2, "fvm_template_solve: unknown method");
+  FVM_CHECK(o->rtol > 0 && o->maxiter > 0, "fvm_template_solve: rtol and maxiter must be positive");
+  FVM_CHECK(t->kind != 2, "fvm_template_solve: a diffusion template is an ODE, not a linear system");
+  hipStream_t str = (hipStream_t)s;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t n = t->n;
+  const int check = o->check_every > 0 ? o->check_every : 32;
+  const int max_outer = o->max_outer > 0 ? o->max_outer : 50;
+  const double irt = o->method == 1 ? (o->inner_rtol > 0 ? o->inner_rtol : 1e-4) : 0.0;
+  double* w = t->work;
+  double *d = w, *r = w + n, *z = w + 2 * n, *p0 = w + 3 * n, *p1 = w + 4 * n, *q = w + 5 * n;
+  float *df = (float*)w, *rf = df + n, *zf = df + 2 * n, *p0f = df + 3 * n, *p1f = df + 4 * n, *qf = df + 5 * n;
+  hipLaunchKernelGGL(init_x, grid(t), dim3(kRT), 0, str, n, t->fixed, t->b, d_x, o->use_x0 ? 1 : 0);
+  double h[S_COUNT];
+  int rc = o->method == 1 ? true_res<float>(t, d_x, rf, str) : true_res<double>(t, d_x, r, str);
+  if (rc || (rc = read_scalars(t, h, str))) return rc;
+  const double bn = std::sqrt(h[S_B2]) > 0 ? std::sqrt(h[S_B2]) : 1.0;
+  const double target = o->rtol * bn;
+  double res = std::sqrt(h[S_RES2]);
+  // the recursive residual must reach half the target before the true residual is checked
+  const double tabs2 = 0.25 * target * target;
+  int total = 0, outer = 0, flag = 0;
+  while (res > target && total < o->maxiter && outer < max_outer) {
+    int it = 0;
+    if (o->method == 1) {
+      rc = inner_cg<float>(t, t->valM32, t->dinv32, t->wv32, df, rf, zf, p0f, p1f, qf, irt * irt, tabs2,
+                           o->maxiter - total, check, str, &it, &flag);
+      if (rc) return rc;
+      hipLaunchKernelGGL((add_correction<float>), grid(t), dim3(kRT), 0, str, n, df, d_x);
+      rc = true_res<float>(t, d_x, rf, str);
+    } else {
+      if (o->method == 0)
+        rc = inner_cg<double>(t, t->valM, t->dinv, t->wv, d, r, z, p0, p1, q, 0.0, tabs2, o->maxiter - total,
+                              check, str, &it, &flag);
+      else
+        rc = inner_bicgstab(t, d, r, tabs2, o->maxiter - total, check, str, &it, &flag);
+      if (rc) return rc;
+      hipLaunchKernelGGL((add_correction<double>), grid(t), dim3(kRT), 0, str, n, d, d_x);
+      rc = true_res<double>(t, d_x, r, str);
+    }
+    if (rc || (rc = read_scalars(t, h, str))) return rc;
+    total += it;
+    ++outer;
+    res = std::sqrt(h[S_RES2]);
+    if (!(res == res) || it == 0) break;  // NaN, or no progress possible
+  }
+  st->iterations = total;
+  st->outer = outer;
+  st->flag = flag;
+  st->residual = res / bn;
+  st->converged = res <= target ? 1 : 0;
+  st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return FVM_OK;
+}
+
+}  // extern "C"

@@ -4,7 +4,7 @@ The public names mirror the reference module (src/FiniteVolumeMethod.jl:39-52 an
 the template exports): FVMGeometry, FVMProblem, SteadyFVMProblem,
 BoundaryConditions, InternalConditions, Conditions, Neumann, Dudt, Dirichlet,
 Constrained, solve, fvm_eqs (fvm_eqs!), DiffusionEquation, PoissonsEquation,
-MeanExitTimeProblem.  Every numerical kernel runs on the GPU through
+LaplacesEquation, MeanExitTimeProblem, LinearReactionDiffusionEquation.  Every numerical kernel runs on the GPU through
 libfvmhip.so's C ABI (include/fvm_hip.h).
 """
 from ._lib import DeviceArray, FVMError, synchronize  # noqa: F401
@@ -18,6 +18,8 @@ from .geometry import FVMGeometry, TriangleProperties  # noqa: F401
 from .mesh import Triangulation, jittered_rectangle, triangulate, triangulate_rectangle  # noqa: F401
 from .problem import FVMProblem, SteadyFVMProblem, construct_flux_function  # noqa: F401
 from .solve import SSPRK3, Euler, ODESolution, Tsit5, solve  # noqa: F401
-from .templates import AbstractFVMTemplate  # noqa: F401
+from .templates import (AbstractFVMTemplate, BiCGStab, CG, DiffusionEquation,  # noqa: F401
+                        LaplacesEquation, LinearReactionDiffusionEquation, LinearSolution,
+                        MeanExitTimeProblem, MixedCG, PoissonsEquation)
 
 __version__ = "0.1.0"

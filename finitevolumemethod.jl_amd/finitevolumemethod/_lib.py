@@ -62,7 +62,28 @@ class _Tiles(ctypes.Structure):
             "badj_sorted", "tile_of_node_excl")]
 
 
+class TemplateDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("d_edge", _dp), ("d_bedge", _dp),
+                ("node_kind", _i8p), ("bedge_neumann_fidx", _i32p), ("f_node", _dp),
+                ("r_node", _dp), ("neu_a", _dp), ("dir_val", _dp), ("dudt_val", _dp)]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int32), ("maxiter", ctypes.c_int32),
+                ("check_every", ctypes.c_int32), ("max_outer", ctypes.c_int32),
+                ("use_x0", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("rtol", ctypes.c_double), ("inner_rtol", ctypes.c_double)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("outer", ctypes.c_int32), ("flag", ctypes.c_int32),
+                ("residual", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
 FLUX_D_POLY_U, FLUX_D_NODAL, FLUX_D_EDGE = 0, 1, 2
+TEMPLATE_POISSON, TEMPLATE_MEAN_EXIT_TIME, TEMPLATE_DIFFUSION = 0, 1, 2
+SOLVE_CG, SOLVE_CG_MIXED, SOLVE_BICGSTAB = 0, 1, 2
 NODE_NONE, NODE_DIRICHLET, NODE_DUDT = 0, 1, 2
 
 _lock = threading.Lock()
@@ -134,6 +155,14 @@ HIP_API = {
     "fvm_dirichlet_apply": [_vp, _vp, ctypes.c_double, _vp],
     "fvm_jvp": [_vp, _vp, _vp, _vp, ctypes.c_double, _vp],
     "fvm_vjp": [_vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp],
+    "fvm_template_create": [_vp, ctypes.POINTER(TemplateDesc), ctypes.POINTER(_vp)],
+    "fvm_template_destroy": [_vp],
+    "fvm_template_info": [_vp, _i64p, _i64p],
+    "fvm_template_download": [_vp, _i32p, _i32p, _dp, _dp],
+    "fvm_template_apply": [_vp, _vp, _vp, _vp],
+    "fvm_template_spmv": [_vp, ctypes.c_int32, _vp, _vp, _vp],
+    "fvm_template_residual": [_vp, _vp, _dp, _vp],
+    "fvm_template_solve": [_vp, ctypes.POINTER(SolveOpts), _vp, ctypes.POINTER(SolveStats), _vp],
     "fvm_vec_lincomb": [ctypes.c_int64, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), _vp],
     "fvm_vec_dot": [ctypes.c_int64, _vp, _vp, _dp, _vp],
     "fvm_vec_err_norm": [ctypes.c_int64, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _dp,

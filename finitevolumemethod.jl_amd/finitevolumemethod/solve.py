@@ -98,8 +98,10 @@ def solve(prob, alg=None, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None, max
 
     FVMProblem: integrates on the device; returns ODESolution (t, u host arrays).
     SteadyFVMProblem: see solve_steady.  Templates forward to their own solver."""
-    from .templates import AbstractFVMTemplate
+    from .templates import AbstractFVMTemplate, _SteadyTemplate
 
+    if isinstance(prob, _SteadyTemplate):  # a LinearProblem: only the linear solver's options apply
+        return prob.solve(alg, **kwargs)
     if isinstance(prob, AbstractFVMTemplate):
         return prob.solve(alg, saveat=saveat, abstol=abstol, reltol=reltol, dt=dt, **kwargs)
     if isinstance(prob, SteadyFVMProblem):

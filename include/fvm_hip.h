@@ -31,6 +31,7 @@ typedef void* fvm_stream; /* hipStream_t */
 typedef void* fvm_event;  /* hipEvent_t */
 typedef struct fvm_mesh fvm_mesh;
 typedef struct fvm_ctx fvm_ctx;
+typedef struct fvm_template fvm_template;
 
 /* ---- runtime ------------------------------------------------------------- */
 /* Error text of the calling thread's last failure (copied into buf, truncated). */
@@ -150,6 +151,83 @@ int fvm_jvp(fvm_ctx* c, const double* d_u, const double* d_v, double* d_jv, doub
  * ForwardDiff); it is the adjoint of fvm_rhs. */
 int fvm_vjp(fvm_ctx* c, const double* d_u, const double* d_w, double* d_ubar,
             double* d_logd_bar, double t, fvm_stream s);
+
+/* ---- linear templates (src/specific_problems/) ----------------------------
+ * The operator A and vector b of PoissonsEquation / LaplacesEquation
+ * (poissons_equation.jl:58-88, laplaces_equation.jl:51-78), MeanExitTimeProblem
+ * (mean_exit_time.jl:57-94), DiffusionEquation and LinearReactionDiffusionEquation
+ * (diffusion_equation.jl:69-101, linear_reaction_diffusion_equations.jl:76-125),
+ * assembled on the device in the CSR pattern of jacobian_sparsity (solve.jl:50-77):
+ * the contributions of abstract_templates.jl:73-325, including the division of the
+ * j row of a non-Neumann boundary edge by V_i (:262).  Rows carry every point
+ * (num_points); points that are not vertices get identity rows. */
+enum {
+  FVM_TEMPLATE_POISSON = 0,        /* A u = b, Dirichlet rows A_ii = 1, b_i = value */
+  FVM_TEMPLATE_MEAN_EXIT_TIME = 1, /* triangle terms only, b = -1, Dirichlet rows A_ii = 1 */
+  FVM_TEMPLATE_DIFFUSION = 2       /* du/dt = A u + b (Dudt rows: b_i = value, A row 0) */
+};
+
+typedef struct fvm_template_desc {
+  int32_t kind;                       /* FVM_TEMPLATE_* */
+  const double* d_edge;               /* [3*n_triangles] D(x, y, p) at the three CV-edge
+                                         midpoints of each input triangle (edges (i,j), (j,k), (k,i)) */
+  const double* d_bedge;              /* [2*n_boundary_edges] D at the half-edge points m_i, m_j */
+  const int8_t* node_kind;            /* [n_points] 0 none, 1 Dirichlet, 2 Dudt */
+  const int32_t* bedge_neumann_fidx;  /* [n_boundary_edges] >= 0 for a Neumann edge, else -1 */
+  const double* f_node;               /* [n_points] Poisson source f(x_i, y_i, p), or NULL (0) */
+  const double* r_node;               /* [n_points] linear reaction coefficient, or NULL (0) */
+  const double* neu_a;                /* [2*n_boundary_edges] Neumann value at m_i, m_j, or NULL */
+  const double* dir_val;              /* [n_points] Dirichlet value (Poisson), or NULL (0) */
+  const double* dudt_val;             /* [n_points] Dudt value (diffusion), or NULL (0) */
+} fvm_template_desc;
+
+int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_template** out);
+int fvm_template_destroy(fvm_template* t);
+int fvm_template_info(const fvm_template* t, int64_t* n, int64_t* nnz);
+/* the assembled A (CSR, 0-based, int32) and b; any pointer may be NULL */
+int fvm_template_download(const fvm_template* t, int32_t* row_ptr, int32_t* col, double* val,
+                          double* b);
+/* du = A u + b: the MatrixOperator of the augmented [A b; 0 0] system
+ * (diffusion_equation.jl:82-94) applied to (u, 1). */
+int fvm_template_apply(const fvm_template* t, const double* d_u, double* d_du, fvm_stream s);
+/* y = A x (which 0), y = M x in FP64 (1) or FP32 (2), where M = -diag(V) A on the free
+ * rows and columns and the identity on fixed rows: the operator of the steady solvers. */
+int fvm_template_spmv(const fvm_template* t, int32_t which, const double* d_x, double* d_y,
+                      fvm_stream s);
+/* |b - A x|_2 / |b|_2 (waits for completion) */
+int fvm_template_residual(fvm_template* t, const double* d_x, double* rel, fvm_stream s);
+
+enum {
+  FVM_SOLVE_CG = 0,       /* FP64 Jacobi-PCG on M */
+  FVM_SOLVE_CG_MIXED = 1, /* FP64 iterative refinement around FP32 Jacobi-PCG on M */
+  FVM_SOLVE_BICGSTAB = 2  /* FP64 Jacobi-BiCGStab on M */
+};
+
+typedef struct fvm_solve_opts {
+  int32_t method;       /* FVM_SOLVE_* */
+  int32_t maxiter;      /* total inner iterations */
+  int32_t check_every;  /* iterations between host reads of the device scalars (0: 32) */
+  int32_t max_outer;    /* refinement / restart steps (0: 50) */
+  int32_t use_x0;       /* start from d_x (its fixed rows are reset to b) instead of 0 */
+  int32_t reserved;
+  double rtol;          /* stop when |b - A x|_2 <= rtol |b|_2 */
+  double inner_rtol;    /* FVM_SOLVE_CG_MIXED: relative drop of each FP32 solve (0: 1e-4) */
+} fvm_solve_opts;
+
+typedef struct fvm_solve_stats {
+  int32_t converged;
+  int32_t iterations;   /* inner iterations over all refinement steps */
+  int32_t outer;        /* refinement / restart steps */
+  int32_t flag;         /* 2: breakdown (non-positive curvature or a zero BiCGStab scalar) */
+  double residual;      /* final |b - A x|_2 / |b|_2 */
+  double seconds;       /* wall time of the call, including its host synchronisations */
+} fvm_solve_stats;
+
+/* LinearProblem(A, b) of a steady template solved on the device (replaces the
+ * sparse direct solve behind solve(::AbstractFVMTemplate), abstract_templates.jl:58-60).
+ * d_x [n_points] receives the solution. */
+int fvm_template_solve(fvm_template* t, const fvm_solve_opts* o, double* d_x, fvm_solve_stats* st,
+                       fvm_stream s);
 
 /* ---- device vector operations (used by the integrators and solvers) ------
  * The reference broadcasts over Vector{Float64} inside OrdinaryDiffEq; device-

@@ -343,7 +343,7 @@ static void bcv(const orc_mesh* m, int64_t e, double* nx, double* ny, double* mi
 int64_t orc_template_assemble(const orc_mesh* m, const orc_conditions* c, int kind,
                               orc_dxy_fn Dfn, void* Dctx, const double* f_node,
                               const double* neu_a, const double* dir_val, const double* dudt_val,
-                              int64_t* rows, int64_t* cols, double* vals, double* bvec) {
+                              const double* r_node, int64_t* rows, int64_t* cols, double* vals, double* bvec) {
   coo A = {0, rows, cols, vals};
   const int64_t N = m->n_points;
   if (bvec) for (int64_t i = 0; i < N; ++i) bvec[i] = 0.0;
@@ -423,15 +423,20 @@ int64_t orc_template_assemble(const orc_mesh* m, const orc_conditions* c, int ki
       }
     }
   } else {
+    /* linear_source_contributions! (linear_reaction_diffusion_equations.jl:115-125) */
+    if (r_node)
+      for (int64_t i = 0; i < N; ++i)
+        if (m->is_vertex[i] && !has_cond(c, i)) push(&A, i, i, r_node[i]);
     /* apply_dudt_conditions! (abstract_templates.jl:127-135) */
     if (bvec)
       for (int64_t i = 0; i < N; ++i)
         if (c->node_kind[i] == 2) bvec[i] = dudt_val[i];
   }
-  /* fix_missing_vertices! (abstract_templates.jl:317-325) */
+  /* fix_missing_vertices! (abstract_templates.jl:317-325); A_ii = 1.0 is an
+   * assignment, so a Poisson Dirichlet row that is also a missing vertex keeps 1 */
   for (int64_t i = 0; i < N; ++i)
     if (!m->is_vertex[i]) {
-      push(&A, i, i, 1.0);
+      if (!(kind == 0 && c->node_kind[i] == 1)) push(&A, i, i, 1.0);
       if (bvec) bvec[i] = 0.0;
     }
   return A.n;

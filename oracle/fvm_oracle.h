@@ -128,19 +128,23 @@ void orc_fvm_eqs_serial_cs(const orc_mesh* m, const orc_conditions* c, const orc
  * Writes the COO triplets the reference adds into its dense matrix, in the
  * reference's order, and the vector b.  Duplicates are summed by the caller
  * (sparse(A) does the same).  The template diffusion function is D(x,y,p).
- * kind: 0 Poisson (poissons_equation.jl:159-189), 1 MeanExitTime
- * (mean_exit_time.jl:246-283), 2 DiffusionEquation (diffusion_equation.jl:69-101;
- * A and b of du/dt = A u + b, without the augmented row/column).
+ * kind: 0 Poisson / Laplace (poissons_equation.jl:58-88, laplaces_equation.jl:51-78),
+ * 1 MeanExitTime (mean_exit_time.jl:57-94), 2 DiffusionEquation /
+ * LinearReactionDiffusionEquation (diffusion_equation.jl:69-101,
+ * linear_reaction_diffusion_equations.jl:76-125; A and b of du/dt = A u + b,
+ * without the augmented row/column).
  * f_node: Poisson source values f(x_i, y_i) per node (ignored otherwise).
  * neu_a: Neumann function values at the two half-edge points of each boundary
  * edge (2*n_bedges, ignored for non-Neumann edges).  dir_val / dudt_val: condition
- * values per node for Dirichlet / Dudt nodes.
+ * values per node for Dirichlet / Dudt nodes.  r_node: linear reaction
+ * coefficient added to A_ii of rows without a condition (NULL: none).
+ * Identity rows (A_ii = 1.0 assignments in the reference) are emitted once.
  * Returns the number of triplets written (call with rows == NULL to count). */
 typedef double (*orc_dxy_fn)(double x, double y, void* ctx);
 int64_t orc_template_assemble(const orc_mesh* m, const orc_conditions* c, int kind,
                               orc_dxy_fn D, void* Dctx, const double* f_node,
                               const double* neu_a, const double* dir_val, const double* dudt_val,
-                              int64_t* rows, int64_t* cols, double* vals, double* bvec);
+                              const double* r_node, int64_t* rows, int64_t* cols, double* vals, double* bvec);
 double orc_dxy_exp_sincos(double x, double y, void* ctx); /* *(double*)ctx * exp(0.5 sin 4pi x cos 4pi y) */
 double orc_dxy_const(double x, double y, void* ctx);      /* *(double*)ctx */
 

@@ -87,7 +87,7 @@ def lib():
         L.orc_fvm_eqs_serial_cs.restype = None
         L.orc_template_assemble.argtypes = [ctypes.POINTER(_Mesh), ctypes.POINTER(_Cond),
                                             ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, _dp,
-                                            _dp, _dp, _dp, _i64p, _i64p, _dp, _dp]
+                                            _dp, _dp, _dp, _dp, _i64p, _i64p, _dp, _dp]
         L.orc_template_assemble.restype = ctypes.c_int64
         _LIB = L
     return _LIB
@@ -292,9 +292,11 @@ class OracleProblem:
                                    ctypes.byref(self._phys), t, _p(u, _dp))
         return u
 
-    def template_assemble(self, kind, D, f_node=None, neu_a=None, dir_val=None, dudt_val=None):
-        """COO triplets + b of the template operator (kind 0 Poisson, 1 MeanExitTime,
-        2 Diffusion).  D: a float (constant), the string "exp_sincos:<scale>", or a
+    def template_assemble(self, kind, D, f_node=None, neu_a=None, dir_val=None, dudt_val=None,
+                          r_node=None):
+        """COO triplets + b of the template operator (kind 0 Poisson / Laplace,
+        1 MeanExitTime, 2 Diffusion / linear reaction-diffusion with the diagonal
+        r_node).  D: a float (constant), the string "exp_sincos:<scale>", or a
         Python callable D(x, y)."""
         L = lib()
         keep = []
@@ -314,8 +316,10 @@ class OracleProblem:
         arr = lambda a, n: np.ascontiguousarray(a if a is not None else np.zeros(n), np.float64)
         f_node, neu_a = arr(f_node, N), arr(neu_a, 2 * len(self.bedges))
         dir_val, dudt_val = arr(dir_val, N), arr(dudt_val, N)
+        r_arr = None if r_node is None else arr(r_node, N)
         args = (ctypes.byref(self._mesh), ctypes.byref(self._cond), kind, fn, ctx,
-                _p(f_node, _dp), _p(neu_a, _dp), _p(dir_val, _dp), _p(dudt_val, _dp))
+                _p(f_node, _dp), _p(neu_a, _dp), _p(dir_val, _dp), _p(dudt_val, _dp),
+                None if r_arr is None else _p(r_arr, _dp))
         n = L.orc_template_assemble(*args, None, None, None, None)
         rows = np.empty(n, np.int64)
         cols = np.empty(n, np.int64)

@@ -145,3 +145,24 @@ def dxy_problem(n=30, seed=13):
     fn = ([0.5, 0.0, 0.0, 1.0], [0.0, -0.25, 0.0, 0.0], [0.0] * 4)
     orc = oracle_for(prob, dict(builtin=dict(flux_kind=2, dscale=1.0, spoly=[0.3, 0, -1, 0], fn=fn)))
     return prob, orc, u
+
+
+def disk_triangulation(nr=12, R=1.0):
+    """Rings of 6k points at radius R k / nr around the origin, Delaunay-triangulated."""
+    pts = [(0.0, 0.0)]
+    for k in range(1, nr + 1):
+        r = R * k / nr
+        m = 6 * k
+        th = 2 * np.pi * np.arange(m) / m + 0.1 * k
+        pts += list(zip(r * np.cos(th), r * np.sin(th)))
+    P = np.array(pts)
+    return FVM.triangulate(P[:, 0], P[:, 1])
+
+
+def template_oracle(prob):
+    """The oracle over a template problem's mesh and flattened conditions (its
+    template_assemble needs no flux closure)."""
+    tri = prob.mesh.triangulation
+    kind, fidx, bneu = prob.conditions.flatten(tri)
+    return OracleProblem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                         kind, fidx, bneu, dict(flux=lambda *a: (0.0, 0.0)))

@@ -1,0 +1,259 @@
+"""Linear templates on the GPU (src/specific_problems/*.jl): device assembly
+against the oracle's restatement of abstract_templates.jl:73-325, the device
+solvers against a sparse direct solve of the oracle's matrix and against the
+closed forms the reference's docs test (poissons_equation.jl:154-162,
+laplaces_equation.jl:159-193, the disk mean exit time), and the ODE templates
+against the general FVMProblem path."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import finitevolumemethod as FVM
+from finitevolumemethod import templates as TM
+from problems import disk_triangulation, template_oracle
+
+pytestmark = pytest.mark.gpu
+
+Dxy = lambda x, y, p: np.exp(0.5 * np.sin(4 * np.pi * x) * np.cos(4 * np.pi * y))  # noqa: E731
+
+
+def _oracle_matrix(prob, kind, D, **kw):
+    orc = template_oracle(prob)
+    r, c, v, b = orc.template_assemble(kind, D, **kw)
+    n = prob.problem.n
+    A = sp.csr_matrix((v, (r, c)), shape=(n, n))
+    A.sum_duplicates()
+    absA = sp.csr_matrix((np.abs(v), (r, c)), shape=(n, n))
+    return A, b, np.asarray(absA.sum(axis=1)).ravel()
+
+
+def _assert_same_operator(prob, kind, D, **kw):
+    A_ref, b_ref, rowscale = _oracle_matrix(prob, kind, D, **kw)
+    A = prob.A
+    n = A.shape[0]
+    # the device pattern is jacobian_sparsity's, bit-exact
+    rp, col = FVM.equations.csr_pattern(prob.mesh.triangulation)
+    np.testing.assert_array_equal(A.indptr, rp)
+    np.testing.assert_array_equal(A.indices, col)
+    # every oracle entry lies in the pattern, and the values agree to rounding
+    D_ = (A - A_ref).tocoo()
+    scale = np.maximum(rowscale, 1e-300)
+    assert np.all(np.abs(D_.data) <= 1e-13 * scale[D_.row]), np.max(np.abs(D_.data) / scale[D_.row])
+    pattern = sp.csr_matrix((np.ones(A.nnz), A.indices, A.indptr), shape=A.shape)
+    ref = A_ref.tocoo()
+    assert np.all(np.asarray(pattern[ref.row, ref.col]).ravel() == 1.0)
+    bs = np.max(np.abs(b_ref)) if n else 1.0
+    np.testing.assert_allclose(prob.b, b_ref, rtol=1e-13, atol=1e-15 * bs)
+    return A_ref, b_ref
+
+
+def _rect4(types, fns, nx=23, ny=19, jitter=0.2, seed=8, params=None):
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, nx, ny, jitter=jitter, seed=seed,
+                                 single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    return tri, mesh, FVM.BoundaryConditions(mesh, fns, types, parameters=params)
+
+
+def test_poisson_assembly_matches_oracle(gpu):
+    types = (FVM.Neumann, FVM.Constrained, FVM.Dirichlet, FVM.Neumann)
+    fns = (lambda x, y, t, u, p: p + x, lambda x, y, t, u, p: 0.0,
+           lambda x, y, t, u, p: 1 + x * y, lambda x, y, t, u, p: -0.5 * y)
+    tri, mesh, BCs = _rect4(types, fns, params=(0.3, None, None, None))
+    nx = 23
+    ICs = FVM.InternalConditions(lambda x, y, t, u, p: x - y, dirichlet_nodes={5 + 7 * nx: 0, 9 + 11 * nx: 0})
+    src = lambda x, y, p: x ** 2 - p * y  # noqa: E731
+    prob = FVM.PoissonsEquation(mesh, BCs, ICs, diffusion_function=Dxy, source_function=src,
+                                source_parameters=2.0)
+    kind, fidx, bn = prob.conditions.flatten(tri)
+    mx, my = FVM.problem._half_edge_points(tri)
+    neu_a = np.zeros(2 * len(tri.boundary_edges))
+    for e in np.nonzero(bn >= 0)[0]:
+        f = prob.conditions.functions[bn[e]]
+        neu_a[2 * e:2 * e + 2] = [f(mx[2 * e + k], my[2 * e + k], None, None) for k in (0, 1)]
+    dir_val = np.array([prob.conditions.functions[fidx[i]](tri.x[i], tri.y[i], None, None)
+                        if kind[i] == 1 else 0.0 for i in range(tri.num_points())])
+    _assert_same_operator(prob, 0, lambda x, y: float(Dxy(x, y, None)),
+                          f_node=src(tri.x, tri.y, 2.0), neu_a=neu_a, dir_val=dir_val)
+
+
+def test_mean_exit_time_assembly_matches_oracle(gpu):
+    types = (FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet, FVM.Neumann)
+    fns = tuple(lambda x, y, t, u, p: 0.0 for _ in range(4))
+    tri, mesh, BCs = _rect4(types, fns, nx=21, ny=24, seed=2)
+    D = lambda x, y, p: 1 + x + y ** 2  # noqa: E731
+    prob = FVM.MeanExitTimeProblem(mesh, BCs, diffusion_function=D)
+    _assert_same_operator(prob, 1, lambda x, y: 1 + x + y ** 2)
+
+
+@pytest.mark.parametrize("reaction", [False, True])
+def test_diffusion_assembly_matches_oracle(gpu, reaction):
+    types = (FVM.Neumann, FVM.Dudt, FVM.Dirichlet, FVM.Constrained)
+    fns = (lambda x, y, t, u, p: 0.3 + x, lambda x, y, t, u, p: 1.5 - y,
+           lambda x, y, t, u, p: 0.0, lambda x, y, t, u, p: 0.0)
+    tri, mesh, BCs = _rect4(types, fns, seed=11)
+    u0 = np.random.default_rng(0).random(tri.num_points())
+    kw = dict(diffusion_function=Dxy, initial_condition=u0, final_time=0.1)
+    if reaction:
+        prob = FVM.LinearReactionDiffusionEquation(mesh, BCs, source_function=lambda x, y, p: 0.3 - x, **kw)
+    else:
+        prob = FVM.DiffusionEquation(mesh, BCs, **kw)
+    kind, fidx, bn = prob.conditions.flatten(tri)
+    mx, my = FVM.problem._half_edge_points(tri)
+    neu_a = np.zeros(2 * len(tri.boundary_edges))
+    for e in np.nonzero(bn >= 0)[0]:
+        neu_a[2 * e:2 * e + 2] = 0.3 + mx[2 * e:2 * e + 2]
+    dudt = np.where(kind == 2, 1.5 - tri.y, 0.0)
+    A_ref, b_ref = _assert_same_operator(prob, 2, lambda x, y: float(Dxy(x, y, None)), neu_a=neu_a,
+                                         dudt_val=dudt, r_node=(0.3 - tri.x) if reaction else None)
+    # du = A u + b on the device
+    u = FVM.DeviceArray.from_host(u0)
+    du = FVM.DeviceArray(len(u0))
+    prob.problem.apply(u, du)
+    ref = A_ref @ u0 + b_ref
+    scale = abs(A_ref) @ np.abs(u0) + np.abs(b_ref)
+    assert np.all(np.abs(du.to_host() - ref) <= 1e-13 * scale)
+
+
+def test_points_that_are_not_vertices_get_identity_rows(gpu):
+    base = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 15, 13, jitter=0.25, seed=4)
+    x = np.concatenate([base.x, [0.5, 0.25]])
+    y = np.concatenate([base.y, [0.5, 0.75]])
+    tri = FVM.Triangulation(x, y, base.triangles, base.boundary_edges, base.boundary_segments)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    n = len(x)
+    # an internal Dirichlet condition on a point that is not a vertex keeps A_ii = 1
+    ICs = FVM.InternalConditions(lambda x, y, t, u, p: 2.0, dirichlet_nodes={n - 1: 0})
+    prob = FVM.PoissonsEquation(mesh, BCs, ICs, source_function=lambda x, y, p: 1.0)
+    kind, fidx, _ = prob.conditions.flatten(tri)
+    f = np.where(kind == 1, 0.0, 1.0)
+    f[-2:] = 1.0
+    dv = np.where(kind == 1, 0.0, 0.0)
+    dv[-1] = 2.0
+    _assert_same_operator(prob, 0, 1.0, f_node=f, dir_val=dv)
+    A = prob.A
+    assert A[n - 1, n - 1] == 1.0 and A[n - 2, n - 2] == 1.0
+    assert prob.b[-1] == 0.0 and prob.b[-2] == 0.0
+    sol = FVM.solve(prob, TM.CG())
+    assert sol.retcode == "Success" and sol.u[-1] == 0.0 and sol.u[-2] == 0.0
+
+
+ALGS = [TM.CG(), TM.MixedCG(), TM.BiCGStab()]
+
+
+@pytest.mark.parametrize("alg", ALGS, ids=["cg", "mixed", "bicgstab"])
+def test_poisson_closed_form(gpu, alg):
+    """poissons_equation.jl:154-162: laplace(u) = -sin(pi x) sin(pi y) on [0,1]^2
+    (100 x 100), u = 0 on the boundary: u = sin(pi x) sin(pi y) / (2 pi^2), rtol 1e-4."""
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 100, 100, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    prob = FVM.PoissonsEquation(mesh, BCs, source_function=lambda x, y, p: -np.sin(np.pi * x) * np.sin(np.pi * y))
+    sol = FVM.solve(prob, alg)
+    assert sol.retcode == "Success", sol.stats
+    exact = np.sin(np.pi * tri.x) * np.sin(np.pi * tri.y) / (2 * np.pi ** 2)
+    assert np.linalg.norm(sol.u - exact) <= 1e-4 * np.linalg.norm(exact)
+    direct = spla.spsolve(prob.A.tocsc(), prob.b)
+    assert np.linalg.norm(sol.u - direct) <= 1e-6 * np.linalg.norm(direct)
+
+
+@pytest.mark.parametrize("alg", ALGS, ids=["cg", "mixed", "bicgstab"])
+def test_laplace_closed_form(gpu, alg):
+    """laplaces_equation.jl:159-193: D = (x+1)(y+2) on [0,5]^2, Neumann bottom / top,
+    u = 5 right, u = 0 left: u = 5 log(1 + x) / log 6, rtol 1e-3."""
+    tri = FVM.triangulate_rectangle(0.0, 5.0, 0.0, 5.0, 100, 100, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    zero = lambda x, y, t, u, p: 0.0  # noqa: E731
+    five = lambda x, y, t, u, p: 5.0  # noqa: E731
+    BCs = FVM.BoundaryConditions(mesh, (zero, five, zero, zero),
+                                 (FVM.Neumann, FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet))
+    prob = FVM.LaplacesEquation(mesh, BCs, diffusion_function=lambda x, y, p: (x + 1) * (y + 2))
+    sol = FVM.solve(prob, alg)
+    assert sol.retcode == "Success", sol.stats
+    exact = 5 * np.log(1 + tri.x) / np.log(6)
+    assert np.linalg.norm(sol.u - exact) <= 1e-3 * np.linalg.norm(exact)
+
+
+def test_mean_exit_time_disk_closed_form(gpu):
+    """T(r) = (R^2 - r^2) / (4 D) on a disk with an absorbing boundary."""
+    tri = disk_triangulation(nr=24, R=2.0)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    prob = FVM.MeanExitTimeProblem(mesh, BCs, diffusion_function=lambda x, y, p: 0.3)
+    _assert_same_operator(prob, 1, 0.3)
+    sol = FVM.solve(prob, TM.CG())
+    exact = (4.0 - (tri.x ** 2 + tri.y ** 2)) / (4 * 0.3)
+    assert np.linalg.norm(sol.u - exact) <= 5e-3 * np.linalg.norm(exact)
+
+
+@pytest.mark.parametrize("alg", ALGS, ids=["cg", "mixed", "bicgstab"])
+def test_solvers_reach_the_true_residual(gpu, alg):
+    """The variable-D Poisson system at 200 x 200 (jittered): every method stops at
+    |b - A x| <= 1e-10 |b|, checked on the host with the oracle's own matrix."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 200, 200, jitter=0.25, seed=20261202)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    f = np.random.default_rng(7).uniform(-1, 1, tri.num_points())
+    prob = FVM.PoissonsEquation(mesh, BCs, diffusion_function=Dxy,
+                                source_function=lambda x, y, p: p, source_parameters=f)
+    kind, _, _ = prob.conditions.flatten(tri)
+    A_ref, b_ref, _ = _oracle_matrix(prob, 0, "exp_sincos:1.0", f_node=np.where(kind == 1, 0.0, f))
+    sol = FVM.solve(prob, alg, rtol=1e-10)
+    assert sol.retcode == "Success", sol.stats
+    rel = np.linalg.norm(b_ref - A_ref @ sol.u) / np.linalg.norm(b_ref)
+    assert rel <= 1.5e-10, (rel, sol.stats)
+    assert abs(sol.resid - rel) <= 0.05 * rel + 1e-14
+    direct = spla.spsolve(A_ref.tocsc(), b_ref)
+    assert np.linalg.norm(sol.u - direct) <= 1e-5 * np.linalg.norm(direct)
+
+
+def test_solve_is_bitwise_reproducible(gpu):
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 60, 60, jitter=0.25, seed=3)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    prob = FVM.PoissonsEquation(mesh, BCs, diffusion_function=Dxy, source_function=lambda x, y, p: x - y)
+    for alg in ALGS:
+        a = FVM.solve(prob, alg).u
+        b = FVM.solve(prob, alg).u
+        assert np.array_equal(a, b)
+
+
+def test_diffusion_template_matches_fvm_problem(gpu):
+    """The DiffusionEquation template and the general FVMProblem give the same
+    solution (diffusion_equations.jl:345-347 compares them the same way)."""
+    tri = FVM.triangulate_rectangle(0.0, 2.0, 0.0, 2.0, 30, 30, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    u0 = np.where(tri.y <= 1.0, 50.0, 0.0)
+    tmpl = FVM.DiffusionEquation(mesh, BCs, diffusion_function=lambda x, y, p: 1 / 9,
+                                 initial_condition=u0, final_time=0.5)
+    gen = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 1 / 9,
+                         initial_condition=u0, final_time=0.5)
+    s1 = FVM.solve(tmpl, FVM.Tsit5(), saveat=0.1, abstol=1e-10, reltol=1e-10)
+    s2 = FVM.solve(gen, FVM.Tsit5(), saveat=0.1, abstol=1e-10, reltol=1e-10)
+    assert len(s1.u) == len(s2.u) == 6
+    for a, b in zip(s1.u, s2.u):
+        assert len(a) == tri.num_points() + 1 and a[-1] == 1.0
+        assert np.max(np.abs(a[:-1] - b)) <= 1e-7 * 50
+
+
+def test_template_argument_errors(gpu):
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 10, 10, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    z = lambda x, y, t, u, p: 0.0  # noqa: E731
+    BCs = FVM.BoundaryConditions(mesh, (z, z, z, z), (FVM.Dirichlet, FVM.Dudt, FVM.Neumann, FVM.Constrained))
+    with pytest.raises(ValueError):
+        FVM.PoissonsEquation(mesh, BCs, source_function=lambda x, y, p: 1.0)
+    with pytest.raises(ValueError):
+        FVM.MeanExitTimeProblem(mesh, BCs, diffusion_function=lambda x, y, p: 1.0)
+    BCs2 = FVM.BoundaryConditions(mesh, (z, z, z, z), (FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet, FVM.Constrained))
+    with pytest.raises(ValueError):
+        FVM.MeanExitTimeProblem(mesh, BCs2, diffusion_function=lambda x, y, p: 1.0)
+    d = FVM.DiffusionEquation(mesh, BCs2, diffusion_function=lambda x, y, p: 1.0,
+                              initial_condition=np.zeros(tri.num_points()), final_time=1.0)
+    with pytest.raises(FVM.FVMError):
+        d.problem.solve(TM.CG())
+    p = FVM.PoissonsEquation(mesh, BCs2, source_function=lambda x, y, p: 1.0)
+    with pytest.raises(TypeError):
+        p.problem.solve(object())

@@ -26,17 +26,11 @@ import scipy.sparse.linalg as spla
 import finitevolumemethod as FVM
 from brute import control_volumes, dudt as brute_dudt
 from oracle import OracleProblem, flatten_conditions, geometry, merge_conditions
+from problems import disk_triangulation
 
 
-def _disk(nr=12, R=1.0, seed=0):
-    pts = [(0.0, 0.0)]
-    for k in range(1, nr + 1):
-        r = R * k / nr
-        m = 6 * k
-        th = 2 * np.pi * np.arange(m) / m + 0.1 * k
-        pts += list(zip(r * np.cos(th), r * np.sin(th)))
-    P = np.array(pts)
-    return FVM.triangulate(P[:, 0], P[:, 1])
+def _disk(nr=12, R=1.0):
+    return disk_triangulation(nr, R)
 
 
 def _meshes():
