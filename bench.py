@@ -171,6 +171,88 @@ def template_solve(n_side, rtol=1e-10):
     return out
 
 
+def ensemble_bench(P=256, steps=1000, graph_steps=100):
+    """The ensemble workload: P rectangles [0, 2]^2 with nx, ny ~ round(exp(U[log 20,
+    log 300])) drawn independently, interior points jittered by U(-h/4, h/4), constant
+    D_p ~ LogNormal(log 0.1, 0.5), homogeneous Dirichlet, u0 ~ U[0, 1), forward Euler
+    with dt_p = 0.9 x the Gershgorin stability limit, `steps` steps.  Times the packed
+    ensemble (autotuned tiles, graph replay) against looping the single-problem path
+    over the members (each member's steps issued back to back from C)."""
+    import finitevolumemethod as FVM
+    from finitevolumemethod import _lib
+    from finitevolumemethod import ensemble as EN
+
+    rng = np.random.default_rng(SEED + 5)
+    t0 = time.time()
+    probs = []
+    for p in range(P):
+        nx, ny = (int(round(np.exp(rng.uniform(np.log(20), np.log(300))))) for _ in range(2))
+        tri = FVM.jittered_rectangle(0.0, 2.0, 0.0, 2.0, nx, ny, jitter=0.25, seed=SEED + 1000 + p)
+        mesh = FVM.FVMGeometry(tri)
+        BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+        D = float(np.exp(np.log(0.1) + 0.5 * rng.standard_normal()))
+        u0 = rng.random(tri.num_points())
+        probs.append(FVM.FVMProblem(mesh, BCs, diffusion_function=(lambda d: (lambda x, y, t, u, p: d + 0.0 * u))(D),
+                                    initial_condition=u0, final_time=1.0))
+    dts = [EN.explicit_dt(q) for q in probs]
+    N = sum(q.mesh.triangulation.num_points() for q in probs)
+    T = sum(len(q.mesh.triangulation.triangles) for q in probs)
+    log(f"ensemble: {P} members, {N} nodes, {T} triangles ({time.time() - t0:.1f}s setup)")
+    t0 = time.time()
+    b = EN.DeviceBatch(probs, dts, autotune=True, graph_steps=graph_steps)
+    log(f"packed batch built and autotuned in {time.time() - t0:.1f}s: tile {b.info.tile_triangles}, "
+        f"candidates {list(b.info.candidate_tiles)[:b.info.n_candidates]} "
+        f"{[round(v, 4) for v in list(b.info.candidate_ms)[:b.info.n_candidates]]} ms")
+    u0 = b.pack([q.initial_condition for q in probs])
+    d_u = _lib.DeviceArray.from_host(u0)
+    b.euler(d_u, graph_steps, 0.0)  # warm-up (capture + one replay)
+    d_u.upload(u0)
+    _lib.synchronize()
+    tm = _lib.Timer()
+    tm.start()
+    b.euler(d_u, steps, 0.0)
+    ms_batch = tm.stop()
+    batched = d_u.to_host()
+    # the same steps without the graph (one launch sequence per step)
+    b0 = EN.DeviceBatch(probs, dts, tile_triangles=b.info.tile_triangles, graph_steps=0)
+    d_u.upload(u0)
+    b0.euler(d_u, 10, 0.0)
+    d_u.upload(u0)
+    _lib.synchronize()
+    tm.start()
+    b0.euler(d_u, steps, 0.0)
+    ms_nograph = tm.stop()
+    # looping the single-problem path over the members
+    singles = [EN.DeviceBatch([q], [dt], graph_steps=0) for q, dt in zip(probs, dts)]
+    us = [_lib.DeviceArray.from_host(q.initial_condition) for q in probs]
+    for s, u in zip(singles, us):
+        s.euler(u, 5, 0.0)
+    for q, u in zip(probs, us):
+        u.upload(q.initial_condition)
+    _lib.synchronize()
+    tm.start()
+    for s, u in zip(singles, us):
+        s.euler(u, steps, 0.0)
+    ms_loop = tm.stop()
+    looped = np.concatenate([u.to_host() for u in us])
+    err = float(np.max(np.abs(batched - looped)) / max(1e-300, np.max(np.abs(looped))))
+    per_step = 132 * T + 25 * N
+    out = dict(workload=f"{P} jittered rectangles on [0,2]^2, nx,ny~logU[20,300], D~LogNormal(log 0.1,0.5), "
+                        f"homogeneous Dirichlet, forward Euler dt=0.9x stability limit, {steps} steps",
+               members=P, nodes=int(N), triangles=int(T), steps=steps,
+               tile_triangles=int(b.info.tile_triangles),
+               autotune_ms={int(t): round(float(m), 5) for t, m in
+                            zip(list(b.info.candidate_tiles)[:b.info.n_candidates],
+                                list(b.info.candidate_ms)[:b.info.n_candidates])},
+               batched_graph_ms=round(ms_batch, 3), batched_nograph_ms=round(ms_nograph, 3),
+               looped_ms=round(ms_loop, 3), speedup_vs_loop=round(ms_loop / ms_batch, 2),
+               algorithmic_GBps=round(per_step * steps / (ms_batch * 1e-3) / 1e9, 1),
+               max_rel_diff_vs_loop=err)
+    log(f"ensemble: batched {ms_batch:.1f} ms (no graph {ms_nograph:.1f} ms), looped {ms_loop:.1f} ms, "
+        f"diff {err:.2e}")
+    return out
+
+
 def cpu_baseline(tri, u, budget_s, nthreads):
     """The oracle's restatement of parallel_fvm_eqs! (per-thread du columns, serial
     combine) on the same mesh and state, for about budget_s seconds."""
@@ -217,6 +299,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--template-side", type=int, default=1000,
                     help="PoissonsEquation solve workload side (0: skip)")
+    ap.add_argument("--ensemble", type=int, default=256, help="ensemble members (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the finite-volume path runs on a single MI355X here "
@@ -284,6 +368,8 @@ def main():
         log(f"VJP/RHS {out['vjp_rhs']}")
         if args.template_side > 0:
             out["template_solve"] = template_solve(args.template_side)
+        if args.ensemble > 0:
+            out["ensemble"] = ensemble_bench(args.ensemble, args.ensemble_steps)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         du_cpu, cb = cpu_baseline(tri, u, args.cpu_seconds, threads)

@@ -207,6 +207,14 @@ void fvmh_tiles_free(fvmh_tiles* t) {
 int fvmh_tiles_build(int64_t n_points, const double* x, const double* y, int64_t n_tri,
                      const int32_t* tri, int64_t n_bedges, const int32_t* bedges,
                      int tile_tris, int max_local_nodes, fvmh_tiles** out) {
+  return fvmh_tiles_build_grouped(n_points, x, y, n_tri, tri, nullptr, n_bedges, bedges, tile_tris,
+                                  max_local_nodes, out);
+}
+
+int fvmh_tiles_build_grouped(int64_t n_points, const double* x, const double* y, int64_t n_tri,
+                             const int32_t* tri, const int32_t* tri_group, int64_t n_bedges,
+                             const int32_t* bedges, int tile_tris, int max_local_nodes,
+                             fvmh_tiles** out) {
   *out = nullptr;
   if (tile_tris < 1 || tile_tris > 4096) return fail("fvmh_tiles_build: tile_tris must lie in [1, 4096]");
   if (max_local_nodes < 3 || max_local_nodes > 1024)
@@ -239,6 +247,10 @@ int fvmh_tiles_build(int64_t n_points, const double* x, const double* y, int64_t
     keys[t] = (m << 32) | (uint64_t)t;
   }
   std::sort(keys.begin(), keys.end());
+  if (tri_group)  // members stay contiguous, each in its own Morton order
+    std::stable_sort(keys.begin(), keys.end(), [&](uint64_t a, uint64_t b) {
+      return tri_group[a & 0xFFFFFFFFu] < tri_group[b & 0xFFFFFFFFu];
+    });
   T->perm = xalloc<int32_t>(n_tri);
   std::vector<int32_t> invperm(n_tri);
   for (int64_t s = 0; s < n_tri; ++s) {
@@ -261,7 +273,8 @@ int fvmh_tiles_build(int64_t n_points, const double* x, const double* y, int64_t
       int add = 0;
       for (int v = 0; v < 3; ++v) add += stamp[vert(s, v)] != tile;
       // a triangle's vertices are distinct, so add counts new nodes exactly
-      if (count == tile_tris || nodes + add > max_local_nodes) {
+      const bool new_group = tri_group && s > 0 && tri_group[T->perm[s]] != tri_group[T->perm[s - 1]];
+      if (count == tile_tris || nodes + add > max_local_nodes || (new_group && count > 0)) {
         ++tile;
         tile_start.push_back((int32_t)s);
         count = 0; nodes = 0;

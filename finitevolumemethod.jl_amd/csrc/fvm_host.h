@@ -99,6 +99,13 @@ typedef struct fvmh_tiles {
 int fvmh_tiles_build(int64_t n_points, const double* x, const double* y, int64_t n_tri,
                      const int32_t* tri, int64_t n_bedges, const int32_t* bedges,
                      int tile_tris, int max_local_nodes, fvmh_tiles** out);
+// fvmh_tiles_build with the triangles ordered by tri_group first (then Morton
+// order within a group): an ensemble of disjoint meshes packed into one mesh keeps
+// each member's triangles contiguous, so no tile interleaves two members.
+int fvmh_tiles_build_grouped(int64_t n_points, const double* x, const double* y, int64_t n_tri,
+                             const int32_t* tri, const int32_t* tri_group, int64_t n_bedges,
+                             const int32_t* bedges, int tile_tris, int max_local_nodes,
+                             fvmh_tiles** out);
 void fvmh_tiles_free(fvmh_tiles* t);
 
 // Sparsity of the Jacobian / template operators: row i holds i and every

@@ -13,6 +13,8 @@ namespace fvm {
 
 void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
+// fvm_mesh_create with the triangles of each tri_group kept contiguous (NULL: one group)
+int mesh_create_grouped(const fvm_mesh_desc* d, const int32_t* tri_group, fvm_mesh** out);
 
 #define FVM_HIP(call)                                        \
   do {                                                       \

@@ -122,7 +122,7 @@ void free_mesh(fvm_mesh* m) {
   delete m;
 }
 
-int build(const fvm_mesh_desc* d, fvm_mesh* m) {
+int build(const fvm_mesh_desc* d, const int32_t* tri_group, fvm_mesh* m) {
   // Tiles of up to 512 triangles run on 256-thread workgroups with room for 512
   // local nodes; tiles of up to 1024 triangles on 512-thread workgroups with 1024.
   const int tile_tris = d->tile_triangles > 0 ? d->tile_triangles : 512;
@@ -132,8 +132,8 @@ int build(const fvm_mesh_desc* d, fvm_mesh* m) {
   FVM_CHECK(max_nodes <= node_limit,
             "fvm_mesh_create: tile_max_nodes must be <= 512 (<= 1024 for tiles above 512 triangles)");
   fvmh_tiles* tl = nullptr;
-  if (fvmh_tiles_build(d->n_points, d->x, d->y, d->n_triangles, d->triangles, d->n_boundary_edges,
-                       d->boundary_edges, tile_tris, max_nodes, &tl) != 0) {
+  if (fvmh_tiles_build_grouped(d->n_points, d->x, d->y, d->n_triangles, d->triangles, tri_group,
+                               d->n_boundary_edges, d->boundary_edges, tile_tris, max_nodes, &tl) != 0) {
     set_error(std::string("fvm_mesh_create: ") + fvmh_last_error());
     return FVM_ERR_ARG;
   }
@@ -213,9 +213,9 @@ int build(const fvm_mesh_desc* d, fvm_mesh* m) {
 
 }  // namespace
 
-extern "C" {
+namespace fvm {
 
-int fvm_mesh_create(const fvm_mesh_desc* d, fvm_mesh** out) {
+int mesh_create_grouped(const fvm_mesh_desc* d, const int32_t* tri_group, fvm_mesh** out) {
   FVM_CHECK(d && out, "fvm_mesh_create: null argument");
   *out = nullptr;
   FVM_CHECK(d->n_points > 0 && d->n_points < INT32_MAX, "fvm_mesh_create: n_points out of range");
@@ -224,13 +224,21 @@ int fvm_mesh_create(const fvm_mesh_desc* d, fvm_mesh** out) {
   FVM_CHECK(d->x && d->y && d->triangles, "fvm_mesh_create: null mesh array");
   FVM_CHECK(d->n_boundary_edges == 0 || d->boundary_edges, "fvm_mesh_create: null boundary edges");
   fvm_mesh* m = new fvm_mesh();
-  int rc = build(d, m);
+  int rc = build(d, tri_group, m);
   if (rc != FVM_OK) {
     free_mesh(m);
     return rc;
   }
   *out = m;
   return FVM_OK;
+}
+
+}  // namespace fvm
+
+extern "C" {
+
+int fvm_mesh_create(const fvm_mesh_desc* d, fvm_mesh** out) {
+  return fvm::mesh_create_grouped(d, nullptr, out);
 }
 
 int fvm_mesh_destroy(fvm_mesh* m) {

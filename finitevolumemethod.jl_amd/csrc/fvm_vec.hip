@@ -27,6 +27,14 @@ __global__ __launch_bounds__(256) void lincomb_kernel(int64_t n, double* __restr
   }
 }
 
+__global__ __launch_bounds__(256) void mul_kernel(int64_t n, double* __restrict__ out,
+                                                  const double* __restrict__ x,
+                                                  const double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = x[i] * y[i];
+}
+
 // kind 0: sum x*y;  kind 1: sum (e / (abstol + reltol*max(|a|,|b|)))^2
 __device__ __forceinline__ double red_term(int kind, const double* x, const double* y,
                                            const double* a, const double* b, double abstol,
@@ -111,6 +119,16 @@ int fvm_vec_lincomb(int64_t n, double* d_out, int nterms, const double* coefs,
   int64_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(lincomb_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s, n, d_out, t);
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_vec_mul(int64_t n, double* d_out, const double* d_x, const double* d_y, fvm_stream s) {
+  FVM_CHECK(d_out && d_x && d_y, "fvm_vec_mul: null argument");
+  if (n <= 0) return FVM_OK;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(mul_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s, n, d_out, d_x, d_y);
   FVM_HIP(hipGetLastError());
   return FVM_OK;
 }

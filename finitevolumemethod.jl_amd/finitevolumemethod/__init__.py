@@ -13,6 +13,7 @@ from .conditions import (BoundaryConditions, ConditionType, Conditions, Constrai
 from .equations import (fvm_eqs, fvm_eqs_, get_fvm_parameters,  # noqa: F401
                         get_multithreading_parameters, get_serial_parameters, jacobian_sparsity,
                         update_dirichlet_nodes)
+from .ensemble import EnsembleProblem, EnsembleSolution, explicit_dt  # noqa: F401
 from .forms import NodalDiffusion, UnsupportedClosure  # noqa: F401
 from .geometry import FVMGeometry, TriangleProperties  # noqa: F401
 from .mesh import Triangulation, jittered_rectangle, triangulate, triangulate_rectangle  # noqa: F401

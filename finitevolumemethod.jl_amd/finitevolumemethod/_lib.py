@@ -81,6 +81,19 @@ class SolveStats(ctypes.Structure):
                 ("residual", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("tile_triangles", ctypes.c_int32), ("autotune", ctypes.c_int32),
+                ("graph_steps", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("n_problems", ctypes.c_int32), ("tile_triangles", ctypes.c_int32),
+                ("n_points", ctypes.c_int64), ("n_triangles", ctypes.c_int64),
+                ("n_boundary_edges", ctypes.c_int64), ("n_tiles", ctypes.c_int32),
+                ("n_fix_nodes", ctypes.c_int32), ("n_candidates", ctypes.c_int32),
+                ("candidate_tiles", ctypes.c_int32 * 4), ("candidate_ms", ctypes.c_float * 4)]
+
+
 FLUX_D_POLY_U, FLUX_D_NODAL, FLUX_D_EDGE = 0, 1, 2
 TEMPLATE_POISSON, TEMPLATE_MEAN_EXIT_TIME, TEMPLATE_DIFFUSION = 0, 1, 2
 SOLVE_CG, SOLVE_CG_MIXED, SOLVE_BICGSTAB = 0, 1, 2
@@ -163,7 +176,15 @@ HIP_API = {
     "fvm_template_spmv": [_vp, ctypes.c_int32, _vp, _vp, _vp],
     "fvm_template_residual": [_vp, _vp, _dp, _vp],
     "fvm_template_solve": [_vp, ctypes.POINTER(SolveOpts), _vp, ctypes.POINTER(SolveStats), _vp],
+    "fvm_batch_create": [ctypes.c_int32, ctypes.POINTER(MeshDesc), ctypes.POINTER(PhysicsDesc), _dp,
+                         ctypes.POINTER(BatchOpts), ctypes.POINTER(_vp)],
+    "fvm_batch_destroy": [_vp],
+    "fvm_batch_get_info": [_vp, ctypes.POINTER(BatchInfo), _i64p],
+    "fvm_batch_mesh": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
+    "fvm_batch_rhs": [_vp, _vp, _vp, ctypes.c_double, _vp],
+    "fvm_batch_euler": [_vp, _vp, ctypes.c_int32, ctypes.c_double, _vp],
     "fvm_vec_lincomb": [ctypes.c_int64, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), _vp],
+    "fvm_vec_mul": [ctypes.c_int64, _vp, _vp, _vp, _vp],
     "fvm_vec_dot": [ctypes.c_int64, _vp, _vp, _dp, _vp],
     "fvm_vec_err_norm": [ctypes.c_int64, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _dp,
                          _vp],
@@ -302,6 +323,11 @@ def lincomb(out, coefs, xs, stream=None):
     c = (ctypes.c_double * n)(*[float(v) for v in coefs])
     p = (_vp * n)(*[x.ptr for x in xs])
     check(hip_lib().fvm_vec_lincomb(out.n, out.ptr, n, c, p, stream), "fvm_vec_lincomb")
+
+
+def mul(out, x, y, stream=None):
+    """out = x * y elementwise on the device."""
+    check(hip_lib().fvm_vec_mul(len(out), out.ptr, x.ptr, y.ptr, stream), "fvm_vec_mul")
 
 
 def dot(x, y, stream=None):

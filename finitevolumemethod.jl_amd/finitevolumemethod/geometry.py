@@ -52,7 +52,7 @@ class FVMGeometry:
     """Mesh plus control-volume geometry, resident on the GPU.
 
     tile_triangles / tile_max_nodes tune the triangle tiles the kernels stream
-    (0 keeps the defaults, 512 / 640)."""
+    (0 keeps the defaults: 512 triangles, 512 nodes)."""
 
     def __init__(self, tri, tile_triangles=0, tile_max_nodes=0):
         _lib.require_gpu()

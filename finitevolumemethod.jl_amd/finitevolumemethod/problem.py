@@ -120,6 +120,42 @@ def _half_edge_points(tri):
     return mx, my
 
 
+def physics_desc(phys_kwargs):
+    """fvm_physics_desc of traced physics (see physics_of); returns (desc, arrays the
+    desc points into, which must outlive its use)."""
+    keep = []
+
+    def arr(a, dt=np.float64):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a
+
+    d = _lib.PhysicsDesc()
+    d.flux_kind = phys_kwargs["flux_kind"]
+    for k, v in enumerate(phys_kwargs.get("dpoly", [0.0] * 4)):
+        d.dpoly[k] = v
+    for k, v in enumerate(phys_kwargs.get("spoly", [0.0] * 4)):
+        d.spoly[k] = v
+    d.st = phys_kwargs.get("st", 0.0)
+    d.d_edge = _lib.ptr(arr(phys_kwargs.get("d_edge")))
+    d.d_bedge = _lib.ptr(arr(phys_kwargs.get("d_bedge")))
+    d.log_d = _lib.ptr(arr(phys_kwargs.get("log_d")))
+    d.s_node = _lib.ptr(arr(phys_kwargs.get("s_node")))
+    d.node_kind = _lib.ptr(arr(phys_kwargs["node_kind"], np.int8), _lib._i8p)
+    d.node_fidx = _lib.ptr(arr(phys_kwargs["node_fidx"], np.int32), _lib._i32p)
+    d.bedge_neumann_fidx = _lib.ptr(arr(phys_kwargs["bedge_neumann_fidx"], np.int32), _lib._i32p)
+    fn = phys_kwargs["fn"]
+    d.n_functions = len(fn[0])
+    d.fn_c0, d.fn_cu, d.fn_ct = (_lib.ptr(arr(v)) for v in fn[:3])
+    d.fn_cx = _lib.ptr(arr(fn[3]) if len(fn) > 3 else None)
+    d.fn_cy = _lib.ptr(arr(fn[4]) if len(fn) > 4 else None)
+    d.node_g = _lib.ptr(arr(phys_kwargs.get("node_g")))
+    d.bedge_g = _lib.ptr(arr(phys_kwargs.get("bedge_g")))
+    return d, keep
+
+
 class DeviceContext:
     """An fvm_ctx: mesh + traced physics + conditions on the GPU."""
 
@@ -127,36 +163,7 @@ class DeviceContext:
         self.mesh = mesh
         self.n = mesh.triangulation.num_points()
         self.flux_kind = phys_kwargs["flux_kind"]
-        keep = []
-
-        def arr(a, dt=np.float64):
-            if a is None:
-                return None
-            a = np.ascontiguousarray(a, dt)
-            keep.append(a)
-            return a
-
-        d = _lib.PhysicsDesc()
-        d.flux_kind = phys_kwargs["flux_kind"]
-        for k, v in enumerate(phys_kwargs.get("dpoly", [0.0] * 4)):
-            d.dpoly[k] = v
-        for k, v in enumerate(phys_kwargs.get("spoly", [0.0] * 4)):
-            d.spoly[k] = v
-        d.st = phys_kwargs.get("st", 0.0)
-        d.d_edge = _lib.ptr(arr(phys_kwargs.get("d_edge")))
-        d.d_bedge = _lib.ptr(arr(phys_kwargs.get("d_bedge")))
-        d.log_d = _lib.ptr(arr(phys_kwargs.get("log_d")))
-        d.s_node = _lib.ptr(arr(phys_kwargs.get("s_node")))
-        d.node_kind = _lib.ptr(arr(phys_kwargs["node_kind"], np.int8), _lib._i8p)
-        d.node_fidx = _lib.ptr(arr(phys_kwargs["node_fidx"], np.int32), _lib._i32p)
-        d.bedge_neumann_fidx = _lib.ptr(arr(phys_kwargs["bedge_neumann_fidx"], np.int32), _lib._i32p)
-        fn = phys_kwargs["fn"]
-        d.n_functions = len(fn[0])
-        d.fn_c0, d.fn_cu, d.fn_ct = (_lib.ptr(arr(v)) for v in fn[:3])
-        d.fn_cx = _lib.ptr(arr(fn[3]) if len(fn) > 3 else None)
-        d.fn_cy = _lib.ptr(arr(fn[4]) if len(fn) > 4 else None)
-        d.node_g = _lib.ptr(arr(phys_kwargs.get("node_g")))
-        d.bedge_g = _lib.ptr(arr(phys_kwargs.get("bedge_g")))
+        d, self._keep = physics_desc(phys_kwargs)
         h = ctypes.c_void_p()
         _lib.check(_lib.hip_lib().fvm_ctx_create(mesh.handle, ctypes.byref(d), ctypes.byref(h)),
                    "fvm_ctx_create")

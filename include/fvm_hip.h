@@ -7,7 +7,7 @@
  *    crosses this boundary.
  *  - Pointers named d_* are device pointers (from fvm_alloc); all others are host
  *    pointers.  Kernels run asynchronously on the given stream (NULL = the null
- *    stream); only *_synchronize and fvm_memcpy_d2h block.
+ *    stream); only *_synchronize and the host copies (fvm_memcpy_h2d, fvm_memcpy_d2h) block.
  *  - Indices are 0-based int32: reference node k (1-based Julia) is node k-1.
  *  - A context (fvm_ctx) holds scratch buffers: calls on one context must not
  *    overlap on different streams.
@@ -32,6 +32,7 @@ typedef void* fvm_event;  /* hipEvent_t */
 typedef struct fvm_mesh fvm_mesh;
 typedef struct fvm_ctx fvm_ctx;
 typedef struct fvm_template fvm_template;
+typedef struct fvm_batch fvm_batch;
 
 /* ---- runtime ------------------------------------------------------------- */
 /* Error text of the calling thread's last failure (copied into buf, truncated). */
@@ -229,6 +230,49 @@ typedef struct fvm_solve_stats {
 int fvm_template_solve(fvm_template* t, const fvm_solve_opts* o, double* d_x, fvm_solve_stats* st,
                        fvm_stream s);
 
+/* ---- ensembles ------------------------------------------------------------
+ * Many FVMProblems on meshes of uneven size advanced together (what the reference
+ * does by looping solve over an EnsembleProblem, solve.jl:222-256): the members are
+ * packed into one device mesh and context, so one launch of the RHS kernels covers
+ * every member.  Members share flux_kind (and D(u) for FVM_FLUX_D_POLY_U) and the
+ * source polynomial; their meshes, D fields, conditions and s_node are their own. */
+typedef struct fvm_batch_opts {
+  int32_t tile_triangles; /* 0: 512, or the autotuner's choice */
+  int32_t autotune;       /* nonzero (with tile_triangles 0): time 256-, 512- and 1024-triangle
+                             tiles on the packed RHS and keep the fastest */
+  int32_t graph_steps;    /* > 0: fvm_batch_euler replays a HIP graph of this many steps
+                             (closures independent of t only) */
+  int32_t reserved;
+} fvm_batch_opts;
+
+typedef struct fvm_batch_info {
+  int32_t n_problems;
+  int32_t tile_triangles;          /* the tile size in use */
+  int64_t n_points, n_triangles, n_boundary_edges;  /* of the packed mesh */
+  int32_t n_tiles, n_fix_nodes;
+  int32_t n_candidates;            /* autotuner candidates timed */
+  int32_t candidate_tiles[4];
+  float candidate_ms[4];           /* packed RHS time of each candidate */
+} fvm_batch_info;
+
+/* dt [n_problems]: each member's explicit time step (NULL: fvm_batch_euler unavailable).
+ * Closures that depend on t need one common dt. */
+int fvm_batch_create(int32_t n_problems, const fvm_mesh_desc* meshes,
+                     const fvm_physics_desc* physics, const double* dt,
+                     const fvm_batch_opts* opts, fvm_batch** out);
+int fvm_batch_destroy(fvm_batch* b);
+/* node_offsets [n_problems + 1] (may be NULL): member p owns packed nodes
+ * [node_offsets[p], node_offsets[p+1]), in its own node order. */
+int fvm_batch_get_info(const fvm_batch* b, fvm_batch_info* info, int64_t* node_offsets);
+/* the packed mesh and context (owned by the batch): fvm_rhs / fvm_jvp / fvm_vjp on them
+ * evaluate every member at once */
+int fvm_batch_mesh(const fvm_batch* b, const fvm_mesh** mesh, const fvm_ctx** ctx);
+/* fvm_eqs! of every member at time t, one launch of the tile and fix-up kernels */
+int fvm_batch_rhs(fvm_batch* b, const double* d_u, double* d_du, double t, fvm_stream s);
+/* n_steps explicit Euler steps of every member from t0: u += dt_p fvm_eqs!(u), then the
+ * Dirichlet callback (solve.jl:133-165) at the member's new time. */
+int fvm_batch_euler(fvm_batch* b, double* d_u, int32_t n_steps, double t0, fvm_stream s);
+
 /* ---- device vector operations (used by the integrators and solvers) ------
  * The reference broadcasts over Vector{Float64} inside OrdinaryDiffEq; device-
  * resident state needs these as kernels.  Reductions use a fixed grid and order,
@@ -236,6 +280,8 @@ int fvm_template_solve(fvm_template* t, const fvm_solve_opts* o, double* d_x, fv
 /* d_out = sum_k coefs[k] * d_xs[k], 1 <= nterms <= 8 (d_out may alias an input). */
 int fvm_vec_lincomb(int64_t n, double* d_out, int nterms, const double* coefs,
                     const double* const* d_xs, fvm_stream s);
+/* d_out = d_x * d_y elementwise (d_out may alias an input). */
+int fvm_vec_mul(int64_t n, double* d_out, const double* d_x, const double* d_y, fvm_stream s);
 int fvm_vec_dot(int64_t n, const double* d_x, const double* d_y, double* result, fvm_stream s);
 /* sqrt(mean((err_i / (abstol + reltol * max(|u0_i|, |u1_i|)))^2)): the
  * error norm of OrdinaryDiffEq's adaptive step control. */

@@ -1,0 +1,124 @@
+"""Packed ensembles (fvm_batch_*): one launch evaluates every member's fvm_eqs!,
+and explicit Euler steps (graph-replayed or not) match stepping each member
+alone through the single-problem path."""
+import numpy as np
+import pytest
+
+import finitevolumemethod as FVM
+from finitevolumemethod import ensemble as EN
+
+pytestmark = pytest.mark.gpu
+
+
+def members(P=6, seed=5):
+    """Uneven jittered rectangles on [0, 2]^2; constant D per member except one with
+    D(x, y); homogeneous Dirichlet, or Dirichlet + Neumann segments."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for p in range(P):
+        nx, ny = int(rng.integers(8, 41)), int(rng.integers(8, 41))
+        single = p % 3 != 2
+        tri = FVM.jittered_rectangle(0.0, 2.0, 0.0, 2.0, nx, ny, jitter=0.25, seed=100 + p,
+                                     single_boundary=single)
+        mesh = FVM.FVMGeometry(tri)
+        if single:
+            BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+        else:
+            BCs = FVM.BoundaryConditions(
+                mesh, (lambda x, y, t, u, p: 0.0 * u, lambda x, y, t, u, p: 0.5 + 0.0 * u,
+                       lambda x, y, t, u, p: 1.0 + 0.0 * u, lambda x, y, t, u, p: -0.25 * u),
+                (FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet, FVM.Neumann))
+        D = float(np.exp(np.log(0.1) + 0.5 * rng.standard_normal()))
+        if p == 1:
+            dfun = lambda x, y, t, u, p: 0.1 * (1 + x * y)  # noqa: E731
+        else:
+            dfun = (lambda d: (lambda x, y, t, u, p: d + 0.0 * u))(D)
+        u0 = rng.random(tri.num_points())
+        probs.append(FVM.FVMProblem(mesh, BCs, diffusion_function=dfun, initial_condition=u0,
+                                    final_time=1.0))
+    return probs
+
+
+def test_packed_rhs_matches_each_member(gpu):
+    probs = members()
+    b = EN.DeviceBatch(probs)
+    u = b.pack([p.initial_condition for p in probs])
+    du = FVM.DeviceArray(b.n)
+    b.rhs(FVM.DeviceArray.from_host(u), du, 0.3)
+    got = b.unpack(du.to_host())
+    for p, g in zip(probs, got):
+        ref = FVM.fvm_eqs(np.zeros_like(p.initial_condition), p.initial_condition, p, 0.3)
+        scale = np.max(np.abs(ref)) + 1e-300
+        assert np.max(np.abs(g - ref)) <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("graph_steps", [0, 5])
+def test_batched_euler_matches_member_loop(gpu, graph_steps):
+    probs = members()
+    dts = [0.5 * EN.explicit_dt(p) for p in probs]
+    nsteps = 23
+    b = EN.DeviceBatch(probs, dts, graph_steps=graph_steps)
+    d_u = FVM.DeviceArray.from_host(b.pack([p.initial_condition for p in probs]))
+    b.euler(d_u, nsteps, 0.0)
+    got = b.unpack(d_u.to_host())
+    for p, dt, g in zip(probs, dts, got):
+        ctx = p.device_context()
+        u = FVM.DeviceArray.from_host(p.initial_condition)
+        du = FVM.DeviceArray(len(u))
+        for k in range(nsteps):
+            ctx.rhs(u, du, k * dt)
+            FVM._lib.lincomb(u, (1.0, dt), (u, du))
+            ctx.apply_dirichlet(u, (k + 1) * dt)
+        ref = u.to_host()
+        assert np.max(np.abs(g - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_graph_replay_and_direct_steps_agree_bitwise(gpu):
+    probs = members(P=4, seed=9)
+    dts = [0.5 * EN.explicit_dt(p) for p in probs]
+    outs = []
+    for gs in (0, 4):
+        b = EN.DeviceBatch(probs, dts, graph_steps=gs)
+        d_u = FVM.DeviceArray.from_host(b.pack([p.initial_condition for p in probs]))
+        b.euler(d_u, 12, 0.0)  # 3 graph replays
+        b.euler(d_u, 6, 0.0)   # 1 replay + 2 direct steps
+        outs.append(d_u.to_host())
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_autotuner_reports_candidates(gpu):
+    probs = members(P=3, seed=2)
+    b = EN.DeviceBatch(probs, autotune=True)
+    info = b.info
+    assert info.n_candidates == 3
+    tiles = list(info.candidate_tiles)[:3]
+    assert sorted(tiles) == [256, 512, 1024]
+    ms = list(info.candidate_ms)[:3]
+    assert all(m > 0 for m in ms)
+    assert info.tile_triangles == tiles[int(np.argmin(ms))]
+
+
+def test_ensemble_solve_and_errors(gpu):
+    probs = members(P=3, seed=4)
+    dts = [0.5 * EN.explicit_dt(p) for p in probs]
+    n = 10
+    for p, dt in zip(probs, dts):
+        p.final_time = p.initial_time + n * dt
+    ens = FVM.EnsembleProblem(probs)
+    sol = ens.solve(FVM.Euler(), dt=dts, graph_steps=4, autotune=False)
+    assert len(sol) == 3 and sol.stats["steps"] == n
+    b = EN.DeviceBatch(probs, dts)
+    d_u = FVM.DeviceArray.from_host(b.pack([p.initial_condition for p in probs]))
+    b.euler(d_u, n, 0.0)
+    for s, ref in zip(sol.u, b.unpack(d_u.to_host())):
+        assert np.array_equal(s.u[-1], ref)
+    with pytest.raises(ValueError):
+        ens.solve(FVM.Euler(), dt=[dts[0], dts[1] / 2, dts[2]])
+    # members must share the source polynomial
+    q = probs[0]
+    other = FVM.FVMProblem(q.mesh, FVM.BoundaryConditions(q.mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet),
+                           diffusion_function=lambda x, y, t, u, p: 0.2 + 0.0 * u,
+                           source_function=lambda x, y, t, u, p: u, initial_condition=q.initial_condition,
+                           final_time=1.0)
+    with pytest.raises(FVM.FVMError):
+        EN.DeviceBatch([q, other])
