@@ -74,10 +74,31 @@ struct KArgs {
   double t;
 };
 
-// Fields of the triangle record a flux form reads (FVM_FLUX_D_EDGE needs neither
-// gamma nor the CV-edge midpoints).
-__host__ __device__ constexpr bool needs_field(int fk, int f) {
-  return fk != FVM_FLUX_D_EDGE || (f < 6 || f >= P_NX);
+// Fields of the triangle record the tile kernels stream: s1..s6, normals and
+// lengths.  FVM_FLUX_D_EDGE needs neither gamma (s7..s9) nor the CV-edge
+// midpoints; the other forms rebuild both from the vertex coordinates
+// (recompute_geometry), which is cheaper than streaming them.
+__host__ __device__ constexpr bool loads_field(int fk, int f) { return f < 6 || f >= P_NX; }
+
+// s7..s9 and the three CV-edge midpoints of triangle (p, q, r), with exactly the
+// operations of the geometry kernel (geometry.jl:114-151), so the values equal the
+// stored ones bit for bit.
+__device__ __forceinline__ void recompute_geometry(double px, double py, double qx, double qy,
+                                                   double rx, double ry, double* P) {
+  const double cx = (px + qx + rx) / 3, cy = (py + qy + ry) / 3;
+  const double m1x = (px + qx) / 2, m1y = (py + qy) / 2;
+  const double m2x = (qx + rx) / 2, m2y = (qy + ry) / 2;
+  const double m3x = (rx + px) / 2, m3y = (ry + py) / 2;
+  const double D = qx * ry - qy * rx - px * ry + rx * py + px * qy - qx * py;
+  P[6] = (qx * ry - rx * qy) / D;
+  P[7] = (rx * py - px * ry) / D;
+  P[8] = (px * qy - qx * py) / D;
+  P[9] = (m1x + cx) / 2;
+  P[10] = (m1y + cy) / 2;
+  P[11] = (m2x + cx) / 2;
+  P[12] = (m2y + cy) / 2;
+  P[13] = (m3x + cx) / 2;
+  P[14] = (m3y + cy) / 2;
 }
 
 __device__ __forceinline__ double dpoly(const KArgs& a, double u) {
@@ -205,7 +226,7 @@ __device__ double bedge_flux(const KArgs& a, int e, int ep) {
     dbe = s[3] * wa + s[4] * wb + s[5] * wc;
     dga = s[6] * wa + s[7] * wb + s[8] * wc;
   }
-  const double mx = G.mx[ep], my = G.my[ep];
+  const double mx = ep ? G.mx[1] : G.mx[0], my = ep ? G.my[1] : G.my[0];
   const int nf = a.bneu[e];
   double qn;
   if (nf < 0) {
@@ -276,6 +297,9 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
   __shared__ double s_u[MAXN];
   __shared__ double s_v[MODE == M_JVP ? MAXN : 1];
   __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
+  constexpr bool RC = FK != FVM_FLUX_D_EDGE;  // s7..s9 and midpoints rebuilt from coordinates
+  __shared__ double s_x[RC ? MAXN : 1];
+  __shared__ double s_y[RC ? MAXN : 1];
   __shared__ double s_q[3 * TILE];
   __shared__ uint16_t s_inc[3 * TILE];
   __shared__ int s_is[MAXN + 1];
@@ -328,24 +352,28 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
       L[r] = a.loc[s];
 #pragma unroll
       for (int f = 0; f < P_COUNT; ++f)
-        if (needs_field(FK, f)) P[r][f] = __builtin_nontemporal_load(a.props + f * a.T + s);
+        if (loads_field(FK, f)) P[r][f] = __builtin_nontemporal_load(a.props + f * a.T + s);
       if (FK == FVM_FLUX_D_EDGE) {
 #pragma unroll
         for (int e = 0; e < 3; ++e) Ds[r][e] = __builtin_nontemporal_load(a.d_edge + e * a.T + s);
       }
     }
   }
-  double ug[NS], vg[NS], Vg[NS], lg[NS];
+  double ug[NS], vg[NS], Vg[NS], lg[NS], xg[NS], yg[NS];
   int cg[NS];
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
     const int l = tid + r * BLOCK;
-    ug[r] = vg[r] = Vg[r] = lg[r] = 0.0;
+    ug[r] = vg[r] = Vg[r] = lg[r] = xg[r] = yg[r] = 0.0;
     cg[r] = 3;
     if (l < nn) {
       ug[r] = a.u[g[r]];
       if (MODE == M_JVP) vg[r] = a.v[g[r]];
       if (FK == FVM_FLUX_D_NODAL) lg[r] = a.log_d[g[r]];
+      if (RC) {
+        xg[r] = a.x[g[r]];
+        yg[r] = a.y[g[r]];
+      }
       if (pi[r] < 0) {
         cg[r] = -1 - pi[r];
         Vg[r] = a.lcv[n0 + l];
@@ -359,6 +387,10 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
       s_u[l] = ug[r];
       if (MODE == M_JVP) s_v[l] = vg[r];
       if (FK == FVM_FLUX_D_NODAL) s_D[l] = exp(lg[r]);
+      if (RC) {
+        s_x[l] = xg[r];
+        s_y[l] = yg[r];
+      }
       s_is[l] = is[r];
     }
   }
@@ -374,6 +406,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
     const int lt = tid + r * BLOCK;
     if (lt < nt) {
       const int ia = L[r] & 1023u, ib = (L[r] >> 10) & 1023u, ic = (L[r] >> 20) & 1023u;
+      if (RC) recompute_geometry(s_x[ia], s_y[ia], s_x[ib], s_y[ib], s_x[ic], s_y[ic], P[r]);
       double Q[3];
       tri_fluxes<FK, MODE>(a, P[r], s_u[ia], s_u[ib], s_u[ic],
                            MODE == M_JVP ? s_v[ia] : 0.0, MODE == M_JVP ? s_v[ib] : 0.0,
@@ -496,7 +529,7 @@ __device__ void bedge_adj(const KArgs& a, int e, int slot, double& ubar, double&
   for (int ep = 0; ep < 2; ++ep) {
     const int node = a.bedges[ep * a.B + e];
     const double Qbar = -wt_of(a, node);  // du_node -= Q
-    const double mx = G.mx[ep], my = G.my[ep];
+    const double mx = ep ? G.mx[1] : G.mx[0], my = ep ? G.my[1] : G.my[0];
     if (nf < 0) {
       double Dg = 0, Dbar = 0;
       if (FK == FVM_FLUX_D_NODAL) Dg = da * mx + db * my + dg;
@@ -522,6 +555,9 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
   __shared__ double s_u[MAXN];
   __shared__ double s_w[MAXN];
   __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
+  constexpr bool RC = FK != FVM_FLUX_D_EDGE;  // s7..s9 and midpoints rebuilt from coordinates
+  __shared__ double s_x[RC ? MAXN : 1];
+  __shared__ double s_y[RC ? MAXN : 1];
   __shared__ double s_c[CW * TILE];
   const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
   const int wid = xcd_remap(blockIdx.x, gridDim.x);
@@ -554,7 +590,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
       L[r] = a.loc[s];
 #pragma unroll
       for (int f = 0; f < P_COUNT; ++f)
-        if (needs_field(FK, f)) P[r][f] = __builtin_nontemporal_load(a.props + f * a.T + s);
+        if (loads_field(FK, f)) P[r][f] = __builtin_nontemporal_load(a.props + f * a.T + s);
       if (FK == FVM_FLUX_D_EDGE) {
 #pragma unroll
         for (int e = 0; e < 3; ++e) Ds[r][e] = __builtin_nontemporal_load(a.d_edge + e * a.T + s);
@@ -566,14 +602,19 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
     s_u[l] = a.u[g];
     s_w[l] = wt_of(a, g);
     if (FK == FVM_FLUX_D_NODAL) s_D[l] = exp(a.log_d[g]);
+    if (RC) {
+      s_x[l] = a.x[g];
+      s_y[l] = a.y[g];
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
     const int lt = tid + r * BLOCK;
     if (lt < nt) {
-      const double* p = P[r];
       const int id[3] = {(int)(L[r] & 1023u), (int)((L[r] >> 10) & 1023u), (int)((L[r] >> 20) & 1023u)};
+      if (RC) recompute_geometry(s_x[id[0]], s_y[id[0]], s_x[id[1]], s_y[id[1]], s_x[id[2]], s_y[id[2]], P[r]);
+      const double* p = P[r];
       const double ua = s_u[id[0]], ub = s_u[id[1]], uc = s_u[id[2]];
       const double al = p[0] * ua + p[1] * ub + p[2] * uc;
       const double be = p[3] * ua + p[4] * ub + p[5] * uc;

@@ -28,7 +28,7 @@ def explicit_dt(prob, safety=0.9):
         raise ValueError("explicit_dt needs a problem given by a diffusion_function")
     Dp = prob.diffusion_parameters
     tmpl = DeviceTemplate(prob.mesh, prob.conditions, _lib.TEMPLATE_DIFFUSION,
-                          lambda x, y, p: D(x, y, 0.0, 0.0 * x, Dp), None)
+                          lambda x, y, p: D(x, y, 0.0, 0.0 * x, Dp), None, condition_values=False)
     rp, col, val, _ = tmpl.download()
     rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
     rho = np.bincount(rows, weights=np.abs(val), minlength=len(rp) - 1).max()

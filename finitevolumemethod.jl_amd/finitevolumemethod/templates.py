@@ -86,7 +86,7 @@ class DeviceTemplate:
     """An fvm_template: A and b assembled on the GPU."""
 
     def __init__(self, mesh, conditions, kind, diffusion_function, diffusion_parameters,
-                 f_node=None, r_node=None):
+                 f_node=None, r_node=None, condition_values=True):
         tri = mesh.triangulation
         node_kind, fidx, bneu = conditions.flatten(tri)
         P = mesh.props_array
@@ -99,14 +99,15 @@ class DeviceTemplate:
         B = len(tri.boundary_edges)
         funcs = conditions.functions
         neu_a = np.zeros(2 * B)
-        for f in np.unique(bneu[bneu >= 0]):
-            edges = np.nonzero(bneu == f)[0]
-            pts = np.stack([2 * edges, 2 * edges + 1], 1).ravel()
-            neu_a[pts] = _eval_static(funcs[f], mx[pts], my[pts])
         cond_val = np.zeros(n)
-        for f in np.unique(fidx[node_kind != 0]):
-            nodes = np.nonzero((node_kind != 0) & (fidx == f))[0]
-            cond_val[nodes] = _eval_static(funcs[f], tri.x[nodes], tri.y[nodes])
+        if condition_values:  # b only: the operator A does not depend on them
+            for f in np.unique(bneu[bneu >= 0]):
+                edges = np.nonzero(bneu == f)[0]
+                pts = np.stack([2 * edges, 2 * edges + 1], 1).ravel()
+                neu_a[pts] = _eval_static(funcs[f], mx[pts], my[pts])
+            for f in np.unique(fidx[node_kind != 0]):
+                nodes = np.nonzero((node_kind != 0) & (fidx == f))[0]
+                cond_val[nodes] = _eval_static(funcs[f], tri.x[nodes], tri.y[nodes])
         keep = []
 
         def arr(a, dt=np.float64, p=_lib._dp):

@@ -226,6 +226,7 @@ def test_diffusion_template_matches_fvm_problem(gpu):
     mesh = FVM.FVMGeometry(tri)
     BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
     u0 = np.where(tri.y <= 1.0, 50.0, 0.0)
+    u0[tri.boundary_nodes()] = 0.0  # the template applies the Dirichlet values to u0 itself
     tmpl = FVM.DiffusionEquation(mesh, BCs, diffusion_function=lambda x, y, p: 1 / 9,
                                  initial_condition=u0, final_time=0.5)
     gen = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 1 / 9,
