@@ -37,16 +37,26 @@ using namespace fvm;
 
 namespace {
 
-constexpr int kRT = 256;     // threads per workgroup of every solver kernel
-constexpr int kMaxG = 2048;  // reduction grid (fixed per template)
-constexpr int kLPR = 8;      // lanes per CSR row in the SpMV kernels
+#ifndef FVM_SOLVER_THREADS
+#define FVM_SOLVER_THREADS 512
+#endif
+#ifndef FVM_SPMV_ROWS
+#define FVM_SPMV_ROWS 2
+#endif
+constexpr int kRT = FVM_SOLVER_THREADS;  // threads per workgroup of every solver kernel
+constexpr int kMaxG = 1024;              // workgroups of the solver kernels (fixed per template)
+constexpr int kLPR = 8;                  // lanes per CSR row in the SpMV kernels
 constexpr int kRowsPerBlock = kRT / kLPR;
+constexpr int kRPG = FVM_SPMV_ROWS;      // rows per lane group per step of the SpMV loops
 
 // solver scalars (device doubles)
+// (S_RZ, S_ALPHA, S_RHO and S_OMEGA take two slots each: value of an even / odd iteration)
 enum {
-  S_RZ = 0, S_ALPHA, S_BETA, S_WR, S_WR0, S_TOL2, S_ITER, S_DONE, S_RES2, S_B2,
-  S_RHO, S_OMEGA, S_FLAG, S_COUNT = 16
+  S_RZ = 0, S_ALPHA = 2, S_RHO = 4, S_OMEGA = 6,
+  S_WR = 8, S_TOL2, S_ITER, S_DONE, S_RES2, S_B2, S_FLAG, S_COUNT = 16
 };
+// partial-sum slots (kMaxG doubles each)
+enum { PT_RZ = 0, PT_WR, PT_PQ, PT_TT, PT_RES, PT_COUNT };
 
 }  // namespace
 
@@ -67,9 +77,8 @@ struct fvm_template {
   float* dinv32 = nullptr;
   float* wv32 = nullptr;
   double* work = nullptr;    // 10 n doubles
-  double* part = nullptr;    // [4 * kMaxG] reduction partials
+  double* part = nullptr;    // [PT_COUNT * kMaxG] reduction partials
   double* sc = nullptr;      // [S_COUNT] solver scalars
-  unsigned* cnt = nullptr;   // last-workgroup counter
   int G = 1;
 };
 
@@ -250,55 +259,50 @@ __global__ __launch_bounds__(256) void scaled_kernel(int64_t N, const int32_t* _
 }
 
 // ----------------------------------------------------------------------------
-// reductions that finish in the last workgroup
+// reductions
 // ----------------------------------------------------------------------------
-// Sums K per-thread values over the grid.  Returns true in the workgroup that
-// arrives last; there every thread gets the totals, added in an order that does
-// not depend on which workgroup was last.
-template <int K>
-__device__ __forceinline__ bool grid_sum(double (&v)[K], double* part, unsigned* cnt) {
-  __shared__ double sh[K][kRT];
-  __shared__ int last;
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int k = 0; k < K; ++k) sh[k][tid] = v[k];
+// A kernel leaves one partial sum per workgroup in part[]; the next kernel's
+// workgroups each add the partials again, in the same fixed order, so every
+// workgroup holds the same total without atomics or fences, and solver scalars
+// never leave the device.  Scalars that later kernels need are stored by
+// workgroup 0 in slots alternating with the iteration's parity.
+
+// Sum over the workgroup (every thread gets the total).
+__device__ __forceinline__ double wg_sum(double v) {
+  __shared__ double sh[kRT];
+  sh[threadIdx.x] = v;
   __syncthreads();
   for (int s = kRT / 2; s > 0; s >>= 1) {
-    if (tid < s)
-#pragma unroll
-      for (int k = 0; k < K; ++k) sh[k][tid] += sh[k][tid + s];
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  const int G = gridDim.x;
-  if (tid == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      __hip_atomic_store(&part[k * G + blockIdx.x], sh[k][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned prev = atomicAdd(cnt, 1u);
-    last = prev == (unsigned)G - 1;
-  }
+  const double t = sh[0];
   __syncthreads();
-  if (!last) return false;
-  __threadfence();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-    for (int i = tid; i < G; i += kRT)
-      s += __hip_atomic_load(&part[k * G + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    sh[k][tid] = s;
-  }
+  return t;
+}
+
+// Total of the previous kernel's partials part[0..G).
+__device__ __forceinline__ double total_of(const double* __restrict__ part) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += kRT) s += part[i];
+  return wg_sum(s);
+}
+
+// part[blockIdx.x] = this workgroup's sum of v
+__device__ __forceinline__ void store_partial(double v, double* __restrict__ part) {
+  const double t = wg_sum(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// The solver flag, read once per workgroup (a kernel may see it set by its own
+// workgroup 0 only where every workgroup reaches the same decision anyway).
+__device__ __forceinline__ bool wg_done(const double* sc) {
+  __shared__ int f;
+  if (threadIdx.x == 0) f = sc[S_DONE] != 0.0;
   __syncthreads();
-  for (int s = kRT / 2; s > 0; s >>= 1) {
-    if (tid < s)
-#pragma unroll
-      for (int k = 0; k < K; ++k) sh[k][tid] += sh[k][tid + s];
-    __syncthreads();
-  }
-#pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = sh[k][0];
-  if (tid == 0) __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
+  const bool d = f != 0;
+  __syncthreads();
+  return d;
 }
 
 // Sum over the kLPR lanes of a row group.
@@ -310,153 +314,215 @@ __device__ __forceinline__ V group_sum(V a) {
   return a;
 }
 
-// sum_k val[k] * g(col[k]) over row i, kLPR lanes per row; the total is valid in
-// every lane of the group.
-template <class V, class Gather>
-__device__ __forceinline__ V row_product(const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
-                                         const V* __restrict__ val, int64_t i, int lane, Gather g) {
-  V acc = 0;
-  const int k1 = rp[i + 1];
-  for (int k = rp[i] + lane; k < k1; k += kLPR) acc += val[k] * g(col[k]);
-  return group_sum(acc);
+// Rows are taken kRPG at a time by each kLPR-lane group, with every load of the
+// kRPG rows (row starts, then column / value pairs, then the gathers) issued
+// before any is used: the SpMV is bound by memory latency, and this multiplies
+// the loads in flight.  body(i, y_i, lane) runs for each row i < n with
+// y_i = sum_k val[k] g(col[k]) valid in every lane of the group.
+template <class V, class Gather, class Body>
+__device__ __forceinline__ void for_rows(int64_t n, const int32_t* __restrict__ rp,
+                                         const int32_t* __restrict__ col, const V* __restrict__ val,
+                                         Gather g, Body body) {
+  const int lane = threadIdx.x % kLPR;
+  const int64_t step = (int64_t)gridDim.x * kRowsPerBlock * kRPG;
+  for (int64_t i0 = ((int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLPR) * kRPG; i0 < n; i0 += step) {
+    int rs[kRPG + 1];
+#pragma unroll
+    for (int j = 0; j <= kRPG; ++j) rs[j] = rp[i0 + j < n ? i0 + j : n];
+    int c[kRPG];
+    V v[kRPG];
+#pragma unroll
+    for (int j = 0; j < kRPG; ++j) {
+      const int k = rs[j] + lane;
+      c[j] = -1;
+      v[j] = 0;
+      if (k < rs[j + 1]) {
+        c[j] = col[k];
+        v[j] = val[k];
+      }
+    }
+    V acc[kRPG];
+#pragma unroll
+    for (int j = 0; j < kRPG; ++j) {
+      acc[j] = 0;
+      if (c[j] >= 0) acc[j] = v[j] * g(c[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kRPG; ++j)
+      for (int k = rs[j] + lane + kLPR; k < rs[j + 1]; k += kLPR) acc[j] += val[k] * g(col[k]);
+#pragma unroll
+    for (int j = 0; j < kRPG; ++j) {
+      const V y = group_sum(acc[j]);
+      if (i0 + j < n) body(i0 + j, y, lane);
+    }
+  }
 }
 
 // ----------------------------------------------------------------------------
 // CG on M (precision V for the matrix, vectors and arithmetic)
 // ----------------------------------------------------------------------------
-// start from d = 0 with residual r: z = D^-1 r, p = 0; scalars rz, |r|_A^2, tolerance
+// start from d = 0 with residual r: z = D^-1 r, p = 0; partials of r.z and |r|_A^2
 template <class V>
 __global__ __launch_bounds__(kRT) void cg_init(int64_t n, const V* __restrict__ r, const V* __restrict__ dinv,
                                                const V* __restrict__ wv, V* __restrict__ d, V* __restrict__ z,
-                                               V* __restrict__ p, double irt2, double tabs2, double* sc,
-                                               double* part, unsigned* cnt) {
-  double acc[2] = {0.0, 0.0};
+                                               V* __restrict__ p, double* sc, double* part) {
+  double rz = 0.0, wr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     const V ri = r[i];
     const V zi = dinv[i] * ri;
     z[i] = zi;
     d[i] = 0;
     p[i] = 0;
-    acc[0] += (double)ri * (double)zi;
+    rz += (double)ri * (double)zi;
     const double w = (double)ri * (double)wv[i];
-    acc[1] += w * w;
+    wr += w * w;
   }
-  if (grid_sum<2>(acc, part, cnt) && threadIdx.x == 0) {
-    const double tol2 = fmax(irt2 * acc[1], tabs2);
-    sc[S_RZ] = acc[0];
-    sc[S_BETA] = 0.0;
-    sc[S_WR] = acc[1];
-    sc[S_WR0] = acc[1];
-    sc[S_TOL2] = tol2;
-    sc[S_ITER] = 0.0;
+  store_partial(rz, part + PT_RZ * kMaxG);
+  store_partial(wr, part + PT_WR * kMaxG);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_DONE] = 0.0;
     sc[S_FLAG] = 0.0;
-    sc[S_DONE] = acc[1] <= tol2 ? 1.0 : 0.0;
+    sc[S_ITER] = 0.0;
   }
 }
 
-// p_new = z + beta p_old; q = M p_new (p_new of the neighbours formed on the fly
-// from z and p_old, so no separate direction update pass); alpha = rz / (p.q)
+// Iteration k, part 1: stop test on |r|_A^2 (the tolerance is fixed at k = 0 as
+// max(irt2 |r0|_A^2, tabs2)), beta = rz_k / rz_{k-1}; p = z + beta p_old with the
+// neighbours' p formed on the fly from z and p_old; q = M p; partials of p.q.
 template <class V>
-__global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, const int32_t* __restrict__ rp,
+__global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, double irt2, double tabs2,
+                                                   const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ col, const V* __restrict__ val,
                                                    const V* __restrict__ z, const V* __restrict__ pold,
                                                    V* __restrict__ pnew, V* __restrict__ q, double* sc,
-                                                   double* part, unsigned* cnt) {
-  if (sc[S_DONE] != 0.0) return;
-  const V beta = (V)sc[S_BETA];
-  const int lane = threadIdx.x % kLPR;
-  double acc[1] = {0.0};
-  for (int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLPR; i < n;
-       i += (int64_t)gridDim.x * kRowsPerBlock) {
-    const V qi = row_product(rp, col, val, i, lane, [&](int c) { return z[c] + beta * pold[c]; });
+                                                   double* part) {
+  const double rz = total_of(part + PT_RZ * kMaxG);
+  const double wr = total_of(part + PT_WR * kMaxG);
+  const double tol2 = k == 0 ? fmax(irt2 * wr, tabs2) : sc[S_TOL2];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!(wr > tol2)) {  // converged (or NaN): every workgroup decides the same
+    if (lead) {
+      sc[S_DONE] = 1.0;
+      sc[S_ITER] = (double)k;
+      sc[S_WR] = wr;
+    }
+    return;
+  }
+  const double beta = k == 0 ? 0.0 : rz / sc[S_RZ + ((k - 1) & 1)];
+  if (lead) {
+    sc[S_RZ + (k & 1)] = rz;
+    sc[S_WR] = wr;
+    if (k == 0) sc[S_TOL2] = tol2;
+  }
+  const V vb = (V)beta;
+  double pq = 0.0;
+  for_rows(n, rp, col, val, [&](int c) { return z[c] + vb * pold[c]; }, [&](int64_t i, V qi, int lane) {
     if (lane == 0) {
-      const V pi = z[i] + beta * pold[i];
+      const V pi = z[i] + vb * pold[i];
       pnew[i] = pi;
       q[i] = qi;
-      acc[0] += (double)pi * (double)qi;
+      pq += (double)pi * (double)qi;
     }
-  }
-  if (grid_sum<1>(acc, part, cnt) && threadIdx.x == 0) {
-    const double pq = acc[0];
-    if (pq > 0.0) {
-      sc[S_ALPHA] = sc[S_RZ] / pq;
-    } else {  // M is not positive definite on this direction: stop
-      sc[S_ALPHA] = 0.0;
-      sc[S_FLAG] = 2.0;
-      sc[S_DONE] = 1.0;
-    }
-  }
+  });
+  store_partial(pq, part + PT_PQ * kMaxG);
 }
 
-// d += alpha p; r -= alpha q; z = D^-1 r; beta = rz_new / rz; stop test on |r|_A
+// Iteration k, part 2: alpha = rz_k / (p.q); d += alpha p; r -= alpha q;
+// z = D^-1 r; partials of r.z and |r|_A^2.
 template <class V>
-__global__ __launch_bounds__(kRT) void cg_update(int64_t n, V* __restrict__ d, V* __restrict__ r,
+__global__ __launch_bounds__(kRT) void cg_update(int64_t n, int k, V* __restrict__ d, V* __restrict__ r,
                                                  const V* __restrict__ p, const V* __restrict__ q,
                                                  const V* __restrict__ dinv, const V* __restrict__ wv,
-                                                 V* __restrict__ z, double* sc, double* part, unsigned* cnt) {
-  if (sc[S_DONE] != 0.0) return;
-  const V alpha = (V)sc[S_ALPHA];
-  double acc[2] = {0.0, 0.0};
+                                                 V* __restrict__ z, double* sc, double* part) {
+  if (wg_done(sc)) return;
+  const double pq = total_of(part + PT_PQ * kMaxG);
+  if (!(pq > 0.0)) {  // M is not positive definite on this direction: stop
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc[S_FLAG] = 2.0;
+      sc[S_DONE] = 1.0;
+      sc[S_ITER] = (double)k;
+    }
+    return;
+  }
+  const V alpha = (V)(sc[S_RZ + (k & 1)] / pq);
+  double rz = 0.0, wr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     d[i] = d[i] + alpha * p[i];
     const V ri = r[i] - alpha * q[i];
     r[i] = ri;
     const V zi = dinv[i] * ri;
     z[i] = zi;
-    acc[0] += (double)ri * (double)zi;
+    rz += (double)ri * (double)zi;
     const double w = (double)ri * (double)wv[i];
-    acc[1] += w * w;
+    wr += w * w;
   }
-  if (grid_sum<2>(acc, part, cnt) && threadIdx.x == 0) {
-    const double rz = acc[0];
-    sc[S_BETA] = sc[S_RZ] != 0.0 ? rz / sc[S_RZ] : 0.0;
-    sc[S_RZ] = rz;
-    sc[S_WR] = acc[1];
-    sc[S_ITER] = sc[S_ITER] + 1.0;
-    if (!(acc[1] > sc[S_TOL2])) sc[S_DONE] = 1.0;  // also stops on NaN
-  }
+  store_partial(rz, part + PT_RZ * kMaxG);
+  store_partial(wr, part + PT_WR * kMaxG);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_ITER] = (double)(k + 1);
 }
 
 // ----------------------------------------------------------------------------
 // BiCGStab on M, FP64, Jacobi right preconditioning
 // ----------------------------------------------------------------------------
-// d = 0, rhat = r, p = v = 0; rho = rhat.r, alpha = omega = 1
+// d = 0, rhat = r, p = v = 0; partials of rhat.r and |r|_A^2
 __global__ __launch_bounds__(kRT) void bi_init(int64_t n, const double* __restrict__ r,
                                                const double* __restrict__ wv, double* __restrict__ d,
                                                double* __restrict__ rhat, double* __restrict__ p,
-                                               double* __restrict__ v, double tabs2, double* sc, double* part,
-                                               unsigned* cnt) {
-  double acc[2] = {0.0, 0.0};
+                                               double* __restrict__ v, double* sc, double* part) {
+  double rho = 0.0, wr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     const double ri = r[i];
     rhat[i] = ri;
     d[i] = 0.0;
     p[i] = 0.0;
     v[i] = 0.0;
-    acc[0] += ri * ri;
+    rho += ri * ri;
     const double w = ri * wv[i];
-    acc[1] += w * w;
+    wr += w * w;
   }
-  if (grid_sum<2>(acc, part, cnt) && threadIdx.x == 0) {
-    sc[S_RHO] = acc[0];
-    sc[S_BETA] = 0.0;
-    sc[S_ALPHA] = 1.0;
-    sc[S_OMEGA] = 1.0;
-    sc[S_WR] = acc[1];
-    sc[S_WR0] = acc[1];
-    sc[S_TOL2] = tabs2;
-    sc[S_ITER] = 0.0;
+  store_partial(rho, part + PT_RZ * kMaxG);
+  store_partial(wr, part + PT_WR * kMaxG);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_DONE] = 0.0;
     sc[S_FLAG] = 0.0;
-    sc[S_DONE] = acc[1] <= tabs2 ? 1.0 : 0.0;
+    sc[S_ITER] = 0.0;
   }
 }
 
-// p = r + beta (p - omega v); phat = D^-1 p
-__global__ __launch_bounds__(kRT) void bi_dir(int64_t n, const double* __restrict__ r, double* __restrict__ p,
-                                              const double* __restrict__ v, const double* __restrict__ dinv,
-                                              double* __restrict__ phat, const double* sc) {
-  if (sc[S_DONE] != 0.0) return;
-  const double beta = sc[S_BETA], omega = sc[S_OMEGA];
+// stop test; beta = (rho_k / rho_{k-1}) (alpha_{k-1} / omega_{k-1});
+// p = r + beta (p - omega_{k-1} v); phat = D^-1 p
+__global__ __launch_bounds__(kRT) void bi_dir(int64_t n, int k, double tabs2, const double* __restrict__ r,
+                                              double* __restrict__ p, const double* __restrict__ v,
+                                              const double* __restrict__ dinv, double* __restrict__ phat,
+                                              double* sc, const double* part) {
+  const double rho = total_of(part + PT_RZ * kMaxG);
+  const double wr = total_of(part + PT_WR * kMaxG);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  double beta = 0.0, omega = 0.0;
+  bool stop = !(wr > tabs2);
+  if (!stop && k > 0) {
+    const double rho0 = sc[S_RHO + ((k - 1) & 1)];
+    const double alpha0 = sc[S_ALPHA + ((k - 1) & 1)];
+    omega = sc[S_OMEGA + ((k - 1) & 1)];
+    if (rho == 0.0 || omega == 0.0 || rho0 == 0.0) {
+      stop = true;
+      if (lead) sc[S_FLAG] = 2.0;
+    } else {
+      beta = (rho / rho0) * (alpha0 / omega);
+    }
+  }
+  if (stop) {
+    if (lead) {
+      sc[S_DONE] = 1.0;
+      sc[S_ITER] = (double)k;
+      sc[S_WR] = wr;
+    }
+    return;
+  }
+  if (lead) {
+    sc[S_RHO + (k & 1)] = rho;
+    sc[S_WR] = wr;
+  }
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     const double pi = r[i] + beta * (p[i] - omega * v[i]);
     p[i] = pi;
@@ -464,45 +530,41 @@ __global__ __launch_bounds__(kRT) void bi_dir(int64_t n, const double* __restric
   }
 }
 
-// y = M x; MODE 0: alpha = rho / (rhat.y); MODE 1: omega = (y.s) / (y.y)
+// y = M x with partials of a.y (slot A) and, for MODE 1, of y.y (slot B)
 template <int MODE>
 __global__ __launch_bounds__(kRT) void bi_spmv(int64_t n, const int32_t* __restrict__ rp,
                                                const int32_t* __restrict__ col, const double* __restrict__ val,
                                                const double* __restrict__ x, double* __restrict__ y,
-                                               const double* __restrict__ a, double* sc, double* part,
-                                               unsigned* cnt) {
-  if (sc[S_DONE] != 0.0) return;
-  const int lane = threadIdx.x % kLPR;
-  double acc[2] = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLPR; i < n;
-       i += (int64_t)gridDim.x * kRowsPerBlock) {
-    const double yi = row_product(rp, col, val, i, lane, [&](int c) { return x[c]; });
+                                               const double* __restrict__ a, const double* sc, double* part) {
+  if (wg_done(sc)) return;
+  double ay = 0.0, yy = 0.0;
+  for_rows(n, rp, col, val, [&](int c) { return x[c]; }, [&](int64_t i, double yi, int lane) {
     if (lane == 0) {
       y[i] = yi;
-      acc[0] += a[i] * yi;
-      if (MODE == 1) acc[1] += yi * yi;
+      ay += a[i] * yi;
+      if (MODE == 1) yy += yi * yi;
     }
-  }
-  if (grid_sum<2>(acc, part, cnt) && threadIdx.x == 0) {
-    if (MODE == 0) {
-      if (acc[0] != 0.0 && acc[0] == acc[0]) {
-        sc[S_ALPHA] = sc[S_RHO] / acc[0];
-      } else {
-        sc[S_FLAG] = 2.0;
-        sc[S_DONE] = 1.0;
-      }
-    } else {
-      sc[S_OMEGA] = acc[1] != 0.0 ? acc[0] / acc[1] : 0.0;
-    }
-  }
+  });
+  store_partial(ay, part + PT_PQ * kMaxG);
+  if (MODE == 1) store_partial(yy, part + PT_TT * kMaxG);
 }
 
-// s = r - alpha v (into r); shat = D^-1 s
-__global__ __launch_bounds__(kRT) void bi_s(int64_t n, double* __restrict__ r, const double* __restrict__ v,
-                                            const double* __restrict__ dinv, double* __restrict__ shat,
-                                            const double* sc) {
-  if (sc[S_DONE] != 0.0) return;
-  const double alpha = sc[S_ALPHA];
+// alpha = rho_k / (rhat.v); s = r - alpha v (into r); shat = D^-1 s
+__global__ __launch_bounds__(kRT) void bi_s(int64_t n, int k, double* __restrict__ r,
+                                            const double* __restrict__ v, const double* __restrict__ dinv,
+                                            double* __restrict__ shat, double* sc, const double* part) {
+  if (wg_done(sc)) return;
+  const double rv = total_of(part + PT_PQ * kMaxG);
+  if (rv == 0.0 || rv != rv) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc[S_FLAG] = 2.0;
+      sc[S_DONE] = 1.0;
+      sc[S_ITER] = (double)k;
+    }
+    return;
+  }
+  const double alpha = sc[S_RHO + (k & 1)] / rv;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_ALPHA + (k & 1)] = alpha;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     const double si = r[i] - alpha * v[i];
     r[i] = si;
@@ -510,70 +572,70 @@ __global__ __launch_bounds__(kRT) void bi_s(int64_t n, double* __restrict__ r, c
   }
 }
 
-// d += alpha phat + omega shat; r = s - omega t; rho_new = rhat.r, beta, stop test
-__global__ __launch_bounds__(kRT) void bi_update(int64_t n, double* __restrict__ d, double* __restrict__ r,
+// omega = (t.s) / (t.t); d += alpha phat + omega shat; r = s - omega t;
+// partials of rhat.r and |r|_A^2
+__global__ __launch_bounds__(kRT) void bi_update(int64_t n, int k, double* __restrict__ d, double* __restrict__ r,
                                                  const double* __restrict__ phat,
                                                  const double* __restrict__ shat, const double* __restrict__ t,
                                                  const double* __restrict__ rhat, const double* __restrict__ wv,
-                                                 double* sc, double* part, unsigned* cnt) {
-  if (sc[S_DONE] != 0.0) return;
-  const double alpha = sc[S_ALPHA], omega = sc[S_OMEGA];
-  double acc[2] = {0.0, 0.0};
+                                                 double* sc, double* part) {
+  if (wg_done(sc)) return;
+  const double ts = total_of(part + PT_PQ * kMaxG);
+  const double tt = total_of(part + PT_TT * kMaxG);
+  const double omega = tt != 0.0 ? ts / tt : 0.0;
+  const double alpha = sc[S_ALPHA + (k & 1)];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_OMEGA + (k & 1)] = omega;
+    sc[S_ITER] = (double)(k + 1);
+  }
+  double rho = 0.0, wr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     d[i] = d[i] + alpha * phat[i] + omega * shat[i];
     const double ri = r[i] - omega * t[i];
     r[i] = ri;
-    acc[0] += rhat[i] * ri;
+    rho += rhat[i] * ri;
     const double w = ri * wv[i];
-    acc[1] += w * w;
+    wr += w * w;
   }
-  if (grid_sum<2>(acc, part, cnt) && threadIdx.x == 0) {
-    const double rho = acc[0];
-    sc[S_BETA] = (sc[S_RHO] != 0.0 && omega != 0.0) ? (rho / sc[S_RHO]) * (alpha / omega) : 0.0;
-    sc[S_RHO] = rho;
-    sc[S_WR] = acc[1];
-    sc[S_ITER] = sc[S_ITER] + 1.0;
-    if (!(acc[1] > sc[S_TOL2])) {
-      sc[S_DONE] = 1.0;
-    } else if (omega == 0.0 || rho == 0.0) {
-      sc[S_FLAG] = 2.0;
-      sc[S_DONE] = 1.0;
-    }
-  }
+  store_partial(rho, part + PT_RZ * kMaxG);
+  store_partial(wr, part + PT_WR * kMaxG);
 }
 
 // ----------------------------------------------------------------------------
 // residuals and small vector kernels
 // ----------------------------------------------------------------------------
 // ra = b - A x (FP64); r = -V ra on free rows, 0 on fixed rows (precision V);
-// sc[S_RES2] = |ra|^2
+// partials of |ra|^2
 template <class V>
 __global__ __launch_bounds__(kRT) void true_residual(int64_t n, const int32_t* __restrict__ rp,
                                                      const int32_t* __restrict__ col,
                                                      const double* __restrict__ val, const double* __restrict__ b,
                                                      const double* __restrict__ x, const uint8_t* __restrict__ fixed,
-                                                     const double* __restrict__ vol, V* __restrict__ r, double* sc,
-                                                     double* part, unsigned* cnt) {
-  const int lane = threadIdx.x % kLPR;
-  double acc[1] = {0.0};
-  for (int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLPR; i < n;
-       i += (int64_t)gridDim.x * kRowsPerBlock) {
-    const double ax = row_product(rp, col, val, i, lane, [&](int c) { return x[c]; });
+                                                     const double* __restrict__ vol, V* __restrict__ r, double* part) {
+  double acc = 0.0;
+  for_rows(n, rp, col, val, [&](int c) { return x[c]; }, [&](int64_t i, double ax, int lane) {
     if (lane == 0) {
       const double ra = b[i] - ax;
       if (r) r[i] = fixed[i] ? (V)0 : (V)(-vol[i] * ra);
-      acc[0] += ra * ra;
+      acc += ra * ra;
     }
-  }
-  if (grid_sum<1>(acc, part, cnt) && threadIdx.x == 0) sc[S_RES2] = acc[0];
+  });
+  store_partial(acc, part + PT_RES * kMaxG);
 }
 
-__global__ __launch_bounds__(kRT) void norm2_kernel(int64_t n, const double* __restrict__ x, double* sc,
-                                                    int slot, double* part, unsigned* cnt) {
-  double acc[1] = {0.0};
+__global__ __launch_bounds__(kRT) void norm2_kernel(int64_t n, const double* __restrict__ x, double* part) {
+  double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT)
-    acc[0] += x[i] * x[i];
-  if (grid_sum<1>(acc, part, cnt) && threadIdx.x == 0) sc[slot] = acc[0];
+    acc += x[i] * x[i];
+  store_partial(acc, part + PT_RES * kMaxG);
+}
+
+// sc[slot] = total of the G partials in slot PT_RES (one workgroup)
+__global__ __launch_bounds__(kRT) void finish_sum(int G, const double* __restrict__ part, double* sc, int slot) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < G; i += kRT) s += part[PT_RES * kMaxG + i];
+  const double t = wg_sum(s);
+  if (threadIdx.x == 0) sc[slot] = t;
 }
 
 // x = 0 (unless keep) with x_i = b_i on fixed rows
@@ -597,12 +659,9 @@ __global__ __launch_bounds__(kRT) void spmv_kernel(int64_t n, const int32_t* __r
                                                    const int32_t* __restrict__ col, const V* __restrict__ val,
                                                    const double* __restrict__ x, const double* __restrict__ b,
                                                    double* __restrict__ y) {
-  const int lane = threadIdx.x % kLPR;
-  for (int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLPR; i < n;
-       i += (int64_t)gridDim.x * kRowsPerBlock) {
-    const V yi = row_product(rp, col, val, i, lane, [&](int c) { return (V)x[c]; });
+  for_rows(n, rp, col, val, [&](int c) { return (V)x[c]; }, [&](int64_t i, V yi, int lane) {
     if (lane == 0) y[i] = b ? (double)yi + b[i] : (double)yi;
-  }
+  });
 }
 
 // ----------------------------------------------------------------------------
@@ -626,7 +685,7 @@ int dupload(T** dst, const T* src, int64_t n) {
 void free_tmpl(fvm_template* t) {
   if (!t) return;
   void* p[] = {t->row_ptr, t->col, t->val, t->valM, t->valM32, t->b, t->fixed, t->vol, t->dinv,
-               t->wv, t->dinv32, t->wv32, t->work, t->part, t->sc, t->cnt};
+               t->wv, t->dinv32, t->wv32, t->work, t->part, t->sc};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete t;
@@ -641,10 +700,12 @@ int read_scalars(const fvm_template* t, double* h, hipStream_t s) {
 
 inline dim3 grid(const fvm_template* t) { return dim3((unsigned)t->G); }
 
+// |b - A x|^2 into sc[S_RES2] (and r = -V (b - A x) on free rows when r is given)
 template <class V>
 int true_res(fvm_template* t, const double* x, V* r, hipStream_t s) {
   hipLaunchKernelGGL((true_residual<V>), grid(t), dim3(kRT), 0, s, t->n, t->row_ptr, t->col, t->val, t->b, x,
-                     t->fixed, t->vol, r, t->sc, t->part, t->cnt);
+                     t->fixed, t->vol, r, t->part);
+  hipLaunchKernelGGL(finish_sum, dim3(1), dim3(kRT), 0, s, t->G, t->part, t->sc, (int)S_RES2);
   FVM_HIP(hipGetLastError());
   return FVM_OK;
 }
@@ -655,25 +716,23 @@ template <class V>
 int inner_cg(fvm_template* t, const V* valM, const V* dinv, const V* wv, V* d, V* r, V* z, V* p0, V* p1, V* q,
              double irt2, double tabs2, int maxit, int check, hipStream_t s, int* iters, int* flag) {
   const int64_t n = t->n;
-  hipLaunchKernelGGL((cg_init<V>), grid(t), dim3(kRT), 0, s, n, r, dinv, wv, d, z, p0, irt2, tabs2, t->sc,
-                     t->part, t->cnt);
+  hipLaunchKernelGGL((cg_init<V>), grid(t), dim3(kRT), 0, s, n, r, dinv, wv, d, z, p0, t->sc, t->part);
   double h[S_COUNT];
-  int launched = 0;
+  int k = 0;
   V* pp[2] = {p0, p1};
   for (;;) {
-    const int batch = std::min(check, maxit - launched);
-    for (int k = 0; k < batch; ++k) {
-      V* pold = pp[launched & 1];
-      V* pnew = pp[(launched + 1) & 1];
-      hipLaunchKernelGGL((cg_dir_spmv<V>), grid(t), dim3(kRT), 0, s, n, t->row_ptr, t->col, valM, z, pold,
-                         pnew, q, t->sc, t->part, t->cnt);
-      hipLaunchKernelGGL((cg_update<V>), grid(t), dim3(kRT), 0, s, n, d, r, pnew, q, dinv, wv, z, t->sc,
-                         t->part, t->cnt);
-      ++launched;
+    const int batch = std::min(check, maxit - k);
+    for (int j = 0; j < batch; ++j, ++k) {
+      V* pold = pp[k & 1];
+      V* pnew = pp[(k + 1) & 1];
+      hipLaunchKernelGGL((cg_dir_spmv<V>), grid(t), dim3(kRT), 0, s, n, k, irt2, tabs2, t->row_ptr, t->col,
+                         valM, z, pold, pnew, q, t->sc, t->part);
+      hipLaunchKernelGGL((cg_update<V>), grid(t), dim3(kRT), 0, s, n, k, d, r, pnew, q, dinv, wv, z, t->sc,
+                         t->part);
     }
     int rc = read_scalars(t, h, s);
     if (rc) return rc;
-    if (h[S_DONE] != 0.0 || launched >= maxit) break;
+    if (h[S_DONE] != 0.0 || k >= maxit) break;
   }
   *iters = (int)h[S_ITER];
   *flag = (int)h[S_FLAG];
@@ -687,24 +746,21 @@ int inner_bicgstab(fvm_template* t, double* d, double* r, double tabs2, int maxi
   double *rhat = w + 3 * n, *p = w + 4 * n, *v = w + 5 * n, *phat = w + 6 * n, *shat = w + 7 * n,
          *tt = w + 8 * n;
   const dim3 g = grid(t), b(kRT);
-  hipLaunchKernelGGL(bi_init, g, b, 0, s, n, r, t->wv, d, rhat, p, v, tabs2, t->sc, t->part, t->cnt);
+  hipLaunchKernelGGL(bi_init, g, b, 0, s, n, r, t->wv, d, rhat, p, v, t->sc, t->part);
   double h[S_COUNT];
-  int launched = 0;
+  int k = 0;
   for (;;) {
-    const int batch = std::min(check, maxit - launched);
-    for (int k = 0; k < batch; ++k) {
-      hipLaunchKernelGGL(bi_dir, g, b, 0, s, n, r, p, v, t->dinv, phat, t->sc);
-      hipLaunchKernelGGL(bi_spmv<0>, g, b, 0, s, n, t->row_ptr, t->col, t->valM, phat, v, rhat, t->sc, t->part,
-                         t->cnt);
-      hipLaunchKernelGGL(bi_s, g, b, 0, s, n, r, v, t->dinv, shat, t->sc);
-      hipLaunchKernelGGL(bi_spmv<1>, g, b, 0, s, n, t->row_ptr, t->col, t->valM, shat, tt, r, t->sc, t->part,
-                         t->cnt);
-      hipLaunchKernelGGL(bi_update, g, b, 0, s, n, d, r, phat, shat, tt, rhat, t->wv, t->sc, t->part, t->cnt);
-      ++launched;
+    const int batch = std::min(check, maxit - k);
+    for (int j = 0; j < batch; ++j, ++k) {
+      hipLaunchKernelGGL(bi_dir, g, b, 0, s, n, k, tabs2, r, p, v, t->dinv, phat, t->sc, t->part);
+      hipLaunchKernelGGL(bi_spmv<0>, g, b, 0, s, n, t->row_ptr, t->col, t->valM, phat, v, rhat, t->sc, t->part);
+      hipLaunchKernelGGL(bi_s, g, b, 0, s, n, k, r, v, t->dinv, shat, t->sc, t->part);
+      hipLaunchKernelGGL(bi_spmv<1>, g, b, 0, s, n, t->row_ptr, t->col, t->valM, shat, tt, r, t->sc, t->part);
+      hipLaunchKernelGGL(bi_update, g, b, 0, s, n, k, d, r, phat, shat, tt, rhat, t->wv, t->sc, t->part);
     }
     int rc = read_scalars(t, h, s);
     if (rc) return rc;
-    if (h[S_DONE] != 0.0 || launched >= maxit) break;
+    if (h[S_DONE] != 0.0 || k >= maxit) break;
   }
   *iters = (int)h[S_ITER];
   *flag = (int)h[S_FLAG];
@@ -779,8 +835,8 @@ int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_templ
     if ((rc = dalloc(&t->dinv, N)) || (rc = dalloc(&t->wv, N)) || (rc = dalloc(&t->dinv32, N)) ||
         (rc = dalloc(&t->wv32, N)))
       break;
-    if ((rc = dalloc(&t->work, 10 * N)) || (rc = dalloc(&t->part, 4 * kMaxG)) ||
-        (rc = dalloc(&t->sc, S_COUNT)) || (rc = dalloc(&t->cnt, 1)))
+    if ((rc = dalloc(&t->work, 10 * N)) || (rc = dalloc(&t->part, PT_COUNT * kMaxG)) ||
+        (rc = dalloc(&t->sc, S_COUNT)))
       break;
     if ((rc = dupload(&d_nbs, nbs.data(), N + 1)) || (rc = dupload(&d_nb, nb.data(), 2 * B))) break;
     if ((rc = dupload(&d_kind, d->node_kind, N)) || (rc = dupload(&d_neu, neu.data(), B))) break;
@@ -788,8 +844,7 @@ int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_templ
     if ((rc = opt(&d_f, d->f_node, N)) || (rc = opt(&d_r, d->r_node, N)) || (rc = opt(&d_dir, d->dir_val, N)) ||
         (rc = opt(&d_dudt, d->dudt_val, N)) || (rc = opt(&d_na, d->neu_a, 2 * B)))
       break;
-    hipError_t e = hipMemset(t->cnt, 0, sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemset(t->sc, 0, sizeof(double) * S_COUNT);
+    hipError_t e = hipMemset(t->sc, 0, sizeof(double) * S_COUNT);
     if (e == hipSuccess) e = hipMemcpy(t->vol, m->cv, sizeof(double) * N, hipMemcpyDeviceToDevice);
     if (e != hipSuccess) {
       rc = hip_fail(e, "fvm_template_create: initialise");
@@ -830,7 +885,8 @@ int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_templ
     hipLaunchKernelGGL(assemble_kernel, g, dim3(256), 0, 0, a);
     hipLaunchKernelGGL(scaled_kernel, g, dim3(256), 0, 0, N, t->row_ptr, t->col, t->val, t->fixed, t->vol,
                        t->valM, t->valM32, t->dinv, t->dinv32, t->wv, t->wv32);
-    hipLaunchKernelGGL(norm2_kernel, grid(t), dim3(kRT), 0, 0, N, t->b, t->sc, (int)S_B2, t->part, t->cnt);
+    hipLaunchKernelGGL(norm2_kernel, grid(t), dim3(kRT), 0, 0, N, t->b, t->part);
+    hipLaunchKernelGGL(finish_sum, dim3(1), dim3(kRT), 0, 0, t->G, t->part, t->sc, (int)S_B2);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) rc = hip_fail(e, "fvm_template_create: assembly");
