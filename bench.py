@@ -222,6 +222,16 @@ def ensemble_bench(P=256, steps=1000, graph_steps=100):
     tm.start()
     b0.euler(d_u, steps, 0.0)
     ms_nograph = tm.stop()
+    # the update as a separate kernel after the RHS (graph replay as above)
+    bu = EN.DeviceBatch(probs, dts, tile_triangles=b.info.tile_triangles, graph_steps=graph_steps, fused=False)
+    d_u.upload(u0)
+    bu.euler(d_u, graph_steps, 0.0)
+    d_u.upload(u0)
+    _lib.synchronize()
+    tm.start()
+    bu.euler(d_u, steps, 0.0)
+    ms_unfused = tm.stop()
+    del bu
     # looping the single-problem path over the members
     singles = [EN.DeviceBatch([q], [dt], graph_steps=0) for q, dt in zip(probs, dts)]
     us = [_lib.DeviceArray.from_host(q.initial_condition) for q in probs]
@@ -245,10 +255,12 @@ def ensemble_bench(P=256, steps=1000, graph_steps=100):
                             zip(list(b.info.candidate_tiles)[:b.info.n_candidates],
                                 list(b.info.candidate_ms)[:b.info.n_candidates])},
                batched_graph_ms=round(ms_batch, 3), batched_nograph_ms=round(ms_nograph, 3),
+               batched_unfused_update_ms=round(ms_unfused, 3),
                looped_ms=round(ms_loop, 3), speedup_vs_loop=round(ms_loop / ms_batch, 2),
                algorithmic_GBps=round(per_step * steps / (ms_batch * 1e-3) / 1e9, 1),
                max_rel_diff_vs_loop=err)
-    log(f"ensemble: batched {ms_batch:.1f} ms (no graph {ms_nograph:.1f} ms), looped {ms_loop:.1f} ms, "
+    log(f"ensemble: batched {ms_batch:.1f} ms (no graph {ms_nograph:.1f} ms, unfused update {ms_unfused:.1f} ms), "
+        f"looped {ms_loop:.1f} ms, "
         f"diff {err:.2e}")
     return out
 

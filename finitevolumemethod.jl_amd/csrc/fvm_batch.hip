@@ -7,8 +7,9 @@
 // and one context that carries every member's closures, so one launch of the
 // tile and fix-up kernels evaluates the right-hand side of every member.  An
 // explicit Euler step (u += dt_p du, then the Dirichlet callback of solve.jl:
-// 133-165 at the member's new time) is one more kernel, and runs of steps are
-// captured once in a HIP graph and replayed.  Members keep their own time step
+// 133-165 at the member's new time) is fused into those two kernels (each tile
+// updates the nodes only it touches, the fix-up the shared ones), and runs of
+// steps are captured once in a HIP graph and replayed.  Members keep their own time step
 // dt_p; closures that depend on t then require a common dt.
 //
 // An autotuner chooses the tile shape (triangles per tile, which fixes the
@@ -29,6 +30,8 @@ struct fvm_batch {
   std::vector<int64_t> node_off;  // [P+1]
   std::vector<double> dt;         // [P]
   double* dtv = nullptr;          // [N] time step of each node's member
+  double* tile_dt = nullptr;      // [n_tiles] time step of each tile's member
+  std::vector<int64_t> tri_off;   // [P+1]
   double* du = nullptr;           // [N] scratch
   int8_t* code = nullptr;         // [N] node codes (0 free, 1 Dirichlet, 2 Dudt, 3 not a vertex)
   int32_t* fidx = nullptr;        // [N]
@@ -38,6 +41,7 @@ struct fvm_batch {
   bool time_dependent = false;
   int32_t tile_tris = 0;
   int32_t graph_steps = 0;
+  bool fused = true;               // Euler update fused into the RHS kernels
   hipStream_t cap = nullptr;
   hipGraphExec_t exec = nullptr;
   const double* exec_u = nullptr;  // the state buffer the graph was captured for
@@ -81,7 +85,7 @@ void free_batch(fvm_batch* b) {
   if (b->cap) (void)hipStreamDestroy(b->cap);
   if (b->ctx) fvm_ctx_destroy(b->ctx);
   if (b->mesh) fvm_mesh_destroy(b->mesh);
-  void* p[] = {b->dtv, b->du, b->code, b->fidx, b->fn, b->node_g};
+  void* p[] = {b->dtv, b->tile_dt, b->du, b->code, b->fidx, b->fn, b->node_g};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete b;
@@ -125,11 +129,13 @@ int pack(int32_t P, const fvm_mesh_desc* meshes, const fvm_physics_desc* ph, fvm
       FVM_CHECK(q.d_edge && (m.n_boundary_edges == 0 || q.d_bedge), "fvm_batch_create: member d_edge missing");
     if (fk == FVM_FLUX_D_NODAL) FVM_CHECK(q.log_d, "fvm_batch_create: member log_d missing");
     N += m.n_points;
+    b->tri_off.push_back(T);
     T += m.n_triangles;
     B += m.n_boundary_edges;
     F += q.n_functions;
     b->node_off[p + 1] = N;
   }
+  b->tri_off.push_back(T);
   FVM_CHECK(N < INT32_MAX && T < ((int64_t)1 << 29) && F < INT32_MAX, "fvm_batch_create: ensemble too large");
   k.x.reserve(N);
   k.y.reserve(N);
@@ -272,6 +278,7 @@ int time_rhs(fvm_ctx* c, const double* u, double* du, int reps, float* ms) {
 }
 
 int launch_step(fvm_batch* b, double* u, double t, hipStream_t s) {
+  if (b->fused) return euler_step(b->ctx, u, b->tile_dt, b->dtv, t, s);
   int rc = fvm_rhs(b->ctx, u, b->du, t, s);
   if (rc) return rc;
   const int64_t N = b->mesh->N;
@@ -404,6 +411,18 @@ int fvm_batch_create(int32_t n_problems, const fvm_mesh_desc* meshes, const fvm_
   if (!b->dt.empty())
     for (int32_t p = 0; p < n_problems; ++p)
       std::fill(dtv.begin() + b->node_off[p], dtv.begin() + b->node_off[p + 1], b->dt[p]);
+  // each tile lies inside one member (tiles are cut at member boundaries)
+  std::vector<double> tdt(b->mesh->n_tiles, 0.0);
+  if (!b->dt.empty()) {
+    std::vector<int32_t> tts(b->mesh->n_tiles + 1);
+    FVM_HIP(hipMemcpy(tts.data(), b->mesh->tile_tri_start, sizeof(int32_t) * tts.size(), hipMemcpyDeviceToHost));
+    for (int32_t tl = 0; tl < b->mesh->n_tiles; ++tl) {
+      const int64_t tin = b->mesh->h_perm[tts[tl]];
+      const int64_t p = std::upper_bound(b->tri_off.begin(), b->tri_off.end(), tin) - b->tri_off.begin() - 1;
+      tdt[tl] = b->dt[p];
+    }
+  }
+  b->fused = !(opts && opts->unfused == 1);
   std::vector<int8_t> code(N);
   FVM_HIP(hipMemcpy(code.data(), b->ctx->kind, sizeof(int8_t) * N, hipMemcpyDeviceToHost));
   b->n_fn = k.pd.n_functions;
@@ -416,6 +435,7 @@ int fvm_batch_create(int32_t n_problems, const fvm_mesh_desc* meshes, const fvm_
     fn[4 * b->n_fn + f] = k.cy[f];
   }
   if ((rc = dcopy(&b->dtv, dtv.data(), N)) || (rc = dcopy(&b->du, (const double*)nullptr, N)) ||
+      (rc = dcopy(&b->tile_dt, tdt.data(), (int64_t)tdt.size())) ||
       (rc = dcopy(&b->code, code.data(), N)) || (rc = dcopy(&b->fidx, k.node_fidx.data(), N)) ||
       (rc = dcopy(&b->fn, fn.data(), (int64_t)fn.size())) ||
       (k.any_node_g && (rc = dcopy(&b->node_g, k.node_g.data(), N)))) {

@@ -15,6 +15,11 @@ void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 // fvm_mesh_create with the triangles of each tri_group kept contiguous (NULL: one group)
 int mesh_create_grouped(const fvm_mesh_desc* d, const int32_t* tri_group, fvm_mesh** out);
+// One forward-Euler step of fvm_eqs! in place, fused into the RHS kernels:
+// u += dt du (dt per tile, and per node for the shared nodes), then the Dirichlet
+// callback at t + dt.  Exclusive nodes are read and written by one tile only, and
+// shared nodes only by the fix-up after every tile has read them.
+int euler_step(fvm_ctx* c, double* d_u, const double* tile_dt, const double* dtv, double t, hipStream_t s);
 
 #define FVM_HIP(call)                                        \
   do {                                                       \

@@ -26,7 +26,8 @@ using namespace fvm;
 
 namespace {
 
-enum { M_RHS = 0, M_JVP = 1 };
+// RHS, Jacobian-vector product, or one forward-Euler step of the ensemble path
+enum { M_RHS = 0, M_JVP = 1, M_EULER = 2 };
 
 struct KArgs {
   int32_t n_tiles, n_fix;
@@ -72,6 +73,8 @@ struct KArgs {
   const double* lcv;     // per local entry: control volume
   double* bflux;         // boundary terms per work item
   double t;
+  const double* tile_dt;  // M_EULER: time step of each tile's nodes
+  const double* dtv;      // M_EULER: time step per node (fix-up nodes)
 };
 
 // Fields of the triangle record the tile kernels stream: s1..s6, normals and
@@ -119,24 +122,6 @@ __device__ __forceinline__ double source_du(const KArgs& a, double u) {
   return a.s1 + u * (2.0 * a.s2 + 3.0 * a.s3 * u);
 }
 
-// fvm_eqs_single_source_contribution! (source_contributions.jl:33-47) for the RHS
-// and its derivative for the JVP.
-template <int MODE>
-__device__ __forceinline__ double finish(const KArgs& a, int g, double acc, double ug, double vg) {
-  const int c = a.code[g];
-  if (c == 0) {
-    if (MODE == M_RHS) return acc / a.cv[g] + source(a, g, ug);
-    return acc / a.cv[g] + source_du(a, ug) * vg;
-  }
-  if (c == 2) {
-    const int f = a.fidx[g];
-    if (MODE == M_RHS)
-      return eval_cond(a.C, f, a.x[g], a.y[g], a.t, ug, a.C.node_g ? a.C.node_g[g] : 0.0);
-    return a.C.cu[f] * vg;
-  }
-  return 0.0;
-}
-
 // The three CV-edge fluxes Q_e = (q . n) l of one triangle (get_fluxes,
 // triangle_contributions.jl:2-7), or their directional derivatives.
 template <int FK, int MODE>
@@ -172,7 +157,7 @@ __device__ __forceinline__ void tri_fluxes(const KArgs& a, const double* P, doub
     } else {
       D = Ds[e];
     }
-    if (MODE == M_RHS) {
+    if (MODE != M_JVP) {
       const double qx = -D * al, qy = -D * be;
       const double qn = qx * nx + qy * ny;
       Q[e] = qn * l;
@@ -244,7 +229,7 @@ __device__ double bedge_flux(const KArgs& a, int e, int ep) {
     } else {
       D = a.d_bedge[2 * e + ep];
     }
-    if (MODE == M_RHS) {
+    if (MODE != M_JVP) {
       const double qx = -D * al, qy = -D * be;
       qn = qx * G.nx + qy * G.ny;
     } else {
@@ -252,7 +237,7 @@ __device__ double bedge_flux(const KArgs& a, int e, int ep) {
       qn = qx * G.nx + qy * G.ny;
     }
   } else {
-    if (MODE == M_RHS)
+    if (MODE != M_JVP)
       qn = eval_cond(a.C, nf, mx, my, a.t, al * mx + be * my + ga,
                      a.C.bedge_g ? a.C.bedge_g[2 * e + ep] : 0.0);
     else
@@ -268,19 +253,30 @@ __device__ double bedge_flux(const KArgs& a, int e, int ep) {
 // thread, at most 1024 (local ids are packed in 10 bits).
 __host__ __device__ constexpr int node_cap(int block) { return 2 * block < 1024 ? 2 * block : 1024; }
 
-// finish() with the node's code and control volume already in registers.
+// fvm_eqs_single_source_contribution! (source_contributions.jl:33-47) for the RHS,
+// its derivative for the JVP, or (M_EULER) the new state u + dt du followed by the
+// Dirichlet callback (dirichlet.jl:78-86) at t + dt; the node's code and control
+// volume are already in registers.
 template <int MODE>
 __device__ __forceinline__ double finish_pre(const KArgs& a, int g, double acc, double ug, double vg,
-                                             int c, double V) {
+                                             int c, double V, double dt) {
   if (c == 0) {
     if (MODE == M_RHS) return acc / V + source(a, g, ug);
+    if (MODE == M_EULER) return ug + dt * (acc / V + source(a, g, ug));
     return acc / V + source_du(a, ug) * vg;
   }
   if (c == 2) {
     const int f = a.fidx[g];
     if (MODE == M_RHS)
       return eval_cond(a.C, f, a.x[g], a.y[g], a.t, ug, a.C.node_g ? a.C.node_g[g] : 0.0);
+    if (MODE == M_EULER)
+      return ug + dt * eval_cond(a.C, f, a.x[g], a.y[g], a.t, ug, a.C.node_g ? a.C.node_g[g] : 0.0);
     return a.C.cu[f] * vg;
+  }
+  if (MODE == M_EULER) {
+    if (c == 1)
+      return eval_cond(a.C, a.fidx[g], a.x[g], a.y[g], a.t + dt, ug, a.C.node_g ? a.C.node_g[g] : 0.0);
+    return ug;
   }
   return 0.0;
 }
@@ -320,6 +316,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
   const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
   const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
   const int ib0 = 3 * t0;  // a tile's incidence block is [3 t0, 3 (t0 + nt))
+  const double dt = MODE == M_EULER ? a.tile_dt[tile] : 0.0;
 
   int g[NS], pi[NS], is[NS];
 #pragma unroll
@@ -435,7 +432,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
       if (pi[r] >= 0)
         a.partial[pi[r]] = acc;
       else
-        a.out[g[r]] = finish_pre<MODE>(a, g[r], acc, ug[r], vg[r], cg[r], Vg[r]);
+        a.out[g[r]] = finish_pre<MODE>(a, g[r], acc, ug[r], vg[r], cg[r], Vg[r], dt);
     }
   }
 }
@@ -463,7 +460,7 @@ __global__ __launch_bounds__(256) void fix_kernel(KArgs a) {
     if (p0 + k < p1) acc = acc + pv[k];
   for (int p = p0 + 4; p < p1; ++p) acc = acc + a.partial[p];
   for (int k = b0; k < b1; ++k) acc = acc - a.bflux[k];
-  a.out[g] = finish_pre<MODE>(a, g, acc, ug, vg, c, V);
+  a.out[g] = finish_pre<MODE>(a, g, acc, ug, vg, c, V, MODE == M_EULER ? a.dtv[g] : 0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -760,6 +757,7 @@ KArgs make_args(const fvm_ctx* c) {
   a.bflux = c->bflux;
   a.n_bitems = m->n_bedge_items;
   a.t = 0;
+  a.tile_dt = a.dtv = nullptr;
   return a;
 }
 
@@ -927,6 +925,20 @@ int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
 }
 
 }  // namespace
+
+namespace fvm {
+
+int euler_step(fvm_ctx* c, double* d_u, const double* tile_dt, const double* dtv, double t, hipStream_t s) {
+  KArgs a = make_args(c);
+  a.u = d_u;
+  a.out = d_u;  // in place: a tile writes only its exclusive nodes, the fix-up the shared ones
+  a.t = t;
+  a.tile_dt = tile_dt;
+  a.dtv = dtv;
+  return launch<M_EULER>(c, a, s);
+}
+
+}  // namespace fvm
 
 extern "C" {
 

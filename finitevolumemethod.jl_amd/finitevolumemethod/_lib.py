@@ -83,7 +83,7 @@ class SolveStats(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("tile_triangles", ctypes.c_int32), ("autotune", ctypes.c_int32),
-                ("graph_steps", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("graph_steps", ctypes.c_int32), ("unfused", ctypes.c_int32)]
 
 
 class BatchInfo(ctypes.Structure):

@@ -126,7 +126,7 @@ def _uniform_form(ph, tri):
 class DeviceBatch:
     """An fvm_batch: the members packed on the device."""
 
-    def __init__(self, problems, dt=None, tile_triangles=0, autotune=False, graph_steps=0):
+    def __init__(self, problems, dt=None, tile_triangles=0, autotune=False, graph_steps=0, fused=True):
         P = len(problems)
         meshes = (_lib.MeshDesc * P)()
         phys = (_lib.PhysicsDesc * P)()
@@ -140,7 +140,7 @@ class DeviceBatch:
             d, keep = physics_desc(_uniform_form(physics_of(prob), tri))
             phys[k] = d
             self._keep.append(keep)
-        o = _lib.BatchOpts(int(tile_triangles), 1 if autotune else 0, int(graph_steps), 0)
+        o = _lib.BatchOpts(int(tile_triangles), 1 if autotune else 0, int(graph_steps), 0 if fused else 1)
         dts = None if dt is None else np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float64), (P,)))
         h = ctypes.c_void_p()
         _lib.check(_lib.hip_lib().fvm_batch_create(P, meshes, phys, _lib.ptr(dts) if dts is not None else None,

@@ -242,7 +242,8 @@ typedef struct fvm_batch_opts {
                              tiles on the packed RHS and keep the fastest */
   int32_t graph_steps;    /* > 0: fvm_batch_euler replays a HIP graph of this many steps
                              (closures independent of t only) */
-  int32_t reserved;
+  int32_t unfused;        /* 1: Euler steps as RHS + a separate update kernel (for comparison);
+                             0: the update fused into the RHS kernels */
 } fvm_batch_opts;
 
 typedef struct fvm_batch_info {

@@ -86,6 +86,20 @@ def test_graph_replay_and_direct_steps_agree_bitwise(gpu):
     assert np.array_equal(outs[0], outs[1])
 
 
+def test_fused_step_equals_rhs_plus_update(gpu):
+    """The Euler update fused into the RHS kernels gives the same bits as the RHS
+    followed by a separate update kernel."""
+    probs = members(P=5, seed=13)
+    dts = [0.5 * EN.explicit_dt(p) for p in probs]
+    outs = []
+    for fused in (True, False):
+        b = EN.DeviceBatch(probs, dts, fused=fused)
+        d_u = FVM.DeviceArray.from_host(b.pack([p.initial_condition for p in probs]))
+        b.euler(d_u, 9, 0.0)
+        outs.append(d_u.to_host())
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_autotuner_reports_candidates(gpu):
     probs = members(P=3, seed=2)
     b = EN.DeviceBatch(probs, autotune=True)
