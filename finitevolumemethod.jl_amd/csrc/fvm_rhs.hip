@@ -77,15 +77,24 @@ struct KArgs {
   const double* dtv;      // M_EULER: time step per node (fix-up nodes)
 };
 
-// Fields of the triangle record the tile kernels stream: s1..s6, normals and
-// lengths.  FVM_FLUX_D_EDGE needs neither gamma (s7..s9) nor the CV-edge
-// midpoints; the other forms rebuild both from the vertex coordinates
-// (recompute_geometry), which is cheaper than streaming them.
-__host__ __device__ constexpr bool loads_field(int fk, int f) { return f < 6 || f >= P_NX; }
+// How much of the triangle record the tile kernels rebuild from the vertex
+// coordinates instead of streaming it (the rebuilt values equal the stored ones
+// bit for bit: same operations as the geometry kernel):
+//   1: s7..s9 and the CV-edge midpoints (only the forms that need them;
+//      FVM_FLUX_D_EDGE reads neither);
+//   2: also s1..s6;
+//   3: the whole record (also the normals and lengths).
+#ifndef FVM_GEO_LEVEL
+#define FVM_GEO_LEVEL 3
+#endif
+constexpr int kGeo = FVM_GEO_LEVEL;
+__host__ __device__ constexpr bool loads_field(int fk, int f) {
+  return kGeo >= 3 ? false : (kGeo == 2 ? f >= P_NX : (f < 6 || f >= P_NX));
+}
+__host__ __device__ constexpr bool rebuilds(int fk) { return kGeo >= 2 || fk != FVM_FLUX_D_EDGE; }
 
-// s7..s9 and the three CV-edge midpoints of triangle (p, q, r), with exactly the
-// operations of the geometry kernel (geometry.jl:114-151), so the values equal the
-// stored ones bit for bit.
+// The rebuilt fields of triangle (p, q, r), with exactly the operations of the
+// geometry kernel (geometry.jl:114-161).
 __device__ __forceinline__ void recompute_geometry(double px, double py, double qx, double qy,
                                                    double rx, double ry, double* P) {
   const double cx = (px + qx + rx) / 3, cy = (py + qy + ry) / 3;
@@ -93,6 +102,14 @@ __device__ __forceinline__ void recompute_geometry(double px, double py, double 
   const double m2x = (qx + rx) / 2, m2y = (qy + ry) / 2;
   const double m3x = (rx + px) / 2, m3y = (ry + py) / 2;
   const double D = qx * ry - qy * rx - px * ry + rx * py + px * qy - qx * py;
+  if (kGeo >= 2) {
+    P[0] = (qy - ry) / D;
+    P[1] = (ry - py) / D;
+    P[2] = (py - qy) / D;
+    P[3] = (rx - qx) / D;
+    P[4] = (px - rx) / D;
+    P[5] = (qx - px) / D;
+  }
   P[6] = (qx * ry - rx * qy) / D;
   P[7] = (rx * py - px * ry) / D;
   P[8] = (px * qy - qx * py) / D;
@@ -102,6 +119,23 @@ __device__ __forceinline__ void recompute_geometry(double px, double py, double 
   P[12] = (m2y + cy) / 2;
   P[13] = (m3x + cx) / 2;
   P[14] = (m3y + cy) / 2;
+  if (kGeo >= 3) {
+    const double e1x = cx - m1x, e1y = cy - m1y;
+    const double e2x = cx - m2x, e2y = cy - m2y;
+    const double e3x = cx - m3x, e3y = cy - m3y;
+    const double l1 = sqrt(e1x * e1x + e1y * e1y);
+    const double l2 = sqrt(e2x * e2x + e2y * e2y);
+    const double l3 = sqrt(e3x * e3x + e3y * e3y);
+    P[15] = e1y / l1;
+    P[16] = -e1x / l1;
+    P[17] = e2y / l2;
+    P[18] = -e2x / l2;
+    P[19] = e3y / l3;
+    P[20] = -e3x / l3;
+    P[21] = l1;
+    P[22] = l2;
+    P[23] = l3;
+  }
 }
 
 __device__ __forceinline__ double dpoly(const KArgs& a, double u) {
@@ -293,7 +327,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
   __shared__ double s_u[MAXN];
   __shared__ double s_v[MODE == M_JVP ? MAXN : 1];
   __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
-  constexpr bool RC = FK != FVM_FLUX_D_EDGE;  // s7..s9 and midpoints rebuilt from coordinates
+  constexpr bool RC = rebuilds(FK);  // part of the triangle record rebuilt from coordinates
   __shared__ double s_x[RC ? MAXN : 1];
   __shared__ double s_y[RC ? MAXN : 1];
   __shared__ double s_q[3 * TILE];
@@ -552,7 +586,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
   __shared__ double s_u[MAXN];
   __shared__ double s_w[MAXN];
   __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
-  constexpr bool RC = FK != FVM_FLUX_D_EDGE;  // s7..s9 and midpoints rebuilt from coordinates
+  constexpr bool RC = rebuilds(FK);  // part of the triangle record rebuilt from coordinates
   __shared__ double s_x[RC ? MAXN : 1];
   __shared__ double s_y[RC ? MAXN : 1];
   __shared__ double s_c[CW * TILE];
