@@ -348,11 +348,12 @@ def main():
     ms_tiles, ms_fix = float(np.mean(tiles)), float(np.mean(fix))
     alg = canonical_bytes(N, T, B)
     achieved = alg / ((ms_tiles + ms_fix) * 1e-3)
-    traffic = None  # HBM bytes per RHS from the committed rocprofv3 counter runs
+    traffic = None  # HBM bytes per RHS from the committed rocprofv3 counter runs of this build
     tpath = os.path.join(ROOT, "profiles", "fvm_rhs_4M_traffic.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
-        if tr.get("nodes") == N and tr.get("tiles") == int(mesh.info.n_tiles):
+        if (tr.get("nodes") == N and tr.get("tiles") == int(mesh.info.n_tiles)
+                and tr.get("build") == _lib.build_info()):
             traffic = int(tr["hbm_bytes_per_rhs"])
     du_gpu = d_du.to_host()
     log(f"RHS {ms_step * 1e3:.1f} us/step ({value:.3f} Gnode/s); tiles {ms_tiles * 1e3:.1f} us, "
@@ -369,6 +370,9 @@ def main():
                    "tiles": int(mesh.info.n_tiles), "fix_nodes": int(mesh.info.n_fix_nodes)},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "measured_frac": (round(traffic / ((ms_tiles + ms_fix) * 1e-3) / HBM_PEAK, 4)
+                                       if traffic else None),
+                     "build": _lib.build_info(),
                      "kernels": {"tile_kernel_ms": round(ms_tiles, 5),
                                  "fix_kernel_ms": round(ms_fix, 5)},
                      "algorithmic_bytes_per_rhs": int(alg)},

@@ -9,6 +9,21 @@
 #include "../../include/fvm_hip.h"
 #include "fvm_host.h"
 
+// Build-time tuning (fvm_build_info reports the values in use):
+//   FVM_GEO_LEVEL       how much of each triangle record the tile kernels rebuild from
+//                       the vertex coordinates instead of streaming (1, 2 or 3 = all)
+//   FVM_SOLVER_THREADS  workgroup size of the template solver kernels
+//   FVM_SPMV_ROWS       CSR rows per lane group and step in the solver SpMV
+#ifndef FVM_GEO_LEVEL
+#define FVM_GEO_LEVEL 3
+#endif
+#ifndef FVM_SOLVER_THREADS
+#define FVM_SOLVER_THREADS 512
+#endif
+#ifndef FVM_SPMV_ROWS
+#define FVM_SPMV_ROWS 2
+#endif
+
 namespace fvm {
 
 void set_error(const std::string& msg);

@@ -37,12 +37,6 @@ using namespace fvm;
 
 namespace {
 
-#ifndef FVM_SOLVER_THREADS
-#define FVM_SOLVER_THREADS 512
-#endif
-#ifndef FVM_SPMV_ROWS
-#define FVM_SPMV_ROWS 2
-#endif
 constexpr int kRT = FVM_SOLVER_THREADS;  // threads per workgroup of every solver kernel
 constexpr int kMaxG = 1024;              // workgroups of the solver kernels (fixed per template)
 constexpr int kLPR = 8;                  // lanes per CSR row in the SpMV kernels

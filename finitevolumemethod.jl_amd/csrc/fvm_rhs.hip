@@ -84,9 +84,6 @@ struct KArgs {
 //      FVM_FLUX_D_EDGE reads neither);
 //   2: also s1..s6;
 //   3: the whole record (also the normals and lengths).
-#ifndef FVM_GEO_LEVEL
-#define FVM_GEO_LEVEL 3
-#endif
 constexpr int kGeo = FVM_GEO_LEVEL;
 __host__ __device__ constexpr bool loads_field(int fk, int f) {
   return kGeo >= 3 ? false : (kGeo == 2 ? f >= P_NX : (f < 6 || f >= P_NX));

@@ -1,4 +1,5 @@
 // Runtime part of the C ABI: errors, device memory, streams, events.
+#include <cstdio>
 #include <cstring>
 
 #include "fvm_internal.hpp"
@@ -23,6 +24,13 @@ int fvm_last_error(char* buf, size_t n) {
   size_t k = s.size() < n - 1 ? s.size() : n - 1;
   std::memcpy(buf, s.data(), k);
   buf[k] = 0;
+  return FVM_OK;
+}
+
+int fvm_build_info(char* buf, size_t n) {
+  FVM_CHECK(buf && n > 0, "fvm_build_info: null buffer");
+  std::snprintf(buf, n, "gfx950 geo_level=%d solver_threads=%d spmv_rows=%d", FVM_GEO_LEVEL,
+                FVM_SOLVER_THREADS, FVM_SPMV_ROWS);
   return FVM_OK;
 }
 

@@ -140,6 +140,7 @@ def host_lib():
 # (name, argtypes) of every entry point of include/fvm_hip.h
 HIP_API = {
     "fvm_last_error": [ctypes.c_char_p, ctypes.c_size_t],
+    "fvm_build_info": [ctypes.c_char_p, ctypes.c_size_t],
     "fvm_device_count": [_i32p],
     "fvm_device_synchronize": [],
     "fvm_alloc": [ctypes.POINTER(_vp), ctypes.c_size_t],
@@ -206,6 +207,13 @@ def hip_lib():
                 f.restype = ctypes.c_int
             _hip = L
     return _hip
+
+
+def build_info():
+    """The library's build configuration string (fvm_build_info)."""
+    buf = ctypes.create_string_buffer(256)
+    check(hip_lib().fvm_build_info(buf, 256), "fvm_build_info")
+    return buf.value.decode()
 
 
 def last_error():

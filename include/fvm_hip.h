@@ -37,6 +37,9 @@ typedef struct fvm_batch fvm_batch;
 /* ---- runtime ------------------------------------------------------------- */
 /* Error text of the calling thread's last failure (copied into buf, truncated). */
 int fvm_last_error(char* buf, size_t n);
+/* Build configuration of this library (target and tuning constants), as text.  No
+ * reference counterpart: lets benchmarks match profiles to the build that made them. */
+int fvm_build_info(char* buf, size_t n);
 int fvm_device_count(int* n);
 int fvm_device_synchronize(void);
 /* Device memory (what Julia would wrap as a device vector with a finalizer). */
