@@ -73,6 +73,54 @@ enum : int {
   P_COUNT = 24
 };
 
+// The fields of triangle (p, q, r)'s record rebuilt at LEVEL (see FVM_GEO_LEVEL;
+// 1: s7..s9 and the CV-edge midpoints, 2: also s1..s6, 3: everything), with exactly
+// the operations of the geometry kernel (geometry.jl:114-161), so the values equal
+// the stored ones bit for bit.
+template <int LEVEL>
+__device__ __forceinline__ void rebuild_record(double px, double py, double qx, double qy,
+                                               double rx, double ry, double* P) {
+  const double cx = (px + qx + rx) / 3, cy = (py + qy + ry) / 3;
+  const double m1x = (px + qx) / 2, m1y = (py + qy) / 2;
+  const double m2x = (qx + rx) / 2, m2y = (qy + ry) / 2;
+  const double m3x = (rx + px) / 2, m3y = (ry + py) / 2;
+  const double D = qx * ry - qy * rx - px * ry + rx * py + px * qy - qx * py;
+  if (LEVEL >= 2) {
+    P[0] = (qy - ry) / D;
+    P[1] = (ry - py) / D;
+    P[2] = (py - qy) / D;
+    P[3] = (rx - qx) / D;
+    P[4] = (px - rx) / D;
+    P[5] = (qx - px) / D;
+  }
+  P[6] = (qx * ry - rx * qy) / D;
+  P[7] = (rx * py - px * ry) / D;
+  P[8] = (px * qy - qx * py) / D;
+  P[9] = (m1x + cx) / 2;
+  P[10] = (m1y + cy) / 2;
+  P[11] = (m2x + cx) / 2;
+  P[12] = (m2y + cy) / 2;
+  P[13] = (m3x + cx) / 2;
+  P[14] = (m3y + cy) / 2;
+  if (LEVEL >= 3) {
+    const double e1x = cx - m1x, e1y = cy - m1y;
+    const double e2x = cx - m2x, e2y = cy - m2y;
+    const double e3x = cx - m3x, e3y = cy - m3y;
+    const double l1 = sqrt(e1x * e1x + e1y * e1y);
+    const double l2 = sqrt(e2x * e2x + e2y * e2y);
+    const double l3 = sqrt(e3x * e3x + e3y * e3y);
+    P[15] = e1y / l1;
+    P[16] = -e1x / l1;
+    P[17] = e2y / l2;
+    P[18] = -e2x / l2;
+    P[19] = e3y / l3;
+    P[20] = -e3x / l3;
+    P[21] = l1;
+    P[22] = l2;
+    P[23] = l3;
+  }
+}
+
 // Condition functions a(x,y,t,u) = c0 + cu u + ct t + cx x + cy y + g.
 struct CondTable {
   int32_t n;

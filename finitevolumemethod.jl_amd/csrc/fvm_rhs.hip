@@ -90,51 +90,6 @@ __host__ __device__ constexpr bool loads_field(int fk, int f) {
 }
 __host__ __device__ constexpr bool rebuilds(int fk) { return kGeo >= 2 || fk != FVM_FLUX_D_EDGE; }
 
-// The rebuilt fields of triangle (p, q, r), with exactly the operations of the
-// geometry kernel (geometry.jl:114-161).
-__device__ __forceinline__ void recompute_geometry(double px, double py, double qx, double qy,
-                                                   double rx, double ry, double* P) {
-  const double cx = (px + qx + rx) / 3, cy = (py + qy + ry) / 3;
-  const double m1x = (px + qx) / 2, m1y = (py + qy) / 2;
-  const double m2x = (qx + rx) / 2, m2y = (qy + ry) / 2;
-  const double m3x = (rx + px) / 2, m3y = (ry + py) / 2;
-  const double D = qx * ry - qy * rx - px * ry + rx * py + px * qy - qx * py;
-  if (kGeo >= 2) {
-    P[0] = (qy - ry) / D;
-    P[1] = (ry - py) / D;
-    P[2] = (py - qy) / D;
-    P[3] = (rx - qx) / D;
-    P[4] = (px - rx) / D;
-    P[5] = (qx - px) / D;
-  }
-  P[6] = (qx * ry - rx * qy) / D;
-  P[7] = (rx * py - px * ry) / D;
-  P[8] = (px * qy - qx * py) / D;
-  P[9] = (m1x + cx) / 2;
-  P[10] = (m1y + cy) / 2;
-  P[11] = (m2x + cx) / 2;
-  P[12] = (m2y + cy) / 2;
-  P[13] = (m3x + cx) / 2;
-  P[14] = (m3y + cy) / 2;
-  if (kGeo >= 3) {
-    const double e1x = cx - m1x, e1y = cy - m1y;
-    const double e2x = cx - m2x, e2y = cy - m2y;
-    const double e3x = cx - m3x, e3y = cy - m3y;
-    const double l1 = sqrt(e1x * e1x + e1y * e1y);
-    const double l2 = sqrt(e2x * e2x + e2y * e2y);
-    const double l3 = sqrt(e3x * e3x + e3y * e3y);
-    P[15] = e1y / l1;
-    P[16] = -e1x / l1;
-    P[17] = e2y / l2;
-    P[18] = -e2x / l2;
-    P[19] = e3y / l3;
-    P[20] = -e3x / l3;
-    P[21] = l1;
-    P[22] = l2;
-    P[23] = l3;
-  }
-}
-
 __device__ __forceinline__ double dpoly(const KArgs& a, double u) {
   return a.d0 + u * (a.d1 + u * (a.d2 + u * a.d3));
 }
@@ -434,7 +389,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
     const int lt = tid + r * BLOCK;
     if (lt < nt) {
       const int ia = L[r] & 1023u, ib = (L[r] >> 10) & 1023u, ic = (L[r] >> 20) & 1023u;
-      if (RC) recompute_geometry(s_x[ia], s_y[ia], s_x[ib], s_y[ib], s_x[ic], s_y[ic], P[r]);
+      if (RC) rebuild_record<kGeo>(s_x[ia], s_y[ia], s_x[ib], s_y[ib], s_x[ic], s_y[ic], P[r]);
       double Q[3];
       tri_fluxes<FK, MODE>(a, P[r], s_u[ia], s_u[ib], s_u[ic],
                            MODE == M_JVP ? s_v[ia] : 0.0, MODE == M_JVP ? s_v[ib] : 0.0,
@@ -641,7 +596,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
     const int lt = tid + r * BLOCK;
     if (lt < nt) {
       const int id[3] = {(int)(L[r] & 1023u), (int)((L[r] >> 10) & 1023u), (int)((L[r] >> 20) & 1023u)};
-      if (RC) recompute_geometry(s_x[id[0]], s_y[id[0]], s_x[id[1]], s_y[id[1]], s_x[id[2]], s_y[id[2]], P[r]);
+      if (RC) rebuild_record<kGeo>(s_x[id[0]], s_y[id[0]], s_x[id[1]], s_y[id[1]], s_x[id[2]], s_y[id[2]], P[r]);
       const double* p = P[r];
       const double ua = s_u[id[0]], ub = s_u[id[1]], uc = s_u[id[2]];
       const double al = p[0] * ua + p[1] * ub + p[2] * uc;

@@ -1,0 +1,582 @@
+// FVMSystem on the device: fvm_eqs! for nv coupled variables on one mesh.
+//
+// Reference: src/problem.jl:196-413 (FVMSystem, merge_problem_conditions) and the
+// system methods of src/equations/*.jl: shape functions per variable
+// (shape_functions.jl:12-19), every variable's flux from one flux evaluation per
+// CV edge (individual_flux_contributions.jl:36-50), du[var, i] updates
+// (triangle_contributions.jl:18-25, boundary_edge_contributions.jl:15-72),
+// per-variable source / condition rules (source_contributions.jl:17-30, 49-68) and
+// the variable-by-variable Dirichlet update (dirichlet.jl:46-53).
+//
+// Device forms of the closures (traced on the host):
+//   flux    q_v = -sum_w K[v][w] (alpha_w, beta_w)           (constant K)
+//   source  S_v = s_node[v][i] + sum of monomials c prod_w u_w^e_w + st_v t
+//   cond    a   = c0 + sum_w cu[w] u_w + ct t + cx x + cy y + g exp(lam t)
+// The state is the reference's matrix u[var, i] (column-major: var + nv i).
+//
+// Layout and scheme follow the scalar tile kernels: one workgroup per tile, the
+// triangle records rebuilt from the vertex coordinates, nv fluxes per CV edge in
+// LDS, per-node sums in a fixed order, exclusive nodes finished in the tile
+// kernel, shared nodes through nv partial slots and a fix-up kernel.  No atomics.
+#include <algorithm>
+#include <vector>
+
+#include "fvm_internal.hpp"
+
+using namespace fvm;
+
+namespace {
+
+constexpr int kMaxVars = 4;
+constexpr int kMaxTerms = 24;
+
+struct SysSource {
+  int32_t n;
+  int32_t var[kMaxTerms];
+  int8_t e[kMaxTerms][kMaxVars];
+  double c[kMaxTerms];
+};
+
+struct SysArgs {
+  int32_t nv;
+  int32_t n_tiles, n_fix, n_bitems;
+  int64_t N, T, B;
+  const int32_t* tile_tri_start;
+  const int32_t* tile_node_start;
+  const int32_t* tile_nodes;
+  const uint32_t* loc;
+  const int32_t* inc_start;
+  const uint16_t* inc;
+  const int32_t* fix_nodes;
+  const int32_t* fix_part_start;
+  const int32_t* fix_bedge_start;
+  const int32_t* fix_bedge;
+  const double* props;
+  const int32_t* tri;
+  const double* x;
+  const double* y;
+  const double* cv;
+  const int32_t* bedges;
+  const int32_t* badj;
+  const double* lcv;
+  const int32_t* lpart;   // partial slot of a local entry, -1 for an exclusive node
+  const int8_t* code;     // [nv][N] 0 none, 1 Dirichlet, 2 Dudt, 3 not a vertex
+  const int32_t* fidx;    // [nv][N]
+  const int32_t* bneu;    // [nv][B]
+  double K[kMaxVars][kMaxVars];
+  SysSource S;
+  const double* s_node;   // [nv][N] or null
+  double st[kMaxVars];
+  int32_t n_fn;
+  const double* fc0;      // [n_fn]
+  const double* fcu;      // [n_fn][nv]
+  const double* fct;
+  const double* fcx;
+  const double* fcy;
+  const double* flam;
+  const double* node_g;   // [nv][N] or null
+  const double* bedge_g;  // [nv][2B] or null
+  const double* u;
+  double* out;
+  double* partial;        // [n_parts][nv]
+  double* bflux;          // [n_bitems][nv]
+  double t;
+};
+
+// a(x, y, t, u[0..nv)) of condition function f
+template <int NV>
+__device__ __forceinline__ double cond_eval(const SysArgs& a, int f, double x, double y, double t,
+                                            const double* u, double g) {
+  double v = a.fc0[f];
+#pragma unroll
+  for (int w = 0; w < NV; ++w) v = v + a.fcu[f * NV + w] * u[w];
+  v = v + a.fct[f] * t;
+  v = v + a.fcx[f] * x;
+  v = v + a.fcy[f] * y;
+  const double lam = a.flam[f];
+  return lam != 0.0 ? v + g * exp(lam * t) : v + g;
+}
+
+template <int NV>
+__device__ __forceinline__ double source_eval(const SysArgs& a, int v, int g, const double* u) {
+  double S = a.s_node ? a.s_node[(int64_t)v * a.N + g] : 0.0;
+  for (int k = 0; k < a.S.n; ++k) {
+    if (a.S.var[k] != v) continue;
+    double m = a.S.c[k];
+#pragma unroll
+    for (int w = 0; w < NV; ++w)
+      for (int p = 0; p < a.S.e[k][w]; ++p) m = m * u[w];
+    S = S + m;
+  }
+  if (a.st[v] != 0.0) S = S + a.st[v] * a.t;
+  return S;
+}
+
+// source_contributions.jl:49-68 for variable v of node g
+template <int NV>
+__device__ __forceinline__ double finish_var(const SysArgs& a, int v, int g, double acc, double V,
+                                             const double* u) {
+  const int c = a.code[(int64_t)v * a.N + g];
+  if (c == 0) return acc / V + source_eval<NV>(a, v, g, u);
+  if (c == 2)
+    return cond_eval<NV>(a, a.fidx[(int64_t)v * a.N + g], a.x[g], a.y[g], a.t, u,
+                         a.node_g ? a.node_g[(int64_t)v * a.N + g] : 0.0);
+  return 0.0;
+}
+
+// Boundary half-edge fluxes of every variable at endpoint ep of boundary edge e
+// (get_boundary_fluxes for a system, boundary_edge_contributions.jl:51-58).
+template <int NV>
+__device__ void bedge_fluxes(const SysArgs& a, int e, int ep, double* Q) {
+  const int i = a.bedges[e], j = a.bedges[a.B + e];
+  const double px = a.x[i], py = a.y[i], qx = a.x[j], qy = a.y[j];
+  const double lij = sqrt((qx - px) * (qx - px) + (qy - py) * (qy - py));
+  const double nx = (qy - py) / lij, ny = -(qx - px) / lij;
+  const double mijx = (px + qx) / 2, mijy = (py + qy) / 2;
+  const double mx = ep ? (qx + mijx) / 2 : (px + mijx) / 2;
+  const double my = ep ? (qy + mijy) / 2 : (py + mijy) / 2;
+  const double ell = sqrt((mijx - px) * (mijx - px) + (mijy - py) * (mijy - py));
+  const int64_t T = a.badj[e];
+  const int va = a.tri[T], vb = a.tri[a.T + T], vc = a.tri[2 * a.T + T];
+  double s[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) s[f] = a.props[f * a.T + T];
+  double al[NV], be[NV], um[NV];
+#pragma unroll
+  for (int w = 0; w < NV; ++w) {
+    const double ua = a.u[w + NV * (int64_t)va], ub = a.u[w + NV * (int64_t)vb], uc = a.u[w + NV * (int64_t)vc];
+    al[w] = s[0] * ua + s[1] * ub + s[2] * uc;
+    be[w] = s[3] * ua + s[4] * ub + s[5] * uc;
+    const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
+    um[w] = al[w] * mx + be[w] * my + ga;
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const int nf = a.bneu[(int64_t)v * a.B + e];
+    double qn;
+    if (nf < 0) {
+      double qxv = 0.0, qyv = 0.0;
+#pragma unroll
+      for (int w = 0; w < NV; ++w)
+        if (a.K[v][w] != 0.0) {
+          qxv = qxv + a.K[v][w] * al[w];
+          qyv = qyv + a.K[v][w] * be[w];
+        }
+      qn = (-qxv) * nx + (-qyv) * ny;
+    } else {
+      qn = cond_eval<NV>(a, nf, mx, my, a.t, um, a.bedge_g ? a.bedge_g[(int64_t)v * 2 * a.B + 2 * e + ep] : 0.0);
+    }
+    Q[v] = qn * ell;
+  }
+}
+
+template <int NV, int BLOCK, int TPT>
+__global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
+  constexpr int TILE = BLOCK * TPT;
+  constexpr int MAXN = 2 * BLOCK < 1024 ? 2 * BLOCK : 1024;
+  constexpr int NS = MAXN / BLOCK;
+  __shared__ double s_u[NV][MAXN];
+  __shared__ double s_x[MAXN];
+  __shared__ double s_y[MAXN];
+  __shared__ double s_q[NV][3 * TILE];
+  __shared__ uint16_t s_inc[3 * TILE];
+  __shared__ int s_is[MAXN + 1];
+  const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
+  const int wid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  if (wid < nbw) {
+    const int k = wid * BLOCK + tid;
+    if (k < a.n_bitems) {
+      const int w = a.fix_bedge[k];
+      double Q[NV];
+      bedge_fluxes<NV>(a, w >> 1, w & 1, Q);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) a.bflux[(int64_t)k * NV + v] = Q[v];
+    }
+    return;
+  }
+  const int tile = wid - nbw;
+  if (tile >= a.n_tiles) return;
+  const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
+  const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
+  const int ib0 = 3 * t0;
+  int g[NS], pi[NS];
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const int l = tid + r * BLOCK;
+    g[r] = 0;
+    pi[r] = -1;
+    if (l < nn) {
+      g[r] = a.tile_nodes[n0 + l];
+      pi[r] = a.lpart[n0 + l];
+      s_is[l] = a.inc_start[n0 + l] - ib0;
+      s_x[l] = a.x[g[r]];
+      s_y[l] = a.y[g[r]];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s_u[v][l] = a.u[v + NV * (int64_t)g[r]];
+    }
+  }
+  if (tid == 0) s_is[nn] = 3 * nt;
+#pragma unroll
+  for (int k = 0; k < 3 * TPT; ++k) {
+    const int idx = tid + k * BLOCK;
+    if (idx < 3 * nt) s_inc[idx] = a.inc[ib0 + idx];
+  }
+  uint32_t L[TPT];
+#pragma unroll
+  for (int r = 0; r < TPT; ++r) {
+    const int lt = tid + r * BLOCK;
+    L[r] = lt < nt ? a.loc[t0 + lt] : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < TPT; ++r) {
+    const int lt = tid + r * BLOCK;
+    if (lt < nt) {
+      const int ia = L[r] & 1023u, ib = (L[r] >> 10) & 1023u, ic = (L[r] >> 20) & 1023u;
+      double P[P_COUNT];
+      rebuild_record<3>(s_x[ia], s_y[ia], s_x[ib], s_y[ib], s_x[ic], s_y[ic], P);
+      double al[NV], be[NV];
+#pragma unroll
+      for (int w = 0; w < NV; ++w) {
+        al[w] = P[0] * s_u[w][ia] + P[1] * s_u[w][ib] + P[2] * s_u[w][ic];
+        be[w] = P[3] * s_u[w][ia] + P[4] * s_u[w][ib] + P[5] * s_u[w][ic];
+      }
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        const double nx = P[P_NX + 2 * e], ny = P[P_NX + 1 + 2 * e], l = P[P_L + e];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          double qxv = 0.0, qyv = 0.0;
+#pragma unroll
+          for (int w = 0; w < NV; ++w)
+            if (a.K[v][w] != 0.0) {
+              qxv = qxv + a.K[v][w] * al[w];
+              qyv = qyv + a.K[v][w] * be[w];
+            }
+          // (qx nx + qy ny) l with q = -K (alpha, beta)
+          s_q[v][3 * lt + e] = ((-qxv) * nx + (-qyv) * ny) * l;
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const int l = tid + r * BLOCK;
+    if (l < nn) {
+      double acc[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+      const int k1 = s_is[l + 1];
+      for (int k = s_is[l]; k < k1; ++k) {
+        const int w = s_inc[k];
+        const int lt = w >> 2, sl = w & 3;
+        const int prev = 3 * lt + (sl == 0 ? 2 : sl - 1), cur = 3 * lt + sl;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = acc[v] + s_q[v][prev] - s_q[v][cur];
+      }
+      if (pi[r] >= 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a.partial[(int64_t)pi[r] * NV + v] = acc[v];
+      } else {
+        double u[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) u[v] = s_u[v][l];
+        const double V = a.lcv[n0 + l];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a.out[v + NV * (int64_t)g[r]] = finish_var<NV>(a, v, g[r], acc[v], V, u);
+      }
+    }
+  }
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void sys_fix_kernel(SysArgs a) {
+  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fi >= a.n_fix) return;
+  const int g = a.fix_nodes[fi];
+  double acc[NV], u[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    acc[v] = 0.0;
+    u[v] = a.u[v + NV * (int64_t)g];
+  }
+  for (int p = a.fix_part_start[fi]; p < a.fix_part_start[fi + 1]; ++p)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = acc[v] + a.partial[(int64_t)p * NV + v];
+  for (int k = a.fix_bedge_start[fi]; k < a.fix_bedge_start[fi + 1]; ++k)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = acc[v] - a.bflux[(int64_t)k * NV + v];
+  const double V = a.cv[g];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) a.out[v + NV * (int64_t)g] = finish_var<NV>(a, v, g, acc[v], V, u);
+}
+
+// dirichlet.jl:46-53: variable by variable; each thread owns one node, so updating
+// its variables in order reproduces the serial loop.  mask[k] has bit v set when
+// variable v is Dirichlet at nodes[k] (vertex or not, as the reference's loop over
+// the Dirichlet nodes).
+template <int NV>
+__global__ __launch_bounds__(256) void sys_dirichlet_kernel(SysArgs a, const int32_t* __restrict__ nodes,
+                                                            const int32_t* __restrict__ mask, int n,
+                                                            double* __restrict__ u) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int g = nodes[k];
+  const int bits = mask[k];
+  double uv[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) uv[v] = u[v + NV * (int64_t)g];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    if (!((bits >> v) & 1)) continue;
+    uv[v] = cond_eval<NV>(a, a.fidx[(int64_t)v * a.N + g], a.x[g], a.y[g], a.t, uv,
+                          a.node_g ? a.node_g[(int64_t)v * a.N + g] : 0.0);
+    u[v + NV * (int64_t)g] = uv[v];
+  }
+}
+
+template <class T>
+int dcopy(T** dst, const T* src, int64_t n) {
+  *dst = nullptr;
+  FVM_HIP(hipMalloc((void**)dst, sizeof(T) * (size_t)(n > 0 ? n : 1)));
+  if (n > 0 && src) FVM_HIP(hipMemcpy(*dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice));
+  return FVM_OK;
+}
+
+}  // namespace
+
+struct fvm_system {
+  const fvm_mesh* m = nullptr;
+  int32_t nv = 0;
+  SysArgs a{};
+  std::vector<void*> owned;
+  int32_t* dir_nodes = nullptr;
+  int32_t* dir_mask = nullptr;  // bit v: variable v is Dirichlet at the node
+  int32_t n_dir = 0;
+};
+
+namespace {
+
+void free_system(fvm_system* s) {
+  if (!s) return;
+  for (void* p : s->owned)
+    if (p) (void)hipFree(p);
+  delete s;
+}
+
+template <int NV>
+int launch_sys(const fvm_system* s, const SysArgs& a, hipStream_t st) {
+  const int tt = s->m->tile_tris;
+  const int nb256 = (a.n_bitems + 255) / 256, nb512 = (a.n_bitems + 511) / 512;
+  if (tt <= 256)
+    hipLaunchKernelGGL((sys_tile_kernel<NV, 256, 1>), dim3((unsigned)xcd_grid(a.n_tiles + nb256)), dim3(256), 0,
+                       st, a);
+  else if (tt <= 512)
+    hipLaunchKernelGGL((sys_tile_kernel<NV, 256, 2>), dim3((unsigned)xcd_grid(a.n_tiles + nb256)), dim3(256), 0,
+                       st, a);
+  else
+    hipLaunchKernelGGL((sys_tile_kernel<NV, 512, 2>), dim3((unsigned)xcd_grid(a.n_tiles + nb512)), dim3(512), 0,
+                       st, a);
+  if (a.n_fix > 0)
+    hipLaunchKernelGGL((sys_fix_kernel<NV>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0, st, a);
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvm_system_create(const fvm_mesh* m, const fvm_system_desc* d, fvm_system** out) {
+  FVM_CHECK(m && d && out, "fvm_system_create: null argument");
+  *out = nullptr;
+  const int nv = d->n_vars;
+  FVM_CHECK(nv >= 1 && nv <= kMaxVars, "fvm_system_create: n_vars must lie in [1, 4]");
+  FVM_CHECK(d->flux_k, "fvm_system_create: flux_k required");
+  FVM_CHECK(d->n_terms >= 0 && d->n_terms <= kMaxTerms, "fvm_system_create: at most 24 source terms");
+  FVM_CHECK(d->n_terms == 0 || (d->term_var && d->term_exp && d->term_coef),
+            "fvm_system_create: source terms incomplete");
+  FVM_CHECK(d->node_kind && d->node_fidx && (m->B == 0 || d->bedge_neumann_fidx),
+            "fvm_system_create: conditions required");
+  FVM_CHECK(d->n_functions >= 0 && (d->n_functions == 0 || (d->fn_c0 && d->fn_cu)),
+            "fvm_system_create: condition coefficients missing");
+  FVM_CHECK(m->tile_tris <= 1024, "fvm_system_create: tiles of at most 1024 triangles");
+  const int64_t N = m->N, T = m->T, B = m->B;
+  const int nf = d->n_functions;
+  fvm_system* s = new fvm_system();
+  s->m = m;
+  s->nv = nv;
+  SysArgs& a = s->a;
+  a.nv = nv;
+  a.n_tiles = m->n_tiles;
+  a.n_fix = m->n_fix;
+  a.n_bitems = m->n_bedge_items;
+  a.N = N;
+  a.T = T;
+  a.B = B;
+  a.tile_tri_start = m->tile_tri_start;
+  a.tile_node_start = m->tile_node_start;
+  a.tile_nodes = m->tile_nodes;
+  a.loc = m->loc;
+  a.inc_start = m->inc_start;
+  a.inc = m->inc;
+  a.fix_nodes = m->fix_nodes;
+  a.fix_part_start = m->fix_part_start;
+  a.fix_bedge_start = m->fix_bedge_start;
+  a.fix_bedge = m->fix_bedge;
+  a.props = m->props;
+  a.tri = m->tri;
+  a.x = m->x;
+  a.y = m->y;
+  a.cv = m->cv;
+  a.bedges = m->bedges;
+  a.badj = m->badj;
+  a.lcv = m->lcv;
+  for (int v = 0; v < kMaxVars; ++v) {
+    a.st[v] = (v < nv && d->st) ? d->st[v] : 0.0;
+    for (int w = 0; w < kMaxVars; ++w) a.K[v][w] = (v < nv && w < nv) ? d->flux_k[v * nv + w] : 0.0;
+  }
+  a.S.n = d->n_terms;
+  for (int k = 0; k < d->n_terms; ++k) {
+    FVM_CHECK(d->term_var[k] >= 0 && d->term_var[k] < nv, "fvm_system_create: term_var out of range");
+    a.S.var[k] = d->term_var[k];
+    a.S.c[k] = d->term_coef[k];
+    for (int w = 0; w < kMaxVars; ++w) {
+      const int e = w < nv ? d->term_exp[k * nv + w] : 0;
+      FVM_CHECK(e >= 0 && e <= 3, "fvm_system_create: source exponents must lie in [0, 3]");
+      a.S.e[k][w] = (int8_t)e;
+    }
+  }
+  // node codes per variable (3 marks points that are not vertices), Dirichlet node list
+  std::vector<int8_t> code((size_t)nv * N);
+  std::vector<int32_t> dir, dir_mask;
+  for (int64_t g = 0; g < N; ++g) {
+    const bool vert = m->h_node_tri_start[g + 1] > m->h_node_tri_start[g];
+    int32_t bits = 0;
+    for (int v = 0; v < nv; ++v) {
+      const int k = d->node_kind[(int64_t)v * N + g];
+      if (k < 0 || k > 2) {
+        free_system(s);
+        set_error("fvm_system_create: node_kind entries must be 0, 1 or 2");
+        return FVM_ERR_ARG;
+      }
+      if (k != 0) {
+        const int f = d->node_fidx[(int64_t)v * N + g];
+        if (f < 0 || f >= nf) {
+          free_system(s);
+          set_error("fvm_system_create: node_fidx out of range");
+          return FVM_ERR_ARG;
+        }
+      }
+      code[(size_t)v * N + g] = vert ? (int8_t)k : (int8_t)3;
+      if (k == 1) bits |= 1 << v;
+    }
+    if (bits) {
+      dir.push_back((int32_t)g);
+      dir_mask.push_back(bits);
+    }
+  }
+  for (int64_t e = 0; e < (int64_t)nv * B; ++e) {
+    const int f = d->bedge_neumann_fidx[e];
+    if (f != -1 && (f < 0 || f >= nf)) {
+      free_system(s);
+      set_error("fvm_system_create: bedge_neumann_fidx out of range");
+      return FVM_ERR_ARG;
+    }
+  }
+  std::vector<double> zeros(std::max<int64_t>(nf, 1), 0.0);
+  std::vector<int32_t> lp(m->n_local);
+  for (int64_t e = 0; e < m->n_local; ++e) lp[e] = m->h_part_idx[e] >= 0 ? m->h_part_idx[e] : -1;
+  int rc = FVM_OK;
+  auto up = [&](auto** dst, const auto* src, int64_t n) {
+    if (rc) return;
+    rc = dcopy(dst, src, n);
+    if (!rc) s->owned.push_back((void*)*dst);
+  };
+  int8_t* d_code = nullptr;
+  int32_t *d_fidx = nullptr, *d_bneu = nullptr, *d_lpart = nullptr;
+  double *d_c0 = nullptr, *d_cu = nullptr, *d_ct = nullptr, *d_cx = nullptr, *d_cy = nullptr, *d_lam = nullptr;
+  double *d_snode = nullptr, *d_ng = nullptr, *d_bg = nullptr, *d_part = nullptr, *d_bflux = nullptr;
+  up(&d_code, code.data(), (int64_t)nv * N);
+  up(&d_fidx, d->node_fidx, (int64_t)nv * N);
+  up(&d_bneu, d->bedge_neumann_fidx, (int64_t)nv * B);
+  up(&d_lpart, lp.data(), m->n_local);
+  up(&d_c0, nf ? d->fn_c0 : zeros.data(), std::max(nf, 1));
+  up(&d_cu, nf ? d->fn_cu : zeros.data(), nf ? (int64_t)nf * nv : 1);
+  up(&d_ct, nf && d->fn_ct ? d->fn_ct : zeros.data(), std::max(nf, 1));
+  up(&d_cx, nf && d->fn_cx ? d->fn_cx : zeros.data(), std::max(nf, 1));
+  up(&d_cy, nf && d->fn_cy ? d->fn_cy : zeros.data(), std::max(nf, 1));
+  up(&d_lam, nf && d->fn_lam ? d->fn_lam : zeros.data(), std::max(nf, 1));
+  if (d->s_node) up(&d_snode, d->s_node, (int64_t)nv * N);
+  if (d->node_g) up(&d_ng, d->node_g, (int64_t)nv * N);
+  if (d->bedge_g) up(&d_bg, d->bedge_g, (int64_t)nv * 2 * B);
+  up(&d_part, (const double*)nullptr, (int64_t)std::max(m->n_parts, 1) * nv);
+  up(&d_bflux, (const double*)nullptr, (int64_t)std::max(m->n_bedge_items, 1) * nv);
+  up(&s->dir_nodes, dir.data(), (int64_t)dir.size());
+  up(&s->dir_mask, dir_mask.data(), (int64_t)dir_mask.size());
+  if (rc) {
+    free_system(s);
+    return rc;
+  }
+  s->n_dir = (int32_t)dir.size();
+  a.lpart = d_lpart;
+  a.code = d_code;
+  a.fidx = d_fidx;
+  a.bneu = d_bneu;
+  a.s_node = d_snode;
+  a.n_fn = nf;
+  a.fc0 = d_c0;
+  a.fcu = d_cu;
+  a.fct = d_ct;
+  a.fcx = d_cx;
+  a.fcy = d_cy;
+  a.flam = d_lam;
+  a.node_g = d_ng;
+  a.bedge_g = d_bg;
+  a.partial = d_part;
+  a.bflux = d_bflux;
+  *out = s;
+  return FVM_OK;
+}
+
+int fvm_system_destroy(fvm_system* s) {
+  free_system(s);
+  return FVM_OK;
+}
+
+int fvm_system_rhs(fvm_system* s, const double* d_u, double* d_du, double t, fvm_stream st) {
+  FVM_CHECK(s && d_u && d_du, "fvm_system_rhs: null argument");
+  FVM_CHECK(d_u != d_du, "fvm_system_rhs: du must not alias u");
+  SysArgs a = s->a;
+  a.u = d_u;
+  a.out = d_du;
+  a.t = t;
+  hipStream_t hs = (hipStream_t)st;
+  switch (s->nv) {
+    case 1: return launch_sys<1>(s, a, hs);
+    case 2: return launch_sys<2>(s, a, hs);
+    case 3: return launch_sys<3>(s, a, hs);
+    default: return launch_sys<4>(s, a, hs);
+  }
+}
+
+int fvm_system_dirichlet_apply(fvm_system* s, double* d_u, double t, fvm_stream st) {
+  FVM_CHECK(s && d_u, "fvm_system_dirichlet_apply: null argument");
+  if (s->n_dir == 0) return FVM_OK;
+  SysArgs a = s->a;
+  a.t = t;
+  const dim3 g((unsigned)((s->n_dir + 255) / 256));
+  hipStream_t hs = (hipStream_t)st;
+  switch (s->nv) {
+    case 1: hipLaunchKernelGGL((sys_dirichlet_kernel<1>), g, dim3(256), 0, hs, a, s->dir_nodes, s->dir_mask, s->n_dir, d_u); break;
+    case 2: hipLaunchKernelGGL((sys_dirichlet_kernel<2>), g, dim3(256), 0, hs, a, s->dir_nodes, s->dir_mask, s->n_dir, d_u); break;
+    case 3: hipLaunchKernelGGL((sys_dirichlet_kernel<3>), g, dim3(256), 0, hs, a, s->dir_nodes, s->dir_mask, s->n_dir, d_u); break;
+    default: hipLaunchKernelGGL((sys_dirichlet_kernel<4>), g, dim3(256), 0, hs, a, s->dir_nodes, s->dir_mask, s->n_dir, d_u);
+  }
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+}  // extern "C"

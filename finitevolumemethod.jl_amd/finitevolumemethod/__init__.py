@@ -1,7 +1,7 @@
 """MI355X-native vertex-centred finite-volume hot path of FiniteVolumeMethod.jl.
 
 The public names mirror the reference module (src/FiniteVolumeMethod.jl:39-52 and
-the template exports): FVMGeometry, FVMProblem, SteadyFVMProblem,
+the template exports): FVMGeometry, FVMProblem, FVMSystem, SteadyFVMProblem,
 BoundaryConditions, InternalConditions, Conditions, Neumann, Dudt, Dirichlet,
 Constrained, solve, fvm_eqs (fvm_eqs!), DiffusionEquation, PoissonsEquation,
 LaplacesEquation, MeanExitTimeProblem, LinearReactionDiffusionEquation.  Every numerical kernel runs on the GPU through
@@ -18,6 +18,7 @@ from .forms import NodalDiffusion, UnsupportedClosure  # noqa: F401
 from .geometry import FVMGeometry, TriangleProperties  # noqa: F401
 from .mesh import Triangulation, jittered_rectangle, triangulate, triangulate_rectangle  # noqa: F401
 from .problem import FVMProblem, SteadyFVMProblem, construct_flux_function  # noqa: F401
+from .system import FVMSystem, InvalidFluxError, is_system  # noqa: F401
 from .solve import SSPRK3, Euler, ODESolution, Tsit5, solve  # noqa: F401
 from .templates import (AbstractFVMTemplate, BiCGStab, CG, DiffusionEquation,  # noqa: F401
                         LaplacesEquation, LinearReactionDiffusionEquation, LinearSolution,

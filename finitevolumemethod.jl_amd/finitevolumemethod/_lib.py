@@ -94,6 +94,15 @@ class BatchInfo(ctypes.Structure):
                 ("candidate_tiles", ctypes.c_int32 * 4), ("candidate_ms", ctypes.c_float * 4)]
 
 
+class SystemDesc(ctypes.Structure):
+    _fields_ = [("n_vars", ctypes.c_int32), ("flux_k", _dp), ("n_terms", ctypes.c_int32),
+                ("term_var", _i32p), ("term_exp", _i32p), ("term_coef", _dp), ("s_node", _dp),
+                ("st", _dp), ("node_kind", _i8p), ("node_fidx", _i32p),
+                ("bedge_neumann_fidx", _i32p), ("n_functions", ctypes.c_int32), ("fn_c0", _dp),
+                ("fn_cu", _dp), ("fn_ct", _dp), ("fn_cx", _dp), ("fn_cy", _dp), ("fn_lam", _dp),
+                ("node_g", _dp), ("bedge_g", _dp)]
+
+
 FLUX_D_POLY_U, FLUX_D_NODAL, FLUX_D_EDGE = 0, 1, 2
 TEMPLATE_POISSON, TEMPLATE_MEAN_EXIT_TIME, TEMPLATE_DIFFUSION = 0, 1, 2
 SOLVE_CG, SOLVE_CG_MIXED, SOLVE_BICGSTAB = 0, 1, 2
@@ -184,6 +193,10 @@ HIP_API = {
     "fvm_batch_mesh": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "fvm_batch_rhs": [_vp, _vp, _vp, ctypes.c_double, _vp],
     "fvm_batch_euler": [_vp, _vp, ctypes.c_int32, ctypes.c_double, _vp],
+    "fvm_system_create": [_vp, ctypes.POINTER(SystemDesc), ctypes.POINTER(_vp)],
+    "fvm_system_destroy": [_vp],
+    "fvm_system_rhs": [_vp, _vp, _vp, ctypes.c_double, _vp],
+    "fvm_system_dirichlet_apply": [_vp, _vp, ctypes.c_double, _vp],
     "fvm_vec_lincomb": [ctypes.c_int64, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), _vp],
     "fvm_vec_mul": [ctypes.c_int64, _vp, _vp, _vp, _vp],
     "fvm_vec_dot": [ctypes.c_int64, _vp, _vp, _dp, _vp],

@@ -15,6 +15,12 @@ trace must land on one of the forms the kernels implement:
   source S(x, y, t, u, p): s(x, y) + cubic polynomial in u + c t;
   condition a(x, y, t, u, p): g(x, y) + cu u + ct t.
 
+For an FVMSystem (u, alpha, beta, gamma are tuples over the variables):
+
+  flux q(x, y, t, a, b, g, p) -> (qx, qy): -sum_w K[w] (a[w], b[w]), K constant;
+  source S(x, y, t, u, p): s(x, y) + sum of monomials c prod_w u[w]^e_w (e_w <= 3) + c t;
+  condition: g(x, y) + sum_w cu[w] u[w] + ct t, or c0 + ... + g(x, y) exp(lam t).
+
 Anything else raises `UnsupportedClosure` with the reason: there is no silent
 CPU fallback.
 """
@@ -41,30 +47,37 @@ class NodalDiffusion:
 
 
 class Sym:
-    """Polynomial in symbolic variables with coefficients that are scalars or
-    arrays over the evaluation points: {exponent tuple: coefficient}."""
+    """Polynomial in symbolic variables, times exponentials exp(rate t), with
+    coefficients that are scalars or arrays over the evaluation points:
+    {(exponent tuple, rate): coefficient}.  `tvar` is the index of the variable
+    that stands for t (None: exp of t is not traceable)."""
 
     __array_priority__ = 1000
 
-    def __init__(self, nvars, terms=None):
+    def __init__(self, nvars, terms=None, tvar=None):
         self.nvars = nvars
+        self.tvar = tvar
         self.terms = dict(terms or {})
 
     @classmethod
-    def var(cls, nvars, k):
+    def var(cls, nvars, k, tvar=None):
         e = [0] * nvars
         e[k] = 1
-        return cls(nvars, {tuple(e): 1.0})
+        return cls(nvars, {(tuple(e), 0.0): 1.0}, tvar)
 
     @classmethod
-    def const(cls, nvars, c):
-        return cls(nvars, {(0,) * nvars: c})
+    def const(cls, nvars, c, tvar=None):
+        return cls(nvars, {((0,) * nvars, 0.0): c}, tvar)
+
+    def _new(self, terms, o=None):
+        tv = self.tvar if self.tvar is not None else getattr(o, "tvar", None)
+        return Sym(self.nvars, terms, tv)
 
     def _lift(self, o):
         if isinstance(o, Sym):
             return o
         if isinstance(o, (numbers.Number, np.ndarray, np.generic)):
-            return Sym.const(self.nvars, o)
+            return Sym.const(self.nvars, o, self.tvar)
         return NotImplemented
 
     def __add__(self, o):
@@ -72,14 +85,14 @@ class Sym:
         if o is NotImplemented:
             return o
         t = dict(self.terms)
-        for e, c in o.terms.items():
-            t[e] = t[e] + c if e in t else c
-        return Sym(self.nvars, t)
+        for k, c in o.terms.items():
+            t[k] = t[k] + c if k in t else c
+        return self._new(t, o)
 
     __radd__ = __add__
 
     def __neg__(self):
-        return Sym(self.nvars, {e: -c for e, c in self.terms.items()})
+        return self._new({k: -c for k, c in self.terms.items()})
 
     def __pos__(self):
         return self
@@ -101,11 +114,11 @@ class Sym:
         if o is NotImplemented:
             return o
         t = {}
-        for e1, c1 in self.terms.items():
-            for e2, c2 in o.terms.items():
-                e = tuple(a + b for a, b in zip(e1, e2))
-                t[e] = t[e] + c1 * c2 if e in t else c1 * c2
-        return Sym(self.nvars, t)
+        for (e1, r1), c1 in self.terms.items():
+            for (e2, r2), c2 in o.terms.items():
+                k = (tuple(a + b for a, b in zip(e1, e2)), r1 + r2)
+                t[k] = t[k] + c1 * c2 if k in t else c1 * c2
+        return self._new(t, o)
 
     __rmul__ = __mul__
 
@@ -114,7 +127,7 @@ class Sym:
             if not o.is_const():
                 raise UnsupportedClosure("division by a solution-dependent quantity")
             o = o.const_value()
-        return Sym(self.nvars, {e: c / o for e, c in self.terms.items()})
+        return self._new({k: c / o for k, c in self.terms.items()})
 
     def __rtruediv__(self, o):
         if not self.is_const():
@@ -126,9 +139,9 @@ class Sym:
             raise UnsupportedClosure("solution-dependent exponent")
         if int(k) != k or k < 0:
             if self.is_const():
-                return Sym.const(self.nvars, self.const_value() ** k)
+                return Sym.const(self.nvars, self.const_value() ** k, self.tvar)
             raise UnsupportedClosure("non-integer power of a solution-dependent quantity")
-        r = Sym.const(self.nvars, 1.0)
+        r = Sym.const(self.nvars, 1.0, self.tvar)
         for _ in range(int(k)):
             r = r * self
         return r
@@ -141,9 +154,31 @@ class Sym:
 
     __lt__ = __le__ = __gt__ = __ge__ = _cmp
 
+    def _exp(self):
+        """exp(c(x, y) + lam t) = exp(c) exp(lam t) for a constant lam."""
+        zero = (0,) * self.nvars
+        c, lam = 0.0, 0.0
+        for (e, r), v in self.terms.items():
+            if _is_zero(v):
+                continue
+            if r != 0.0:
+                raise UnsupportedClosure("exp of an exponential")
+            if not any(e):
+                c = v
+            elif self.tvar is not None and e[self.tvar] == 1 and sum(e) == 1:
+                a = np.asarray(v, np.float64).ravel()
+                if not np.all(a == a[0]):
+                    raise UnsupportedClosure("exp(lam t) with lam depending on x, y")
+                lam = float(a[0])
+            else:
+                raise UnsupportedClosure("exp of a solution-dependent quantity")
+        return Sym(self.nvars, {(zero, lam): np.exp(c)}, self.tvar)
+
     def __array_ufunc__(self, ufunc, method, *inputs, **kw):
         if method != "__call__":
             return NotImplemented
+        if ufunc is np.exp and len(inputs) == 1 and isinstance(inputs[0], Sym) and not inputs[0].is_const():
+            return inputs[0]._exp()
         vals = []
         for v in inputs:
             if isinstance(v, Sym):
@@ -155,32 +190,35 @@ class Sym:
             else:
                 vals.append(v)
         else:
-            return Sym.const(self.nvars, ufunc(*vals, **kw))
+            return Sym.const(self.nvars, ufunc(*vals, **kw), self.tvar)
         a = inputs[0]
         b = inputs[1] if len(inputs) > 1 else None
+        lift = (lambda v: v if isinstance(v, Sym) else Sym.const(self.nvars, v, self.tvar))
         if ufunc is np.add:
-            return (a if isinstance(a, Sym) else Sym.const(self.nvars, a)) + b
+            return lift(a) + b
         if ufunc is np.subtract:
-            return (a if isinstance(a, Sym) else Sym.const(self.nvars, a)) - b
+            return lift(a) - b
         if ufunc is np.multiply:
-            return (a if isinstance(a, Sym) else Sym.const(self.nvars, a)) * b
+            return lift(a) * b
         if ufunc is np.negative:
             return -a
         if ufunc is np.true_divide:
-            return (a if isinstance(a, Sym) else Sym.const(self.nvars, a)) / b
+            return lift(a) / b
         return NotImplemented
 
     def is_const(self):
-        return all(not any(e) for e, c in self.terms.items() if not _is_zero(c))
+        return all(not any(e) and r == 0.0 for (e, r), c in self.terms.items() if not _is_zero(c))
 
     def const_value(self):
-        return self.terms.get((0,) * self.nvars, 0.0)
+        return self.terms.get(((0,) * self.nvars, 0.0), 0.0)
 
-    def coef(self, e):
-        return self.terms.get(tuple(e), 0.0)
+    def coef(self, e, rate=0.0):
+        return self.terms.get((tuple(e), float(rate)), 0.0)
 
     def nonzero_exponents(self):
-        return [e for e, c in self.terms.items() if not _is_zero(c)]
+        """Exponent tuples of the nonzero terms; a term with an exp(rate t) factor
+        shows as its exponents followed by the rate."""
+        return [e if r == 0.0 else e + (r,) for (e, r), c in self.terms.items() if not _is_zero(c)]
 
 
 def _is_zero(c):
@@ -347,3 +385,135 @@ def group_indices(fidx_of_locations):
         if f >= 0:
             g[int(f)].append(k)
     return {f: np.asarray(v, np.int64) for f, v in g.items()}
+
+
+# ---------------------------------------------------------------------------
+# FVMSystem closures
+# ---------------------------------------------------------------------------
+def _call(fn, *args):
+    try:
+        return fn(*args)
+    except UnsupportedClosure:
+        raise
+    except Exception as err:  # noqa: BLE001 - any failure means the form is not traceable
+        raise UnsupportedClosure(f"tracing the closure failed: {type(err).__name__}: {err}") from err
+
+
+def _uniform(c, n, what):
+    v = _const_over(c, n)
+    if v is None:
+        raise UnsupportedClosure(f"{what} must not depend on x, y")
+    return v
+
+
+def _unit(nvars, k):
+    e = [0] * nvars
+    e[k] = 1
+    return tuple(e)
+
+
+_PROBE_X = np.array([0.1234, 0.7421, 0.3317, 0.9012])
+_PROBE_Y = np.array([0.5531, 0.0713, 0.8822, 0.2417])
+
+
+def classify_system_flux(q, nv):
+    """Trace one variable's flux q(x, y, t, alpha, beta, gamma) (bound to p) of an
+    nv-variable system: it must be -sum_w K[w] (alpha[w], beta[w]) with constant K.
+    Returns K as a list of nv floats."""
+    nvars = 1 + 3 * nv  # t, alpha_w, beta_w, gamma_w
+    t = Sym.var(nvars, 0, 0)
+    al = tuple(Sym.var(nvars, 1 + w, 0) for w in range(nv))
+    be = tuple(Sym.var(nvars, 1 + nv + w, 0) for w in range(nv))
+    ga = tuple(Sym.var(nvars, 1 + 2 * nv + w, 0) for w in range(nv))
+    r = _call(q, _PROBE_X, _PROBE_Y, t, al, be, ga)
+    if not isinstance(r, (tuple, list)) or len(r) != 2:
+        raise UnsupportedClosure("the flux function must return a pair (qx, qy)")
+    qx, qy = (v if isinstance(v, Sym) else Sym.const(nvars, v) for v in r)
+    n = len(_PROBE_X)
+    kx, ky = [0.0] * nv, [0.0] * nv
+    msg = ("system flux functions must have the form q = -sum_w K[w] (alpha[w], beta[w]) "
+           "with constant K")
+    for comp, K, off in ((qx, kx, 1), (qy, ky, 1 + nv)):
+        for e in comp.nonzero_exponents():
+            w = next((w for w in range(nv) if e == _unit(nvars, off + w)), None)
+            if w is None:
+                raise UnsupportedClosure(msg)
+            K[w] = -_uniform(comp.coef(e), n, "the flux coefficients")
+    if kx != ky:
+        raise UnsupportedClosure(msg + " (the same K for both components)")
+    return kx
+
+
+def classify_system_source(S, nv, x, y):
+    """Trace one variable's source S(x, y, t, u) (bound to p) over the points (x, y).
+    Returns (s_node or None, [(exponents, coef)], st)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    n = len(x)
+    nvars = 1 + nv  # t, u_w
+    t = Sym.var(nvars, 0, 0)
+    u = tuple(Sym.var(nvars, 1 + w, 0) for w in range(nv))
+    r = _call(S, x, y, t, u)
+    r = r if isinstance(r, Sym) else Sym.const(nvars, r)
+    terms, st, s_node = [], 0.0, None
+    for e in r.nonzero_exponents():
+        if len(e) != nvars:
+            raise UnsupportedClosure("time-exponential sources are not supported")
+        if e[0] == 1 and not any(e[1:]):
+            st = _uniform(r.coef(e), n, "the t coefficient of the source")
+        elif e[0] != 0:
+            raise UnsupportedClosure("the source must be s(x, y) + polynomial in u + c*t")
+        elif not any(e[1:]):
+            c = _const_over(r.coef(e), n)
+            if c is None:
+                s_node = np.broadcast_to(np.asarray(r.coef(e), np.float64), (n,)).copy()
+            else:
+                terms.append(((0,) * nv, c))
+        else:
+            if max(e[1:]) > 3:
+                raise UnsupportedClosure("source exponents above 3 are not supported")
+            terms.append((tuple(e[1:]), _uniform(r.coef(e), n, "the u coefficients of the source")))
+    return s_node, terms, st
+
+
+def classify_system_condition(fn, nv, x, y):
+    """Trace a condition a(x, y, t, u) (bound to p) of an nv-variable system over
+    the points (x, y): g + sum_w cu[w] u[w] + ct t, where g is g(x, y), or
+    c0 + g(x, y) exp(lam t).  Returns dict(g, c0, cu, ct, lam)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    n = len(x)
+    nvars = 1 + nv
+    if n == 0:
+        return dict(g=np.zeros(0), c0=0.0, cu=[0.0] * nv, ct=0.0, lam=0.0)
+    t = Sym.var(nvars, 0, 0)
+    u = tuple(Sym.var(nvars, 1 + w, 0) for w in range(nv))
+    r = _call(fn, x, y, t, u)
+    r = r if isinstance(r, Sym) else Sym.const(nvars, r)
+    zero = (0,) * nvars
+    cu, ct, lam = [0.0] * nv, 0.0, 0.0
+    g_exp = None
+    for k, c in r.terms.items():
+        e, rate = k
+        if _is_zero(c):
+            continue
+        if rate != 0.0:
+            if e != zero or g_exp is not None:
+                raise UnsupportedClosure("conditions may hold one term g(x, y) exp(lam t)")
+            lam, g_exp = rate, c
+        elif e == zero:
+            continue
+        elif e == _unit(nvars, 0):
+            ct = _uniform(c, n, "the t coefficient of a condition")
+        else:
+            w = next((w for w in range(nv) if e == _unit(nvars, 1 + w)), None)
+            if w is None:
+                raise UnsupportedClosure("conditions must be affine in t and u")
+            cu[w] = _uniform(c, n, "the u coefficients of a condition")
+    c_const = r.coef(zero)
+    if g_exp is None:
+        g = np.broadcast_to(np.asarray(c_const, np.float64), (n,)).copy()
+        return dict(g=g, c0=0.0, cu=cu, ct=ct, lam=0.0)
+    c0 = _uniform(c_const, n, "with a g(x, y) exp(lam t) term, the constant part of a condition")
+    g = np.broadcast_to(np.asarray(g_exp, np.float64), (n,)).copy()
+    return dict(g=g, c0=c0, cu=cu, ct=ct, lam=lam)

@@ -28,7 +28,8 @@ def construct_flux_function(q, D, Dp):
 
 
 def _zero_source(x, y, t, u, p):
-    return 0.0 * u
+    """zero(eltype(u)) (problem.jl:123); u is a tuple for the problems of an FVMSystem."""
+    return 0.0 if isinstance(u, (tuple, list)) else 0.0 * u
 
 
 class AbstractFVMProblem:

@@ -15,6 +15,7 @@ import numpy as np
 
 from . import _lib
 from .problem import FVMProblem, SteadyFVMProblem
+from .system import FVMSystem
 
 # Tsitouras (2011) 5(4) pair: nodes, stage weights (the 7th stage is the
 # solution, first-same-as-last) and the error weights b - bhat.
@@ -106,6 +107,15 @@ def solve(prob, alg=None, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None, max
         return prob.solve(alg, saveat=saveat, abstol=abstol, reltol=reltol, dt=dt, **kwargs)
     if isinstance(prob, SteadyFVMProblem):
         return solve_steady(prob, alg, abstol=abstol, reltol=reltol, **kwargs)
+    if isinstance(prob, FVMSystem):
+        ctx = prob.device_context()
+        cb = (lambda u, t: ctx.apply_dirichlet(u, t)) if prob.has_dirichlet_nodes() else None
+        sol = integrate(ctx.rhs, prob.flat(prob.initial_condition), prob.initial_time,
+                        prob.final_time, alg if alg is not None else Tsit5(), saveat=saveat,
+                        abstol=abstol, reltol=reltol, dt=dt, maxiters=maxiters,
+                        save_start=save_start, step_callback=cb, user_callback=callback)
+        sol.u = [prob.unflat(v) for v in sol.u]
+        return sol
     if not isinstance(prob, FVMProblem):
         raise TypeError(f"solve: unsupported problem type {type(prob).__name__}")
     ctx = prob.device_context()
@@ -266,19 +276,23 @@ def _initial_dt(f, u0, f0, t0, tf, abstol, reltol, n):
 
 def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kwargs):
     """SteadyFVMProblem: Newton's method on fvm_eqs!(du, u) = 0 with the device
-    Jacobian-vector product (fvm_jvp) inside restarted GMRES.  Dirichlet nodes are
+    Jacobian-vector product (fvm_jvp; a forward difference of fvm_system_rhs for an
+    FVMSystem) inside restarted GMRES.  Dirichlet nodes are
     set by the Dirichlet functions first and stay fixed (their du is 0)."""
     from .linsolve import gmres_device
 
     p = prob.problem
     ctx = p.device_context()
-    n = len(p.initial_condition)
-    u = _lib.DeviceArray.from_host(p.initial_condition)
-    if p.conditions.has_dirichlet_nodes():
+    sys = isinstance(p, FVMSystem)
+    u0 = p.flat(p.initial_condition) if sys else p.initial_condition
+    n = len(u0)
+    u = _lib.DeviceArray.from_host(u0)
+    if p.has_dirichlet_nodes():
         ctx.apply_dirichlet(u, p.final_time if np.isfinite(p.final_time) else 0.0)
     t = 0.0
     F = _lib.DeviceArray(n)
     du = _lib.DeviceArray(n)
+    work = tuple(_lib.DeviceArray(n) for _ in range(3)) if sys else None
     sol = ODESolution()
     ctx.rhs(u, F, t)
     r0 = math.sqrt(_lib.dot(F, F))
@@ -288,13 +302,16 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
         if rn <= abstol + reltol * r0:
             break
         it += 1
-        jv = lambda v, out: ctx.jvp(u, v, out, t)  # noqa: E731
+        if sys:
+            jv = lambda v, out: ctx.jvp(u, v, out, t, _work=work)  # noqa: E731
+        else:
+            jv = lambda v, out: ctx.jvp(u, v, out, t)  # noqa: E731
         gmres_device(jv, F, du, rtol=1e-3, restart=50, maxiter=2000)
         _lib.lincomb(u, (1.0, -1.0), (u, du))
         ctx.rhs(u, F, t)
         rn = math.sqrt(_lib.dot(F, F))
     sol.t = [math.inf]
-    sol.u = [u.to_host()]
+    sol.u = [p.unflat(u.to_host()) if sys else u.to_host()]
     sol.retcode = "Success" if rn <= abstol + reltol * r0 else "MaxIters"
     sol.stats = dict(newton_iters=it, residual=rn)
     return sol

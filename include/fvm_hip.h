@@ -33,6 +33,7 @@ typedef struct fvm_mesh fvm_mesh;
 typedef struct fvm_ctx fvm_ctx;
 typedef struct fvm_template fvm_template;
 typedef struct fvm_batch fvm_batch;
+typedef struct fvm_system fvm_system;
 
 /* ---- runtime ------------------------------------------------------------- */
 /* Error text of the calling thread's last failure (copied into buf, truncated). */
@@ -276,6 +277,50 @@ int fvm_batch_rhs(fvm_batch* b, const double* d_u, double* d_du, double t, fvm_s
 /* n_steps explicit Euler steps of every member from t0: u += dt_p fvm_eqs!(u), then the
  * Dirichlet callback (solve.jl:133-165) at the member's new time. */
 int fvm_batch_euler(fvm_batch* b, double* d_u, int32_t n_steps, double t0, fvm_stream s);
+
+/* ---- systems (FVMSystem, problem.jl:196-413) -------------------------------
+ * n_vars coupled variables on one mesh.  The state is the reference's matrix
+ * u[var, i] stored column-major: u[var + n_vars * i].  Device forms of the closures:
+ *   flux       q_v = -sum_w K[v][w] (alpha_w, beta_w)             (eval_flux_function for
+ *              a system, individual_flux_contributions.jl:36-50; K constant)
+ *   source     S_v = s_node[v][i] + sum over terms k of variable v of
+ *              coef_k prod_w u_w^exp_k[w] + st[v] t                 (source_contributions.jl:17-30)
+ *   condition  a = c0 + sum_w cu[w] u_w + ct t + cx x + cy y + g exp(lam t), g per location
+ * Condition functions are numbered in the merged order of merge_problem_conditions
+ * (problem.jl:368-400): variable v's function f is number f + (functions of vars < v). */
+typedef struct fvm_system_desc {
+  int32_t n_vars;                    /* 1..4 */
+  const double* flux_k;              /* [n_vars * n_vars] row-major K[v][w] */
+  int32_t n_terms;                   /* source monomials, at most 24 */
+  const int32_t* term_var;           /* [n_terms] the variable whose source holds the term */
+  const int32_t* term_exp;           /* [n_terms * n_vars] exponents 0..3 */
+  const double* term_coef;           /* [n_terms] */
+  const double* s_node;              /* [n_vars * n_points] var-major, or NULL */
+  const double* st;                  /* [n_vars] or NULL */
+  const int8_t* node_kind;           /* [n_vars * n_points] FVM_NODE_* per variable */
+  const int32_t* node_fidx;          /* [n_vars * n_points] merged function index */
+  const int32_t* bedge_neumann_fidx; /* [n_vars * n_boundary_edges], -1: not Neumann for the variable */
+  int32_t n_functions;
+  const double* fn_c0;               /* [n_functions] */
+  const double* fn_cu;               /* [n_functions * n_vars] */
+  const double* fn_ct;               /* [n_functions] or NULL (0) */
+  const double* fn_cx;               /* [n_functions] or NULL (0) */
+  const double* fn_cy;               /* [n_functions] or NULL (0) */
+  const double* fn_lam;              /* [n_functions] or NULL (0: g enters without a time factor) */
+  const double* node_g;              /* [n_vars * n_points] or NULL (0) */
+  const double* bedge_g;             /* [n_vars * 2 * n_boundary_edges], half-edge points m_i, m_j,
+                                        or NULL (0) */
+} fvm_system_desc;
+
+int fvm_system_create(const fvm_mesh* m, const fvm_system_desc* d, fvm_system** out);
+int fvm_system_destroy(fvm_system* s);
+/* fvm_eqs!(du, u, p, t) for an FVMSystem (main_equations.jl:28-81 with the system
+ * methods in src/equations/): du[var, i] for every variable, 0 at Dirichlet
+ * entries and at points that are not vertices. */
+int fvm_system_rhs(fvm_system* s, const double* d_u, double* d_du, double t, fvm_stream st);
+/* update_dirichlet_nodes! for a system (dirichlet.jl:46-53): variable by variable,
+ * u[var, i] = a(x_i, y_i, t, u[:, i]) with the variables already updated. */
+int fvm_system_dirichlet_apply(fvm_system* s, double* d_u, double t, fvm_stream st);
 
 /* ---- device vector operations (used by the integrators and solvers) ------
  * The reference broadcasts over Vector{Float64} inside OrdinaryDiffEq; device-

@@ -441,3 +441,121 @@ int64_t orc_template_assemble(const orc_mesh* m, const orc_conditions* c, int ki
     }
   return A.n;
 }
+
+/* ------------------------------------------------------------------------ */
+/* FVMSystem: serial fvm_eqs! over nv variables.                              */
+/* ------------------------------------------------------------------------ */
+#define ORC_MAXV 8
+
+/* get_shape_function_coefficients for a system (shape_functions.jl:12-19) */
+static void sys_shape(const orc_mesh* m, int nv, const double* u, int64_t T, double* al, double* be,
+                      double* ga) {
+  const int32_t i = m->tri[3 * T], j = m->tri[3 * T + 1], k = m->tri[3 * T + 2];
+  const double* P = m->props + ORC_PROPS * T;
+  for (int v = 0; v < nv; ++v) {
+    al[v] = P[0] * u[v + nv * i] + P[1] * u[v + nv * j] + P[2] * u[v + nv * k];
+    be[v] = P[3] * u[v + nv * i] + P[4] * u[v + nv * j] + P[5] * u[v + nv * k];
+    ga[v] = P[6] * u[v + nv * i] + P[7] * u[v + nv * j] + P[8] * u[v + nv * k];
+  }
+}
+
+static void sys_body(const orc_mesh* m, const orc_system* s, const double* u, double t, double* du,
+                     int scale) {
+  const int nv = s->nv;
+  const int64_t N = m->n_points;
+  for (int64_t i = 0; i < nv * N; ++i) du[i] = 0.0;
+  double al[ORC_MAXV], be[ORC_MAXV], ga[ORC_MAXV], qx[ORC_MAXV], qy[ORC_MAXV];
+  double Q[3][ORC_MAXV];
+  /* triangle_contributions.jl:10-43 (system update_du! at :18-25) */
+  for (int64_t T = 0; T < m->n_tri; ++T) {
+    const int32_t i = m->tri[3 * T], j = m->tri[3 * T + 1], k = m->tri[3 * T + 2];
+    const double* P = m->props + ORC_PROPS * T;
+    sys_shape(m, nv, u, T, al, be, ga);
+    for (int e = 0; e < 3; ++e) {
+      const double x = P[9 + 2 * e], y = P[10 + 2 * e], nx = P[15 + 2 * e], ny = P[16 + 2 * e];
+      const double ell = P[21 + e];
+      s->flux(x, y, t, al, be, ga, s->ctx, qx, qy);
+      for (int v = 0; v < nv; ++v) Q[e][v] = (qx[v] * nx + qy[v] * ny) * ell;
+    }
+    for (int v = 0; v < nv; ++v) {
+      if (scale) {
+        const double a1 = fabs(Q[0][v]), a2 = fabs(Q[1][v]), a3 = fabs(Q[2][v]);
+        du[v + nv * i] += a3 + a1;
+        du[v + nv * j] += a1 + a2;
+        du[v + nv * k] += a2 + a3;
+      } else {
+        du[v + nv * i] = du[v + nv * i] + Q[2][v] - Q[0][v];
+        du[v + nv * j] = du[v + nv * j] + Q[0][v] - Q[1][v];
+        du[v + nv * k] = du[v + nv * k] + Q[1][v] - Q[2][v];
+      }
+    }
+  }
+  /* boundary_edge_contributions.jl:15-72, 76-86 */
+  for (int64_t e = 0; e < m->n_bedges; ++e) {
+    const int32_t i = m->bedges[2 * e], j = m->bedges[2 * e + 1];
+    double nx, ny, mix, miy, mjx, mjy, ell;
+    bcv(m, e, &nx, &ny, &mix, &miy, &mjx, &mjy, &ell);
+    sys_shape(m, nv, u, m->badj[e], al, be, ga);
+    double ui[ORC_MAXV], uj[ORC_MAXV], q1[ORC_MAXV], q2[ORC_MAXV];
+    for (int v = 0; v < nv; ++v) {
+      ui[v] = al[v] * mix + be[v] * miy + ga[v];
+      uj[v] = al[v] * mjx + be[v] * mjy + ga[v];
+    }
+    double qxi[ORC_MAXV], qyi[ORC_MAXV], qxj[ORC_MAXV], qyj[ORC_MAXV];
+    s->flux(mix, miy, t, al, be, ga, s->ctx, qxi, qyi);
+    s->flux(mjx, mjy, t, al, be, ga, s->ctx, qxj, qyj);
+    for (int v = 0; v < nv; ++v) {
+      const int32_t f = s->bedge_neumann_fidx[v * m->n_bedges + e];
+      if (f < 0) {
+        q1[v] = (qxi[v] * nx + qyi[v] * ny) * ell;
+        q2[v] = (qxj[v] * nx + qyj[v] * ny) * ell;
+      } else {
+        q1[v] = s->cond(f, mix, miy, t, ui, s->ctx) * ell;
+        q2[v] = s->cond(f, mjx, mjy, t, uj, s->ctx) * ell;
+      }
+      if (scale) {
+        du[v + nv * i] += fabs(q1[v]);
+        du[v + nv * j] += fabs(q2[v]);
+      } else {
+        du[v + nv * i] = du[v + nv * i] - q1[v];
+        du[v + nv * j] = du[v + nv * j] - q2[v];
+      }
+    }
+  }
+  /* source_contributions.jl:17-30, 49-68 */
+  for (int64_t i = 0; i < N; ++i) {
+    if (!m->is_vertex[i]) {
+      for (int v = 0; v < nv; ++v) du[v + nv * i] = 0.0;
+      continue;
+    }
+    const double* ui = u + nv * i;
+    for (int v = 0; v < nv; ++v) {
+      const int kd = s->node_kind[v * N + i];
+      double S;
+      if (kd == 0) S = s->source(v, m->x[i], m->y[i], t, ui, s->ctx);
+      else if (kd == 1) S = 0.0;
+      else S = s->cond(s->node_fidx[v * N + i], m->x[i], m->y[i], t, ui, s->ctx);
+      if (scale) du[v + nv * i] = kd == 0 ? du[v + nv * i] / m->cv[i] + fabs(S) : fabs(S);
+      else du[v + nv * i] = kd == 0 ? du[v + nv * i] / m->cv[i] + S : S;
+    }
+  }
+}
+
+void orc_fvm_eqs_system(const orc_mesh* m, const orc_system* s, const double* u, double t,
+                        double* du) {
+  sys_body(m, s, u, t, du, 0);
+}
+
+void orc_fvm_scale_system(const orc_mesh* m, const orc_system* s, const double* u, double t,
+                          double* sc) {
+  sys_body(m, s, u, t, sc, 1);
+}
+
+void orc_update_dirichlet_system(const orc_mesh* m, const orc_system* s, double t, double* u) {
+  const int nv = s->nv;
+  const int64_t N = m->n_points;
+  for (int v = 0; v < nv; ++v)
+    for (int64_t i = 0; i < N; ++i)
+      if (s->node_kind[v * N + i] == 1)
+        u[v + nv * i] = s->cond(s->node_fidx[v * N + i], m->x[i], m->y[i], t, u + nv * i, s->ctx);
+}

@@ -91,6 +91,45 @@ void orc_fvm_scale(const orc_mesh* m, const orc_conditions* c, const orc_physics
 void orc_update_dirichlet(const orc_mesh* m, const orc_conditions* c, const orc_physics* ph,
                           double t, double* u);
 
+/* ---- FVMSystem (problem.jl:196-413) ----
+ * nv variables on one mesh; u and du are [n_points][nv] (du[var, i] of the reference,
+ * column-major, at var + nv*i).  Per-variable conditions: node_kind / node_fidx are
+ * [nv][n_points], bedge_neumann_fidx is [nv][n_bedges]; function indices are already
+ * mapped into the merged function list (map_fidx, problem.jl:322).
+ * flux(x, y, t, alpha[nv], beta[nv], gamma[nv]) returns every variable's flux
+ * (eval_flux_function over the tuple of flux functions, problem.jl:354-357);
+ * source(var, x, y, t, u[nv]) and cond(fidx, x, y, t, u[nv]) take the vector of
+ * every variable's value. */
+typedef void (*orc_sys_flux_fn)(double x, double y, double t, const double* a, const double* b,
+                                const double* g, void* ctx, double* qx, double* qy);
+typedef double (*orc_sys_source_fn)(int32_t var, double x, double y, double t, const double* u,
+                                    void* ctx);
+typedef double (*orc_sys_cond_fn)(int32_t fidx, double x, double y, double t, const double* u,
+                                  void* ctx);
+typedef struct orc_system {
+  int32_t nv;
+  const int8_t* node_kind;
+  const int32_t* node_fidx;
+  const int32_t* bedge_neumann_fidx;
+  orc_sys_flux_fn flux;
+  orc_sys_source_fn source;
+  orc_sys_cond_fn cond;
+  void* ctx;
+} orc_system;
+
+/* serial_fvm_eqs! for an FVMSystem: triangles (triangle_contributions.jl:18-35 and the
+ * system get_flux, individual_flux_contributions.jl:36-50), boundary edges
+ * (boundary_edge_contributions.jl:15-72, 76-86), sources
+ * (source_contributions.jl:17-30, 49-68).  Requires nv <= 8. */
+void orc_fvm_eqs_system(const orc_mesh* m, const orc_system* s, const double* u, double t,
+                        double* du);
+/* Per (node, variable) magnitude scale sum |Q| / V + |S| of orc_fvm_eqs_system. */
+void orc_fvm_scale_system(const orc_mesh* m, const orc_system* s, const double* u, double t,
+                          double* sc);
+/* serial_update_dirichlet_nodes! for a system (dirichlet.jl:46-53): variable by
+ * variable, u[var, i] = a(x_i, y_i, t, u[:, i]). */
+void orc_update_dirichlet_system(const orc_mesh* m, const orc_system* s, double t, double* u);
+
 /* ---- built-in closures (C versions of the closures used by the workloads) ---- */
 enum {
   ORC_FLUX_POLY_U = 0, /* q = -(d0 + d1 u + d2 u^2 + d3 u^3) (alpha, beta), u = alpha x + beta y + gamma */

@@ -33,6 +33,12 @@ SOURCE_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_double, 
 COND_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                            ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
 DXY_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
+SYS_FLUX_FN = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp,
+                               ctypes.c_void_p, _dp, _dp)
+SYS_SOURCE_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, _dp, ctypes.c_void_p)
+SYS_COND_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, _dp, ctypes.c_void_p)
 
 
 class _Mesh(ctypes.Structure):
@@ -48,6 +54,12 @@ class _Cond(ctypes.Structure):
 
 class _Phys(ctypes.Structure):
     _fields_ = [("flux", ctypes.c_void_p), ("source", ctypes.c_void_p),
+                ("cond", ctypes.c_void_p), ("ctx", ctypes.c_void_p)]
+
+
+class _System(ctypes.Structure):
+    _fields_ = [("nv", ctypes.c_int32), ("node_kind", _i8p), ("node_fidx", _i32p),
+                ("bedge_neumann_fidx", _i32p), ("flux", ctypes.c_void_p), ("source", ctypes.c_void_p),
                 ("cond", ctypes.c_void_p), ("ctx", ctypes.c_void_p)]
 
 
@@ -82,6 +94,13 @@ def lib():
         L.orc_update_dirichlet.argtypes = [ctypes.POINTER(_Mesh), ctypes.POINTER(_Cond),
                                            ctypes.POINTER(_Phys), ctypes.c_double, _dp]
         L.orc_update_dirichlet.restype = None
+        for name in ("orc_fvm_eqs_system", "orc_fvm_scale_system"):
+            f = getattr(L, name)
+            f.argtypes = [ctypes.POINTER(_Mesh), ctypes.POINTER(_System), _dp, ctypes.c_double, _dp]
+            f.restype = None
+        L.orc_update_dirichlet_system.argtypes = [ctypes.POINTER(_Mesh), ctypes.POINTER(_System),
+                                                  ctypes.c_double, _dp]
+        L.orc_update_dirichlet_system.restype = None
         L.orc_fvm_eqs_serial_cs.argtypes = [ctypes.POINTER(_Mesh), ctypes.POINTER(_Cond),
                                             ctypes.POINTER(_Builtin), _dp, _dp, ctypes.c_double, _dp]
         L.orc_fvm_eqs_serial_cs.restype = None
@@ -327,3 +346,67 @@ class OracleProblem:
         b = np.empty(N)
         L.orc_template_assemble(*args, _p(rows, _i64p), _p(cols, _i64p), _p(vals, _dp), _p(b, _dp))
         return rows, cols, vals, b
+
+
+class OracleSystem:
+    """An FVMSystem (problem.jl:196-413) evaluated by the C restatement.
+
+    node_kind / node_fidx: [nv, n_points]; bedge_neumann_fidx: [nv, n_bedges] (function
+    indices into the merged list); flux(x, y, t, a, b, g) -> (qx[nv], qy[nv]) with
+    a, b, g arrays over the variables; source(var, x, y, t, u) and cond(fidx, x, y, t, u)
+    with u the vector of every variable's value.  States are [n_points, nv] (C order),
+    the reference's du[var, i] layout."""
+
+    def __init__(self, x, y, tri, bedges, badj, node_kind, node_fidx, bedge_neumann_fidx, flux,
+                 source, cond):
+        base = OracleProblem(x, y, tri, bedges, badj, np.zeros(len(x), np.int8),
+                             np.full(len(x), -1, np.int32), np.full(len(np.asarray(bedges).reshape(-1, 2)), -1, np.int32),
+                             dict(flux=lambda *a: (0.0, 0.0)))
+        self.base = base
+        self.nv = nv = int(np.asarray(node_kind).shape[0])
+        if nv > 8:
+            raise ValueError("at most 8 variables")
+        self.kind = np.ascontiguousarray(node_kind, np.int8)
+        self.fidx = np.ascontiguousarray(node_fidx, np.int32)
+        self.bneu = np.ascontiguousarray(bedge_neumann_fidx, np.int32)
+
+        def _flux(x, y, t, a, b, g, ctx, qx, qy):
+            av = np.ctypeslib.as_array(a, (nv,)).copy()
+            bv = np.ctypeslib.as_array(b, (nv,)).copy()
+            gv = np.ctypeslib.as_array(g, (nv,)).copy()
+            fx, fy = flux(x, y, t, av, bv, gv)
+            for v in range(nv):
+                qx[v] = fx[v]
+                qy[v] = fy[v]
+
+        def _source(var, x, y, t, u, ctx):
+            return float(source(var, x, y, t, np.ctypeslib.as_array(u, (nv,)).copy()))
+
+        def _cond(f, x, y, t, u, ctx):
+            return float(cond(f, x, y, t, np.ctypeslib.as_array(u, (nv,)).copy()))
+
+        self._cbs = (SYS_FLUX_FN(_flux), SYS_SOURCE_FN(_source), SYS_COND_FN(_cond))
+        self._sys = _System(nv, _p(self.kind, _i8p), _p(self.fidx, _i32p), _p(self.bneu, _i32p),
+                            ctypes.cast(self._cbs[0], ctypes.c_void_p).value,
+                            ctypes.cast(self._cbs[1], ctypes.c_void_p).value,
+                            ctypes.cast(self._cbs[2], ctypes.c_void_p).value, None)
+
+    def rhs(self, u, t=0.0):
+        u = np.ascontiguousarray(u, np.float64).reshape(-1, self.nv)
+        du = np.zeros_like(u)
+        lib().orc_fvm_eqs_system(ctypes.byref(self.base._mesh), ctypes.byref(self._sys), _p(u, _dp), t,
+                                 _p(du, _dp))
+        return du
+
+    def scale(self, u, t=0.0):
+        u = np.ascontiguousarray(u, np.float64).reshape(-1, self.nv)
+        sc = np.zeros_like(u)
+        lib().orc_fvm_scale_system(ctypes.byref(self.base._mesh), ctypes.byref(self._sys), _p(u, _dp), t,
+                                   _p(sc, _dp))
+        return sc
+
+    def update_dirichlet(self, u, t=0.0):
+        u = np.array(u, np.float64, order="C").reshape(-1, self.nv)
+        lib().orc_update_dirichlet_system(ctypes.byref(self.base._mesh), ctypes.byref(self._sys), t,
+                                          _p(u, _dp))
+        return u

@@ -8,7 +8,7 @@ a nodal log-diffusivity field, D(x, y) = exp(0.5 sin 4 pi x cos 4 pi y))."""
 import numpy as np
 
 import finitevolumemethod as FVM
-from oracle import OracleProblem
+from oracle import OracleProblem, OracleSystem, flatten_conditions, merge_conditions
 
 
 def oracle_for(prob, physics, tri=None):
@@ -166,3 +166,96 @@ def template_oracle(prob):
     kind, fidx, bneu = prob.conditions.flatten(tri)
     return OracleProblem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
                          kind, fidx, bneu, dict(flux=lambda *a: (0.0, 0.0)))
+
+
+def system_conditions(tri, per_var_types):
+    """Flattened per-variable conditions of an FVMSystem (merge_problem_conditions,
+    problem.jl:378-410): variable v's functions follow those of variables < v."""
+    N = tri.num_points()
+    nv = len(per_var_types)
+    kind = np.zeros((nv, N), np.int8)
+    fidx = np.full((nv, N), -1, np.int32)
+    bn = np.full((nv, len(tri.boundary_edges)), -1, np.int32)
+    off = 0
+    for v, types in enumerate(per_var_types):
+        ne, ce, dn, dd = merge_conditions(tri.boundary_edges, tri.boundary_segments, types)
+        k, f, b = flatten_conditions(N, tri.boundary_edges, ne, dn, dd)
+        kind[v] = k
+        fidx[v] = np.where(k != 0, f + off, -1)
+        bn[v] = np.where(b >= 0, b + off, -1)
+        off += len(types)
+    return kind, fidx, bn
+
+
+def brusselator_oracle(tri):
+    """The Brusselator tutorial system (reaction_diffusion_brusselator_system_of_pdes.jl):
+    exact solution Phi = exp(-x - y - t/2), Psi = exp(x + y + t/2)."""
+    kind, fidx, bn = system_conditions(tri, [["Neumann", "Neumann", "Dirichlet", "Neumann"],
+                                           ["Dirichlet", "Neumann", "Neumann", "Dirichlet"]])
+    fs = [lambda x, y, t: -0.25 * np.exp(-x - t / 2), lambda x, y, t: 0.25 * np.exp(-1 - y - t / 2),
+          lambda x, y, t: np.exp(-1 - x - t / 2), lambda x, y, t: -0.25 * np.exp(-y - t / 2),
+          lambda x, y, t: np.exp(x + t / 2), lambda x, y, t: -0.25 * np.exp(1 + y + t / 2),
+          lambda x, y, t: -0.25 * np.exp(1 + x + t / 2), lambda x, y, t: np.exp(y + t / 2)]
+    return OracleSystem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                        kind, fidx, bn,
+                        lambda x, y, t, a, b, g: (np.array([-a[0] / 4, -a[1] / 4]), np.array([-b[0] / 4, -b[1] / 4])),
+                        lambda v, x, y, t, u: u[0] ** 2 * u[1] - 2 * u[0] if v == 0 else -u[0] ** 2 * u[1] + u[0],
+                        lambda f, x, y, t, u: fs[f](x, y, t))
+
+
+BRUSSELATOR_PHI = (lambda x, y, t, u, p: -1 / 4 * np.exp(-x - t / 2),
+                   lambda x, y, t, u, p: 1 / 4 * np.exp(-1 - y - t / 2),
+                   lambda x, y, t, u, p: np.exp(-1 - x - t / 2),
+                   lambda x, y, t, u, p: -1 / 4 * np.exp(-y - t / 2))
+BRUSSELATOR_PSI = (lambda x, y, t, u, p: np.exp(x + t / 2),
+                   lambda x, y, t, u, p: -1 / 4 * np.exp(1 + y + t / 2),
+                   lambda x, y, t, u, p: -1 / 4 * np.exp(1 + x + t / 2),
+                   lambda x, y, t, u, p: np.exp(y + t / 2))
+
+
+def brusselator_system(tri, final_time=5.0, tile_triangles=0):
+    """The Brusselator tutorial's FVMSystem, built through the public API."""
+    mesh = FVM.FVMGeometry(tri, tile_triangles=tile_triangles)
+    phi_BCs = FVM.BoundaryConditions(mesh, BRUSSELATOR_PHI,
+                                     (FVM.Neumann, FVM.Neumann, FVM.Dirichlet, FVM.Neumann))
+    psi_BCs = FVM.BoundaryConditions(mesh, BRUSSELATOR_PSI,
+                                     (FVM.Dirichlet, FVM.Neumann, FVM.Neumann, FVM.Dirichlet))
+    phi = FVM.FVMProblem(mesh, phi_BCs, flux_function=lambda x, y, t, a, b, g, p: (-a[0] / 4, -b[0] / 4),
+                         source_function=lambda x, y, t, u, p: u[0] ** 2 * u[1] - 2 * u[0],
+                         initial_condition=np.exp(-tri.x - tri.y), final_time=final_time)
+    psi = FVM.FVMProblem(mesh, psi_BCs, flux_function=lambda x, y, t, a, b, g, p: (-a[1] / 4, -b[1] / 4),
+                         source_function=lambda x, y, t, u, p: -u[0] ** 2 * u[1] + u[0],
+                         initial_condition=np.exp(tri.x + tri.y), final_time=final_time)
+    return FVM.FVMSystem(phi, psi)
+
+
+def gray_scott_system(n=200, final_time=6000.0, tile_triangles=0):
+    """The Gray-Scott tutorial's FVMSystem ([-1, 1]^2, n x n grid, zero-flux boundary)."""
+    tri = FVM.triangulate_rectangle(-1.0, 1.0, -1.0, 1.0, n, n, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri, tile_triangles=tile_triangles)
+    bc = lambda x, y, t, u, p: 0.0 * u[0] * 0.0 * u[1]  # noqa: E731
+    r2 = tri.x ** 2 + tri.y ** 2
+    u_prob = FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, bc, FVM.Neumann),
+                            flux_function=lambda x, y, t, a, b, g, e: (-a[0] * e, -b[0] * e),
+                            flux_parameters=0.00002,
+                            source_function=lambda x, y, t, u, b: b * (1 - u[0]) - u[0] * u[1] ** 2,
+                            source_parameters=0.04, initial_condition=1 - np.exp(-80 * r2),
+                            final_time=final_time)
+    v_prob = FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, bc, FVM.Neumann),
+                            flux_function=lambda x, y, t, a, b, g, e: (-a[1] * e, -b[1] * e),
+                            flux_parameters=0.00001,
+                            source_function=lambda x, y, t, u, d: -d * u[1] + u[0] * u[1] ** 2,
+                            source_parameters=0.1, initial_condition=np.exp(-80 * r2),
+                            final_time=final_time)
+    return FVM.FVMSystem(u_prob, v_prob)
+
+
+def gray_scott_oracle(tri):
+    kind, fidx, bn = system_conditions(tri, [["Neumann"], ["Neumann"]])
+    return OracleSystem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                        kind, fidx, bn,
+                        lambda x, y, t, a, b, g: (np.array([-a[0] * 0.00002, -a[1] * 0.00001]),
+                                                  np.array([-b[0] * 0.00002, -b[1] * 0.00001])),
+                        lambda v, x, y, t, u: (0.04 * (1 - u[0]) - u[0] * u[1] ** 2 if v == 0
+                                               else -0.1 * u[1] + u[0] * u[1] ** 2),
+                        lambda f, x, y, t, u: 0.0)

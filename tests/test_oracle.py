@@ -25,8 +25,8 @@ import scipy.sparse.linalg as spla
 
 import finitevolumemethod as FVM
 from brute import control_volumes, dudt as brute_dudt
-from oracle import OracleProblem, flatten_conditions, geometry, merge_conditions
-from problems import disk_triangulation
+from oracle import OracleProblem, OracleSystem, flatten_conditions, geometry, merge_conditions
+from problems import brusselator_oracle, disk_triangulation, system_conditions
 
 
 def _disk(nr=12, R=1.0):
@@ -323,3 +323,99 @@ def test_template_boundary_row_volume_quirk():
         flux_j = 2.0 * (abc[0] * n[0] + abc[1] * n[1]) * ell  # D grad u . n at m_j times length
         expected = flux_j * (1 / orc.cv[i] - 1 / orc.cv[j])
         assert diff[j] == pytest.approx(expected, rel=1e-9, abs=1e-12 * abs(flux_j / orc.cv[j]))
+
+
+# ---------------------------------------------------------------------------- systems
+def test_single_variable_system_equals_scalar_problem():
+    """An FVMSystem of one variable evaluates exactly as the FVMProblem (the system
+    methods of equations/*.jl reduce to the scalar ones)."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 13, 11, jitter=0.2, seed=2, single_boundary=False)
+    types = ["Dirichlet", "Neumann", "Dudt", "Neumann"]
+    kind, fidx, bn = system_conditions(tri, [types])
+    conds = [lambda x, y, t, u: x + u, lambda x, y, t, u: 0.3 * u - t, lambda x, y, t, u: 2.0 * u,
+             lambda x, y, t, u: y - 1.0]
+    flux = lambda x, y, t, a, b, g: (-(1 + (a * x + b * y + g) ** 2) * a, -(1 + (a * x + b * y + g) ** 2) * b)  # noqa: E731
+    src = lambda x, y, t, u: u * (1 - u)  # noqa: E731
+    orc = OracleProblem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                        kind[0], fidx[0], bn[0],
+                        dict(flux=lambda x, y, t, a, b, g, T: flux(x, y, t, a, b, g),
+                             source=lambda x, y, t, u, n: src(x, y, t, u),
+                             cond=lambda f, x, y, t, u: conds[f](x, y, t, u)))
+    sysorc = OracleSystem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                          kind, fidx, bn,
+                          lambda x, y, t, a, b, g: tuple(np.array([q]) for q in flux(x, y, t, a[0], b[0], g[0])),
+                          lambda v, x, y, t, u: src(x, y, t, u[0]),
+                          lambda f, x, y, t, u: conds[f](x, y, t, u[0]))
+    u = np.random.default_rng(4).random(tri.num_points())
+    assert np.array_equal(sysorc.rhs(u[:, None], 0.4)[:, 0], orc.rhs(u, 0.4))
+    assert np.array_equal(sysorc.update_dirichlet(u[:, None], 0.4)[:, 0], orc.update_dirichlet(u, 0.4))
+
+
+def test_decoupled_system_equals_two_problems():
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 12, 14, jitter=0.2, seed=5, single_boundary=False)
+    t1 = ["Dirichlet", "Neumann", "Neumann", "Neumann"]
+    t2 = ["Neumann", "Dirichlet", "Dudt", "Neumann"]
+    kind, fidx, bn = system_conditions(tri, [t1, t2])
+    c1 = [lambda x, y, t, u: 1.0 + x, lambda x, y, t, u: 0.5, lambda x, y, t, u: -0.25 * u, lambda x, y, t, u: y]
+    c2 = [lambda x, y, t, u: 0.2 * u, lambda x, y, t, u: 2.0, lambda x, y, t, u: t - u, lambda x, y, t, u: -x]
+    allc = c1 + c2
+    sysorc = OracleSystem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                          kind, fidx, bn,
+                          lambda x, y, t, a, b, g: (np.array([-0.5 * a[0], -(2 + x) * a[1]]),
+                                                    np.array([-0.5 * b[0], -(2 + x) * b[1]])),
+                          lambda v, x, y, t, u: u[0] - u[0] ** 2 if v == 0 else 0.3 - u[1],
+                          lambda f, x, y, t, u: allc[f](x, y, t, u[0] if f < 4 else u[1]))
+    u = np.random.default_rng(6).random((tri.num_points(), 2))
+    du = sysorc.rhs(u, 0.7)
+    for v, (cs, flux, src) in enumerate([(c1, lambda x, a: -0.5 * a, lambda uu: uu - uu ** 2),
+                                         (c2, lambda x, a: -(2 + x) * a, lambda uu: 0.3 - uu)]):
+        orc = OracleProblem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                            kind[v], np.where(kind[v] != 0, fidx[v] - 4 * v, -1),
+                            np.where(bn[v] >= 0, bn[v] - 4 * v, -1),
+                            dict(flux=lambda x, y, t, a, b, g, T, flux=flux: (flux(x, a), flux(x, b)),
+                                 source=lambda x, y, t, uu, n, src=src: src(uu),
+                                 cond=lambda f, x, y, t, uu, cs=cs: cs[f](x, y, t, uu)))
+        assert np.array_equal(du[:, v], orc.rhs(u[:, v], 0.7))
+
+
+def test_system_dirichlet_update_is_sequential_over_variables():
+    """dirichlet.jl:46-53 updates variable 1 at every node, then variable 2 with the
+    new values of variable 1 in u[:, i]."""
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 6, 6, single_boundary=True)
+    kind, fidx, bn = system_conditions(tri, [["Dirichlet"], ["Dirichlet"]])
+    sysorc = OracleSystem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                          kind, fidx, bn, lambda x, y, t, a, b, g: (-a, -b), lambda v, x, y, t, u: 0.0,
+                          lambda f, x, y, t, u: 2.0 if f == 0 else u[0] + 1.0)
+    u = sysorc.update_dirichlet(np.zeros((tri.num_points(), 2)), 0.0)
+    bnodes = tri.boundary_nodes()
+    assert np.all(u[bnodes, 0] == 2.0) and np.all(u[bnodes, 1] == 3.0)
+
+
+def test_brusselator_rhs_matches_exact_time_derivative():
+    """At the exact solution, fvm_eqs! equals d/dt of it (-Phi/2, Psi/2) at the free
+    nodes, to the discretisation error of a 60 x 60 grid."""
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 60, 60, single_boundary=False)
+    orc = brusselator_oracle(tri)
+    t = 0.3
+    u = np.stack([np.exp(-tri.x - tri.y - t / 2), np.exp(tri.x + tri.y + t / 2)], 1)
+    du = orc.rhs(u, t)
+    dexact = np.stack([-u[:, 0] / 2, u[:, 1] / 2], 1)
+    free = orc.kind.T == 0
+    err = np.abs(du - dexact)[free]
+    assert np.linalg.norm(err) <= 2e-2 * np.linalg.norm(dexact[free])
+    # the Dirichlet rows are zero, and the update puts the exact boundary values in
+    assert np.all(du[orc.kind.T == 1] == 0.0)
+    z = orc.update_dirichlet(np.zeros_like(u), t)
+    d = orc.kind.T == 1
+    np.testing.assert_allclose(z[d], u[d], rtol=1e-15)
+
+
+def test_system_wrapper_reads_transposed_states():
+    """States handed over as views of (nv, N) matrices give what C-ordered copies give."""
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 7, 6, single_boundary=False)
+    orc = brusselator_oracle(tri)
+    U = np.random.default_rng(3).random((2, tri.num_points())) + 0.5
+    for t in (0.0, 0.4):
+        assert np.array_equal(orc.update_dirichlet(U.T, t), orc.update_dirichlet(U.T.copy(), t))
+        assert np.array_equal(orc.rhs(U.T, t), orc.rhs(U.T.copy(), t))
+        assert np.array_equal(orc.scale(U.T, t), orc.scale(U.T.copy(), t))
