@@ -142,6 +142,46 @@ __device__ __forceinline__ double eval_cond(const CondTable& C, int f, double x,
   return v + g;
 }
 
+// Runs launch(d_flag) on stream st with a cleared device flag, waits, and returns
+// FVM_ERR_ARG ("<what>: a queried edge or triangle is not in the mesh") when a
+// kernel set it (the pointwise query kernels).
+template <class Launch>
+int run_flagged(hipStream_t st, const char* what, Launch launch) {
+  int32_t* d_bad = nullptr;
+  FVM_HIP(hipMalloc((void**)&d_bad, sizeof(int32_t)));
+  int32_t bad = 0;
+  hipError_t e = hipMemcpyAsync(d_bad, &bad, sizeof(int32_t), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    launch(d_bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d_bad);
+  if (e != hipSuccess) return hip_fail(e, what);
+  if (bad) {
+    set_error(std::string(what) + ": a queried edge or triangle is not in the mesh");
+    return FVM_ERR_ARG;
+  }
+  return FVM_OK;
+}
+
+// Sorted index of the triangle whose counter-clockwise vertex order holds the
+// directed edge a -> b, or -1: scans the triangles incident to a (node_tri entries
+// are (sorted triangle << 2) | slot of a).
+__device__ __forceinline__ int32_t find_directed_edge(const int32_t* __restrict__ node_tri_start,
+                                                      const int32_t* __restrict__ node_tri,
+                                                      const int32_t* __restrict__ tri, int64_t T,
+                                                      int32_t a, int32_t b) {
+  const int32_t k1 = node_tri_start[a + 1];
+  for (int32_t k = node_tri_start[a]; k < k1; ++k) {
+    const int32_t w = node_tri[k];
+    const int32_t t = w >> 2, s = w & 3;
+    if (tri[(int64_t)(s == 2 ? 0 : s + 1) * T + t] == b) return t;
+  }
+  return -1;
+}
+
 }  // namespace fvm
 
 // Device-resident mesh: geometry SoA in tile (sorted) triangle order plus the

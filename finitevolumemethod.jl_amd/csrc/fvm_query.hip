@@ -1,0 +1,205 @@
+// Pointwise queries of a solution on the mesh, batched: compute_flux (the normal
+// flux across mesh edges) and pl_interpolate (the piecewise-linear interpolant).
+//
+// Reference: compute_flux, src/problem.jl:458-487, and pl_interpolate,
+// src/utils.jl:23-27 (each takes one edge / one point).  Here one thread answers
+// one query; the triangle a query needs is found on the device from the node ->
+// triangle incidence of the mesh (find_directed_edge) in place of the reference's
+// get_adjacent and triangle_props Dict lookups.  Unknown edges or triangles set a
+// flag that the host turns into an error, as the reference's lookups throw.
+#include <cmath>
+
+#include "fvm_internal.hpp"
+
+using namespace fvm;
+
+namespace {
+
+struct FluxQuery {
+  int64_t n, T, N;
+  const int32_t* qi;
+  const int32_t* qj;
+  const int32_t* node_tri_start;
+  const int32_t* node_tri;
+  const int32_t* tri;
+  const double* x;
+  const double* y;
+  const double* props;
+  const double* u;
+  const double* log_d;
+  const double* dq;  // FVM_FLUX_D_EDGE: D at each query's edge midpoint
+  int32_t kind;
+  double d0, d1, d2, d3;
+  double* out;
+  int32_t* bad;
+};
+
+// compute_flux (problem.jl:458-487): the triangle on the right of (i, j) (holding
+// the directed edge j -> i), or, when (i, j) is a boundary edge with the domain on
+// its left, the triangle holding i -> j; q at the edge midpoint from that
+// triangle's shape function; the result q . n with n the clockwise normal of i -> j.
+__global__ __launch_bounds__(256) void flux_query_kernel(FluxQuery a) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.n) return;
+  const int32_t i = a.qi[q], j = a.qj[q];
+  int32_t t = -1;
+  if (i >= 0 && i < a.N && j >= 0 && j < a.N) {
+    t = find_directed_edge(a.node_tri_start, a.node_tri, a.tri, a.T, j, i);
+    if (t < 0) t = find_directed_edge(a.node_tri_start, a.node_tri, a.tri, a.T, i, j);
+  }
+  if (t < 0) {
+    a.out[q] = NAN;
+    a.bad[0] = 1;
+    return;
+  }
+  const double px = a.x[i], py = a.y[i], qx = a.x[j], qy = a.y[j];
+  const double ex = qx - px, ey = qy - py;
+  const double l = sqrt(ex * ex + ey * ey);
+  const double nx = ey / l, ny = -ex / l;
+  const int32_t va = a.tri[t], vb = a.tri[a.T + t], vc = a.tri[2 * a.T + t];
+  double s[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) s[f] = a.props[f * a.T + t];
+  const double ua = a.u[va], ub = a.u[vb], uc = a.u[vc];
+  const double al = s[0] * ua + s[1] * ub + s[2] * uc;
+  const double be = s[3] * ua + s[4] * ub + s[5] * uc;
+  const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
+  const double mx = (px + qx) / 2, my = (py + qy) / 2;
+  double D;
+  if (a.kind == FVM_FLUX_D_POLY_U) {
+    const double um = al * mx + be * my + ga;
+    D = a.d0 + um * (a.d1 + um * (a.d2 + um * a.d3));
+  } else if (a.kind == FVM_FLUX_D_NODAL) {
+    const double Da = exp(a.log_d[va]), Db = exp(a.log_d[vb]), Dc = exp(a.log_d[vc]);
+    const double da = s[0] * Da + s[1] * Db + s[2] * Dc;
+    const double db = s[3] * Da + s[4] * Db + s[5] * Dc;
+    const double dg = s[6] * Da + s[7] * Db + s[8] * Dc;
+    D = da * mx + db * my + dg;
+  } else {
+    D = a.dq[q];
+  }
+  const double fx = -D * al, fy = -D * be;
+  a.out[q] = nx * fx + ny * fy;
+}
+
+struct InterpQuery {
+  int64_t n, T, N;
+  int32_t nv;
+  const int32_t* qtri;
+  const double* px;
+  const double* py;
+  const int32_t* node_tri_start;
+  const int32_t* node_tri;
+  const int32_t* tri;
+  const double* props;
+  const double* u;
+  double* out;
+  int32_t* bad;
+};
+
+// pl_interpolate (utils.jl:23-27): the shape function of the mesh triangle given as
+// (i, j, k) in any rotation (the stored rotation's coefficients, as
+// _safe_get_triangle_props finds them), evaluated at (x, y) for every variable.
+__global__ __launch_bounds__(256) void interp_query_kernel(InterpQuery a) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.n) return;
+  const int32_t i = a.qtri[3 * q], j = a.qtri[3 * q + 1], k = a.qtri[3 * q + 2];
+  const bool inside = i >= 0 && i < a.N && j >= 0 && j < a.N;
+  const int32_t t = inside ? find_directed_edge(a.node_tri_start, a.node_tri, a.tri, a.T, i, j) : -1;
+  bool ok = t >= 0;
+  int32_t va = 0, vb = 0, vc = 0;
+  if (ok) {
+    va = a.tri[t];
+    vb = a.tri[a.T + t];
+    vc = a.tri[2 * a.T + t];
+    ok = (int64_t)va + vb + vc - i - j == k;  // the third vertex
+  }
+  if (!ok) {
+    for (int v = 0; v < a.nv; ++v) a.out[q * a.nv + v] = NAN;
+    a.bad[0] = 1;
+    return;
+  }
+  double s[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) s[f] = a.props[f * a.T + t];
+  const double x = a.px[q], y = a.py[q];
+  for (int v = 0; v < a.nv; ++v) {
+    const double ua = a.u[v + (int64_t)a.nv * va], ub = a.u[v + (int64_t)a.nv * vb],
+                 uc = a.u[v + (int64_t)a.nv * vc];
+    const double al = s[0] * ua + s[1] * ub + s[2] * uc;
+    const double be = s[3] * ua + s[4] * ub + s[5] * uc;
+    const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
+    a.out[q * a.nv + v] = al * x + be * y + ga;
+  }
+}
+
+}  // namespace
+
+
+extern "C" {
+
+int fvm_compute_flux(fvm_ctx* c, int64_t n, const int32_t* d_i, const int32_t* d_j, const double* d_u,
+                     double t, const double* d_dq, double* d_out, fvm_stream s) {
+  (void)t;  // the traced flux forms do not depend on t
+  FVM_CHECK(c && n >= 0, "fvm_compute_flux: bad argument");
+  if (n == 0) return FVM_OK;
+  FVM_CHECK(d_i && d_j && d_u && d_out, "fvm_compute_flux: null argument");
+  FVM_CHECK(c->flux_kind != FVM_FLUX_D_EDGE || d_dq,
+            "fvm_compute_flux: FVM_FLUX_D_EDGE needs D at the edge midpoints (d_dq)");
+  const fvm_mesh* m = c->m;
+  FluxQuery a;
+  a.n = n;
+  a.T = m->T;
+  a.N = m->N;
+  a.qi = d_i;
+  a.qj = d_j;
+  a.node_tri_start = m->node_tri_start;
+  a.node_tri = m->node_tri;
+  a.tri = m->tri;
+  a.x = m->x;
+  a.y = m->y;
+  a.props = m->props;
+  a.u = d_u;
+  a.log_d = c->log_d;
+  a.dq = d_dq;
+  a.kind = c->flux_kind;
+  a.d0 = c->dpoly[0];
+  a.d1 = c->dpoly[1];
+  a.d2 = c->dpoly[2];
+  a.d3 = c->dpoly[3];
+  a.out = d_out;
+  hipStream_t st = (hipStream_t)s;
+  return run_flagged(st, "fvm_compute_flux", [&](int32_t* bad) {
+    a.bad = bad;
+    hipLaunchKernelGGL(flux_query_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  });
+}
+
+int fvm_pl_interpolate(const fvm_mesh* m, int32_t n_vars, int64_t n, const int32_t* d_tri,
+                       const double* d_x, const double* d_y, const double* d_u, double* d_out,
+                       fvm_stream s) {
+  FVM_CHECK(m && n >= 0 && n_vars >= 1, "fvm_pl_interpolate: bad argument");
+  if (n == 0) return FVM_OK;
+  FVM_CHECK(d_tri && d_x && d_y && d_u && d_out, "fvm_pl_interpolate: null argument");
+  InterpQuery a;
+  a.n = n;
+  a.T = m->T;
+  a.N = m->N;
+  a.nv = n_vars;
+  a.qtri = d_tri;
+  a.px = d_x;
+  a.py = d_y;
+  a.node_tri_start = m->node_tri_start;
+  a.node_tri = m->node_tri;
+  a.tri = m->tri;
+  a.props = m->props;
+  a.u = d_u;
+  a.out = d_out;
+  hipStream_t st = (hipStream_t)s;
+  return run_flagged(st, "fvm_pl_interpolate", [&](int32_t* bad) {
+    a.bad = bad;
+    hipLaunchKernelGGL(interp_query_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  });
+}
+
+}  // extern "C"

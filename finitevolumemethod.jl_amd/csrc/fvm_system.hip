@@ -19,6 +19,7 @@
 // LDS, per-node sums in a fixed order, exclusive nodes finished in the tile
 // kernel, shared nodes through nv partial slots and a fix-up kernel.  No atomics.
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "fvm_internal.hpp"
@@ -337,6 +338,56 @@ __global__ __launch_bounds__(256) void sys_dirichlet_kernel(SysArgs a, const int
   }
 }
 
+// compute_flux for a system (problem.jl:458-487, the is_system branch): every
+// variable's q . n across the mesh edge (i, j); out[q * nv + v].
+template <int NV>
+__global__ __launch_bounds__(256) void sys_flux_query_kernel(SysArgs a, const int32_t* __restrict__ node_tri_start,
+                                                             const int32_t* __restrict__ node_tri, int64_t n,
+                                                             const int32_t* __restrict__ qi,
+                                                             const int32_t* __restrict__ qj,
+                                                             double* __restrict__ out, int32_t* __restrict__ bad) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int32_t i = qi[q], j = qj[q];
+  int32_t t = -1;
+  if (i >= 0 && i < a.N && j >= 0 && j < a.N) {
+    t = find_directed_edge(node_tri_start, node_tri, a.tri, a.T, j, i);
+    if (t < 0) t = find_directed_edge(node_tri_start, node_tri, a.tri, a.T, i, j);
+  }
+  if (t < 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[q * NV + v] = NAN;
+    bad[0] = 1;
+    return;
+  }
+  const double px = a.x[i], py = a.y[i], qx = a.x[j], qy = a.y[j];
+  const double ex = qx - px, ey = qy - py;
+  const double l = sqrt(ex * ex + ey * ey);
+  const double nx = ey / l, ny = -ex / l;
+  const int32_t va = a.tri[t], vb = a.tri[a.T + t], vc = a.tri[2 * a.T + t];
+  double s[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) s[f] = a.props[f * a.T + t];
+  double al[NV], be[NV];
+#pragma unroll
+  for (int w = 0; w < NV; ++w) {
+    const double ua = a.u[w + NV * (int64_t)va], ub = a.u[w + NV * (int64_t)vb], uc = a.u[w + NV * (int64_t)vc];
+    al[w] = s[0] * ua + s[1] * ub + s[2] * uc;
+    be[w] = s[3] * ua + s[4] * ub + s[5] * uc;
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double qxv = 0.0, qyv = 0.0;
+#pragma unroll
+    for (int w = 0; w < NV; ++w)
+      if (a.K[v][w] != 0.0) {
+        qxv = qxv + a.K[v][w] * al[w];
+        qyv = qyv + a.K[v][w] * be[w];
+      }
+    out[q * NV + v] = nx * (-qxv) + ny * (-qyv);
+  }
+}
+
 template <class T>
 int dcopy(T** dst, const T* src, int64_t n) {
   *dst = nullptr;
@@ -560,6 +611,27 @@ int fvm_system_rhs(fvm_system* s, const double* d_u, double* d_du, double t, fvm
     case 3: return launch_sys<3>(s, a, hs);
     default: return launch_sys<4>(s, a, hs);
   }
+}
+
+int fvm_system_compute_flux(fvm_system* s, int64_t n, const int32_t* d_i, const int32_t* d_j,
+                            const double* d_u, double t, double* d_out, fvm_stream st) {
+  FVM_CHECK(s && n >= 0, "fvm_system_compute_flux: bad argument");
+  if (n == 0) return FVM_OK;
+  FVM_CHECK(d_i && d_j && d_u && d_out, "fvm_system_compute_flux: null argument");
+  SysArgs a = s->a;
+  a.u = d_u;
+  a.t = t;
+  const fvm_mesh* m = s->m;
+  hipStream_t hs = (hipStream_t)st;
+  const dim3 g((unsigned)((n + 255) / 256));
+  return fvm::run_flagged(hs, "fvm_system_compute_flux", [&](int32_t* bad) {
+    switch (s->nv) {
+      case 1: hipLaunchKernelGGL((sys_flux_query_kernel<1>), g, dim3(256), 0, hs, a, m->node_tri_start, m->node_tri, n, d_i, d_j, d_out, bad); break;
+      case 2: hipLaunchKernelGGL((sys_flux_query_kernel<2>), g, dim3(256), 0, hs, a, m->node_tri_start, m->node_tri, n, d_i, d_j, d_out, bad); break;
+      case 3: hipLaunchKernelGGL((sys_flux_query_kernel<3>), g, dim3(256), 0, hs, a, m->node_tri_start, m->node_tri, n, d_i, d_j, d_out, bad); break;
+      default: hipLaunchKernelGGL((sys_flux_query_kernel<4>), g, dim3(256), 0, hs, a, m->node_tri_start, m->node_tri, n, d_i, d_j, d_out, bad);
+    }
+  });
 }
 
 int fvm_system_dirichlet_apply(fvm_system* s, double* d_u, double t, fvm_stream st) {

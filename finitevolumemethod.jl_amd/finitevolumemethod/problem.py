@@ -270,7 +270,7 @@ def physics_of(prob):
             ey = P[:, [10, 12, 14]].ravel()
             mx, my = _half_edge_points(tri)
             ph.update(flux_kind=_lib.FLUX_D_EDGE, d_edge=forms.eval_xy(Dfun, ex, ey),
-                      d_bedge=forms.eval_xy(Dfun, mx, my))
+                      d_bedge=forms.eval_xy(Dfun, mx, my), d_xy=Dfun)
     S = prob.source_function
     Sp = prob.source_parameters
     s_node, spoly, st = forms.classify_source(lambda x, y, t, u: S(x, y, t, u, Sp), tri.x, tri.y)

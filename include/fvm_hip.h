@@ -157,6 +157,24 @@ int fvm_jvp(fvm_ctx* c, const double* d_u, const double* d_v, double* d_jv, doub
 int fvm_vjp(fvm_ctx* c, const double* d_u, const double* d_w, double* d_ubar,
             double* d_logd_bar, double t, fvm_stream s);
 
+/* ---- pointwise queries ------------------------------------------------------
+ * compute_flux (problem.jl:458-487) and pl_interpolate (utils.jl:23-27), each for a
+ * batch of queries (the reference answers one per call).  Both wait for completion
+ * and return FVM_ERR_ARG when a queried edge or triangle is not in the mesh. */
+/* out[q] = q(m, t, alpha, beta, gamma) . n across the edge (d_i[q], d_j[q]): m the
+ * edge midpoint, n the clockwise normal of i -> j, the shape function from the
+ * triangle on the right of i -> j (or on its left for a boundary edge with the
+ * domain on the left).  d_dq [n]: D at the edge midpoints, required for
+ * FVM_FLUX_D_EDGE (D(x, y) is a host closure), NULL otherwise. */
+int fvm_compute_flux(fvm_ctx* c, int64_t n, const int32_t* d_i, const int32_t* d_j, const double* d_u,
+                     double t, const double* d_dq, double* d_out, fvm_stream s);
+/* out[q * n_vars + v] = alpha_v x_q + beta_v y_q + gamma_v on the mesh triangle
+ * (d_tri[3q], d_tri[3q+1], d_tri[3q+2]) given in any rotation of its counter-clockwise
+ * order; d_u [n_vars * n_points] column-major (n_vars = 1 for an FVMProblem). */
+int fvm_pl_interpolate(const fvm_mesh* m, int32_t n_vars, int64_t n, const int32_t* d_tri,
+                       const double* d_x, const double* d_y, const double* d_u, double* d_out,
+                       fvm_stream s);
+
 /* ---- linear templates (src/specific_problems/) ----------------------------
  * The operator A and vector b of PoissonsEquation / LaplacesEquation
  * (poissons_equation.jl:58-88, laplaces_equation.jl:51-78), MeanExitTimeProblem
@@ -321,6 +339,10 @@ int fvm_system_rhs(fvm_system* s, const double* d_u, double* d_du, double t, fvm
 /* update_dirichlet_nodes! for a system (dirichlet.jl:46-53): variable by variable,
  * u[var, i] = a(x_i, y_i, t, u[:, i]) with the variables already updated. */
 int fvm_system_dirichlet_apply(fvm_system* s, double* d_u, double t, fvm_stream st);
+/* compute_flux for a system (problem.jl:458-487, is_system branch): out[q * n_vars + v]
+ * is variable v's q . n across the edge (d_i[q], d_j[q]); waits for completion. */
+int fvm_system_compute_flux(fvm_system* s, int64_t n, const int32_t* d_i, const int32_t* d_j,
+                            const double* d_u, double t, double* d_out, fvm_stream st);
 
 /* ---- device vector operations (used by the integrators and solvers) ------
  * The reference broadcasts over Vector{Float64} inside OrdinaryDiffEq; device-

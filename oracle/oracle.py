@@ -410,3 +410,57 @@ class OracleSystem:
         lib().orc_update_dirichlet_system(ctypes.byref(self.base._mesh), ctypes.byref(self._sys), t,
                                           _p(u, _dp))
         return u
+
+
+def directed_edge_triangles(tri):
+    """{(a, b): triangle index} for every directed edge a -> b of the counter-clockwise
+    triangles (what get_adjacent(tri, a, b) answers with the third vertex)."""
+    m = {}
+    for T, (a, b, c) in enumerate(np.asarray(tri).reshape(-1, 3)):
+        a, b, c = int(a), int(b), int(c)
+        m[(a, b)] = T
+        m[(b, c)] = T
+        m[(c, a)] = T
+    return m
+
+
+def compute_flux(x, y, tri, props, edge_tri, u, i, j, t, flux):
+    """compute_flux (problem.jl:458-487) restated for one edge.  u [n_points, nv] (nv
+    columns; a 1-D u is one variable); flux(x, y, t, alpha, beta, gamma, T) with arrays
+    over the variables (T: the triangle used) returns (qx, qy) arrays.  Returns q . n
+    per variable."""
+    tri = np.asarray(tri).reshape(-1, 3)
+    u = np.asarray(u, np.float64)
+    u2 = u[:, None] if u.ndim == 1 else u
+    px, py, qx, qy = x[i], y[i], x[j], y[j]
+    ex, ey = qx - px, qy - py
+    ell = np.sqrt(ex * ex + ey * ey)
+    nx, ny = ey / ell, -ex / ell
+    T = edge_tri.get((j, i))  # the triangle in the direction of the normal
+    if T is None:
+        T = edge_tri[(i, j)]  # boundary edge with the domain on its left
+    a, b, c = tri[T]
+    s = props[T]
+    al = s[0] * u2[a] + s[1] * u2[b] + s[2] * u2[c]
+    be = s[3] * u2[a] + s[4] * u2[b] + s[5] * u2[c]
+    ga = s[6] * u2[a] + s[7] * u2[b] + s[8] * u2[c]
+    fx, fy = flux((px + qx) / 2, (py + qy) / 2, t, al, be, ga, T)
+    return nx * np.asarray(fx) + ny * np.asarray(fy)
+
+
+def pl_interpolate(tri, props, edge_tri, u, T, px, py):
+    """pl_interpolate (utils.jl:23-27) restated: the stored triangle holding the given
+    rotation (i, j, k), alpha x + beta y + gamma per variable."""
+    tri = np.asarray(tri).reshape(-1, 3)
+    u = np.asarray(u, np.float64)
+    u2 = u[:, None] if u.ndim == 1 else u
+    i, j, k = (int(v) for v in T)
+    Ti = edge_tri[(i, j)]
+    a, b, c = tri[Ti]
+    if {int(a), int(b), int(c)} != {i, j, k}:
+        raise KeyError((i, j, k))
+    s = props[Ti]
+    al = s[0] * u2[a] + s[1] * u2[b] + s[2] * u2[c]
+    be = s[3] * u2[a] + s[4] * u2[b] + s[5] * u2[c]
+    ga = s[6] * u2[a] + s[7] * u2[b] + s[8] * u2[c]
+    return al * px + be * py + ga

@@ -419,3 +419,70 @@ def test_system_wrapper_reads_transposed_states():
         assert np.array_equal(orc.update_dirichlet(U.T, t), orc.update_dirichlet(U.T.copy(), t))
         assert np.array_equal(orc.rhs(U.T, t), orc.rhs(U.T.copy(), t))
         assert np.array_equal(orc.scale(U.T, t), orc.scale(U.T.copy(), t))
+
+
+# ---------------------------------------------------------------------------- queries
+def test_compute_flux_oracle_is_exact_for_linear_fields():
+    """u = a x + b y + c with q = -D grad u: compute_flux = -D (a, b) . n on every edge,
+    swapping (i, j) negates it, and boundary edges use their one triangle."""
+    from oracle import compute_flux, directed_edge_triangles
+
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 2.0, 9, 11, jitter=0.25, seed=4, single_boundary=False)
+    cv, props = geometry(tri.x, tri.y, tri.triangles)
+    et = directed_edge_triangles(tri.triangles)
+    a, b, c, D = 1.5, -0.75, 0.3, 0.7
+    u = a * tri.x + b * tri.y + c
+    flux = lambda x, y, t, al, be, ga, T: (-D * al, -D * be)  # noqa: E731
+    for (i, j) in list(et.keys()):
+        ex, ey = tri.x[j] - tri.x[i], tri.y[j] - tri.y[i]
+        ell = np.hypot(ex, ey)
+        want = -D * (a * ey / ell - b * ex / ell)
+        got = compute_flux(tri.x, tri.y, tri.triangles, props, et, u, i, j, 0.0, flux)[0]
+        assert got == pytest.approx(want, rel=1e-12, abs=1e-12)
+        back = compute_flux(tri.x, tri.y, tri.triangles, props, et, u, j, i, 0.0, flux)[0]
+        assert back == pytest.approx(-got, rel=1e-12, abs=1e-12)
+
+
+def test_compute_flux_oracle_picks_the_triangle_on_the_normal_side():
+    """For a nonlinear u the interior-edge flux uses the triangle right of i -> j."""
+    from oracle import compute_flux, directed_edge_triangles
+
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 5, 5, single_boundary=True)
+    cv, props = geometry(tri.x, tri.y, tri.triangles)
+    et = directed_edge_triangles(tri.triangles)
+    u = tri.x ** 2 + np.sin(3 * tri.y)
+    flux = lambda x, y, t, al, be, ga, T: (-al, -be)  # noqa: E731
+    (i, j), T = next(((e, T) for e, T in et.items() if (e[1], e[0]) in et))
+    Tr = et[(j, i)]
+    s = props[Tr]
+    va, vb, vc = tri.triangles[Tr]
+    al = s[0] * u[va] + s[1] * u[vb] + s[2] * u[vc]
+    be = s[3] * u[va] + s[4] * u[vb] + s[5] * u[vc]
+    ex, ey = tri.x[j] - tri.x[i], tri.y[j] - tri.y[i]
+    ell = np.hypot(ex, ey)
+    # the normal (ey, -ex)/ell points into triangle Tr: its centroid lies on the right
+    cx, cy = tri.x[tri.triangles[Tr]].mean(), tri.y[tri.triangles[Tr]].mean()
+    assert (cx - tri.x[i]) * ey - (cy - tri.y[i]) * ex > 0
+    got = compute_flux(tri.x, tri.y, tri.triangles, props, et, u, i, j, 0.0, flux)[0]
+    assert got == pytest.approx((ey / ell) * -al + (-ex / ell) * -be, rel=1e-14)
+
+
+def test_pl_interpolate_oracle_reproduces_linear_fields_and_vertex_values():
+    from oracle import directed_edge_triangles, pl_interpolate
+
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 8, 7, jitter=0.25, seed=6)
+    cv, props = geometry(tri.x, tri.y, tri.triangles)
+    et = directed_edge_triangles(tri.triangles)
+    rng = np.random.default_rng(2)
+    u = np.stack([2.0 * tri.x - tri.y + 0.5, rng.random(tri.num_points())], 1)
+    for T in tri.triangles[::3]:
+        i, j, k = (int(v) for v in T)
+        w = rng.random(3)
+        w /= w.sum()
+        px, py = w @ tri.x[[i, j, k]], w @ tri.y[[i, j, k]]
+        for rot in ((i, j, k), (j, k, i), (k, i, j)):
+            v = pl_interpolate(tri.triangles, props, et, u, rot, px, py)
+            assert v[0] == pytest.approx(2.0 * px - py + 0.5, rel=1e-13)
+            assert v[1] == pytest.approx(w @ u[[i, j, k], 1], rel=1e-12)
+        with pytest.raises(KeyError):
+            pl_interpolate(tri.triangles, props, et, u, (i, k, j), px, py)
