@@ -265,6 +265,44 @@ def ensemble_bench(P=256, steps=1000, graph_steps=100):
     return out
 
 
+def system_bench(mesh, reps=50):
+    """FVMSystem RHS (fvm_system_rhs) of the Gray-Scott tutorial's closures (two
+    variables, q_v = -eps_v grad u_v, S = b (1 - u) - u v^2 and -d v + u v^2,
+    zero-flux boundary) on the throughput mesh, u, v ~ U[0, 1).  Canonical bytes per
+    RHS: the 204-B triangle record, per node 8 nv (u) + 8 (V) + 8 nv (du) + nv (codes),
+    44 B per boundary edge."""
+    import finitevolumemethod as FVM
+    from finitevolumemethod import _lib
+
+    tri = mesh.triangulation
+    N, T, B = tri.num_points(), len(tri.triangles), len(tri.boundary_edges)
+    rng = np.random.default_rng(SEED + 11)
+    bc = lambda x, y, t, u, p: 0.0 * u[0] * 0.0 * u[1]  # noqa: E731
+    probs = [FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, bc, FVM.Neumann),
+                            flux_function=lambda x, y, t, a, b, g, e, v=v: (-a[v] * e, -b[v] * e),
+                            flux_parameters=eps, source_function=S, source_parameters=sp,
+                            initial_condition=rng.random(N), final_time=1.0)
+             for v, eps, S, sp in ((0, 2e-5, lambda x, y, t, u, b: b * (1 - u[0]) - u[0] * u[1] ** 2, 0.04),
+                                   (1, 1e-5, lambda x, y, t, u, d: -d * u[1] + u[0] * u[1] ** 2, 0.1))]
+    sys_ = FVM.FVMSystem(*probs)
+    ctx = sys_.device_context()
+    d_u = _lib.DeviceArray.from_host(sys_.flat(sys_.initial_condition))
+    d_du = _lib.DeviceArray(2 * N)
+    for _ in range(5):
+        ctx.rhs(d_u, d_du)
+    tm = _lib.Timer()
+    tm.start()
+    for _ in range(reps):
+        ctx.rhs(d_u, d_du)
+    ms = tm.stop() / reps
+    alg = canonical_bytes(N, T, B, per_node=8 * 2 + 8 + 8 * 2 + 2)
+    return dict(workload="FVMSystem, Gray-Scott closures (2 variables), throughput mesh, zero-flux boundary",
+                nodes=int(N), variables=2, rhs_ms=round(ms, 5),
+                Gnode_updates_per_s=round(N / (ms * 1e-3) / 1e9, 3),
+                algorithmic_bytes_per_rhs=int(alg),
+                frac_hbm_peak=round(alg / (ms * 1e-3) / HBM_PEAK, 4))
+
+
 def cpu_baseline(tri, u, budget_s, nthreads):
     """The oracle's restatement of parallel_fvm_eqs! (per-thread du columns, serial
     combine) on the same mesh and state, for about budget_s seconds."""
@@ -382,6 +420,8 @@ def main():
         log(f"copy kernel {out['copy_bandwidth_GBps']} GB/s")
         out["vjp_rhs"] = vjp_ratio(args.vjp_side)
         log(f"VJP/RHS {out['vjp_rhs']}")
+        out["system_rhs"] = system_bench(mesh)
+        log(f"system RHS {out['system_rhs']}")
         if args.template_side > 0:
             out["template_solve"] = template_solve(args.template_side)
         if args.ensemble > 0:

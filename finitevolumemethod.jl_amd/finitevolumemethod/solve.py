@@ -93,20 +93,25 @@ def _hermite(out, u0, f0, u1, f1, dt, theta):
     _lib.lincomb(out, (h00, h10 * dt, h01, h11 * dt), (u0, f0, u1, f1))
 
 
-def solve(prob, alg=None, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None, maxiters=10**7,
+def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, maxiters=10**7,
           save_start=True, callback=None, parallel=True, **kwargs):
     """solve(prob, alg; saveat, abstol, reltol, dt, ...).
 
-    FVMProblem: integrates on the device; returns ODESolution (t, u host arrays).
-    SteadyFVMProblem: see solve_steady.  Templates forward to their own solver."""
+    FVMProblem / FVMSystem: integrates on the device (abstol 1e-6, reltol 1e-3 unless
+    given, OrdinaryDiffEq's defaults); returns ODESolution (t, u host arrays).
+    SteadyFVMProblem: see solve_steady (its own defaults).  Templates forward to
+    their own solver."""
     from .templates import AbstractFVMTemplate, _SteadyTemplate
 
     if isinstance(prob, _SteadyTemplate):  # a LinearProblem: only the linear solver's options apply
         return prob.solve(alg, **kwargs)
+    if isinstance(prob, SteadyFVMProblem):
+        tol = {k: v for k, v in (("abstol", abstol), ("reltol", reltol)) if v is not None}
+        return solve_steady(prob, alg, **tol, **kwargs)
+    abstol = 1e-6 if abstol is None else abstol
+    reltol = 1e-3 if reltol is None else reltol
     if isinstance(prob, AbstractFVMTemplate):
         return prob.solve(alg, saveat=saveat, abstol=abstol, reltol=reltol, dt=dt, **kwargs)
-    if isinstance(prob, SteadyFVMProblem):
-        return solve_steady(prob, alg, abstol=abstol, reltol=reltol, **kwargs)
     if isinstance(prob, FVMSystem):
         ctx = prob.device_context()
         cb = (lambda u, t: ctx.apply_dirichlet(u, t)) if prob.has_dirichlet_nodes() else None
@@ -278,7 +283,8 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
     """SteadyFVMProblem: Newton's method on fvm_eqs!(du, u) = 0 with the device
     Jacobian-vector product (fvm_jvp; a forward difference of fvm_system_rhs for an
     FVMSystem) inside restarted GMRES.  Dirichlet nodes are
-    set by the Dirichlet functions first and stay fixed (their du is 0)."""
+    set by the Dirichlet functions first and stay fixed (their du is 0).  Stops when
+    |F| <= abstol + reltol |F0| and the last correction is below sqrt(reltol) (1 + |u|)."""
     from .linsolve import gmres_device
 
     p = prob.problem
@@ -298,8 +304,17 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
     r0 = math.sqrt(_lib.dot(F, F))
     it = 0
     rn = r0
+    step = math.inf
+
+    def converged():
+        # the residual has dropped, and the last Newton correction is small next to u
+        # (GMRES solves each correction to 1e-3 only)
+        if rn > abstol + reltol * r0:
+            return False
+        return it == 0 or step <= math.sqrt(reltol) * (1.0 + math.sqrt(_lib.dot(u, u)))
+
     while it < maxiters:
-        if rn <= abstol + reltol * r0:
+        if converged():
             break
         it += 1
         if sys:
@@ -308,10 +323,11 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
             jv = lambda v, out: ctx.jvp(u, v, out, t)  # noqa: E731
         gmres_device(jv, F, du, rtol=1e-3, restart=50, maxiter=2000)
         _lib.lincomb(u, (1.0, -1.0), (u, du))
+        step = math.sqrt(_lib.dot(du, du))
         ctx.rhs(u, F, t)
         rn = math.sqrt(_lib.dot(F, F))
     sol.t = [math.inf]
     sol.u = [p.unflat(u.to_host()) if sys else u.to_host()]
-    sol.retcode = "Success" if rn <= abstol + reltol * r0 else "MaxIters"
-    sol.stats = dict(newton_iters=it, residual=rn)
+    sol.retcode = "Success" if converged() else "MaxIters"
+    sol.stats = dict(newton_iters=it, residual=rn, initial_residual=r0, last_step=step)
     return sol

@@ -92,3 +92,25 @@ def test_dirichlet_nodes_stay_fixed_after_steps(gpu):
     bn = tri.boundary_nodes()
     for u in sol.u[1:]:
         assert np.all(u[bn] == 0.0)
+
+
+def test_steady_problem_solves_to_a_zero_residual(gpu):
+    """SteadyFVMProblem(FVMProblem): Newton-GMRES on fvm_jvp to F(u) = 0 for a
+    nonlinear D(u) with a reaction term; the oracle's du vanishes at the answer."""
+    import oracle as O  # noqa: F401
+    from problems import oracle_for
+
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 23, 21, jitter=0.2, seed=12)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 0.1 + u ** 2,
+                          source_function=lambda x, y, t, u, p: 1.0 - u,
+                          initial_condition=np.zeros(tri.num_points()), final_time=np.inf)
+    sol = FVM.solve(FVM.SteadyFVMProblem(prob))
+    assert sol.retcode == "Success", sol.stats
+    u = sol.u[-1]
+    orc = oracle_for(prob, dict(builtin=dict(flux_kind=0, dpoly=[0.1, 0, 1, 0], spoly=[1, -1, 0, 0],
+                                             fn=([0.0], [0.0], [0.0]))))
+    r = orc.rhs(u, 0.0)
+    assert np.linalg.norm(r) <= 1e-7 * np.linalg.norm(orc.scale(u, 0.0)), sol.stats
+    assert np.all(u[tri.boundary_nodes()] == 0.0) and u.max() > 0.05

@@ -203,3 +203,35 @@ def test_system_jacobian_sparsity_holds_the_jacobian(gpu):
         fc = FVM.fvm_eqs(np.zeros_like(U), Up.reshape(-1, 2).T, sys, 0.2).T.ravel()
         nz = np.nonzero(fc != f0)[0]
         assert S[nz, c].all(), (c, nz[~S[nz, c]])
+
+
+def test_steady_system_solves_to_a_zero_residual(gpu):
+    """SteadyFVMProblem(FVMSystem): Newton with GMRES on forward differences of
+    fvm_system_rhs; at the answer the oracle's du vanishes at the free nodes."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 21, 19, jitter=0.2, seed=4)
+    mesh = FVM.FVMGeometry(tri)
+    B0 = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u[0], FVM.Dirichlet)
+    B1 = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: x + 0.0 * u[0], FVM.Dirichlet)
+    N = tri.num_points()
+    p0 = FVM.FVMProblem(mesh, B0, flux_function=lambda x, y, t, a, b, g, p: (-a[0], -b[0]),
+                        source_function=lambda x, y, t, u, p: 1.0 - u[0] + 0.3 * u[1],
+                        initial_condition=np.zeros(N), final_time=np.inf)
+    p1 = FVM.FVMProblem(mesh, B1, flux_function=lambda x, y, t, a, b, g, p: (-0.5 * a[1], -0.5 * b[1]),
+                        source_function=lambda x, y, t, u, p: 0.5 - u[1] + 0.2 * u[0],
+                        initial_condition=np.zeros(N), final_time=np.inf)
+    sys = FVM.FVMSystem(p0, p1)
+    sol = FVM.solve(FVM.SteadyFVMProblem(sys))
+    assert sol.retcode == "Success"
+    U = sol.u[-1]
+    assert U.shape == (2, N)
+    kind, fidx, bn = system_conditions(tri, [["Dirichlet"], ["Dirichlet"]])
+    orc = OracleSystem(tri.x, tri.y, tri.triangles, tri.boundary_edges, tri.boundary_adjacent(),
+                       kind, fidx, bn,
+                       lambda x, y, t, a, b, g: (np.array([-a[0], -0.5 * a[1]]), np.array([-b[0], -0.5 * b[1]])),
+                       lambda v, x, y, t, u: 1.0 - u[0] + 0.3 * u[1] if v == 0 else 0.5 - u[1] + 0.2 * u[0],
+                       lambda f, x, y, t, u: 0.0 if f == 0 else x)
+    r = orc.rhs(U.T, 0.0)
+    sc = orc.scale(U.T, 0.0)
+    assert np.linalg.norm(r) <= 1e-7 * np.linalg.norm(sc)
+    bnd = tri.boundary_nodes()
+    assert np.all(U[0, bnd] == 0.0) and np.allclose(U[1, bnd], tri.x[bnd], rtol=0, atol=0)
