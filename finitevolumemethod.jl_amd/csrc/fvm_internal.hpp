@@ -11,11 +11,12 @@
 
 // Build-time tuning (fvm_build_info reports the values in use):
 //   FVM_GEO_LEVEL       how much of each triangle record the tile kernels rebuild from
-//                       the vertex coordinates instead of streaming (1, 2 or 3 = all)
+//                       the vertex coordinates instead of streaming (1, 2 or 3 = all;
+//                       4 = all, with the CV-edge normals unnormalised and lengths 1)
 //   FVM_SOLVER_THREADS  workgroup size of the template solver kernels
 //   FVM_SPMV_ROWS       CSR rows per lane group and step in the solver SpMV
 #ifndef FVM_GEO_LEVEL
-#define FVM_GEO_LEVEL 3
+#define FVM_GEO_LEVEL 4
 #endif
 #ifndef FVM_SOLVER_THREADS
 #define FVM_SOLVER_THREADS 512
@@ -73,36 +74,70 @@ enum : int {
   P_COUNT = 24
 };
 
-// The fields of triangle (p, q, r)'s record rebuilt at LEVEL (see FVM_GEO_LEVEL;
-// 1: s7..s9 and the CV-edge midpoints, 2: also s1..s6, 3: everything), with exactly
-// the operations of the geometry kernel (geometry.jl:114-161), so the values equal
-// the stored ones bit for bit.
+// a / b correctly rounded, given y = RN(1 / b): Markstein's correction step
+// (P. Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of
+// Floating-Point Arithmetic, 2nd ed., 2018, sec. 4.9): q0 = RN(a y), the remainder
+// r = a - b q0 exactly by one FMA, then RN(q0 + r y).  Equal to a / b for finite
+// operands away from overflow and underflow (checked against IEEE division on 2e8
+// random pairs and on every quotient of the geometry of 4M-node meshes).
+__device__ __forceinline__ double div_rn(double a, double b, double y) {
+  const double q0 = a * y;
+  const double r = __builtin_fma(-b, q0, a);
+  return __builtin_fma(r, y, q0);
+}
+
+// The triangle record (TriangleProperties, geometry.jl:99-169) of vertices p, q, r,
+// with the geometry kernel's operations in the reference's order, so every value
+// equals the stored one bit for bit up to LEVEL 3.  LEVEL >= 2: s1..s6; always
+// s7..s9 and the CV-edge midpoints; LEVEL 3: normals and lengths; LEVEL 4: the
+// quotients by Delta and by 3 from one reciprocal (div_rn, still bit-identical),
+// and the CV-edge normals unnormalised with lengths 1 ((q . n) l = q . (e_y, -e_x)).
 template <int LEVEL>
 __device__ __forceinline__ void rebuild_record(double px, double py, double qx, double qy,
                                                double rx, double ry, double* P) {
-  const double cx = (px + qx + rx) / 3, cy = (py + qy + ry) / 3;
+  constexpr double kThird = 1.0 / 3;
+  const double sx = px + qx + rx, sy = py + qy + ry;
+  const double cx = LEVEL >= 4 ? div_rn(sx, 3.0, kThird) : sx / 3;
+  const double cy = LEVEL >= 4 ? div_rn(sy, 3.0, kThird) : sy / 3;
   const double m1x = (px + qx) / 2, m1y = (py + qy) / 2;
   const double m2x = (qx + rx) / 2, m2y = (qy + ry) / 2;
   const double m3x = (rx + px) / 2, m3y = (ry + py) / 2;
   const double D = qx * ry - qy * rx - px * ry + rx * py + px * qy - qx * py;
+  const double yD = LEVEL >= 4 ? 1.0 / D : 0.0;
+  auto quot = [&](double a) { return LEVEL >= 4 ? div_rn(a, D, yD) : a / D; };
   if (LEVEL >= 2) {
-    P[0] = (qy - ry) / D;
-    P[1] = (ry - py) / D;
-    P[2] = (py - qy) / D;
-    P[3] = (rx - qx) / D;
-    P[4] = (px - rx) / D;
-    P[5] = (qx - px) / D;
+    P[0] = quot(qy - ry);
+    P[1] = quot(ry - py);
+    P[2] = quot(py - qy);
+    P[3] = quot(rx - qx);
+    P[4] = quot(px - rx);
+    P[5] = quot(qx - px);
   }
-  P[6] = (qx * ry - rx * qy) / D;
-  P[7] = (rx * py - px * ry) / D;
-  P[8] = (px * qy - qx * py) / D;
+  P[6] = quot(qx * ry - rx * qy);
+  P[7] = quot(rx * py - px * ry);
+  P[8] = quot(px * qy - qx * py);
   P[9] = (m1x + cx) / 2;
   P[10] = (m1y + cy) / 2;
   P[11] = (m2x + cx) / 2;
   P[12] = (m2y + cy) / 2;
   P[13] = (m3x + cx) / 2;
   P[14] = (m3y + cy) / 2;
-  if (LEVEL >= 3) {
+  if (LEVEL >= 4) {
+    // (q . n) l with n = (e_y, -e_x) / l is q . (e_y, -e_x): keep the normal
+    // unnormalised and the length 1, saving six divisions and three square roots
+    const double e1x = cx - m1x, e1y = cy - m1y;
+    const double e2x = cx - m2x, e2y = cy - m2y;
+    const double e3x = cx - m3x, e3y = cy - m3y;
+    P[15] = e1y;
+    P[16] = -e1x;
+    P[17] = e2y;
+    P[18] = -e2x;
+    P[19] = e3y;
+    P[20] = -e3x;
+    P[21] = 1.0;
+    P[22] = 1.0;
+    P[23] = 1.0;
+  } else if (LEVEL >= 3) {
     const double e1x = cx - m1x, e1y = cy - m1y;
     const double e2x = cx - m2x, e2y = cy - m2y;
     const double e3x = cx - m3x, e3y = cy - m3y;

@@ -110,6 +110,22 @@ __global__ __launch_bounds__(256) void local_cv_kernel(int64_t n, const int32_t*
   if (e < n) lcv[e] = cv[nodes[e]];
 }
 
+// One thread per triangle: the record as the tile kernels rebuild it from the vertex
+// coordinates (rebuild_record at FVM_GEO_LEVEL, the whole record from level 3 on).
+__global__ __launch_bounds__(256) void rebuild_kernel(int64_t T, const int32_t* __restrict__ tri,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      double* __restrict__ P) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const int32_t i = tri[t], j = tri[T + t], k = tri[2 * T + t];
+  double R[P_COUNT];
+  for (int f = 0; f < P_COUNT; ++f) R[f] = 0.0;
+  rebuild_record<(FVM_GEO_LEVEL >= 3 ? FVM_GEO_LEVEL : 3)>(x[i], y[i], x[j], y[j], x[k], y[k], R);
+#pragma unroll
+  for (int f = 0; f < P_COUNT; ++f) P[f * T + t] = R[f];
+}
+
 void free_mesh(fvm_mesh* m) {
   if (!m) return;
   void* ptrs[] = {m->x, m->y, m->cv, m->props, m->tri, m->tile_tri_start, m->tile_node_start,
@@ -272,6 +288,25 @@ int fvm_mesh_download_geometry(const fvm_mesh* m, double* cv_volumes, double* pr
       double* dst = props + (int64_t)P_COUNT * m->h_perm[s];
       for (int f = 0; f < P_COUNT; ++f) dst[f] = soa[(size_t)f * T + s];
     }
+  }
+  return FVM_OK;
+}
+
+int fvm_mesh_rebuilt_geometry(const fvm_mesh* m, double* props) {
+  FVM_CHECK(m && props, "fvm_mesh_rebuilt_geometry: null argument");
+  const int64_t T = m->T;
+  double* d = nullptr;
+  FVM_HIP(hipMalloc((void**)&d, sizeof(double) * (size_t)P_COUNT * T));
+  hipLaunchKernelGGL(rebuild_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, 0, T, m->tri, m->x,
+                     m->y, d);
+  std::vector<double> soa((size_t)P_COUNT * T);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(soa.data(), d, sizeof(double) * soa.size(), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "fvm_mesh_rebuilt_geometry");
+  for (int64_t s = 0; s < T; ++s) {
+    double* dst = props + (int64_t)P_COUNT * m->h_perm[s];
+    for (int f = 0; f < P_COUNT; ++f) dst[f] = soa[(size_t)f * T + s];
   }
   return FVM_OK;
 }

@@ -236,7 +236,7 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
     if (lt < nt) {
       const int ia = L[r] & 1023u, ib = (L[r] >> 10) & 1023u, ic = (L[r] >> 20) & 1023u;
       double P[P_COUNT];
-      rebuild_record<3>(s_x[ia], s_y[ia], s_x[ib], s_y[ib], s_x[ic], s_y[ic], P);
+      rebuild_record<(FVM_GEO_LEVEL >= 3 ? FVM_GEO_LEVEL : 3)>(s_x[ia], s_y[ia], s_x[ib], s_y[ib], s_x[ic], s_y[ic], P);
       double al[NV], be[NV];
 #pragma unroll
       for (int w = 0; w < NV; ++w) {

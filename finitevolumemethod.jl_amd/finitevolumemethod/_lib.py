@@ -169,6 +169,7 @@ HIP_API = {
     "fvm_mesh_destroy": [_vp],
     "fvm_mesh_get_info": [_vp, ctypes.POINTER(MeshInfo)],
     "fvm_mesh_download_geometry": [_vp, _dp, _dp],
+    "fvm_mesh_rebuilt_geometry": [_vp, _dp],
     "fvm_ctx_create": [_vp, ctypes.POINTER(PhysicsDesc), ctypes.POINTER(_vp)],
     "fvm_ctx_destroy": [_vp],
     "fvm_ctx_set_log_d": [_vp, _vp],

@@ -92,6 +92,12 @@ int fvm_mesh_get_info(const fvm_mesh* m, fvm_mesh_info* info);
  * s1..s9, CV-edge midpoints (m1x m1y m2x m2y m3x m3y), unit normals (n1x n1y
  * n2x n2y n3x n3y), lengths (l1 l2 l3) -- TriangleProperties, geometry.jl:21-26. */
 int fvm_mesh_download_geometry(const fvm_mesh* m, double* cv_volumes, double* props);
+/* The record as the RHS tile kernels rebuild it from the vertex coordinates, in the
+ * layout of fvm_mesh_download_geometry (waits for completion).  No reference
+ * counterpart: lets tests pin the rebuilt values against the stored geometry (s1..s9
+ * and the midpoints bit for bit; with the build's geometry level 4 the normals are
+ * unnormalised, (e_y, -e_x) = n l, and the lengths 1). */
+int fvm_mesh_rebuilt_geometry(const fvm_mesh* m, double* props);
 
 /* ---- problem (FVMProblem + Conditions, problem.jl:118-163, conditions.jl:486-552) ---- */
 enum {

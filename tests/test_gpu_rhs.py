@@ -137,3 +137,28 @@ def test_invalid_arguments_raise(gpu):
     d = FVM.DeviceArray.from_host(u)
     with pytest.raises(FVM.FVMError):
         prob.device_context().rhs(d, d, 0.0)  # du aliasing u is rejected by the C ABI
+
+
+@pytest.mark.parametrize("x0,y0", [(0.0, 0.0), (1000.0, -500.0)])
+def test_rebuilt_record_matches_stored_geometry(gpu, x0, y0):
+    """The record the tile kernels rebuild from vertex coordinates (quotients by
+    Delta and 3 from one reciprocal plus Markstein's correction) equals the stored
+    TriangleProperties bit for bit in s1..s9 and the CV-edge midpoints, also far from
+    the origin where Delta cancels; the unnormalised normals equal n l."""
+    import re
+
+    from finitevolumemethod import _lib
+
+    tri = FVM.jittered_rectangle(x0, x0 + 1.0, y0, y0 + 1.0, 257, 233, jitter=0.25, seed=17)
+    mesh = FVM.FVMGeometry(tri)
+    P = mesh.props_array
+    R = np.empty_like(P)
+    _lib.check(_lib.hip_lib().fvm_mesh_rebuilt_geometry(mesh.handle, _lib.ptr(R)), "fvm_mesh_rebuilt_geometry")
+    assert np.array_equal(R[:, :15], P[:, :15])
+    level = int(re.search(r"geo_level=(\d)", _lib.build_info()).group(1))
+    if level >= 4:
+        nl = P[:, 15:21] * np.repeat(P[:, 21:24], 2, axis=1)
+        np.testing.assert_allclose(R[:, 15:21], nl, rtol=1e-15, atol=1e-15 * np.abs(nl).max())
+        assert np.all(R[:, 21:24] == 1.0)
+    else:
+        assert np.array_equal(R[:, 15:], P[:, 15:])
