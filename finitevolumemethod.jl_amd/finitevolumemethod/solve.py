@@ -56,6 +56,26 @@ class SSPRK3:
 
 
 @dataclass
+class TRBDF2:
+    """TR-BDF2 (Bank et al. 1985; Hosea & Shampine, Appl. Numer. Math. 20, 1996): a
+    trapezoidal stage to t + gamma h, gamma = 2 - sqrt(2), then BDF2 through
+    (t, t + gamma h, t + h); L-stable, order 2, the reference tutorials' stiff
+    integrator.  Both stages solve z - d h f(z) = r, d = gamma / 2, by Newton with the
+    Jacobian frozen at the step's start and each linear system W dz = -G,
+    W = I - d h J, by restarted GMRES on fvm_jvp (matrix-free: no Jacobian is
+    formed, so `linsolve` has no counterpart).  The local error estimate
+    k h (f_n / gamma - f_gamma / (gamma (1 - gamma)) + f_{n+1} / (1 - gamma)),
+    k = (3 gamma^2 - 4 gamma + 2) / (6 (2 - gamma)), is filtered through W^-1 for
+    stiff problems (smooth_est), and the step follows the standard controller
+    dt *= clamp(0.9 EEst^(-1/3), 0.2, 5)."""
+    newton_kappa: float = 1e-2     # stop Newton when theta / (1 - theta) |dz| <= kappa (error norm)
+    max_newton: int = 10
+    krylov_rtol: float = 1e-3
+    krylov_restart: int = 30
+    smooth_est: bool = True
+
+
+@dataclass
 class ODESolution:
     t: list = field(default_factory=list)
     u: list = field(default_factory=list)
@@ -118,7 +138,8 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
         sol = integrate(ctx.rhs, prob.flat(prob.initial_condition), prob.initial_time,
                         prob.final_time, alg if alg is not None else Tsit5(), saveat=saveat,
                         abstol=abstol, reltol=reltol, dt=dt, maxiters=maxiters,
-                        save_start=save_start, step_callback=cb, user_callback=callback)
+                        save_start=save_start, step_callback=cb, user_callback=callback,
+                        jvp=ctx.jvp)
         sol.u = [prob.unflat(v) for v in sol.u]
         return sol
     if not isinstance(prob, FVMProblem):
@@ -130,12 +151,20 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
     return integrate(rhs, prob.initial_condition, prob.initial_time, prob.final_time,
                      alg if alg is not None else Tsit5(), saveat=saveat, abstol=abstol,
                      reltol=reltol, dt=dt, maxiters=maxiters, save_start=save_start,
-                     step_callback=cb, user_callback=callback)
+                     step_callback=cb, user_callback=callback,
+                     jvp=lambda u, v, out, t: ctx.jvp(u, v, out, t))
 
 
 def integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None,
-              maxiters=10**7, save_start=True, step_callback=None, user_callback=None):
-    """Integrate du/dt = rhs(u, t) on the device from t0 to tf."""
+              maxiters=10**7, save_start=True, step_callback=None, user_callback=None, jvp=None):
+    """Integrate du/dt = rhs(u, t) on the device from t0 to tf.  Implicit methods
+    need jvp(u, v, out, t) = J(u) v."""
+    if isinstance(alg, TRBDF2):
+        if jvp is None:
+            raise TypeError("TRBDF2 needs the Jacobian-vector product of the right-hand side")
+        return _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, saveat=saveat, abstol=abstol,
+                                 reltol=reltol, dt=dt, maxiters=maxiters, save_start=save_start,
+                                 step_callback=step_callback, user_callback=user_callback)
     n = len(u0)
     u = _lib.DeviceArray.from_host(np.asarray(u0, np.float64))
     sol = ODESolution()
@@ -262,8 +291,153 @@ def integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt
     return sol
 
 
-def _initial_dt(f, u0, f0, t0, tf, abstol, reltol, n):
-    """Hairer, Norsett & Wanner (1993), II.4: starting step size for order 5."""
+_TRB_G = 2.0 - math.sqrt(2.0)
+_TRB_D = _TRB_G / 2                                    # = (1 - gamma) / (2 - gamma)
+_TRB_A = 1.0 / (_TRB_G * (2.0 - _TRB_G))               # BDF2 weight of u_gamma
+_TRB_B = -(1.0 - _TRB_G) ** 2 / (_TRB_G * (2.0 - _TRB_G))  # and of u_n
+_TRB_K = (3 * _TRB_G ** 2 - 4 * _TRB_G + 2) / (6 * (2.0 - _TRB_G))
+
+
+def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, maxiters,
+                      save_start, step_callback, user_callback):
+    from .linsolve import gmres_device
+
+    n = len(u0)
+    u = _lib.DeviceArray.from_host(np.asarray(u0, np.float64))
+    sol = ODESolution()
+    saver = _Saver(saveat, t0, tf)
+    stats = dict(nf=0, njvp=0, naccept=0, nreject=0, nnewton=0, nkrylov=0, nnewton_fail=0)
+    V = {k: _lib.DeviceArray(n) for k in ("fn", "z", "fg", "u1", "f1", "r", "G", "dz", "est", "tmp", "interp")}
+
+    def f(uu, out, tt):
+        stats["nf"] += 1
+        rhs(uu, out, tt)
+
+    def save(tt, arr):
+        sol.t.append(tt)
+        sol.u.append(arr.to_host())
+
+    def W_apply(ubase, tbase, h):
+        def apply(v, out):
+            stats["njvp"] += 1
+            jvp(ubase, v, out, tbase)
+            _lib.lincomb(out, (1.0, -_TRB_D * h), (v, out))
+        return apply
+
+    nl = dict(eta=1.0)  # Newton's contraction estimate theta / (1 - theta), kept between solves
+
+    def newton(z, r, ts, ubase, tbase, h, fz):
+        """z <- solution of z - d h f(z, ts) = r (z holds the predictor); fz <- f(z).
+        Stops when eta |dz| <= kappa (Hairer & Wanner, Solving ODEs II, IV.8), eta
+        starting from the previous solve's; returns False when Newton diverges or
+        does not converge in max_newton iterations."""
+        A = W_apply(ubase, tbase, h)
+        eta = max(nl["eta"], 1e-16) ** 0.8
+        prev = None
+        for _ in range(alg.max_newton):
+            f(z, fz, ts)
+            _lib.lincomb(V["G"], (-1.0, _TRB_D * h, 1.0), (z, fz, r))  # -G = r + d h f(z) - z
+            stats["nkrylov"] += gmres_device(A, V["G"], V["dz"], rtol=alg.krylov_rtol,
+                                             restart=alg.krylov_restart, maxiter=20 * alg.krylov_restart)
+            _lib.lincomb(z, (1.0, 1.0), (z, V["dz"]))
+            stats["nnewton"] += 1
+            ndz = _lib.err_norm(V["dz"], ubase, z, abstol, reltol)
+            if not math.isfinite(ndz):
+                return False
+            if prev is not None:
+                theta = ndz / prev
+                if theta >= 1.0:
+                    return False
+                eta = theta / (1.0 - theta)
+            if eta * ndz <= alg.newton_kappa or ndz == 0.0:
+                break
+            prev = ndz
+        else:
+            return False
+        nl["eta"] = eta
+        f(z, fz, ts)
+        return True
+
+    t = t0
+    f(u, V["fn"], t)
+    if saver.times is None:
+        if save_start:
+            save(t0, u)
+    else:
+        while saver.k < len(saver.times) and saver.times[saver.k] <= t + 1e-14 * max(1, abs(t)):
+            save(saver.times[saver.k], u)
+            saver.k += 1
+    if dt is None:
+        dt = _initial_dt(f, u, V["fn"], t, tf, abstol, reltol, n, exponent=1 / 3)
+    it = 0
+    while t < tf and it < maxiters:
+        it += 1
+        h = min(dt, tf - t)
+        fn, z, fg, u1, f1 = V["fn"], V["z"], V["fg"], V["u1"], V["f1"]
+        # TR stage: z - d h f(z) = u + d h f_n, predictor u + gamma h f_n
+        _lib.lincomb(V["r"], (1.0, _TRB_D * h), (u, fn))
+        _lib.lincomb(z, (1.0, _TRB_G * h), (u, fn))
+        ok = newton(z, V["r"], t + _TRB_G * h, u, t, h, fg)
+        if ok:
+            # BDF2 stage: u1 - d h f(u1) = a z + b u, predictor z + (1 - gamma) h f_gamma
+            _lib.lincomb(V["r"], (_TRB_A, _TRB_B), (z, u))
+            _lib.lincomb(u1, (1.0, (1.0 - _TRB_G) * h), (z, fg))
+            ok = newton(u1, V["r"], t + h, u, t, h, f1)
+        if not ok:
+            stats["nnewton_fail"] += 1
+            stats["nreject"] += 1
+            dt = h / 4
+            if dt < 1e-14 * max(1.0, abs(t)):
+                sol.retcode = "ConvergenceFailure"
+                break
+            continue
+        est = V["est"]
+        _lib.lincomb(est, (_TRB_K * h / _TRB_G, -_TRB_K * h / (_TRB_G * (1 - _TRB_G)),
+                           _TRB_K * h / (1 - _TRB_G)), (fn, fg, f1))
+        if alg.smooth_est:  # W^-1 est (Hosea & Shampine): keeps stiff components from inflating it
+            stats["nkrylov"] += gmres_device(W_apply(u, t, h), est, V["tmp"], rtol=alg.krylov_rtol,
+                                             restart=alg.krylov_restart, maxiter=20 * alg.krylov_restart)
+            est = V["tmp"]
+        EEst = _lib.err_norm(est, u, u1, abstol, reltol)
+        if not math.isfinite(EEst):
+            sol.retcode = "Unstable"
+            break
+        if EEst <= 1.0:
+            tn = t + h
+            if saver.times is not None:
+                while saver.k < len(saver.times) and saver.times[saver.k] <= tn + 1e-12 * max(1, abs(tn)):
+                    ts = saver.times[saver.k]
+                    theta = min(1.0, max(0.0, (ts - t) / h))
+                    _hermite(V["interp"], u, fn, u1, f1, h, theta)
+                    save(ts, V["interp"])
+                    saver.k += 1
+            u.copy_from(u1)
+            t = tn
+            stats["naccept"] += 1
+            if step_callback is not None:
+                step_callback(u, t)
+            if user_callback is not None:
+                user_callback(u, t)
+            if saver.times is None:
+                save(t, u)
+            if step_callback is not None or user_callback is not None:
+                f(u, fn, t)  # the callback may have changed u
+            else:
+                fn.copy_from(f1)
+        else:
+            stats["nreject"] += 1
+        q = 0.9 * max(EEst, 1e-10) ** (-1.0 / 3.0)
+        dt = h * min(5.0, max(0.2, q))
+    if sol.retcode == "Default":
+        sol.retcode = "Success" if t >= tf else "MaxIters"
+    sol.stats = stats
+    sol.final_device = u
+    return sol
+
+
+def _initial_dt(f, u0, f0, t0, tf, abstol, reltol, n, exponent=1 / 5):
+    """Hairer, Norsett & Wanner (1993), II.4: starting step size, (0.01 / m)^exponent
+    (1/5 for Tsit5 here, 1/3 for TR-BDF2)."""
     tmp = _lib.DeviceArray(n)
     f1 = _lib.DeviceArray(n)
     d0 = _lib.err_norm(u0, u0, u0, abstol, reltol)
@@ -275,7 +449,7 @@ def _initial_dt(f, u0, f0, t0, tf, abstol, reltol, n):
     _lib.lincomb(tmp, (1.0 / dt0, -1.0 / dt0), (f1, f0))
     d2 = _lib.err_norm(tmp, u0, u0, abstol, reltol)
     m = max(d1, d2)
-    dt1 = max(1e-6, dt0 * 1e-3) if m <= 1e-15 else (0.01 / m) ** (1 / 5)
+    dt1 = max(1e-6, dt0 * 1e-3) if m <= 1e-15 else (0.01 / m) ** exponent
     return min(100 * dt0, dt1, tf - t0)
 
 
