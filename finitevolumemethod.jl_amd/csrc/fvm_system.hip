@@ -78,6 +78,7 @@ struct SysArgs {
   const double* node_g;   // [nv][N] or null
   const double* bedge_g;  // [nv][2B] or null
   const double* u;
+  const double* v;        // tangent direction (Jacobian-vector products)
   double* out;
   double* partial;        // [n_parts][nv]
   double* bflux;          // [n_bitems][nv]
@@ -113,22 +114,59 @@ __device__ __forceinline__ double source_eval(const SysArgs& a, int v, int g, co
   return S;
 }
 
-// source_contributions.jl:49-68 for variable v of node g
+// sum_w dS_v/du_w (u) dv_w, the derivative of variable v's source monomials
 template <int NV>
+__device__ __forceinline__ double source_tangent(const SysArgs& a, int v, const double* u, const double* dv) {
+  double S = 0.0;
+  for (int k = 0; k < a.S.n; ++k) {
+    if (a.S.var[k] != v) continue;
+#pragma unroll
+    for (int w = 0; w < NV; ++w) {
+      const int ew = a.S.e[k][w];
+      if (ew == 0) continue;
+      double m = a.S.c[k] * ew;
+#pragma unroll
+      for (int x = 0; x < NV; ++x) {
+        const int ex = x == w ? ew - 1 : a.S.e[k][x];
+        for (int p = 0; p < ex; ++p) m = m * u[x];
+      }
+      S = S + m * dv[w];
+    }
+  }
+  return S;
+}
+
+// sum_w da/du_w dv_w of condition function f (the conditions are affine in u)
+template <int NV>
+__device__ __forceinline__ double cond_tangent(const SysArgs& a, int f, const double* dv) {
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < NV; ++w) r = r + a.fcu[f * NV + w] * dv[w];
+  return r;
+}
+
+// source_contributions.jl:49-68 for variable v of node g, or (JV) its derivative
+// along dv
+template <int NV, int JV>
 __device__ __forceinline__ double finish_var(const SysArgs& a, int v, int g, double acc, double V,
-                                             const double* u) {
+                                             const double* u, const double* dv) {
   const int c = a.code[(int64_t)v * a.N + g];
-  if (c == 0) return acc / V + source_eval<NV>(a, v, g, u);
-  if (c == 2)
-    return cond_eval<NV>(a, a.fidx[(int64_t)v * a.N + g], a.x[g], a.y[g], a.t, u,
-                         a.node_g ? a.node_g[(int64_t)v * a.N + g] : 0.0);
+  if (c == 0) return acc / V + (JV ? source_tangent<NV>(a, v, u, dv) : source_eval<NV>(a, v, g, u));
+  if (c == 2) {
+    const int f = a.fidx[(int64_t)v * a.N + g];
+    if (JV) return cond_tangent<NV>(a, f, dv);
+    return cond_eval<NV>(a, f, a.x[g], a.y[g], a.t, u, a.node_g ? a.node_g[(int64_t)v * a.N + g] : 0.0);
+  }
   return 0.0;
 }
 
 // Boundary half-edge fluxes of every variable at endpoint ep of boundary edge e
-// (get_boundary_fluxes for a system, boundary_edge_contributions.jl:51-58).
-template <int NV>
+// (get_boundary_fluxes for a system, boundary_edge_contributions.jl:51-58), or (JV)
+// their derivatives along v: the fluxes are linear in u and the Neumann functions
+// affine, so the derivative is the flux of v, or cu . v at the half-edge point.
+template <int NV, int JV>
 __device__ void bedge_fluxes(const SysArgs& a, int e, int ep, double* Q) {
+  const double* src = JV ? a.v : a.u;
   const int i = a.bedges[e], j = a.bedges[a.B + e];
   const double px = a.x[i], py = a.y[i], qx = a.x[j], qy = a.y[j];
   const double lij = sqrt((qx - px) * (qx - px) + (qy - py) * (qy - py));
@@ -145,7 +183,7 @@ __device__ void bedge_fluxes(const SysArgs& a, int e, int ep, double* Q) {
   double al[NV], be[NV], um[NV];
 #pragma unroll
   for (int w = 0; w < NV; ++w) {
-    const double ua = a.u[w + NV * (int64_t)va], ub = a.u[w + NV * (int64_t)vb], uc = a.u[w + NV * (int64_t)vc];
+    const double ua = src[w + NV * (int64_t)va], ub = src[w + NV * (int64_t)vb], uc = src[w + NV * (int64_t)vc];
     al[w] = s[0] * ua + s[1] * ub + s[2] * uc;
     be[w] = s[3] * ua + s[4] * ub + s[5] * uc;
     const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
@@ -164,6 +202,8 @@ __device__ void bedge_fluxes(const SysArgs& a, int e, int ep, double* Q) {
           qyv = qyv + a.K[v][w] * be[w];
         }
       qn = (-qxv) * nx + (-qyv) * ny;
+    } else if (JV) {
+      qn = cond_tangent<NV>(a, nf, um);
     } else {
       qn = cond_eval<NV>(a, nf, mx, my, a.t, um, a.bedge_g ? a.bedge_g[(int64_t)v * 2 * a.B + 2 * e + ep] : 0.0);
     }
@@ -171,7 +211,9 @@ __device__ void bedge_fluxes(const SysArgs& a, int e, int ep, double* Q) {
   }
 }
 
-template <int NV, int BLOCK, int TPT>
+// JV = 1: the Jacobian-vector product; s_u then holds the direction v (the fluxes
+// are linear), and exclusive nodes read their own u from memory for the source.
+template <int NV, int JV, int BLOCK, int TPT>
 __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
   constexpr int TILE = BLOCK * TPT;
   constexpr int MAXN = 2 * BLOCK < 1024 ? 2 * BLOCK : 1024;
@@ -190,7 +232,7 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
     if (k < a.n_bitems) {
       const int w = a.fix_bedge[k];
       double Q[NV];
-      bedge_fluxes<NV>(a, w >> 1, w & 1, Q);
+      bedge_fluxes<NV, JV>(a, w >> 1, w & 1, Q);
 #pragma unroll
       for (int v = 0; v < NV; ++v) a.bflux[(int64_t)k * NV + v] = Q[v];
     }
@@ -201,6 +243,7 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
   const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
   const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
   const int ib0 = 3 * t0;
+  const double* src = JV ? a.v : a.u;
   int g[NS], pi[NS];
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
@@ -214,7 +257,7 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
       s_x[l] = a.x[g[r]];
       s_y[l] = a.y[g[r]];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) s_u[v][l] = a.u[v + NV * (int64_t)g[r]];
+      for (int v = 0; v < NV; ++v) s_u[v][l] = src[v + NV * (int64_t)g[r]];
     }
   }
   if (tid == 0) s_is[nn] = 3 * nt;
@@ -281,27 +324,32 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) a.partial[(int64_t)pi[r] * NV + v] = acc[v];
       } else {
-        double u[NV];
+        double u[NV], dv[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) u[v] = s_u[v][l];
+        for (int v = 0; v < NV; ++v) {
+          dv[v] = s_u[v][l];
+          u[v] = JV ? a.u[v + NV * (int64_t)g[r]] : dv[v];
+        }
         const double V = a.lcv[n0 + l];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) a.out[v + NV * (int64_t)g[r]] = finish_var<NV>(a, v, g[r], acc[v], V, u);
+        for (int v = 0; v < NV; ++v)
+          a.out[v + NV * (int64_t)g[r]] = finish_var<NV, JV>(a, v, g[r], acc[v], V, u, dv);
       }
     }
   }
 }
 
-template <int NV>
+template <int NV, int JV>
 __global__ __launch_bounds__(256) void sys_fix_kernel(SysArgs a) {
   const int fi = blockIdx.x * blockDim.x + threadIdx.x;
   if (fi >= a.n_fix) return;
   const int g = a.fix_nodes[fi];
-  double acc[NV], u[NV];
+  double acc[NV], u[NV], dv[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     acc[v] = 0.0;
     u[v] = a.u[v + NV * (int64_t)g];
+    dv[v] = JV ? a.v[v + NV * (int64_t)g] : 0.0;
   }
   for (int p = a.fix_part_start[fi]; p < a.fix_part_start[fi + 1]; ++p)
 #pragma unroll
@@ -311,7 +359,7 @@ __global__ __launch_bounds__(256) void sys_fix_kernel(SysArgs a) {
     for (int v = 0; v < NV; ++v) acc[v] = acc[v] - a.bflux[(int64_t)k * NV + v];
   const double V = a.cv[g];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) a.out[v + NV * (int64_t)g] = finish_var<NV>(a, v, g, acc[v], V, u);
+  for (int v = 0; v < NV; ++v) a.out[v + NV * (int64_t)g] = finish_var<NV, JV>(a, v, g, acc[v], V, u, dv);
 }
 
 // dirichlet.jl:46-53: variable by variable; each thread owns one node, so updating
@@ -417,21 +465,21 @@ void free_system(fvm_system* s) {
   delete s;
 }
 
-template <int NV>
+template <int NV, int JV>
 int launch_sys(const fvm_system* s, const SysArgs& a, hipStream_t st) {
   const int tt = s->m->tile_tris;
   const int nb256 = (a.n_bitems + 255) / 256, nb512 = (a.n_bitems + 511) / 512;
   if (tt <= 256)
-    hipLaunchKernelGGL((sys_tile_kernel<NV, 256, 1>), dim3((unsigned)xcd_grid(a.n_tiles + nb256)), dim3(256), 0,
+    hipLaunchKernelGGL((sys_tile_kernel<NV, JV, 256, 1>), dim3((unsigned)xcd_grid(a.n_tiles + nb256)), dim3(256), 0,
                        st, a);
   else if (tt <= 512)
-    hipLaunchKernelGGL((sys_tile_kernel<NV, 256, 2>), dim3((unsigned)xcd_grid(a.n_tiles + nb256)), dim3(256), 0,
+    hipLaunchKernelGGL((sys_tile_kernel<NV, JV, 256, 2>), dim3((unsigned)xcd_grid(a.n_tiles + nb256)), dim3(256), 0,
                        st, a);
   else
-    hipLaunchKernelGGL((sys_tile_kernel<NV, 512, 2>), dim3((unsigned)xcd_grid(a.n_tiles + nb512)), dim3(512), 0,
+    hipLaunchKernelGGL((sys_tile_kernel<NV, JV, 512, 2>), dim3((unsigned)xcd_grid(a.n_tiles + nb512)), dim3(512), 0,
                        st, a);
   if (a.n_fix > 0)
-    hipLaunchKernelGGL((sys_fix_kernel<NV>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((sys_fix_kernel<NV, JV>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0, st, a);
   FVM_HIP(hipGetLastError());
   return FVM_OK;
 }
@@ -606,10 +654,28 @@ int fvm_system_rhs(fvm_system* s, const double* d_u, double* d_du, double t, fvm
   a.t = t;
   hipStream_t hs = (hipStream_t)st;
   switch (s->nv) {
-    case 1: return launch_sys<1>(s, a, hs);
-    case 2: return launch_sys<2>(s, a, hs);
-    case 3: return launch_sys<3>(s, a, hs);
-    default: return launch_sys<4>(s, a, hs);
+    case 1: return launch_sys<1, 0>(s, a, hs);
+    case 2: return launch_sys<2, 0>(s, a, hs);
+    case 3: return launch_sys<3, 0>(s, a, hs);
+    default: return launch_sys<4, 0>(s, a, hs);
+  }
+}
+
+int fvm_system_jvp(fvm_system* s, const double* d_u, const double* d_v, double* d_jv, double t,
+                   fvm_stream st) {
+  FVM_CHECK(s && d_u && d_v && d_jv, "fvm_system_jvp: null argument");
+  FVM_CHECK(d_jv != d_u && d_jv != d_v, "fvm_system_jvp: jv must not alias u or v");
+  SysArgs a = s->a;
+  a.u = d_u;
+  a.v = d_v;
+  a.out = d_jv;
+  a.t = t;
+  hipStream_t hs = (hipStream_t)st;
+  switch (s->nv) {
+    case 1: return launch_sys<1, 1>(s, a, hs);
+    case 2: return launch_sys<2, 1>(s, a, hs);
+    case 3: return launch_sys<3, 1>(s, a, hs);
+    default: return launch_sys<4, 1>(s, a, hs);
   }
 }
 

@@ -198,6 +198,7 @@ HIP_API = {
     "fvm_system_destroy": [_vp],
     "fvm_system_rhs": [_vp, _vp, _vp, ctypes.c_double, _vp],
     "fvm_system_dirichlet_apply": [_vp, _vp, ctypes.c_double, _vp],
+    "fvm_system_jvp": [_vp, _vp, _vp, _vp, ctypes.c_double, _vp],
     "fvm_system_compute_flux": [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_double, _vp, _vp],
     "fvm_compute_flux": [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp],
     "fvm_pl_interpolate": [_vp, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp],

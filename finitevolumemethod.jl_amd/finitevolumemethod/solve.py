@@ -455,8 +455,7 @@ def _initial_dt(f, u0, f0, t0, tf, abstol, reltol, n, exponent=1 / 5):
 
 def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kwargs):
     """SteadyFVMProblem: Newton's method on fvm_eqs!(du, u) = 0 with the device
-    Jacobian-vector product (fvm_jvp; a forward difference of fvm_system_rhs for an
-    FVMSystem) inside restarted GMRES.  Dirichlet nodes are
+    Jacobian-vector product (fvm_jvp, fvm_system_jvp) inside restarted GMRES.  Dirichlet nodes are
     set by the Dirichlet functions first and stay fixed (their du is 0).  Stops when
     |F| <= abstol + reltol |F0| and the last correction is below sqrt(reltol) (1 + |u|)."""
     from .linsolve import gmres_device
@@ -472,7 +471,6 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
     t = 0.0
     F = _lib.DeviceArray(n)
     du = _lib.DeviceArray(n)
-    work = tuple(_lib.DeviceArray(n) for _ in range(3)) if sys else None
     sol = ODESolution()
     ctx.rhs(u, F, t)
     r0 = math.sqrt(_lib.dot(F, F))
@@ -491,10 +489,7 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
         if converged():
             break
         it += 1
-        if sys:
-            jv = lambda v, out: ctx.jvp(u, v, out, t, _work=work)  # noqa: E731
-        else:
-            jv = lambda v, out: ctx.jvp(u, v, out, t)  # noqa: E731
+        jv = lambda v, out: ctx.jvp(u, v, out, t)  # noqa: E731
         gmres_device(jv, F, du, rtol=1e-3, restart=50, maxiter=2000)
         _lib.lincomb(u, (1.0, -1.0), (u, du))
         step = math.sqrt(_lib.dot(du, du))

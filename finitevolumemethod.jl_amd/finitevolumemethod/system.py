@@ -261,21 +261,9 @@ class SystemContext:
         _lib.check(_lib.hip_lib().fvm_system_dirichlet_apply(self._handle, u.ptr, float(t), stream),
                    "fvm_system_dirichlet_apply")
 
-    def jvp(self, u, v, jv, t=0.0, stream=None, _work=None):
-        """J_u v by a forward difference of fvm_system_rhs (step sqrt(eps) scaled by |u|/|v|)."""
-        n = self.n
-        w = _work if _work is not None else (_lib.DeviceArray(n), _lib.DeviceArray(n), _lib.DeviceArray(n))
-        up, f0, f1 = w
-        vn = np.sqrt(_lib.dot(v, v))
-        if vn == 0.0:
-            jv.fill_zero()
-            return
-        un = np.sqrt(_lib.dot(u, u))
-        h = np.sqrt(np.finfo(float).eps) * (1.0 + un) / vn
-        _lib.lincomb(up, (1.0, h), (u, v), stream)
-        self.rhs(u, f0, t, stream)
-        self.rhs(up, f1, t, stream)
-        _lib.lincomb(jv, (1.0 / h, -1.0 / h), (f1, f0), stream)
+    def jvp(self, u, v, jv, t=0.0, stream=None):
+        _lib.check(_lib.hip_lib().fvm_system_jvp(self._handle, u.ptr, v.ptr, jv.ptr, float(t), stream),
+                   "fvm_system_jvp")
 
     def __del__(self):
         h = getattr(self, "_handle", None)

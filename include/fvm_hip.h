@@ -345,6 +345,12 @@ int fvm_system_rhs(fvm_system* s, const double* d_u, double* d_du, double t, fvm
 /* update_dirichlet_nodes! for a system (dirichlet.jl:46-53): variable by variable,
  * u[var, i] = a(x_i, y_i, t, u[:, i]) with the variables already updated. */
 int fvm_system_dirichlet_apply(fvm_system* s, double* d_u, double t, fvm_stream st);
+/* Jacobian-vector product of fvm_system_rhs along d_v (what ForwardDiff through
+ * DiffCache computes for implicit steppers on a system, main_equations.jl:56): the
+ * flux of v, the source monomials' derivatives and cu . v for Dudt / Neumann
+ * conditions; 0 at Dirichlet entries and points that are not vertices. */
+int fvm_system_jvp(fvm_system* s, const double* d_u, const double* d_v, double* d_jv, double t,
+                   fvm_stream st);
 /* compute_flux for a system (problem.jl:458-487, is_system branch): out[q * n_vars + v]
  * is variable v's q . n across the edge (d_i[q], d_j[q]); waits for completion. */
 int fvm_system_compute_flux(fvm_system* s, int64_t n, const int32_t* d_i, const int32_t* d_j,

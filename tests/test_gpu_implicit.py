@@ -1,5 +1,5 @@
 """TR-BDF2 on the device (the reference tutorials' TRBDF2): Newton-Krylov on fvm_jvp
-(or, for the ODE templates, the template matrix; for systems, forward differences).
+(fvm_system_jvp for systems; for the ODE templates, the template matrix).
 
 Checks: the plate problem against a tight reference integration of the oracle's RHS
 with the same Dirichlet semantics and against the exact series; a nonlinear problem

@@ -206,8 +206,8 @@ def test_system_jacobian_sparsity_holds_the_jacobian(gpu):
 
 
 def test_steady_system_solves_to_a_zero_residual(gpu):
-    """SteadyFVMProblem(FVMSystem): Newton with GMRES on forward differences of
-    fvm_system_rhs; at the answer the oracle's du vanishes at the free nodes."""
+    """SteadyFVMProblem(FVMSystem): Newton with GMRES on fvm_system_jvp; at the answer
+    the oracle's du vanishes at the free nodes."""
     tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 21, 19, jitter=0.2, seed=4)
     mesh = FVM.FVMGeometry(tri)
     B0 = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u[0], FVM.Dirichlet)
@@ -235,3 +235,63 @@ def test_steady_system_solves_to_a_zero_residual(gpu):
     assert np.linalg.norm(r) <= 1e-7 * np.linalg.norm(sc)
     bnd = tri.boundary_nodes()
     assert np.all(U[0, bnd] == 0.0) and np.allclose(U[1, bnd], tri.x[bnd], rtol=0, atol=0)
+
+
+def _system_jvp(sys, U, Vd, t):
+    ctx = sys.device_context()
+    d_u = FVM.DeviceArray.from_host(sys.flat(U))
+    d_v = FVM.DeviceArray.from_host(sys.flat(Vd))
+    d_jv = FVM.DeviceArray(U.size)
+    ctx.jvp(d_u, d_v, d_jv, t)
+    return sys.unflat(d_jv.to_host())
+
+
+def test_system_jvp_matches_central_differences(gpu):
+    """fvm_system_jvp against central differences of the oracle's system RHS (step
+    1e-5: truncation ~1e-10 of the scale) on the Brusselator and the three-variable
+    system (sources of degree 3, u-dependent Dudt / Neumann / Dirichlet conditions)."""
+    rng = np.random.default_rng(21)
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 19, 17, jitter=0.2, seed=7, single_boundary=False)
+    cases = [(brusselator_system(tri), brusselator_oracle(tri), 0.3)]
+    t3, s3, o3 = _three_variable()
+    cases.append((s3, o3, 0.45))
+    for sys, orc, t in cases:
+        nv, N = sys.initial_condition.shape
+        U = rng.random((nv, N)) + 0.5
+        Vd = rng.standard_normal((nv, N))
+        jv = _system_jvp(sys, U, Vd, t)
+        h = 1e-5
+        fd = (orc.rhs((U + h * Vd).T, t) - orc.rhs((U - h * Vd).T, t)).T / (2 * h)
+        scale = orc.scale(U.T, t).T * (1 + np.abs(Vd)) + np.abs(fd)
+        assert np.all(np.abs(jv - fd) <= 1e-6 * scale + 1e-12), np.max(np.abs(jv - fd) / scale)
+        # Dirichlet entries and their derivative are zero
+        assert np.all(jv[orc.kind == 1] == 0.0)
+
+
+def test_system_jvp_is_exact_for_a_linear_system(gpu):
+    """F(u) = A u + b (linear fluxes, affine sources and conditions): J v = F(v) - F(0)
+    to rounding."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 15, 13, jitter=0.2, seed=3, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    N = tri.num_points()
+    B0 = FVM.BoundaryConditions(mesh, (lambda x, y, t, u, p: 0.5 * u[1] + x, lambda x, y, t, u, p: 1.0 + 0.0 * u[0],
+                                       lambda x, y, t, u, p: -0.25 * u[0], lambda x, y, t, u, p: 0.0 * u[0]),
+                                (FVM.Neumann, FVM.Dirichlet, FVM.Dudt, FVM.Neumann))
+    B1 = FVM.BoundaryConditions(mesh, (lambda x, y, t, u, p: 0.0 * u[0], lambda x, y, t, u, p: u[0] - u[1],
+                                       lambda x, y, t, u, p: y + 0.0 * u[0], lambda x, y, t, u, p: 2.0 * u[1]),
+                                (FVM.Dirichlet, FVM.Neumann, FVM.Neumann, FVM.Dudt))
+    p0 = FVM.FVMProblem(mesh, B0, flux_function=lambda x, y, t, a, b, g, p: (-(a[0] + 0.3 * a[1]), -(b[0] + 0.3 * b[1])),
+                        source_function=lambda x, y, t, u, p: 1.0 - u[0] + 0.3 * u[1],
+                        initial_condition=np.zeros(N), final_time=1.0)
+    p1 = FVM.FVMProblem(mesh, B1, flux_function=lambda x, y, t, a, b, g, p: (-0.5 * a[1], -0.5 * b[1]),
+                        source_function=lambda x, y, t, u, p: 0.5 - u[1] + 0.2 * u[0] + x * y,
+                        initial_condition=np.zeros(N), final_time=1.0)
+    sys = FVM.FVMSystem(p0, p1)
+    rng = np.random.default_rng(4)
+    U = rng.random((2, N))
+    Vd = rng.standard_normal((2, N))
+    jv = _system_jvp(sys, U, Vd, 0.2)
+    F = lambda W: FVM.fvm_eqs(np.zeros_like(W), W, sys, 0.2)  # noqa: E731
+    ref = F(Vd) - F(np.zeros_like(Vd))
+    mag = np.abs(F(Vd)) + np.abs(F(np.zeros_like(Vd))) + 1.0
+    assert np.all(np.abs(jv - ref) <= 1e-12 * mag * 1e3)
