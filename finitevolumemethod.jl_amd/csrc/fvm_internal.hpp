@@ -15,6 +15,9 @@
 //                       4 = all, with the CV-edge normals unnormalised and lengths 1)
 //   FVM_SOLVER_THREADS  workgroup size of the template solver kernels
 //   FVM_SPMV_ROWS       CSR rows per lane group and step in the solver SpMV
+// Revision of the RHS kernels' memory layout, part of fvm_build_info so that measured
+// traffic is matched to the build that produced it (2: fix-up records in fix order).
+#define FVM_RHS_REV 2
 #ifndef FVM_GEO_LEVEL
 #define FVM_GEO_LEVEL 4
 #endif
@@ -247,6 +250,8 @@ struct fvm_mesh {
   std::vector<int32_t> h_node_tri_start;
   std::vector<int32_t> h_part_idx;    // [n_local]
   std::vector<int32_t> h_tile_nodes;  // [n_local]
+  std::vector<int32_t> h_fix_part_start;   // [n_fix + 1]
+  std::vector<int32_t> h_fix_bedge_start;  // [n_fix + 1]
 };
 
 struct fvm_ctx {
@@ -272,6 +277,9 @@ struct fvm_ctx {
   double* partial = nullptr;   // [n_parts] (x2 for the VJP)
   int32_t* lpart = nullptr;    // [n_local] partial slot, or -1 - node code for exclusive entries
   double* bflux = nullptr;     // boundary terms per fix_bedge item (RHS/JVP) or 2 per fix_bvert item (VJP)
+  int32_t* fix_meta = nullptr; // [4][n_fix] per fix node: node, first slot, first boundary item,
+                               // slots << 20 | boundary items << 8 | code
+  double* fix_cv = nullptr;    // [n_fix] control volume of each fix node
   fvm::CondTable cond() const {
     fvm::CondTable C;
     C.n = n_fn;

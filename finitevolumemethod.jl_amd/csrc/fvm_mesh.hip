@@ -209,6 +209,8 @@ int build(const fvm_mesh_desc* d, const int32_t* tri_group, fvm_mesh* m) {
   m->h_tile_nodes.assign(tl->tile_nodes, tl->tile_nodes + m->n_local);
   m->h_perm.assign(tl->perm, tl->perm + T);
   m->h_node_tri_start.assign(tl->node_tri_start, tl->node_tri_start + N + 1);
+  m->h_fix_part_start.assign(tl->fix_part_start, tl->fix_part_start + m->n_fix + 1);
+  m->h_fix_bedge_start.assign(tl->fix_bedge_start, tl->fix_bedge_start + m->n_fix + 1);
   double* sub = nullptr;
   FVM_HIP(hipMalloc((void**)&sub, sizeof(double) * 3 * (size_t)(T > 0 ? T : 1)));
   if (T > 0)

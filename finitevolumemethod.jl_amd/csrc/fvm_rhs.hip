@@ -43,6 +43,8 @@ struct KArgs {
   const int32_t* fix_nodes;
   const int32_t* fix_part_start;
   const int32_t* fix_bedge_start;
+  const int32_t* fix_meta;  // [4][n_fix]: node, first slot, first boundary item, counts and code
+  const double* fix_cv;     // [n_fix]
   const int32_t* fix_bedge;
   const int32_t* fix_bvert_start;
   const int32_t* fix_bvert;
@@ -430,13 +432,14 @@ template <int FK, int MODE>
 __global__ __launch_bounds__(256) void fix_kernel(KArgs a) {
   const int fi = blockIdx.x * blockDim.x + threadIdx.x;
   if (fi >= a.n_fix) return;
-  const int g = a.fix_nodes[fi];
-  const int p0 = a.fix_part_start[fi], p1 = a.fix_part_start[fi + 1];
-  const int b0 = a.fix_bedge_start[fi], b1 = a.fix_bedge_start[fi + 1];
+  // the node's static data in fix order: one coalesced record, one gather (u)
+  const int n = a.n_fix;
+  const int g = a.fix_meta[fi], p0 = a.fix_meta[n + fi], b0 = a.fix_meta[2 * n + fi];
+  const uint32_t w = (uint32_t)a.fix_meta[3 * n + fi];
+  const int p1 = p0 + (int)(w >> 20), b1 = b0 + (int)((w >> 8) & 4095u), c = (int)(w & 255u);
+  const double V = a.fix_cv[fi];
   const double ug = a.u[g];
   const double vg = MODE == M_JVP ? a.v[g] : 0.0;
-  const int c = a.code[g];
-  const double V = a.cv[g];
   double pv[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) pv[k] = p0 + k < p1 ? a.partial[p0 + k] : 0.0;
@@ -707,6 +710,8 @@ KArgs make_args(const fvm_ctx* c) {
   a.fix_nodes = m->fix_nodes;
   a.fix_part_start = m->fix_part_start;
   a.fix_bedge_start = m->fix_bedge_start;
+  a.fix_meta = c->fix_meta;
+  a.fix_cv = c->fix_cv;
   a.fix_bedge = m->fix_bedge;
   a.fix_bvert_start = m->fix_bvert_start;
   a.fix_bvert = m->fix_bvert;
@@ -809,11 +814,18 @@ int dev_copy(T** dst, const T* src, int64_t n) {
   return FVM_OK;
 }
 
+// out[k] = x[idx[k]]
+__global__ __launch_bounds__(256) void gather_kernel(int64_t n, const int32_t* __restrict__ idx,
+                                                     const double* __restrict__ x, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = x[idx[k]];
+}
+
 void free_ctx(fvm_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->d_edge, c->d_bedge, c->log_d_own, c->s_node, c->kind, c->fidx,
                   c->bneu, c->fn, c->node_g, c->bedge_g, c->dir_nodes, c->partial,
-                  c->lpart, c->bflux};
+                  c->lpart, c->bflux, c->fix_meta, c->fix_cv};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete c;
@@ -905,6 +917,30 @@ int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
   {
     const int64_t nb = std::max<int64_t>(m->n_bedge_items, 2 * (int64_t)m->n_bvert_items);
     DC(c->bflux, (const double*)nullptr, nb);
+  }
+  {
+    // per fix node: node, first slot, first boundary item, slots << 20 | items << 8 | code
+    const int64_t nf = m->n_fix;
+    std::vector<int32_t> meta(4 * (size_t)std::max<int64_t>(nf, 1), 0);
+    std::vector<int32_t> hfix(std::max<int64_t>(nf, 1), 0);
+    if (nf > 0) FVM_HIP(hipMemcpy(hfix.data(), m->fix_nodes, sizeof(int32_t) * nf, hipMemcpyDeviceToHost));
+    for (int64_t f = 0; f < nf; ++f) {
+      const int32_t np = m->h_fix_part_start[f + 1] - m->h_fix_part_start[f];
+      const int32_t nb = m->h_fix_bedge_start[f + 1] - m->h_fix_bedge_start[f];
+      FVM_CHECK(np < 4096 && nb < 4096, "fvm_ctx_create: a node shared by 4096 or more tiles");
+      meta[f] = hfix[f];
+      meta[nf + f] = m->h_fix_part_start[f];
+      meta[2 * nf + f] = m->h_fix_bedge_start[f];
+      meta[3 * nf + f] = (int32_t)(((uint32_t)np << 20) | ((uint32_t)nb << 8) | (uint32_t)(uint8_t)code[hfix[f]]);
+    }
+    DC(c->fix_meta, meta.data(), (int64_t)meta.size());
+    DC(c->fix_cv, (const double*)nullptr, std::max<int64_t>(nf, 1));
+    if (nf > 0) {
+      hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, 0, nf, m->fix_nodes,
+                         (const double*)m->cv, c->fix_cv);
+      FVM_HIP(hipGetLastError());
+      FVM_HIP(hipDeviceSynchronize());
+    }
   }
 #undef DC
   return FVM_OK;

@@ -29,8 +29,8 @@ int fvm_last_error(char* buf, size_t n) {
 
 int fvm_build_info(char* buf, size_t n) {
   FVM_CHECK(buf && n > 0, "fvm_build_info: null buffer");
-  std::snprintf(buf, n, "gfx950 geo_level=%d solver_threads=%d spmv_rows=%d", FVM_GEO_LEVEL,
-                FVM_SOLVER_THREADS, FVM_SPMV_ROWS);
+  std::snprintf(buf, n, "gfx950 rhs_rev=%d geo_level=%d solver_threads=%d spmv_rows=%d", FVM_RHS_REV,
+                FVM_GEO_LEVEL, FVM_SOLVER_THREADS, FVM_SPMV_ROWS);
   return FVM_OK;
 }
 
