@@ -76,13 +76,92 @@ __global__ __launch_bounds__(kRedThreads) void reduce_stage2(const double* part,
   if (threadIdx.x == 0) *res = sh[0];
 }
 
+// Several dot products x_j . y in one pass over y (the Gram-Schmidt step of GMRES):
+// each workgroup leaves one partial per vector, reduced in a fixed order (waves by
+// shuffles, then the block's waves in order), then one workgroup per vector sums the
+// kRedBlocks partials in order: bitwise reproducible.
+constexpr int kMaxDots = 64;
+constexpr int kDotChunk = 16;  // vectors per pass (their accumulators live in registers)
+
+struct DotChunk {
+  int k;                        // vectors in this chunk
+  int first;                    // index of the chunk's first vector
+  const double* x[kDotChunk];
+};
+
+__global__ __launch_bounds__(kRedThreads) void mdot_stage1(int64_t n, DotChunk c, const double* __restrict__ y,
+                                                           double* __restrict__ part) {
+  __shared__ double sh[kRedThreads / 64][kDotChunk];
+  double acc[kDotChunk];
+#pragma unroll
+  for (int j = 0; j < kDotChunk; ++j) acc[j] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double yi = y[i];
+#pragma unroll
+    for (int j = 0; j < kDotChunk; ++j)
+      if (j < c.k) acc[j] += c.x[j][i] * yi;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kDotChunk; ++j) {
+    double v = acc[j];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) sh[wave][j] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < c.k) {
+    double v = 0.0;
+    for (int w = 0; w < kRedThreads / 64; ++w) v += sh[w][threadIdx.x];
+    part[(int64_t)(c.first + threadIdx.x) * kRedBlocks + blockIdx.x] = v;
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void mdot_stage2(const double* __restrict__ part, double* __restrict__ res) {
+  __shared__ double sh[kRedThreads];
+  const double* p = part + (int64_t)blockIdx.x * kRedBlocks;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) acc += p[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) res[blockIdx.x] = sh[0];
+}
+
+struct AxpyTerms {
+  int k;
+  double c[kMaxDots];
+  const double* x[kMaxDots];
+};
+
+// y += sum_j c_j x_j
+__global__ __launch_bounds__(256) void maxpy_kernel(int64_t n, double* __restrict__ y, AxpyTerms t) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = y[i];
+    for (int j = 0; j < t.k; ++j) v = v + t.c[j] * t.x[j][i];
+    y[i] = v;
+  }
+}
+
 std::mutex g_mu;
-double* g_scratch = nullptr;  // kRedBlocks partials + 1 result
+double* g_scratch = nullptr;   // kRedBlocks partials + 1 result
+double* g_mscratch = nullptr;  // kMaxDots * kRedBlocks partials + kMaxDots results
 
 int scratch(double** out) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_scratch) FVM_HIP(hipMalloc((void**)&g_scratch, sizeof(double) * (kRedBlocks + 8)));
   *out = g_scratch;
+  return FVM_OK;
+}
+
+int mscratch(double** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_mscratch) FVM_HIP(hipMalloc((void**)&g_mscratch, sizeof(double) * (size_t)kMaxDots * (kRedBlocks + 1)));
+  *out = g_mscratch;
   return FVM_OK;
 }
 
@@ -119,6 +198,49 @@ int fvm_vec_lincomb(int64_t n, double* d_out, int nterms, const double* coefs,
   int64_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(lincomb_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s, n, d_out, t);
+  FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_vec_mdot(int64_t n, int32_t k, const double* const* d_xs, const double* d_y, double* results,
+                 fvm_stream s) {
+  FVM_CHECK(d_xs && d_y && results, "fvm_vec_mdot: null argument");
+  FVM_CHECK(k >= 1 && k <= kMaxDots, "fvm_vec_mdot: 1..64 vectors");
+  for (int j = 0; j < k; ++j) FVM_CHECK(d_xs[j], "fvm_vec_mdot: null input vector");
+  double* buf;
+  int rc = mscratch(&buf);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)s;
+  for (int first = 0; first < k; first += kDotChunk) {
+    DotChunk c;
+    c.first = first;
+    c.k = k - first < kDotChunk ? k - first : kDotChunk;
+    for (int j = 0; j < kDotChunk; ++j) c.x[j] = j < c.k ? d_xs[first + j] : d_xs[first];
+    hipLaunchKernelGGL(mdot_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, st, n, c, d_y, buf);
+  }
+  double* res = buf + (size_t)kMaxDots * kRedBlocks;
+  hipLaunchKernelGGL(mdot_stage2, dim3((unsigned)k), dim3(kRedThreads), 0, st, buf, res);
+  FVM_HIP(hipGetLastError());
+  FVM_HIP(hipMemcpyAsync(results, res, sizeof(double) * (size_t)k, hipMemcpyDeviceToHost, st));
+  FVM_HIP(hipStreamSynchronize(st));
+  return FVM_OK;
+}
+
+int fvm_vec_maxpy(int64_t n, double* d_y, int32_t k, const double* coefs, const double* const* d_xs,
+                  fvm_stream s) {
+  FVM_CHECK(d_y && coefs && d_xs, "fvm_vec_maxpy: null argument");
+  FVM_CHECK(k >= 1 && k <= kMaxDots, "fvm_vec_maxpy: 1..64 vectors");
+  if (n <= 0) return FVM_OK;
+  AxpyTerms t;
+  t.k = k;
+  for (int j = 0; j < k; ++j) {
+    FVM_CHECK(d_xs[j] && d_xs[j] != d_y, "fvm_vec_maxpy: null or aliased input vector");
+    t.c[j] = coefs[j];
+    t.x[j] = d_xs[j];
+  }
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(maxpy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s, n, d_y, t);
   FVM_HIP(hipGetLastError());
   return FVM_OK;
 }

@@ -205,6 +205,8 @@ HIP_API = {
     "fvm_vec_lincomb": [ctypes.c_int64, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), _vp],
     "fvm_vec_mul": [ctypes.c_int64, _vp, _vp, _vp, _vp],
     "fvm_vec_dot": [ctypes.c_int64, _vp, _vp, _dp, _vp],
+    "fvm_vec_mdot": [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_vp), _vp, _dp, _vp],
+    "fvm_vec_maxpy": [ctypes.c_int64, _vp, ctypes.c_int32, _dp, ctypes.POINTER(_vp), _vp],
     "fvm_vec_err_norm": [ctypes.c_int64, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _dp,
                          _vp],
 }
@@ -360,6 +362,23 @@ def dot(x, y, stream=None):
     r = ctypes.c_double(0)
     check(hip_lib().fvm_vec_dot(x.n, x.ptr, y.ptr, ctypes.byref(r), stream), "fvm_vec_dot")
     return r.value
+
+
+def mdot(xs, y, stream=None):
+    """[x . y for x in xs] (one pass over y)."""
+    k = len(xs)
+    ptrs = (_vp * k)(*[x.ptr for x in xs])
+    out = np.empty(k)
+    check(hip_lib().fvm_vec_mdot(y.n, k, ptrs, y.ptr, ptr(out), stream), "fvm_vec_mdot")
+    return out
+
+
+def maxpy(y, coefs, xs, stream=None):
+    """y += sum_j coefs[j] xs[j]."""
+    k = len(xs)
+    ptrs = (_vp * k)(*[x.ptr for x in xs])
+    c = np.ascontiguousarray(coefs, np.float64)
+    check(hip_lib().fvm_vec_maxpy(y.n, y.ptr, k, ptr(c), ptrs, stream), "fvm_vec_maxpy")
 
 
 def err_norm(err, u0, u1, abstol, reltol, stream=None):

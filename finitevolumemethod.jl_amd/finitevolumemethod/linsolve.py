@@ -8,8 +8,10 @@ from . import _lib
 
 def gmres_device(apply_A, b, x, rtol=1e-6, restart=50, maxiter=1000):
     """Restarted GMRES(m) (Saad & Schultz 1986) for A x = b with x0 = 0; the
-    operator is a callable apply_A(v, out) on device vectors.  Returns the
+    operator is a callable apply_A(v, out) on device vectors; m <= 63.  Returns the
     number of iterations."""
+    if not 1 <= restart <= 63:
+        raise ValueError("gmres_device: restart must lie in [1, 63]")
     n = b.n
     V = [_lib.DeviceArray(n) for _ in range(restart + 1)]
     w = _lib.DeviceArray(n)
@@ -34,9 +36,15 @@ def gmres_device(apply_A, b, x, rtol=1e-6, restart=50, maxiter=1000):
         k_done = 0
         for j in range(restart):
             apply_A(V[j], w)
-            for i in range(j + 1):  # modified Gram-Schmidt
-                H[i, j] = _lib.dot(w, V[i])
-                _lib.lincomb(w, (1.0, -H[i, j]), (w, V[i]))
+            # classical Gram-Schmidt applied twice (CGS2: as stable as modified
+            # Gram-Schmidt, Giraud et al. 2005), each pass one fused multi-dot and one
+            # multi-axpy: two host synchronisations instead of j + 1
+            basis = V[:j + 1]
+            h = _lib.mdot(basis, w)
+            _lib.maxpy(w, -h, basis)
+            h2 = _lib.mdot(basis, w)
+            _lib.maxpy(w, -h2, basis)
+            H[:j + 1, j] = h + h2
             H[j + 1, j] = math.sqrt(_lib.dot(w, w))
             if H[j + 1, j] > 0:
                 _lib.lincomb(V[j + 1], (1.0 / H[j + 1, j],), (w,))
@@ -54,9 +62,8 @@ def gmres_device(apply_A, b, x, rtol=1e-6, restart=50, maxiter=1000):
             if abs(g[j + 1]) <= rtol * bnorm or total >= maxiter:
                 break
         y = np.linalg.solve(np.triu(H[:k_done, :k_done]), g[:k_done]) if k_done else np.zeros(0)
-        for i in range(0, k_done, 7):
-            idx = list(range(i, min(k_done, i + 7)))
-            _lib.lincomb(x, [1.0] + [float(y[q]) for q in idx], [x] + [V[q] for q in idx])
+        if k_done:
+            _lib.maxpy(x, y, V[:k_done])
         if abs(g[k_done]) <= rtol * bnorm:
             break
     return total

@@ -366,6 +366,13 @@ int fvm_vec_lincomb(int64_t n, double* d_out, int nterms, const double* coefs,
 /* d_out = d_x * d_y elementwise (d_out may alias an input). */
 int fvm_vec_mul(int64_t n, double* d_out, const double* d_x, const double* d_y, fvm_stream s);
 int fvm_vec_dot(int64_t n, const double* d_x, const double* d_y, double* result, fvm_stream s);
+/* results[j] = d_xs[j] . d_y for j < k <= 64, one pass over d_y (waits for completion;
+ * the Gram-Schmidt step of the Krylov solvers). */
+int fvm_vec_mdot(int64_t n, int32_t k, const double* const* d_xs, const double* d_y, double* results,
+                 fvm_stream s);
+/* d_y += sum_j coefs[j] * d_xs[j], k <= 64 (no d_xs[j] may alias d_y). */
+int fvm_vec_maxpy(int64_t n, double* d_y, int32_t k, const double* coefs, const double* const* d_xs,
+                  fvm_stream s);
 /* sqrt(mean((err_i / (abstol + reltol * max(|u0_i|, |u1_i|)))^2)): the
  * error norm of OrdinaryDiffEq's adaptive step control. */
 int fvm_vec_err_norm(int64_t n, const double* d_err, const double* d_u0, const double* d_u1,
