@@ -49,19 +49,25 @@ class EnsembleSolution:
 
 
 class EnsembleProblem:
-    """EnsembleProblem(prob; prob_func, trajectories) over FVMProblems, or
-    EnsembleProblem(list_of_problems).  prob_func(prob, i, repeat) returns member i."""
+    """EnsembleProblem(prob; prob_func, trajectories) over FVMProblems or FVMSystems, or
+    EnsembleProblem(list_of_problems).  prob_func(prob, i, repeat) returns member i
+    (1-based i, as SciML's); trajectories may also be given to solve."""
 
     def __init__(self, prob, prob_func=None, trajectories=None):
+        from .system import FVMSystem
+
+        self._prob, self._prob_func = prob, prob_func
         if isinstance(prob, (list, tuple)):
             self.problems = list(prob)
         else:
-            n = int(trajectories or 1)
-            f = prob_func if prob_func is not None else (lambda p, i, repeat: p)
-            self.problems = [f(prob, i, 1) for i in range(n)]
-        if not self.problems or not all(isinstance(p, FVMProblem) for p in self.problems):
-            raise TypeError("EnsembleProblem members must be FVMProblems")
+            self._make(trajectories or 1)
+        if not self.problems or not all(isinstance(p, (FVMProblem, FVMSystem)) for p in self.problems):
+            raise TypeError("EnsembleProblem members must be FVMProblems or FVMSystems")
         self._batch = None
+
+    def _make(self, trajectories):
+        f = self._prob_func if self._prob_func is not None else (lambda p, i, repeat: p)
+        self.problems = [f(self._prob, i + 1, 1) for i in range(int(trajectories))]
 
     def __len__(self):
         return len(self.problems)
@@ -70,13 +76,25 @@ class EnsembleProblem:
         """The packed device ensemble (an fvm_batch)."""
         return DeviceBatch(self.problems, dt, tile_triangles, autotune, graph_steps)
 
-    def solve(self, alg=None, *, dt=None, autotune=True, graph_steps=100, **kwargs):
-        """Forward Euler for every member from its initial_time to its final_time with
-        its own dt (a scalar, a list, or None for explicit_dt); every member must take
-        the same number of steps.  Returns an EnsembleSolution."""
+    def solve(self, alg=None, *, dt=None, autotune=True, graph_steps=100, trajectories=None, **kwargs):
+        """Euler() (the default): every member advanced together in the packed device
+        batch from its initial_time to its final_time with its own dt (a scalar, a
+        list, or None for explicit_dt); every member must take the same number of
+        steps; the solutions hold the initial and final states.  Any other algorithm
+        (Tsit5(), TRBDF2(), ...): the members are solved one after the other on the
+        device, each with its own adaptive steps (SciML's EnsembleSerial), keyword
+        arguments passed to solve.  Returns an EnsembleSolution."""
+        from .solve import solve as solve_one
+        from .system import FVMSystem
+
+        if trajectories is not None and not isinstance(self._prob, (list, tuple)):
+            self._make(trajectories)
         alg = alg if alg is not None else Euler()
-        if not isinstance(alg, Euler):
-            raise TypeError("ensembles are advanced with Euler() steps")
+        if not isinstance(alg, Euler) or any(isinstance(p, FVMSystem) for p in self.problems):
+            tm = _lib.Timer()
+            tm.start()
+            sols = [solve_one(p, alg, dt=dt, **kwargs) for p in self.problems]
+            return EnsembleSolution(u=sols, stats=dict(ms=tm.stop(), mode="serial"))
         P = len(self.problems)
         if dt is None:
             dt = alg.dt

@@ -121,8 +121,16 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
     given, OrdinaryDiffEq's defaults); returns ODESolution (t, u host arrays).
     SteadyFVMProblem: see solve_steady (its own defaults).  Templates forward to
     their own solver."""
+    from .ensemble import EnsembleProblem
     from .templates import AbstractFVMTemplate, _SteadyTemplate
 
+    if isinstance(prob, EnsembleProblem):  # solve(ensemble_prob, alg; trajectories, ...)
+        if saveat is not None:
+            kwargs["saveat"] = saveat
+        for k, v in (("abstol", abstol), ("reltol", reltol)):
+            if v is not None:
+                kwargs[k] = v
+        return prob.solve(alg, dt=dt, **kwargs)
     if isinstance(prob, _SteadyTemplate):  # a LinearProblem: only the linear solver's options apply
         return prob.solve(alg, **kwargs)
     if isinstance(prob, SteadyFVMProblem):

@@ -336,6 +336,8 @@ typedef struct fvm_system_desc {
                                         or NULL (0) */
 } fvm_system_desc;
 
+/* A system holds scratch buffers (partials, boundary terms), like a context: calls on
+ * one system must not overlap on different streams. */
 int fvm_system_create(const fvm_mesh* m, const fvm_system_desc* d, fvm_system** out);
 int fvm_system_destroy(fvm_system* s);
 /* fvm_eqs!(du, u, p, t) for an FVMSystem (main_equations.jl:28-81 with the system

@@ -136,3 +136,28 @@ def test_ensemble_solve_and_errors(gpu):
                            final_time=1.0)
     with pytest.raises(FVM.FVMError):
         EN.DeviceBatch([q, other])
+
+
+def test_ensemble_with_adaptive_algorithm_solves_members_in_turn(gpu):
+    """solve(EnsembleProblem(prob; prob_func), Tsit5(); trajectories) solves each member
+    on the device with its own adaptive steps; each equals that member solved alone."""
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 15, 15, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    u0 = np.sin(np.pi * tri.x) * np.sin(np.pi * tri.y)
+    base = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 0.1 + 0.0 * u,
+                          initial_condition=u0, final_time=0.2)
+
+    def prob_func(p, i, repeat):
+        return FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, q, d=0.05 * i: d + 0.0 * u,
+                              initial_condition=u0, final_time=0.2)
+
+    ens = FVM.EnsembleProblem(base, prob_func=prob_func)
+    sol = FVM.solve(ens, FVM.Tsit5(), trajectories=3, saveat=0.1)
+    assert len(sol) == 3 and sol.stats["mode"] == "serial"
+    for i, s in enumerate(sol.u):
+        alone = FVM.solve(prob_func(base, i + 1, 1), FVM.Tsit5(), saveat=0.1)
+        assert len(s.u) == len(alone.u) == 3
+        assert all(np.array_equal(a, b) for a, b in zip(s.u, alone.u))
+    # decay grows with the member's D
+    assert sol.u[0].u[-1].max() > sol.u[1].u[-1].max() > sol.u[2].u[-1].max()

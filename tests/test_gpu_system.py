@@ -295,3 +295,48 @@ def test_system_jvp_is_exact_for_a_linear_system(gpu):
     ref = F(Vd) - F(np.zeros_like(Vd))
     mag = np.abs(F(Vd)) + np.abs(F(np.zeros_like(Vd))) + 1.0
     assert np.all(np.abs(jv - ref) <= 1e-12 * mag * 1e3)
+
+
+def test_system_c_abi_rejects_bad_arguments(gpu):
+    """fvm_system_* and the query entry points turn bad input into FVM_ERR_ARG with a
+    message (no kernel runs on it): aliasing, unknown variables counts, out-of-range
+    function indices and source exponents, and edges / triangles not in the mesh."""
+    import ctypes
+
+    from finitevolumemethod import _lib
+    from finitevolumemethod.system import system_desc, system_physics
+
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 9, 8, jitter=0.2, seed=1, single_boundary=False)
+    sys = brusselator_system(tri)
+    ctx = sys.device_context()
+    d = FVM.DeviceArray(2 * tri.num_points())
+    with pytest.raises(FVM.FVMError):
+        ctx.rhs(d, d, 0.0)
+    with pytest.raises(FVM.FVMError):
+        ctx.jvp(d, d, d, 0.0)
+    L = _lib.hip_lib()
+    ph = system_physics(sys)
+
+    def create(**changes):
+        bad = dict(ph)
+        bad.update(changes)
+        desc, keep = system_desc(bad)
+        h = ctypes.c_void_p()
+        st = L.fvm_system_create(sys.mesh.handle, ctypes.byref(desc), ctypes.byref(h))
+        if st == 0:
+            L.fvm_system_destroy(h)
+        return st, _lib.last_error()
+
+    assert create()[0] == 0
+    st, msg = create(n_vars=5, flux_k=np.eye(5))
+    assert st != 0 and "n_vars" in msg
+    fidx = ph["node_fidx"].copy()
+    fidx[ph["node_kind"] != 0] = 99
+    st, msg = create(node_fidx=fidx)
+    assert st != 0 and "node_fidx" in msg
+    st, msg = create(terms=[(0, (4, 0), 1.0)])
+    assert st != 0 and "exponents" in msg
+    with pytest.raises(FVM.FVMError, match="not in the mesh"):
+        FVM.compute_flux(sys, 0, tri.num_points() - 1, sys.initial_condition, 0.0)
+    with pytest.raises(FVM.FVMError, match="not in the mesh"):
+        FVM.pl_interpolate(sys, (0, 1, 2 * tri.num_points()), sys.initial_condition, 0.1, 0.1)
