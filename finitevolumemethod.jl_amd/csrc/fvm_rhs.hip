@@ -282,9 +282,13 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
   __shared__ double s_v[MODE == M_JVP ? MAXN : 1];
   __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
   constexpr bool RC = rebuilds(FK);  // part of the triangle record rebuilt from coordinates
-  __shared__ double s_x[RC ? MAXN : 1];
-  __shared__ double s_y[RC ? MAXN : 1];
-  __shared__ double s_q[3 * TILE];
+  // the node coordinates are dead once the fluxes are formed: the fluxes reuse their
+  // space (fewer LDS bytes per workgroup, more workgroups per CU)
+  constexpr int XY = RC ? 2 * MAXN : 0;
+  __shared__ double s_xyq[XY > 3 * TILE ? XY : 3 * TILE];
+  double* const s_x = s_xyq;
+  double* const s_y = s_xyq + MAXN;
+  double* const s_q = s_xyq;
   __shared__ uint16_t s_inc[3 * TILE];
   __shared__ int s_is[MAXN + 1];
   const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
@@ -386,22 +390,30 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
     if (idx < 3 * nt) s_inc[idx] = iv[k];
   }
   __syncthreads();
+  double Q[TPT][3];
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
     const int lt = tid + r * BLOCK;
+    Q[r][0] = Q[r][1] = Q[r][2] = 0.0;
     if (lt < nt) {
       const int ia = L[r] & 1023u, ib = (L[r] >> 10) & 1023u, ic = (L[r] >> 20) & 1023u;
       if (RC) rebuild_record<kGeo>(s_x[ia], s_y[ia], s_x[ib], s_y[ib], s_x[ic], s_y[ic], P[r]);
-      double Q[3];
       tri_fluxes<FK, MODE>(a, P[r], s_u[ia], s_u[ib], s_u[ic],
                            MODE == M_JVP ? s_v[ia] : 0.0, MODE == M_JVP ? s_v[ib] : 0.0,
                            MODE == M_JVP ? s_v[ic] : 0.0,
                            FK == FVM_FLUX_D_NODAL ? s_D[ia] : 0.0,
                            FK == FVM_FLUX_D_NODAL ? s_D[ib] : 0.0,
-                           FK == FVM_FLUX_D_NODAL ? s_D[ic] : 0.0, Ds[r], Q);
-      s_q[3 * lt] = Q[0];
-      s_q[3 * lt + 1] = Q[1];
-      s_q[3 * lt + 2] = Q[2];
+                           FK == FVM_FLUX_D_NODAL ? s_D[ic] : 0.0, Ds[r], Q[r]);
+    }
+  }
+  if (RC) __syncthreads();  // every thread has read the coordinates that s_q overwrites
+#pragma unroll
+  for (int r = 0; r < TPT; ++r) {
+    const int lt = tid + r * BLOCK;
+    if (lt < nt) {
+      s_q[3 * lt] = Q[r][0];
+      s_q[3 * lt + 1] = Q[r][1];
+      s_q[3 * lt + 2] = Q[r][2];
     }
   }
   __syncthreads();
