@@ -551,12 +551,17 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
   constexpr int MAXN = node_cap(BLOCK);
   constexpr int CW = FK == FVM_FLUX_D_NODAL ? 6 : 3;  // u-adjoint (+ log D adjoint) per vertex
   __shared__ double s_u[MAXN];
-  __shared__ double s_w[MAXN];
-  __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
   constexpr bool RC = rebuilds(FK);  // part of the triangle record rebuilt from coordinates
-  __shared__ double s_x[RC ? MAXN : 1];
-  __shared__ double s_y[RC ? MAXN : 1];
-  __shared__ double s_c[CW * TILE];
+  // w~, D, x and y are dead once the per-vertex adjoints are formed: the adjoints
+  // reuse their space (fewer LDS bytes per workgroup, more workgroups per CU)
+  constexpr int ND = FK == FVM_FLUX_D_NODAL ? MAXN : 0;
+  constexpr int PRE = MAXN + ND + (RC ? 2 * MAXN : 0);
+  __shared__ double s_big[PRE > CW * TILE ? PRE : CW * TILE];
+  double* const s_w = s_big;
+  double* const s_D = s_big + MAXN;
+  double* const s_x = s_big + MAXN + ND;
+  double* const s_y = s_x + MAXN;
+  double* const s_c = s_big;
   const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
   const int wid = xcd_remap(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
@@ -606,9 +611,12 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
     }
   }
   __syncthreads();
+  double cr[TPT][CW];
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
     const int lt = tid + r * BLOCK;
+#pragma unroll
+    for (int v = 0; v < CW; ++v) cr[r][v] = 0.0;
     if (lt < nt) {
       const int id[3] = {(int)(L[r] & 1023u), (int)((L[r] >> 10) & 1023u), (int)((L[r] >> 20) & 1023u)};
       if (RC) rebuild_record<kGeo>(s_x[id[0]], s_y[id[0]], s_x[id[1]], s_y[id[1]], s_x[id[2]], s_y[id[2]], P[r]);
@@ -650,9 +658,18 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
       for (int v = 0; v < 3; ++v) {
         double ub = p[v] * abar + p[3 + v] * bbar;
         if (FK == FVM_FLUX_D_POLY_U) ub = ub + p[6 + v] * gbar;  // gamma only enters D(u)
-        s_c[CW * lt + v] = ub;
-        if (FK == FVM_FLUX_D_NODAL) s_c[CW * lt + 3 + v] = lbar[v] * Dn[v];
+        cr[r][v] = ub;
+        if (FK == FVM_FLUX_D_NODAL) cr[r][3 + v] = lbar[v] * Dn[v];
       }
+    }
+  }
+  __syncthreads();  // every thread has read the weights / D / coordinates that s_c overwrites
+#pragma unroll
+  for (int r = 0; r < TPT; ++r) {
+    const int lt = tid + r * BLOCK;
+    if (lt < nt) {
+#pragma unroll
+      for (int v = 0; v < CW; ++v) s_c[CW * lt + v] = cr[r][v];
     }
   }
   __syncthreads();
