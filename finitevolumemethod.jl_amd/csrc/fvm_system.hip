@@ -49,6 +49,9 @@ struct SysArgs {
   const int32_t* inc_start;
   const uint16_t* inc;
   const int32_t* fix_nodes;
+  const int32_t* fix_meta;  // [4][n_fix]: node, first slot, first boundary item,
+                            // slots << 20 | items << 8 | the variables' codes, 2 bits each
+  const double* fix_cv;     // [n_fix]
   const int32_t* fix_part_start;
   const int32_t* fix_bedge_start;
   const int32_t* fix_bedge;
@@ -145,12 +148,11 @@ __device__ __forceinline__ double cond_tangent(const SysArgs& a, int f, const do
   return r;
 }
 
-// source_contributions.jl:49-68 for variable v of node g, or (JV) its derivative
-// along dv
+// source_contributions.jl:49-68 for variable v of node g (code c), or (JV) its
+// derivative along dv
 template <int NV, int JV>
-__device__ __forceinline__ double finish_var(const SysArgs& a, int v, int g, double acc, double V,
-                                             const double* u, const double* dv) {
-  const int c = a.code[(int64_t)v * a.N + g];
+__device__ __forceinline__ double finish_var_c(const SysArgs& a, int v, int g, int c, double acc, double V,
+                                               const double* u, const double* dv) {
   if (c == 0) return acc / V + (JV ? source_tangent<NV>(a, v, u, dv) : source_eval<NV>(a, v, g, u));
   if (c == 2) {
     const int f = a.fidx[(int64_t)v * a.N + g];
@@ -158,6 +160,12 @@ __device__ __forceinline__ double finish_var(const SysArgs& a, int v, int g, dou
     return cond_eval<NV>(a, f, a.x[g], a.y[g], a.t, u, a.node_g ? a.node_g[(int64_t)v * a.N + g] : 0.0);
   }
   return 0.0;
+}
+
+template <int NV, int JV>
+__device__ __forceinline__ double finish_var(const SysArgs& a, int v, int g, double acc, double V,
+                                             const double* u, const double* dv) {
+  return finish_var_c<NV, JV>(a, v, g, a.code[(int64_t)v * a.N + g], acc, V, u, dv);
 }
 
 // Boundary half-edge fluxes of every variable at endpoint ep of boundary edge e
@@ -219,9 +227,12 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
   constexpr int MAXN = 2 * BLOCK < 1024 ? 2 * BLOCK : 1024;
   constexpr int NS = MAXN / BLOCK;
   __shared__ double s_u[NV][MAXN];
-  __shared__ double s_x[MAXN];
-  __shared__ double s_y[MAXN];
-  __shared__ double s_q[NV][3 * TILE];
+  // the node coordinates are dead once the fluxes are formed: the fluxes reuse their space
+  constexpr int XY = 2 * MAXN, QN = NV * 3 * TILE;
+  __shared__ double s_xyq[XY > QN ? XY : QN];
+  double* const s_x = s_xyq;
+  double* const s_y = s_xyq + MAXN;
+  double* const s_q = s_xyq;  // [NV][3 TILE]
   __shared__ uint16_t s_inc[3 * TILE];
   __shared__ int s_is[MAXN + 1];
   const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
@@ -244,27 +255,25 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
   const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
   const int ib0 = 3 * t0;
   const double* src = JV ? a.v : a.u;
-  int g[NS], pi[NS];
+  // every global load of the tile before the first LDS store
+  int g[NS], pi[NS], is[NS];
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
     const int l = tid + r * BLOCK;
     g[r] = 0;
     pi[r] = -1;
+    is[r] = 0;
     if (l < nn) {
       g[r] = a.tile_nodes[n0 + l];
       pi[r] = a.lpart[n0 + l];
-      s_is[l] = a.inc_start[n0 + l] - ib0;
-      s_x[l] = a.x[g[r]];
-      s_y[l] = a.y[g[r]];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) s_u[v][l] = src[v + NV * (int64_t)g[r]];
+      is[r] = a.inc_start[n0 + l] - ib0;
     }
   }
-  if (tid == 0) s_is[nn] = 3 * nt;
+  uint16_t iv[3 * TPT];
 #pragma unroll
   for (int k = 0; k < 3 * TPT; ++k) {
     const int idx = tid + k * BLOCK;
-    if (idx < 3 * nt) s_inc[idx] = a.inc[ib0 + idx];
+    iv[k] = idx < 3 * nt ? a.inc[ib0 + idx] : (uint16_t)0;
   }
   uint32_t L[TPT];
 #pragma unroll
@@ -272,7 +281,39 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
     const int lt = tid + r * BLOCK;
     L[r] = lt < nt ? a.loc[t0 + lt] : 0u;
   }
+  double xg[NS], yg[NS], ug[NS][NV];
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const int l = tid + r * BLOCK;
+    xg[r] = yg[r] = 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) ug[r][v] = 0.0;
+    if (l < nn) {
+      xg[r] = a.x[g[r]];
+      yg[r] = a.y[g[r]];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) ug[r][v] = src[v + NV * (int64_t)g[r]];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const int l = tid + r * BLOCK;
+    if (l < nn) {
+      s_is[l] = is[r];
+      s_x[l] = xg[r];
+      s_y[l] = yg[r];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) s_u[v][l] = ug[r][v];
+    }
+  }
+  if (tid == 0) s_is[nn] = 3 * nt;
+#pragma unroll
+  for (int k = 0; k < 3 * TPT; ++k) {
+    const int idx = tid + k * BLOCK;
+    if (idx < 3 * nt) s_inc[idx] = iv[k];
+  }
   __syncthreads();
+  double Qr[TPT][3][NV];
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
     const int lt = tid + r * BLOCK;
@@ -299,9 +340,25 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
               qyv = qyv + a.K[v][w] * be[w];
             }
           // (qx nx + qy ny) l with q = -K (alpha, beta)
-          s_q[v][3 * lt + e] = ((-qxv) * nx + (-qyv) * ny) * l;
+          Qr[r][e][v] = ((-qxv) * nx + (-qyv) * ny) * l;
         }
       }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Qr[r][e][v] = 0.0;
+    }
+  }
+  __syncthreads();  // every thread has read the coordinates that s_q overwrites
+#pragma unroll
+  for (int r = 0; r < TPT; ++r) {
+    const int lt = tid + r * BLOCK;
+    if (lt < nt) {
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s_q[v * 3 * TILE + 3 * lt + e] = Qr[r][e][v];
     }
   }
   __syncthreads();
@@ -318,7 +375,7 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
         const int lt = w >> 2, sl = w & 3;
         const int prev = 3 * lt + (sl == 0 ? 2 : sl - 1), cur = 3 * lt + sl;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] = acc[v] + s_q[v][prev] - s_q[v][cur];
+        for (int v = 0; v < NV; ++v) acc[v] = acc[v] + s_q[v * 3 * TILE + prev] - s_q[v * 3 * TILE + cur];
       }
       if (pi[r] >= 0) {
 #pragma unroll
@@ -327,7 +384,7 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
         double u[NV], dv[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          dv[v] = s_u[v][l];
+          dv[v] = ug[r][v];
           u[v] = JV ? a.u[v + NV * (int64_t)g[r]] : dv[v];
         }
         const double V = a.lcv[n0 + l];
@@ -343,7 +400,11 @@ template <int NV, int JV>
 __global__ __launch_bounds__(256) void sys_fix_kernel(SysArgs a) {
   const int fi = blockIdx.x * blockDim.x + threadIdx.x;
   if (fi >= a.n_fix) return;
-  const int g = a.fix_nodes[fi];
+  // the node's static data in fix order: one coalesced record, gathers of u only
+  const int n = a.n_fix;
+  const int g = a.fix_meta[fi], p0 = a.fix_meta[n + fi], b0 = a.fix_meta[2 * n + fi];
+  const uint32_t m = (uint32_t)a.fix_meta[3 * n + fi];
+  const int p1 = p0 + (int)(m >> 20), b1 = b0 + (int)((m >> 8) & 4095u);
   double acc[NV], u[NV], dv[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
@@ -351,15 +412,23 @@ __global__ __launch_bounds__(256) void sys_fix_kernel(SysArgs a) {
     u[v] = a.u[v + NV * (int64_t)g];
     dv[v] = JV ? a.v[v + NV * (int64_t)g] : 0.0;
   }
-  for (int p = a.fix_part_start[fi]; p < a.fix_part_start[fi + 1]; ++p)
+  for (int p = p0; p < p1; ++p)
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = acc[v] + a.partial[(int64_t)p * NV + v];
-  for (int k = a.fix_bedge_start[fi]; k < a.fix_bedge_start[fi + 1]; ++k)
+  for (int k = b0; k < b1; ++k)
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = acc[v] - a.bflux[(int64_t)k * NV + v];
-  const double V = a.cv[g];
+  const double V = a.fix_cv[fi];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) a.out[v + NV * (int64_t)g] = finish_var<NV, JV>(a, v, g, acc[v], V, u, dv);
+  for (int v = 0; v < NV; ++v)
+    a.out[v + NV * (int64_t)g] = finish_var_c<NV, JV>(a, v, g, (int)((m >> (2 * v)) & 3u), acc[v], V, u, dv);
+}
+
+// out[k] = x[idx[k]]
+__global__ __launch_bounds__(256) void sys_gather_kernel(int64_t n, const int32_t* __restrict__ idx,
+                                                         const double* __restrict__ x, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = x[idx[k]];
 }
 
 // dirichlet.jl:46-53: variable by variable; each thread owns one node, so updating
@@ -614,6 +683,42 @@ int fvm_system_create(const fvm_mesh* m, const fvm_system_desc* d, fvm_system** 
   up(&d_part, (const double*)nullptr, (int64_t)std::max(m->n_parts, 1) * nv);
   up(&d_bflux, (const double*)nullptr, (int64_t)std::max(m->n_bedge_items, 1) * nv);
   up(&s->dir_nodes, dir.data(), (int64_t)dir.size());
+  // per fix node: node, first slot, first boundary item, slots << 20 | items << 8 | codes
+  int32_t* d_meta = nullptr;
+  double* d_fcv = nullptr;
+  {
+    const int64_t nf = m->n_fix;
+    std::vector<int32_t> hfix(std::max<int64_t>(nf, 1), 0), meta(4 * (size_t)std::max<int64_t>(nf, 1), 0);
+    if (!rc && nf > 0 && hipMemcpy(hfix.data(), m->fix_nodes, sizeof(int32_t) * nf, hipMemcpyDeviceToHost) != hipSuccess) {
+      set_error("fvm_system_create: reading the fix-up nodes");
+      rc = FVM_ERR_HIP;
+    }
+    for (int64_t f = 0; !rc && f < nf; ++f) {
+      const int32_t np = m->h_fix_part_start[f + 1] - m->h_fix_part_start[f];
+      const int32_t nb = m->h_fix_bedge_start[f + 1] - m->h_fix_bedge_start[f];
+      if (np >= 4096 || nb >= 4096) {
+        set_error("fvm_system_create: a node shared by 4096 or more tiles");
+        rc = FVM_ERR_ARG;
+        break;
+      }
+      uint32_t codes = 0;
+      for (int v = 0; v < nv; ++v) codes |= (uint32_t)(uint8_t)code[(size_t)v * N + hfix[f]] << (2 * v);
+      meta[f] = hfix[f];
+      meta[nf + f] = m->h_fix_part_start[f];
+      meta[2 * nf + f] = m->h_fix_bedge_start[f];
+      meta[3 * nf + f] = (int32_t)(((uint32_t)np << 20) | ((uint32_t)nb << 8) | codes);
+    }
+    up(&d_meta, meta.data(), (int64_t)meta.size());
+    up(&d_fcv, (const double*)nullptr, std::max<int64_t>(nf, 1));
+    if (!rc && nf > 0) {
+      hipLaunchKernelGGL(sys_gather_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, 0, nf, m->fix_nodes,
+                         (const double*)m->cv, d_fcv);
+      if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        set_error("fvm_system_create: gathering the fix-up volumes");
+        rc = FVM_ERR_HIP;
+      }
+    }
+  }
   up(&s->dir_mask, dir_mask.data(), (int64_t)dir_mask.size());
   if (rc) {
     free_system(s);
@@ -636,6 +741,8 @@ int fvm_system_create(const fvm_mesh* m, const fvm_system_desc* d, fvm_system** 
   a.bedge_g = d_bg;
   a.partial = d_part;
   a.bflux = d_bflux;
+  a.fix_meta = d_meta;
+  a.fix_cv = d_fcv;
   *out = s;
   return FVM_OK;
 }
