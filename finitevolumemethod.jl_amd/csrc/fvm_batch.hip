@@ -35,7 +35,7 @@ struct fvm_batch {
   double* du = nullptr;           // [N] scratch
   int8_t* code = nullptr;         // [N] node codes (0 free, 1 Dirichlet, 2 Dudt, 3 not a vertex)
   int32_t* fidx = nullptr;        // [N]
-  double* fn = nullptr;           // condition table (5 * n_fn)
+  double* fn = nullptr;           // condition table (6 * n_fn)
   double* node_g = nullptr;
   int32_t n_fn = 0;
   bool time_dependent = false;
@@ -98,7 +98,7 @@ struct Packed {
   std::vector<double> d_edge, d_bedge, log_d, s_node, node_g, bedge_g;
   std::vector<int8_t> node_kind;
   std::vector<int32_t> node_fidx, bneu;
-  std::vector<double> c0, cu, ct, cx, cy;
+  std::vector<double> c0, cu, ct, cx, cy, lam;
   bool any_s_node = false, any_node_g = false, any_bedge_g = false;
   fvm_mesh_desc md{};
   fvm_physics_desc pd{};
@@ -179,7 +179,8 @@ int pack(int32_t P, const fvm_mesh_desc* meshes, const fvm_physics_desc* ph, fvm
       k.ct.push_back(q.fn_ct[f]);
       k.cx.push_back(q.fn_cx ? q.fn_cx[f] : 0.0);
       k.cy.push_back(q.fn_cy ? q.fn_cy[f] : 0.0);
-      if (q.fn_ct[f] != 0.0) b->time_dependent = true;
+      k.lam.push_back(q.fn_lam ? q.fn_lam[f] : 0.0);
+      if (q.fn_ct[f] != 0.0 || k.lam.back() != 0.0) b->time_dependent = true;
     }
     if (fk == FVM_FLUX_D_EDGE) {
       k.d_edge.insert(k.d_edge.end(), q.d_edge, q.d_edge + 3 * t);
@@ -233,6 +234,7 @@ int pack(int32_t P, const fvm_mesh_desc* meshes, const fvm_physics_desc* ph, fvm
   k.pd.fn_ct = k.ct.data();
   k.pd.fn_cx = k.cx.data();
   k.pd.fn_cy = k.cy.data();
+  k.pd.fn_lam = k.lam.data();
   k.pd.node_g = k.any_node_g ? k.node_g.data() : nullptr;
   k.pd.bedge_g = k.any_bedge_g ? k.bedge_g.data() : nullptr;
   return FVM_OK;
@@ -289,6 +291,7 @@ int launch_step(fvm_batch* b, double* u, double t, hipStream_t s) {
   C.ct = b->fn + 2 * b->n_fn;
   C.cx = b->fn + 3 * b->n_fn;
   C.cy = b->fn + 4 * b->n_fn;
+  C.lam = b->fn + 5 * b->n_fn;
   C.node_g = b->node_g;
   C.bedge_g = nullptr;
   const double t_new = t + (b->dt.empty() ? 0.0 : b->dt[0]);
@@ -426,13 +429,14 @@ int fvm_batch_create(int32_t n_problems, const fvm_mesh_desc* meshes, const fvm_
   std::vector<int8_t> code(N);
   FVM_HIP(hipMemcpy(code.data(), b->ctx->kind, sizeof(int8_t) * N, hipMemcpyDeviceToHost));
   b->n_fn = k.pd.n_functions;
-  std::vector<double> fn(5 * (size_t)std::max(b->n_fn, 1), 0.0);
+  std::vector<double> fn(6 * (size_t)std::max(b->n_fn, 1), 0.0);
   for (int f = 0; f < b->n_fn; ++f) {
     fn[f] = k.c0[f];
     fn[b->n_fn + f] = k.cu[f];
     fn[2 * b->n_fn + f] = k.ct[f];
     fn[3 * b->n_fn + f] = k.cx[f];
     fn[4 * b->n_fn + f] = k.cy[f];
+    fn[5 * b->n_fn + f] = k.lam[f];
   }
   if ((rc = dcopy(&b->dtv, dtv.data(), N)) || (rc = dcopy(&b->du, (const double*)nullptr, N)) ||
       (rc = dcopy(&b->tile_dt, tdt.data(), (int64_t)tdt.size())) ||

@@ -160,7 +160,8 @@ __device__ __forceinline__ void rebuild_record(double px, double py, double qx, 
   }
 }
 
-// Condition functions a(x,y,t,u) = c0 + cu u + ct t + cx x + cy y + g.
+// Condition functions a(x,y,t,u) = c0 + cu u + ct t + cx x + cy y + g, with g
+// multiplied by exp(lam t) where lam (may be null) is nonzero.
 struct CondTable {
   int32_t n;
   const double* c0;
@@ -168,6 +169,7 @@ struct CondTable {
   const double* ct;
   const double* cx;
   const double* cy;
+  const double* lam;      // may be null
   const double* node_g;   // may be null
   const double* bedge_g;  // may be null
 };
@@ -178,6 +180,7 @@ __device__ __forceinline__ double eval_cond(const CondTable& C, int f, double x,
   v = v + C.ct[f] * t;
   v = v + C.cx[f] * x;
   v = v + C.cy[f] * y;
+  if (C.lam && C.lam[f] != 0.0) return v + g * exp(C.lam[f] * t);
   return v + g;
 }
 
@@ -269,7 +272,7 @@ struct fvm_ctx {
   int8_t* kind = nullptr;     // 0 none, 1 Dirichlet, 2 Dudt, 3 not a vertex
   int32_t* fidx = nullptr;
   int32_t* bneu = nullptr;    // [B]
-  double* fn = nullptr;       // 5 * n_functions: c0, cu, ct, cx, cy
+  double* fn = nullptr;       // 6 * n_functions: c0, cu, ct, cx, cy, lam
   double* node_g = nullptr;
   double* bedge_g = nullptr;
   int32_t n_fn = 0;
@@ -289,6 +292,7 @@ struct fvm_ctx {
     C.ct = fn + 2 * n_fn;
     C.cx = fn + 3 * n_fn;
     C.cy = fn + 4 * n_fn;
+    C.lam = fn + 5 * n_fn;
     C.node_g = node_g;
     C.bedge_g = bedge_g;
     return C;

@@ -907,13 +907,14 @@ int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
   DC(c->bneu, ph->bedge_neumann_fidx, B);
   {
     const int n = ph->n_functions;
-    std::vector<double> fn(5 * (size_t)(n > 0 ? n : 1), 0.0);
+    std::vector<double> fn(6 * (size_t)(n > 0 ? n : 1), 0.0);
     for (int f = 0; f < n; ++f) {
       fn[f] = ph->fn_c0[f];
       fn[n + f] = ph->fn_cu[f];
       fn[2 * n + f] = ph->fn_ct[f];
       fn[3 * n + f] = ph->fn_cx ? ph->fn_cx[f] : 0.0;
       fn[4 * n + f] = ph->fn_cy ? ph->fn_cy[f] : 0.0;
+      fn[5 * n + f] = ph->fn_lam ? ph->fn_lam[f] : 0.0;
     }
     DC(c->fn, fn.data(), (int64_t)fn.size());
   }

@@ -46,7 +46,7 @@ class PhysicsDesc(ctypes.Structure):
                 ("node_kind", _i8p), ("node_fidx", _i32p), ("bedge_neumann_fidx", _i32p),
                 ("n_functions", ctypes.c_int32), ("fn_c0", _dp), ("fn_cu", _dp),
                 ("fn_ct", _dp), ("fn_cx", _dp), ("fn_cy", _dp), ("node_g", _dp),
-                ("bedge_g", _dp)]
+                ("bedge_g", _dp), ("fn_lam", _dp)]
 
 
 class _Tiles(ctypes.Structure):

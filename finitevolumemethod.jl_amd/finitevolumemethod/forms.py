@@ -235,11 +235,12 @@ def _const_over(c, n):
     return None
 
 
-def _trace(fn, nvars, x, y, extra_args, sym_positions):
-    """Call fn(x, y, *args) with Sym variables at sym_positions of args."""
+def _trace(fn, nvars, x, y, extra_args, sym_positions, tvar=None):
+    """Call fn(x, y, *args) with Sym variables at sym_positions of args (variable
+    tvar, if given, stands for t: exp of it is traced)."""
     args = list(extra_args)
     for k, pos in enumerate(sym_positions):
-        args[pos] = Sym.var(nvars, k)
+        args[pos] = Sym.var(nvars, k, tvar)
     try:
         r = fn(x, y, *args)
     except UnsupportedClosure:
@@ -255,25 +256,17 @@ def _as_sym(r, nvars):
 
 # ---------------------------------------------------------------------------
 def classify_condition(fn, x, y):
-    """a(x, y, t, u) = g(x, y) + cu u + ct t over the points (x, y).
-
-    Returns (g[len(x)], cu, ct)."""
+    """a(x, y, t, u) over the points (x, y): g(x, y) + cu u + ct t, or
+    c0 + cu u + ct t + g(x, y) exp(lam t).  Returns dict(g, c0, cu, ct, lam)."""
     x = np.asarray(x, np.float64)
     y = np.asarray(y, np.float64)
     n = len(x)
     if n == 0:
-        return np.zeros(0), 0.0, 0.0
-    r = _as_sym(_trace(fn, 2, x, y, [None, None], [0, 1]), 2)  # vars: t, u
-    for e in r.nonzero_exponents():
-        if e not in ((0, 0), (1, 0), (0, 1)):
-            raise UnsupportedClosure("condition functions must be affine in t and u "
-                                     "(a = g(x, y) + cu*u + ct*t)")
-    cu = _const_over(r.coef((0, 1)), n)
-    ct = _const_over(r.coef((1, 0)), n)
-    if cu is None or ct is None:
-        raise UnsupportedClosure("the u and t coefficients of a condition must not depend on x, y")
-    g = np.broadcast_to(np.asarray(r.coef((0, 0)), np.float64), (n,)).copy()
-    return g, cu, ct
+        return dict(g=np.zeros(0), c0=0.0, cu=0.0, ct=0.0, lam=0.0)
+    r = _as_sym(_trace(fn, 2, x, y, [None, None], [0, 1], tvar=0), 2)  # vars: t, u
+    out = _condition_from_sym(r, 2, n)
+    out["cu"] = out["cu"][0]
+    return out
 
 
 def classify_source(fn, x, y):
@@ -476,22 +469,11 @@ def classify_system_source(S, nv, x, y):
     return s_node, terms, st
 
 
-def classify_system_condition(fn, nv, x, y):
-    """Trace a condition a(x, y, t, u) (bound to p) of an nv-variable system over
-    the points (x, y): g + sum_w cu[w] u[w] + ct t, where g is g(x, y), or
-    c0 + g(x, y) exp(lam t).  Returns dict(g, c0, cu, ct, lam)."""
-    x = np.asarray(x, np.float64)
-    y = np.asarray(y, np.float64)
-    n = len(x)
-    nvars = 1 + nv
-    if n == 0:
-        return dict(g=np.zeros(0), c0=0.0, cu=[0.0] * nv, ct=0.0, lam=0.0)
-    t = Sym.var(nvars, 0, 0)
-    u = tuple(Sym.var(nvars, 1 + w, 0) for w in range(nv))
-    r = _call(fn, x, y, t, u)
-    r = r if isinstance(r, Sym) else Sym.const(nvars, r)
+def _condition_from_sym(r, nvars, n):
+    """g, c0, cu (one per u variable), ct, lam of a traced condition with variables
+    (t, u_0, ...): affine in t and u plus at most one g(x, y) exp(lam t) term."""
     zero = (0,) * nvars
-    cu, ct, lam = [0.0] * nv, 0.0, 0.0
+    cu, ct, lam = [0.0] * (nvars - 1), 0.0, 0.0
     g_exp = None
     for k, c in r.terms.items():
         e, rate = k
@@ -506,7 +488,7 @@ def classify_system_condition(fn, nv, x, y):
         elif e == _unit(nvars, 0):
             ct = _uniform(c, n, "the t coefficient of a condition")
         else:
-            w = next((w for w in range(nv) if e == _unit(nvars, 1 + w)), None)
+            w = next((w for w in range(nvars - 1) if e == _unit(nvars, 1 + w)), None)
             if w is None:
                 raise UnsupportedClosure("conditions must be affine in t and u")
             cu[w] = _uniform(c, n, "the u coefficients of a condition")
@@ -517,3 +499,20 @@ def classify_system_condition(fn, nv, x, y):
     c0 = _uniform(c_const, n, "with a g(x, y) exp(lam t) term, the constant part of a condition")
     g = np.broadcast_to(np.asarray(g_exp, np.float64), (n,)).copy()
     return dict(g=g, c0=c0, cu=cu, ct=ct, lam=lam)
+
+
+def classify_system_condition(fn, nv, x, y):
+    """Trace a condition a(x, y, t, u) (bound to p) of an nv-variable system over
+    the points (x, y): g + sum_w cu[w] u[w] + ct t, where g is g(x, y), or
+    c0 + g(x, y) exp(lam t).  Returns dict(g, c0, cu, ct, lam)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    n = len(x)
+    nvars = 1 + nv
+    if n == 0:
+        return dict(g=np.zeros(0), c0=0.0, cu=[0.0] * nv, ct=0.0, lam=0.0)
+    t = Sym.var(nvars, 0, 0)
+    u = tuple(Sym.var(nvars, 1 + w, 0) for w in range(nv))
+    r = _call(fn, x, y, t, u)
+    r = r if isinstance(r, Sym) else Sym.const(nvars, r)
+    return _condition_from_sym(r, nvars, n)

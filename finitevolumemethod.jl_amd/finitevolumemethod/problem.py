@@ -152,6 +152,7 @@ def physics_desc(phys_kwargs):
     d.fn_c0, d.fn_cu, d.fn_ct = (_lib.ptr(arr(v)) for v in fn[:3])
     d.fn_cx = _lib.ptr(arr(fn[3]) if len(fn) > 3 else None)
     d.fn_cy = _lib.ptr(arr(fn[4]) if len(fn) > 4 else None)
+    d.fn_lam = _lib.ptr(arr(fn[5]) if len(fn) > 5 else None)
     d.node_g = _lib.ptr(arr(phys_kwargs.get("node_g")))
     d.bedge_g = _lib.ptr(arr(phys_kwargs.get("bedge_g")))
     return d, keep
@@ -206,12 +207,14 @@ class DeviceContext:
 
 
 def conditions_physics(conds, tri):
-    """Flattened conditions plus the traced condition functions (a = g + cu u + ct t)."""
+    """Flattened conditions plus the traced condition functions
+    (a = c0 + cu u + ct t + g, or + g exp(lam t), g per location)."""
     kind, fidx, bneu = conds.flatten(tri)
     nfn = len(conds.functions)
     c0 = np.zeros(max(nfn, 1))
     cu = np.zeros(max(nfn, 1))
     ct = np.zeros(max(nfn, 1))
+    lam = np.zeros(max(nfn, 1))
     node_g = np.zeros(tri.num_points())
     bedge_g = np.zeros(2 * len(tri.boundary_edges))
     any_node_g = any_bedge_g = False
@@ -220,25 +223,27 @@ def conditions_physics(conds, tri):
     mx, my = _half_edge_points(tri)
     edge_groups = forms.group_indices(bneu)
 
-    def record(f, cu_f, ct_f):
-        if f in seen and seen[f] != (cu_f, ct_f):
+    def record(f, r):
+        key = (r["c0"], r["cu"], r["ct"], r["lam"])
+        if f in seen and seen[f] != key:
             raise forms.UnsupportedClosure(f"condition function {f} traced differently at two places")
-        seen[f] = (cu_f, ct_f)
-        cu[f], ct[f] = cu_f, ct_f
+        seen[f] = key
+        c0[f], cu[f], ct[f], lam[f] = key
 
     for f, nodes in node_groups.items():
-        g, cu_f, ct_f = forms.classify_condition(conds.functions[f], tri.x[nodes], tri.y[nodes])
-        record(f, cu_f, ct_f)
-        node_g[nodes] = g
-        any_node_g = any_node_g or bool(np.any(g != 0))
+        r = forms.classify_condition(conds.functions[f], tri.x[nodes], tri.y[nodes])
+        record(f, r)
+        node_g[nodes] = r["g"]
+        any_node_g = any_node_g or bool(np.any(r["g"] != 0))
     for f, edges in edge_groups.items():
         pts = np.stack([2 * edges, 2 * edges + 1], 1).ravel()
-        g, cu_f, ct_f = forms.classify_condition(conds.functions[f], mx[pts], my[pts])
-        record(f, cu_f, ct_f)
-        bedge_g[pts] = g
-        any_bedge_g = any_bedge_g or bool(np.any(g != 0))
+        r = forms.classify_condition(conds.functions[f], mx[pts], my[pts])
+        record(f, r)
+        bedge_g[pts] = r["g"]
+        any_bedge_g = any_bedge_g or bool(np.any(r["g"] != 0))
+    zeros = np.zeros(max(nfn, 1))
     return dict(node_kind=kind, node_fidx=fidx, bedge_neumann_fidx=bneu,
-                fn=(c0[:max(nfn, 1)], cu, ct), node_g=node_g if any_node_g else None,
+                fn=(c0, cu, ct, zeros, zeros, lam), node_g=node_g if any_node_g else None,
                 bedge_g=bedge_g if any_bedge_g else None)
 
 

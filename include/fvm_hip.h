@@ -122,7 +122,8 @@ typedef struct fvm_physics_desc {
   const int8_t* node_kind;      /* [n_points] FVM_NODE_*; a node in both Dicts is Dirichlet */
   const int32_t* node_fidx;     /* [n_points] condition function of Dirichlet / Dudt nodes */
   const int32_t* bedge_neumann_fidx; /* [n_boundary_edges] function of a Neumann edge, -1 otherwise */
-  int32_t n_functions;          /* condition functions a(x,y,t,u) = c0 + cu u + ct t + cx x + cy y + g */
+  int32_t n_functions;          /* condition functions a(x,y,t,u) = c0 + cu u + ct t + cx x + cy y + g,
+                                   or + g exp(lam t) where fn_lam is given and nonzero */
   const double* fn_c0;
   const double* fn_cu;
   const double* fn_ct;
@@ -130,6 +131,7 @@ typedef struct fvm_physics_desc {
   const double* fn_cy;          /* may be NULL (0) */
   const double* node_g;         /* [n_points] g at Dirichlet / Dudt nodes, or NULL (0) */
   const double* bedge_g;        /* [2*n_boundary_edges] g at Neumann half-edge points, or NULL */
+  const double* fn_lam;         /* [n_functions] exp rate of the g term, or NULL (0) */
 } fvm_physics_desc;
 
 int fvm_ctx_create(const fvm_mesh* m, const fvm_physics_desc* phys, fvm_ctx** out);

@@ -259,3 +259,58 @@ def gray_scott_oracle(tri):
                         lambda v, x, y, t, u: (0.04 * (1 - u[0]) - u[0] * u[1] ** 2 if v == 0
                                                else -0.1 * u[1] + u[0] * u[1] ** 2),
                         lambda f, x, y, t, u: 0.0)
+
+
+def annulus_triangulation(R1=0.2, R2=1.0, nr=14, seed=0):
+    """An annulus R1 <= r <= R2: concentric rings of points, Delaunay, the triangles of
+    the hole removed; boundary segment 0 is the outer circle, 1 the inner (the
+    domain on the left of every boundary edge)."""
+    pts = []
+    for k in range(nr + 1):
+        r = R1 + (R2 - R1) * k / nr
+        m = max(12, int(round(2 * np.pi * r / ((R2 - R1) / nr))))
+        th = 2 * np.pi * np.arange(m) / m + 0.37 * k
+        pts += list(zip(r * np.cos(th), r * np.sin(th)))
+    P = np.array(pts)
+    full = FVM.triangulate(P[:, 0], P[:, 1])
+    T = full.triangles
+    cx = P[T, 0].mean(1)
+    cy = P[T, 1].mean(1)
+    T = T[np.hypot(cx, cy) > R1]
+    count = {}
+    for a, b, c in T:
+        for e in ((a, b), (b, c), (c, a)):
+            count[e] = count.get(e, 0) + 1
+    bedges = [e for e in count if (e[1], e[0]) not in count]
+    bedges = np.array(sorted(bedges), np.int32)
+    mid = np.hypot(P[bedges, 0].mean(1), P[bedges, 1].mean(1))
+    seg = (mid < 0.5 * (R1 + R2)).astype(np.int32)
+    return FVM.Triangulation(P[:, 0], P[:, 1], T, bedges, seg)
+
+
+def annulus_problem(variant="tutorial", nr=14, final_time=2.0):
+    """diffusion_equation_on_an_annulus.jl:54-78 on the annulus 0.2 <= r <= 1: D = 1,
+    outer boundary Neumann zero, inner boundary Dirichlet 50 (1 - exp(-t/2)), the
+    tutorial's three-Gaussian initial condition.  variant "exp" exercises the
+    other condition kinds with an exp(lam t) term: outer Neumann
+    0.25 exp(-x - t/2), inner Dudt y exp(t/2) - u/4."""
+    tri = annulus_triangulation(nr=nr)
+    mesh = FVM.FVMGeometry(tri)
+    if variant == "tutorial":
+        fns = (lambda x, y, t, u, p: 0.0 * u, lambda x, y, t, u, p: 50 * (1 - np.exp(-t / 2)) + 0.0 * u)
+        types = (FVM.Neumann, FVM.Dirichlet)
+    else:
+        fns = (lambda x, y, t, u, p: 0.25 * np.exp(-x - t / 2) + 0.0 * u,
+               lambda x, y, t, u, p: y * np.exp(0.5 * t) - 0.25 * u)
+        types = (FVM.Neumann, FVM.Dudt)
+    BCs = FVM.BoundaryConditions(mesh, fns, types)
+    x, y = tri.x, tri.y
+    u0 = (10 * np.exp(-25 * ((x + 0.5) ** 2 + (y + 0.5) ** 2))
+          - 5 * np.exp(-50 * ((x + 0.3) ** 2 + (y + 0.5) ** 2))
+          - 10 * np.exp(-45 * ((x - 0.5) ** 2 + (y - 0.5) ** 2)))
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 1.0 + 0.0 * u,
+                          initial_condition=u0, final_time=final_time)
+    orc = oracle_for(prob, dict(
+        flux=lambda x, y, t, a, b, g, T: (-a, -b),
+        cond=lambda f, x, y, t, u: prob.conditions.functions[f](x, y, t, u)))
+    return prob, orc, u0

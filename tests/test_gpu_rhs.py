@@ -11,8 +11,8 @@ import pytest
 
 import finitevolumemethod as FVM
 from oracle import geometry as oracle_geometry
-from problems import (diffusion_problem, dxy_problem, heat_convection_problem, mixed_problem,
-                      nodal_problem, nonlinear_problem)
+from problems import (annulus_problem, diffusion_problem, dxy_problem, heat_convection_problem,
+                      mixed_problem, nodal_problem, nonlinear_problem)
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -162,3 +162,39 @@ def test_rebuilt_record_matches_stored_geometry(gpu, x0, y0):
         assert np.all(R[:, 21:24] == 1.0)
     else:
         assert np.array_equal(R[:, 15:], P[:, 15:])
+
+
+@pytest.mark.parametrize("variant", ["tutorial", "exp"])
+def test_conditions_with_exponentials_in_time(gpu, variant):
+    """Conditions c0 + cu u + ct t + g(x, y) exp(lam t) on a domain with a hole: the
+    annulus tutorial's inner Dirichlet 50 (1 - exp(-t/2))
+    (diffusion_equation_on_an_annulus.jl:55), and Neumann / Dudt conditions of that
+    form, against the oracle calling the closures themselves."""
+    prob, orc, u0 = annulus_problem(variant)
+    u = u0 + np.random.default_rng(3).random(len(u0))
+    for t in (0.0, 0.7, 1.9):
+        du = FVM.fvm_eqs(np.zeros_like(u), u, prob, t)
+        assert_parity(du, orc.rhs(u, t), orc.scale(u, t))
+        got = FVM.update_dirichlet_nodes(u.copy(), t, prob)
+        np.testing.assert_allclose(got, orc.update_dirichlet(u, t), rtol=1e-15, atol=1e-13)
+
+
+def test_annulus_trajectory_matches_oracle(gpu):
+    """SSPRK3 (dt = 2e-4, 150 steps) with the Dirichlet callback after every step:
+    the inner boundary follows 50 (1 - exp(-t/2)) and every node the oracle's
+    trajectory stepped by the same scheme."""
+    dt, nsteps = 2e-4, 150
+    prob, orc, u0 = annulus_problem("tutorial", final_time=dt * nsteps)
+    sol = FVM.solve(prob, FVM.SSPRK3(dt=dt))
+    tri = prob.mesh.triangulation
+    inner = np.unique(tri.boundary_edges[tri.boundary_segments == 1])
+    u, t = u0.copy(), 0.0
+    for _ in range(nsteps):
+        u1 = u + dt * orc.rhs(u, t)
+        u2 = 0.75 * u + 0.25 * u1 + 0.25 * dt * orc.rhs(u1, t + dt)
+        u = 1 / 3 * u + 2 / 3 * u2 + 2 / 3 * dt * orc.rhs(u2, t + 0.5 * dt)
+        t = t + dt
+        u = orc.update_dirichlet(u, t)
+    assert len(sol.u) == nsteps + 1 and sol.t[-1] == pytest.approx(t, abs=1e-15)
+    np.testing.assert_allclose(sol.u[-1][inner], 50 * (1 - np.exp(-t / 2)), rtol=1e-13)
+    assert np.max(np.abs(sol.u[-1] - u)) <= 1e-11 * np.max(np.abs(u))

@@ -205,9 +205,13 @@ def test_closure_tracing_forms():
     s_node, poly, st = forms.classify_source(lambda x, y, t, u: u * (1 - u), np.zeros(3), np.zeros(3))
     assert s_node is None and poly == [0.0, 1.0, -1.0, 0.0] and st == 0.0
     x = np.array([0.0, 1.0, 2.0])
-    g, cu, ct = forms.classify_condition(lambda x, y, t, u: x * y + u - 0.5, x, x + 1)
-    np.testing.assert_array_equal(g, x * (x + 1) - 0.5)
-    assert (cu, ct) == (1.0, 0.0)
+    r = forms.classify_condition(lambda x, y, t, u: x * y + u - 0.5, x, x + 1)
+    np.testing.assert_array_equal(r["g"], x * (x + 1) - 0.5)
+    assert (r["cu"], r["ct"], r["c0"], r["lam"]) == (1.0, 0.0, 0.0, 0.0)
+    # the annulus tutorial's inner boundary: 50 (1 - exp(-t / 2)) = 50 - 50 exp(-t / 2)
+    r = forms.classify_condition(lambda x, y, t, u: 50 * (1 - np.exp(-t / 2)) + 0.0 * u, x, x)
+    assert (r["c0"], r["lam"], r["cu"], r["ct"]) == (50.0, -0.5, 0.0, 0.0)
+    np.testing.assert_array_equal(r["g"], np.full(3, -50.0))
     with pytest.raises(forms.UnsupportedClosure):
         forms.classify_condition(lambda x, y, t, u: x * t * u, x, x)
     D = forms.classify_flux(lambda x, y, t, a, b, g: (-80e-6 * a, -80e-6 * b))

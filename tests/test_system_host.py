@@ -115,10 +115,12 @@ def test_untraceable_closures_raise(fn, kind):
             forms.classify_system_condition(fn, 2, x, y)
 
 
-def test_scalar_tracing_rejects_exponentials_it_cannot_run():
-    """The scalar kernels have no exp(lam t) factor: such a condition is refused."""
+def test_scalar_conditions_trace_exponentials_in_time():
+    """exp(lam t) conditions trace for scalar problems too; two rates do not."""
+    r = forms.classify_condition(lambda x, y, t, u: np.exp(-t) + 0.0 * u, np.array([0.5]), np.array([0.5]))
+    assert r["lam"] == -1.0 and r["g"][0] == 1.0 and r["c0"] == 0.0
     with pytest.raises(forms.UnsupportedClosure):
-        forms.classify_condition(lambda x, y, t, u: np.exp(-t) + 0.0 * u, np.array([0.5]), np.array([0.5]))
+        forms.classify_condition(lambda x, y, t, u: np.exp(-t) + np.exp(-2 * t), np.array([0.5]), np.array([0.5]))
 
 
 def test_diffusion_only_problem_is_an_invalid_flux(mesh):
