@@ -118,43 +118,58 @@ def vjp_ratio(n_side, reps=20):
                 rhs_ms=round(t_rhs, 4), vjp_ms=round(t_vjp, 4), ratio=round(t_vjp / t_rhs, 3))
 
 
-def template_solve(n_side, rtol=1e-10):
-    """The steady template workload: PoissonsEquation on an n_side x n_side jittered
-    mesh of [0, 1]^2, homogeneous Dirichlet, D = exp(0.5 sin 4 pi x cos 4 pi y),
-    f ~ U[-1, 1].  Time to |b - A x| <= rtol |b| for each device solver (after one
-    warm-up solve), and the SpMV rates of the solver operator."""
-    import finitevolumemethod as FVM
-    from finitevolumemethod import _lib
+def _solve_all(dt, rtol, tag):
+    """Each device solver on one template operator (after one warm-up solve):
+    (per-solver stats, host solutions)."""
     from finitevolumemethod import templates as TM
 
-    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n_side, n_side, jitter=0.25, seed=SEED + 3)
-    mesh = FVM.FVMGeometry(tri)
-    N = tri.num_points()
-    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
-    f = np.random.default_rng(SEED + 4).uniform(-1.0, 1.0, N)
-    t0 = time.time()
-    prob = FVM.PoissonsEquation(
-        mesh, BCs, diffusion_function=lambda x, y, p: np.exp(0.5 * np.sin(4 * np.pi * x) * np.cos(4 * np.pi * y)),
-        source_function=lambda x, y, p: p, source_parameters=f)
-    t_build = time.time() - t0
-    dt = prob.problem
-    out = dict(workload=f"PoissonsEquation {n_side}x{n_side} jittered, D=exp(0.5 sin4pix cos4piy), "
-                        "f~U[-1,1], homogeneous Dirichlet", nodes=int(N), nnz=int(dt.nnz),
-               rtol=rtol, build_seconds=round(t_build, 3))
-    sols = {}
+    out, sols = {}, {}
     for name, alg in (("cg_fp64", TM.CG()), ("mixed_fp32_fp64", TM.MixedCG()),
+                      ("mixed_restart_fp32_fp64", TM.MixedCG(update="restart")),
                       ("bicgstab_fp64", TM.BiCGStab())):
         dt.solve(alg, rtol=rtol)
         x, st = dt.solve(alg, rtol=rtol)
         sols[name] = x.to_host()
         out[name] = dict(seconds=round(st["seconds"], 4), iterations=st["iterations"],
                          outer=st["outer"], residual=st["residual"], converged=st["converged"])
-        log(f"template {name}: {st}")
+        log(f"template {tag} {name}: {st}")
     ref = sols["cg_fp64"]
     out["max_rel_diff_vs_cg"] = {k: float(np.linalg.norm(v - ref) / np.linalg.norm(ref))
                                  for k, v in sols.items() if k != "cg_fp64"}
     out["mixed_speedup_vs_cg"] = round(out["cg_fp64"]["seconds"] / out["mixed_fp32_fp64"]["seconds"], 3)
-    x = _lib.DeviceArray.from_host(ref)
+    return out, sols
+
+
+def template_solve(n_side, rtol=1e-10, direct=True):
+    """The steady template workload: PoissonsEquation and MeanExitTimeProblem on an
+    n_side x n_side jittered mesh of [0, 1]^2, homogeneous Dirichlet,
+    D = exp(0.5 sin 4 pi x cos 4 pi y), f ~ U[-1, 1] (Poisson).  Time to
+    |b - A x| <= rtol |b| for each device solver (after one warm-up solve), the SpMV
+    rates of the solver operator, and (direct=True) each solution's relative error
+    against an FP64 sparse LU solve (SuperLU, COLAMD ordering) of the same A and b on
+    the host."""
+    import finitevolumemethod as FVM
+    from finitevolumemethod import _lib
+
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n_side, n_side, jitter=0.25, seed=SEED + 3)
+    mesh = FVM.FVMGeometry(tri)
+    N = tri.num_points()
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    f = np.random.default_rng(SEED + 4).uniform(-1.0, 1.0, N)
+    D = lambda x, y, p: np.exp(0.5 * np.sin(4 * np.pi * x) * np.cos(4 * np.pi * y))  # noqa: E731
+    t0 = time.time()
+    prob = FVM.PoissonsEquation(mesh, BCs, diffusion_function=D, source_function=lambda x, y, p: p,
+                                source_parameters=f)
+    t_build = time.time() - t0
+    met = FVM.MeanExitTimeProblem(mesh, BCs, diffusion_function=D)
+    dt = prob.problem
+    out = dict(workload=f"PoissonsEquation and MeanExitTimeProblem {n_side}x{n_side} jittered, "
+                        "D=exp(0.5 sin4pix cos4piy), f~U[-1,1] (Poisson), homogeneous Dirichlet",
+               nodes=int(N), nnz=int(dt.nnz), rtol=rtol, build_seconds=round(t_build, 3))
+    res, sols = _solve_all(dt, rtol, "poisson")
+    out.update(res)
+    out["mean_exit_time"], met_sols = _solve_all(met.problem, rtol, "mean_exit_time")
+    x = _lib.DeviceArray.from_host(sols["cg_fp64"])
     y = _lib.DeviceArray(N)
     rows = N
     for which, key, per_nnz, per_row in ((1, "spmv_fp64", 12, 20), (2, "spmv_fp32", 8, 12)):
@@ -168,6 +183,27 @@ def template_solve(n_side, rtol=1e-10):
         ms = tm.stop() / reps
         out[key] = dict(us=round(ms * 1e3, 2),
                         GBps=round((per_nnz * dt.nnz + per_row * rows) / (ms * 1e-3) / 1e9, 1))
+    if direct:
+        import scipy.sparse.linalg as spla
+
+        t0 = time.time()
+        A = prob.A
+        lu = spla.splu(A.tocsc(), permc_spec="COLAMD")
+        A2 = met.A
+        same = (A.shape == A2.shape and np.array_equal(A.indptr, A2.indptr)
+                and np.array_equal(A.indices, A2.indices) and np.array_equal(A.data, A2.data))
+        lu2 = lu if same else spla.splu(A2.tocsc(), permc_spec="COLAMD")
+        errs = {}
+        for tag, M, b, fac, ss in (("poisson", A, prob.b, lu, sols),
+                                   ("mean_exit_time", A2, met.b, lu2, met_sols)):
+            xd = fac.solve(b)
+            nd = np.linalg.norm(xd)
+            errs[tag] = dict(direct_residual=float(np.linalg.norm(M @ xd - b) / np.linalg.norm(b)),
+                             **{k: float(np.linalg.norm(v - xd) / nd) for k, v in ss.items()})
+        out["error_vs_fp64_direct"] = dict(
+            solver="scipy SuperLU (splu, COLAMD), FP64, on the host copy of the device-assembled A, b",
+            seconds=round(time.time() - t0, 1), shared_factorisation=bool(same), **errs)
+        log(f"direct FP64 check {out['error_vs_fp64_direct']}")
     return out
 
 
@@ -349,6 +385,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--template-side", type=int, default=1000,
                     help="PoissonsEquation solve workload side (0: skip)")
+    ap.add_argument("--no-direct", action="store_true",
+                    help="skip the host sparse-LU error check of the template solves")
     ap.add_argument("--ensemble", type=int, default=256, help="ensemble members (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     args = ap.parse_args()
@@ -423,7 +461,7 @@ def main():
         out["system_rhs"] = system_bench(mesh)
         log(f"system RHS {out['system_rhs']}")
         if args.template_side > 0:
-            out["template_solve"] = template_solve(args.template_side)
+            out["template_solve"] = template_solve(args.template_side, direct=not args.no_direct)
         if args.ensemble > 0:
             out["ensemble"] = ensemble_bench(args.ensemble, args.ensemble_steps)
     if not args.no_cpu:

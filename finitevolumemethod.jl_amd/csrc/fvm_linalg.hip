@@ -22,6 +22,10 @@
 //                       M, vectors and arithmetic in FP32 (dot products summed in
 //                       FP64), stopped at a relative drop of inner_rtol; the
 //                       residual and the solution are FP64.
+//   FVM_SOLVE_CG_RELIABLE  one FP32 CG sequence with reliable updates: the FP32
+//                       correction is folded into the FP64 x and the residual
+//                       recomputed in FP64 whenever it has dropped by inner_rtol
+//                       (default 0.1), keeping the search direction.
 //   FVM_SOLVE_BICGSTAB  FP64 BiCGStab on M with Jacobi right preconditioning.
 // Every reduction finishes in the last workgroup to arrive, in a fixed order, so
 // solver scalars stay on the device and results are bitwise reproducible; the
@@ -222,7 +226,14 @@ __global__ __launch_bounds__(256) void assemble_kernel(AsmArgs a) {
 }
 
 // M = -V A on free rows and columns, identity on fixed rows; Jacobi inverse of M
-// and the weights that turn a residual of M back into one of A.
+// and the weights that turn a residual of M back into one of A.  The FP32 copy is
+// kept in difference form: off-diagonal m_ij as they are, and in the diagonal slot
+// the row sum s_i = sum_j m_ij (in FP64, then rounded), so that
+// (M p)_i = sum_{j != i} m_ij (p_j - p_i) + s_i p_i.  The two forms are equal in
+// exact arithmetic; in FP32 the direct form loses about eps |M| |p| to
+// cancellation, which for a smooth p (a diffusion operator's row sums are near
+// zero) is kappa(M) times the size of M p itself, while the difference form's
+// rounding stays relative to the small differences.
 __global__ __launch_bounds__(256) void scaled_kernel(int64_t N, const int32_t* __restrict__ row_ptr,
                                                      const int32_t* __restrict__ col,
                                                      const double* __restrict__ valA,
@@ -233,7 +244,7 @@ __global__ __launch_bounds__(256) void scaled_kernel(int64_t N, const int32_t* _
                                                      double* __restrict__ wv, float* __restrict__ wv32) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= N) return;
-  double dM = 1.0;
+  double dM = 1.0, rsum = 0.0;
   const bool fr = fixed[r] != 0;
   for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
     const int c = col[k];
@@ -241,9 +252,11 @@ __global__ __launch_bounds__(256) void scaled_kernel(int64_t N, const int32_t* _
     if (fr) v = c == r ? 1.0 : 0.0;
     else v = fixed[c] ? 0.0 : -vol[r] * valA[k];
     valM[k] = v;
-    valM32[k] = (float)v;
+    rsum += v;
     if (c == r) dM = v;
   }
+  for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+    valM32[k] = col[k] == r ? (float)rsum : (float)valM[k];
   const double di = dM != 0.0 ? 1.0 / dM : 1.0;
   dinv[r] = di;
   dinv32[r] = (float)di;
@@ -308,12 +321,18 @@ __device__ __forceinline__ V group_sum(V a) {
   return a;
 }
 
+// The FP32 operator is stored in difference form (scaled_kernel), the FP64 one directly.
+template <class V>
+constexpr bool kDiffForm = sizeof(V) == sizeof(float);
+
 // Rows are taken kRPG at a time by each kLPR-lane group, with every load of the
 // kRPG rows (row starts, then column / value pairs, then the gathers) issued
 // before any is used: the SpMV is bound by memory latency, and this multiplies
 // the loads in flight.  body(i, y_i, lane) runs for each row i < n with
-// y_i = sum_k val[k] g(col[k]) valid in every lane of the group.
-template <class V, class Gather, class Body>
+// y_i = sum_k val[k] g(col[k]) valid in every lane of the group, or, for a matrix
+// in difference form (DIFF), y_i = sum_{k: col[k] != i} val[k] (g(col[k]) - g(i))
+// + val[diagonal] g(i).
+template <bool DIFF, class V, class Gather, class Body>
 __device__ __forceinline__ void for_rows(int64_t n, const int32_t* __restrict__ rp,
                                          const int32_t* __restrict__ col, const V* __restrict__ val,
                                          Gather g, Body body) {
@@ -335,15 +354,24 @@ __device__ __forceinline__ void for_rows(int64_t n, const int32_t* __restrict__ 
         v[j] = val[k];
       }
     }
+    V gi[kRPG];
+    if constexpr (DIFF) {
+#pragma unroll
+      for (int j = 0; j < kRPG; ++j) gi[j] = i0 + j < n ? (V)g((int)(i0 + j)) : (V)0;
+    }
+    auto term = [&](int j, int cc) -> V {
+      if constexpr (DIFF) return cc == (int)(i0 + j) ? gi[j] : (V)(g(cc) - gi[j]);
+      else return g(cc);
+    };
     V acc[kRPG];
 #pragma unroll
     for (int j = 0; j < kRPG; ++j) {
       acc[j] = 0;
-      if (c[j] >= 0) acc[j] = v[j] * g(c[j]);
+      if (c[j] >= 0) acc[j] = v[j] * term(j, c[j]);
     }
 #pragma unroll
     for (int j = 0; j < kRPG; ++j)
-      for (int k = rs[j] + lane + kLPR; k < rs[j + 1]; k += kLPR) acc[j] += val[k] * g(col[k]);
+      for (int k = rs[j] + lane + kLPR; k < rs[j + 1]; k += kLPR) acc[j] += val[k] * term(j, col[k]);
 #pragma unroll
     for (int j = 0; j < kRPG; ++j) {
       const V y = group_sum(acc[j]);
@@ -356,40 +384,51 @@ __device__ __forceinline__ void for_rows(int64_t n, const int32_t* __restrict__ 
 // CG on M (precision V for the matrix, vectors and arithmetic)
 // ----------------------------------------------------------------------------
 // start from d = 0 with residual r: z = D^-1 r, p = 0; partials of r.z and |r|_A^2
+// A reliable update passes keep = the current direction instead (p == nullptr):
+// the direction and the iteration count are kept, and the partials of r.keep
+// (slot PT_TT) let the next iteration make the direction orthogonal to the new r.
 template <class V>
 __global__ __launch_bounds__(kRT) void cg_init(int64_t n, const V* __restrict__ r, const V* __restrict__ dinv,
                                                const V* __restrict__ wv, V* __restrict__ d, V* __restrict__ z,
-                                               V* __restrict__ p, double* sc, double* part) {
-  double rz = 0.0, wr = 0.0;
+                                               V* __restrict__ p, const V* __restrict__ keep, double* sc,
+                                               double* part) {
+  double rz = 0.0, wr = 0.0, rk = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     const V ri = r[i];
     const V zi = dinv[i] * ri;
     z[i] = zi;
     d[i] = 0;
-    p[i] = 0;
+    if (p) p[i] = 0;
+    if (keep) rk += (double)ri * (double)keep[i];
     rz += (double)ri * (double)zi;
     const double w = (double)ri * (double)wv[i];
     wr += w * w;
   }
   store_partial(rz, part + PT_RZ * kMaxG);
   store_partial(wr, part + PT_WR * kMaxG);
+  if (keep) store_partial(rk, part + PT_TT * kMaxG);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc[S_DONE] = 0.0;
     sc[S_FLAG] = 0.0;
-    sc[S_ITER] = 0.0;
+    if (p) sc[S_ITER] = 0.0;
   }
 }
 
 // Iteration k, part 1: stop test on |r|_A^2 (the tolerance is fixed at k = 0 as
 // max(irt2 |r0|_A^2, tabs2)), beta = rz_k / rz_{k-1}; p = z + beta p_old with the
 // neighbours' p formed on the fly from z and p_old; q = M p; partials of p.q.
+// First iteration after a reliable update: the replaced r is not exactly
+// orthogonal to p_old, as CG assumes, so p_old first loses its component along z
+// (proj 1): p_old - g z with g = r.p_old / r.z, i.e. p = (1 - beta g) z + beta p_old;
+// or the direction restarts, p = z (proj 2).
 template <class V>
-__global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, double irt2, double tabs2,
+__global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, int proj, double irt2, double tabs2,
                                                    const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ col, const V* __restrict__ val,
                                                    const V* __restrict__ z, const V* __restrict__ pold,
                                                    V* __restrict__ pnew, V* __restrict__ q, double* sc,
                                                    double* part) {
+  if (wg_done(sc)) return;  // launches after the stop leave S_ITER at the stopping iteration
   const double rz = total_of(part + PT_RZ * kMaxG);
   const double wr = total_of(part + PT_WR * kMaxG);
   const double tol2 = k == 0 ? fmax(irt2 * wr, tabs2) : sc[S_TOL2];
@@ -402,17 +441,19 @@ __global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, double irt2
     }
     return;
   }
-  const double beta = k == 0 ? 0.0 : rz / sc[S_RZ + ((k - 1) & 1)];
+  const double beta = k == 0 || proj == 2 ? 0.0 : rz / sc[S_RZ + ((k - 1) & 1)];
   if (lead) {
     sc[S_RZ + (k & 1)] = rz;
     sc[S_WR] = wr;
     if (k == 0) sc[S_TOL2] = tol2;
   }
   const V vb = (V)beta;
+  const V va = proj == 1 ? (V)(1.0 - beta * (total_of(part + PT_TT * kMaxG) / rz)) : (V)1;
   double pq = 0.0;
-  for_rows(n, rp, col, val, [&](int c) { return z[c] + vb * pold[c]; }, [&](int64_t i, V qi, int lane) {
+  auto dir = [&](int c) { return va * z[c] + vb * pold[c]; };
+  for_rows<kDiffForm<V>>(n, rp, col, val, dir, [&](int64_t i, V qi, int lane) {
     if (lane == 0) {
-      const V pi = z[i] + vb * pold[i];
+      const V pi = dir((int)i);
       pnew[i] = pi;
       q[i] = qi;
       pq += (double)pi * (double)qi;
@@ -421,10 +462,12 @@ __global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, double irt2
   store_partial(pq, part + PT_PQ * kMaxG);
 }
 
-// Iteration k, part 2: alpha = rz_k / (p.q); d += alpha p; r -= alpha q;
-// z = D^-1 r; partials of r.z and |r|_A^2.
+// Iteration k, part 2: alpha = rz_k / (p.q); d += alpha p (or, given x64, the
+// FP64 solution itself: x64 += alpha p, exact products of the FP32 factors);
+// r -= alpha q; z = D^-1 r; partials of r.z and |r|_A^2.
 template <class V>
-__global__ __launch_bounds__(kRT) void cg_update(int64_t n, int k, V* __restrict__ d, V* __restrict__ r,
+__global__ __launch_bounds__(kRT) void cg_update(int64_t n, int k, V* __restrict__ d, double* __restrict__ x64,
+                                                 V* __restrict__ r,
                                                  const V* __restrict__ p, const V* __restrict__ q,
                                                  const V* __restrict__ dinv, const V* __restrict__ wv,
                                                  V* __restrict__ z, double* sc, double* part) {
@@ -441,7 +484,8 @@ __global__ __launch_bounds__(kRT) void cg_update(int64_t n, int k, V* __restrict
   const V alpha = (V)(sc[S_RZ + (k & 1)] / pq);
   double rz = 0.0, wr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
-    d[i] = d[i] + alpha * p[i];
+    if (x64) x64[i] = x64[i] + (double)alpha * (double)p[i];
+    else d[i] = d[i] + alpha * p[i];
     const V ri = r[i] - alpha * q[i];
     r[i] = ri;
     const V zi = dinv[i] * ri;
@@ -489,6 +533,7 @@ __global__ __launch_bounds__(kRT) void bi_dir(int64_t n, int k, double tabs2, co
                                               double* __restrict__ p, const double* __restrict__ v,
                                               const double* __restrict__ dinv, double* __restrict__ phat,
                                               double* sc, const double* part) {
+  if (wg_done(sc)) return;
   const double rho = total_of(part + PT_RZ * kMaxG);
   const double wr = total_of(part + PT_WR * kMaxG);
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -532,7 +577,7 @@ __global__ __launch_bounds__(kRT) void bi_spmv(int64_t n, const int32_t* __restr
                                                const double* __restrict__ a, const double* sc, double* part) {
   if (wg_done(sc)) return;
   double ay = 0.0, yy = 0.0;
-  for_rows(n, rp, col, val, [&](int c) { return x[c]; }, [&](int64_t i, double yi, int lane) {
+  for_rows<false>(n, rp, col, val, [&](int c) { return x[c]; }, [&](int64_t i, double yi, int lane) {
     if (lane == 0) {
       y[i] = yi;
       ay += a[i] * yi;
@@ -607,7 +652,7 @@ __global__ __launch_bounds__(kRT) void true_residual(int64_t n, const int32_t* _
                                                      const double* __restrict__ x, const uint8_t* __restrict__ fixed,
                                                      const double* __restrict__ vol, V* __restrict__ r, double* part) {
   double acc = 0.0;
-  for_rows(n, rp, col, val, [&](int c) { return x[c]; }, [&](int64_t i, double ax, int lane) {
+  for_rows<false>(n, rp, col, val, [&](int c) { return x[c]; }, [&](int64_t i, double ax, int lane) {
     if (lane == 0) {
       const double ra = b[i] - ax;
       if (r) r[i] = fixed[i] ? (V)0 : (V)(-vol[i] * ra);
@@ -653,7 +698,7 @@ __global__ __launch_bounds__(kRT) void spmv_kernel(int64_t n, const int32_t* __r
                                                    const int32_t* __restrict__ col, const V* __restrict__ val,
                                                    const double* __restrict__ x, const double* __restrict__ b,
                                                    double* __restrict__ y) {
-  for_rows(n, rp, col, val, [&](int c) { return (V)x[c]; }, [&](int64_t i, V yi, int lane) {
+  for_rows<kDiffForm<V>>(n, rp, col, val, [&](int c) { return (V)x[c]; }, [&](int64_t i, V yi, int lane) {
     if (lane == 0) y[i] = b ? (double)yi + b[i] : (double)yi;
   });
 }
@@ -710,7 +755,8 @@ template <class V>
 int inner_cg(fvm_template* t, const V* valM, const V* dinv, const V* wv, V* d, V* r, V* z, V* p0, V* p1, V* q,
              double irt2, double tabs2, int maxit, int check, hipStream_t s, int* iters, int* flag) {
   const int64_t n = t->n;
-  hipLaunchKernelGGL((cg_init<V>), grid(t), dim3(kRT), 0, s, n, r, dinv, wv, d, z, p0, t->sc, t->part);
+  hipLaunchKernelGGL((cg_init<V>), grid(t), dim3(kRT), 0, s, n, r, dinv, wv, d, z, p0, (const V*)nullptr, t->sc,
+                     t->part);
   double h[S_COUNT];
   int k = 0;
   V* pp[2] = {p0, p1};
@@ -719,10 +765,10 @@ int inner_cg(fvm_template* t, const V* valM, const V* dinv, const V* wv, V* d, V
     for (int j = 0; j < batch; ++j, ++k) {
       V* pold = pp[k & 1];
       V* pnew = pp[(k + 1) & 1];
-      hipLaunchKernelGGL((cg_dir_spmv<V>), grid(t), dim3(kRT), 0, s, n, k, irt2, tabs2, t->row_ptr, t->col,
+      hipLaunchKernelGGL((cg_dir_spmv<V>), grid(t), dim3(kRT), 0, s, n, k, 0, irt2, tabs2, t->row_ptr, t->col,
                          valM, z, pold, pnew, q, t->sc, t->part);
-      hipLaunchKernelGGL((cg_update<V>), grid(t), dim3(kRT), 0, s, n, k, d, r, pnew, q, dinv, wv, z, t->sc,
-                         t->part);
+      hipLaunchKernelGGL((cg_update<V>), grid(t), dim3(kRT), 0, s, n, k, d, (double*)nullptr, r, pnew, q, dinv, wv,
+                         z, t->sc, t->part);
     }
     int rc = read_scalars(t, h, s);
     if (rc) return rc;
@@ -730,6 +776,60 @@ int inner_cg(fvm_template* t, const V* valM, const V* dinv, const V* wv, V* d, V
   }
   *iters = (int)h[S_ITER];
   *flag = (int)h[S_FLAG];
+  return FVM_OK;
+}
+
+// FP32 CG with reliable updates (Sleijpen & van der Vorst 1996): one Krylov
+// sequence for the whole solve.  Each step goes straight into the FP64 x (an FP32
+// correction vector would carry a rounding error of about eps32 kappa(M) of the
+// residual for smooth solutions, which at 10^6 unknowns is the size of a whole
+// segment's reduction); whenever the recursive residual estimate has dropped by
+// delta since the last replacement (tested every `check` iterations) or reaches
+// the target, r is replaced by -V (b - A x) computed in FP64, keeping the
+// direction p and rz of the last iteration, so beta continues the sequence.  If
+// the true residual is more than twice the recursive one (the recursion ran far
+// below what FP32 can represent of the correction, e.g. when CG converges in a few
+// iterations between two tests), beta = rz_new / rz_old would swamp z and the
+// direction restarts instead.  On entry r holds -V (b - A x) and *res = |b - A x|_2.
+int reliable_cg(fvm_template* t, double* x, double target, double delta, int maxit, int max_updates, int check,
+                hipStream_t s, int* iters, int* updates, int* flag, double* res) {
+  const int64_t n = t->n;
+  float* w = (float*)t->work;
+  float *d = w, *r = w + n, *z = w + 2 * n, *q = w + 5 * n;
+  float* pp[2] = {w + 3 * n, w + 4 * n};
+  const double tabs2 = 0.25 * target * target;
+  hipLaunchKernelGGL((cg_init<float>), grid(t), dim3(kRT), 0, s, n, r, t->dinv32, t->wv32, d, z, pp[0],
+                     (const float*)nullptr, t->sc, t->part);
+  double h[S_COUNT];
+  double wref = (*res) * (*res);  // |b - A x|^2 at the last replacement
+  int k = 0, upd = 0, proj = 0;
+  *flag = 0;
+  for (;;) {
+    const int batch = std::min(check, maxit - k);
+    for (int j = 0; j < batch; ++j, ++k) {
+      hipLaunchKernelGGL((cg_dir_spmv<float>), grid(t), dim3(kRT), 0, s, n, k, j == 0 ? proj : 0, 0.0, tabs2,
+                         t->row_ptr, t->col, t->valM32, z, pp[k & 1], pp[(k + 1) & 1], q, t->sc, t->part);
+      hipLaunchKernelGGL((cg_update<float>), grid(t), dim3(kRT), 0, s, n, k, d, x, r, pp[(k + 1) & 1], q,
+                         t->dinv32, t->wv32, z, t->sc, t->part);
+    }
+    int rc = read_scalars(t, h, s);
+    if (rc) return rc;
+    const bool stop = h[S_DONE] != 0.0;
+    if (stop) k = (int)h[S_ITER];  // the iteration that stopped; p of the one before is in pp[k & 1]
+    if (h[S_FLAG] != 0.0) *flag = (int)h[S_FLAG];
+    proj = 0;
+    if (!(stop || h[S_WR] < delta * delta * wref || k >= maxit)) continue;
+    if ((rc = true_res<float>(t, x, r, s)) || (rc = read_scalars(t, h, s))) return rc;
+    ++upd;
+    *res = std::sqrt(h[S_RES2]);
+    if (*res <= target || k >= maxit || upd >= max_updates || *flag || !(*res == *res)) break;
+    hipLaunchKernelGGL((cg_init<float>), grid(t), dim3(kRT), 0, s, n, r, t->dinv32, t->wv32, d, z,
+                       (float*)nullptr, (const float*)pp[k & 1], t->sc, t->part);
+    wref = (*res) * (*res);
+    proj = k == 0 ? 0 : *res > 2.0 * std::sqrt(h[S_WR]) ? 2 : 1;
+  }
+  *iters = k;
+  *updates = upd;
   return FVM_OK;
 }
 
@@ -959,7 +1059,7 @@ int fvm_template_residual(fvm_template* t, const double* d_x, double* rel, fvm_s
 int fvm_template_solve(fvm_template* t, const fvm_solve_opts* o, double* d_x, fvm_solve_stats* st,
                        fvm_stream s) {
   FVM_CHECK(t && o && d_x && st, "fvm_template_solve: null argument");
-  FVM_CHECK(o->method >= 0 && o->method <= 2, "fvm_template_solve: unknown method");
+  FVM_CHECK(o->method >= 0 && o->method <= 3, "fvm_template_solve: unknown method");
   FVM_CHECK(o->rtol > 0 && o->maxiter > 0, "fvm_template_solve: rtol and maxiter must be positive");
   FVM_CHECK(t->kind != 2, "fvm_template_solve: a diffusion template is an ODE, not a linear system");
   hipStream_t str = (hipStream_t)s;
@@ -973,7 +1073,8 @@ int fvm_template_solve(fvm_template* t, const fvm_solve_opts* o, double* d_x, fv
   float *df = (float*)w, *rf = df + n, *zf = df + 2 * n, *p0f = df + 3 * n, *p1f = df + 4 * n, *qf = df + 5 * n;
   hipLaunchKernelGGL(init_x, grid(t), dim3(kRT), 0, str, n, t->fixed, t->b, d_x, o->use_x0 ? 1 : 0);
   double h[S_COUNT];
-  int rc = o->method == 1 ? true_res<float>(t, d_x, rf, str) : true_res<double>(t, d_x, r, str);
+  const bool fp32 = o->method == FVM_SOLVE_CG_MIXED || o->method == FVM_SOLVE_CG_RELIABLE;
+  int rc = fp32 ? true_res<float>(t, d_x, rf, str) : true_res<double>(t, d_x, r, str);
   if (rc || (rc = read_scalars(t, h, str))) return rc;
   const double bn = std::sqrt(h[S_B2]) > 0 ? std::sqrt(h[S_B2]) : 1.0;
   const double target = o->rtol * bn;
@@ -981,7 +1082,13 @@ int fvm_template_solve(fvm_template* t, const fvm_solve_opts* o, double* d_x, fv
   // the recursive residual must reach half the target before the true residual is checked
   const double tabs2 = 0.25 * target * target;
   int total = 0, outer = 0, flag = 0;
-  while (res > target && total < o->maxiter && outer < max_outer) {
+  if (o->method == FVM_SOLVE_CG_RELIABLE && res > target) {
+    const double delta = o->inner_rtol > 0 ? o->inner_rtol : 0.1;
+    rc = reliable_cg(t, d_x, target, delta, o->maxiter, o->max_outer > 0 ? o->max_outer : 1000, check, str,
+                     &total, &outer, &flag, &res);
+    if (rc) return rc;
+  }
+  while (o->method != FVM_SOLVE_CG_RELIABLE && res > target && total < o->maxiter && outer < max_outer) {
     int it = 0;
     if (o->method == 1) {
       rc = inner_cg<float>(t, t->valM32, t->dinv32, t->wv32, df, rf, zf, p0f, p1f, qf, irt * irt, tabs2,

@@ -105,7 +105,7 @@ class SystemDesc(ctypes.Structure):
 
 FLUX_D_POLY_U, FLUX_D_NODAL, FLUX_D_EDGE = 0, 1, 2
 TEMPLATE_POISSON, TEMPLATE_MEAN_EXIT_TIME, TEMPLATE_DIFFUSION = 0, 1, 2
-SOLVE_CG, SOLVE_CG_MIXED, SOLVE_BICGSTAB = 0, 1, 2
+SOLVE_CG, SOLVE_CG_MIXED, SOLVE_BICGSTAB, SOLVE_CG_RELIABLE = 0, 1, 2, 3
 NODE_NONE, NODE_DIRICHLET, NODE_DUDT = 0, 1, 2
 
 _lock = threading.Lock()

@@ -32,10 +32,21 @@ class CG:
 
 @dataclass
 class MixedCG:
-    """FP64 iterative refinement around FP32 Jacobi-PCG: each FP32 solve reduces the
-    residual by inner_rtol; the residual b - A x and the solution stay FP64."""
-    inner_rtol: float = 1e-4
+    """FP32 Jacobi-PCG inside FP64 iterative refinement: the residual b - A x, the
+    solution and the convergence test stay FP64.
+
+    update="reliable" (default): one CG sequence; each time its residual has dropped
+    by inner_rtol (0: 0.1) the FP32 correction is added to x and the FP32 residual
+    replaced by the FP64 one, keeping the search direction.  update="restart": each
+    refinement step is a fresh FP32 CG solve that reduces the residual by
+    inner_rtol (0: 1e-4)."""
+    update: str = "reliable"
+    inner_rtol: float = 0.0
     check_every: int = 32
+
+    def __post_init__(self):
+        if self.update not in ("reliable", "restart"):
+            raise ValueError(f"MixedCG update must be 'reliable' or 'restart', not {self.update!r}")
 
 
 @dataclass
@@ -181,6 +192,8 @@ class DeviceTemplate:
             raise TypeError(f"unsupported linear solver {alg!r}; use CG(), MixedCG() or BiCGStab()")
         o = _lib.SolveOpts()
         o.method = _METHOD[type(alg)]
+        if isinstance(alg, MixedCG) and alg.update == "reliable":
+            o.method = _lib.SOLVE_CG_RELIABLE
         o.maxiter = int(maxiter)
         o.check_every = int(alg.check_every)
         o.max_outer = int(max_outer)

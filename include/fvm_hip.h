@@ -231,18 +231,20 @@ int fvm_template_residual(fvm_template* t, const double* d_x, double* rel, fvm_s
 enum {
   FVM_SOLVE_CG = 0,       /* FP64 Jacobi-PCG on M */
   FVM_SOLVE_CG_MIXED = 1, /* FP64 iterative refinement around FP32 Jacobi-PCG on M */
-  FVM_SOLVE_BICGSTAB = 2  /* FP64 Jacobi-BiCGStab on M */
+  FVM_SOLVE_BICGSTAB = 2, /* FP64 Jacobi-BiCGStab on M */
+  FVM_SOLVE_CG_RELIABLE = 3 /* one FP32 Jacobi-PCG sequence with FP64 reliable updates of x and the residual */
 };
 
 typedef struct fvm_solve_opts {
   int32_t method;       /* FVM_SOLVE_* */
   int32_t maxiter;      /* total inner iterations */
   int32_t check_every;  /* iterations between host reads of the device scalars (0: 32) */
-  int32_t max_outer;    /* refinement / restart steps (0: 50) */
+  int32_t max_outer;    /* refinement / restart steps (0: 50; FVM_SOLVE_CG_RELIABLE: updates, 0: 1000) */
   int32_t use_x0;       /* start from d_x (its fixed rows are reset to b) instead of 0 */
   int32_t reserved;
   double rtol;          /* stop when |b - A x|_2 <= rtol |b|_2 */
-  double inner_rtol;    /* FVM_SOLVE_CG_MIXED: relative drop of each FP32 solve (0: 1e-4) */
+  double inner_rtol;    /* FVM_SOLVE_CG_MIXED: relative drop of each FP32 solve (0: 1e-4);
+                           FVM_SOLVE_CG_RELIABLE: residual drop between updates (0: 0.1) */
 } fvm_solve_opts;
 
 typedef struct fvm_solve_stats {

@@ -139,10 +139,11 @@ def test_points_that_are_not_vertices_get_identity_rows(gpu):
     assert sol.retcode == "Success" and sol.u[-1] == 0.0 and sol.u[-2] == 0.0
 
 
-ALGS = [TM.CG(), TM.MixedCG(), TM.BiCGStab()]
+ALGS = [TM.CG(), TM.MixedCG(), TM.MixedCG(update="restart"), TM.BiCGStab()]
+IDS = ["cg", "mixed", "mixed_restart", "bicgstab"]
 
 
-@pytest.mark.parametrize("alg", ALGS, ids=["cg", "mixed", "bicgstab"])
+@pytest.mark.parametrize("alg", ALGS, ids=IDS)
 def test_poisson_closed_form(gpu, alg):
     """poissons_equation.jl:154-162: laplace(u) = -sin(pi x) sin(pi y) on [0,1]^2
     (100 x 100), u = 0 on the boundary: u = sin(pi x) sin(pi y) / (2 pi^2), rtol 1e-4."""
@@ -158,7 +159,7 @@ def test_poisson_closed_form(gpu, alg):
     assert np.linalg.norm(sol.u - direct) <= 1e-6 * np.linalg.norm(direct)
 
 
-@pytest.mark.parametrize("alg", ALGS, ids=["cg", "mixed", "bicgstab"])
+@pytest.mark.parametrize("alg", ALGS, ids=IDS)
 def test_laplace_closed_form(gpu, alg):
     """laplaces_equation.jl:159-193: D = (x+1)(y+2) on [0,5]^2, Neumann bottom / top,
     u = 5 right, u = 0 left: u = 5 log(1 + x) / log 6, rtol 1e-3."""
@@ -187,7 +188,7 @@ def test_mean_exit_time_disk_closed_form(gpu):
     assert np.linalg.norm(sol.u - exact) <= 5e-3 * np.linalg.norm(exact)
 
 
-@pytest.mark.parametrize("alg", ALGS, ids=["cg", "mixed", "bicgstab"])
+@pytest.mark.parametrize("alg", ALGS, ids=IDS)
 def test_solvers_reach_the_true_residual(gpu, alg):
     """The variable-D Poisson system at 200 x 200 (jittered): every method stops at
     |b - A x| <= 1e-10 |b|, checked on the host with the oracle's own matrix."""
@@ -217,6 +218,68 @@ def test_solve_is_bitwise_reproducible(gpu):
         a = FVM.solve(prob, alg).u
         b = FVM.solve(prob, alg).u
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("alg", [TM.CG, lambda **k: TM.MixedCG(update="restart", **k), TM.BiCGStab],
+                         ids=["cg", "mixed_restart", "bicgstab"])
+def test_iteration_count_does_not_depend_on_the_check_interval(gpu, alg):
+    """The host reads the solver scalars every check_every iterations; launches past
+    the stopping iteration are no-ops, so reading after every iteration and every 32
+    gives the same iteration count and the same solution bit for bit."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 40, 40, jitter=0.25, seed=4)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    prob = FVM.PoissonsEquation(mesh, BCs, diffusion_function=Dxy, source_function=lambda x, y, p: x - y)
+    one = FVM.solve(prob, alg(check_every=1))
+    many = FVM.solve(prob, alg(check_every=32))
+    assert one.retcode == "Success" and one.iters == many.iters, (one.stats, many.stats)
+    assert np.array_equal(one.u, many.u)
+
+
+def test_fp32_operator_keeps_accuracy_on_smooth_vectors(gpu):
+    """The FP32 solver operator (difference form) against the FP64 one, on vectors
+    that FP32 holds exactly (inside the solver every vector is FP32): on a smooth
+    vector, whose product cancels to about 1e-4 of |M| |p|, the FP32 product stays
+    within 1e-4 of |M p| (the direct FP32 form loses about 1e-3 to cancellation);
+    on a rough vector within 1e-6."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 300, 300, jitter=0.25, seed=6)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    dt = FVM.MeanExitTimeProblem(mesh, BCs, diffusion_function=Dxy).problem
+    rp, col, val, _ = dt.download()
+    n = dt.n
+    free = dt.node_kind == 0
+    absA = sp.csr_matrix((np.abs(val), col, rp), shape=(n, n))
+    for x, cancel, tol in ((np.sin(np.pi * tri.x) * np.sin(np.pi * tri.y) * free, True, 1e-4),
+                           (np.random.default_rng(1).uniform(-1, 1, n) * free, False, 1e-6)):
+        x = x.astype(np.float32).astype(np.float64)
+        d_x = FVM.DeviceArray.from_host(x)
+        y64, y32 = FVM.DeviceArray(n), FVM.DeviceArray(n)
+        dt.spmv(1, d_x, y64)
+        dt.spmv(2, d_x, y32)
+        a, b = y64.to_host(), y32.to_host()
+        A = sp.csr_matrix((val, col, rp), shape=(n, n))
+        assert (np.linalg.norm(A @ x) < 1e-3 * np.linalg.norm(absA @ np.abs(x))) == cancel
+        assert np.linalg.norm(b - a) <= tol * np.linalg.norm(a)
+
+
+def test_reliable_updates_keep_the_iteration_count_of_fp64_cg(gpu):
+    """One FP32 CG sequence with reliable updates needs about as many iterations as
+    FP64 CG (within 15 %) and fewer than restarted refinement, and reaches the same
+    true residual; its solution agrees with FP64 CG to the tolerance."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 150, 150, jitter=0.25, seed=9)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    for prob in (FVM.PoissonsEquation(mesh, BCs, diffusion_function=Dxy, source_function=lambda x, y, p: x - y),
+                 FVM.MeanExitTimeProblem(mesh, BCs, diffusion_function=Dxy)):
+        cg = FVM.solve(prob, TM.CG(), rtol=1e-10)
+        rel = FVM.solve(prob, TM.MixedCG(), rtol=1e-10)
+        rst = FVM.solve(prob, TM.MixedCG(update="restart"), rtol=1e-10)
+        assert cg.retcode == rel.retcode == rst.retcode == "Success"
+        assert rel.resid <= 1e-10 and rel.stats["outer"] >= 2
+        assert rel.iters <= 1.15 * cg.iters, (rel.stats, cg.stats)
+        assert rel.iters < rst.iters, (rel.stats, rst.stats)
+        assert np.linalg.norm(rel.u - cg.u) <= 1e-8 * np.linalg.norm(cg.u)
 
 
 def test_diffusion_template_matches_fvm_problem(gpu):
