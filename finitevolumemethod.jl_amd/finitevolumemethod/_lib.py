@@ -75,6 +75,25 @@ class SolveOpts(ctypes.Structure):
                 ("rtol", ctypes.c_double), ("inner_rtol", ctypes.c_double)]
 
 
+class OdeOpts(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int32), ("op_kind", ctypes.c_int32), ("op", _vp),
+                ("dirichlet", ctypes.c_int32), ("reserved", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("t0", ctypes.c_double), ("tf", ctypes.c_double), ("dt", ctypes.c_double),
+                ("abstol", ctypes.c_double), ("reltol", ctypes.c_double), ("maxiters", ctypes.c_int64),
+                ("n_save", ctypes.c_int64), ("save_t", _dp), ("d_saves", _vp)]
+
+
+class OdeStats(ctypes.Structure):
+    _fields_ = [("nf", ctypes.c_int64), ("naccept", ctypes.c_int64), ("nreject", ctypes.c_int64),
+                ("nsaved", ctypes.c_int64), ("retcode", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("t", ctypes.c_double)]
+
+
+ODE_TSIT5, ODE_EULER, ODE_SSPRK3 = 0, 1, 2
+OP_PROBLEM, OP_SYSTEM, OP_TEMPLATE = 0, 1, 2
+ODE_RETCODES = {0: "Success", 1: "MaxIters", 2: "Unstable"}
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [("converged", ctypes.c_int32), ("iterations", ctypes.c_int32),
                 ("outer", ctypes.c_int32), ("flag", ctypes.c_int32),
@@ -187,6 +206,7 @@ HIP_API = {
     "fvm_template_spmv": [_vp, ctypes.c_int32, _vp, _vp, _vp],
     "fvm_template_residual": [_vp, _vp, _dp, _vp],
     "fvm_template_solve": [_vp, ctypes.POINTER(SolveOpts), _vp, ctypes.POINTER(SolveStats), _vp],
+    "fvm_ode_solve": [ctypes.POINTER(OdeOpts), _vp, ctypes.POINTER(OdeStats), _vp],
     "fvm_batch_create": [ctypes.c_int32, ctypes.POINTER(MeshDesc), ctypes.POINTER(PhysicsDesc), _dp,
                          ctypes.POINTER(BatchOpts), ctypes.POINTER(_vp)],
     "fvm_batch_destroy": [_vp],

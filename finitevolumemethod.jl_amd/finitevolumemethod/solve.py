@@ -8,6 +8,7 @@ combination kernel plus one fvm_rhs launch) and applies the Dirichlet callback
 reference's DiscreteCallback(true, update_dirichlet_nodes!).  Saved states at
 `saveat` times come from cubic Hermite interpolation of the accepted steps.
 """
+import ctypes
 import math
 from dataclasses import dataclass, field
 
@@ -106,10 +107,13 @@ class _Saver:
 
 
 def _hermite(out, u0, f0, u1, f1, dt, theta):
-    h00 = 2 * theta ** 3 - 3 * theta ** 2 + 1
-    h10 = theta ** 3 - 2 * theta ** 2 + theta
-    h01 = -2 * theta ** 3 + 3 * theta ** 2
-    h11 = theta ** 3 - theta ** 2
+    # explicit products (the library's integrator forms the same weights)
+    t2 = theta * theta
+    t3 = t2 * theta
+    h00 = 2 * t3 - 3 * t2 + 1
+    h10 = t3 - 2 * t2 + theta
+    h01 = -2 * t3 + 3 * t2
+    h11 = t3 - t2
     _lib.lincomb(out, (h00, h10 * dt, h01, h11 * dt), (u0, f0, u1, f1))
 
 
@@ -147,7 +151,7 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
                         prob.final_time, alg if alg is not None else Tsit5(), saveat=saveat,
                         abstol=abstol, reltol=reltol, dt=dt, maxiters=maxiters,
                         save_start=save_start, step_callback=cb, user_callback=callback,
-                        jvp=ctx.jvp)
+                        jvp=ctx.jvp, op=(_lib.OP_SYSTEM, ctx.handle, cb is not None))
         sol.u = [prob.unflat(v) for v in sol.u]
         return sol
     if not isinstance(prob, FVMProblem):
@@ -160,13 +164,25 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
                      alg if alg is not None else Tsit5(), saveat=saveat, abstol=abstol,
                      reltol=reltol, dt=dt, maxiters=maxiters, save_start=save_start,
                      step_callback=cb, user_callback=callback,
-                     jvp=lambda u, v, out, t: ctx.jvp(u, v, out, t))
+                     jvp=lambda u, v, out, t: ctx.jvp(u, v, out, t),
+                     op=(_lib.OP_PROBLEM, ctx.handle, has_dir))
 
 
 def integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None,
-              maxiters=10**7, save_start=True, step_callback=None, user_callback=None, jvp=None):
+              maxiters=10**7, save_start=True, step_callback=None, user_callback=None, jvp=None,
+              op=None):
     """Integrate du/dt = rhs(u, t) on the device from t0 to tf.  Implicit methods
-    need jvp(u, v, out, t) = J(u) v."""
+    need jvp(u, v, out, t) = J(u) v.
+
+    op = (kind, handle, dirichlet): the library operator that rhs evaluates (and,
+    with dirichlet, whose Dirichlet update step_callback applies).  Explicit
+    methods with saveat times and no user callback then run whole in the library
+    (fvm_ode_solve: the same arithmetic, one call instead of one per stage)."""
+    if (op is not None and user_callback is None and isinstance(alg, (Tsit5, Euler, SSPRK3))
+            and saveat is not None):
+        times = _Saver(saveat, t0, tf).times
+        if times is not None and 8 * len(times) * len(u0) <= _DEVICE_SAVE_BYTES:
+            return _integrate_device(op, u0, t0, tf, alg, times, abstol, reltol, dt, maxiters)
     if isinstance(alg, TRBDF2):
         if jvp is None:
             raise TypeError("TRBDF2 needs the Jacobian-vector product of the right-hand side")
@@ -439,6 +455,43 @@ def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, 
     if sol.retcode == "Default":
         sol.retcode = "Success" if t >= tf else "MaxIters"
     sol.stats = stats
+    sol.final_device = u
+    return sol
+
+
+_DEVICE_SAVE_BYTES = 8 << 30  # saved states held on the device by fvm_ode_solve
+
+
+def _integrate_device(op, u0, t0, tf, alg, times, abstol, reltol, dt, maxiters):
+    """The explicit loops of integrate() run by fvm_ode_solve; the saved states are
+    written to one device buffer and copied back once."""
+    kind, handle, dirichlet = op
+    n = len(u0)
+    o = _lib.OdeOpts()
+    if isinstance(alg, Tsit5):
+        o.method = _lib.ODE_TSIT5
+        o.dt = float(dt) if dt is not None else 0.0
+    else:
+        h = dt if dt is not None else alg.dt
+        if h is None:
+            raise ValueError("fixed-step integrators need dt")
+        o.method = _lib.ODE_EULER if isinstance(alg, Euler) else _lib.ODE_SSPRK3
+        o.dt = float(h)
+    u = _lib.DeviceArray.from_host(np.asarray(u0, np.float64))
+    ts = np.ascontiguousarray(times, np.float64)
+    saves = _lib.DeviceArray(max(len(ts), 1) * n)
+    o.op_kind, o.op, o.dirichlet = int(kind), handle, int(bool(dirichlet))
+    o.n, o.t0, o.tf = n, float(t0), float(tf)
+    o.abstol, o.reltol, o.maxiters = float(abstol), float(reltol), int(maxiters)
+    o.n_save, o.save_t, o.d_saves = len(ts), _lib.ptr(ts), saves.ptr
+    st = _lib.OdeStats()
+    _lib.check(_lib.hip_lib().fvm_ode_solve(ctypes.byref(o), u.ptr, ctypes.byref(st), None), "fvm_ode_solve")
+    host = saves.to_host()
+    sol = ODESolution()
+    sol.t = [float(v) for v in ts[:st.nsaved]]
+    sol.u = [host[k * n:(k + 1) * n].copy() for k in range(st.nsaved)]
+    sol.retcode = _lib.ODE_RETCODES[st.retcode]
+    sol.stats = dict(nf=st.nf, naccept=st.naccept, nreject=st.nreject)
     sol.final_device = u
     return sol
 

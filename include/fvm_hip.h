@@ -384,6 +384,57 @@ int fvm_vec_maxpy(int64_t n, double* d_y, int32_t k, const double* coefs, const 
 int fvm_vec_err_norm(int64_t n, const double* d_err, const double* d_u0, const double* d_u1,
                      double abstol, double reltol, double* result, fvm_stream s);
 
+/* ---- explicit time integration (solve(::FVMProblem), solve.jl:133-220) -----
+ * The whole adaptive or fixed-step integration of du/dt = f(u, t) in one call, f one
+ * of the library's operators, with the Dirichlet update (dirichlet.jl:78-86) after
+ * every accepted step, as the reference's DiscreteCallback(true, ...) of
+ * solve.jl:133-165, and the states at the saveat times written to device memory.
+ * Replaces handing fvm_eqs! to OrdinaryDiffEq (solve.jl:167-220) for Tsit5 and the
+ * fixed-step explicit methods: a step costs its kernel launches and, for Tsit5, one
+ * host synchronisation. */
+typedef enum {
+  FVM_ODE_TSIT5 = 0,  /* Tsitouras 5(4), PI controller (beta1 7/50, beta2 2/25, safety 0.9,
+                         step ratio in [1/10, 5]), Hermite dense output for saveat */
+  FVM_ODE_EULER = 1,  /* forward Euler, fixed dt */
+  FVM_ODE_SSPRK3 = 2  /* Shu-Osher SSPRK3, fixed dt */
+} fvm_ode_method;
+
+typedef enum {
+  FVM_OP_PROBLEM = 0,  /* op: fvm_ctx*     f = fvm_rhs, update = fvm_dirichlet_apply */
+  FVM_OP_SYSTEM = 1,   /* op: fvm_system*  f = fvm_system_rhs, update = fvm_system_dirichlet_apply */
+  FVM_OP_TEMPLATE = 2  /* op: fvm_template* f = fvm_template_apply (A u + b), no update */
+} fvm_op_kind;
+
+typedef enum { FVM_ODE_SUCCESS = 0, FVM_ODE_MAXITERS = 1, FVM_ODE_UNSTABLE = 2 } fvm_ode_retcode;
+
+typedef struct fvm_ode_opts {
+  int32_t method;         /* FVM_ODE_* */
+  int32_t op_kind;        /* FVM_OP_*: what op points to */
+  void* op;
+  int32_t dirichlet;      /* apply the operator's Dirichlet update after every accepted step */
+  int32_t reserved;
+  int64_t n;              /* state length */
+  double t0, tf;
+  double dt;              /* the fixed step; Tsit5: the first step (<= 0: automatic) */
+  double abstol, reltol;  /* Tsit5 error control */
+  int64_t maxiters;       /* Tsit5: accepted + rejected steps */
+  int64_t n_save;         /* number of saveat times (0: none) */
+  const double* save_t;   /* host [n_save], ascending */
+  double* d_saves;        /* device [n_save * n]: the state at save_t[k] at d_saves + k n */
+} fvm_ode_opts;
+
+typedef struct fvm_ode_stats {
+  int64_t nf;             /* evaluations of f */
+  int64_t naccept, nreject;
+  int64_t nsaved;         /* save slots written */
+  int32_t retcode;        /* fvm_ode_retcode */
+  int32_t reserved;
+  double t;               /* time reached */
+} fvm_ode_stats;
+
+/* d_u [n]: the initial state in, the state at st->t out.  Blocks until done. */
+int fvm_ode_solve(const fvm_ode_opts* o, double* d_u, fvm_ode_stats* st, fvm_stream s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,147 @@
+"""fvm_ode_solve: the explicit integrators run whole in the library (solve.jl:133-220
+semantics: Dirichlet update after every accepted step, saveat) against the
+host-language loop over the same kernels.  The two perform the same arithmetic in
+the same order, so every saved state, time and counter must agree bit for bit;
+the host loop itself is pinned against the oracle's RHS in test_gpu_solve.py."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import finitevolumemethod as FVM
+from finitevolumemethod import _lib
+from finitevolumemethod.solve import integrate
+from problems import annulus_problem, brusselator_system, mixed_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def plate(n=50):
+    tri = FVM.triangulate_rectangle(0.0, 2.0, 0.0, 2.0, n, n, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    u0 = np.where(tri.y <= 1.0, 50.0, 0.0)
+    return FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 1 / 9,
+                          initial_condition=u0, final_time=0.5)
+
+
+def host_loop(prob, alg, **kw):
+    """The same integration through the host-language loop (no library operator)."""
+    ctx = prob.device_context()
+    cb = (lambda u, t: ctx.apply_dirichlet(u, t)) if prob.conditions.has_dirichlet_nodes() else None
+    return integrate(ctx.rhs, prob.initial_condition, prob.initial_time, prob.final_time, alg,
+                     step_callback=cb, **kw)
+
+
+def assert_same(a, b):
+    assert a.retcode == b.retcode == "Success"
+    assert a.stats == b.stats
+    assert a.t == b.t and len(a.u) == len(b.u)
+    for k, (x, y) in enumerate(zip(a.u, b.u)):
+        assert np.all(np.isfinite(x)), k
+        assert np.array_equal(x, y), (k, a.t[k], np.max(np.abs(x - y)))
+
+
+@pytest.mark.parametrize("saveat", [0.05, 0.001, [0.0, 0.013, 0.2, 0.5]])
+def test_tsit5_plate_matches_host_loop(gpu, saveat):
+    prob = plate()
+    a = FVM.solve(prob, FVM.Tsit5(), saveat=saveat)
+    b = host_loop(prob, FVM.Tsit5(), saveat=saveat)
+    assert_same(a, b)
+    assert a.stats["naccept"] > 50
+
+
+@pytest.mark.parametrize("alg", [FVM.SSPRK3(dt=2e-4), FVM.Euler(dt=1e-4)], ids=["ssprk3", "euler"])
+@pytest.mark.parametrize("variant", ["tutorial", "exp"])
+def test_fixed_step_matches_host_loop(gpu, alg, variant):
+    """The annulus with a time-dependent Dirichlet boundary (updated after every
+    step) and with Neumann / Dudt boundaries, to t = 0.05 (250 or 500 steps)."""
+    prob, _, _ = annulus_problem(variant, final_time=0.05)
+    a = FVM.solve(prob, alg, saveat=0.01)
+    b = host_loop(prob, alg, saveat=0.01)
+    assert_same(a, b)
+    assert len(a.u) == 6
+
+
+def test_internal_conditions_match_host_loop(gpu):
+    """mixed_problem (internal Dirichlet and Dudt nodes, t0 = 2) over a few steps:
+    its Dirichlet functions move u by about -0.5 per update, so only a short run
+    stays well posed."""
+    prob, _, _ = mixed_problem()
+    prob.final_time = 2.001
+    a = FVM.solve(prob, FVM.SSPRK3(dt=1e-4), saveat=0.0005)
+    b = host_loop(prob, FVM.SSPRK3(dt=1e-4), saveat=0.0005)
+    assert_same(a, b)
+    assert len(a.u) == 3
+
+
+def test_tsit5_time_dependent_dirichlet_matches_host_loop(gpu):
+    prob, _, _ = annulus_problem("tutorial", final_time=0.3)
+    a = FVM.solve(prob, FVM.Tsit5(), saveat=0.1)
+    b = host_loop(prob, FVM.Tsit5(), saveat=0.1)
+    assert_same(a, b)
+    tri = prob.mesh.triangulation
+    inner = np.unique(tri.boundary_edges[tri.boundary_segments == 1])
+    np.testing.assert_allclose(a.final_device.to_host()[inner], 50 * (1 - np.exp(-0.3 / 2)), rtol=1e-13)
+
+
+def test_system_matches_host_loop(gpu):
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 20, 18, jitter=0.2, seed=2, single_boundary=False)
+    sys = brusselator_system(tri, final_time=0.5)
+    a = FVM.solve(sys, FVM.Tsit5(), saveat=0.1)
+    ctx = sys.device_context()
+    cb = (lambda u, t: ctx.apply_dirichlet(u, t)) if sys.has_dirichlet_nodes() else None
+    b = integrate(ctx.rhs, sys.flat(sys.initial_condition), sys.initial_time, sys.final_time, FVM.Tsit5(),
+                  saveat=0.1, step_callback=cb)
+    b.u = [sys.unflat(v) for v in b.u]
+    assert_same(a, b)
+    assert a.u[0].shape == (2, tri.num_points())
+
+
+def test_diffusion_template_matches_host_loop(gpu):
+    tri = FVM.triangulate_rectangle(0.0, 2.0, 0.0, 2.0, 30, 30, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    u0 = np.where(tri.y <= 1.0, 50.0, 0.0)
+    prob = FVM.DiffusionEquation(mesh, BCs, diffusion_function=lambda x, y, p: 1 / 9,
+                                 initial_condition=u0, final_time=0.5)
+    a = FVM.solve(prob, FVM.Tsit5(), saveat=0.05)
+    p = prob.problem
+    b = integrate(lambda u, du, t: p.apply(u, du), prob._u0[:p.n], 0.0, 0.5, FVM.Tsit5(), saveat=0.05)
+    b.u = [np.concatenate([s, [1.0]]) for s in b.u]
+    assert_same(a, b)
+
+
+def test_paths_without_save_times_or_with_callbacks_stay_on_the_host_loop(gpu):
+    """saveat=None saves every step and a user callback runs between steps: both
+    keep the host loop and give the same states as the library at the common times."""
+    prob = plate(20)
+    every = FVM.solve(prob, FVM.Tsit5())
+    assert len(every.u) == every.stats["naccept"] + 1
+    seen = []
+    cbk = FVM.solve(prob, FVM.Tsit5(), saveat=0.1, callback=lambda u, t: seen.append(t))
+    lib = FVM.solve(prob, FVM.Tsit5(), saveat=0.1)
+    assert len(seen) == lib.stats["naccept"]
+    assert_same(cbk, lib)
+
+
+def test_ode_solve_rejects_bad_options(gpu):
+    prob = plate(10)
+    ctx = prob.device_context()
+    u = _lib.DeviceArray.from_host(prob.initial_condition)
+    st = _lib.OdeStats()
+
+    def call(**kw):
+        o = _lib.OdeOpts()
+        o.method, o.op_kind, o.op, o.n = _lib.ODE_TSIT5, _lib.OP_PROBLEM, ctx.handle, len(prob.initial_condition)
+        o.t0, o.tf, o.abstol, o.reltol, o.maxiters = 0.0, 0.1, 1e-6, 1e-3, 1000
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return _lib.check(_lib.hip_lib().fvm_ode_solve(ctypes.byref(o), u.ptr, ctypes.byref(st), None),
+                          "fvm_ode_solve")
+
+    call()
+    for bad in (dict(op=None), dict(op_kind=7), dict(method=9), dict(tf=-1.0), dict(abstol=0.0),
+                dict(n_save=3), dict(method=_lib.ODE_EULER, dt=0.0)):
+        with pytest.raises(FVM.FVMError):
+            call(**bad)
