@@ -106,6 +106,7 @@ struct Packed {
 
 int pack(int32_t P, const fvm_mesh_desc* meshes, const fvm_physics_desc* ph, fvm_batch* b, Packed& k) {
   const int fk = ph[0].flux_kind;
+  FVM_CHECK(fk != FVM_FLUX_LINEAR, "fvm_batch_create: linear flux forms (FVM_FLUX_LINEAR) are not batched");
   int64_t N = 0, T = 0, B = 0, F = 0;
   b->node_off.assign(P + 1, 0);
   for (int32_t p = 0; p < P; ++p) {

@@ -27,7 +27,7 @@ struct FluxQuery {
   const double* props;
   const double* u;
   const double* log_d;
-  const double* dq;  // FVM_FLUX_D_EDGE: D at each query's edge midpoint
+  const double* dq;  // FVM_FLUX_D_EDGE: D at each query's edge midpoint; FVM_FLUX_LINEAR: a1..a6
   int32_t kind;
   double d0, d1, d2, d3;
   double* out;
@@ -65,6 +65,12 @@ __global__ __launch_bounds__(256) void flux_query_kernel(FluxQuery a) {
   const double be = s[3] * ua + s[4] * ub + s[5] * uc;
   const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
   const double mx = (px + qx) / 2, my = (py + qy) / 2;
+  if (a.kind == FVM_FLUX_LINEAR) {
+    const double* A = a.dq + 6 * q;
+    const double fx = A[0] * al + A[1] * be + A[2] * ga, fy = A[3] * al + A[4] * be + A[5] * ga;
+    a.out[q] = nx * fx + ny * fy;
+    return;
+  }
   double D;
   if (a.kind == FVM_FLUX_D_POLY_U) {
     const double um = al * mx + be * my + ga;
@@ -144,8 +150,9 @@ int fvm_compute_flux(fvm_ctx* c, int64_t n, const int32_t* d_i, const int32_t* d
   FVM_CHECK(c && n >= 0, "fvm_compute_flux: bad argument");
   if (n == 0) return FVM_OK;
   FVM_CHECK(d_i && d_j && d_u && d_out, "fvm_compute_flux: null argument");
-  FVM_CHECK(c->flux_kind != FVM_FLUX_D_EDGE || d_dq,
-            "fvm_compute_flux: FVM_FLUX_D_EDGE needs D at the edge midpoints (d_dq)");
+  FVM_CHECK((c->flux_kind != FVM_FLUX_D_EDGE && c->flux_kind != FVM_FLUX_LINEAR) || d_dq,
+            "fvm_compute_flux: FVM_FLUX_D_EDGE / FVM_FLUX_LINEAR need the flux coefficients at the edge "
+            "midpoints (d_dq)");
   const fvm_mesh* m = c->m;
   FluxQuery a;
   a.n = n;

@@ -119,11 +119,12 @@ __device__ __forceinline__ void tri_fluxes(const KArgs& a, const double* P, doub
   const double al = P[0] * ua + P[1] * ub + P[2] * uc;
   const double be = P[3] * ua + P[4] * ub + P[5] * uc;
   double ga = 0.0, dal = 0.0, dbe = 0.0, dga = 0.0, da = 0.0, db = 0.0, dg = 0.0;
-  if (FK == FVM_FLUX_D_POLY_U) ga = P[6] * ua + P[7] * ub + P[8] * uc;
+  constexpr bool G = FK == FVM_FLUX_D_POLY_U || FK == FVM_FLUX_LINEAR;  // gamma enters q
+  if (G) ga = P[6] * ua + P[7] * ub + P[8] * uc;
   if (MODE == M_JVP) {
     dal = P[0] * va + P[1] * vb + P[2] * vc;
     dbe = P[3] * va + P[4] * vb + P[5] * vc;
-    if (FK == FVM_FLUX_D_POLY_U) dga = P[6] * va + P[7] * vb + P[8] * vc;
+    if (G) dga = P[6] * va + P[7] * vb + P[8] * vc;
   }
   if (FK == FVM_FLUX_D_NODAL) {
     da = P[0] * Da + P[1] * Db + P[2] * Dc;
@@ -132,6 +133,12 @@ __device__ __forceinline__ void tri_fluxes(const KArgs& a, const double* P, doub
   }
 #pragma unroll
   for (int e = 0; e < 3; ++e) {
+    if (FK == FVM_FLUX_LINEAR) {
+      // (q . n) l with q = A (alpha, beta, gamma): the coefficients come projected
+      const double* c = Ds + 3 * e;
+      Q[e] = MODE != M_JVP ? c[0] * al + c[1] * be + c[2] * ga : c[0] * dal + c[1] * dbe + c[2] * dga;
+      continue;
+    }
     const double nx = P[P_NX + 2 * e], ny = P[P_NX + 1 + 2 * e], l = P[P_L + e];
     double D, Dd = 0.0;
     if (FK == FVM_FLUX_D_POLY_U) {
@@ -201,6 +208,10 @@ __device__ double bedge_flux(const KArgs& a, int e, int ep) {
   }
   const double mx = ep ? G.mx[1] : G.mx[0], my = ep ? G.my[1] : G.my[0];
   const int nf = a.bneu[e];
+  if (FK == FVM_FLUX_LINEAR && nf < 0) {
+    const double* c = a.d_bedge + 3 * (2 * e + ep);  // projected onto n, times the length
+    return MODE != M_JVP ? c[0] * al + c[1] * be + c[2] * ga : c[0] * dal + c[1] * dbe + c[2] * dga;
+  }
   double qn;
   if (nf < 0) {
     double D, Dd = 0.0;
@@ -330,7 +341,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
     iv[k] = idx < 3 * nt ? a.inc[ib0 + idx] : (uint16_t)0;
   }
   double P[TPT][P_COUNT];
-  double Ds[TPT][3];
+  double Ds[TPT][FK == FVM_FLUX_LINEAR ? 9 : 3];
   uint32_t L[TPT];
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
@@ -345,6 +356,10 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
       if (FK == FVM_FLUX_D_EDGE) {
 #pragma unroll
         for (int e = 0; e < 3; ++e) Ds[r][e] = __builtin_nontemporal_load(a.d_edge + e * a.T + s);
+      }
+      if (FK == FVM_FLUX_LINEAR) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Ds[r][k] = __builtin_nontemporal_load(a.d_edge + k * a.T + s);
       }
     }
   }
@@ -528,7 +543,12 @@ __device__ void bedge_adj(const KArgs& a, int e, int slot, double& ubar, double&
     const int node = a.bedges[ep * a.B + e];
     const double Qbar = -wt_of(a, node);  // du_node -= Q
     const double mx = ep ? G.mx[1] : G.mx[0], my = ep ? G.my[1] : G.my[0];
-    if (nf < 0) {
+    if (nf < 0 && FK == FVM_FLUX_LINEAR) {
+      const double* c = a.d_bedge + 3 * (2 * e + ep);
+      abar += Qbar * c[0];
+      bbar += Qbar * c[1];
+      gbar += Qbar * c[2];
+    } else if (nf < 0) {
       double Dg = 0, Dbar = 0;
       if (FK == FVM_FLUX_D_NODAL) Dg = da * mx + db * my + dg;
       if (FK == FVM_FLUX_D_EDGE) Dg = a.d_bedge[2 * e + ep];
@@ -582,7 +602,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
   const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
   const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
   double P[TPT][P_COUNT];
-  double Ds[TPT][3];
+  double Ds[TPT][FK == FVM_FLUX_LINEAR ? 9 : 3];
   uint32_t L[TPT];
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
@@ -597,6 +617,10 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
       if (FK == FVM_FLUX_D_EDGE) {
 #pragma unroll
         for (int e = 0; e < 3; ++e) Ds[r][e] = __builtin_nontemporal_load(a.d_edge + e * a.T + s);
+      }
+      if (FK == FVM_FLUX_LINEAR) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Ds[r][k] = __builtin_nontemporal_load(a.d_edge + k * a.T + s);
       }
     }
   }
@@ -648,7 +672,13 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
         if (FK == FVM_FLUX_D_NODAL) Dg = da * x + db * y + dg;
         if (FK == FVM_FLUX_D_EDGE) Dg = Ds[r][e];
         double Dbar = 0;
-        edge_adj<FK>(a, Qbar, x, y, nx, ny, l, al, be, ga, Dg, abar, bbar, gbar, Dbar);
+        if (FK == FVM_FLUX_LINEAR) {
+          abar += Qbar * Ds[r][3 * e];
+          bbar += Qbar * Ds[r][3 * e + 1];
+          gbar += Qbar * Ds[r][3 * e + 2];
+        } else {
+          edge_adj<FK>(a, Qbar, x, y, nx, ny, l, al, be, ga, Dg, abar, bbar, gbar, Dbar);
+        }
         if (FK == FVM_FLUX_D_NODAL) {
 #pragma unroll
           for (int v = 0; v < 3; ++v) lbar[v] += Dbar * (p[v] * x + p[3 + v] * y + p[6 + v]);
@@ -657,7 +687,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
 #pragma unroll
       for (int v = 0; v < 3; ++v) {
         double ub = p[v] * abar + p[3 + v] * bbar;
-        if (FK == FVM_FLUX_D_POLY_U) ub = ub + p[6 + v] * gbar;  // gamma only enters D(u)
+        if (FK == FVM_FLUX_D_POLY_U || FK == FVM_FLUX_LINEAR) ub = ub + p[6 + v] * gbar;  // gamma enters q
         cr[r][v] = ub;
         if (FK == FVM_FLUX_D_NODAL) cr[r][3 + v] = lbar[v] * Dn[v];
       }
@@ -816,6 +846,7 @@ int launch(const fvm_ctx* c, const KArgs& a, hipStream_t s) {
   switch (c->flux_kind) {
     case FVM_FLUX_D_POLY_U: return launch_fk<FVM_FLUX_D_POLY_U, MODE>(a, c->m->tile_tris, s);
     case FVM_FLUX_D_NODAL: return launch_fk<FVM_FLUX_D_NODAL, MODE>(a, c->m->tile_tris, s);
+    case FVM_FLUX_LINEAR: return launch_fk<FVM_FLUX_LINEAR, MODE>(a, c->m->tile_tris, s);
     default: return launch_fk<FVM_FLUX_D_EDGE, MODE>(a, c->m->tile_tris, s);
   }
 }
@@ -850,6 +881,62 @@ __global__ __launch_bounds__(256) void gather_kernel(int64_t n, const int32_t* _
   if (k < n) out[k] = x[idx[k]];
 }
 
+// FVM_FLUX_LINEAR: q = (a1 alpha + a2 beta + a3 gamma, a4 alpha + a5 beta + a6 gamma)
+// is projected once onto each CV edge's unit normal and length, so the kernels form
+// (q . n) l = c1 alpha + c2 beta + c3 gamma.  raw [6][3][T] sorted (row 6 e + k),
+// out [3][3][T] (row 3 e + k).
+__global__ __launch_bounds__(256) void project_edge_kernel(int64_t T, const double* __restrict__ props,
+                                                           const double* __restrict__ raw, double* __restrict__ out) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= T) return;
+  for (int e = 0; e < 3; ++e) {
+    const double nx = props[(P_NX + 2 * e) * T + s], ny = props[(P_NX + 1 + 2 * e) * T + s];
+    const double l = props[(P_L + e) * T + s];
+    for (int k = 0; k < 3; ++k)
+      out[(3 * e + k) * T + s] = (raw[(6 * e + k) * T + s] * nx + raw[(6 * e + 3 + k) * T + s] * ny) * l;
+  }
+}
+
+// The boundary half-edges: raw [2B][6], out [2B][3] with the outward normal and the
+// half-edge length of get_boundary_cv_components (control_volumes.jl:41-56).
+__global__ __launch_bounds__(256) void project_bedge_kernel(KArgs a, const double* __restrict__ raw,
+                                                            double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 2 * a.B) return;
+  const BGeo G = bgeo(a, (int)(k >> 1));
+  const double* A = raw + 6 * k;
+  for (int c = 0; c < 3; ++c) out[3 * k + c] = (A[c] * G.nx + A[3 + c] * G.ny) * G.ell;
+}
+
+int project_linear(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
+  const int64_t T = m->T, B = m->B;
+  std::vector<double> raw(18 * (size_t)T);
+  for (int64_t s = 0; s < T; ++s)
+    for (int j = 0; j < 18; ++j) raw[(size_t)j * T + s] = ph->d_edge[18 * (int64_t)m->h_perm[s] + j];
+  double *d_raw = nullptr, *d_braw = nullptr;
+  int rc = dev_copy(&d_raw, raw.data(), 18 * T);
+  if (rc == FVM_OK) rc = dev_copy(&d_braw, ph->d_bedge, 12 * B);
+  if (rc == FVM_OK) rc = dev_copy(&c->d_edge, (const double*)nullptr, 9 * T);
+  if (rc == FVM_OK) rc = dev_copy(&c->d_bedge, (const double*)nullptr, 6 * B);
+  if (rc == FVM_OK && T > 0)
+    hipLaunchKernelGGL(project_edge_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, nullptr, T, m->props,
+                       d_raw, c->d_edge);
+  if (rc == FVM_OK && B > 0) {
+    KArgs k{};
+    k.B = (int32_t)B;
+    k.x = m->x;
+    k.y = m->y;
+    k.bedges = m->bedges;
+    hipLaunchKernelGGL(project_bedge_kernel, dim3((unsigned)((2 * B + 255) / 256)), dim3(256), 0, nullptr, k,
+                       d_braw, c->d_bedge);
+  }
+  if (rc == FVM_OK && hipDeviceSynchronize() != hipSuccess) rc = FVM_ERR_HIP;
+  if (rc == FVM_OK && hipGetLastError() != hipSuccess) rc = FVM_ERR_HIP;
+  if (d_raw) (void)hipFree(d_raw);
+  if (d_braw) (void)hipFree(d_braw);
+  return rc;
+}
+
 void free_ctx(fvm_ctx* c) {
   if (!c) return;
   void* ptrs[] = {c->d_edge, c->d_bedge, c->log_d_own, c->s_node, c->kind, c->fidx,
@@ -862,7 +949,7 @@ void free_ctx(fvm_ctx* c) {
 
 int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
   const int64_t N = m->N, T = m->T, B = m->B;
-  FVM_CHECK(ph->flux_kind >= FVM_FLUX_D_POLY_U && ph->flux_kind <= FVM_FLUX_D_EDGE,
+  FVM_CHECK(ph->flux_kind >= FVM_FLUX_D_POLY_U && ph->flux_kind <= FVM_FLUX_LINEAR,
             "fvm_ctx_create: unknown flux_kind");
   FVM_CHECK(ph->node_kind, "fvm_ctx_create: node_kind is required");
   FVM_CHECK(ph->node_fidx, "fvm_ctx_create: node_fidx is required");
@@ -870,7 +957,7 @@ int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
   FVM_CHECK(ph->n_functions >= 0, "fvm_ctx_create: negative n_functions");
   FVM_CHECK(ph->n_functions == 0 || (ph->fn_c0 && ph->fn_cu && ph->fn_ct),
             "fvm_ctx_create: condition function coefficients missing");
-  if (ph->flux_kind == FVM_FLUX_D_EDGE)
+  if (ph->flux_kind == FVM_FLUX_D_EDGE || ph->flux_kind == FVM_FLUX_LINEAR)
     FVM_CHECK(ph->d_edge && (B == 0 || ph->d_bedge), "fvm_ctx_create: d_edge / d_bedge required");
   if (ph->flux_kind == FVM_FLUX_D_NODAL) FVM_CHECK(ph->log_d, "fvm_ctx_create: log_d required");
   c->m = m;
@@ -928,6 +1015,7 @@ int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
     DC(c->d_edge, de.data(), 3 * T);
     DC(c->d_bedge, ph->d_bedge, 2 * B);
   }
+  if (ph->flux_kind == FVM_FLUX_LINEAR && (rc = project_linear(m, ph, c)) != FVM_OK) return rc;
   if (ph->log_d) {
     DC(c->log_d_own, ph->log_d, N);
     c->log_d = c->log_d_own;
@@ -1046,12 +1134,14 @@ int fvm_rhs_timed(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_str
   switch (c->flux_kind) {
     case FVM_FLUX_D_POLY_U: launch_tiles<FVM_FLUX_D_POLY_U, M_RHS>(a, tt, st); break;
     case FVM_FLUX_D_NODAL: launch_tiles<FVM_FLUX_D_NODAL, M_RHS>(a, tt, st); break;
+    case FVM_FLUX_LINEAR: launch_tiles<FVM_FLUX_LINEAR, M_RHS>(a, tt, st); break;
     default: launch_tiles<FVM_FLUX_D_EDGE, M_RHS>(a, tt, st);
   }
   FVM_HIP(hipEventRecord(ev[1], st));
   switch (c->flux_kind) {
     case FVM_FLUX_D_POLY_U: launch_fix<FVM_FLUX_D_POLY_U, M_RHS>(a, st); break;
     case FVM_FLUX_D_NODAL: launch_fix<FVM_FLUX_D_NODAL, M_RHS>(a, st); break;
+    case FVM_FLUX_LINEAR: launch_fix<FVM_FLUX_LINEAR, M_RHS>(a, st); break;
     default: launch_fix<FVM_FLUX_D_EDGE, M_RHS>(a, st);
   }
   FVM_HIP(hipEventRecord(ev[2], st));
@@ -1092,6 +1182,7 @@ int fvm_vjp(fvm_ctx* c, const double* d_u, const double* d_w, double* d_ubar, do
   switch (c->flux_kind) {
     case FVM_FLUX_D_POLY_U: return launch_vjp_fk<FVM_FLUX_D_POLY_U>(a, c->m->tile_tris, st);
     case FVM_FLUX_D_NODAL: return launch_vjp_fk<FVM_FLUX_D_NODAL>(a, c->m->tile_tris, st);
+    case FVM_FLUX_LINEAR: return launch_vjp_fk<FVM_FLUX_LINEAR>(a, c->m->tile_tris, st);
     default: return launch_vjp_fk<FVM_FLUX_D_EDGE>(a, c->m->tile_tris, st);
   }
 }

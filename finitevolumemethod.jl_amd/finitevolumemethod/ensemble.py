@@ -11,7 +11,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import _lib
+from . import _lib, forms
 from .problem import FVMProblem, physics_desc, physics_of
 from .solve import Euler, ODESolution
 
@@ -81,16 +81,18 @@ class EnsembleProblem:
         batch from its initial_time to its final_time with its own dt (a scalar, a
         list, or None for explicit_dt); every member must take the same number of
         steps; the solutions hold the initial and final states.  Any other algorithm
-        (Tsit5(), TRBDF2(), ...): the members are solved one after the other on the
-        device, each with its own adaptive steps (SciML's EnsembleSerial), keyword
-        arguments passed to solve.  Returns an EnsembleSolution."""
+        (Tsit5(), TRBDF2(), ...), systems, and members whose flux function is a linear
+        form other than -D(x, y) grad u (advection; not packed): the members are solved
+        one after the other on the device, each with its own adaptive steps (SciML's
+        EnsembleSerial), keyword arguments passed to solve.  Returns an EnsembleSolution."""
         from .solve import solve as solve_one
         from .system import FVMSystem
 
         if trajectories is not None and not isinstance(self._prob, (list, tuple)):
             self._make(trajectories)
         alg = alg if alg is not None else Euler()
-        if not isinstance(alg, Euler) or any(isinstance(p, FVMSystem) for p in self.problems):
+        if (not isinstance(alg, Euler) or any(isinstance(p, FVMSystem) for p in self.problems)
+                or any(_linear_flux(p) for p in self.problems)):
             tm = _lib.Timer()
             tm.start()
             sols = [solve_one(p, alg, dt=dt, **kwargs) for p in self.problems]
@@ -127,6 +129,15 @@ class EnsembleProblem:
             sols.append(s)
         return EnsembleSolution(u=sols, stats=dict(steps=nsteps, ms=ms, dt=dts,
                                                    tile_triangles=b.info.tile_triangles))
+
+
+def _linear_flux(p):
+    """Whether an FVMProblem's flux function traces to the general linear form."""
+    q = getattr(p, "user_flux_function", None)
+    if q is None:
+        return False
+    fp = p.flux_parameters
+    return forms.classify_flux(lambda x, y, t, a, b, g: q(x, y, t, a, b, g, fp))[0] == "linear"
 
 
 def _uniform_form(ph, tri):

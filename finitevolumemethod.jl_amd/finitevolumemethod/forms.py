@@ -11,7 +11,8 @@ trace must land on one of the forms the kernels implement:
       coefficients (FVM_FLUX_D_POLY_U), or any function of (x, y) alone
       (FVM_FLUX_D_EDGE: evaluated at every CV-edge midpoint and boundary
       half-edge point), or a `NodalDiffusion` field (FVM_FLUX_D_NODAL);
-  flux q(x, y, t, a, b, g, p) -> (qx, qy): -D(x, y) (a, b);
+  flux q(x, y, t, a, b, g, p) -> (qx, qy): -D(x, y) (a, b) (FVM_FLUX_D_EDGE), or
+      any q linear in (a, b, g) with coefficients of (x, y) (FVM_FLUX_LINEAR);
   source S(x, y, t, u, p): s(x, y) + cubic polynomial in u + c t;
   condition a(x, y, t, u, p): g(x, y) + cu u + ct t.
 
@@ -345,30 +346,40 @@ def eval_xy(fn, x, y):
 
 
 def classify_flux(q):
-    """Trace q(x, y, t, a, b, g, p) -> (qx, qy) (already bound to p): must be
-    -D(x, y) (a, b).  Returns D as a function of (x, y) (arrays)."""
+    """Trace q(x, y, t, a, b, g, p) -> (qx, qy) (already bound to p).  It must be
+    linear in (alpha, beta, gamma), with coefficients that may depend on (x, y) but
+    not on t.  Returns ("xy", D) when q = -D(x, y) (alpha, beta) (FVM_FLUX_D_EDGE,
+    D a function of (x, y) arrays), otherwise ("linear", A) with A(x, y) -> [6, n]
+    the coefficients of q = (a1 alpha + a2 beta + a3 gamma, a4 alpha + a5 beta +
+    a6 gamma) (FVM_FLUX_LINEAR: advection-diffusion, anisotropic diffusion)."""
     probe_x = np.array([0.1234, 0.7421, 0.3317])
     probe_y = np.array([0.5531, 0.0713, 0.8822])
-    r = _trace(q, 4, probe_x, probe_y, [0.0, None, None, None], [1, 2, 3])  # vars: a, b, g
-    if not isinstance(r, (tuple, list)) or len(r) != 2:
-        raise UnsupportedClosure("the flux function must return a pair (qx, qy)")
-    qx, qy = (_as_sym(v, 4) for v in r)
-    ok = all(e == (1, 0, 0, 0) for e in qx.nonzero_exponents()) and all(
-        e == (0, 1, 0, 0) for e in qy.nonzero_exponents())
-    if not ok:
-        raise UnsupportedClosure("flux functions must have the form q = -D(x, y) * (alpha, beta); "
-                                 "give u-dependent diffusion as diffusion_function")
-    if not np.allclose(np.asarray(qx.coef((1, 0, 0, 0))), np.asarray(qy.coef((0, 1, 0, 0))),
-                       rtol=0, atol=0):
-        raise UnsupportedClosure("flux functions must use the same D for both components")
+    lin = ((1, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 0))  # alpha, beta, gamma
 
-    def D(x, y):
-        rr = q(np.asarray(x, np.float64), np.asarray(y, np.float64), 0.0,
-               Sym.var(4, 0), Sym.var(4, 1), Sym.var(4, 2))
-        c = _as_sym(rr[0], 4).coef((1, 0, 0, 0))
-        return -np.broadcast_to(np.asarray(c, np.float64), np.shape(x))
+    def coefs(x, y, t):
+        x = np.asarray(x, np.float64)
+        y = np.asarray(y, np.float64)
+        r = _trace(q, 4, x, y, [t, None, None, None], [1, 2, 3])  # vars: a, b, g
+        if not isinstance(r, (tuple, list)) or len(r) != 2:
+            raise UnsupportedClosure("the flux function must return a pair (qx, qy)")
+        out = np.zeros((6, len(x)))
+        for c, comp in enumerate(_as_sym(v, 4) for v in r):
+            for e in comp.nonzero_exponents():
+                if e not in lin:
+                    raise UnsupportedClosure(
+                        "flux functions must be linear in (alpha, beta, gamma), e.g. "
+                        "q = -D(x, y) (alpha, beta) + v(x, y) u; give u-dependent diffusion as "
+                        "diffusion_function")
+            for k, e in enumerate(lin):
+                out[3 * c + k] = np.broadcast_to(np.asarray(comp.coef(e), np.float64), x.shape)
+        return out
 
-    return D
+    A0 = coefs(probe_x, probe_y, 0.0)
+    if not np.array_equal(A0, coefs(probe_x, probe_y, 1.37)):
+        raise UnsupportedClosure("time-dependent flux functions are not supported by the device kernels")
+    if not A0[[1, 2, 3, 5]].any() and np.array_equal(A0[0], A0[4]):
+        return "xy", lambda x, y: -coefs(x, y, 0.0)[0]
+    return "linear", lambda x, y: coefs(x, y, 0.0)
 
 
 def group_indices(fidx_of_locations):

@@ -258,9 +258,9 @@ def physics_of(prob):
     else:
         if prob.user_flux_function is not None:
             q = prob.user_flux_function
-            Dxy = forms.classify_flux(lambda x, y, t, a, b, g: q(x, y, t, a, b, g, prob.flux_parameters))
-            kind, poly = "xy", None
-            Dfun = lambda x, y, t, u: Dxy(x, y)  # noqa: E731
+            kind, F = forms.classify_flux(lambda x, y, t, a, b, g: q(x, y, t, a, b, g, prob.flux_parameters))
+            poly = None
+            Dfun = lambda x, y, t, u: F(x, y)  # noqa: E731
         else:
             if D is None:
                 raise ValueError("FVMProblem needs a diffusion_function or a flux_function")
@@ -269,6 +269,14 @@ def physics_of(prob):
             kind, poly = forms.classify_diffusion(Dfun)
         if kind == "poly":
             ph.update(flux_kind=_lib.FLUX_D_POLY_U, dpoly=poly)
+        elif kind == "linear":
+            # a1..a6 per CV edge [T][3][6] and per boundary half-edge point [2B][6]
+            P = mesh.props_array
+            ex = P[:, [9, 11, 13]].ravel()
+            ey = P[:, [10, 12, 14]].ravel()
+            mx, my = _half_edge_points(tri)
+            ph.update(flux_kind=_lib.FLUX_LINEAR, d_edge=F(ex, ey).T.ravel(), d_bedge=F(mx, my).T.ravel(),
+                      a_xy=F)
         else:
             P = mesh.props_array
             ex = P[:, [9, 11, 13]].ravel()

@@ -47,11 +47,16 @@ def compute_flux(prob, i, j, u, t):
         r = out.to_host()[:n * nv].reshape(n, nv)
         return tuple(float(v) for v in r[0]) if scalar else r
     dq = None
-    if ctx.flux_kind == _lib.FLUX_D_EDGE:
+    if ctx.flux_kind in (_lib.FLUX_D_EDGE, _lib.FLUX_LINEAR):
         tri = prob.mesh.triangulation
-        mx = (tri.x[qi] + tri.x[qj]) / 2
-        my = (tri.y[qi] + tri.y[qj]) / 2
-        dq = _lib.DeviceArray.from_host(forms.eval_xy(ctx.physics["d_xy"], mx, my))
+        ok = (qi >= 0) & (qi < tri.num_points()) & (qj >= 0) & (qj < tri.num_points())
+        ii, jj = np.where(ok, qi, 0), np.where(ok, qj, 0)  # unknown nodes are the kernel's error
+        mx = (tri.x[ii] + tri.x[jj]) / 2
+        my = (tri.y[ii] + tri.y[jj]) / 2
+        if ctx.flux_kind == _lib.FLUX_D_EDGE:
+            dq = _lib.DeviceArray.from_host(forms.eval_xy(ctx.physics["d_xy"], mx, my))
+        else:
+            dq = _lib.DeviceArray.from_host(np.ascontiguousarray(ctx.physics["a_xy"](mx, my).T.ravel()))
     out = _lib.DeviceArray(max(n, 1))
     _lib.check(L.fvm_compute_flux(ctx.handle, n, d_i.ptr, d_j.ptr, d_u.ptr, float(t),
                                   dq.ptr if dq is not None else None, out.ptr, None), "compute_flux")

@@ -105,16 +105,22 @@ enum {
                             u_s = alpha x_s + beta y_s + gamma (construct_flux_function) */
   FVM_FLUX_D_NODAL = 1,  /* q = -D(x_s)(alpha, beta), D the piecewise-linear interpolant of
                             exp(logD) (pl_interpolate of a nodal field) */
-  FVM_FLUX_D_EDGE = 2    /* q = -D_s (alpha, beta) with D_s given at every CV-edge midpoint and
+  FVM_FLUX_D_EDGE = 2,   /* q = -D_s (alpha, beta) with D_s given at every CV-edge midpoint and
                             boundary half-edge point (D(x, y) independent of t and u) */
+  FVM_FLUX_LINEAR = 3    /* q = (a1 alpha + a2 beta + a3 gamma, a4 alpha + a5 beta + a6 gamma), the
+                            a's given at every CV-edge midpoint and boundary half-edge point
+                            (independent of t and u): advection-diffusion, anisotropic
+                            diffusion, any flux linear in u and its gradient */
 };
 enum { FVM_NODE_NONE = 0, FVM_NODE_DIRICHLET = 1, FVM_NODE_DUDT = 2 };
 
 typedef struct fvm_physics_desc {
   int32_t flux_kind;
   double dpoly[4];              /* FVM_FLUX_D_POLY_U */
-  const double* d_edge;         /* [3*n_triangles] input triangle order, CV edge (i,j),(j,k),(k,i) */
-  const double* d_bedge;        /* [2*n_boundary_edges] at the half-edge points m_i, m_j */
+  const double* d_edge;         /* [3*n_triangles] input triangle order, CV edge (i,j),(j,k),(k,i);
+                                   FVM_FLUX_LINEAR: [18*n_triangles], a1..a6 of each CV edge */
+  const double* d_bedge;        /* [2*n_boundary_edges] at the half-edge points m_i, m_j;
+                                   FVM_FLUX_LINEAR: [12*n_boundary_edges], a1..a6 of each */
   const double* log_d;          /* [n_points] FVM_FLUX_D_NODAL (replaceable by fvm_set_log_d) */
   double spoly[4];              /* S(x,y,t,u) = s_node[i] + s0 + u(s1 + u(s2 + u s3)) + st t */
   double st;
@@ -173,7 +179,8 @@ int fvm_vjp(fvm_ctx* c, const double* d_u, const double* d_w, double* d_ubar,
  * edge midpoint, n the clockwise normal of i -> j, the shape function from the
  * triangle on the right of i -> j (or on its left for a boundary edge with the
  * domain on the left).  d_dq [n]: D at the edge midpoints, required for
- * FVM_FLUX_D_EDGE (D(x, y) is a host closure), NULL otherwise. */
+ * FVM_FLUX_D_EDGE (D(x, y) is a host closure); [6n] a1..a6 at the edge midpoints
+ * for FVM_FLUX_LINEAR; NULL otherwise. */
 int fvm_compute_flux(fvm_ctx* c, int64_t n, const int32_t* d_i, const int32_t* d_j, const double* d_u,
                      double t, const double* d_dq, double* d_out, fvm_stream s);
 /* out[q * n_vars + v] = alpha_v x_q + beta_v y_q + gamma_v on the mesh triangle

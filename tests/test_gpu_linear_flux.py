@@ -1,0 +1,125 @@
+"""FVM_FLUX_LINEAR: flux functions linear in (alpha, beta, gamma) with coefficients
+of (x, y), e.g. the advection-diffusion flux of
+piecewise_linear_and_natural_neighbour_interpolation_for_an_advection_diffusion_equation.jl:80-87,
+q = (nu u - D alpha, -D beta).  RHS against the oracle calling the closure itself
+(1e-12 bar of the RHS tests), JVP against central differences, VJP by the adjoint
+identity, compute_flux against the oracle, and the tutorial's closed-form solution
+with its own error measure."""
+import numpy as np
+import pytest
+
+import finitevolumemethod as FVM
+from finitevolumemethod import _lib
+from problems import oracle_for
+from test_gpu_queries import check_flux
+from test_gpu_rhs import assert_parity
+
+pytestmark = pytest.mark.gpu
+
+
+def advective(x, y, t, a, b, g, p):
+    u = a * x + b * y + g
+    return p["vx"] * u - p["D"] * a, p["vy"] * u - p["D"] * b
+
+
+def anisotropic(x, y, t, a, b, g, p):
+    """q = -K(x, y) grad u with a symmetric positive definite K varying in space."""
+    kxx, kxy, kyy = 0.5 + x * x, 0.2 * np.sin(3 * y), 0.7 + 0.3 * y
+    return -(kxx * a + kxy * b), -(kxy * a + kyy * b)
+
+
+def problem(flux, params, nx=30, ny=26, seed=4):
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, nx, ny, jitter=0.25, seed=seed, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(
+        mesh, (lambda x, y, t, u, p: 0.1 * x + 0.0 * u, lambda x, y, t, u, p: 0.0 * u,
+               lambda x, y, t, u, p: -0.3 + 0.0 * u, lambda x, y, t, u, p: 1.0 + 0.0 * u),
+        (FVM.Neumann, FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet))
+    u = np.random.default_rng(seed).random(tri.num_points())
+    prob = FVM.FVMProblem(mesh, BCs, flux_function=flux, flux_parameters=params,
+                          source_function=lambda x, y, t, u, p: u * (1 - u), initial_condition=u,
+                          final_time=1.0)
+    orc = oracle_for(prob, dict(flux=lambda x, y, t, a, b, g, T: flux(x, y, t, a, b, g, params),
+                                source=lambda x, y, t, u, n: u * (1 - u),
+                                cond=lambda f, x, y, t, u: prob.conditions.functions[f](x, y, t, u)))
+    return prob, orc, u
+
+
+CASES = {"advective": (advective, dict(vx=0.8, vy=-0.35, D=0.02)), "anisotropic": (anisotropic, None)}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_rhs_matches_oracle(gpu, case):
+    prob, orc, u = problem(*CASES[case])
+    assert prob.device_context().flux_kind == _lib.FLUX_LINEAR
+    for t in (0.0, 0.4):
+        du = FVM.fvm_eqs(np.zeros_like(u), u, prob, t)
+        assert_parity(du, orc.rhs(u, t), orc.scale(u, t))
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_jvp_and_vjp(gpu, case):
+    """J v against central differences (exact for this quadratic F up to rounding),
+    and w . (J v) = (J^T w) . v."""
+    prob, orc, u = problem(*CASES[case])
+    ctx = prob.device_context()
+    n = len(u)
+    rng = np.random.default_rng(9)
+    v, w = rng.standard_normal(n), rng.standard_normal(n)
+    d = {k: _lib.DeviceArray.from_host(a) for k, a in (("u", u), ("v", v), ("w", w))}
+    jv, jtw = _lib.DeviceArray(n), _lib.DeviceArray(n)
+    ctx.jvp(d["u"], d["v"], jv, 0.2)
+    ctx.vjp(d["u"], d["w"], jtw, None, 0.2)
+    Jv, JTw = jv.to_host(), jtw.to_host()
+    h = 1e-6
+    fd = (FVM.fvm_eqs(np.zeros(n), u + h * v, prob, 0.2) - FVM.fvm_eqs(np.zeros(n), u - h * v, prob, 0.2)) / (2 * h)
+    scale = orc.scale(u, 0.2) * (1 + np.abs(v))
+    assert np.all(np.abs(Jv - fd) <= 1e-6 * scale), np.max(np.abs(Jv - fd) / scale)
+    lhs, rhs = w @ Jv, JTw @ v
+    assert abs(lhs - rhs) <= 1e-11 * np.sum(np.abs(w * Jv))
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_compute_flux_matches_oracle(gpu, case):
+    flux, params = CASES[case]
+    prob, orc, u = problem(flux, params, nx=13, ny=11)
+    check_flux(prob, u, lambda x, y, t, a, b, g, T: flux(x, y, t, a, b, g, params))
+
+
+def test_advection_diffusion_closed_form(gpu):
+    """u_t + div(nu u e_x - D grad u) = 0 from the exact solution at t = 1
+    (1/(4 pi D t)) exp(-((x - nu t)^2 + y^2) / (4 D t)) (the tutorial's
+    exact_solution; its own run starts from a near-delta on a graded mesh that needs
+    DelaunayTriangulation's refinement) to t = 11 on a 161 x 129 jittered mesh of
+    [-3, 5] x [-3, 3.4], homogeneous Dirichlet; the tutorial's measure: the mean over
+    nodes of 100 |u - exact| / max u at every saved time <= 0.16."""
+    Dc, nu = 0.02, 0.05
+
+    def exact(x, y, t):
+        return 1 / (4 * Dc * np.pi * t) * np.exp(1 / (4 * Dc * t) * (-(x - nu * t) ** 2 - y ** 2))
+
+    tri = FVM.jittered_rectangle(-3.0, 5.0, -3.0, 3.4, 161, 129, jitter=0.2, seed=3)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    u0 = exact(tri.x, tri.y, 1.0)
+    u0[tri.boundary_nodes()] = 0.0
+    prob = FVM.FVMProblem(mesh, BCs, flux_function=advective, flux_parameters=dict(vx=nu, vy=0.0, D=Dc),
+                          initial_condition=u0, initial_time=1.0, final_time=11.0)
+    sol = FVM.solve(prob, FVM.Tsit5(), saveat=[1.0, 3.0, 6.0, 11.0], abstol=1e-9, reltol=1e-7)
+    assert sol.retcode == "Success"
+    for t, u in zip(sol.t[1:], sol.u[1:]):
+        err = np.mean(100 * np.abs(u - exact(tri.x, tri.y, t)) / np.max(u))
+        assert err <= 0.16, (t, err)
+    # mass is carried, not lost: the integral drifts only by the (tiny) boundary outflow
+    V = mesh.cv_volumes
+    assert abs(V @ sol.u[-1] - V @ sol.u[0]) <= 1e-3 * (V @ sol.u[0])
+
+
+def test_ensemble_of_linear_flux_problems_runs_member_by_member(gpu):
+    flux, params = CASES["advective"]
+    probs = [problem(flux, params, nx=12, ny=10, seed=s)[0] for s in (1, 2)]
+    for p in probs:
+        p.final_time = 0.01
+    ens = FVM.EnsembleProblem(probs)
+    sol = FVM.solve(ens, FVM.Euler(), dt=1e-3)
+    assert sol.stats["mode"] == "serial" and len(sol.u) == 2

@@ -214,8 +214,41 @@ def test_closure_tracing_forms():
     np.testing.assert_array_equal(r["g"], np.full(3, -50.0))
     with pytest.raises(forms.UnsupportedClosure):
         forms.classify_condition(lambda x, y, t, u: x * t * u, x, x)
-    D = forms.classify_flux(lambda x, y, t, a, b, g: (-80e-6 * a, -80e-6 * b))
+    kind, D = forms.classify_flux(lambda x, y, t, a, b, g: (-80e-6 * a, -80e-6 * b))
+    assert kind == "xy"
     np.testing.assert_array_equal(D(x, x), np.full(3, 80e-6))
+
+
+def test_linear_flux_forms_trace():
+    """piecewise_linear_and_natural_neighbour_interpolation_for_an_advection_diffusion_equation.jl:80-87:
+    q = (nu u - D alpha, -D beta) with u = alpha x + beta y + gamma is linear in (alpha,
+    beta, gamma) with (x, y)-dependent coefficients; an anisotropic tensor too; a
+    t-dependent, a constant or a quadratic term is not."""
+    x = np.array([0.1, -0.4, 0.9])
+    y = np.array([0.3, 0.8, -0.2])
+    D, nu = 0.02, 0.05
+
+    def adv(x, y, t, a, b, g):
+        u = a * x + b * y + g
+        return nu * u - D * a, -D * b
+
+    kind, A = forms.classify_flux(adv)
+    assert kind == "linear"
+    got = A(x, y)
+    assert got.shape == (6, 3)
+    np.testing.assert_array_equal(got[0], nu * x - D)
+    np.testing.assert_array_equal(got[1], nu * y)
+    np.testing.assert_array_equal(got[2], np.full(3, nu))
+    np.testing.assert_array_equal(got[3:5], np.array([[0.0] * 3, [-D] * 3]))
+    np.testing.assert_array_equal(got[5], np.zeros(3))
+    kind, A = forms.classify_flux(lambda x, y, t, a, b, g: (-(2 + x) * a - 0.5 * b, -0.5 * a - (1 + y ** 2) * b))
+    assert kind == "linear"
+    np.testing.assert_array_equal(A(x, y)[[0, 1, 3, 4]], np.array([-(2 + x), [-0.5] * 3, [-0.5] * 3, -(1 + y ** 2)]))
+    for bad in (lambda x, y, t, a, b, g: (-(1 + t) * a, -b),
+                lambda x, y, t, a, b, g: (-a + 1.0, -b),
+                lambda x, y, t, a, b, g: (-(a * x + b * y + g) ** 2 * a, -b)):
+        with pytest.raises(forms.UnsupportedClosure):
+            forms.classify_flux(bad)
 
 
 def test_c_abi_library_exports_every_declared_symbol():
