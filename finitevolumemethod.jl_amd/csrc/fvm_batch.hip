@@ -122,7 +122,7 @@ int pack(int32_t P, const fvm_mesh_desc* meshes, const fvm_physics_desc* ph, fvm
               "fvm_batch_create: member condition coefficients missing");
     for (int c = 0; c < 4; ++c) {
       FVM_CHECK(q.spoly[c] == ph[0].spoly[c], "fvm_batch_create: members must share the source polynomial");
-      if (fk == FVM_FLUX_D_POLY_U)
+      if (fk == FVM_FLUX_D_POLY_U || fk == FVM_FLUX_D_POWER)
         FVM_CHECK(q.dpoly[c] == ph[0].dpoly[c], "fvm_batch_create: members must share D(u)");
     }
     FVM_CHECK(q.st == ph[0].st, "fvm_batch_create: members must share the source's t coefficient");

@@ -75,6 +75,9 @@ __global__ __launch_bounds__(256) void flux_query_kernel(FluxQuery a) {
   if (a.kind == FVM_FLUX_D_POLY_U) {
     const double um = al * mx + be * my + ga;
     D = a.d0 + um * (a.d1 + um * (a.d2 + um * a.d3));
+  } else if (a.kind == FVM_FLUX_D_POWER) {
+    const double um = al * mx + be * my + ga;
+    D = a.d0 + a.d1 * pow(fabs(um), a.d2);
   } else if (a.kind == FVM_FLUX_D_NODAL) {
     const double Da = exp(a.log_d[va]), Db = exp(a.log_d[vb]), Dc = exp(a.log_d[vc]);
     const double da = s[0] * Da + s[1] * Db + s[2] * Dc;

@@ -99,6 +99,26 @@ __device__ __forceinline__ double dpoly_du(const KArgs& a, double u) {
   return a.d1 + u * (2.0 * a.d2 + 3.0 * a.d3 * u);
 }
 
+// Forms whose D depends on u_s = alpha x_s + beta y_s + gamma: the cubic, or the
+// power law D = d0 + d1 |u|^d2 (porous-medium diffusivity, d2 > 0).
+__host__ __device__ constexpr bool d_of_u(int fk) { return fk == FVM_FLUX_D_POLY_U || fk == FVM_FLUX_D_POWER; }
+template <int FK>
+__device__ __forceinline__ double dfun(const KArgs& a, double u) {
+  if (FK == FVM_FLUX_D_POWER) return a.d0 + a.d1 * pow(fabs(u), a.d2);
+  return dpoly(a, u);
+}
+// dD/du; for the power law d1 d2 |u|^(d2 - 1) sign(u), taken as 0 at u = 0
+template <int FK>
+__device__ __forceinline__ double dfun_du(const KArgs& a, double u) {
+  if (FK == FVM_FLUX_D_POWER) {
+    const double au = fabs(u);
+    if (!(au > 0.0)) return 0.0;
+    const double d = a.d1 * a.d2 * pow(au, a.d2 - 1.0);
+    return u > 0.0 ? d : -d;
+  }
+  return dpoly_du(a, u);
+}
+
 // S(x, y, t, u) of the source term (source_contributions.jl:2-14)
 __device__ __forceinline__ double source(const KArgs& a, int g, double u) {
   double S = a.s0 + u * (a.s1 + u * (a.s2 + u * a.s3));
@@ -119,7 +139,7 @@ __device__ __forceinline__ void tri_fluxes(const KArgs& a, const double* P, doub
   const double al = P[0] * ua + P[1] * ub + P[2] * uc;
   const double be = P[3] * ua + P[4] * ub + P[5] * uc;
   double ga = 0.0, dal = 0.0, dbe = 0.0, dga = 0.0, da = 0.0, db = 0.0, dg = 0.0;
-  constexpr bool G = FK == FVM_FLUX_D_POLY_U || FK == FVM_FLUX_LINEAR;  // gamma enters q
+  constexpr bool G = d_of_u(FK) || FK == FVM_FLUX_LINEAR;  // gamma enters q
   if (G) ga = P[6] * ua + P[7] * ub + P[8] * uc;
   if (MODE == M_JVP) {
     dal = P[0] * va + P[1] * vb + P[2] * vc;
@@ -141,11 +161,11 @@ __device__ __forceinline__ void tri_fluxes(const KArgs& a, const double* P, doub
     }
     const double nx = P[P_NX + 2 * e], ny = P[P_NX + 1 + 2 * e], l = P[P_L + e];
     double D, Dd = 0.0;
-    if (FK == FVM_FLUX_D_POLY_U) {
+    if (d_of_u(FK)) {
       const double x = P[P_MX + 2 * e], y = P[P_MX + 1 + 2 * e];
       const double us = al * x + be * y + ga;
-      D = dpoly(a, us);
-      if (MODE == M_JVP) Dd = dpoly_du(a, us) * (dal * x + dbe * y + dga);
+      D = dfun<FK>(a, us);
+      if (MODE == M_JVP) Dd = dfun_du<FK>(a, us) * (dal * x + dbe * y + dga);
     } else if (FK == FVM_FLUX_D_NODAL) {
       const double x = P[P_MX + 2 * e], y = P[P_MX + 1 + 2 * e];
       D = da * x + db * y + dg;
@@ -215,10 +235,10 @@ __device__ double bedge_flux(const KArgs& a, int e, int ep) {
   double qn;
   if (nf < 0) {
     double D, Dd = 0.0;
-    if (FK == FVM_FLUX_D_POLY_U) {
+    if (d_of_u(FK)) {
       const double um = al * mx + be * my + ga;
-      D = dpoly(a, um);
-      if (MODE == M_JVP) Dd = dpoly_du(a, um) * (dal * mx + dbe * my + dga);
+      D = dfun<FK>(a, um);
+      if (MODE == M_JVP) Dd = dfun_du<FK>(a, um) * (dal * mx + dbe * my + dga);
     } else if (FK == FVM_FLUX_D_NODAL) {
       const double Da = exp(a.log_d[va]), Db = exp(a.log_d[vb]), Dc = exp(a.log_d[vc]);
       const double da = s[0] * Da + s[1] * Db + s[2] * Dc;
@@ -503,10 +523,10 @@ __device__ __forceinline__ void edge_adj(const KArgs& a, double Qbar, double x, 
                                          double& Dbar) {
   const double gn = al * nx + be * ny;
   double D, Du = 0.0;
-  if (FK == FVM_FLUX_D_POLY_U) {
+  if (d_of_u(FK)) {
     const double us = al * x + be * y + ga;
-    D = dpoly(a, us);
-    Du = dpoly_du(a, us);
+    D = dfun<FK>(a, us);
+    Du = dfun_du<FK>(a, us);
   } else {
     D = D_given;
   }
@@ -648,7 +668,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
       const double ua = s_u[id[0]], ub = s_u[id[1]], uc = s_u[id[2]];
       const double al = p[0] * ua + p[1] * ub + p[2] * uc;
       const double be = p[3] * ua + p[4] * ub + p[5] * uc;
-      const double ga = FK == FVM_FLUX_D_POLY_U ? p[6] * ua + p[7] * ub + p[8] * uc : 0.0;
+      const double ga = d_of_u(FK) ? p[6] * ua + p[7] * ub + p[8] * uc : 0.0;
       double Dn[3] = {0, 0, 0}, da = 0, db = 0, dg = 0;
       if (FK == FVM_FLUX_D_NODAL) {
         Dn[0] = s_D[id[0]];
@@ -687,7 +707,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
 #pragma unroll
       for (int v = 0; v < 3; ++v) {
         double ub = p[v] * abar + p[3 + v] * bbar;
-        if (FK == FVM_FLUX_D_POLY_U || FK == FVM_FLUX_LINEAR) ub = ub + p[6 + v] * gbar;  // gamma enters q
+        if (d_of_u(FK) || FK == FVM_FLUX_LINEAR) ub = ub + p[6 + v] * gbar;  // gamma enters q
         cr[r][v] = ub;
         if (FK == FVM_FLUX_D_NODAL) cr[r][3 + v] = lbar[v] * Dn[v];
       }
@@ -847,6 +867,7 @@ int launch(const fvm_ctx* c, const KArgs& a, hipStream_t s) {
     case FVM_FLUX_D_POLY_U: return launch_fk<FVM_FLUX_D_POLY_U, MODE>(a, c->m->tile_tris, s);
     case FVM_FLUX_D_NODAL: return launch_fk<FVM_FLUX_D_NODAL, MODE>(a, c->m->tile_tris, s);
     case FVM_FLUX_LINEAR: return launch_fk<FVM_FLUX_LINEAR, MODE>(a, c->m->tile_tris, s);
+    case FVM_FLUX_D_POWER: return launch_fk<FVM_FLUX_D_POWER, MODE>(a, c->m->tile_tris, s);
     default: return launch_fk<FVM_FLUX_D_EDGE, MODE>(a, c->m->tile_tris, s);
   }
 }
@@ -949,8 +970,10 @@ void free_ctx(fvm_ctx* c) {
 
 int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
   const int64_t N = m->N, T = m->T, B = m->B;
-  FVM_CHECK(ph->flux_kind >= FVM_FLUX_D_POLY_U && ph->flux_kind <= FVM_FLUX_LINEAR,
+  FVM_CHECK(ph->flux_kind >= FVM_FLUX_D_POLY_U && ph->flux_kind <= FVM_FLUX_D_POWER,
             "fvm_ctx_create: unknown flux_kind");
+  FVM_CHECK(ph->flux_kind != FVM_FLUX_D_POWER || ph->dpoly[2] > 0,
+            "fvm_ctx_create: FVM_FLUX_D_POWER needs a positive exponent dpoly[2]");
   FVM_CHECK(ph->node_kind, "fvm_ctx_create: node_kind is required");
   FVM_CHECK(ph->node_fidx, "fvm_ctx_create: node_fidx is required");
   FVM_CHECK(B == 0 || ph->bedge_neumann_fidx, "fvm_ctx_create: bedge_neumann_fidx is required");
@@ -1135,6 +1158,7 @@ int fvm_rhs_timed(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_str
     case FVM_FLUX_D_POLY_U: launch_tiles<FVM_FLUX_D_POLY_U, M_RHS>(a, tt, st); break;
     case FVM_FLUX_D_NODAL: launch_tiles<FVM_FLUX_D_NODAL, M_RHS>(a, tt, st); break;
     case FVM_FLUX_LINEAR: launch_tiles<FVM_FLUX_LINEAR, M_RHS>(a, tt, st); break;
+    case FVM_FLUX_D_POWER: launch_tiles<FVM_FLUX_D_POWER, M_RHS>(a, tt, st); break;
     default: launch_tiles<FVM_FLUX_D_EDGE, M_RHS>(a, tt, st);
   }
   FVM_HIP(hipEventRecord(ev[1], st));
@@ -1142,6 +1166,7 @@ int fvm_rhs_timed(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_str
     case FVM_FLUX_D_POLY_U: launch_fix<FVM_FLUX_D_POLY_U, M_RHS>(a, st); break;
     case FVM_FLUX_D_NODAL: launch_fix<FVM_FLUX_D_NODAL, M_RHS>(a, st); break;
     case FVM_FLUX_LINEAR: launch_fix<FVM_FLUX_LINEAR, M_RHS>(a, st); break;
+    case FVM_FLUX_D_POWER: launch_fix<FVM_FLUX_D_POWER, M_RHS>(a, st); break;
     default: launch_fix<FVM_FLUX_D_EDGE, M_RHS>(a, st);
   }
   FVM_HIP(hipEventRecord(ev[2], st));
@@ -1183,6 +1208,7 @@ int fvm_vjp(fvm_ctx* c, const double* d_u, const double* d_w, double* d_ubar, do
     case FVM_FLUX_D_POLY_U: return launch_vjp_fk<FVM_FLUX_D_POLY_U>(a, c->m->tile_tris, st);
     case FVM_FLUX_D_NODAL: return launch_vjp_fk<FVM_FLUX_D_NODAL>(a, c->m->tile_tris, st);
     case FVM_FLUX_LINEAR: return launch_vjp_fk<FVM_FLUX_LINEAR>(a, c->m->tile_tris, st);
+    case FVM_FLUX_D_POWER: return launch_vjp_fk<FVM_FLUX_D_POWER>(a, c->m->tile_tris, st);
     default: return launch_vjp_fk<FVM_FLUX_D_EDGE>(a, c->m->tile_tris, st);
   }
 }

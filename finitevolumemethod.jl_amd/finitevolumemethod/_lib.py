@@ -122,7 +122,7 @@ class SystemDesc(ctypes.Structure):
                 ("node_g", _dp), ("bedge_g", _dp)]
 
 
-FLUX_D_POLY_U, FLUX_D_NODAL, FLUX_D_EDGE, FLUX_LINEAR = 0, 1, 2, 3
+FLUX_D_POLY_U, FLUX_D_NODAL, FLUX_D_EDGE, FLUX_LINEAR, FLUX_D_POWER = 0, 1, 2, 3, 4
 TEMPLATE_POISSON, TEMPLATE_MEAN_EXIT_TIME, TEMPLATE_DIFFUSION = 0, 1, 2
 SOLVE_CG, SOLVE_CG_MIXED, SOLVE_BICGSTAB, SOLVE_CG_RELIABLE = 0, 1, 2, 3
 NODE_NONE, NODE_DIRICHLET, NODE_DUDT = 0, 1, 2

@@ -8,7 +8,8 @@ arguments (u, t, and alpha/beta/gamma) and NumPy arrays for the coordinates.  Th
 trace must land on one of the forms the kernels implement:
 
   diffusion D(x, y, t, u, p): polynomial of degree <= 3 in u with constant
-      coefficients (FVM_FLUX_D_POLY_U), or any function of (x, y) alone
+      coefficients (FVM_FLUX_D_POLY_U), d0 + c |u|^p (FVM_FLUX_D_POWER, recognised
+      by evaluation), or any function of (x, y) alone
       (FVM_FLUX_D_EDGE: evaluated at every CV-edge midpoint and boundary
       half-edge point), or a `NodalDiffusion` field (FVM_FLUX_D_NODAL);
   flux q(x, y, t, a, b, g, p) -> (qx, qy): -D(x, y) (a, b) (FVM_FLUX_D_EDGE), or
@@ -303,13 +304,53 @@ def classify_source(fn, x, y):
     return np.broadcast_to(np.asarray(c0, np.float64), (n,)).copy(), poly, st
 
 
+def _power_law(fn):
+    """D(x, y, t, u) = d0 + c |u|^p with p > 0, recognised by evaluation (the symbolic
+    stand-in has no abs or non-integer powers): D must not depend on (x, y, t), and
+    d0 = D(0), c = D(1) - d0, p = log2((D(2) - d0) / c) must reproduce D at every
+    probe, negative u included, to 1e-13.  Returns (d0, c, p) or None."""
+    us = (0.0, 1.0, 2.0, 0.5, 3.7, 0.013, -1.3, -2.0, -0.25)
+    pts = ((0.37, 0.61, 0.0), (-1.9, 2.3, 0.0), (0.37, 0.61, 1.7))
+
+    def fits(p):
+        return all(abs(v - (d0 + c * abs(u) ** p)) <= 1e-13 * max(abs(v), abs(d0), abs(c))
+                   for u, v in zip(us, vals))
+
+    try:
+        vals = []
+        for u in us:
+            with np.errstate(all="ignore"):
+                row = [fn(np.float64(x), np.float64(y), np.float64(t), np.float64(u)) for x, y, t in pts]
+            row = [complex(v) for v in row]
+            if any(v.imag != 0 or not np.isfinite(v.real) for v in row) or len({v.real for v in row}) != 1:
+                return None
+            vals.append(row[0].real)
+        d0, c = vals[0], vals[1] - vals[0]
+        if c == 0.0 or not (vals[2] - d0) / c > 0:
+            return None
+        p = float(np.log2((vals[2] - d0) / c))
+        if not p > 0 or not fits(p):
+            return None
+        pr = round(p, 10)  # the exponent as written (log2 leaves an ulp or two)
+        return d0, c, pr if fits(pr) else p
+    except Exception:  # noqa: BLE001 - not this form
+        return None
+
+
 def classify_diffusion(fn):
     """Trace D(x, y, t, u, p) (already bound to p).  Returns ('poly', [d0..d3]) for
-    a constant-coefficient cubic in u, or ('xy', None) when D depends on (x, y)
-    only (the caller evaluates it at the CV-edge points)."""
+    a constant-coefficient cubic in u, ('power', (d0, c, p)) for d0 + c |u|^p, or
+    ('xy', None) when D depends on (x, y) only (the caller evaluates it at the
+    CV-edge points)."""
     probe_x = np.array([0.1234, 0.7421, 0.3317, 0.9012])
     probe_y = np.array([0.5531, 0.0713, 0.8822, 0.2417])
-    r = _as_sym(_trace(fn, 2, probe_x, probe_y, [None, None], [0, 1]), 2)  # vars: t, u
+    try:
+        r = _as_sym(_trace(fn, 2, probe_x, probe_y, [None, None], [0, 1]), 2)  # vars: t, u
+    except UnsupportedClosure:
+        pw = _power_law(fn)
+        if pw is None:
+            raise
+        return "power", pw
     poly = [0.0, 0.0, 0.0, 0.0]
     depends_xy = False
     for e in r.nonzero_exponents():

@@ -269,6 +269,8 @@ def physics_of(prob):
             kind, poly = forms.classify_diffusion(Dfun)
         if kind == "poly":
             ph.update(flux_kind=_lib.FLUX_D_POLY_U, dpoly=poly)
+        elif kind == "power":
+            ph.update(flux_kind=_lib.FLUX_D_POWER, dpoly=[poly[0], poly[1], poly[2], 0.0])
         elif kind == "linear":
             # a1..a6 per CV edge [T][3][6] and per boundary half-edge point [2B][6]
             P = mesh.props_array

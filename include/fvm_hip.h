@@ -107,16 +107,18 @@ enum {
                             exp(logD) (pl_interpolate of a nodal field) */
   FVM_FLUX_D_EDGE = 2,   /* q = -D_s (alpha, beta) with D_s given at every CV-edge midpoint and
                             boundary half-edge point (D(x, y) independent of t and u) */
-  FVM_FLUX_LINEAR = 3    /* q = (a1 alpha + a2 beta + a3 gamma, a4 alpha + a5 beta + a6 gamma), the
+  FVM_FLUX_LINEAR = 3,   /* q = (a1 alpha + a2 beta + a3 gamma, a4 alpha + a5 beta + a6 gamma), the
                             a's given at every CV-edge midpoint and boundary half-edge point
                             (independent of t and u): advection-diffusion, anisotropic
                             diffusion, any flux linear in u and its gradient */
+  FVM_FLUX_D_POWER = 4   /* q = -D(u_s)(alpha, beta), D = dpoly[0] + dpoly[1] |u_s|^dpoly[2],
+                            dpoly[2] > 0 (porous-medium diffusivity) */
 };
 enum { FVM_NODE_NONE = 0, FVM_NODE_DIRICHLET = 1, FVM_NODE_DUDT = 2 };
 
 typedef struct fvm_physics_desc {
   int32_t flux_kind;
-  double dpoly[4];              /* FVM_FLUX_D_POLY_U */
+  double dpoly[4];              /* FVM_FLUX_D_POLY_U; FVM_FLUX_D_POWER: d0, c, p */
   const double* d_edge;         /* [3*n_triangles] input triangle order, CV edge (i,j),(j,k),(k,i);
                                    FVM_FLUX_LINEAR: [18*n_triangles], a1..a6 of each CV edge */
   const double* d_bedge;        /* [2*n_boundary_edges] at the half-edge points m_i, m_j;

@@ -1,0 +1,104 @@
+"""FVM_FLUX_D_POWER: D(u) = d0 + c |u|^p, the porous-medium diffusivity of
+porous_medium_equation.jl:164 (D |u|^(m - 1), m = 3.4).  RHS against the oracle
+calling the closure (1e-12 bar of the RHS tests, u of both signs), JVP against
+central differences, VJP by the adjoint identity, compute_flux against the
+oracle, and the Barenblatt solution of porous_medium_equation.jl:88-99."""
+import numpy as np
+import pytest
+
+import finitevolumemethod as FVM
+from finitevolumemethod import _lib
+from problems import oracle_for
+from test_gpu_queries import check_flux
+from test_gpu_rhs import assert_parity
+
+pytestmark = pytest.mark.gpu
+DC, M_EXP = 0.581, 3.4
+
+
+def power_problem(nx=30, ny=26, seed=6, d0=0.0):
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, nx, ny, jitter=0.25, seed=seed, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(
+        mesh, (lambda x, y, t, u, p: 0.2 + 0.0 * u, lambda x, y, t, u, p: 0.0 * u,
+               lambda x, y, t, u, p: -0.1 * u, lambda x, y, t, u, p: 0.5 + 0.0 * u),
+        (FVM.Neumann, FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet))
+    D = lambda x, y, t, u, p: d0 + p["D"] * abs(u) ** (p["m"] - 1)  # noqa: E731
+    params = dict(D=DC, m=M_EXP)
+    u = np.random.default_rng(seed).uniform(-0.5, 1.0, tri.num_points())
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=D, diffusion_parameters=params,
+                          source_function=lambda x, y, t, u, lam: lam * u, source_parameters=0.2,
+                          initial_condition=u, final_time=1.0)
+
+    def flux(x, y, t, a, b, g, T):
+        Dv = d0 + DC * np.abs(a * x + b * y + g) ** (M_EXP - 1)
+        return -Dv * a, -Dv * b
+
+    orc = oracle_for(prob, dict(flux=flux, source=lambda x, y, t, u, n: 0.2 * u,
+                                cond=lambda f, x, y, t, u: prob.conditions.functions[f](x, y, t, u)))
+    return prob, orc, u, flux
+
+
+@pytest.mark.parametrize("d0", [0.0, 0.05])
+def test_rhs_matches_oracle(gpu, d0):
+    prob, orc, u, _ = power_problem(d0=d0)
+    ctx = prob.device_context()
+    assert ctx.flux_kind == _lib.FLUX_D_POWER
+    for t in (0.0, 0.3):
+        du = FVM.fvm_eqs(np.zeros_like(u), u, prob, t)
+        assert_parity(du, orc.rhs(u, t), orc.scale(u, t))
+
+
+def test_jvp_and_vjp(gpu):
+    prob, orc, u, _ = power_problem(d0=0.05)
+    u = u + 0.7  # keep u_s away from 0, where |u|^p has no third derivative
+    ctx = prob.device_context()
+    n = len(u)
+    rng = np.random.default_rng(4)
+    v, w = rng.standard_normal(n), rng.standard_normal(n)
+    jv, jtw = _lib.DeviceArray(n), _lib.DeviceArray(n)
+    ctx.jvp(_lib.DeviceArray.from_host(u), _lib.DeviceArray.from_host(v), jv, 0.0)
+    ctx.vjp(_lib.DeviceArray.from_host(u), _lib.DeviceArray.from_host(w), jtw, None, 0.0)
+    Jv, JTw = jv.to_host(), jtw.to_host()
+    h = 1e-6
+    fd = (FVM.fvm_eqs(np.zeros(n), u + h * v, prob, 0.0) - FVM.fvm_eqs(np.zeros(n), u - h * v, prob, 0.0)) / (2 * h)
+    scale = orc.scale(u, 0.0) * (1 + np.abs(v))
+    assert np.all(np.abs(Jv - fd) <= 1e-6 * scale), np.max(np.abs(Jv - fd) / scale)
+    assert abs(w @ Jv - JTw @ v) <= 1e-11 * np.sum(np.abs(w * Jv))
+
+
+def test_compute_flux_matches_oracle(gpu):
+    prob, orc, u, flux = power_problem(nx=13, ny=11, d0=0.05)
+    check_flux(prob, u, flux)
+
+
+def test_barenblatt_solution(gpu):
+    """u_t = div(D u^(m-1) grad u), m = 3.4, M = 2.3, D = 0.581 (the tutorial's second
+    example without its source): from the Barenblatt solution at t = 1 to t = 4 on
+    a 125 x 125 grid of [-L, L]^2 sized as the tutorial does for t = 4; relative L2
+    error at the saved times below 3 % (the free boundary, where the solution is
+    only Hoelder continuous, bounds the accuracy of any linear-element scheme)."""
+    m, M, D = M_EXP, 2.3, DC
+    RmM = 4 * m / (m - 1) * (M / (4 * np.pi)) ** ((m - 1) / m)
+
+    def exact(x, y, t):
+        r2 = x * x + y * y
+        inner = (M / (4 * np.pi)) ** ((m - 1) / m) - (m - 1) / (4 * m) * r2 * (D * t) ** (-1 / m)
+        return np.where(r2 < RmM * (D * t) ** (1 / m), (D * t) ** (-1 / m) * np.maximum(inner, 0.0) ** (1 / (m - 1)), 0.0)
+
+    L = np.sqrt(RmM) * (D * 4.0) ** (1 / (2 * m)) * 1.1
+    tri = FVM.triangulate_rectangle(-L, L, -L, L, 125, 125, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: p[0] * abs(u) ** (p[1] - 1),
+                          diffusion_parameters=(D, m), initial_condition=exact(tri.x, tri.y, 1.0),
+                          initial_time=1.0, final_time=4.0)
+    assert prob.device_context().flux_kind == _lib.FLUX_D_POWER
+    sol = FVM.solve(prob, FVM.Tsit5(), saveat=[1.0, 2.0, 4.0], abstol=1e-8, reltol=1e-6)
+    assert sol.retcode == "Success"
+    V = mesh.cv_volumes
+    for t, u in zip(sol.t[1:], sol.u[1:]):
+        e = exact(tri.x, tri.y, t)
+        assert np.linalg.norm(u - e) <= 0.03 * np.linalg.norm(e), (t, np.linalg.norm(u - e) / np.linalg.norm(e))
+        # mass M is conserved (the support stays inside the domain)
+        assert abs(V @ u - V @ sol.u[0]) <= 1e-6 * (V @ sol.u[0])

@@ -219,6 +219,20 @@ def test_closure_tracing_forms():
     np.testing.assert_array_equal(D(x, x), np.full(3, 80e-6))
 
 
+def test_power_law_diffusion_is_recognised():
+    """porous_medium_equation.jl:164: D = p.D |u|^(p.m - 1) with m = 3.4 does not
+    trace symbolically; it is recognised by evaluation as d0 + c |u|^p.  A
+    u^2.4 without abs (complex for u < 0), exp(u) or an (x, y) factor are not."""
+    D, m = 0.581, 3.4
+    assert forms.classify_diffusion(lambda x, y, t, u: D * abs(u) ** (m - 1)) == ("power", (0.0, D, 2.4))
+    assert forms.classify_diffusion(lambda x, y, t, u: 0.1 + 2 * np.abs(u) ** 0.5) == ("power", (0.1, 2.0, 0.5))
+    assert forms.classify_diffusion(lambda x, y, t, u: 2.53 * u ** (2 - 1)) == ("poly", [0.0, 2.53, 0.0, 0.0])
+    for bad in (lambda x, y, t, u: u ** 2.4, lambda x, y, t, u: np.exp(u),
+                lambda x, y, t, u: (1 + x) * np.abs(u) ** 1.5, lambda x, y, t, u: (1 + t) * abs(u) ** 1.5):
+        with pytest.raises(forms.UnsupportedClosure):
+            forms.classify_diffusion(bad)
+
+
 def test_linear_flux_forms_trace():
     """piecewise_linear_and_natural_neighbour_interpolation_for_an_advection_diffusion_equation.jl:80-87:
     q = (nu u - D alpha, -D beta) with u = alpha x + beta y + gamma is linear in (alpha,
