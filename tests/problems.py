@@ -314,3 +314,32 @@ def annulus_problem(variant="tutorial", nr=14, final_time=2.0):
         flux=lambda x, y, t, a, b, g, T: (-a, -b),
         cond=lambda f, x, y, t, u: prob.conditions.functions[f](x, y, t, u)))
     return prob, orc, u0
+
+
+def wedge_triangulation(alpha=np.pi / 4, nr=40):
+    """The sector 0 <= theta <= alpha, r <= 1: rings of points (about as many per
+    ring as its arc length allows at spacing 1/nr), Delaunay; boundary segments
+    0: theta = 0, 1: the arc r = 1, 2: theta = alpha (the points on both straight
+    edges are exactly collinear)."""
+    c, s = np.cos(alpha), np.sin(alpha)
+    if abs(alpha - np.pi / 4) < 1e-15:
+        s = c  # y = x exactly on the upper edge
+    pts = [(0.0, 0.0)]
+    for k in range(1, nr + 1):
+        r = k / nr
+        m = max(1, int(np.ceil(alpha * r * nr)))
+        for j in range(m + 1):
+            th = alpha * j / m
+            if j == 0:
+                pts.append((r, 0.0))
+            elif j == m:
+                pts.append((r * c, r * s))
+            else:
+                pts.append((r * np.cos(th), r * np.sin(th)))
+    P = np.array(pts)
+    full = FVM.triangulate(P[:, 0], P[:, 1])
+    be = full.boundary_edges
+    mx, my = P[be, 0].mean(1), P[be, 1].mean(1)
+    th = np.arctan2(my, mx)
+    seg = np.where(np.abs(my) < 1e-12, 0, np.where(np.abs(th - alpha) < 1e-9, 2, 1)).astype(np.int32)
+    return FVM.Triangulation(P[:, 0], P[:, 1], full.triangles, be, seg)

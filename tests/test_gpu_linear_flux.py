@@ -91,8 +91,9 @@ def test_advection_diffusion_closed_form(gpu):
     (1/(4 pi D t)) exp(-((x - nu t)^2 + y^2) / (4 D t)) (the tutorial's
     exact_solution; its own run starts from a near-delta on a graded mesh that needs
     DelaunayTriangulation's refinement) to t = 11 on a 161 x 129 jittered mesh of
-    [-3, 5] x [-3, 3.4], homogeneous Dirichlet; the tutorial's measure: the mean over
-    nodes of 100 |u - exact| / max u at every saved time <= 0.16."""
+    [-3, 5] x [-3, 3.4], homogeneous Dirichlet; the tutorial's measure, the mean over
+    nodes of 100 |u - exact| / max u, at every saved time: <= 0.03 (measured 0.005 to
+    0.009; the tutorial asks <= 0.16 of its own run)."""
     Dc, nu = 0.02, 0.05
 
     def exact(x, y, t):
@@ -109,7 +110,8 @@ def test_advection_diffusion_closed_form(gpu):
     assert sol.retcode == "Success"
     for t, u in zip(sol.t[1:], sol.u[1:]):
         err = np.mean(100 * np.abs(u - exact(tri.x, tri.y, t)) / np.max(u))
-        assert err <= 0.16, (t, err)
+        print("advection", t, err)
+        assert err <= 0.03, (t, err)
     # mass is carried, not lost: the integral drifts only by the (tiny) boundary outflow
     V = mesh.cv_volumes
     assert abs(V @ sol.u[-1] - V @ sol.u[0]) <= 1e-3 * (V @ sol.u[0])

@@ -76,8 +76,9 @@ def test_barenblatt_solution(gpu):
     """u_t = div(D u^(m-1) grad u), m = 3.4, M = 2.3, D = 0.581 (the tutorial's second
     example without its source): from the Barenblatt solution at t = 1 to t = 4 on
     a 125 x 125 grid of [-L, L]^2 sized as the tutorial does for t = 4; relative L2
-    error at the saved times below 3 % (the free boundary, where the solution is
-    only Hoelder continuous, bounds the accuracy of any linear-element scheme)."""
+    error at the saved times below 2 % (measured 1.0 %, 1.1 %: the free boundary,
+    where the solution is only Hoelder continuous, bounds the accuracy of any
+    linear-element scheme)."""
     m, M, D = M_EXP, 2.3, DC
     RmM = 4 * m / (m - 1) * (M / (4 * np.pi)) ** ((m - 1) / m)
 
@@ -99,6 +100,7 @@ def test_barenblatt_solution(gpu):
     V = mesh.cv_volumes
     for t, u in zip(sol.t[1:], sol.u[1:]):
         e = exact(tri.x, tri.y, t)
-        assert np.linalg.norm(u - e) <= 0.03 * np.linalg.norm(e), (t, np.linalg.norm(u - e) / np.linalg.norm(e))
+        print("barenblatt", t, np.linalg.norm(u - e) / np.linalg.norm(e))
+        assert np.linalg.norm(u - e) <= 0.02 * np.linalg.norm(e), (t, np.linalg.norm(u - e) / np.linalg.norm(e))
         # mass M is conserved (the support stays inside the domain)
         assert abs(V @ u - V @ sol.u[0]) <= 1e-6 * (V @ sol.u[0])

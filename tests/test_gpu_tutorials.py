@@ -1,0 +1,86 @@
+"""The reference tutorials whose closures trace onto the device forms, against the
+closed-form solutions the tutorials themselves compare with (their meshes come from
+DelaunayTriangulation's refinement, absent here, so each test builds a mesh of the
+same domain; the tolerances are discretisation-level, stated per test).
+
+  diffusion_equation_in_a_wedge_with_mixed_boundary_conditions.jl: a Fourier-Bessel
+      series (for f = 1 - r only its n = 0 terms are nonzero);
+  helmholtz_equation_with_inhomogeneous_boundary_conditions.jl: steady, closed form;
+  reaction_diffusion_equation_with_a_time_dependent_dirichlet_boundary_condition_on_a_disk.jl:
+      exp(t) sqrt(I0(sqrt(2) r)), D = u, R = u (1 - u), du/dt = u on the boundary."""
+import numpy as np
+import pytest
+import scipy.integrate as si
+import scipy.special as sp
+
+import finitevolumemethod as FVM
+from problems import disk_triangulation, wedge_triangulation
+
+pytestmark = pytest.mark.gpu
+
+
+def test_wedge_mixed_conditions_series(gpu):
+    """u_t = lap u on 0 < theta < pi/4, r < 1; du/dn = 0 on both straight edges,
+    u = 0 on the arc, u0 = 1 - r: u = sum_m A_m exp(-z_m^2 t) J0(z_m r) with z_m the
+    zeros of J0 and A_m = 2 / J1(z_m)^2 int_0^1 (1 - r) J0(z_m r) r dr (the
+    tutorial's exact_solution, :107-156).  Relative L2 error below 1e-3 at t = 0.05
+    and 0.1 on a 40-ring mesh (measured 1.8e-4, 2.6e-4)."""
+    tri = wedge_triangulation(np.pi / 4, 40)
+    mesh = FVM.FVMGeometry(tri)
+    zero = lambda x, y, t, u, p: 0.0 * u  # noqa: E731
+    BCs = FVM.BoundaryConditions(mesh, (zero, zero, zero), (FVM.Neumann, FVM.Dirichlet, FVM.Neumann))
+    r = np.hypot(tri.x, tri.y)
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 1.0 + 0.0 * u,
+                          initial_condition=1.0 - r, final_time=0.1)
+    sol = FVM.solve(prob, FVM.Tsit5(), saveat=[0.0, 0.05, 0.1], abstol=1e-9, reltol=1e-7)
+    assert sol.retcode == "Success"
+    z = sp.jn_zeros(0, 60)
+    A = np.array([2 / sp.j1(zm) ** 2 * si.quad(lambda s: (1 - s) * sp.j0(zm * s) * s, 0, 1, limit=200)[0]
+                  for zm in z])
+    for t, u in zip(sol.t[1:], sol.u[1:]):
+        e = np.sum(A[:, None] * np.exp(-z[:, None] ** 2 * t) * sp.j0(z[:, None] * r[None, :]), axis=0)
+        print("wedge", t, np.linalg.norm(u - e) / np.linalg.norm(e))
+        assert np.linalg.norm(u - e) <= 1e-3 * np.linalg.norm(e), (t, np.linalg.norm(u - e) / np.linalg.norm(e))
+
+
+def test_helmholtz_inhomogeneous_neumann(gpu):
+    """lap u + u = 0 on [-1, 1]^2, du/dn = 1 (q . n = -1), solved as a
+    SteadyFVMProblem (D = 1, S = u): u = -(cos(x+1) + cos(1-x) + cos(y+1) + cos(1-y))
+    / sin 2 (the tutorial's exact_solution, :80-82).  80 x 80 grid, relative L2
+    error below 2e-4 (measured 5.8e-5)."""
+    tri = FVM.triangulate_rectangle(-1.0, 1.0, -1.0, 1.0, 80, 80, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: -1.0 + 0.0 * u, FVM.Neumann)
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 1.0 + 0.0 * u,
+                          source_function=lambda x, y, t, u, p: u,
+                          initial_condition=np.zeros(tri.num_points()), final_time=np.inf)
+    sol = FVM.solve(FVM.SteadyFVMProblem(prob))
+    assert sol.retcode == "Success", sol.stats
+    x, y = tri.x, tri.y
+    e = -(np.cos(x + 1) + np.cos(1 - x) + np.cos(y + 1) + np.cos(1 - y)) / np.sin(2)
+    u = sol.u[-1]
+    print("helmholtz", np.linalg.norm(u - e) / np.linalg.norm(e), sol.stats)
+    assert np.linalg.norm(u - e) <= 2e-4 * np.linalg.norm(e), np.linalg.norm(u - e) / np.linalg.norm(e)
+
+
+def test_disk_time_dependent_boundary(gpu):
+    """u_t = div(u grad u) + u (1 - u) on the unit disk, du/dt = u on the boundary
+    circle, u0 = sqrt(I0(sqrt(2) r)): u = exp(t) sqrt(I0(sqrt(2) r)) (the tutorial's
+    exact_solution, :82-85).  30-ring mesh to t = 0.1; relative error below 2e-4
+    (measured 6.2e-5)."""
+    tri = disk_triangulation(nr=30)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: u, FVM.Dudt)
+    r = np.hypot(tri.x, tri.y)
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: u,
+                          source_function=lambda x, y, t, u, p: u * (1 - u),
+                          initial_condition=np.sqrt(sp.i0(np.sqrt(2) * r)), final_time=0.1)
+    sol = FVM.solve(prob, FVM.Tsit5(), saveat=[0.0, 0.05, 0.1], abstol=1e-10, reltol=1e-8)
+    assert sol.retcode == "Success"
+    bn = tri.boundary_nodes()
+    for t, u in zip(sol.t[1:], sol.u[1:]):
+        e = np.exp(t) * np.sqrt(sp.i0(np.sqrt(2) * r))
+        print("disk", t, np.max(np.abs(u - e) / e))
+        assert np.max(np.abs(u - e) / e) <= 2e-4, (t, np.max(np.abs(u - e) / e))
+        # the boundary follows du/dt = u exactly up to the integrator's tolerance
+        np.testing.assert_allclose(u[bn], np.exp(t) * sol.u[0][bn], rtol=1e-6)
