@@ -224,6 +224,12 @@ __device__ __forceinline__ int32_t find_directed_edge(const int32_t* __restrict_
   return -1;
 }
 
+
+// Classical Gram-Schmidt applied twice to w against xs[0..k) (k <= 64) plus |w|^2,
+// intermediates on the device, one host synchronisation: h_host[0..k), h_host[k..2k)
+// the two passes' coefficients, h_host[2k] = |w|^2 (fvm_vec.hip).
+int cgs2_norm(int64_t n, int k, const double* const* xs, double* w, double* h_host, hipStream_t s);
+
 }  // namespace fvm
 
 // Device-resident mesh: geometry SoA in tile (sorted) triangle order plus the

@@ -52,6 +52,13 @@ struct Op {
       default: return fvm_template_apply((const fvm_template*)h, u, du, s);
     }
   }
+  int jvp(const double* u, const double* v, double* jv, double t, fvm_stream s) const {
+    switch (kind) {
+      case FVM_OP_PROBLEM: return fvm_jvp((fvm_ctx*)h, u, v, jv, t, s);
+      case FVM_OP_SYSTEM: return fvm_system_jvp((fvm_system*)h, u, v, jv, t, s);
+      default: return fvm_template_spmv((const fvm_template*)h, 0, v, jv, s);
+    }
+  }
   int apply_dirichlet(double* u, double t, fvm_stream s) const {
     if (!dirichlet) return FVM_OK;
     switch (kind) {
@@ -251,6 +258,24 @@ int fixed_step(const fvm_ode_opts* o, const Op& op, double* d_u, fvm_ode_stats* 
 }  // namespace
 
 extern "C" {
+
+int fvm_arnoldi_step(int32_t op_kind, void* op, int64_t n, const double* d_u, double t, double a, double b,
+                     int32_t j, double* const* d_V, double* d_w, double* h, fvm_stream s) {
+  FVM_CHECK(op && d_u && d_V && d_w && h, "fvm_arnoldi_step: null argument");
+  FVM_CHECK(op_kind >= FVM_OP_PROBLEM && op_kind <= FVM_OP_TEMPLATE, "fvm_arnoldi_step: unknown operator kind");
+  FVM_CHECK(j >= 0 && j < 64 && n > 0, "fvm_arnoldi_step: 0 <= j < 64 and n > 0 required");
+  for (int i = 0; i <= j; ++i) FVM_CHECK(d_V[i] && d_V[i] != d_w, "fvm_arnoldi_step: null or aliased basis vector");
+  int64_t nf = 0;
+  Op o{op_kind, op, false, &nf};
+  int rc = o.jvp(d_u, d_V[j], d_w, t, s);
+  if (rc) return rc;
+  if (!(a == 0.0 && b == 1.0)) {  // A = J takes the product as it is
+    const double c[2] = {a, b};
+    const double* x[2] = {d_V[j], d_w};
+    if ((rc = lc(n, d_w, 2, c, x, s))) return rc;
+  }
+  return fvm::cgs2_norm(n, j + 1, d_V, d_w, h, (hipStream_t)s);
+}
 
 int fvm_ode_solve(const fvm_ode_opts* o, double* d_u, fvm_ode_stats* st, fvm_stream s) {
   FVM_CHECK(o && d_u && st && o->op, "fvm_ode_solve: null argument");

@@ -147,7 +147,21 @@ __global__ __launch_bounds__(256) void maxpy_kernel(int64_t n, double* __restric
   }
 }
 
+// y -= sum_j c_j x_j with the coefficients c in device memory (the Gram-Schmidt
+// step of an Arnoldi iteration whose dot products never left the device); the same
+// arithmetic, in the same order, as maxpy_kernel given -c
+__global__ __launch_bounds__(256) void maxpy_neg_dev_kernel(int64_t n, double* __restrict__ y, AxpyTerms t,
+                                                            const double* __restrict__ c) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = y[i];
+    for (int j = 0; j < t.k; ++j) v = v + (-c[j]) * t.x[j][i];
+    y[i] = v;
+  }
+}
+
 std::mutex g_mu;
+double* g_ascratch = nullptr;  // Arnoldi step results: 2 kMaxDots coefficients + |w|^2
 double* g_scratch = nullptr;   // kRedBlocks partials + 1 result
 double* g_mscratch = nullptr;  // kMaxDots * kRedBlocks partials + kMaxDots results
 
@@ -180,6 +194,53 @@ int reduce(int kind, int64_t n, const double* x, const double* y, const double* 
 }
 
 }  // namespace
+
+namespace fvm {
+
+// Classical Gram-Schmidt applied twice to w against xs[0..k) and |w|^2, with every
+// intermediate on the device and one host synchronisation at the end:
+// h_host[0..k) and h_host[k..2k) the two passes' coefficients, h_host[2k] = |w|^2.
+int cgs2_norm(int64_t n, int k, const double* const* xs, double* w, double* h_host, hipStream_t s) {
+  if (k < 1 || k > kMaxDots) return FVM_ERR_ARG;
+  double *buf, *red;
+  int rc = mscratch(&buf);
+  if (rc || (rc = scratch(&red))) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ascratch) FVM_HIP(hipMalloc((void**)&g_ascratch, sizeof(double) * (2 * kMaxDots + 8)));
+  }
+  AxpyTerms t;
+  t.k = k;
+  for (int j = 0; j < k; ++j) {
+    t.c[j] = 0.0;
+    t.x[j] = xs[j];
+  }
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  for (int pass = 0; pass < 2; ++pass) {
+    double* res = g_ascratch + pass * kMaxDots;
+    for (int first = 0; first < k; first += kDotChunk) {
+      DotChunk c;
+      c.first = first;
+      c.k = k - first < kDotChunk ? k - first : kDotChunk;
+      for (int j = 0; j < kDotChunk; ++j) c.x[j] = j < c.k ? xs[first + j] : xs[first];
+      hipLaunchKernelGGL(mdot_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, c, w, buf);
+    }
+    hipLaunchKernelGGL(mdot_stage2, dim3((unsigned)k), dim3(kRedThreads), 0, s, buf, res);
+    hipLaunchKernelGGL(maxpy_neg_dev_kernel, dim3((unsigned)blocks), dim3(256), 0, s, n, w, t, res);
+  }
+  hipLaunchKernelGGL(reduce_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, s, 0, n, w, w, nullptr, nullptr,
+                     0.0, 0.0, red);
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kRedThreads), 0, s, red, g_ascratch + 2 * kMaxDots);
+  FVM_HIP(hipGetLastError());
+  FVM_HIP(hipMemcpyAsync(h_host, g_ascratch, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+  FVM_HIP(hipMemcpyAsync(h_host + k, g_ascratch + kMaxDots, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+  FVM_HIP(hipMemcpyAsync(h_host + 2 * k, g_ascratch + 2 * kMaxDots, sizeof(double), hipMemcpyDeviceToHost, s));
+  FVM_HIP(hipStreamSynchronize(s));
+  return FVM_OK;
+}
+
+}  // namespace fvm
 
 extern "C" {
 

@@ -188,7 +188,7 @@ def integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt
             raise TypeError("TRBDF2 needs the Jacobian-vector product of the right-hand side")
         return _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, saveat=saveat, abstol=abstol,
                                  reltol=reltol, dt=dt, maxiters=maxiters, save_start=save_start,
-                                 step_callback=step_callback, user_callback=user_callback)
+                                 step_callback=step_callback, user_callback=user_callback, op=op)
     n = len(u0)
     u = _lib.DeviceArray.from_host(np.asarray(u0, np.float64))
     sol = ODESolution()
@@ -323,8 +323,8 @@ _TRB_K = (3 * _TRB_G ** 2 - 4 * _TRB_G + 2) / (6 * (2.0 - _TRB_G))
 
 
 def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, maxiters,
-                      save_start, step_callback, user_callback):
-    from .linsolve import gmres_device
+                      save_start, step_callback, user_callback, op=None):
+    from .linsolve import gmres_device, library_arnoldi
 
     n = len(u0)
     u = _lib.DeviceArray.from_host(np.asarray(u0, np.float64))
@@ -348,6 +348,17 @@ def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, 
             _lib.lincomb(out, (1.0, -_TRB_D * h), (v, out))
         return apply
 
+    def W_arnoldi(ubase, tbase, h):
+        """The Arnoldi step for W = I - d h J in the library (op given), else None."""
+        if op is None:
+            return None
+        step = library_arnoldi(op[0], op[1], ubase, tbase, 1.0, -_TRB_D * h)
+
+        def counted(V, j, w):
+            stats["njvp"] += 1
+            return step(V, j, w)
+        return counted
+
     nl = dict(eta=1.0)  # Newton's contraction estimate theta / (1 - theta), kept between solves
 
     def newton(z, r, ts, ubase, tbase, h, fz):
@@ -362,7 +373,8 @@ def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, 
             f(z, fz, ts)
             _lib.lincomb(V["G"], (-1.0, _TRB_D * h, 1.0), (z, fz, r))  # -G = r + d h f(z) - z
             stats["nkrylov"] += gmres_device(A, V["G"], V["dz"], rtol=alg.krylov_rtol,
-                                             restart=alg.krylov_restart, maxiter=20 * alg.krylov_restart)
+                                             restart=alg.krylov_restart, maxiter=20 * alg.krylov_restart,
+                                             arnoldi=W_arnoldi(ubase, tbase, h))
             _lib.lincomb(z, (1.0, 1.0), (z, V["dz"]))
             stats["nnewton"] += 1
             ndz = _lib.err_norm(V["dz"], ubase, z, abstol, reltol)
@@ -420,7 +432,8 @@ def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, 
                            _TRB_K * h / (1 - _TRB_G)), (fn, fg, f1))
         if alg.smooth_est:  # W^-1 est (Hosea & Shampine): keeps stiff components from inflating it
             stats["nkrylov"] += gmres_device(W_apply(u, t, h), est, V["tmp"], rtol=alg.krylov_rtol,
-                                             restart=alg.krylov_restart, maxiter=20 * alg.krylov_restart)
+                                             restart=alg.krylov_restart, maxiter=20 * alg.krylov_restart,
+                                             arnoldi=W_arnoldi(u, t, h))
             est = V["tmp"]
         EEst = _lib.err_norm(est, u, u1, abstol, reltol)
         if not math.isfinite(EEst):
@@ -519,7 +532,7 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
     Jacobian-vector product (fvm_jvp, fvm_system_jvp) inside restarted GMRES.  Dirichlet nodes are
     set by the Dirichlet functions first and stay fixed (their du is 0).  Stops when
     |F| <= abstol + reltol |F0| and the last correction is below sqrt(reltol) (1 + |u|)."""
-    from .linsolve import gmres_device
+    from .linsolve import gmres_device, library_arnoldi
 
     p = prob.problem
     ctx = p.device_context()
@@ -551,7 +564,8 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
             break
         it += 1
         jv = lambda v, out: ctx.jvp(u, v, out, t)  # noqa: E731
-        gmres_device(jv, F, du, rtol=1e-3, restart=50, maxiter=2000)
+        gmres_device(jv, F, du, rtol=1e-3, restart=50, maxiter=2000,
+                     arnoldi=library_arnoldi(_lib.OP_SYSTEM if sys else _lib.OP_PROBLEM, ctx.handle, u, t, 0.0, 1.0))
         _lib.lincomb(u, (1.0, -1.0), (u, du))
         step = math.sqrt(_lib.dot(du, du))
         ctx.rhs(u, F, t)

@@ -444,6 +444,17 @@ typedef struct fvm_ode_stats {
 /* d_u [n]: the initial state in, the state at st->t out.  Blocks until done. */
 int fvm_ode_solve(const fvm_ode_opts* o, double* d_u, fvm_ode_stats* st, fvm_stream s);
 
+/* One Arnoldi step of restarted GMRES for A = a I + b J(d_u, t), J the Jacobian of an
+ * operator (fvm_op_kind: fvm_jvp, fvm_system_jvp, or the template matrix itself):
+ * w = A d_V[j], then classical Gram-Schmidt applied twice against d_V[0..j] on the
+ * device, the coefficients never leaving it until one final read.  h [2 (j+1) + 1]
+ * receives the first pass's coefficients, the second's, and |w|^2; the caller
+ * normalises w into d_V[j+1].  The Newton-Krylov stage solves of TR-BDF2
+ * (W = I - d h J) and of SteadyFVMProblem (J), which the reference does with KLU
+ * or a Krylov method from LinearSolve (solve.jl:167-220).  Blocks until done. */
+int fvm_arnoldi_step(int32_t op_kind, void* op, int64_t n, const double* d_u, double t, double a, double b,
+                     int32_t j, double* const* d_V, double* d_w, double* h, fvm_stream s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -145,3 +145,38 @@ def test_ode_solve_rejects_bad_options(gpu):
                 dict(n_save=3), dict(method=_lib.ODE_EULER, dt=0.0)):
         with pytest.raises(FVM.FVMError):
             call(**bad)
+
+
+def test_library_arnoldi_matches_host_gram_schmidt(gpu):
+    """gmres_device with fvm_arnoldi_step (A = J and A = I - c J) gives the host
+    Gram-Schmidt's iterates bit for bit."""
+    from finitevolumemethod.linsolve import gmres_device, library_arnoldi
+
+    prob = plate(40)
+    ctx = prob.device_context()
+    n = len(prob.initial_condition)
+    rng = np.random.default_rng(5)
+    u = _lib.DeviceArray.from_host(rng.random(n))
+    b = _lib.DeviceArray.from_host(rng.standard_normal(n))
+    for a, c in ((0.0, 1.0), (1.0, -0.05)):
+        def apply(v, out, a=a, c=c):
+            ctx.jvp(u, v, out, 0.1)
+            if not (a == 0.0 and c == 1.0):
+                _lib.lincomb(out, (a, c), (v, out))
+        x1, x2 = _lib.DeviceArray(n), _lib.DeviceArray(n)
+        k1 = gmres_device(apply, b, x1, rtol=1e-10, restart=20, maxiter=400)
+        k2 = gmres_device(apply, b, x2, rtol=1e-10, restart=20, maxiter=400,
+                          arnoldi=library_arnoldi(_lib.OP_PROBLEM, ctx.handle, u, 0.1, a, c))
+        assert k1 == k2 and k1 > 20, (k1, k2)  # restarts included
+        assert np.array_equal(x1.to_host(), x2.to_host())
+
+
+def test_trbdf2_with_library_arnoldi_matches_host_loop(gpu):
+    prob = plate(30)
+    a = FVM.solve(prob, FVM.TRBDF2(), saveat=0.1)
+    ctx = prob.device_context()
+    cb = lambda u, t: ctx.apply_dirichlet(u, t)  # noqa: E731
+    b = integrate(ctx.rhs, prob.initial_condition, prob.initial_time, prob.final_time, FVM.TRBDF2(),
+                  saveat=0.1, step_callback=cb, jvp=lambda u, v, out, t: ctx.jvp(u, v, out, t))
+    assert_same(a, b)
+    assert a.stats["nkrylov"] > 50
