@@ -160,7 +160,11 @@ __global__ __launch_bounds__(256) void maxpy_neg_dev_kernel(int64_t n, double* _
   }
 }
 
-std::mutex g_mu;
+std::mutex g_mu;   // allocation of the scratch buffers
+// The reductions share these buffers: each call holds g_use from its first launch
+// to its final synchronisation, so calls from several host threads (on different
+// streams) are serialised instead of mixing their partial sums.
+std::mutex g_use;
 double* g_ascratch = nullptr;  // Arnoldi step results: 2 kMaxDots coefficients + |w|^2
 double* g_scratch = nullptr;   // kRedBlocks partials + 1 result
 double* g_mscratch = nullptr;  // kMaxDots * kRedBlocks partials + kMaxDots results
@@ -184,6 +188,7 @@ int reduce(int kind, int64_t n, const double* x, const double* y, const double* 
   double* buf;
   int rc = scratch(&buf);
   if (rc) return rc;
+  std::lock_guard<std::mutex> use(g_use);
   hipLaunchKernelGGL(reduce_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, s, kind, n, x, y, a, b,
                      abstol, reltol, buf);
   hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kRedThreads), 0, s, buf, buf + kRedBlocks);
@@ -209,6 +214,7 @@ int cgs2_norm(int64_t n, int k, const double* const* xs, double* w, double* h_ho
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_ascratch) FVM_HIP(hipMalloc((void**)&g_ascratch, sizeof(double) * (2 * kMaxDots + 8)));
   }
+  std::lock_guard<std::mutex> use(g_use);
   AxpyTerms t;
   t.k = k;
   for (int j = 0; j < k; ++j) {
@@ -271,6 +277,7 @@ int fvm_vec_mdot(int64_t n, int32_t k, const double* const* d_xs, const double* 
   double* buf;
   int rc = mscratch(&buf);
   if (rc) return rc;
+  std::lock_guard<std::mutex> use(g_use);
   hipStream_t st = (hipStream_t)s;
   for (int first = 0; first < k; first += kDotChunk) {
     DotChunk c;

@@ -374,7 +374,9 @@ int fvm_system_compute_flux(fvm_system* s, int64_t n, const int32_t* d_i, const 
 /* ---- device vector operations (used by the integrators and solvers) ------
  * The reference broadcasts over Vector{Float64} inside OrdinaryDiffEq; device-
  * resident state needs these as kernels.  Reductions use a fixed grid and order,
- * so they are bitwise reproducible. */
+ * so they are bitwise reproducible; they block until their result is on the host
+ * and share one scratch area per process, so calls from several host threads are
+ * serialised (each is still correct). */
 /* d_out = sum_k coefs[k] * d_xs[k], 1 <= nterms <= 8 (d_out may alias an input). */
 int fvm_vec_lincomb(int64_t n, double* d_out, int nterms, const double* coefs,
                     const double* const* d_xs, fvm_stream s);

@@ -45,3 +45,48 @@ def test_gmres_solves_a_nonsymmetric_system(gpu):
     xs = np.linalg.solve(A, b)
     assert np.linalg.norm(x.to_host() - xs) <= 1e-9 * np.linalg.norm(xs)
     del dA_cols
+
+
+def test_reductions_from_several_host_threads(gpu):
+    """Dot products and error norms issued concurrently from four host threads,
+    each on its own stream, equal the same reductions done one at a time (the
+    reductions share a scratch area, which calls hold for their whole duration)."""
+    import ctypes
+    import threading
+
+    from finitevolumemethod import _lib
+
+    rng = np.random.default_rng(11)
+    n = 300_000
+    xs = [rng.standard_normal(n) for _ in range(4)]
+    d = [_lib.DeviceArray.from_host(x) for x in xs]
+    ref = [_lib.dot(a, a) for a in d]
+    refn = [_lib.err_norm(a, a, a, 1e-6, 1e-3) for a in d]
+    out = [[None, None] for _ in d]
+    errs = []
+
+    L = _lib.hip_lib()
+    streams = []
+    for _ in range(4):
+        h = ctypes.c_void_p()
+        _lib.check(L.fvm_stream_create(ctypes.byref(h)), "fvm_stream_create")
+        streams.append(h.value)
+
+    def work(k):
+        try:
+            for _ in range(20):
+                out[k][0] = _lib.dot(d[k], d[k], stream=streams[k])
+                out[k][1] = _lib.err_norm(d[k], d[k], d[k], 1e-6, 1e-3, stream=streams[k])
+        except Exception as e:  # noqa: BLE001
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    for h in streams:
+        _lib.check(L.fvm_stream_destroy(h), "fvm_stream_destroy")
+    for k in range(4):
+        assert out[k][0] == ref[k] and out[k][1] == refn[k]
