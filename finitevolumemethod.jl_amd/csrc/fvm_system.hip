@@ -227,12 +227,14 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
   constexpr int MAXN = 2 * BLOCK < 1024 ? 2 * BLOCK : 1024;
   constexpr int NS = MAXN / BLOCK;
   __shared__ double s_u[NV][MAXN];
-  // the node coordinates are dead once the fluxes are formed: the fluxes reuse their space
-  constexpr int XY = 2 * MAXN, QN = NV * 3 * TILE;
+  // the node coordinates are dead once the fluxes are formed: the fluxes reuse their
+  // space, one variable at a time (the others wait in registers), so a workgroup's LDS
+  // does not grow with NV's flux block and more workgroups fit on a CU
+  constexpr int XY = 2 * MAXN, QN = 3 * TILE;
   __shared__ double s_xyq[XY > QN ? XY : QN];
   double* const s_x = s_xyq;
   double* const s_y = s_xyq + MAXN;
-  double* const s_q = s_xyq;  // [NV][3 TILE]
+  double* const s_q = s_xyq;  // [3 TILE] fluxes of the variable of the current round
   __shared__ uint16_t s_inc[3 * TILE];
   __shared__ int s_is[MAXN + 1];
   const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
@@ -350,36 +352,45 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
         for (int v = 0; v < NV; ++v) Qr[r][e][v] = 0.0;
     }
   }
-  __syncthreads();  // every thread has read the coordinates that s_q overwrites
+  double acc[NS][NV];
 #pragma unroll
-  for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * BLOCK;
-    if (lt < nt) {
+  for (int r = 0; r < NS; ++r)
 #pragma unroll
-      for (int e = 0; e < 3; ++e)
+    for (int v = 0; v < NV; ++v) acc[r][v] = 0.0;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) s_q[v * 3 * TILE + 3 * lt + e] = Qr[r][e][v];
+  for (int v = 0; v < NV; ++v) {
+    // v = 0: every thread has read the coordinates that s_q overwrites;
+    // v > 0: every node has summed the previous variable's fluxes
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < TPT; ++r) {
+      const int lt = tid + r * BLOCK;
+      if (lt < nt) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) s_q[3 * lt + e] = Qr[r][e][v];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+      const int l = tid + r * BLOCK;
+      if (l < nn) {
+        const int k1 = s_is[l + 1];
+        for (int k = s_is[l]; k < k1; ++k) {
+          const int w = s_inc[k];
+          const int lt = w >> 2, sl = w & 3;
+          acc[r][v] = acc[r][v] + s_q[3 * lt + (sl == 0 ? 2 : sl - 1)] - s_q[3 * lt + sl];
+        }
+      }
     }
   }
-  __syncthreads();
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
     const int l = tid + r * BLOCK;
     if (l < nn) {
-      double acc[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-      const int k1 = s_is[l + 1];
-      for (int k = s_is[l]; k < k1; ++k) {
-        const int w = s_inc[k];
-        const int lt = w >> 2, sl = w & 3;
-        const int prev = 3 * lt + (sl == 0 ? 2 : sl - 1), cur = 3 * lt + sl;
-#pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] = acc[v] + s_q[v * 3 * TILE + prev] - s_q[v * 3 * TILE + cur];
-      }
       if (pi[r] >= 0) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) a.partial[(int64_t)pi[r] * NV + v] = acc[v];
+        for (int v = 0; v < NV; ++v) a.partial[(int64_t)pi[r] * NV + v] = acc[r][v];
       } else {
         double u[NV], dv[NV];
 #pragma unroll
@@ -390,7 +401,7 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
         const double V = a.lcv[n0 + l];
 #pragma unroll
         for (int v = 0; v < NV; ++v)
-          a.out[v + NV * (int64_t)g[r]] = finish_var<NV, JV>(a, v, g[r], acc[v], V, u, dv);
+          a.out[v + NV * (int64_t)g[r]] = finish_var<NV, JV>(a, v, g[r], acc[r][v], V, u, dv);
       }
     }
   }
