@@ -464,10 +464,13 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
         // update_du! (triangle_contributions.jl:10-15): i += Q3 - Q1, j += Q1 - Q2, k += Q2 - Q3
         acc = acc + s_q[3 * lt + (sl == 0 ? 2 : sl - 1)] - s_q[3 * lt + sl];
       }
+      // the node's u (and v) are re-read from LDS and its volume from memory here rather
+      // than held in registers across the barriers: fewer VGPRs, more waves per SIMD
       if (pi[r] >= 0)
         a.partial[pi[r]] = acc;
       else
-        a.out[g[r]] = finish_pre<MODE>(a, g[r], acc, ug[r], vg[r], cg[r], Vg[r], dt);
+        a.out[g[r]] = finish_pre<MODE>(a, g[r], acc, s_u[l], MODE == M_JVP ? s_v[l] : 0.0, -1 - pi[r],
+                                       a.lcv[n0 + l], dt);
     }
   }
 }
