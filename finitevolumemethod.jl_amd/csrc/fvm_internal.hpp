@@ -18,7 +18,7 @@
 // Revision of the RHS kernels' memory layout, part of fvm_build_info so that measured
 // traffic is matched to the build that produced it (2: fix-up records in fix order;
 // 3: tile kernels' fluxes share LDS with the node coordinates; 4: also in the VJP).
-#define FVM_RHS_REV 4
+#define FVM_RHS_REV 5
 #ifndef FVM_GEO_LEVEL
 #define FVM_GEO_LEVEL 4
 #endif
