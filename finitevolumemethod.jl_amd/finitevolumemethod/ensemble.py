@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib, forms
-from .problem import FVMProblem, physics_desc, physics_of
+from .problem import FVMProblem, SteadyFVMProblem, physics_desc, physics_of
 from .solve import Euler, ODESolution
 
 
@@ -49,9 +49,9 @@ class EnsembleSolution:
 
 
 class EnsembleProblem:
-    """EnsembleProblem(prob; prob_func, trajectories) over FVMProblems or FVMSystems, or
-    EnsembleProblem(list_of_problems).  prob_func(prob, i, repeat) returns member i
-    (1-based i, as SciML's); trajectories may also be given to solve."""
+    """EnsembleProblem(prob; prob_func, trajectories) over FVMProblems, FVMSystems or
+    SteadyFVMProblems, or EnsembleProblem(list_of_problems).  prob_func(prob, i, repeat)
+    returns member i (1-based i, as SciML's); trajectories may also be given to solve."""
 
     def __init__(self, prob, prob_func=None, trajectories=None):
         from .system import FVMSystem
@@ -61,8 +61,9 @@ class EnsembleProblem:
             self.problems = list(prob)
         else:
             self._make(trajectories or 1)
-        if not self.problems or not all(isinstance(p, (FVMProblem, FVMSystem)) for p in self.problems):
-            raise TypeError("EnsembleProblem members must be FVMProblems or FVMSystems")
+        if not self.problems or not all(isinstance(p, (FVMProblem, FVMSystem, SteadyFVMProblem))
+                                        for p in self.problems):
+            raise TypeError("EnsembleProblem members must be FVMProblems, FVMSystems or SteadyFVMProblems")
         self._batch = None
 
     def _make(self, trajectories):
@@ -81,17 +82,19 @@ class EnsembleProblem:
         batch from its initial_time to its final_time with its own dt (a scalar, a
         list, or None for explicit_dt); every member must take the same number of
         steps; the solutions hold the initial and final states.  Any other algorithm
-        (Tsit5(), TRBDF2(), ...), systems, and members whose flux function is a linear
-        form other than -D(x, y) grad u (advection; not packed): the members are solved
-        one after the other on the device, each with its own adaptive steps (SciML's
-        EnsembleSerial), keyword arguments passed to solve.  Returns an EnsembleSolution."""
+        (Tsit5(), TRBDF2(), ...), systems, steady problems, and members whose flux function
+        is a linear form other than -D(x, y) grad u (advection; not packed): the members are
+        solved one after the other on the device, each with its own adaptive steps or
+        Newton iterations (SciML's EnsembleSerial), keyword arguments passed to solve.
+        Returns an EnsembleSolution."""
         from .solve import solve as solve_one
         from .system import FVMSystem
 
         if trajectories is not None and not isinstance(self._prob, (list, tuple)):
             self._make(trajectories)
-        alg = alg if alg is not None else Euler()
-        if (not isinstance(alg, Euler) or any(isinstance(p, FVMSystem) for p in self.problems)
+        steady = any(isinstance(p, SteadyFVMProblem) for p in self.problems)
+        alg = alg if alg is not None else (None if steady else Euler())
+        if (steady or not isinstance(alg, Euler) or any(isinstance(p, FVMSystem) for p in self.problems)
                 or any(_linear_flux(p) for p in self.problems)):
             tm = _lib.Timer()
             tm.start()

@@ -161,3 +161,35 @@ def test_ensemble_with_adaptive_algorithm_solves_members_in_turn(gpu):
         assert all(np.array_equal(a, b) for a, b in zip(s.u, alone.u))
     # decay grows with the member's D
     assert sol.u[0].u[-1].max() > sol.u[1].u[-1].max() > sol.u[2].u[-1].max()
+
+
+def test_ensemble_of_steady_problems(gpu):
+    """The equilibrium-temperature tutorial's pattern (equilibrium_temperature_..._using_ensembleproblems.jl:70-98,
+    overview.md:222): an EnsembleProblem over a SteadyFVMProblem whose prob_func varies
+    T_inf in the Robin condition k grad T . n = h (T_inf - T), solved member by member;
+    each member equals its own steady solve bit for bit, and the temperature rises with
+    T_inf."""
+    tri = FVM.triangulate_rectangle(0.0, 0.06, 0.0, 0.06, 30, 30, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    k, h = 3.0, 20.0
+
+    def make(Tinf):
+        bcs = (lambda x, y, t, T, p: 0.0 * T, lambda x, y, t, T, p: 40.0 + 0.0 * T,
+               lambda x, y, t, T, p: -p["h"] * (p["Tinf"] - T) / p["k"], lambda x, y, t, T, p: 70.0 + 0.0 * T)
+        BCs = FVM.BoundaryConditions(mesh, bcs, (FVM.Neumann, FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet),
+                                     parameters=(None, None, dict(h=h, Tinf=Tinf, k=k), None))
+        prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, T, p: 1.0 + 0.0 * T,
+                              initial_condition=500 * tri.y + 40, final_time=np.inf)
+        return FVM.SteadyFVMProblem(prob)
+
+    Tinfs = [10.0, 20.0, 30.0, 40.0]
+    ens = FVM.EnsembleProblem(make(Tinfs[0]), prob_func=lambda p, i, repeat: make(Tinfs[i - 1]))
+    sol = FVM.solve(ens, trajectories=len(Tinfs))
+    assert sol.stats["mode"] == "serial" and len(sol) == len(Tinfs)
+    means = []
+    for Tinf, s in zip(Tinfs, sol.u):
+        assert s.retcode == "Success", s.stats
+        single = FVM.solve(make(Tinf))
+        assert np.array_equal(s.u[-1], single.u[-1])
+        means.append(np.mean(s.u[-1]))
+    assert all(b > a for a, b in zip(means, means[1:]))
