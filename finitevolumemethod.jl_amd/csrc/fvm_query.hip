@@ -7,6 +7,7 @@
 // triangle incidence of the mesh (find_directed_edge) in place of the reference's
 // get_adjacent and triangle_props Dict lookups.  Unknown edges or triangles set a
 // flag that the host turns into an error, as the reference's lookups throw.
+#include <algorithm>
 #include <cmath>
 
 #include "fvm_internal.hpp"
@@ -142,10 +143,137 @@ __global__ __launch_bounds__(256) void interp_query_kernel(InterpQuery a) {
   }
 }
 
+// ----------------------------------------------------------------------------
+// point location (DelaunayTriangulation's jump_and_march, which the reference's
+// tutorials call before pl_interpolate)
+// ----------------------------------------------------------------------------
+struct LocQuery {
+  int64_t n, T, N;
+  int32_t max_steps;
+  const double* px;
+  const double* py;
+  const int32_t* start;  // a start vertex per query (may be null: vertex 0)
+  const int32_t* node_tri_start;
+  const int32_t* node_tri;
+  const int32_t* tri;
+  const int32_t* perm;  // sorted -> input triangle
+  const double* x;
+  const double* y;
+  int32_t* out;  // input triangle index, -1 outside the mesh, -2 while undecided
+};
+
+__device__ __forceinline__ double orient(double ax, double ay, double bx, double by, double px, double py) {
+  return (bx - ax) * (py - ay) - (by - ay) * (px - ax);
+}
+
+__device__ __forceinline__ bool contains(const LocQuery& a, int32_t t, double px, double py, double* o) {
+  const int32_t v[3] = {a.tri[t], a.tri[a.T + t], a.tri[2 * a.T + t]};
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const int32_t p0 = v[e], p1 = v[e == 2 ? 0 : e + 1];
+    o[e] = orient(a.x[p0], a.y[p0], a.x[p1], a.y[p1], px, py);
+  }
+  return o[0] >= 0.0 && o[1] >= 0.0 && o[2] >= 0.0;
+}
+
+// Visibility walk from a triangle of the start vertex: cross the edge the point
+// lies most outside of (or another edge it lies outside of when that one is on the
+// boundary).  A walk that leaves through the boundary or runs out of steps marks
+// the query undecided (-2): a non-convex domain can trap a straight walk.
+__global__ __launch_bounds__(256) void locate_walk_kernel(LocQuery a) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.n) return;
+  const double px = a.px[q], py = a.py[q];
+  int32_t v0 = a.start ? a.start[q] : 0;
+  if (v0 < 0 || v0 >= a.N) v0 = 0;
+  int32_t k = a.node_tri_start[v0];
+  if (k == a.node_tri_start[v0 + 1]) {  // not a vertex of the mesh
+    a.out[q] = -2;
+    return;
+  }
+  int32_t t = a.node_tri[k] >> 2;
+  for (int step = 0; step < a.max_steps; ++step) {
+    double o[3];
+    if (contains(a, t, px, py, o)) {
+      a.out[q] = a.perm[t];
+      return;
+    }
+    const int32_t v[3] = {a.tri[t], a.tri[a.T + t], a.tri[2 * a.T + t]};
+    // the outside edges, most negative first
+    int order[3] = {0, 1, 2};
+    for (int i = 0; i < 2; ++i)
+      for (int j = i + 1; j < 3; ++j)
+        if (o[order[j]] < o[order[i]]) {
+          const int tmp = order[i];
+          order[i] = order[j];
+          order[j] = tmp;
+        }
+    int32_t next = -1;
+    for (int i = 0; i < 3 && next < 0; ++i) {
+      const int e = order[i];
+      if (!(o[e] < 0.0)) break;
+      // the neighbour across edge v[e] -> v[e+1] holds the directed edge v[e+1] -> v[e]
+      next = find_directed_edge(a.node_tri_start, a.node_tri, a.tri, a.T, v[e == 2 ? 0 : e + 1], v[e]);
+    }
+    if (next < 0) break;
+    t = next;
+  }
+  a.out[q] = -2;
+}
+
+// Undecided queries: one workgroup scans every triangle; the smallest sorted index
+// containing the point wins (an LDS minimum: the same answer on every run), -1 if none.
+__global__ __launch_bounds__(256) void locate_scan_kernel(LocQuery a) {
+  const int64_t q = blockIdx.x;
+  if (a.out[q] != -2) return;
+  __shared__ int32_t best;
+  if (threadIdx.x == 0) best = 0x7fffffff;
+  __syncthreads();
+  const double px = a.px[q], py = a.py[q];
+  for (int64_t t = threadIdx.x; t < a.T; t += blockDim.x) {
+    double o[3];
+    if (contains(a, (int32_t)t, px, py, o)) {
+      atomicMin(&best, (int32_t)t);
+      break;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.out[q] = best == 0x7fffffff ? -1 : a.perm[best];
+}
+
 }  // namespace
 
 
 extern "C" {
+
+int fvm_locate_points(const fvm_mesh* m, int64_t n, const double* d_x, const double* d_y, const int32_t* d_start,
+                      int32_t* d_tri_out, fvm_stream s) {
+  FVM_CHECK(m && n >= 0, "fvm_locate_points: bad argument");
+  if (n == 0) return FVM_OK;
+  FVM_CHECK(d_x && d_y && d_tri_out, "fvm_locate_points: null argument");
+  FVM_CHECK(n <= 0x7fffffff, "fvm_locate_points: at most 2^31 - 1 points per call");
+  LocQuery a;
+  a.n = n;
+  a.T = m->T;
+  a.N = m->N;
+  a.max_steps = (int32_t)std::min<int64_t>(4096, 8 + 4 * (int64_t)std::sqrt((double)m->T));
+  a.px = d_x;
+  a.py = d_y;
+  a.start = d_start;
+  a.node_tri_start = m->node_tri_start;
+  a.node_tri = m->node_tri;
+  a.tri = m->tri;
+  a.perm = m->perm;
+  a.x = m->x;
+  a.y = m->y;
+  a.out = d_tri_out;
+  hipStream_t st = (hipStream_t)s;
+  hipLaunchKernelGGL(locate_walk_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(locate_scan_kernel, dim3((unsigned)n), dim3(256), 0, st, a);
+  FVM_HIP(hipGetLastError());
+  FVM_HIP(hipStreamSynchronize(st));
+  return FVM_OK;
+}
 
 int fvm_compute_flux(fvm_ctx* c, int64_t n, const int32_t* d_i, const int32_t* d_j, const double* d_u,
                      double t, const double* d_dq, double* d_out, fvm_stream s) {

@@ -3,7 +3,7 @@
 The public names mirror the reference module (src/FiniteVolumeMethod.jl:39-52 and
 the template exports): FVMGeometry, FVMProblem, FVMSystem, SteadyFVMProblem,
 BoundaryConditions, InternalConditions, Conditions, Neumann, Dudt, Dirichlet,
-Constrained, solve, fvm_eqs (fvm_eqs!), compute_flux, pl_interpolate, DiffusionEquation, PoissonsEquation,
+Constrained, solve, fvm_eqs (fvm_eqs!), compute_flux, pl_interpolate, find_triangle, DiffusionEquation, PoissonsEquation,
 LaplacesEquation, MeanExitTimeProblem, LinearReactionDiffusionEquation.  Every numerical kernel runs on the GPU through
 libfvmhip.so's C ABI (include/fvm_hip.h).
 """
@@ -19,7 +19,7 @@ from .geometry import FVMGeometry, TriangleProperties  # noqa: F401
 from .mesh import Triangulation, jittered_rectangle, triangulate, triangulate_rectangle  # noqa: F401
 from .problem import FVMProblem, SteadyFVMProblem, construct_flux_function  # noqa: F401
 from .system import FVMSystem, InvalidFluxError, is_system  # noqa: F401
-from .queries import compute_flux, pl_interpolate  # noqa: F401
+from .queries import compute_flux, find_triangle, pl_interpolate  # noqa: F401
 from .solve import SSPRK3, TRBDF2, Euler, ODESolution, Tsit5, solve  # noqa: F401
 from .templates import (AbstractFVMTemplate, BiCGStab, CG, DiffusionEquation,  # noqa: F401
                         LaplacesEquation, LinearReactionDiffusionEquation, LinearSolution,

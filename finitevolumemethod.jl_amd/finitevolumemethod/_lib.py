@@ -207,6 +207,7 @@ HIP_API = {
     "fvm_template_residual": [_vp, _vp, _dp, _vp],
     "fvm_template_solve": [_vp, ctypes.POINTER(SolveOpts), _vp, ctypes.POINTER(SolveStats), _vp],
     "fvm_ode_solve": [ctypes.POINTER(OdeOpts), _vp, ctypes.POINTER(OdeStats), _vp],
+    "fvm_locate_points": [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp],
     "fvm_arnoldi_step": [ctypes.c_int32, _vp, ctypes.c_int64, _vp, ctypes.c_double, ctypes.c_double,
                          ctypes.c_double, ctypes.c_int32, ctypes.POINTER(_vp), _vp, _dp, _vp],
     "fvm_batch_create": [ctypes.c_int32, ctypes.POINTER(MeshDesc), ctypes.POINTER(PhysicsDesc), _dp,

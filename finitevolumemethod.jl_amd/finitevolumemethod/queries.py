@@ -9,6 +9,7 @@ matrix for a system) or a DeviceArray already in the device layout.
 import numpy as np
 
 from . import _lib, forms
+from .geometry import FVMGeometry
 from .problem import SteadyFVMProblem
 from .system import FVMSystem
 
@@ -90,3 +91,61 @@ def pl_interpolate(prob, T, u, x, y):
         return float(r[0]) if scalar else r
     r = r.reshape(n, nv)
     return r[0].copy() if scalar else r
+
+
+def _start_grid(mesh):
+    """A G x G grid over the mesh's bounding box with, per cell, the mesh vertex nearest
+    to the cell centre (built once per mesh): the start vertices of point location."""
+    g = getattr(mesh, "_locate_grid", None)
+    if g is not None:
+        return g
+    from scipy.spatial import cKDTree
+
+    tri = mesh.triangulation
+    verts = np.unique(tri.triangles)
+    px, py = tri.x[verts], tri.y[verts]
+    x0, x1, y0, y1 = px.min(), px.max(), py.min(), py.max()
+    G = int(max(1, min(2048, np.sqrt(len(verts) / 4))))
+    hx = (x1 - x0) / G if x1 > x0 else 1.0
+    hy = (y1 - y0) / G if y1 > y0 else 1.0
+    cx = x0 + (np.arange(G) + 0.5) * hx
+    cy = y0 + (np.arange(G) + 0.5) * hy
+    CX, CY = np.meshgrid(cx, cy, indexing="ij")
+    _, nearest = cKDTree(np.stack([px, py], 1)).query(np.stack([CX.ravel(), CY.ravel()], 1))
+    g = (x0, y0, hx, hy, G, verts[nearest].astype(np.int32))
+    mesh._locate_grid = g
+    return g
+
+
+def find_triangle(obj, x, y, start=None):
+    """The mesh triangle (i, j, k), counter-clockwise, containing each point (x, y), or
+    (-1, -1, -1) outside the mesh: what the reference's tutorials obtain from
+    DelaunayTriangulation's jump_and_march / find_triangle before calling
+    pl_interpolate.  obj is an FVMGeometry or a problem; start is an initial vertex per
+    point (or one for all), by default the vertex nearest to the point's cell of a grid
+    over the mesh.  The search runs on the device (fvm_locate_points)."""
+    mesh = obj if isinstance(obj, FVMGeometry) else obj.mesh
+    tri = mesh.triangulation
+    scalar = np.ndim(x) == 0 and np.ndim(y) == 0
+    xs = np.ascontiguousarray(np.atleast_1d(x), np.float64)
+    ys = np.ascontiguousarray(np.atleast_1d(y), np.float64)
+    if xs.shape != ys.shape:
+        raise ValueError("find_triangle: x and y must have the same shape")
+    n = len(xs)
+    if start is None:
+        x0, y0, hx, hy, G, reps = _start_grid(mesh)
+        ix = np.clip(((xs - x0) / hx).astype(np.int64), 0, G - 1)
+        iy = np.clip(((ys - y0) / hy).astype(np.int64), 0, G - 1)
+        st = reps[ix * G + iy]
+    else:
+        st = np.broadcast_to(np.asarray(start, np.int32), (n,))
+    st = np.ascontiguousarray(st, np.int32)
+    d_x, d_y, d_s = (_lib.DeviceArray.from_host(a) for a in (xs, ys, st))
+    d_o = _lib.DeviceArray(max(n, 1), np.int32)
+    _lib.check(_lib.hip_lib().fvm_locate_points(mesh.handle, n, d_x.ptr, d_y.ptr, d_s.ptr, d_o.ptr, None),
+               "fvm_locate_points")
+    t = d_o.to_host()[:n]
+    out = np.full((n, 3), -1, np.int64)
+    ok = t >= 0
+    out[ok] = tri.triangles[t[ok]]
+    return tuple(int(v) for v in out[0]) if scalar else out

@@ -191,6 +191,15 @@ int fvm_compute_flux(fvm_ctx* c, int64_t n, const int32_t* d_i, const int32_t* d
 int fvm_pl_interpolate(const fvm_mesh* m, int32_t n_vars, int64_t n, const int32_t* d_tri,
                        const double* d_x, const double* d_y, const double* d_u, double* d_out,
                        fvm_stream s);
+/* d_tri_out[q] = the (input) index of a mesh triangle containing (d_x[q], d_y[q]) (on an
+ * edge or vertex: one of the triangles sharing it), -1 outside the mesh.  Each query
+ * walks from a triangle of its start vertex d_start[q] (NULL: vertex 0) across the
+ * edges the point lies outside of; a walk trapped by the boundary of a non-convex
+ * domain is finished by a scan of all triangles.  Replaces DelaunayTriangulation's
+ * jump_and_march / find_triangle, which the reference's tutorials call before
+ * pl_interpolate.  Blocks until done. */
+int fvm_locate_points(const fvm_mesh* m, int64_t n, const double* d_x, const double* d_y, const int32_t* d_start,
+                      int32_t* d_tri_out, fvm_stream s);
 
 /* ---- linear templates (src/specific_problems/) ----------------------------
  * The operator A and vector b of PoissonsEquation / LaplacesEquation

@@ -164,3 +164,61 @@ def test_pl_interpolate_matches_oracle(gpu, nv):
         assert np.array_equal(one, got[4])
     with pytest.raises(FVM.FVMError):  # clockwise: not a triangle of the mesh
         FVM.pl_interpolate(prob, (Ts[0][0], Ts[0][2], Ts[0][1]), u, px[0], py[0])
+
+
+def test_find_triangle_locates_points(gpu):
+    """Points drawn strictly inside random triangles of a jittered mesh are found in
+    exactly those triangles (also from a fixed start vertex, walking far); points
+    outside the rectangle give (-1, -1, -1)."""
+    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 40, 35, jitter=0.25, seed=21, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    rng = np.random.default_rng(2)
+    T = rng.integers(0, len(tri.triangles), 3000)
+    w = rng.uniform(0.05, 1.0, (len(T), 3))
+    w /= w.sum(1, keepdims=True)
+    V = tri.triangles[T]
+    px = np.sum(w * tri.x[V], 1)
+    py = np.sum(w * tri.y[V], 1)
+    got = FVM.find_triangle(mesh, px, py)
+    assert np.array_equal(got, V)
+    assert np.array_equal(FVM.find_triangle(mesh, px, py, start=0), V)
+    assert FVM.find_triangle(mesh, float(px[7]), float(py[7])) == tuple(int(v) for v in V[7])
+    out = FVM.find_triangle(mesh, np.array([-0.1, 0.5, 1.3]), np.array([0.5, 1.2, 0.5]))
+    assert np.all(out == -1)
+
+
+def test_find_triangle_on_a_domain_with_a_hole(gpu):
+    """On the annulus 0.2 <= r <= 1 a straight walk from the far side is blocked by the
+    hole; such queries are finished by the scan.  Points in the hole are outside."""
+    from problems import annulus_triangulation
+
+    tri = annulus_triangulation()
+    mesh = FVM.FVMGeometry(tri)
+    rng = np.random.default_rng(4)
+    T = rng.integers(0, len(tri.triangles), 500)
+    w = rng.uniform(0.05, 1.0, (len(T), 3))
+    w /= w.sum(1, keepdims=True)
+    V = tri.triangles[T]
+    px, py = np.sum(w * tri.x[V], 1), np.sum(w * tri.y[V], 1)
+    far = int(np.argmin(tri.x))  # a vertex on the left of the hole
+    assert np.array_equal(FVM.find_triangle(mesh, px, py, start=far), V)
+    th = rng.uniform(0, 2 * np.pi, 50)
+    r = rng.uniform(0.0, 0.15, 50)
+    assert np.all(FVM.find_triangle(mesh, r * np.cos(th), r * np.sin(th)) == -1)
+
+
+def test_pl_interpolate_at_located_points(gpu):
+    """find_triangle then pl_interpolate, as the advection-diffusion tutorial does with
+    jump_and_march: equal to the oracle's interpolant at the same points."""
+    prob = nonlinear_problem(nx=20, ny=17, seed=8, jitter=0.25)[0]
+    tri = prob.mesh.triangulation
+    _, props = geometry(tri.x, tri.y, tri.triangles)
+    et = directed_edge_triangles(tri.triangles)
+    rng = np.random.default_rng(6)
+    px, py = rng.uniform(0.0, 1.0, 400), rng.uniform(0.0, 1.0, 400)
+    u = rng.random(tri.num_points())
+    Ts = FVM.find_triangle(prob, px, py)
+    assert np.all(Ts >= 0)
+    got = FVM.pl_interpolate(prob, Ts, u, px, py)
+    ref = np.array([orc_interp(tri.triangles, props, et, u, T, x, y)[0] for T, x, y in zip(Ts, px, py)])
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-13)
