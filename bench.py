@@ -207,6 +207,34 @@ def template_solve(n_side, rtol=1e-10, direct=True):
     return out
 
 
+def solve_bench(prob, rhs_ms, tf=4e-6):
+    """End-to-end explicit integration of the throughput problem: Tsit5 (abstol 1e-6,
+    reltol 1e-3, the library's integrator: every stage, the error norm, the step
+    control and the Dirichlet update on the device) from t = 0 to tf, after one
+    warm-up solve.  The i.i.d. random state makes the steps stability-bound (about
+    1e-7), so tf fixes roughly 40 steps; reported against the RHS alone."""
+    import finitevolumemethod as FVM
+
+    keep = prob.final_time
+    try:
+        prob.final_time = tf / 8
+        FVM.solve(prob, FVM.Tsit5(), saveat=[0.0, prob.final_time])
+        prob.final_time = tf
+        t0 = time.perf_counter()
+        sol = FVM.solve(prob, FVM.Tsit5(), saveat=[0.0, tf])
+        sec = time.perf_counter() - t0
+    finally:
+        prob.final_time = keep
+    st = sol.stats
+    out = dict(workload=f"Tsit5 on the throughput problem from t = 0 to {tf:g} (abstol 1e-6, reltol 1e-3)",
+               retcode=sol.retcode, seconds=round(sec, 4), accepted_steps=int(st["naccept"]),
+               rejected_steps=int(st["nreject"]), rhs_evaluations=int(st["nf"]),
+               ms_per_step=round(1e3 * sec / max(1, st["naccept"]), 4),
+               rhs_share=round(st["nf"] * rhs_ms * 1e-3 / sec, 3))
+    log(f"solve {out}")
+    return out
+
+
 def ensemble_bench(P=256, steps=1000, graph_steps=100):
     """The ensemble workload: P rectangles [0, 2]^2 with nx, ny ~ round(exp(U[log 20,
     log 300])) drawn independently, interior points jittered by U(-h/4, h/4), constant
@@ -458,6 +486,7 @@ def main():
         log(f"copy kernel {out['copy_bandwidth_GBps']} GB/s")
         out["vjp_rhs"] = vjp_ratio(args.vjp_side)
         log(f"VJP/RHS {out['vjp_rhs']}")
+        out["solve_tsit5"] = solve_bench(prob, ms_step)
         out["system_rhs"] = system_bench(mesh)
         log(f"system RHS {out['system_rhs']}")
         if args.template_side > 0:
@@ -468,6 +497,10 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         du_cpu, cb = cpu_baseline(tri, u, args.cpu_seconds, threads)
         out["cpu_baseline"] = cb
+        if "solve_tsit5" in out:  # the same integration's RHS evaluations alone, at the CPU rate
+            sv = out["solve_tsit5"]
+            sv["cpu_rhs_only_seconds"] = round(sv["rhs_evaluations"] * N / (cb["value"] * 1e9), 2)
+            sv["speedup_vs_cpu_rhs_only"] = round(sv["cpu_rhs_only_seconds"] / sv["seconds"], 1)
         err = float(np.max(np.abs(du_gpu - du_cpu)) / np.max(np.abs(du_cpu)))
         out["parity_max_rel_err_vs_cpu"] = err
         log(f"CPU baseline {cb['value']} Gnode/s on {threads} threads; max rel err {err:.2e}")
