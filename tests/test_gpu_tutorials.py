@@ -84,3 +84,73 @@ def test_disk_time_dependent_boundary(gpu):
         assert np.max(np.abs(u - e) / e) <= 2e-4, (t, np.max(np.abs(u - e) / e))
         # the boundary follows du/dt = u exactly up to the integrator's tolerance
         np.testing.assert_allclose(u[bn], np.exp(t) * sol.u[0][bn], rtol=1e-6)
+
+
+def test_porous_fisher_travelling_wave(gpu):
+    """porous_fisher_equation_and_travelling_waves.jl:40-75: u_t = div(D u grad u) +
+    lam u (1 - u) on [0, 3] x [0, 40], u = 1 at y = 0, u = 0 at y = 40, zero flux on the
+    sides, u0 = 0; D = 0.9, lam = 0.99.  The sharp-front travelling wave
+    u = 1 - exp(a z), z = y - s t - z0 <= 0, has s = sqrt(lam D / 2) and a = sqrt(lam /
+    (2 D)) (substituting the profile into the travelling-wave ODE; the tutorial's
+    variable names c and c_min hold these two the other way round).  The solution does
+    not depend on x, so a coarse x spacing suffices (6 x 161 grid).  The front (u = 1/2)
+    moves at s within 1 % between t = 15 and t = 25 (measured 0.6 %), and behind it the
+    profile matches the exact one within 5e-3 (measured 1.3e-3)."""
+    Dd, lam = 0.9, 0.99
+    tri = FVM.triangulate_rectangle(0.0, 3.0, 0.0, 40.0, 6, 161, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    one = lambda x, y, t, u, p: 1.0 + 0.0 * u  # noqa: E731
+    zero = lambda x, y, t, u, p: 0.0 * u  # noqa: E731
+    BCs = FVM.BoundaryConditions(mesh, (one, zero, zero, zero),
+                                 (FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet, FVM.Neumann))
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, D: D * u, diffusion_parameters=Dd,
+                          source_function=lambda x, y, t, u, l: l * u * (1 - u), source_parameters=lam,
+                          initial_condition=np.zeros(tri.num_points()), final_time=25.0)
+    sol = FVM.solve(prob, FVM.Tsit5(), saveat=[15.0, 25.0])
+    assert sol.retcode == "Success"
+    col = np.nonzero(np.isclose(tri.x, 1.8))[0]
+    col = col[np.argsort(tri.y[col])]
+    y = tri.y[col]
+
+    def front(u):
+        v = u[col]
+        k = np.nonzero(v < 0.5)[0][0]
+        return y[k - 1] + (v[k - 1] - 0.5) / (v[k - 1] - v[k]) * (y[k] - y[k - 1])
+
+    s, a = np.sqrt(lam * Dd / 2), np.sqrt(lam / (2 * Dd))
+    speed = (front(sol.u[1]) - front(sol.u[0])) / 10.0
+    print("porous fisher speed", speed, "exact", s)
+    assert abs(speed - s) <= 0.01 * s
+    y0 = front(sol.u[1]) + np.log(2) / a  # where the exact profile reaches 0
+    behind = (y < y0 - 0.5) & (y > y0 - 8)
+    err = np.max(np.abs(sol.u[1][col][behind] - (1 - np.exp(a * (y[behind] - y0)))))
+    print("porous fisher profile", err)
+    assert err <= 5e-3
+
+
+def test_laplace_internal_dirichlet_steady_equals_template(gpu):
+    """laplaces_equation_with_internal_dirichlet_conditions.jl:47-112: lap u = 0 on
+    [0, 1]^2 with u = 0 (bottom), 100 y (sides), 100 (top) and internal Dirichlet nodes
+    u = 0 on x = 1/2, y <= 2/5.  The SteadyFVMProblem (Newton-Krylov on fvm_eqs!) and the
+    LaplacesEquation template (device CG) give the same solution to the solvers'
+    tolerances (1e-5 absolute on values up to 100; measured 2.8e-7); the internal nodes
+    hold 0 and the maximum principle holds."""
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 41, 41, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    bcs = (lambda x, y, t, u, p: 0.0 * y, lambda x, y, t, u, p: 100.0 * y,
+           lambda x, y, t, u, p: 100.0 + 0.0 * y, lambda x, y, t, u, p: 100.0 * y)
+    BCs = FVM.BoundaryConditions(mesh, bcs, (FVM.Dirichlet,) * 4)
+    line = np.nonzero((tri.x == 0.5) & (tri.y <= 0.4))[0]
+    ICs = FVM.InternalConditions((lambda x, y, t, u, p: 0.0 * y,), dirichlet_nodes={int(v): 0 for v in line})
+    u0 = np.where((tri.x == 0.5) & (tri.y <= 0.4), 0.0, 100 * tri.y)
+    prob = FVM.FVMProblem(mesh, BCs, ICs, diffusion_function=lambda x, y, t, u, p: 1.0 + 0.0 * u,
+                          initial_condition=u0, final_time=np.inf)
+    s1 = FVM.solve(FVM.SteadyFVMProblem(prob))
+    assert s1.retcode == "Success", s1.stats
+    tmpl = FVM.LaplacesEquation(mesh, BCs, ICs, diffusion_function=lambda x, y, p: 1.0)
+    s2 = FVM.solve(tmpl, rtol=1e-12)
+    u1, u2 = s1.u[-1], s2.u
+    print("laplace steady vs template", np.max(np.abs(u1 - u2)))
+    assert np.max(np.abs(u1 - u2)) <= 1e-5
+    assert np.all(u1[line] == 0.0)
+    assert u1.min() >= -1e-9 and u1.max() <= 100 + 1e-9
