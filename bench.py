@@ -250,6 +250,47 @@ def solve_bench(prob, rhs_ms, tf=4e-6):
     return out
 
 
+def published_walkthroughs():
+    """The reference's own @btime'd template walkthroughs, run end to end through solve()
+    (best of 3 after one warm-up), beside the published times (their CPU is unstated:
+    context, not the metric).  diffusion_equations.jl:327-331 (50 x 50 plate on [0, 2]^2,
+    D = 1/9, Dirichlet 0, Tsit5, saveat 0.05: 5.736 ms) and :343-370, 442-446 (100 x 100
+    on [0, 320]^2, D = 2, Neumann, u0 = 1 on 0.4 L <= y <= 0.6 L, Tsit5 to 500, saveat 100:
+    78.761 ms)."""
+    import finitevolumemethod as FVM
+
+    def best(fn):
+        fn()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return round(1e3 * min(ts), 2)
+
+    out = []
+    tri = FVM.triangulate_rectangle(0.0, 2.0, 0.0, 2.0, 50, 50, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    plate = FVM.DiffusionEquation(mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet),
+                                  diffusion_function=lambda x, y, p: 1 / 9,
+                                  initial_condition=np.where(tri.y <= 1.0, 50.0, 0.0), final_time=0.5)
+    out.append(dict(workload="DiffusionEquation 50x50 plate, Tsit5, saveat 0.05", ms=best(
+        lambda: FVM.solve(plate, FVM.Tsit5(), saveat=0.05)), published_ms=5.736,
+        source="docs/src/literate_wyos/diffusion_equations.jl:327-331"))
+    L = 320.0
+    tri = FVM.triangulate_rectangle(0.0, L, 0.0, L, 100, 100, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    big = FVM.DiffusionEquation(mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 2.0, FVM.Neumann),
+                                diffusion_function=lambda x, y, p: 2.0,
+                                initial_condition=np.where((0.4 * L <= tri.y) & (tri.y <= 0.6 * L), 1.0, 0.0),
+                                final_time=500.0)
+    out.append(dict(workload="DiffusionEquation 100x100 on [0,320]^2, Neumann, Tsit5, saveat 100", ms=best(
+        lambda: FVM.solve(big, FVM.Tsit5(), saveat=100.0)), published_ms=78.761,
+        source="docs/src/literate_wyos/diffusion_equations.jl:442-446"))
+    log(f"published walkthroughs {out}")
+    return out
+
+
 def ensemble_bench(P=256, steps=1000, graph_steps=100):
     """The ensemble workload: P rectangles [0, 2]^2 with nx, ny ~ round(exp(U[log 20,
     log 300])) drawn independently, interior points jittered by U(-h/4, h/4), constant
@@ -502,6 +543,7 @@ def main():
         out["vjp_rhs"] = vjp_ratio(args.vjp_side)
         log(f"VJP/RHS {out['vjp_rhs']}")
         out["solve_tsit5"] = solve_bench(prob, ms_step)
+        out["published_walkthroughs"] = published_walkthroughs()
         out["system_rhs"] = system_bench(mesh)
         log(f"system RHS {out['system_rhs']}")
         if args.template_side > 0:

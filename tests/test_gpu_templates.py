@@ -321,3 +321,26 @@ def test_template_argument_errors(gpu):
     p = FVM.PoissonsEquation(mesh, BCs2, source_function=lambda x, y, p: 1.0)
     with pytest.raises(TypeError):
         p.problem.solve(object())
+
+
+def test_large_diffusion_template_matches_fvm_problem(gpu):
+    """diffusion_equations.jl:343-452: the 100 x 100 DiffusionEquation on [0, 320]^2
+    (D = 2, Neumann 2 in the template's convention, -4 as the FVMProblem's q . n) and
+    the equivalent FVMProblem, both with Tsit5 to t = 500, saveat 100: the reference
+    asserts rtol 1e-3 between them; here they agree to 1e-8."""
+    L = 320.0
+    tri = FVM.triangulate_rectangle(0.0, L, 0.0, L, 100, 100, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    u0 = np.where((0.4 * L <= tri.y) & (tri.y <= 0.6 * L), 1.0, 0.0)
+    tm = FVM.DiffusionEquation(mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 2.0, FVM.Neumann),
+                               diffusion_function=lambda x, y, p: 2.0, initial_condition=u0, final_time=500.0)
+    fp = FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: -4.0 + 0.0 * u, FVM.Neumann),
+                        diffusion_function=lambda x, y, t, u, p: 2.0 + 0.0 * u, initial_condition=u0,
+                        final_time=500.0)
+    a = FVM.solve(tm, FVM.Tsit5(), saveat=100.0)
+    b = FVM.solve(fp, FVM.Tsit5(), saveat=100.0)
+    assert a.retcode == b.retcode == "Success" and len(a.u) == len(b.u) == 6
+    ua = np.array([u[:-1] for u in a.u[1:]])
+    ub = np.array(b.u[1:])
+    assert np.all(np.array([u[-1] for u in a.u]) == 1.0)
+    assert np.max(np.abs(ua - ub)) <= 1e-8 * np.max(np.abs(ub))
