@@ -256,7 +256,9 @@ def published_walkthroughs():
     context, not the metric).  diffusion_equations.jl:327-331 (50 x 50 plate on [0, 2]^2,
     D = 1/9, Dirichlet 0, Tsit5, saveat 0.05: 5.736 ms) and :343-370, 442-446 (100 x 100
     on [0, 320]^2, D = 2, Neumann, u0 = 1 on 0.4 L <= y <= 0.6 L, Tsit5 to 500, saveat 100:
-    78.761 ms)."""
+    78.761 ms); poissons_equation.jl:92-95, 123-130, 168-180 (laplace u = -sin(pi x) sin(pi y) on
+    a 100 x 100 grid of [0, 1]^2, u = 0 on the boundary, as the PoissonsEquation template:
+    15.442 ms with KLU, and as a SteadyFVMProblem: 189.406 ms)."""
     import finitevolumemethod as FVM
 
     def best(fn):
@@ -287,6 +289,21 @@ def published_walkthroughs():
     out.append(dict(workload="DiffusionEquation 100x100 on [0,320]^2, Neumann, Tsit5, saveat 100", ms=best(
         lambda: FVM.solve(big, FVM.Tsit5(), saveat=100.0)), published_ms=78.761,
         source="docs/src/literate_wyos/diffusion_equations.jl:442-446"))
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 100, 100, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * x, FVM.Dirichlet)
+    src = lambda x, y, p: -np.sin(np.pi * x) * np.sin(np.pi * y)  # noqa: E731
+    pois = FVM.PoissonsEquation(mesh, BCs, source_function=src)
+    out.append(dict(workload="PoissonsEquation 100x100 on [0,1]^2, device CG to 1e-10 (published: KLU)", ms=best(
+        lambda: FVM.solve(pois)), published_ms=15.442,
+        source="docs/src/literate_wyos/poissons_equation.jl:168-172"))
+    steady = FVM.SteadyFVMProblem(FVM.FVMProblem(
+        mesh, BCs, diffusion_function=lambda x, y, t, u, p: 1.0 + 0.0 * u,
+        source_function=lambda x, y, t, u, p: -src(x, y, None) + 0.0 * u,
+        initial_condition=np.zeros(tri.num_points()), final_time=np.inf))
+    out.append(dict(workload="the same Poisson problem as a SteadyFVMProblem, device Newton-GMRES "
+                    "(published: DynamicSS(TRBDF2(KLU)))", ms=best(lambda: FVM.solve(steady)), published_ms=189.406,
+                    source="docs/src/literate_wyos/poissons_equation.jl:176-180"))
     log(f"published walkthroughs {out}")
     return out
 

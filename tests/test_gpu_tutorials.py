@@ -154,3 +154,25 @@ def test_laplace_internal_dirichlet_steady_equals_template(gpu):
     assert np.max(np.abs(u1 - u2)) <= 1e-5
     assert np.all(u1[line] == 0.0)
     assert u1.min() >= -1e-9 and u1.max() <= 100 + 1e-9
+
+
+def test_poisson_template_equals_steady_problem(gpu):
+    """poissons_equation.jl:92-163: laplace u = -sin(pi x) sin(pi y) on a 100 x 100 grid
+    of [0, 1]^2 with u = 0 on the boundary, solved by the PoissonsEquation template and as
+    a SteadyFVMProblem (the source's sign flipped, :119-130): both are within rtol 1e-4 of
+    sin(pi x) sin(pi y) / (2 pi^2) and of each other (the tutorial's two @test lines)."""
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 100, 100, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * x, FVM.Dirichlet)
+    src = lambda x, y, p: -np.sin(np.pi * x) * np.sin(np.pi * y)  # noqa: E731
+    sol = FVM.solve(FVM.PoissonsEquation(mesh, BCs, source_function=src))
+    fvm_sol = FVM.solve(FVM.SteadyFVMProblem(FVM.FVMProblem(
+        mesh, BCs, diffusion_function=lambda x, y, t, u, p: 1.0 + 0.0 * u,
+        source_function=lambda x, y, t, u, p: -src(x, y, p) + 0.0 * u,
+        initial_condition=np.zeros(tri.num_points()), final_time=np.inf)))
+    assert sol.retcode == "Success" and fvm_sol.retcode == "Success", fvm_sol.stats
+    exact = np.sin(np.pi * tri.x) * np.sin(np.pi * tri.y) / (2 * np.pi ** 2)
+    u1, u2 = sol.u, fvm_sol.u[-1]
+    print("poisson template vs steady", np.linalg.norm(u1 - u2) / np.linalg.norm(u1), fvm_sol.stats)
+    assert np.linalg.norm(u1 - exact) <= 1e-4 * np.linalg.norm(exact)
+    assert np.linalg.norm(u2 - u1) <= 1e-4 * np.linalg.norm(u1)
