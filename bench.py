@@ -119,8 +119,9 @@ def vjp_ratio(n_side, reps=20):
 
 
 def _solve_all(dt, rtol, tag):
-    """Each device solver on one template operator (after one warm-up solve):
-    (per-solver stats, host solutions)."""
+    """Each device solver on one template operator (after one warm-up solve), CG with
+    its direction kernel on the ELL copy of M (the default) and, for comparison, on
+    CSR: (per-solver stats, host solutions)."""
     from finitevolumemethod import templates as TM
 
     out, sols = {}, {}
@@ -137,6 +138,16 @@ def _solve_all(dt, rtol, tag):
     out["max_rel_diff_vs_cg"] = {k: float(np.linalg.norm(v - ref) / np.linalg.norm(ref))
                                  for k, v in sols.items() if k != "cg_fp64"}
     out["mixed_speedup_vs_cg"] = round(out["cg_fp64"]["seconds"] / out["mixed_fp32_fp64"]["seconds"], 3)
+    # the CG direction kernel on the CSR form of M instead of its ELL copy
+    csr = {}
+    for name, alg in (("cg_fp64", TM.CG()), ("mixed_fp32_fp64", TM.MixedCG())):
+        dt.solve(alg, rtol=rtol, spmv_format="csr")
+        x, st = dt.solve(alg, rtol=rtol, spmv_format="csr")
+        csr[name] = dict(seconds=round(st["seconds"], 4), iterations=st["iterations"], converged=st["converged"],
+                         ell_speedup=round(st["seconds"] / out[name]["seconds"], 3))
+        log(f"template {tag} {name} (CSR form): {st}")
+    csr["mixed_speedup_vs_cg"] = round(csr["cg_fp64"]["seconds"] / csr["mixed_fp32_fp64"]["seconds"], 3)
+    out["csr_form"] = csr
     return out, sols
 
 

@@ -46,6 +46,7 @@ constexpr int kMaxG = 1024;              // workgroups of the solver kernels (fi
 constexpr int kLPR = 8;                  // lanes per CSR row in the SpMV kernels
 constexpr int kRowsPerBlock = kRT / kLPR;
 constexpr int kRPG = FVM_SPMV_ROWS;      // rows per lane group per step of the SpMV loops
+constexpr int kEllMin = 4, kEllMax = 10; // ELL slots per row: the fewest covering 99 % of the rows
 
 // solver scalars (device doubles)
 // (S_RZ, S_ALPHA, S_RHO and S_OMEGA take two slots each: value of an even / odd iteration)
@@ -78,6 +79,21 @@ struct fvm_template {
   double* part = nullptr;    // [PT_COUNT * kMaxG] reduction partials
   double* sc = nullptr;      // [S_COUNT] solver scalars
   int G = 1;
+  // ELL copy of M for the CG direction kernel (ellW = 0: not built): ellW
+  // off-diagonal slots per row stored slot-major, [s][n], padded with the row's own
+  // index and a zero value; further off-diagonals of a row in ovf_rp / ovf_col; the
+  // diagonal slot's value (FP32: the row sum of the difference form) in diagM*.
+  int ellW = 0;
+  bool use_ell = false;      // this solve's choice (fvm_solve_opts.spmv_format)
+  int32_t* ell_col = nullptr;
+  double* ellM = nullptr;
+  float* ellM32 = nullptr;
+  double* diagM = nullptr;
+  float* diagM32 = nullptr;
+  int32_t* ovf_rp = nullptr;
+  int32_t* ovf_col = nullptr;
+  double* ovfM = nullptr;
+  float* ovfM32 = nullptr;
 };
 
 namespace {
@@ -421,14 +437,13 @@ __global__ __launch_bounds__(kRT) void cg_init(int64_t n, const V* __restrict__ 
 // orthogonal to p_old, as CG assumes, so p_old first loses its component along z
 // (proj 1): p_old - g z with g = r.p_old / r.z, i.e. p = (1 - beta g) z + beta p_old;
 // or the direction restarts, p = z (proj 2).
+// The stop test and the direction coefficients shared by both forms of the
+// direction kernel: false when the solve has stopped (the kernel returns),
+// otherwise p = va z + vb p_old.
 template <class V>
-__global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, int proj, double irt2, double tabs2,
-                                                   const int32_t* __restrict__ rp,
-                                                   const int32_t* __restrict__ col, const V* __restrict__ val,
-                                                   const V* __restrict__ z, const V* __restrict__ pold,
-                                                   V* __restrict__ pnew, V* __restrict__ q, double* sc,
-                                                   double* part) {
-  if (wg_done(sc)) return;  // launches after the stop leave S_ITER at the stopping iteration
+__device__ __forceinline__ bool cg_dir_coeffs(int k, int proj, double irt2, double tabs2, double* sc,
+                                              double* part, V* va, V* vb) {
+  if (wg_done(sc)) return false;  // launches after the stop leave S_ITER at the stopping iteration
   const double rz = total_of(part + PT_RZ * kMaxG);
   const double wr = total_of(part + PT_WR * kMaxG);
   const double tol2 = k == 0 ? fmax(irt2 * wr, tabs2) : sc[S_TOL2];
@@ -439,7 +454,7 @@ __global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, int proj, d
       sc[S_ITER] = (double)k;
       sc[S_WR] = wr;
     }
-    return;
+    return false;
   }
   const double beta = k == 0 || proj == 2 ? 0.0 : rz / sc[S_RZ + ((k - 1) & 1)];
   if (lead) {
@@ -447,8 +462,20 @@ __global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, int proj, d
     sc[S_WR] = wr;
     if (k == 0) sc[S_TOL2] = tol2;
   }
-  const V vb = (V)beta;
-  const V va = proj == 1 ? (V)(1.0 - beta * (total_of(part + PT_TT * kMaxG) / rz)) : (V)1;
+  *vb = (V)beta;
+  *va = proj == 1 ? (V)(1.0 - beta * (total_of(part + PT_TT * kMaxG) / rz)) : (V)1;
+  return true;
+}
+
+template <class V>
+__global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, int proj, double irt2, double tabs2,
+                                                   const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ col, const V* __restrict__ val,
+                                                   const V* __restrict__ z, const V* __restrict__ pold,
+                                                   V* __restrict__ pnew, V* __restrict__ q, double* sc,
+                                                   double* part) {
+  V va, vb;
+  if (!cg_dir_coeffs<V>(k, proj, irt2, tabs2, sc, part, &va, &vb)) return;
   double pq = 0.0;
   auto dir = [&](int c) { return va * z[c] + vb * pold[c]; };
   for_rows<kDiffForm<V>>(n, rp, col, val, dir, [&](int64_t i, V qi, int lane) {
@@ -460,6 +487,57 @@ __global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, int proj, d
     }
   });
   store_partial(pq, part + PT_PQ * kMaxG);
+}
+
+// The same iteration on the ELL copy of M, one row per thread: the W column /
+// value pairs of a row are coalesced loads issued together, then the W gathers of
+// the direction, so a row costs two dependent loads instead of the CSR form's
+// three (row start, entries, gathers) and every lane works.  Off-diagonals are
+// summed in CSR order, then the row's overflow entries, then the diagonal term.
+template <class V, int W>
+__global__ __launch_bounds__(kRT) void cg_dir_ell(int64_t n, int k, int proj, double irt2, double tabs2,
+                                                  const int32_t* __restrict__ ecol, const V* __restrict__ eval,
+                                                  const V* __restrict__ diag, const int32_t* __restrict__ orp,
+                                                  const int32_t* __restrict__ ocol, const V* __restrict__ oval,
+                                                  const V* __restrict__ z, const V* __restrict__ pold,
+                                                  V* __restrict__ pnew, V* __restrict__ q, double* sc,
+                                                  double* part) {
+  V va, vb;
+  if (!cg_dir_coeffs<V>(k, proj, irt2, tabs2, sc, part, &va, &vb)) return;
+  constexpr bool DIFF = kDiffForm<V>;
+  double pq = 0.0;
+  auto dir = [&](int c) { return va * z[c] + vb * pold[c]; };
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+    int c[W];
+    V v[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+      c[s] = ecol[s * n + i];
+      v[s] = eval[s * n + i];
+    }
+    const int o0 = orp[i], o1 = orp[i + 1];
+    const V pi = dir((int)i);
+    V acc = 0;
+#pragma unroll
+    for (int s = 0; s < W; ++s) acc += v[s] * (DIFF ? dir(c[s]) - pi : dir(c[s]));
+    for (int kk = o0; kk < o1; ++kk) acc += oval[kk] * (DIFF ? dir(ocol[kk]) - pi : dir(ocol[kk]));
+    acc += diag[i] * pi;
+    pnew[i] = pi;
+    q[i] = acc;
+    pq += (double)pi * (double)acc;
+  }
+  store_partial(pq, part + PT_PQ * kMaxG);
+}
+
+// out[j] = src[j] >= 0 ? v[src[j]] : 0 in both precisions (the ELL values from M)
+__global__ __launch_bounds__(256) void gather_vals(int64_t m, const int32_t* __restrict__ src,
+                                                   const double* __restrict__ v64, const float* __restrict__ v32,
+                                                   double* __restrict__ o64, float* __restrict__ o32) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
+    const int k = src[j];
+    o64[j] = k >= 0 ? v64[k] : 0.0;
+    o32[j] = k >= 0 ? v32[k] : 0.0f;
+  }
 }
 
 // Iteration k, part 2: alpha = rz_k / (p.q); d += alpha p (or, given x64, the
@@ -724,7 +802,8 @@ int dupload(T** dst, const T* src, int64_t n) {
 void free_tmpl(fvm_template* t) {
   if (!t) return;
   void* p[] = {t->row_ptr, t->col, t->val, t->valM, t->valM32, t->b, t->fixed, t->vol, t->dinv,
-               t->wv, t->dinv32, t->wv32, t->work, t->part, t->sc};
+               t->wv, t->dinv32, t->wv32, t->work, t->part, t->sc, t->ell_col, t->ellM,
+               t->ellM32, t->diagM, t->diagM32, t->ovf_rp, t->ovf_col, t->ovfM, t->ovfM32};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete t;
@@ -738,6 +817,116 @@ int read_scalars(const fvm_template* t, double* h, hipStream_t s) {
 }
 
 inline dim3 grid(const fvm_template* t) { return dim3((unsigned)t->G); }
+
+template <class V>
+struct EllVals;
+template <>
+struct EllVals<double> {
+  static const double* ell(const fvm_template* t) { return t->ellM; }
+  static const double* diag(const fvm_template* t) { return t->diagM; }
+  static const double* ovf(const fvm_template* t) { return t->ovfM; }
+};
+template <>
+struct EllVals<float> {
+  static const float* ell(const fvm_template* t) { return t->ellM32; }
+  static const float* diag(const fvm_template* t) { return t->diagM32; }
+  static const float* ovf(const fvm_template* t) { return t->ovfM32; }
+};
+
+template <class V, int W>
+void launch_ell(const fvm_template* t, int k, int proj, double irt2, double tabs2, const V* z, const V* pold,
+                V* pnew, V* q, hipStream_t s) {
+  hipLaunchKernelGGL((cg_dir_ell<V, W>), grid(t), dim3(kRT), 0, s, t->n, k, proj, irt2, tabs2, t->ell_col,
+                     EllVals<V>::ell(t), EllVals<V>::diag(t), t->ovf_rp, t->ovf_col, EllVals<V>::ovf(t), z, pold,
+                     pnew, q, t->sc, t->part);
+}
+
+// One CG direction step (p, q = M p, partials of p.q) on this solve's form of M;
+// valM is the CSR value array of the precision V.
+template <class V>
+void launch_dir(const fvm_template* t, const V* valM, int k, int proj, double irt2, double tabs2, const V* z,
+                const V* pold, V* pnew, V* q, hipStream_t s) {
+  if (!t->use_ell) {
+    hipLaunchKernelGGL((cg_dir_spmv<V>), grid(t), dim3(kRT), 0, s, t->n, k, proj, irt2, tabs2, t->row_ptr,
+                       t->col, valM, z, pold, pnew, q, t->sc, t->part);
+    return;
+  }
+  static_assert(kEllMin == 4 && kEllMax == 10, "one case per ELL width");
+  switch (t->ellW) {
+    case 4: launch_ell<V, 4>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
+    case 5: launch_ell<V, 5>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
+    case 6: launch_ell<V, 6>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
+    case 7: launch_ell<V, 7>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
+    case 8: launch_ell<V, 8>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
+    case 9: launch_ell<V, 9>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
+    default: launch_ell<V, 10>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
+  }
+}
+
+// The ELL copy of M (values gathered from valM / valM32 after their assembly).
+int build_ell(fvm_template* t, const std::vector<int32_t>& rp, const std::vector<int32_t>& col) {
+  const int64_t N = t->n;
+  std::vector<int64_t> over(kEllMax + 2, 0);  // over[w]: rows with more than w off-diagonals
+  for (int64_t i = 0; i < N; ++i) {
+    int off = 0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) off += col[k] != i;
+    for (int w = kEllMin; w <= kEllMax; ++w) over[w] += off > w;
+  }
+  int W = kEllMax;
+  for (int w = kEllMin; w <= kEllMax; ++w)
+    if (over[w] * 100 <= N) {
+      W = w;
+      break;
+    }
+  std::vector<int32_t> ecol((size_t)W * N), esrc((size_t)W * N), dsrc(N, -1), orp(N + 1, 0), ocol, osrc;
+  for (int64_t i = 0; i < N; ++i) {
+    int sl = 0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      if (col[k] == i) {
+        dsrc[i] = k;
+      } else if (sl < W) {
+        ecol[(size_t)sl * N + i] = col[k];
+        esrc[(size_t)sl * N + i] = k;
+        ++sl;
+      } else {
+        ocol.push_back(col[k]);
+        osrc.push_back(k);
+      }
+    }
+    for (; sl < W; ++sl) {
+      ecol[(size_t)sl * N + i] = (int32_t)i;
+      esrc[(size_t)sl * N + i] = -1;
+    }
+    orp[i + 1] = (int32_t)ocol.size();
+  }
+  const int64_t no = (int64_t)ocol.size();
+  int32_t *d_esrc = nullptr, *d_dsrc = nullptr, *d_osrc = nullptr;
+  int rc = FVM_OK;
+  do {
+    if ((rc = dupload(&t->ell_col, ecol.data(), (int64_t)W * N)) || (rc = dupload(&t->ovf_rp, orp.data(), N + 1)) ||
+        (rc = dupload(&t->ovf_col, ocol.data(), no)))
+      break;
+    if ((rc = dalloc(&t->ellM, (int64_t)W * N)) || (rc = dalloc(&t->ellM32, (int64_t)W * N)) ||
+        (rc = dalloc(&t->diagM, N)) || (rc = dalloc(&t->diagM32, N)) || (rc = dalloc(&t->ovfM, no)) ||
+        (rc = dalloc(&t->ovfM32, no)))
+      break;
+    if ((rc = dupload(&d_esrc, esrc.data(), (int64_t)W * N)) || (rc = dupload(&d_dsrc, dsrc.data(), N)) ||
+        (rc = dupload(&d_osrc, osrc.data(), no)))
+      break;
+    auto g = [](int64_t m) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, 4096))); };
+    hipLaunchKernelGGL(gather_vals, g((int64_t)W * N), dim3(256), 0, 0, (int64_t)W * N, d_esrc, t->valM, t->valM32,
+                       t->ellM, t->ellM32);
+    hipLaunchKernelGGL(gather_vals, g(N), dim3(256), 0, 0, N, d_dsrc, t->valM, t->valM32, t->diagM, t->diagM32);
+    if (no) hipLaunchKernelGGL(gather_vals, g(no), dim3(256), 0, 0, no, d_osrc, t->valM, t->valM32, t->ovfM, t->ovfM32);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) rc = hip_fail(e, "fvm_template_create: ELL copy");
+  } while (false);
+  for (void* p : {(void*)d_esrc, (void*)d_dsrc, (void*)d_osrc})
+    if (p) (void)hipFree(p);
+  if (rc == FVM_OK) t->ellW = W;
+  return rc;
+}
 
 // |b - A x|^2 into sc[S_RES2] (and r = -V (b - A x) on free rows when r is given)
 template <class V>
@@ -765,8 +954,7 @@ int inner_cg(fvm_template* t, const V* valM, const V* dinv, const V* wv, V* d, V
     for (int j = 0; j < batch; ++j, ++k) {
       V* pold = pp[k & 1];
       V* pnew = pp[(k + 1) & 1];
-      hipLaunchKernelGGL((cg_dir_spmv<V>), grid(t), dim3(kRT), 0, s, n, k, 0, irt2, tabs2, t->row_ptr, t->col,
-                         valM, z, pold, pnew, q, t->sc, t->part);
+      launch_dir<V>(t, valM, k, 0, irt2, tabs2, z, pold, pnew, q, s);
       hipLaunchKernelGGL((cg_update<V>), grid(t), dim3(kRT), 0, s, n, k, d, (double*)nullptr, r, pnew, q, dinv, wv,
                          z, t->sc, t->part);
     }
@@ -807,8 +995,7 @@ int reliable_cg(fvm_template* t, double* x, double target, double delta, int max
   for (;;) {
     const int batch = std::min(check, maxit - k);
     for (int j = 0; j < batch; ++j, ++k) {
-      hipLaunchKernelGGL((cg_dir_spmv<float>), grid(t), dim3(kRT), 0, s, n, k, j == 0 ? proj : 0, 0.0, tabs2,
-                         t->row_ptr, t->col, t->valM32, z, pp[k & 1], pp[(k + 1) & 1], q, t->sc, t->part);
+      launch_dir<float>(t, t->valM32, k, j == 0 ? proj : 0, 0.0, tabs2, z, pp[k & 1], pp[(k + 1) & 1], q, s);
       hipLaunchKernelGGL((cg_update<float>), grid(t), dim3(kRT), 0, s, n, k, d, x, r, pp[(k + 1) & 1], q,
                          t->dinv32, t->wv32, z, t->sc, t->part);
     }
@@ -983,7 +1170,11 @@ int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_templ
     hipLaunchKernelGGL(finish_sum, dim3(1), dim3(kRT), 0, 0, t->G, t->part, t->sc, (int)S_B2);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) rc = hip_fail(e, "fvm_template_create: assembly");
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "fvm_template_create: assembly");
+      break;
+    }
+    rc = build_ell(t, rp, col);
   } while (false);
   void* tmp[] = {d_nbs, d_nb, d_kind, d_neu, d_de, d_dbe, d_f, d_r, d_dir, d_dudt, d_na};
   for (void* p : tmp)
@@ -1062,6 +1253,10 @@ int fvm_template_solve(fvm_template* t, const fvm_solve_opts* o, double* d_x, fv
   FVM_CHECK(o->method >= 0 && o->method <= 3, "fvm_template_solve: unknown method");
   FVM_CHECK(o->rtol > 0 && o->maxiter > 0, "fvm_template_solve: rtol and maxiter must be positive");
   FVM_CHECK(t->kind != 2, "fvm_template_solve: a diffusion template is an ODE, not a linear system");
+  FVM_CHECK(o->spmv_format >= FVM_SPMV_AUTO && o->spmv_format <= FVM_SPMV_ELL,
+            "fvm_template_solve: unknown spmv_format");
+  FVM_CHECK(o->spmv_format != FVM_SPMV_ELL || t->ellW > 0, "fvm_template_solve: no ELL copy of this operator");
+  t->use_ell = t->ellW > 0 && o->spmv_format != FVM_SPMV_CSR;
   hipStream_t str = (hipStream_t)s;
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t n = t->n;

@@ -71,7 +71,7 @@ class TemplateDesc(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("maxiter", ctypes.c_int32),
                 ("check_every", ctypes.c_int32), ("max_outer", ctypes.c_int32),
-                ("use_x0", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("use_x0", ctypes.c_int32), ("spmv_format", ctypes.c_int32),
                 ("rtol", ctypes.c_double), ("inner_rtol", ctypes.c_double)]
 
 

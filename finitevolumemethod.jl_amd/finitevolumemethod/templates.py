@@ -93,6 +93,9 @@ def _eval_xyp(fn, p, xs, ys):
         return np.array([float(fn(float(x), float(y), p)) for x, y in zip(xs, ys)])
 
 
+_SPMV_FORMATS = {"auto": 0, "csr": 1, "ell": 2}
+
+
 class DeviceTemplate:
     """An fvm_template: A and b assembled on the GPU."""
 
@@ -185,8 +188,10 @@ class DeviceTemplate:
                    "fvm_template_residual")
         return r.value
 
-    def solve(self, alg=None, *, rtol=1e-10, maxiter=100000, x=None, max_outer=0, stream=None):
-        """Solve A x = b on the device.  Returns (x DeviceArray, fvm_solve_stats dict)."""
+    def solve(self, alg=None, *, rtol=1e-10, maxiter=100000, x=None, max_outer=0, stream=None, spmv_format="auto"):
+        """Solve A x = b on the device.  Returns (x DeviceArray, fvm_solve_stats dict).
+        spmv_format: the form of the scaled operator in the CG direction kernel,
+        "auto" (= "ell"), "ell" or "csr"."""
         alg = alg if alg is not None else CG()
         if type(alg) not in _METHOD:
             raise TypeError(f"unsupported linear solver {alg!r}; use CG(), MixedCG() or BiCGStab()")
@@ -199,6 +204,7 @@ class DeviceTemplate:
         o.max_outer = int(max_outer)
         o.rtol = float(rtol)
         o.inner_rtol = float(getattr(alg, "inner_rtol", 0.0))
+        o.spmv_format = _SPMV_FORMATS[spmv_format]
         if x is None:
             x = _lib.DeviceArray(self.n)
             o.use_x0 = 0
@@ -243,10 +249,10 @@ class AbstractFVMTemplate:
 
 
 class _SteadyTemplate(AbstractFVMTemplate):
-    def solve(self, alg=None, *, rtol=1e-10, maxiter=100000, max_outer=0):
+    def solve(self, alg=None, *, rtol=1e-10, maxiter=100000, max_outer=0, spmv_format="auto"):
         """LinearProblem(A, b) solved on the device to |b - A u| <= rtol |b|;
         returns a LinearSolution."""
-        x, st = self.problem.solve(alg, rtol=rtol, maxiter=maxiter, max_outer=max_outer)
+        x, st = self.problem.solve(alg, rtol=rtol, maxiter=maxiter, max_outer=max_outer, spmv_format=spmv_format)
         return LinearSolution(u=x.to_host(), retcode="Success" if st["converged"] else "MaxIters",
                               resid=st["residual"], iters=st["iterations"], stats=st)
 

@@ -344,3 +344,53 @@ def test_large_diffusion_template_matches_fvm_problem(gpu):
     ub = np.array(b.u[1:])
     assert np.all(np.array([u[-1] for u in a.u]) == 1.0)
     assert np.max(np.abs(ua - ub)) <= 1e-8 * np.max(np.abs(ub))
+
+
+def wheel_triangulation(k=40):
+    """A centre point inside a ring of k points (its row has k off-diagonals: the
+    ELL copy keeps the slots beyond its width in the overflow list), then rings of
+    40, 60 and 80 points out to radius 2."""
+    pts = [(0.0, 0.0)]
+    for m, r in ((k, 0.5), (40, 1.0), (60, 1.5), (80, 2.0)):
+        th = 2 * np.pi * np.arange(m) / m + 0.05 * r
+        pts += list(zip(r * np.cos(th), r * np.sin(th)))
+    P = np.array(pts)
+    return FVM.triangulate(P[:, 0], P[:, 1])
+
+
+@pytest.mark.parametrize("mesh", ["jittered", "wheel"])
+@pytest.mark.parametrize("alg", ALGS[:3], ids=IDS[:3])
+def test_ell_and_csr_forms_of_the_cg_operator_agree(gpu, mesh, alg):
+    """The CG direction kernel on the ELL copy of M (one thread per row, overflow
+    list for long rows) and on CSR: both converge to the direct solution of the
+    same A u = b (1e-8 relative), and their iteration counts agree within a few
+    percent (the two forms round the products' sums differently)."""
+    if mesh == "wheel":
+        tri = wheel_triangulation()
+        rp, _ = FVM.equations.csr_pattern(tri)
+        assert np.diff(rp).max() - 1 == 40
+    else:
+        tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 90, 80, jitter=0.25, seed=21)
+    m = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(m, lambda x, y, t, u, p: 0.0 * x, FVM.Dirichlet)
+    prob = FVM.PoissonsEquation(m, BCs, diffusion_function=lambda x, y, p: 1.0 + 0.5 * np.sin(3 * x) ** 2,
+                                source_function=lambda x, y, p: np.cos(2 * x) - y)
+    direct = spla.spsolve(prob.A.tocsc(), prob.b)
+    sols = {f: FVM.solve(prob, alg, rtol=1e-11, spmv_format=f) for f in ("ell", "csr")}
+    for f, s in sols.items():
+        assert s.retcode == "Success", (f, s.stats)
+        assert np.linalg.norm(s.u - direct) <= 1e-8 * np.linalg.norm(direct), f
+    it_e, it_c = sols["ell"].stats["iterations"], sols["csr"].stats["iterations"]
+    assert abs(it_e - it_c) <= 0.05 * it_c + 3, (it_e, it_c)
+    # the default form is the ELL one, and both are deterministic
+    again = FVM.solve(prob, alg, rtol=1e-11)
+    assert np.array_equal(again.u, sols["ell"].u)
+
+
+def test_unknown_spmv_format_is_rejected(gpu):
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 8, 8, single_boundary=True)
+    m = FVM.FVMGeometry(tri)
+    prob = FVM.PoissonsEquation(m, FVM.BoundaryConditions(m, lambda x, y, t, u, p: 0.0 * x, FVM.Dirichlet),
+                                source_function=lambda x, y, p: 1.0)
+    with pytest.raises(KeyError):
+        FVM.solve(prob, spmv_format="dense")
