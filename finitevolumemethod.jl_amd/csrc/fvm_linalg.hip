@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "fvm_internal.hpp"
@@ -489,11 +490,37 @@ __global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, int proj, d
   store_partial(pq, part + PT_PQ * kMaxG);
 }
 
-// The same iteration on the ELL copy of M, one row per thread: the W column /
-// value pairs of a row are coalesced loads issued together, then the W gathers of
-// the direction, so a row costs two dependent loads instead of the CSR form's
-// three (row start, entries, gathers) and every lane works.  Off-diagonals are
-// summed in CSR order, then the row's overflow entries, then the diagonal term.
+// Rows of the ELL copy of M, one per thread: the W column / value pairs of a row
+// are coalesced loads issued together, then the W gathers, so a row costs two
+// dependent loads instead of the CSR form's three (row start, entries, gathers)
+// and every lane works.  body(i, g(i), y_i) with y_i = the off-diagonal terms in
+// CSR order (v (g(c) - g(i)) in the difference form, v g(c) otherwise), then the
+// row's overflow entries, then diag_i g(i).
+template <bool DIFF, int W, class V, class Gather, class Body>
+__device__ __forceinline__ void ell_rows(int64_t n, const int32_t* __restrict__ ecol, const V* __restrict__ eval,
+                                         const V* __restrict__ diag, const int32_t* __restrict__ orp,
+                                         const int32_t* __restrict__ ocol, const V* __restrict__ oval, Gather g,
+                                         Body body) {
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+    int c[W];
+    V v[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+      c[s] = ecol[s * n + i];
+      v[s] = eval[s * n + i];
+    }
+    const int o0 = orp[i], o1 = orp[i + 1];
+    const V gi = g((int)i);
+    V acc = 0;
+#pragma unroll
+    for (int s = 0; s < W; ++s) acc += v[s] * (DIFF ? g(c[s]) - gi : g(c[s]));
+    for (int kk = o0; kk < o1; ++kk) acc += oval[kk] * (DIFF ? g(ocol[kk]) - gi : g(ocol[kk]));
+    acc += diag[i] * gi;
+    body(i, gi, acc);
+  }
+}
+
+// The CG direction iteration on the ELL copy of M.
 template <class V, int W>
 __global__ __launch_bounds__(kRT) void cg_dir_ell(int64_t n, int k, int proj, double irt2, double tabs2,
                                                   const int32_t* __restrict__ ecol, const V* __restrict__ eval,
@@ -504,28 +531,13 @@ __global__ __launch_bounds__(kRT) void cg_dir_ell(int64_t n, int k, int proj, do
                                                   double* part) {
   V va, vb;
   if (!cg_dir_coeffs<V>(k, proj, irt2, tabs2, sc, part, &va, &vb)) return;
-  constexpr bool DIFF = kDiffForm<V>;
   double pq = 0.0;
   auto dir = [&](int c) { return va * z[c] + vb * pold[c]; };
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
-    int c[W];
-    V v[W];
-#pragma unroll
-    for (int s = 0; s < W; ++s) {
-      c[s] = ecol[s * n + i];
-      v[s] = eval[s * n + i];
-    }
-    const int o0 = orp[i], o1 = orp[i + 1];
-    const V pi = dir((int)i);
-    V acc = 0;
-#pragma unroll
-    for (int s = 0; s < W; ++s) acc += v[s] * (DIFF ? dir(c[s]) - pi : dir(c[s]));
-    for (int kk = o0; kk < o1; ++kk) acc += oval[kk] * (DIFF ? dir(ocol[kk]) - pi : dir(ocol[kk]));
-    acc += diag[i] * pi;
+  ell_rows<kDiffForm<V>, W>(n, ecol, eval, diag, orp, ocol, oval, dir, [&](int64_t i, V pi, V qi) {
     pnew[i] = pi;
-    q[i] = acc;
-    pq += (double)pi * (double)acc;
-  }
+    q[i] = qi;
+    pq += (double)pi * (double)qi;
+  });
   store_partial(pq, part + PT_PQ * kMaxG);
 }
 
@@ -662,6 +674,26 @@ __global__ __launch_bounds__(kRT) void bi_spmv(int64_t n, const int32_t* __restr
       if (MODE == 1) yy += yi * yi;
     }
   });
+  store_partial(ay, part + PT_PQ * kMaxG);
+  if (MODE == 1) store_partial(yy, part + PT_TT * kMaxG);
+}
+
+// bi_spmv on the ELL copy of M (FP64).
+template <int MODE, int W>
+__global__ __launch_bounds__(kRT) void bi_spmv_ell(int64_t n, const int32_t* __restrict__ ecol,
+                                                   const double* __restrict__ eval, const double* __restrict__ diag,
+                                                   const int32_t* __restrict__ orp, const int32_t* __restrict__ ocol,
+                                                   const double* __restrict__ oval, const double* __restrict__ x,
+                                                   double* __restrict__ y, const double* __restrict__ a,
+                                                   const double* sc, double* part) {
+  if (wg_done(sc)) return;
+  double ay = 0.0, yy = 0.0;
+  ell_rows<false, W>(n, ecol, eval, diag, orp, ocol, oval, [&](int c) { return x[c]; },
+                     [&](int64_t i, double, double yi) {
+                       y[i] = yi;
+                       ay += a[i] * yi;
+                       if (MODE == 1) yy += yi * yi;
+                     });
   store_partial(ay, part + PT_PQ * kMaxG);
   if (MODE == 1) store_partial(yy, part + PT_TT * kMaxG);
 }
@@ -833,12 +865,19 @@ struct EllVals<float> {
   static const float* ovf(const fvm_template* t) { return t->ovfM32; }
 };
 
-template <class V, int W>
-void launch_ell(const fvm_template* t, int k, int proj, double irt2, double tabs2, const V* z, const V* pold,
-                V* pnew, V* q, hipStream_t s) {
-  hipLaunchKernelGGL((cg_dir_ell<V, W>), grid(t), dim3(kRT), 0, s, t->n, k, proj, irt2, tabs2, t->ell_col,
-                     EllVals<V>::ell(t), EllVals<V>::diag(t), t->ovf_rp, t->ovf_col, EllVals<V>::ovf(t), z, pold,
-                     pnew, q, t->sc, t->part);
+// f(std::integral_constant<int, W>) for the template's ELL width
+template <class F>
+void by_width(int W, F&& f) {
+  static_assert(kEllMin == 4 && kEllMax == 10, "one case per ELL width");
+  switch (W) {
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 9: f(std::integral_constant<int, 9>{}); break;
+    default: f(std::integral_constant<int, 10>{}); break;
+  }
 }
 
 // One CG direction step (p, q = M p, partials of p.q) on this solve's form of M;
@@ -851,16 +890,25 @@ void launch_dir(const fvm_template* t, const V* valM, int k, int proj, double ir
                        t->col, valM, z, pold, pnew, q, t->sc, t->part);
     return;
   }
-  static_assert(kEllMin == 4 && kEllMax == 10, "one case per ELL width");
-  switch (t->ellW) {
-    case 4: launch_ell<V, 4>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
-    case 5: launch_ell<V, 5>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
-    case 6: launch_ell<V, 6>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
-    case 7: launch_ell<V, 7>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
-    case 8: launch_ell<V, 8>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
-    case 9: launch_ell<V, 9>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
-    default: launch_ell<V, 10>(t, k, proj, irt2, tabs2, z, pold, pnew, q, s); break;
+  by_width(t->ellW, [&](auto w) {
+    hipLaunchKernelGGL((cg_dir_ell<V, decltype(w)::value>), grid(t), dim3(kRT), 0, s, t->n, k, proj, irt2, tabs2,
+                       t->ell_col, EllVals<V>::ell(t), EllVals<V>::diag(t), t->ovf_rp, t->ovf_col,
+                       EllVals<V>::ovf(t), z, pold, pnew, q, t->sc, t->part);
+  });
+}
+
+// y = M x with the partials of a.y (and y.y for MODE 1), BiCGStab's products.
+template <int MODE>
+void launch_bi_spmv(const fvm_template* t, const double* x, double* y, const double* a, hipStream_t s) {
+  if (!t->use_ell) {
+    hipLaunchKernelGGL(bi_spmv<MODE>, grid(t), dim3(kRT), 0, s, t->n, t->row_ptr, t->col, t->valM, x, y, a, t->sc,
+                       t->part);
+    return;
   }
+  by_width(t->ellW, [&](auto w) {
+    hipLaunchKernelGGL((bi_spmv_ell<MODE, decltype(w)::value>), grid(t), dim3(kRT), 0, s, t->n, t->ell_col, t->ellM,
+                       t->diagM, t->ovf_rp, t->ovf_col, t->ovfM, x, y, a, t->sc, t->part);
+  });
 }
 
 // The ELL copy of M (values gathered from valM / valM32 after their assembly).
@@ -1034,9 +1082,9 @@ int inner_bicgstab(fvm_template* t, double* d, double* r, double tabs2, int maxi
     const int batch = std::min(check, maxit - k);
     for (int j = 0; j < batch; ++j, ++k) {
       hipLaunchKernelGGL(bi_dir, g, b, 0, s, n, k, tabs2, r, p, v, t->dinv, phat, t->sc, t->part);
-      hipLaunchKernelGGL(bi_spmv<0>, g, b, 0, s, n, t->row_ptr, t->col, t->valM, phat, v, rhat, t->sc, t->part);
+      launch_bi_spmv<0>(t, phat, v, rhat, s);
       hipLaunchKernelGGL(bi_s, g, b, 0, s, n, k, r, v, t->dinv, shat, t->sc, t->part);
-      hipLaunchKernelGGL(bi_spmv<1>, g, b, 0, s, n, t->row_ptr, t->col, t->valM, shat, tt, r, t->sc, t->part);
+      launch_bi_spmv<1>(t, shat, tt, r, s);
       hipLaunchKernelGGL(bi_update, g, b, 0, s, n, k, d, r, phat, shat, tt, rhat, t->wv, t->sc, t->part);
     }
     int rc = read_scalars(t, h, s);

@@ -253,7 +253,7 @@ enum {
   FVM_SOLVE_CG_RELIABLE = 3 /* one FP32 Jacobi-PCG sequence with FP64 reliable updates of x and the residual */
 };
 
-/* Forms of the scaled operator M for CG: CSR with eight lanes per row, or an ELL
+/* Forms of the scaled operator M in the Krylov products: CSR with eight lanes per row, or an ELL
  * copy built at fvm_template_create (one thread per row, W off-diagonal slots
  * stored slot-major, W the fewest of 4..10 that hold every off-diagonal of 99 % of
  * the rows, the rest in a per-row overflow list).  AUTO takes ELL. */
@@ -265,8 +265,8 @@ typedef struct fvm_solve_opts {
   int32_t check_every;  /* iterations between host reads of the device scalars (0: 32) */
   int32_t max_outer;    /* refinement / restart steps (0: 50; FVM_SOLVE_CG_RELIABLE: updates, 0: 1000) */
   int32_t use_x0;       /* start from d_x (its fixed rows are reset to b) instead of 0 */
-  int32_t spmv_format;  /* FVM_SPMV_*: the form of M in the CG direction kernel (BiCGStab and
-                           residuals always use CSR) */
+  int32_t spmv_format;  /* FVM_SPMV_*: the form of M in the CG and BiCGStab products (the
+                           FP64 residuals b - A x always use CSR) */
   double rtol;          /* stop when |b - A x|_2 <= rtol |b|_2 */
   double inner_rtol;    /* FVM_SOLVE_CG_MIXED: relative drop of each FP32 solve (0: 1e-4);
                            FVM_SOLVE_CG_RELIABLE: residual drop between updates (0: 0.1) */

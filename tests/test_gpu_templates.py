@@ -359,12 +359,12 @@ def wheel_triangulation(k=40):
 
 
 @pytest.mark.parametrize("mesh", ["jittered", "wheel"])
-@pytest.mark.parametrize("alg", ALGS[:3], ids=IDS[:3])
-def test_ell_and_csr_forms_of_the_cg_operator_agree(gpu, mesh, alg):
-    """The CG direction kernel on the ELL copy of M (one thread per row, overflow
-    list for long rows) and on CSR: both converge to the direct solution of the
-    same A u = b (1e-8 relative), and their iteration counts agree within a few
-    percent (the two forms round the products' sums differently)."""
+@pytest.mark.parametrize("alg", ALGS, ids=IDS)
+def test_ell_and_csr_forms_of_the_krylov_operator_agree(gpu, mesh, alg):
+    """The Krylov products on the ELL copy of M (one thread per row, overflow list
+    for long rows) and on CSR: both converge to the direct solution of the same
+    A u = b (1e-8 relative), and their iteration counts agree within a few percent
+    (the two forms round the products' sums differently)."""
     if mesh == "wheel":
         tri = wheel_triangulation()
         rp, _ = FVM.equations.csr_pattern(tri)
