@@ -12,7 +12,7 @@ import pytest
 import finitevolumemethod as FVM
 from oracle import geometry as oracle_geometry
 from problems import (annulus_problem, diffusion_problem, dxy_problem, heat_convection_problem,
-                      mixed_problem, nodal_problem, nonlinear_problem)
+                      mixed_problem, nodal_problem, nonlinear_problem, oracle_for)
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -198,3 +198,44 @@ def test_annulus_trajectory_matches_oracle(gpu):
     assert len(sol.u) == nsteps + 1 and sol.t[-1] == pytest.approx(t, abs=1e-15)
     np.testing.assert_allclose(sol.u[-1][inner], 50 * (1 - np.exp(-t / 2)), rtol=1e-13)
     assert np.max(np.abs(sol.u[-1] - u)) <= 1e-11 * np.max(np.abs(u))
+
+
+def test_points_outside_the_triangulation(gpu):
+    """Points that belong to no triangle (source_contributions.jl:36, has_vertex; the
+    templates' fix_missing_vertices!, abstract_templates.jl:317-325): their du is 0,
+    the other nodes' du matches the oracle at the 1e-12 bar and equals the du of the
+    same mesh without the extra points to the same bar; the Poisson template gives
+    them identity rows (solution 0) and leaves the solution elsewhere unchanged."""
+    base = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 23, 19, jitter=0.25, seed=12)
+    n0 = base.num_points()
+    extra_x, extra_y = np.array([0.3, 2.0, -1.0]), np.array([0.7, 2.0, 0.5])  # inside and outside the domain
+    ext = FVM.Triangulation(np.r_[base.x, extra_x], np.r_[base.y, extra_y], base.triangles, base.boundary_edges,
+                            base.boundary_segments)
+    rng = np.random.default_rng(3)
+    u_ext = rng.random(ext.num_points())
+    dus = {}
+    for name, tri, u in (("ext", ext, u_ext), ("base", base, u_ext[:n0])):
+        mesh = FVM.FVMGeometry(tri)
+        BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
+        prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 0.1 + u ** 2,
+                              source_function=lambda x, y, t, u, p: u * (1 - u), initial_condition=u,
+                              final_time=1.0)
+        du = FVM.fvm_eqs(np.zeros_like(u), u, prob, 0.0)
+        if name == "ext":
+            orc = oracle_for(prob, dict(builtin=dict(flux_kind=0, dpoly=[0.1, 0, 1, 0], spoly=[0, 1, -1, 0],
+                                                     fn=([0.0], [0.0], [0.0]))))
+            assert_parity(du, orc.rhs(u, 0.0), orc.scale(u, 0.0))
+            assert np.all(du[n0:] == 0.0)
+            assert np.all(mesh.cv_volumes[n0:] == 0.0)
+            scale = orc.scale(u, 0.0)[:n0]
+        dus[name] = du
+    assert_parity(dus["ext"][:n0], dus["base"], scale)
+    sols = {}
+    for name, tri in (("ext", ext), ("base", base)):
+        mesh = FVM.FVMGeometry(tri)
+        BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * x, FVM.Dirichlet)
+        prob = FVM.PoissonsEquation(mesh, BCs, source_function=lambda x, y, p: np.cos(3 * x) + y)
+        sols[name] = FVM.solve(prob, rtol=1e-12)
+        assert sols[name].retcode == "Success"
+    assert np.all(sols["ext"].u[n0:] == 0.0)
+    np.testing.assert_allclose(sols["ext"].u[:n0], sols["base"].u, rtol=0, atol=1e-10 * np.abs(sols["base"].u).max())
