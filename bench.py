@@ -182,8 +182,15 @@ def template_solve(n_side, rtol=1e-10, direct=True):
     out["mean_exit_time"], met_sols = _solve_all(met.problem, rtol, "mean_exit_time")
     x = _lib.DeviceArray.from_host(sols["cg_fp64"])
     y = _lib.DeviceArray(N)
+    W, n_over = dt.ell_info()
+    out["operator_form"] = dict(ell_slots=W, overflow_entries=n_over)
+    # bytes per product on the ELL copy: W (column, value) slots, the diagonal, the
+    # overflow start, x gathered once and y written once per row, and the overflow
+    # entries (column, value, gather)
     rows = N
-    for which, key, per_nnz, per_row in ((1, "spmv_fp64", 12, 20), (2, "spmv_fp32", 8, 12)):
+    for which, key, s in ((1, "spmv_fp64", 8), (2, "spmv_fp32", 4)):
+        per_row = W * (4 + s) + s + 4 + 8 + 8
+        per_over = 4 + s + 8
         for _ in range(3):
             dt.spmv(which, x, y)
         tm = _lib.Timer()
@@ -193,7 +200,7 @@ def template_solve(n_side, rtol=1e-10, direct=True):
             dt.spmv(which, x, y)
         ms = tm.stop() / reps
         out[key] = dict(us=round(ms * 1e3, 2),
-                        GBps=round((per_nnz * dt.nnz + per_row * rows) / (ms * 1e-3) / 1e9, 1))
+                        GBps=round((per_row * rows + per_over * n_over) / (ms * 1e-3) / 1e9, 1))
     if direct:
         import scipy.sparse.linalg as spla
 

@@ -85,6 +85,7 @@ struct fvm_template {
   // index and a zero value; further off-diagonals of a row in ovf_rp / ovf_col; the
   // diagonal slot's value (FP32: the row sum of the difference form) in diagM*.
   int ellW = 0;
+  int64_t ell_overflow = 0;
   bool use_ell = false;      // this solve's choice (fvm_solve_opts.spmv_format)
   int32_t* ell_col = nullptr;
   double* ellM = nullptr;
@@ -95,6 +96,9 @@ struct fvm_template {
   int32_t* ovf_col = nullptr;
   double* ovfM = nullptr;
   float* ovfM32 = nullptr;
+  double* ellA = nullptr;    // the same slots of A itself (fvm_template_apply / _spmv)
+  double* diagA = nullptr;
+  double* ovfA = nullptr;
 };
 
 namespace {
@@ -541,14 +545,14 @@ __global__ __launch_bounds__(kRT) void cg_dir_ell(int64_t n, int k, int proj, do
   store_partial(pq, part + PT_PQ * kMaxG);
 }
 
-// out[j] = src[j] >= 0 ? v[src[j]] : 0 in both precisions (the ELL values from M)
+// out[j] = src[j] >= 0 ? v[src[j]] : 0, in FP64 and (given o32) in FP32: the ELL values
 __global__ __launch_bounds__(256) void gather_vals(int64_t m, const int32_t* __restrict__ src,
                                                    const double* __restrict__ v64, const float* __restrict__ v32,
                                                    double* __restrict__ o64, float* __restrict__ o32) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
     const int k = src[j];
     o64[j] = k >= 0 ? v64[k] : 0.0;
-    o32[j] = k >= 0 ? v32[k] : 0.0f;
+    if (o32) o32[j] = k >= 0 ? v32[k] : 0.0f;
   }
 }
 
@@ -813,6 +817,18 @@ __global__ __launch_bounds__(kRT) void spmv_kernel(int64_t n, const int32_t* __r
   });
 }
 
+// y = A x (+ b) on an ELL copy (precision V, the difference form for FP32)
+template <class V, int W>
+__global__ __launch_bounds__(kRT) void spmv_ell_kernel(int64_t n, const int32_t* __restrict__ ecol,
+                                                       const V* __restrict__ eval, const V* __restrict__ diag,
+                                                       const int32_t* __restrict__ orp,
+                                                       const int32_t* __restrict__ ocol, const V* __restrict__ oval,
+                                                       const double* __restrict__ x, const double* __restrict__ b,
+                                                       double* __restrict__ y) {
+  ell_rows<kDiffForm<V>, W>(n, ecol, eval, diag, orp, ocol, oval, [&](int c) { return (V)x[c]; },
+                            [&](int64_t i, V, V yi) { y[i] = b ? (double)yi + b[i] : (double)yi; });
+}
+
 // ----------------------------------------------------------------------------
 // host helpers
 // ----------------------------------------------------------------------------
@@ -835,7 +851,8 @@ void free_tmpl(fvm_template* t) {
   if (!t) return;
   void* p[] = {t->row_ptr, t->col, t->val, t->valM, t->valM32, t->b, t->fixed, t->vol, t->dinv,
                t->wv, t->dinv32, t->wv32, t->work, t->part, t->sc, t->ell_col, t->ellM,
-               t->ellM32, t->diagM, t->diagM32, t->ovf_rp, t->ovf_col, t->ovfM, t->ovfM32};
+               t->ellM32, t->diagM, t->diagM32, t->ovf_rp, t->ovf_col, t->ovfM, t->ovfM32,
+               t->ellA, t->diagA, t->ovfA};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete t;
@@ -911,6 +928,33 @@ void launch_bi_spmv(const fvm_template* t, const double* x, double* y, const dou
   });
 }
 
+// y = X x (+ b) for X = A (which 0), M in FP64 (1) or M in FP32 (2), on the ELL
+// copy when there is one, else on CSR.
+void spmv_any(const fvm_template* t, int which, const double* x, const double* b, double* y, hipStream_t s) {
+  if (t->ellW > 0) {
+    by_width(t->ellW, [&](auto w) {
+      constexpr int W = decltype(w)::value;
+      if (which == 0)
+        hipLaunchKernelGGL((spmv_ell_kernel<double, W>), grid(t), dim3(kRT), 0, s, t->n, t->ell_col, t->ellA,
+                           t->diagA, t->ovf_rp, t->ovf_col, t->ovfA, x, b, y);
+      else if (which == 1)
+        hipLaunchKernelGGL((spmv_ell_kernel<double, W>), grid(t), dim3(kRT), 0, s, t->n, t->ell_col, t->ellM,
+                           t->diagM, t->ovf_rp, t->ovf_col, t->ovfM, x, b, y);
+      else
+        hipLaunchKernelGGL((spmv_ell_kernel<float, W>), grid(t), dim3(kRT), 0, s, t->n, t->ell_col, t->ellM32,
+                           t->diagM32, t->ovf_rp, t->ovf_col, t->ovfM32, x, b, y);
+    });
+    return;
+  }
+  const void* v = which == 0 ? (const void*)t->val : which == 1 ? (const void*)t->valM : (const void*)t->valM32;
+  if (which == 2)
+    hipLaunchKernelGGL((spmv_kernel<float>), grid(t), dim3(kRT), 0, s, t->n, t->row_ptr, t->col, (const float*)v, x,
+                       b, y);
+  else
+    hipLaunchKernelGGL((spmv_kernel<double>), grid(t), dim3(kRT), 0, s, t->n, t->row_ptr, t->col, (const double*)v,
+                       x, b, y);
+}
+
 // The ELL copy of M (values gathered from valM / valM32 after their assembly).
 int build_ell(fvm_template* t, const std::vector<int32_t>& rp, const std::vector<int32_t>& col) {
   const int64_t N = t->n;
@@ -958,6 +1002,8 @@ int build_ell(fvm_template* t, const std::vector<int32_t>& rp, const std::vector
         (rc = dalloc(&t->diagM, N)) || (rc = dalloc(&t->diagM32, N)) || (rc = dalloc(&t->ovfM, no)) ||
         (rc = dalloc(&t->ovfM32, no)))
       break;
+    if ((rc = dalloc(&t->ellA, (int64_t)W * N)) || (rc = dalloc(&t->diagA, N)) || (rc = dalloc(&t->ovfA, no)))
+      break;
     if ((rc = dupload(&d_esrc, esrc.data(), (int64_t)W * N)) || (rc = dupload(&d_dsrc, dsrc.data(), N)) ||
         (rc = dupload(&d_osrc, osrc.data(), no)))
       break;
@@ -966,13 +1012,23 @@ int build_ell(fvm_template* t, const std::vector<int32_t>& rp, const std::vector
                        t->ellM, t->ellM32);
     hipLaunchKernelGGL(gather_vals, g(N), dim3(256), 0, 0, N, d_dsrc, t->valM, t->valM32, t->diagM, t->diagM32);
     if (no) hipLaunchKernelGGL(gather_vals, g(no), dim3(256), 0, 0, no, d_osrc, t->valM, t->valM32, t->ovfM, t->ovfM32);
+    hipLaunchKernelGGL(gather_vals, g((int64_t)W * N), dim3(256), 0, 0, (int64_t)W * N, d_esrc, t->val,
+                       (const float*)nullptr, t->ellA, (float*)nullptr);
+    hipLaunchKernelGGL(gather_vals, g(N), dim3(256), 0, 0, N, d_dsrc, t->val, (const float*)nullptr, t->diagA,
+                       (float*)nullptr);
+    if (no)
+      hipLaunchKernelGGL(gather_vals, g(no), dim3(256), 0, 0, no, d_osrc, t->val, (const float*)nullptr, t->ovfA,
+                         (float*)nullptr);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) rc = hip_fail(e, "fvm_template_create: ELL copy");
   } while (false);
   for (void* p : {(void*)d_esrc, (void*)d_dsrc, (void*)d_osrc})
     if (p) (void)hipFree(p);
-  if (rc == FVM_OK) t->ellW = W;
+  if (rc == FVM_OK) {
+    t->ellW = W;
+    t->ell_overflow = no;
+  }
   return rc;
 }
 
@@ -1240,6 +1296,13 @@ int fvm_template_destroy(fvm_template* t) {
   return FVM_OK;
 }
 
+int fvm_template_ell(const fvm_template* t, int32_t* width, int64_t* overflow) {
+  FVM_CHECK(t && width && overflow, "fvm_template_ell: null argument");
+  *width = t->ellW;
+  *overflow = t->ell_overflow;
+  return FVM_OK;
+}
+
 int fvm_template_info(const fvm_template* t, int64_t* n, int64_t* nnz) {
   FVM_CHECK(t, "fvm_template_info: null template");
   if (n) *n = t->n;
@@ -1259,8 +1322,7 @@ int fvm_template_download(const fvm_template* t, int32_t* row_ptr, int32_t* col,
 int fvm_template_apply(const fvm_template* t, const double* d_u, double* d_du, fvm_stream s) {
   FVM_CHECK(t && d_u && d_du, "fvm_template_apply: null argument");
   FVM_CHECK(d_u != d_du, "fvm_template_apply: du must not alias u");
-  hipLaunchKernelGGL((spmv_kernel<double>), grid(t), dim3(kRT), 0, (hipStream_t)s, t->n, t->row_ptr, t->col,
-                     t->val, d_u, t->b, d_du);
+  spmv_any(t, 0, d_u, t->b, d_du, (hipStream_t)s);
   FVM_HIP(hipGetLastError());
   return FVM_OK;
 }
@@ -1269,16 +1331,7 @@ int fvm_template_spmv(const fvm_template* t, int32_t which, const double* d_x, d
   FVM_CHECK(t && d_x && d_y, "fvm_template_spmv: null argument");
   FVM_CHECK(d_x != d_y, "fvm_template_spmv: y must not alias x");
   FVM_CHECK(which >= 0 && which <= 2, "fvm_template_spmv: which must be 0, 1 or 2");
-  hipStream_t st = (hipStream_t)s;
-  if (which == 0)
-    hipLaunchKernelGGL((spmv_kernel<double>), grid(t), dim3(kRT), 0, st, t->n, t->row_ptr, t->col, t->val, d_x,
-                       nullptr, d_y);
-  else if (which == 1)
-    hipLaunchKernelGGL((spmv_kernel<double>), grid(t), dim3(kRT), 0, st, t->n, t->row_ptr, t->col, t->valM, d_x,
-                       nullptr, d_y);
-  else
-    hipLaunchKernelGGL((spmv_kernel<float>), grid(t), dim3(kRT), 0, st, t->n, t->row_ptr, t->col, t->valM32,
-                       d_x, nullptr, d_y);
+  spmv_any(t, which, d_x, nullptr, d_y, (hipStream_t)s);
   FVM_HIP(hipGetLastError());
   return FVM_OK;
 }

@@ -201,6 +201,7 @@ HIP_API = {
     "fvm_template_create": [_vp, ctypes.POINTER(TemplateDesc), ctypes.POINTER(_vp)],
     "fvm_template_destroy": [_vp],
     "fvm_template_info": [_vp, _i64p, _i64p],
+    "fvm_template_ell": [_vp, ctypes.POINTER(ctypes.c_int32), _i64p],
     "fvm_template_download": [_vp, _i32p, _i32p, _dp, _dp],
     "fvm_template_apply": [_vp, _vp, _vp, _vp],
     "fvm_template_spmv": [_vp, ctypes.c_int32, _vp, _vp, _vp],

@@ -233,6 +233,11 @@ typedef struct fvm_template_desc {
 int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_template** out);
 int fvm_template_destroy(fvm_template* t);
 int fvm_template_info(const fvm_template* t, int64_t* n, int64_t* nnz);
+/* The ELL copy of the operator used by the Krylov products and fvm_template_apply /
+ * fvm_template_spmv: slots per row (0: none, CSR throughout) and the number of
+ * off-diagonals kept in the overflow list.  No counterpart in the reference (its
+ * operator is a SparseMatrixCSC handed to KLU). */
+int fvm_template_ell(const fvm_template* t, int32_t* width, int64_t* overflow);
 /* the assembled A (CSR, 0-based, int32) and b; any pointer may be NULL */
 int fvm_template_download(const fvm_template* t, int32_t* row_ptr, int32_t* col, double* val,
                           double* b);

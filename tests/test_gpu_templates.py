@@ -375,6 +375,9 @@ def test_ell_and_csr_forms_of_the_krylov_operator_agree(gpu, mesh, alg):
     BCs = FVM.BoundaryConditions(m, lambda x, y, t, u, p: 0.0 * x, FVM.Dirichlet)
     prob = FVM.PoissonsEquation(m, BCs, diffusion_function=lambda x, y, p: 1.0 + 0.5 * np.sin(3 * x) ** 2,
                                 source_function=lambda x, y, p: np.cos(2 * x) - y)
+    W, n_over = prob.problem.ell_info()
+    assert 4 <= W <= 10
+    assert (n_over == 40 - W) if mesh == "wheel" else (W == 8 and n_over == 0)
     direct = spla.spsolve(prob.A.tocsc(), prob.b)
     sols = {f: FVM.solve(prob, alg, rtol=1e-11, spmv_format=f) for f in ("ell", "csr")}
     for f, s in sols.items():
