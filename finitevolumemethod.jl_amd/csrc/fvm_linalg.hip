@@ -295,17 +295,19 @@ __global__ __launch_bounds__(256) void scaled_kernel(int64_t N, const int32_t* _
 // never leave the device.  Scalars that later kernels need are stored by
 // workgroup 0 in slots alternating with the iteration's parity.
 
-// Sum over the workgroup (every thread gets the total).
+// Sum over the workgroup (every thread gets the total): each wave folds its 64
+// values by shuffles, then every thread adds the waves' sums in wave order, so the
+// order is fixed and one barrier replaces a tree of them.
 __device__ __forceinline__ double wg_sum(double v) {
-  __shared__ double sh[kRT];
-  sh[threadIdx.x] = v;
+  static_assert(kRT % 64 == 0, "whole waves");
+  __shared__ double sh[kRT / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int s = kRT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  const double t = sh[0];
-  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kRT / 64; ++w) t += sh[w];
+  __syncthreads();  // sh is reused by the next call
   return t;
 }
 
