@@ -288,7 +288,9 @@ typedef struct fvm_solve_stats {
 
 /* LinearProblem(A, b) of a steady template solved on the device (replaces the
  * sparse direct solve behind solve(::AbstractFVMTemplate), abstract_templates.jl:58-60).
- * d_x [n_points] receives the solution. */
+ * d_x [n_points] receives the solution.  A template's work vectors and solver
+ * scalars serve one solve at a time: calls on the same template must not overlap
+ * (different templates are independent). */
 int fvm_template_solve(fvm_template* t, const fvm_solve_opts* o, double* d_x, fvm_solve_stats* st,
                        fvm_stream s);
 
