@@ -247,11 +247,13 @@ def solve_bench(prob, rhs_ms, tf=4e-6):
         prob.final_time = keep
     # the integration alone: the state already on the device, no saved states
     from finitevolumemethod import _lib
+    from finitevolumemethod.solve import fsal_ok
 
     ctx = prob.device_context()
     d_u = _lib.DeviceArray.from_host(prob.initial_condition)
     o = _lib.OdeOpts()
     o.method, o.op_kind, o.op, o.dirichlet = _lib.ODE_TSIT5, _lib.OP_PROBLEM, ctx.handle, 1
+    o.fsal = int(fsal_ok(prob))  # what solve() passes: homogeneous Dirichlet values are constant
     o.n, o.t0, o.tf, o.dt = len(prob.initial_condition), 0.0, tf, 0.0
     o.abstol, o.reltol, o.maxiters, o.n_save = 1e-6, 1e-3, 10**7, 0
     ost = _lib.OdeStats()

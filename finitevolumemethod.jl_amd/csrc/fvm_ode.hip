@@ -184,7 +184,11 @@ int tsit5(const fvm_ode_opts* o, const Op& op, double* d_u, fvm_ode_stats* st, f
       std::swap(u, unew);
       t = tn;
       ++st->naccept;
-      if ((rc = op.apply_dirichlet(u, t, s)) || (rc = op.rhs(u, k[0], t, s))) return rc;
+      if ((rc = op.apply_dirichlet(u, t, s))) return rc;
+      // first same as last: the last stage is f(u_new, t + dt), and u is u_new unless
+      // the Dirichlet update changed it, which a constant update can do only once
+      if (o->fsal && (!op.dirichlet || st->naccept >= 2)) std::swap(k[0], k[6]);
+      else if ((rc = op.rhs(u, k[0], t, s))) return rc;
       double q = std::pow(std::max(EEst, 1e-16), 7.0 / 50.0) / std::pow(qold, 2.0 / 25.0);
       q = std::min(5.0, std::max(0.1, q / 0.9));
       dt = dt / q;

@@ -77,7 +77,7 @@ class SolveOpts(ctypes.Structure):
 
 class OdeOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int32), ("op_kind", ctypes.c_int32), ("op", _vp),
-                ("dirichlet", ctypes.c_int32), ("reserved", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("dirichlet", ctypes.c_int32), ("fsal", ctypes.c_int32), ("n", ctypes.c_int64),
                 ("t0", ctypes.c_double), ("tf", ctypes.c_double), ("dt", ctypes.c_double),
                 ("abstol", ctypes.c_double), ("reltol", ctypes.c_double), ("maxiters", ctypes.c_int64),
                 ("n_save", ctypes.c_int64), ("save_t", _dp), ("d_saves", _vp)]

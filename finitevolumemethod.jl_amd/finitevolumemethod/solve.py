@@ -165,14 +165,34 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
                      reltol=reltol, dt=dt, maxiters=maxiters, save_start=save_start,
                      step_callback=cb, user_callback=callback,
                      jvp=lambda u, v, out, t: ctx.jvp(u, v, out, t),
-                     op=(_lib.OP_PROBLEM, ctx.handle, has_dir))
+                     op=(_lib.OP_PROBLEM, ctx.handle, has_dir), fsal=fsal_ok(prob))
+
+
+def fsal_ok(prob):
+    """Whether Tsit5 may reuse its last stage as f(u) after an accepted step (first
+    same as last) for this FVMProblem: the Dirichlet update after each step can change
+    u only where its values depend on t or u, or at the first step (the initial
+    condition need not hold them); du is 0 at Dirichlet nodes, so a stage never moves
+    them.  True without Dirichlet nodes or when every Dirichlet function is constant in
+    t and u."""
+    if not prob.conditions.has_dirichlet_nodes():
+        return True
+    fn = prob.device_context().physics["fn"]
+    kind, fidx, _ = prob.conditions.flatten(prob.mesh.triangulation)
+    used = np.unique(fidx[kind == _lib.NODE_DIRICHLET])
+    coef = [np.asarray(fn[1]), np.asarray(fn[2])] + ([np.asarray(fn[5])] if len(fn) > 5 else [])
+    return all(float(c[f]) == 0.0 for c in coef for f in used)
 
 
 def integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None,
               maxiters=10**7, save_start=True, step_callback=None, user_callback=None, jvp=None,
-              op=None):
+              op=None, fsal=False):
     """Integrate du/dt = rhs(u, t) on the device from t0 to tf.  Implicit methods
     need jvp(u, v, out, t) = J(u) v.
+
+    fsal: step_callback cannot change u after the first accepted step (see
+    fsal_ok), so Tsit5 takes f(u) after an accepted step from its last stage instead
+    of evaluating it again, unless a user callback runs (it may change u).
 
     op = (kind, handle, dirichlet): the library operator that rhs evaluates (and,
     with dirichlet, whose Dirichlet update step_callback applies).  Explicit
@@ -182,7 +202,7 @@ def integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt
             and saveat is not None):
         times = _Saver(saveat, t0, tf).times
         if times is not None and 8 * len(times) * len(u0) <= _DEVICE_SAVE_BYTES:
-            return _integrate_device(op, u0, t0, tf, alg, times, abstol, reltol, dt, maxiters)
+            return _integrate_device(op, u0, t0, tf, alg, times, abstol, reltol, dt, maxiters, fsal)
     if isinstance(alg, TRBDF2):
         if jvp is None:
             raise TypeError("TRBDF2 needs the Jacobian-vector product of the right-hand side")
@@ -256,7 +276,10 @@ def integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt
                     user_callback(u, t)
                 if saver.times is None:
                     save(t, u)
-                f(u, k[0], t)
+                if fsal and user_callback is None and (step_callback is None or naccept >= 2):
+                    k[0], k[6] = k[6], k[0]  # first same as last
+                else:
+                    f(u, k[0], t)
                 q = max(EEst, 1e-16) ** (7 / 50) / qold ** (2 / 25)
                 q = min(5.0, max(0.1, q / 0.9))
                 dt = dt / q
@@ -475,7 +498,7 @@ def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, 
 _DEVICE_SAVE_BYTES = 8 << 30  # saved states held on the device by fvm_ode_solve
 
 
-def _integrate_device(op, u0, t0, tf, alg, times, abstol, reltol, dt, maxiters):
+def _integrate_device(op, u0, t0, tf, alg, times, abstol, reltol, dt, maxiters, fsal=False):
     """The explicit loops of integrate() run by fvm_ode_solve; the saved states are
     written to one device buffer and copied back once."""
     kind, handle, dirichlet = op
@@ -494,6 +517,7 @@ def _integrate_device(op, u0, t0, tf, alg, times, abstol, reltol, dt, maxiters):
     ts = np.ascontiguousarray(times, np.float64)
     saves = _lib.DeviceArray(max(len(ts), 1) * n)
     o.op_kind, o.op, o.dirichlet = int(kind), handle, int(bool(dirichlet))
+    o.fsal = int(bool(fsal))
     o.n, o.t0, o.tf = n, float(t0), float(tf)
     o.abstol, o.reltol, o.maxiters = float(abstol), float(reltol), int(maxiters)
     o.n_save, o.save_t, o.d_saves = len(ts), _lib.ptr(ts), saves.ptr

@@ -352,7 +352,7 @@ class _OdeTemplate(AbstractFVMTemplate):
         sol = integrate(rhs, self._u0[:n], self.initial_time, self.final_time,
                         alg if alg is not None else Tsit5(), saveat=saveat, abstol=abstol,
                         reltol=reltol, dt=dt, jvp=lambda u, v, out, t: p.spmv(0, v, out),
-                        op=(_lib.OP_TEMPLATE, p.handle, False), **kwargs)
+                        op=(_lib.OP_TEMPLATE, p.handle, False), fsal=True, **kwargs)
         sol.u = [np.concatenate([s, [1.0]]) for s in sol.u]
         return sol
 

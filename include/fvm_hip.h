@@ -446,7 +446,10 @@ typedef struct fvm_ode_opts {
   int32_t op_kind;        /* FVM_OP_*: what op points to */
   void* op;
   int32_t dirichlet;      /* apply the operator's Dirichlet update after every accepted step */
-  int32_t reserved;
+  int32_t fsal;           /* Tsit5: nonzero when the Dirichlet update cannot change u after the
+                             first accepted step (none, or values constant in t and u): f(u)
+                             after an accepted step is then the step's last stage, not a new
+                             evaluation (first same as last) */
   int64_t n;              /* state length */
   double t0, tf;
   double dt;              /* the fixed step; Tsit5: the first step (<= 0: automatic) */

@@ -10,7 +10,7 @@ import pytest
 
 import finitevolumemethod as FVM
 from finitevolumemethod import _lib
-from finitevolumemethod.solve import integrate
+from finitevolumemethod.solve import fsal_ok, integrate
 from problems import annulus_problem, brusselator_system, mixed_problem
 
 pytestmark = pytest.mark.gpu
@@ -30,12 +30,15 @@ def host_loop(prob, alg, **kw):
     ctx = prob.device_context()
     cb = (lambda u, t: ctx.apply_dirichlet(u, t)) if prob.conditions.has_dirichlet_nodes() else None
     return integrate(ctx.rhs, prob.initial_condition, prob.initial_time, prob.final_time, alg,
-                     step_callback=cb, **kw)
+                     step_callback=cb, fsal=fsal_ok(prob), **kw)
 
 
-def assert_same(a, b):
+def assert_same(a, b, nf=True):
     assert a.retcode == b.retcode == "Success"
-    assert a.stats == b.stats
+    if nf:
+        assert a.stats == b.stats
+    else:
+        assert {k: v for k, v in a.stats.items() if k != "nf"} == {k: v for k, v in b.stats.items() if k != "nf"}
     assert a.t == b.t and len(a.u) == len(b.u)
     for k, (x, y) in enumerate(zip(a.u, b.u)):
         assert np.all(np.isfinite(x)), k
@@ -107,7 +110,8 @@ def test_diffusion_template_matches_host_loop(gpu):
                                  initial_condition=u0, final_time=0.5)
     a = FVM.solve(prob, FVM.Tsit5(), saveat=0.05)
     p = prob.problem
-    b = integrate(lambda u, du, t: p.apply(u, du), prob._u0[:p.n], 0.0, 0.5, FVM.Tsit5(), saveat=0.05)
+    b = integrate(lambda u, du, t: p.apply(u, du), prob._u0[:p.n], 0.0, 0.5, FVM.Tsit5(), saveat=0.05,
+                  fsal=True)
     b.u = [np.concatenate([s, [1.0]]) for s in b.u]
     assert_same(a, b)
 
@@ -122,7 +126,9 @@ def test_paths_without_save_times_or_with_callbacks_stay_on_the_host_loop(gpu):
     cbk = FVM.solve(prob, FVM.Tsit5(), saveat=0.1, callback=lambda u, t: seen.append(t))
     lib = FVM.solve(prob, FVM.Tsit5(), saveat=0.1)
     assert len(seen) == lib.stats["naccept"]
-    assert_same(cbk, lib)
+    # a user callback may change u, so f(u) is evaluated again after each of its steps
+    assert_same(cbk, lib, nf=False)
+    assert cbk.stats["nf"] - lib.stats["nf"] == lib.stats["naccept"] - 1
 
 
 def test_ode_solve_rejects_bad_options(gpu):
@@ -180,3 +186,33 @@ def test_trbdf2_with_library_arnoldi_matches_host_loop(gpu):
                   saveat=0.1, step_callback=cb, jvp=lambda u, v, out, t: ctx.jvp(u, v, out, t))
     assert_same(a, b)
     assert a.stats["nkrylov"] > 50
+
+
+def test_first_same_as_last_saves_one_evaluation_per_step(gpu):
+    """Tsit5 takes f(u) after an accepted step from its last stage when the Dirichlet
+    update cannot change u: constant Dirichlet values (from the second step on; the
+    initial condition need not hold them) or none at all.  The states are the same
+    bit for bit as with a fresh evaluation; time-dependent Dirichlet values keep it."""
+    prob = plate(30)
+    assert fsal_ok(prob)
+    ctx = prob.device_context()
+    cb = lambda u, t: ctx.apply_dirichlet(u, t)  # noqa: E731
+    run = lambda fsal: integrate(ctx.rhs, prob.initial_condition, prob.initial_time, prob.final_time,  # noqa: E731
+                                 FVM.Tsit5(), saveat=0.1, step_callback=cb, fsal=fsal)
+    a, b = run(True), run(False)
+    assert_same(a, b, nf=False)
+    assert b.stats["nf"] - a.stats["nf"] == a.stats["naccept"] - 1
+    # without Dirichlet nodes every accepted step reuses its last stage
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 25, 25, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    neu = FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Neumann),
+                         diffusion_function=lambda x, y, t, u, p: 0.1 + 0.0 * u,
+                         initial_condition=np.sin(3 * tri.x) * np.cos(2 * tri.y), final_time=0.2)
+    assert fsal_ok(neu)
+    c = FVM.solve(neu, FVM.Tsit5(), saveat=0.1)
+    d = integrate(neu.device_context().rhs, neu.initial_condition, 0.0, 0.2, FVM.Tsit5(), saveat=0.1)
+    assert_same(c, d, nf=False)
+    assert d.stats["nf"] - c.stats["nf"] == c.stats["naccept"]
+    # a Dirichlet boundary that moves with t does not qualify
+    moving, _, _ = annulus_problem("tutorial", final_time=0.05)
+    assert not fsal_ok(moving)
