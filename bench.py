@@ -276,7 +276,8 @@ def published_walkthroughs():
     context, not the metric).  diffusion_equations.jl:327-331 (50 x 50 plate on [0, 2]^2,
     D = 1/9, Dirichlet 0, Tsit5, saveat 0.05: 5.736 ms) and :343-370, 442-446 (100 x 100
     on [0, 320]^2, D = 2, Neumann, u0 = 1 on 0.4 L <= y <= 0.6 L, Tsit5 to 500, saveat 100:
-    78.761 ms); poissons_equation.jl:92-95, 123-130, 168-180 (laplace u = -sin(pi x) sin(pi y) on
+    78.761 ms), with the FVMProblem forms of both (:336-340, TRBDF2 with KLU: 49.237 ms;
+    :451-455, CVODE_BDF with GMRES: 94.839 ms); poissons_equation.jl:92-95, 123-130, 168-180 (laplace u = -sin(pi x) sin(pi y) on
     a 100 x 100 grid of [0, 1]^2, u = 0 on the boundary, as the PoissonsEquation template:
     15.442 ms with KLU, and as a SteadyFVMProblem: 189.406 ms)."""
     import finitevolumemethod as FVM
@@ -299,6 +300,13 @@ def published_walkthroughs():
     out.append(dict(workload="DiffusionEquation 50x50 plate, Tsit5, saveat 0.05", ms=best(
         lambda: FVM.solve(plate, FVM.Tsit5(), saveat=0.05)), published_ms=5.736,
         source="docs/src/literate_wyos/diffusion_equations.jl:327-331"))
+    plate_fvm = FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet),
+                               diffusion_function=lambda x, y, t, u, p: 1 / 9 + 0.0 * u,
+                               initial_condition=np.where(tri.y <= 1.0, 50.0, 0.0), final_time=0.5)
+    out.append(dict(workload="the plate as an FVMProblem, TRBDF2 (ours: matrix-free Newton-GMRES on fvm_jvp; "
+                             "published: KLU on the sparse Jacobian), saveat 0.05",
+                    ms=best(lambda: FVM.solve(plate_fvm, FVM.TRBDF2(), saveat=0.05)), published_ms=49.237,
+                    source="docs/src/literate_wyos/diffusion_equations.jl:336-340"))
     L = 320.0
     tri = FVM.triangulate_rectangle(0.0, L, 0.0, L, 100, 100, single_boundary=True)
     mesh = FVM.FVMGeometry(tri)
@@ -309,6 +317,14 @@ def published_walkthroughs():
     out.append(dict(workload="DiffusionEquation 100x100 on [0,320]^2, Neumann, Tsit5, saveat 100", ms=best(
         lambda: FVM.solve(big, FVM.Tsit5(), saveat=100.0)), published_ms=78.761,
         source="docs/src/literate_wyos/diffusion_equations.jl:442-446"))
+    big_fvm = FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 2.0 + 0.0 * u, FVM.Neumann),
+                             diffusion_function=lambda x, y, t, u, p: 2.0 + 0.0 * u,
+                             initial_condition=np.where((0.4 * L <= tri.y) & (tri.y <= 0.6 * L), 1.0, 0.0),
+                             final_time=500.0)
+    out.append(dict(workload="the same as an FVMProblem, stiff (ours: TRBDF2, matrix-free Newton-GMRES; "
+                             "published: CVODE_BDF with GMRES), saveat 100",
+                    ms=best(lambda: FVM.solve(big_fvm, FVM.TRBDF2(), saveat=100.0)), published_ms=94.839,
+                    source="docs/src/literate_wyos/diffusion_equations.jl:451-455"))
     tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 100, 100, single_boundary=True)
     mesh = FVM.FVMGeometry(tri)
     BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * x, FVM.Dirichlet)
