@@ -151,7 +151,7 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
                         prob.final_time, alg if alg is not None else Tsit5(), saveat=saveat,
                         abstol=abstol, reltol=reltol, dt=dt, maxiters=maxiters,
                         save_start=save_start, step_callback=cb, user_callback=callback,
-                        jvp=ctx.jvp, op=(_lib.OP_SYSTEM, ctx.handle, cb is not None))
+                        jvp=ctx.jvp, op=(_lib.OP_SYSTEM, ctx.handle, cb is not None), fsal=fsal_ok(prob))
         sol.u = [prob.unflat(v) for v in sol.u]
         return sol
     if not isinstance(prob, FVMProblem):
@@ -170,11 +170,17 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
 
 def fsal_ok(prob):
     """Whether Tsit5 may reuse its last stage as f(u) after an accepted step (first
-    same as last) for this FVMProblem: the Dirichlet update after each step can change
+    same as last) for this FVMProblem or FVMSystem: the Dirichlet update after each step can change
     u only where its values depend on t or u, or at the first step (the initial
     condition need not hold them); du is 0 at Dirichlet nodes, so a stage never moves
     them.  True without Dirichlet nodes or when every Dirichlet function is constant in
     t and u."""
+    if isinstance(prob, FVMSystem):  # per-variable codes, condition forms over all variables
+        if not prob.has_dirichlet_nodes():
+            return True
+        ph = prob.device_context().physics
+        used = np.unique(ph["node_fidx"][ph["node_kind"] == _lib.NODE_DIRICHLET])
+        return all(not np.any(np.asarray(ph[k])[used]) for k in ("fn_cu", "fn_ct", "fn_lam"))
     if not prob.conditions.has_dirichlet_nodes():
         return True
     fn = prob.device_context().physics["fn"]

@@ -95,7 +95,7 @@ def test_system_matches_host_loop(gpu):
     ctx = sys.device_context()
     cb = (lambda u, t: ctx.apply_dirichlet(u, t)) if sys.has_dirichlet_nodes() else None
     b = integrate(ctx.rhs, sys.flat(sys.initial_condition), sys.initial_time, sys.final_time, FVM.Tsit5(),
-                  saveat=0.1, step_callback=cb)
+                  saveat=0.1, step_callback=cb, fsal=fsal_ok(sys))
     b.u = [sys.unflat(v) for v in b.u]
     assert_same(a, b)
     assert a.u[0].shape == (2, tri.num_points())
