@@ -42,20 +42,12 @@ def canonical_bytes(n_nodes, n_tri, n_bedges, per_tri=204, per_node=25, per_bedg
 
 
 def build_throughput_problem(n_side, tile=0):
-    import finitevolumemethod as FVM
+    from finitevolumemethod import workloads as WL
 
     t0 = time.time()
-    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n_side, n_side, jitter=0.25, seed=SEED)
-    log(f"mesh {tri.num_points()} nodes {len(tri.triangles)} triangles ({time.time() - t0:.1f}s)")
-    t0 = time.time()
-    mesh = FVM.FVMGeometry(tri, tile_triangles=tile)
-    log(f"FVMGeometry on device: {mesh.info.n_tiles} tiles, {mesh.info.n_fix_nodes} fix nodes "
-        f"({time.time() - t0:.1f}s)")
-    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
-    u = np.random.default_rng(SEED + 1).random(tri.num_points())
-    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: 0.1 + u ** 2,
-                          source_function=lambda x, y, t, u, p: u * (1 - u),
-                          initial_condition=u, final_time=1.0)
+    tri, mesh, prob, u = WL.throughput_problem(n_side, tile)
+    log(f"mesh {tri.num_points()} nodes {len(tri.triangles)} triangles, FVMGeometry on device: "
+        f"{mesh.info.n_tiles} tiles, {mesh.info.n_fix_nodes} fix nodes ({time.time() - t0:.1f}s)")
     prob.device_context()
     return tri, mesh, prob, u
 
@@ -87,18 +79,10 @@ def copy_bandwidth(n=1 << 27, reps=20):
 
 def vjp_ratio(n_side, reps=20):
     """Time fvm_rhs and fvm_vjp (u and log D) on the nodal-log-D workload."""
-    import finitevolumemethod as FVM
     from finitevolumemethod import _lib
+    from finitevolumemethod import workloads as WL
 
-    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n_side, n_side, jitter=0.25, seed=SEED + 7)
-    mesh = FVM.FVMGeometry(tri)
-    rng = np.random.default_rng(SEED + 8)
-    logd = np.log(0.1) + 0.3 * rng.standard_normal(tri.num_points())
-    u = rng.random(tri.num_points())
-    w = rng.standard_normal(tri.num_points())
-    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
-    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=FVM.NodalDiffusion(logd),
-                          initial_condition=u, final_time=0.1)
+    tri, prob, u, logd, w = WL.differentiation_problem(n_side)
     ctx = prob.device_context()
     d_u, d_w = _lib.DeviceArray.from_host(u), _lib.DeviceArray.from_host(w)
     d_du, d_ub, d_lb = (_lib.DeviceArray(len(u)) for _ in range(3))
@@ -158,21 +142,14 @@ def template_solve(n_side, rtol=1e-10, direct=True):
     |b - A x| <= rtol |b| for each device solver (after one warm-up solve), the SpMV
     rates of the solver operator, and (direct=True) each solution's relative error
     against an FP64 sparse LU solve (SuperLU, COLAMD ordering) of the same A and b on
-    the host."""
-    import finitevolumemethod as FVM
+    the host (build_seconds: both templates, mesh included)."""
     from finitevolumemethod import _lib
+    from finitevolumemethod import workloads as WL
 
-    tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, n_side, n_side, jitter=0.25, seed=SEED + 3)
-    mesh = FVM.FVMGeometry(tri)
-    N = tri.num_points()
-    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
-    f = np.random.default_rng(SEED + 4).uniform(-1.0, 1.0, N)
-    D = lambda x, y, p: np.exp(0.5 * np.sin(4 * np.pi * x) * np.cos(4 * np.pi * y))  # noqa: E731
     t0 = time.time()
-    prob = FVM.PoissonsEquation(mesh, BCs, diffusion_function=D, source_function=lambda x, y, p: p,
-                                source_parameters=f)
+    tri, prob, met, f = WL.template_problems(n_side)
     t_build = time.time() - t0
-    met = FVM.MeanExitTimeProblem(mesh, BCs, diffusion_function=D)
+    N = tri.num_points()
     dt = prob.problem
     out = dict(workload=f"PoissonsEquation and MeanExitTimeProblem {n_side}x{n_side} jittered, "
                         "D=exp(0.5 sin4pix cos4piy), f~U[-1,1] (Poisson), homogeneous Dirichlet",
@@ -345,28 +322,19 @@ def published_walkthroughs():
 
 
 def ensemble_bench(P=256, steps=1000, graph_steps=100):
-    """The ensemble workload: P rectangles [0, 2]^2 with nx, ny ~ round(exp(U[log 20,
-    log 300])) drawn independently, interior points jittered by U(-h/4, h/4), constant
-    D_p ~ LogNormal(log 0.1, 0.5), homogeneous Dirichlet, u0 ~ U[0, 1), forward Euler
-    with dt_p = 0.9 x the Gershgorin stability limit, `steps` steps.  Times the packed
-    ensemble (autotuned tiles, graph replay) against looping the single-problem path
-    over the members (each member's steps issued back to back from C)."""
-    import finitevolumemethod as FVM
+    """The ensemble workload (workloads.ensemble_members): P rectangles [0, 2]^2 with
+    nx, ny ~ round(exp(U[log 20, log 300])) drawn independently, interior points
+    jittered by U(-h/4, h/4), constant D_p ~ LogNormal(log 0.1, 0.5), homogeneous
+    Dirichlet, u0 = 50 where y <= 1 (mid-height), else 0, forward Euler with dt_p =
+    0.9 x the Gershgorin stability limit, `steps` steps.  Times the packed ensemble
+    (autotuned tiles, graph replay) against looping the single-problem path over the
+    members (each member's steps issued back to back from C)."""
     from finitevolumemethod import _lib
     from finitevolumemethod import ensemble as EN
+    from finitevolumemethod import workloads as WL
 
-    rng = np.random.default_rng(SEED + 5)
     t0 = time.time()
-    probs = []
-    for p in range(P):
-        nx, ny = (int(round(np.exp(rng.uniform(np.log(20), np.log(300))))) for _ in range(2))
-        tri = FVM.jittered_rectangle(0.0, 2.0, 0.0, 2.0, nx, ny, jitter=0.25, seed=SEED + 1000 + p)
-        mesh = FVM.FVMGeometry(tri)
-        BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet)
-        D = float(np.exp(np.log(0.1) + 0.5 * rng.standard_normal()))
-        u0 = rng.random(tri.num_points())
-        probs.append(FVM.FVMProblem(mesh, BCs, diffusion_function=(lambda d: (lambda x, y, t, u, p: d + 0.0 * u))(D),
-                                    initial_condition=u0, final_time=1.0))
+    probs, _ = WL.ensemble_members(P)
     dts = [EN.explicit_dt(q) for q in probs]
     N = sum(q.mesh.triangulation.num_points() for q in probs)
     T = sum(len(q.mesh.triangulation.triangles) for q in probs)
@@ -421,7 +389,8 @@ def ensemble_bench(P=256, steps=1000, graph_steps=100):
     err = float(np.max(np.abs(batched - looped)) / max(1e-300, np.max(np.abs(looped))))
     per_step = 132 * T + 25 * N
     out = dict(workload=f"{P} jittered rectangles on [0,2]^2, nx,ny~logU[20,300], D~LogNormal(log 0.1,0.5), "
-                        f"homogeneous Dirichlet, forward Euler dt=0.9x stability limit, {steps} steps",
+                        f"u0=50 for y<=1, homogeneous Dirichlet, forward Euler dt=0.9x stability limit, "
+                        f"{steps} steps",
                members=P, nodes=int(N), triangles=int(T), steps=steps,
                tile_triangles=int(b.info.tile_triangles),
                autotune_ms={int(t): round(float(m), 5) for t, m in

@@ -187,10 +187,10 @@ __device__ __forceinline__ double eval_cond(const CondTable& C, int f, double x,
 // Runs launch(d_flag) on stream st with a cleared device flag, waits, and returns
 // FVM_ERR_ARG ("<what>: a queried edge or triangle is not in the mesh") when a
 // kernel set it (the pointwise query kernels).
+// d_bad is the mesh's own query flag (fvm_mesh::qflag): queries on one mesh must not
+// overlap on different streams or host threads.
 template <class Launch>
-int run_flagged(hipStream_t st, const char* what, Launch launch) {
-  int32_t* d_bad = nullptr;
-  FVM_HIP(hipMalloc((void**)&d_bad, sizeof(int32_t)));
+int run_flagged(hipStream_t st, const char* what, int32_t* d_bad, Launch launch) {
   int32_t bad = 0;
   hipError_t e = hipMemcpyAsync(d_bad, &bad, sizeof(int32_t), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) {
@@ -199,7 +199,6 @@ int run_flagged(hipStream_t st, const char* what, Launch launch) {
   }
   if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, sizeof(int32_t), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  (void)hipFree(d_bad);
   if (e != hipSuccess) return hip_fail(e, what);
   if (bad) {
     set_error(std::string(what) + ": a queried edge or triangle is not in the mesh");
@@ -253,6 +252,9 @@ struct fvm_mesh {
   int32_t* perm = nullptr;                      // sorted -> input triangle
   uint8_t* is_vertex = nullptr;
   double* lcv = nullptr;            // [n_local] control volume of each local-node entry
+  int32_t* qflag = nullptr;         // device flag of the pointwise queries (run_flagged)
+  bool convex = false;              // one boundary loop that turns left (or goes straight) at
+                                    // every vertex: a point outside a boundary edge is outside
   int32_t n_bedge_items = 0;        // entries of fix_bedge (boundary edge endpoints)
   int32_t n_bvert_items = 0;        // entries of fix_bvert (boundary triangle vertices)
   // host copies needed to build contexts

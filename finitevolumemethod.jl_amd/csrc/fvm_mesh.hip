@@ -132,10 +132,48 @@ void free_mesh(fvm_mesh* m) {
                   m->tile_nodes, m->loc, m->inc_start, m->inc, m->part_idx, m->fix_nodes,
                   m->fix_part_start, m->fix_bedge_start, m->fix_bedge, m->fix_bvert_start,
                   m->fix_bvert, m->lcv, m->node_tri_start,
-                  m->node_tri, m->bedges, m->badj, m->perm, m->is_vertex};
+                  m->node_tri, m->bedges, m->badj, m->perm, m->is_vertex, m->qflag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;
+}
+
+// Whether the mesh covers a convex domain: its boundary edges (counter-clockwise, the
+// domain on their left) form one closed loop, V - E + F = 1 (a disk: no holes), and
+// the loop turns left or goes straight at every vertex.  Then a point strictly on the
+// outer side of a boundary edge lies outside the domain.
+bool convex_domain(const fvm_mesh_desc* d) {
+  const int64_t N = d->n_points, T = d->n_triangles, B = d->n_boundary_edges;
+  if (B < 3) return false;
+  std::vector<int32_t> next(N, -1);
+  for (int64_t e = 0; e < B; ++e) {
+    const int32_t i = d->boundary_edges[2 * e], j = d->boundary_edges[2 * e + 1];
+    if (next[i] >= 0) return false;  // a vertex where two boundary loops touch
+    next[i] = j;
+  }
+  // one loop through every boundary edge
+  const int32_t s = d->boundary_edges[0];
+  int32_t v = s;
+  int64_t steps = 0;
+  do {
+    const int32_t w = next[v];
+    if (w < 0) return false;
+    const int32_t z = next[w];
+    if (z < 0) return false;
+    const double ax = d->x[w] - d->x[v], ay = d->y[w] - d->y[v];
+    const double bx = d->x[z] - d->x[w], by = d->y[z] - d->y[w];
+    if (ax * by - ay * bx < 0.0) return false;  // a right turn: reflex vertex
+    v = w;
+    ++steps;
+  } while (v != s && steps <= B);
+  if (v != s || steps != B) return false;
+  // Euler characteristic of the solid triangles' vertices, edges and faces: 1 for a disk
+  std::vector<uint8_t> isv(N, 0);
+  for (int64_t t = 0; t < 3 * T; ++t) isv[d->triangles[t]] = 1;
+  int64_t nv = 0;
+  for (int64_t i = 0; i < N; ++i) nv += isv[i];
+  const int64_t E = (3 * T + B) / 2;  // every interior edge in two triangles, boundary edges in one
+  return (3 * T + B) % 2 == 0 && nv - E + T == 1;
 }
 
 int build(const fvm_mesh_desc* d, const int32_t* tri_group, fvm_mesh* m) {
@@ -202,7 +240,9 @@ int build(const fvm_mesh_desc* d, const int32_t* tri_group, fvm_mesh* m) {
   UP(m->cv, (const double*)nullptr, N);
   UP(m->is_vertex, (const uint8_t*)nullptr, N);
   UP(m->lcv, (const double*)nullptr, m->n_local);
+  UP(m->qflag, (const int32_t*)nullptr, 1);
 #undef UP
+  m->convex = convex_domain(d);
   m->n_bedge_items = tl->fix_bedge_start[m->n_fix];
   m->n_bvert_items = tl->fix_bvert_start[m->n_fix];
   m->h_part_idx.assign(tl->part_idx, tl->part_idx + m->n_local);

@@ -160,6 +160,8 @@ struct LocQuery {
   const double* x;
   const double* y;
   int32_t* out;  // input triangle index, -1 outside the mesh, -2 while undecided
+  int32_t convex;  // the domain is convex: a walk stopped by the boundary means outside
+  int64_t q0;      // first query of this scan launch
 };
 
 __device__ __forceinline__ double orient(double ax, double ay, double bx, double by, double px, double py) {
@@ -178,8 +180,10 @@ __device__ __forceinline__ bool contains(const LocQuery& a, int32_t t, double px
 
 // Visibility walk from a triangle of the start vertex: cross the edge the point
 // lies most outside of (or another edge it lies outside of when that one is on the
-// boundary).  A walk that leaves through the boundary or runs out of steps marks
-// the query undecided (-2): a non-convex domain can trap a straight walk.
+// boundary).  A walk stopped by the boundary of a convex domain has found the point
+// outside (-1): it lies strictly on the outer side of a boundary edge.  Otherwise a
+// walk that leaves through the boundary or runs out of steps marks the query
+// undecided (-2): a non-convex domain can trap a straight walk.
 __global__ __launch_bounds__(256) void locate_walk_kernel(LocQuery a) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.n) return;
@@ -215,7 +219,10 @@ __global__ __launch_bounds__(256) void locate_walk_kernel(LocQuery a) {
       // the neighbour across edge v[e] -> v[e+1] holds the directed edge v[e+1] -> v[e]
       next = find_directed_edge(a.node_tri_start, a.node_tri, a.tri, a.T, v[e == 2 ? 0 : e + 1], v[e]);
     }
-    if (next < 0) break;
+    if (next < 0) {
+      a.out[q] = a.convex ? -1 : -2;
+      return;
+    }
     t = next;
   }
   a.out[q] = -2;
@@ -224,7 +231,7 @@ __global__ __launch_bounds__(256) void locate_walk_kernel(LocQuery a) {
 // Undecided queries: one workgroup scans every triangle; the smallest sorted index
 // containing the point wins (an LDS minimum: the same answer on every run), -1 if none.
 __global__ __launch_bounds__(256) void locate_scan_kernel(LocQuery a) {
-  const int64_t q = blockIdx.x;
+  const int64_t q = a.q0 + blockIdx.x;
   if (a.out[q] != -2) return;
   __shared__ int32_t best;
   if (threadIdx.x == 0) best = 0x7fffffff;
@@ -267,9 +274,17 @@ int fvm_locate_points(const fvm_mesh* m, int64_t n, const double* d_x, const dou
   a.x = m->x;
   a.y = m->y;
   a.out = d_tri_out;
+  a.convex = m->convex ? 1 : 0;
+  a.q0 = 0;
   hipStream_t st = (hipStream_t)s;
   hipLaunchKernelGGL(locate_walk_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(locate_scan_kernel, dim3((unsigned)n), dim3(256), 0, st, a);
+  // one workgroup per query, at most 2^22 workgroups (2^30 work-items) per launch;
+  // workgroups of decided queries return at once
+  constexpr int64_t kChunk = (int64_t)1 << 22;
+  for (int64_t q0 = 0; q0 < n; q0 += kChunk) {
+    a.q0 = q0;
+    hipLaunchKernelGGL(locate_scan_kernel, dim3((unsigned)std::min(kChunk, n - q0)), dim3(256), 0, st, a);
+  }
   FVM_HIP(hipGetLastError());
   FVM_HIP(hipStreamSynchronize(st));
   return FVM_OK;
@@ -307,7 +322,7 @@ int fvm_compute_flux(fvm_ctx* c, int64_t n, const int32_t* d_i, const int32_t* d
   a.d3 = c->dpoly[3];
   a.out = d_out;
   hipStream_t st = (hipStream_t)s;
-  return run_flagged(st, "fvm_compute_flux", [&](int32_t* bad) {
+  return run_flagged(st, "fvm_compute_flux", m->qflag, [&](int32_t* bad) {
     a.bad = bad;
     hipLaunchKernelGGL(flux_query_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
   });
@@ -334,7 +349,7 @@ int fvm_pl_interpolate(const fvm_mesh* m, int32_t n_vars, int64_t n, const int32
   a.u = d_u;
   a.out = d_out;
   hipStream_t st = (hipStream_t)s;
-  return run_flagged(st, "fvm_pl_interpolate", [&](int32_t* bad) {
+  return run_flagged(st, "fvm_pl_interpolate", m->qflag, [&](int32_t* bad) {
     a.bad = bad;
     hipLaunchKernelGGL(interp_query_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
   });

@@ -808,7 +808,7 @@ int fvm_system_compute_flux(fvm_system* s, int64_t n, const int32_t* d_i, const 
   const fvm_mesh* m = s->m;
   hipStream_t hs = (hipStream_t)st;
   const dim3 g((unsigned)((n + 255) / 256));
-  return fvm::run_flagged(hs, "fvm_system_compute_flux", [&](int32_t* bad) {
+  return fvm::run_flagged(hs, "fvm_system_compute_flux", m->qflag, [&](int32_t* bad) {
     switch (s->nv) {
       case 1: hipLaunchKernelGGL((sys_flux_query_kernel<1>), g, dim3(256), 0, hs, a, m->node_tri_start, m->node_tri, n, d_i, d_j, d_out, bad); break;
       case 2: hipLaunchKernelGGL((sys_flux_query_kernel<2>), g, dim3(256), 0, hs, a, m->node_tri_start, m->node_tri, n, d_i, d_j, d_out, bad); break;

@@ -114,6 +114,9 @@ class EnsembleProblem:
                              "steps: choose final_time = initial_time + n dt per member")
         nsteps = steps.pop()
         t0 = {p.initial_time for p in self.problems}
+        if len(t0) > 1 and any(_time_dependent(p) for p in self.problems):
+            raise ValueError("the packed Euler path steps every member from one initial_time: members "
+                             "with time-dependent sources or conditions need the same initial_time")
         b = DeviceBatch(self.problems, dts, autotune=autotune,
                         graph_steps=min(graph_steps, nsteps) if graph_steps else 0)
         u = b.pack([p.initial_condition for p in self.problems])
@@ -132,6 +135,14 @@ class EnsembleProblem:
             sols.append(s)
         return EnsembleSolution(u=sols, stats=dict(steps=nsteps, ms=ms, dt=dts,
                                                    tile_triangles=b.info.tile_triangles))
+
+
+def _time_dependent(p):
+    """Whether an FVMProblem's traced source or condition functions depend on t."""
+    ph = physics_of(p)
+    fn = ph["fn"]
+    return bool(ph.get("st", 0.0) != 0.0 or np.any(np.asarray(fn[2]) != 0.0)
+                or np.any(np.asarray(fn[5]) != 0.0))
 
 
 def _linear_flux(p):

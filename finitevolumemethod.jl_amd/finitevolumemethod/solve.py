@@ -136,14 +136,23 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
                 kwargs[k] = v
         return prob.solve(alg, dt=dt, **kwargs)
     if isinstance(prob, _SteadyTemplate):  # a LinearProblem: only the linear solver's options apply
+        bad = [k for k, v in (("saveat", saveat), ("dt", dt), ("callback", callback), ("abstol", abstol))
+               if v is not None]
+        if bad or not save_start:
+            raise TypeError(f"solve: a steady template (a LinearProblem) takes no {', '.join(bad) or 'save_start'}")
+        if reltol is not None:  # LinearSolve's reltol is the Krylov stopping tolerance
+            kwargs.setdefault("rtol", reltol)
+        if maxiters != 10**7:
+            kwargs.setdefault("maxiter", maxiters)
         return prob.solve(alg, **kwargs)
     if isinstance(prob, SteadyFVMProblem):
         tol = {k: v for k, v in (("abstol", abstol), ("reltol", reltol)) if v is not None}
         return solve_steady(prob, alg, **tol, **kwargs)
     abstol = 1e-6 if abstol is None else abstol
     reltol = 1e-3 if reltol is None else reltol
-    if isinstance(prob, AbstractFVMTemplate):
-        return prob.solve(alg, saveat=saveat, abstol=abstol, reltol=reltol, dt=dt, **kwargs)
+    if isinstance(prob, AbstractFVMTemplate):  # every option goes on to the ODEProblem's solve
+        return prob.solve(alg, saveat=saveat, abstol=abstol, reltol=reltol, dt=dt, maxiters=maxiters,
+                          save_start=save_start, callback=callback, **kwargs)
     if isinstance(prob, FVMSystem):
         ctx = prob.device_context()
         cb = (lambda u, t: ctx.apply_dirichlet(u, t)) if prob.has_dirichlet_nodes() else None
@@ -190,9 +199,21 @@ def fsal_ok(prob):
     return all(float(c[f]) == 0.0 for c in coef for f in used)
 
 
-def integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None,
-              maxiters=10**7, save_start=True, step_callback=None, user_callback=None, jvp=None,
-              op=None, fsal=False):
+def integrate(rhs, u0, t0, tf, alg, *, saveat=None, save_start=True, **kwargs):
+    """Integrate du/dt = rhs(u, t) on the device from t0 to tf (see _integrate).
+    save_start=False drops the initial state from the saved ones, also when it is one
+    of the saveat times (OrdinaryDiffEq's save_start)."""
+    sol = _integrate(rhs, u0, t0, tf, alg, saveat=saveat, save_start=save_start, **kwargs)
+    if not save_start and saveat is not None:
+        keep = [k for k, tt in enumerate(sol.t) if tt > t0]
+        sol.t = [sol.t[k] for k in keep]
+        sol.u = [sol.u[k] for k in keep]
+    return sol
+
+
+def _integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None,
+               maxiters=10**7, save_start=True, step_callback=None, user_callback=None, jvp=None,
+               op=None, fsal=False):
     """Integrate du/dt = rhs(u, t) on the device from t0 to tf.  Implicit methods
     need jvp(u, v, out, t) = J(u) v.
 

@@ -336,9 +336,12 @@ class _OdeTemplate(AbstractFVMTemplate):
         ic[dn] = p.cond_val[dn]
         self._u0 = np.concatenate([ic, [1.0]])
 
-    def solve(self, alg=None, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None, **kwargs):
+    def solve(self, alg=None, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None, maxiters=10**7,
+              save_start=True, callback=None, **kwargs):
         """ODEProblem(MatrixOperator([A b; 0 0]), [u0; 1]) integrated on the device;
-        the states keep the augmented last entry (always 1), as in the reference."""
+        the states keep the augmented last entry (always 1), as in the reference.
+        callback(u, t) runs after every accepted step on the device state (its first n
+        entries), as a DiscreteCallback(true, ...) handed to the ODEProblem would."""
         from .solve import Tsit5, integrate
 
         p = self.problem
@@ -351,8 +354,9 @@ class _OdeTemplate(AbstractFVMTemplate):
 
         sol = integrate(rhs, self._u0[:n], self.initial_time, self.final_time,
                         alg if alg is not None else Tsit5(), saveat=saveat, abstol=abstol,
-                        reltol=reltol, dt=dt, jvp=lambda u, v, out, t: p.spmv(0, v, out),
-                        op=(_lib.OP_TEMPLATE, p.handle, False), fsal=True, **kwargs)
+                        reltol=reltol, dt=dt, maxiters=maxiters, save_start=save_start,
+                        user_callback=callback, jvp=lambda u, v, out, t: p.spmv(0, v, out),
+                        op=(_lib.OP_TEMPLATE, p.handle, False), fsal=callback is None, **kwargs)
         sol.u = [np.concatenate([s, [1.0]]) for s in sol.u]
         return sol
 

@@ -193,3 +193,31 @@ def test_ensemble_of_steady_problems(gpu):
         assert np.array_equal(s.u[-1], single.u[-1])
         means.append(np.mean(s.u[-1]))
     assert all(b > a for a, b in zip(means, means[1:]))
+
+
+def test_differing_initial_times_need_time_independent_members(gpu):
+    """The packed Euler path starts every member at one time: members that start at
+    different times are accepted only when no source or condition depends on t."""
+    tri = FVM.jittered_rectangle(0.0, 2.0, 0.0, 2.0, 12, 14, jitter=0.25, seed=31)
+    mesh = FVM.FVMGeometry(tri)
+    u0 = np.where(tri.y <= 1.0, 50.0, 0.0)
+
+    def make(bc, t0):
+        p = FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, bc, FVM.Dirichlet),
+                           diffusion_function=lambda x, y, t, u, p: 0.1 + 0.0 * u,
+                           initial_condition=u0, initial_time=t0, final_time=t0 + 1.0)
+        return p
+
+    steady_bc = lambda x, y, t, u, p: 0.0 * u  # noqa: E731
+    moving_bc = lambda x, y, t, u, p: t + 0.0 * u  # noqa: E731
+    dt = EN.explicit_dt(make(steady_bc, 0.0))
+    n = 8
+    for p in (a := [make(steady_bc, 0.0), make(steady_bc, 0.25)]):
+        p.final_time = p.initial_time + n * dt
+    sol = FVM.EnsembleProblem(a).solve(FVM.Euler(), dt=dt, graph_steps=0, autotune=False)
+    assert np.array_equal(sol.u[0].u[-1], sol.u[1].u[-1])  # nothing depends on t
+    b = [make(moving_bc, 0.0), make(moving_bc, 0.25)]
+    for p in b:
+        p.final_time = p.initial_time + n * dt
+    with pytest.raises(ValueError):
+        FVM.EnsembleProblem(b).solve(FVM.Euler(), dt=dt, graph_steps=0, autotune=False)

@@ -397,3 +397,33 @@ def test_unknown_spmv_format_is_rejected(gpu):
                                 source_function=lambda x, y, p: 1.0)
     with pytest.raises(KeyError):
         FVM.solve(prob, spmv_format="dense")
+
+
+def test_template_solve_forwards_callback_and_options(gpu):
+    """solve(::AbstractFVMTemplate, alg; kwargs...) hands every keyword to the ODEProblem's
+    solve (abstract_templates.jl:58-60): a callback runs once per accepted step, maxiters
+    and save_start apply; a steady template rejects options a LinearProblem has no use for."""
+    tri = FVM.triangulate_rectangle(0.0, 2.0, 0.0, 2.0, 20, 20, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    prob = FVM.DiffusionEquation(mesh, BCs, diffusion_function=lambda x, y, p: 1 / 9,
+                                 initial_condition=np.where(tri.y <= 1.0, 50.0, 0.0), final_time=0.1)
+    calls = []
+    sol = FVM.solve(prob, FVM.Tsit5(), saveat=0.05, callback=lambda u, t: calls.append((t, u.to_host()[0])))
+    assert sol.retcode == "Success"
+    assert len(calls) == sol.stats["naccept"] > 3
+    ts = [c[0] for c in calls]
+    assert all(b > a for a, b in zip(ts, ts[1:])) and abs(ts[-1] - 0.1) <= 1e-12
+    plain = FVM.solve(prob, FVM.Tsit5(), saveat=0.05)
+    assert len(plain.u) == len(sol.u) == 3
+    assert np.max(np.abs(plain.u[-1] - sol.u[-1])) <= 1e-12 * np.max(np.abs(plain.u[-1]))
+    short = FVM.solve(prob, FVM.Tsit5(), saveat=0.05, maxiters=2, callback=lambda u, t: None)
+    assert short.retcode == "MaxIters" and short.stats["naccept"] + short.stats["nreject"] == 2
+    nostart = FVM.solve(prob, FVM.Tsit5(), saveat=0.05, save_start=False, callback=lambda u, t: None)
+    assert nostart.t[0] > 0.0
+    pois = FVM.PoissonsEquation(mesh, BCs, source_function=lambda x, y, p: 1.0)
+    with pytest.raises(TypeError):
+        FVM.solve(pois, saveat=0.1)
+    with pytest.raises(TypeError):
+        FVM.solve(pois, callback=lambda u, t: None)
+    assert FVM.solve(pois, reltol=1e-12).resid <= 1e-12
