@@ -7,10 +7,13 @@ homogeneous Dirichlet boundary, D(u) = 0.1 + u^2, S(u) = u (1 - u), u ~ U[0, 1)
 i.i.d.  One step = one fvm_eqs!(du, u, p, 0) over the whole mesh with u and du
 resident in HBM.  Throughput = node updates per second.
 
-Printed JSON (one line): the metric, the roofline of the RHS kernels from HIP
-events on the launch stream, the VJP/RHS cost ratio on the 1000 x 1000 nodal-
-log-D mesh, a copy-kernel bandwidth, and the CPU baseline: the oracle's OpenMP
-restatement of parallel_fvm_eqs! timed on a bounded number of evaluations.
+Printed JSON (one line): the metric; the roofline of the two RHS kernels on the
+step time, bound by VALU issue (VALU instructions per RHS and HBM bytes per RHS from
+the committed rocprofv3 counter runs of this build, profiles/fvm_rhs_4M_counters.json),
+with the HBM fraction and the reference data model's canonical bytes beside it; a
+copy-kernel bandwidth; the VJP/RHS cost ratio on the 1000 x 1000 nodal-log-D mesh and
+the log-D fit; and the CPU baseline: the oracle's OpenMP restatement of
+parallel_fvm_eqs! timed on a bounded number of evaluations.
 """
 import argparse
 import ctypes
@@ -28,7 +31,13 @@ for _p in (ROOT, os.path.join(ROOT, "finitevolumemethod.jl_amd")):
 
 METRIC = "FP64 fvm_eqs! Gnode-updates/s @4M nodes (%HBM peak); VJP/RHS ratio"
 HBM_PEAK = 8.0e12  # B/s, MI355X
+# FP64 FMA peak of the vector units, MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2 flops x
+# 2.4 GHz.  Every wave64 VALU instruction (FP64, FP32 or integer) holds its SIMD for the
+# 4 cycles of an FP64 FMA, so this is also the VALU issue ceiling when each VALU
+# instruction is counted as an FMA's 64 x 2 flops.
+VALU_PEAK = 78.6e12
 SEED = 20261202
+COUNTERS = os.path.join(ROOT, "profiles", "fvm_rhs_4M_counters.json")
 
 
 def log(msg):
@@ -62,7 +71,9 @@ def rhs_timed(ctx, d_u, d_du):
 
 
 def copy_bandwidth(n=1 << 27, reps=20):
-    """A plain copy kernel (out = 1.0 * x) of n doubles: bytes moved / time."""
+    """The library's vector kernel as a copy (out = 1.0 * x, 16-byte loads and stores,
+    two pairs of doubles per thread and iteration) of n doubles: bytes moved / time, the
+    achievable HBM rate on this box."""
     from finitevolumemethod import _lib
 
     x = _lib.DeviceArray(n)
@@ -100,6 +111,55 @@ def vjp_ratio(n_side, reps=20):
     t_vjp = tm.stop() / reps
     return dict(nodes=int(tri.num_points()), triangles=int(len(tri.triangles)),
                 rhs_ms=round(t_rhs, 4), vjp_ms=round(t_vjp, 4), ratio=round(t_vjp / t_rhs, 3))
+
+
+def fit_bench(n_side, iters=50, tf=0.1, nsteps=10):
+    """The diffusivity-field fit of the differentiation workload (n_side x n_side jittered
+    mesh, u0 ~ U[0, 1), homogeneous Dirichlet): `iters` gradient-descent iterations with
+    Armijo backtracking of the nodal log D field, from log D = log 0.1 everywhere, to
+    match the state at t = tf of the workload's own field (log D ~ N(log 0.1, 0.3^2)).
+    Forward: nsteps RKC steps (damping 10, stages covering 1.25 dt rho(J) of the true
+    field); gradient: their discrete adjoint on fvm_vjp."""
+    from finitevolumemethod import adjoint as AD
+    from finitevolumemethod import workloads as WL
+
+    tri, prob, u, logd_true, _ = WL.differentiation_problem(n_side, final_time=tf)
+    t0 = time.time()
+    fit = AD.DiffusivityFit(prob, tf / nsteps, nsteps, method="rkc")
+    target = fit.forward(logd_true)
+    t_setup = time.time() - t0
+    logd0 = np.full(len(u), np.log(0.1))
+    counts = {"rhs": 0, "vjp": 0}
+    ctx = fit.ctx
+    rhs0, vjp0 = ctx.rhs, ctx.vjp
+
+    def rhs(*a, **k):
+        counts["rhs"] += 1
+        return rhs0(*a, **k)
+
+    def vjp(*a, **k):
+        counts["vjp"] += 1
+        return vjp0(*a, **k)
+
+    ctx.rhs, ctx.vjp = rhs, vjp
+    try:
+        t0 = time.time()
+        logd, hist = AD.fit_log_diffusivity(prob, target, logd0, None, None, iters=iters, fit=fit)
+        sec = time.time() - t0
+    finally:
+        ctx.rhs, ctx.vjp = rhs0, vjp0
+    err0 = float(np.linalg.norm(logd0 - logd_true) / np.linalg.norm(logd_true))
+    err1 = float(np.linalg.norm(logd - logd_true) / np.linalg.norm(logd_true))
+    out = dict(workload=f"log D fit, {n_side}x{n_side} jittered mesh, u0~U[0,1), target u(t={tf:g}) of "
+                        "log D~N(log 0.1,0.3^2), start log D=log 0.1, gradient descent with Armijo backtracking",
+               nodes=int(tri.num_points()), iterations=len(hist), rkc_steps=nsteps, rkc_stages=fit.stages,
+               rkc_damping=10.0, dt=tf / nsteps, setup_seconds=round(t_setup, 2),
+               loss_first=hist[0], loss_last=hist[-1], loss_drop=round(hist[-1] / hist[0], 6),
+               seconds=round(sec, 2), seconds_per_iteration=round(sec / len(hist), 3),
+               rhs_evaluations=counts["rhs"], vjp_evaluations=counts["vjp"],
+               logd_rel_error_start=round(err0, 4), logd_rel_error_end=round(err1, 4))
+    log(f"fit {out}")
+    return out
 
 
 def _solve_all(dt, rtol, tag):
@@ -159,14 +219,14 @@ def template_solve(n_side, rtol=1e-10, direct=True):
     out["mean_exit_time"], met_sols = _solve_all(met.problem, rtol, "mean_exit_time")
     x = _lib.DeviceArray.from_host(sols["cg_fp64"])
     y = _lib.DeviceArray(N)
-    W, n_over = dt.ell_info()
-    out["operator_form"] = dict(ell_slots=W, overflow_entries=n_over)
+    W, n_over, ib = dt.ell_info(index_bytes=True)
+    out["operator_form"] = dict(ell_slots=W, overflow_entries=n_over, column_index_bytes=ib)
     # bytes per product on the ELL copy: W (column, value) slots, the diagonal, the
     # overflow start, x gathered once and y written once per row, and the overflow
     # entries (column, value, gather)
     rows = N
     for which, key, s in ((1, "spmv_fp64", 8), (2, "spmv_fp32", 4)):
-        per_row = W * (4 + s) + s + 4 + 8 + 8
+        per_row = W * (ib + s) + s + 4 + 8 + 8
         per_over = 4 + s + 8
         for _ in range(3):
             dt.spmv(which, x, y)
@@ -396,6 +456,10 @@ def ensemble_bench(P=256, steps=1000, graph_steps=100):
                autotune_ms={int(t): round(float(m), 5) for t, m in
                             zip(list(b.info.candidate_tiles)[:b.info.n_candidates],
                                 list(b.info.candidate_ms)[:b.info.n_candidates])},
+               launches_per_step=2,
+               launches_note="tile kernel + shared-node fix-up, the Euler update and the Dirichlet "
+                             "callback fused into both; one graph replay holds graph_steps steps",
+               graph_steps=graph_steps, us_per_step=round(1e3 * ms_batch / steps, 2),
                batched_graph_ms=round(ms_batch, 3), batched_nograph_ms=round(ms_nograph, 3),
                batched_unfused_update_ms=round(ms_unfused, 3),
                looped_ms=round(ms_loop, 3), speedup_vs_loop=round(ms_loop / ms_batch, 2),
@@ -441,8 +505,10 @@ def system_bench(mesh, reps=50):
     return dict(workload="FVMSystem, Gray-Scott closures (2 variables), throughput mesh, zero-flux boundary",
                 nodes=int(N), variables=2, rhs_ms=round(ms, 5),
                 Gnode_updates_per_s=round(N / (ms * 1e-3) / 1e9, 3),
-                algorithmic_bytes_per_rhs=int(alg),
-                frac_hbm_peak=round(alg / (ms * 1e-3) / HBM_PEAK, 4))
+                canonical_bytes_per_rhs=int(alg),
+                canonical_bytes_GBps=round(alg / (ms * 1e-3) / 1e9, 1),
+                note="canonical bytes: the reference's data model (204-B triangle record); the kernel "
+                     "rebuilds the record from coordinates, so these bytes are not moved")
 
 
 def cpu_baseline(tri, u, budget_s, nthreads):
@@ -471,8 +537,22 @@ def cpu_baseline(tri, u, budget_s, nthreads):
         du = orc.rhs(u, 0.0, nthreads=nthreads)
         t_total += time.perf_counter() - t0
         reps += 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
     return du, dict(value=round(N * reps / t_total / 1e9, 5), unit="Gnode-updates/s",
-                    cores=nthreads, kind="port",
+                    cores=nthreads, kind="port", cpu_model=model, nproc=usable,
+                    cpu_count=os.cpu_count(),
                     sample=f"{reps} full evaluations of the same {N}-node mesh "
                            f"({t_total:.1f} s, OpenMP restatement of parallel_fvm_eqs!, "
                            f"{nthreads} threads)")
@@ -495,6 +575,7 @@ def main():
                     help="skip the host sparse-LU error check of the template solves")
     ap.add_argument("--ensemble", type=int, default=256, help="ensemble members (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--fit-iters", type=int, default=50, help="log D fit iterations (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the finite-volume path runs on a single MI355X here "
@@ -529,17 +610,48 @@ def main():
         fix.append(b)
     ms_tiles, ms_fix = float(np.mean(tiles)), float(np.mean(fix))
     alg = canonical_bytes(N, T, B)
-    achieved = alg / ((ms_tiles + ms_fix) * 1e-3)
-    traffic = None  # HBM bytes per RHS from the committed rocprofv3 counter runs of this build
-    tpath = os.path.join(ROOT, "profiles", "fvm_rhs_4M_traffic.json")
-    if os.path.exists(tpath):
-        tr = json.load(open(tpath))
-        if (tr.get("nodes") == N and tr.get("tiles") == int(mesh.info.n_tiles)
-                and tr.get("build") == _lib.build_info()):
-            traffic = int(tr["hbm_bytes_per_rhs"])
+    # per-RHS counts from the committed rocprofv3 counter runs of this build and mesh:
+    # HBM bytes (FETCH_SIZE + WRITE_SIZE, calibrated on the copy kernel) and VALU
+    # instructions of the two RHS kernels
+    cnt = None
+    if os.path.exists(COUNTERS):
+        c = json.load(open(COUNTERS))
+        if (c.get("nodes") == N and c.get("tiles") == int(mesh.info.n_tiles)
+                and c.get("build") == _lib.build_info()):
+            cnt = c
+    copy_gbps = copy_bandwidth()
+    log(f"copy kernel {copy_gbps / 1e9:.0f} GB/s")
     du_gpu = d_du.to_host()
     log(f"RHS {ms_step * 1e3:.1f} us/step ({value:.3f} Gnode/s); tiles {ms_tiles * 1e3:.1f} us, "
-        f"fix-up {ms_fix * 1e3:.1f} us; {achieved / 1e9:.0f} GB/s algorithmic")
+        f"fix-up {ms_fix * 1e3:.1f} us (HIP events)")
+    t_s = ms_step * 1e-3
+    if cnt is not None:
+        traffic = int(cnt["hbm_bytes_per_rhs"])
+        valu = float(cnt["valu_insts_per_rhs"])  # wave64 VALU instructions, tile + fix-up
+        achieved = valu * 64 * 2 / t_s  # FMA-equivalent flops of the issued VALU instructions
+        roof = {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": VALU_PEAK / 1e12,
+                "unit": "TFLOP/s", "frac": round(achieved / VALU_PEAK, 4), "traffic": traffic,
+                "valu_insts_per_rhs": int(valu),
+                "fp64_flops_per_rhs": int(cnt["fp64_flops_per_rhs"]),
+                "fp64_TFLOPs": round(cnt["fp64_flops_per_rhs"] / t_s / 1e12, 3),
+                "valu_busy_tile_kernel_rocprof": cnt["tile_kernel"]["valu_busy"],
+                "hbm": {"achieved": round(traffic / t_s / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": round(traffic / t_s / HBM_PEAK, 4),
+                        "frac_of_copy_kernel": round(traffic / t_s / copy_gbps, 4)}}
+    else:
+        traffic = None
+        roof = {"bound": "valu", "achieved": None, "peak": VALU_PEAK / 1e12, "unit": "TFLOP/s", "frac": None,
+                "traffic": None, "note": "no counter profile of this build and mesh under profiles/"}
+    roof.update({
+        "basis": "ms_per_step (back-to-back RHS loop; tile + fix-up kernels and their launch gap)",
+        "canonical_bytes_equivalent": {
+            "bytes_per_rhs": int(alg), "GBps": round(alg / t_s / 1e9, 1),
+            "note": "the reference's data model (204 B per triangle record, 25 B per node, 44 B per "
+                    "boundary edge) over the step time; these bytes are not moved: the tile kernel "
+                    "rebuilds each triangle record from the vertex coordinates"},
+        "build": _lib.build_info(),
+        "kernels": {"tile_kernel_ms": round(ms_tiles, 5), "fix_kernel_ms": round(ms_fix, 5),
+                    "timing": "HIP events around each kernel on the launch stream"}})
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "Gnode-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 5),
@@ -550,20 +662,14 @@ def main():
                                "homogeneous Dirichlet",
                    "nodes": N, "triangles": T, "boundary_edges": B,
                    "tiles": int(mesh.info.n_tiles), "fix_nodes": int(mesh.info.n_fix_nodes)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                     "measured_frac": (round(traffic / ((ms_tiles + ms_fix) * 1e-3) / HBM_PEAK, 4)
-                                       if traffic else None),
-                     "build": _lib.build_info(),
-                     "kernels": {"tile_kernel_ms": round(ms_tiles, 5),
-                                 "fix_kernel_ms": round(ms_fix, 5)},
-                     "algorithmic_bytes_per_rhs": int(alg)},
+        "roofline": roof,
+        "copy_bandwidth_GBps": round(copy_gbps / 1e9, 1),
     }
     if not args.no_extras:
-        out["copy_bandwidth_GBps"] = round(copy_bandwidth() / 1e9, 1)
-        log(f"copy kernel {out['copy_bandwidth_GBps']} GB/s")
         out["vjp_rhs"] = vjp_ratio(args.vjp_side)
         log(f"VJP/RHS {out['vjp_rhs']}")
+        if args.fit_iters > 0:
+            out["log_d_fit"] = fit_bench(args.vjp_side, iters=args.fit_iters)
         out["solve_tsit5"] = solve_bench(prob, ms_step)
         out["published_walkthroughs"] = published_walkthroughs()
         out["system_rhs"] = system_bench(mesh)

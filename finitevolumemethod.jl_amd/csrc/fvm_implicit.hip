@@ -1,0 +1,452 @@
+// TR-BDF2 for small problems whose right-hand side is affine in u, in one launch.
+//
+// The reference integrates the tutorials' stiff problems with TRBDF2 and a KLU
+// factorisation of the sparse Jacobian (solve.jl:167-183, jacobian_sparsity
+// solve.jl:50-77).  When f(u) = J u + c (linear fluxes, sources affine in u,
+// conditions constant in t), J is constant: the host package extracts it once
+// from fvm_jvp probes over a colouring of jacobian_sparsity's pattern, and this
+// kernel runs the whole integration in one workgroup of 1024 threads: both stages'
+// Newton iterations with restarted GMRES(m) on W = I - d h J (classical
+// Gram-Schmidt applied twice, the Hessenberg least squares by Givens rotations),
+// the W^-1-filtered error estimate, the step controller, the Dirichlet update and
+// the saveat interpolation, with barriers between phases and no host round trip.
+// A problem of a few thousand nodes is bound by kernel launches and host
+// synchronisations otherwise (about a dozen launches and a synchronisation per
+// Krylov iteration).  The algorithm is the host integrator's (solve.py), step for
+// step; reductions here sum in a different order, so results agree with it to the
+// Krylov and Newton tolerances, not bit for bit.
+#include <cmath>
+
+#include "fvm_internal.hpp"
+
+namespace {
+
+constexpr int kT = 1024;       // threads of the one workgroup
+constexpr int kWaves = kT / 64;
+constexpr int kMaxRestart = 30;
+constexpr int kChunk = 8;      // dot products reduced together
+constexpr int64_t kMaxN = 65536;
+
+constexpr double kG = 0.58578643762690485;  // 2 - sqrt(2)
+
+struct Trb {
+  int64_t n;
+  int W;
+  const int32_t* col;  // [W][n] slot-major
+  const double* val;
+  const double* diag;
+  const int32_t* orp;
+  const int32_t* ocol;
+  const double* oval;
+  const double* c;
+  int32_t n_dir;
+  const int32_t* dir_idx;
+  const double* dir_val;
+  double* v[14];  // u fn z fg u1 f1 r G dz est tmp interp w spare
+  double* basis;  // (restart + 1) n
+  double t0, tf, dt, abstol, reltol;
+  int64_t maxiters;
+  double kappa, krtol;
+  int max_newton, restart, smooth;
+  int64_t n_save;
+  const double* save_t;
+  double* saves;
+  double* stats;
+};
+
+enum { V_U, V_FN, V_Z, V_FG, V_U1, V_F1, V_R, V_G, V_DZ, V_EST, V_TMP, V_INTERP, V_W, V_SPARE };
+
+struct Shared {
+  double red[kWaves][kChunk];
+  double out[kChunk];
+  double H[kMaxRestart + 1][kMaxRestart];
+  double g[kMaxRestart + 1], cs[kMaxRestart], sn[kMaxRestart], y[kMaxRestart];
+};
+
+__device__ __forceinline__ void bar() { __syncthreads(); }
+
+// sums of kChunk per-thread values over the workgroup, in a fixed order (waves by
+// shuffles, then the waves in order); every thread gets the results
+__device__ void block_sums(Shared& S, double* v, int k) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = 0; j < k; ++j) {
+    double x = v[j];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) S.red[wave][j] = x;
+  }
+  bar();
+  if ((int)threadIdx.x < k) {
+    double x = 0.0;
+    for (int w = 0; w < kWaves; ++w) x += S.red[w][threadIdx.x];
+    S.out[threadIdx.x] = x;
+  }
+  bar();
+  for (int j = 0; j < k; ++j) v[j] = S.out[j];
+  bar();
+}
+
+__device__ double block_sum(Shared& S, double x) {
+  double v[kChunk] = {x};
+  block_sums(S, v, 1);
+  return v[0];
+}
+
+// y = J x (+ c when affine)
+__device__ void apply_J(const Trb& a, const double* x, double* y, bool affine) {
+  const int64_t n = a.n;
+  for (int64_t i = threadIdx.x; i < n; i += kT) {
+    double acc = 0.0;
+    for (int s = 0; s < a.W; ++s) acc += a.val[s * n + i] * x[a.col[s * n + i]];
+    for (int k = a.orp[i]; k < a.orp[i + 1]; ++k) acc += a.oval[k] * x[a.ocol[k]];
+    acc += a.diag[i] * x[i];
+    y[i] = affine ? acc + a.c[i] : acc;
+  }
+  bar();
+}
+
+// sqrt(mean((e / (abstol + reltol max(|p|, |q|)))^2)), the integrators' error norm
+__device__ double err_norm(Shared& S, const Trb& a, const double* e, const double* p, const double* q) {
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < a.n; i += kT) {
+    const double r = e[i] / (a.abstol + a.reltol * fmax(fabs(p[i]), fabs(q[i])));
+    acc += r * r;
+  }
+  return sqrt(block_sum(S, acc) / (double)a.n);
+}
+
+// x = W^-1 b approximately, W = I - dh J, restarted GMRES(m) from x = 0 (solve.py's
+// gmres_device: stop when |r| <= rtol |b|); returns the Arnoldi steps taken
+__device__ int gmres(Shared& S, const Trb& a, const double* b, double* x, double dh, int64_t maxiter) {
+  const int64_t n = a.n;
+  const int m = a.restart;
+  double* w = a.v[V_W];
+  double* r = a.v[V_SPARE];
+  for (int64_t i = threadIdx.x; i < n; i += kT) x[i] = 0.0;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kT) acc += b[i] * b[i];
+  const double bnorm = sqrt(block_sum(S, acc));
+  if (bnorm == 0.0) return 0;
+  int total = 0;
+  while (total < maxiter) {
+    // r = b - W x
+    apply_J(a, x, w, false);
+    acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kT) {
+      const double wx = x[i] - dh * w[i];
+      r[i] = b[i] - wx;
+      acc += r[i] * r[i];
+    }
+    const double beta = sqrt(block_sum(S, acc));
+    if (beta <= a.krtol * bnorm) break;
+    for (int64_t i = threadIdx.x; i < n; i += kT) a.basis[i] = r[i] / beta;
+    if (threadIdx.x == 0) {
+      for (int j = 0; j <= m; ++j) S.g[j] = 0.0;
+      S.g[0] = beta;
+    }
+    bar();
+    int kd = 0;
+    for (int j = 0; j < m; ++j) {
+      const double* vj = a.basis + j * n;
+      apply_J(a, vj, w, false);
+      for (int64_t i = threadIdx.x; i < n; i += kT) w[i] = vj[i] - dh * w[i];
+      bar();
+      // classical Gram-Schmidt applied twice: each pass forms every dot product with
+      // the same w (kChunk per sweep), then subtracts them all in one sweep
+      double h[kMaxRestart + 1], hp[kMaxRestart + 1];
+      for (int q = 0; q <= j; ++q) h[q] = 0.0;
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int q0 = 0; q0 <= j; q0 += kChunk) {
+          const int k = j + 1 - q0 < kChunk ? j + 1 - q0 : kChunk;
+          double dd[kChunk];
+          for (int q = 0; q < k; ++q) dd[q] = 0.0;
+          for (int64_t i = threadIdx.x; i < n; i += kT) {
+            const double wi = w[i];
+            for (int q = 0; q < k; ++q) dd[q] += a.basis[(q0 + q) * n + i] * wi;
+          }
+          block_sums(S, dd, k);
+          for (int q = 0; q < k; ++q) hp[q0 + q] = dd[q];
+        }
+        for (int64_t i = threadIdx.x; i < n; i += kT) {
+          double wi = w[i];
+          for (int q = 0; q <= j; ++q) wi = wi - hp[q] * a.basis[q * n + i];
+          w[i] = wi;
+        }
+        for (int q = 0; q <= j; ++q) h[q] += hp[q];
+        bar();
+      }
+      acc = 0.0;
+      for (int64_t i = threadIdx.x; i < n; i += kT) acc += w[i] * w[i];
+      const double hn = sqrt(block_sum(S, acc));
+      if (hn > 0.0)
+        for (int64_t i = threadIdx.x; i < n; i += kT) a.basis[(j + 1) * n + i] = w[i] / hn;
+      if (threadIdx.x == 0) {
+        for (int q = 0; q <= j; ++q) S.H[q][j] = h[q];
+        S.H[j + 1][j] = hn;
+        for (int q = 0; q < j; ++q) {  // previous rotations
+          const double p = S.H[q][j], o = S.H[q + 1][j];
+          S.H[q][j] = S.cs[q] * p + S.sn[q] * o;
+          S.H[q + 1][j] = -S.sn[q] * p + S.cs[q] * o;
+        }
+        const double dd = hypot(S.H[j][j], S.H[j + 1][j]);
+        S.cs[j] = dd == 0.0 ? 1.0 : S.H[j][j] / dd;
+        S.sn[j] = dd == 0.0 ? 0.0 : S.H[j + 1][j] / dd;
+        S.H[j][j] = dd;
+        S.H[j + 1][j] = 0.0;
+        S.g[j + 1] = -S.sn[j] * S.g[j];
+        S.g[j] = S.cs[j] * S.g[j];
+      }
+      bar();
+      ++total;
+      kd = j + 1;
+      if (fabs(S.g[j + 1]) <= a.krtol * bnorm || total >= maxiter) break;
+    }
+    if (threadIdx.x == 0) {  // back substitution
+      for (int q = kd - 1; q >= 0; --q) {
+        double sum = S.g[q];
+        for (int p = q + 1; p < kd; ++p) sum -= S.H[q][p] * S.y[p];
+        S.y[q] = sum / S.H[q][q];
+      }
+    }
+    bar();
+    for (int64_t i = threadIdx.x; i < n; i += kT) {
+      double xi = x[i];
+      for (int q = 0; q < kd; ++q) xi = xi + S.y[q] * a.basis[q * n + i];
+      x[i] = xi;
+    }
+    bar();
+    if (fabs(S.g[kd]) <= a.krtol * bnorm) break;
+  }
+  return total;
+}
+
+__device__ __forceinline__ void lincomb2(const Trb& a, double* out, double p, const double* x, double q,
+                                         const double* y) {
+  for (int64_t i = threadIdx.x; i < a.n; i += kT) out[i] = p * x[i] + q * y[i];
+  bar();
+}
+
+__global__ __launch_bounds__(kT) void trbdf2_affine_kernel(Trb a) {
+  __shared__ Shared S;
+  const int64_t n = a.n;
+  double *u = a.v[V_U], *fn = a.v[V_FN], *z = a.v[V_Z], *fg = a.v[V_FG], *u1 = a.v[V_U1], *f1 = a.v[V_F1];
+  double *r = a.v[V_R], *G = a.v[V_G], *dz = a.v[V_DZ], *est = a.v[V_EST], *tmp = a.v[V_TMP];
+  const double d = kG / 2;
+  const double TA = 1.0 / (kG * (2.0 - kG)), TB = -(1.0 - kG) * (1.0 - kG) / (kG * (2.0 - kG));
+  const double TK = (3 * kG * kG - 4 * kG + 2) / (6 * (2.0 - kG));
+  int64_t nf = 0, njvp = 0, naccept = 0, nreject = 0, nnewton = 0, nkrylov = 0, nfail = 0, ks = 0, it = 0;
+  int retcode = 0;  // FVM_ODE_SUCCESS
+  double t = a.t0, dt = a.dt, eta_keep = 1.0;
+  apply_J(a, u, fn, true);
+  ++nf;
+  // Newton on z - d h f(z) = rhs from the predictor in z; fz <- f(z); false on failure
+  auto newton = [&](double* zz, const double* rhs, double h, double* fz) -> bool {
+    const double dh = d * h;
+    double eta = fvm::ctrl_pow(fmax(eta_keep, 1e-16), 0.8), prev = -1.0;
+    bool ok = false;
+    for (int itn = 0; itn < a.max_newton; ++itn) {
+      apply_J(a, zz, fz, true);
+      ++nf;
+      for (int64_t i = threadIdx.x; i < n; i += kT) G[i] = -1.0 * zz[i] + dh * fz[i] + rhs[i];
+      bar();
+      const int ki = gmres(S, a, G, dz, dh, 20 * (int64_t)a.restart);
+      nkrylov += ki;
+      njvp += ki + 1;
+      for (int64_t i = threadIdx.x; i < n; i += kT) zz[i] = zz[i] + dz[i];
+      bar();
+      ++nnewton;
+      const double ndz = err_norm(S, a, dz, u, zz);
+      if (!isfinite(ndz)) return false;
+      if (prev >= 0.0) {
+        const double theta = ndz / prev;
+        if (theta >= 1.0) return false;
+        eta = theta / (1.0 - theta);
+      }
+      if (eta * ndz <= a.kappa || ndz == 0.0) {
+        ok = true;
+        break;
+      }
+      prev = ndz;
+    }
+    if (!ok) return false;
+    eta_keep = eta;
+    apply_J(a, zz, fz, true);
+    ++nf;
+    return true;
+  };
+  while (t < a.tf && it < a.maxiters) {
+    ++it;
+    const double h = fmin(dt, a.tf - t);
+    // TR stage: z - d h f(z) = u + d h f_n, predictor u + gamma h f_n
+    lincomb2(a, r, 1.0, u, d * h, fn);
+    lincomb2(a, z, 1.0, u, kG * h, fn);
+    bool ok = newton(z, r, h, fg);
+    if (ok) {
+      // BDF2 stage: u1 - d h f(u1) = A z + B u, predictor z + (1 - gamma) h f_gamma
+      lincomb2(a, r, TA, z, TB, u);
+      lincomb2(a, u1, 1.0, z, (1.0 - kG) * h, fg);
+      ok = newton(u1, r, h, f1);
+    }
+    if (!ok) {
+      ++nfail;
+      ++nreject;
+      dt = h / 4;
+      if (dt < 1e-14 * fmax(1.0, fabs(t))) {
+        retcode = 3;  // ConvergenceFailure
+        break;
+      }
+      continue;
+    }
+    {
+      const double c0 = TK * h / kG, c1 = -TK * h / (kG * (1 - kG)), c2 = TK * h / (1 - kG);
+      for (int64_t i = threadIdx.x; i < n; i += kT) {
+        double v = c0 * fn[i];
+        v = v + c1 * fg[i];
+        v = v + c2 * f1[i];
+        est[i] = v;
+      }
+      bar();
+    }
+    const double* e = est;
+    if (a.smooth) {
+      const int ki = gmres(S, a, est, tmp, d * h, 20 * (int64_t)a.restart);
+      nkrylov += ki;
+      njvp += ki + 1;
+      e = tmp;
+    }
+    const double EEst = err_norm(S, a, e, u, u1);
+    if (!isfinite(EEst)) {
+      retcode = 2;  // Unstable
+      break;
+    }
+    if (EEst <= 1.0) {
+      const double tn = t + h;
+      while (ks < a.n_save && a.save_t[ks] <= tn + 1e-12 * fmax(1.0, fabs(tn))) {
+        const double th = fmin(1.0, fmax(0.0, (a.save_t[ks] - t) / h));
+        const double t2 = th * th, t3 = t2 * th;
+        const double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + th, h01 = -2 * t3 + 3 * t2, h11 = t3 - t2;
+        const double q0 = h00, q1 = h10 * h, q2 = h01, q3 = h11 * h;
+        double* out = a.saves + ks * n;
+        for (int64_t i = threadIdx.x; i < n; i += kT) {
+          double v = q0 * u[i];
+          v = v + q1 * fn[i];
+          v = v + q2 * u1[i];
+          v = v + q3 * f1[i];
+          out[i] = v;
+        }
+        ++ks;
+      }
+      bar();
+      for (int64_t i = threadIdx.x; i < n; i += kT) u[i] = u1[i];
+      bar();
+      t = tn;
+      ++naccept;
+      if (a.n_dir > 0) {  // the Dirichlet callback (values constant), then f(u) afresh
+        for (int k = threadIdx.x; k < a.n_dir; k += kT) u[a.dir_idx[k]] = a.dir_val[k];
+        bar();
+        apply_J(a, u, fn, true);
+        ++nf;
+      } else {
+        for (int64_t i = threadIdx.x; i < n; i += kT) fn[i] = f1[i];
+        bar();
+      }
+    } else {
+      ++nreject;
+    }
+    const double q = 0.9 * fvm::ctrl_pow(fmax(EEst, 1e-10), -1.0 / 3.0);
+    dt = h * fmin(5.0, fmax(0.2, q));
+    if (dt < 1e-14 * fmax(1.0, fabs(t))) {  // the step collapsed: stop instead of creeping on
+      retcode = 3;
+      break;
+    }
+  }
+  if (retcode == 0 && !(t >= a.tf)) retcode = 1;  // MaxIters
+  if (threadIdx.x == 0) {
+    double* st = a.stats;
+    st[0] = t;
+    st[1] = (double)nf;
+    st[2] = (double)njvp;
+    st[3] = (double)naccept;
+    st[4] = (double)nreject;
+    st[5] = (double)nnewton;
+    st[6] = (double)nkrylov;
+    st[7] = (double)nfail;
+    st[8] = (double)ks;
+    st[9] = (double)retcode;
+  }
+}
+
+}  // namespace
+
+extern "C" int fvm_trbdf2_affine(const fvm_affine_op* op, const fvm_trbdf2_opts* o, double* d_u,
+                                 fvm_trbdf2_stats* st, fvm_stream s) {
+  FVM_CHECK(op && o && d_u && st, "fvm_trbdf2_affine: null argument");
+  FVM_CHECK(op->n > 0 && op->n <= kMaxN, "fvm_trbdf2_affine: 1 <= n <= 65536 (one workgroup)");
+  FVM_CHECK(op->width >= 0 && op->d_col && op->d_val && op->d_diag && op->d_ovf_rp && op->d_c,
+            "fvm_trbdf2_affine: incomplete operator");
+  FVM_CHECK(op->n_dirichlet == 0 || (op->d_dir_nodes && op->d_dir_values), "fvm_trbdf2_affine: Dirichlet lists");
+  FVM_CHECK(o->krylov_restart >= 1 && o->krylov_restart <= kMaxRestart, "fvm_trbdf2_affine: restart 1..30");
+  FVM_CHECK(o->max_newton >= 1 && o->abstol > 0 && o->reltol >= 0 && o->maxiters > 0 && o->tf >= o->t0 &&
+                o->dt > 0,
+            "fvm_trbdf2_affine: bad options");
+  FVM_CHECK(o->n_save >= 0 && (o->n_save == 0 || (o->save_t && o->d_saves)), "fvm_trbdf2_affine: save times");
+  const int64_t n = op->n;
+  Trb a;
+  a.n = n;
+  a.W = op->width;
+  a.col = op->d_col;
+  a.val = op->d_val;
+  a.diag = op->d_diag;
+  a.orp = op->d_ovf_rp;
+  a.ocol = op->d_ovf_col;
+  a.oval = op->d_ovf_val;
+  a.c = op->d_c;
+  a.n_dir = op->n_dirichlet;
+  a.dir_idx = op->d_dir_nodes;
+  a.dir_val = op->d_dir_values;
+  a.t0 = o->t0;
+  a.tf = o->tf;
+  a.dt = o->dt;
+  a.abstol = o->abstol;
+  a.reltol = o->reltol;
+  a.maxiters = o->maxiters;
+  a.kappa = o->newton_kappa;
+  a.krtol = o->krylov_rtol;
+  a.max_newton = o->max_newton;
+  a.restart = o->krylov_restart;
+  a.smooth = o->smooth_est;
+  a.n_save = o->n_save;
+  a.saves = o->d_saves;
+  const int nvec = 14 + o->krylov_restart + 1;
+  double* base = nullptr;
+  FVM_HIP(hipMalloc((void**)&base, sizeof(double) * ((size_t)nvec * n + (size_t)o->n_save + 16)));
+  for (int k = 0; k < 14; ++k) a.v[k] = base + k * n;
+  a.basis = base + 14 * n;
+  double* tail = base + (size_t)nvec * n;
+  a.save_t = tail;
+  a.stats = tail + o->n_save;
+  hipStream_t st_ = (hipStream_t)s;
+  hipError_t e = hipMemcpyAsync(a.v[V_U], d_u, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st_);
+  if (e == hipSuccess && o->n_save > 0)
+    e = hipMemcpyAsync(tail, o->save_t, sizeof(double) * (size_t)o->n_save, hipMemcpyHostToDevice, st_);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(trbdf2_affine_kernel, dim3(1), dim3(kT), 0, st_, a);
+    e = hipGetLastError();
+  }
+  double r[10] = {0};
+  if (e == hipSuccess) e = hipMemcpyAsync(d_u, a.v[V_U], sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st_);
+  if (e == hipSuccess) e = hipMemcpyAsync(r, a.stats, sizeof r, hipMemcpyDeviceToHost, st_);
+  if (e == hipSuccess) e = hipStreamSynchronize(st_);
+  (void)hipFree(base);
+  if (e != hipSuccess) return fvm::hip_fail(e, "fvm_trbdf2_affine");
+  st->t = r[0];
+  st->nf = (int64_t)r[1];
+  st->njvp = (int64_t)r[2];
+  st->naccept = (int64_t)r[3];
+  st->nreject = (int64_t)r[4];
+  st->nnewton = (int64_t)r[5];
+  st->nkrylov = (int64_t)r[6];
+  st->nnewton_fail = (int64_t)r[7];
+  st->nsaved = (int64_t)r[8];
+  st->retcode = (int32_t)r[9];
+  return FVM_OK;
+}

@@ -87,7 +87,8 @@ struct fvm_template {
   int ellW = 0;
   int64_t ell_overflow = 0;
   bool use_ell = false;      // this solve's choice (fvm_solve_opts.spmv_format)
-  int32_t* ell_col = nullptr;
+  int32_t* ell_col = nullptr;  // column of each slot, or (ell_off) its offset from the row
+  int16_t* ell_off = nullptr;  // used instead of ell_col when every |column - row| < 2^15
   double* ellM = nullptr;
   float* ellM32 = nullptr;
   double* diagM = nullptr;
@@ -496,14 +497,19 @@ __global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, int proj, d
   store_partial(pq, part + PT_PQ * kMaxG);
 }
 
+// The column of an ELL slot: stored as is (int32) or as its offset from the row
+// (int16, when the whole operator allows it: two bytes fewer per slot).
+__device__ __forceinline__ int ell_column(int32_t c, int64_t) { return c; }
+__device__ __forceinline__ int ell_column(int16_t d, int64_t i) { return (int)i + (int)d; }
+
 // Rows of the ELL copy of M, one per thread: the W column / value pairs of a row
 // are coalesced loads issued together, then the W gathers, so a row costs two
 // dependent loads instead of the CSR form's three (row start, entries, gathers)
 // and every lane works.  body(i, g(i), y_i) with y_i = the off-diagonal terms in
 // CSR order (v (g(c) - g(i)) in the difference form, v g(c) otherwise), then the
 // row's overflow entries, then diag_i g(i).
-template <bool DIFF, int W, class V, class Gather, class Body>
-__device__ __forceinline__ void ell_rows(int64_t n, const int32_t* __restrict__ ecol, const V* __restrict__ eval,
+template <bool DIFF, int W, class V, class CI, class Gather, class Body>
+__device__ __forceinline__ void ell_rows(int64_t n, const CI* __restrict__ ecol, const V* __restrict__ eval,
                                          const V* __restrict__ diag, const int32_t* __restrict__ orp,
                                          const int32_t* __restrict__ ocol, const V* __restrict__ oval, Gather g,
                                          Body body) {
@@ -512,7 +518,7 @@ __device__ __forceinline__ void ell_rows(int64_t n, const int32_t* __restrict__ 
     V v[W];
 #pragma unroll
     for (int s = 0; s < W; ++s) {
-      c[s] = ecol[s * n + i];
+      c[s] = ell_column(ecol[s * n + i], i);
       v[s] = eval[s * n + i];
     }
     const int o0 = orp[i], o1 = orp[i + 1];
@@ -527,9 +533,9 @@ __device__ __forceinline__ void ell_rows(int64_t n, const int32_t* __restrict__ 
 }
 
 // The CG direction iteration on the ELL copy of M.
-template <class V, int W>
+template <class V, int W, class CI>
 __global__ __launch_bounds__(kRT) void cg_dir_ell(int64_t n, int k, int proj, double irt2, double tabs2,
-                                                  const int32_t* __restrict__ ecol, const V* __restrict__ eval,
+                                                  const CI* __restrict__ ecol, const V* __restrict__ eval,
                                                   const V* __restrict__ diag, const int32_t* __restrict__ orp,
                                                   const int32_t* __restrict__ ocol, const V* __restrict__ oval,
                                                   const V* __restrict__ z, const V* __restrict__ pold,
@@ -685,8 +691,8 @@ __global__ __launch_bounds__(kRT) void bi_spmv(int64_t n, const int32_t* __restr
 }
 
 // bi_spmv on the ELL copy of M (FP64).
-template <int MODE, int W>
-__global__ __launch_bounds__(kRT) void bi_spmv_ell(int64_t n, const int32_t* __restrict__ ecol,
+template <int MODE, int W, class CI>
+__global__ __launch_bounds__(kRT) void bi_spmv_ell(int64_t n, const CI* __restrict__ ecol,
                                                    const double* __restrict__ eval, const double* __restrict__ diag,
                                                    const int32_t* __restrict__ orp, const int32_t* __restrict__ ocol,
                                                    const double* __restrict__ oval, const double* __restrict__ x,
@@ -820,8 +826,8 @@ __global__ __launch_bounds__(kRT) void spmv_kernel(int64_t n, const int32_t* __r
 }
 
 // y = A x (+ b) on an ELL copy (precision V, the difference form for FP32)
-template <class V, int W>
-__global__ __launch_bounds__(kRT) void spmv_ell_kernel(int64_t n, const int32_t* __restrict__ ecol,
+template <class V, int W, class CI>
+__global__ __launch_bounds__(kRT) void spmv_ell_kernel(int64_t n, const CI* __restrict__ ecol,
                                                        const V* __restrict__ eval, const V* __restrict__ diag,
                                                        const int32_t* __restrict__ orp,
                                                        const int32_t* __restrict__ ocol, const V* __restrict__ oval,
@@ -829,6 +835,200 @@ __global__ __launch_bounds__(kRT) void spmv_ell_kernel(int64_t n, const int32_t*
                                                        double* __restrict__ y) {
   ell_rows<kDiffForm<V>, W>(n, ecol, eval, diag, orp, ocol, oval, [&](int c) { return (V)x[c]; },
                             [&](int64_t i, V, V yi) { y[i] = b ? (double)yi + b[i] : (double)yi; });
+}
+
+// ----------------------------------------------------------------------------
+// Tsit5 on a small ODE template in one workgroup
+// ----------------------------------------------------------------------------
+// The adaptive loop of fvm_ode_solve (fvm_ode.hip) for du/dt = A u + b when the
+// state is small: one workgroup of 1024 threads runs every step (stages, products
+// on the ELL copy of A, error norm, step control, saveat, first same as last) with
+// barriers between the phases, so the integration is one launch instead of ~13
+// launches and a host synchronisation per step.  Each element's arithmetic is the
+// multi-kernel path's: the same row products (ell_rows' order), the same linear
+// combinations (terms left to right), and the error norm's sum in the fixed order
+// of its 512-workgroup grid, emulated here block by block; the step controller's
+// power is fvm_ctrl.hpp's, the host loops' own: the same states bit for bit.
+constexpr int kSmallThreads = 1024;
+constexpr int64_t kSmallTsit5Max = 65536;  // one workgroup pays off below this many unknowns
+constexpr int kRedBlocksE = 512, kRedThreadsE = 256;  // the grid whose summation order is reproduced
+
+struct SmallTsit5 {
+  int64_t n;
+  const int32_t* ecol;  // one of ecol / eoff
+  const int16_t* eoff;
+  const double* eval;
+  const double* diag;
+  const int32_t* orp;
+  const int32_t* ocol;
+  const double* oval;
+  const double* b;
+  double* k[7];  // the stages (k[0] = f(u) on entry)
+  double* utmp;
+  double* u;     // the state on entry
+  double* unew;
+  double* u_out;
+  double t0, tf, dt;
+  double abstol, reltol;
+  int64_t maxiters, it0;
+  int32_t fsal;
+  int64_t n_save, ks0;
+  const double* save_t;  // device copy
+  double* saves;
+  double* stats;  // [0] t, [1] naccept, [2] nreject, [3] nf, [4] nsaved, [5] retcode
+  double A[7][6], C[7], Bt[7];
+};
+
+template <int W, bool OFF>
+__device__ __forceinline__ double ell_row_apply(const SmallTsit5& a, int64_t i, const double* x) {
+  const int64_t n = a.n;
+  int c[W];
+  double v[W];
+#pragma unroll
+  for (int s = 0; s < W; ++s) {
+    c[s] = OFF ? ell_column(a.eoff[s * n + i], i) : a.ecol[s * n + i];
+    v[s] = a.eval[s * n + i];
+  }
+  const int o0 = a.orp[i], o1 = a.orp[i + 1];
+  const double gi = x[i];
+  double acc = 0;
+#pragma unroll
+  for (int s = 0; s < W; ++s) acc += v[s] * x[c[s]];
+  for (int kk = o0; kk < o1; ++kk) acc += a.oval[kk] * x[a.ocol[kk]];
+  acc += a.diag[i] * gi;
+  return acc + a.b[i];
+}
+
+template <int W, bool OFF>
+__global__ __launch_bounds__(kSmallThreads) void tsit5_small_kernel(SmallTsit5 a) {
+  __shared__ double sh[kSmallThreads];
+  __shared__ double part[kRedBlocksE];
+  const int tid = threadIdx.x;
+  const int64_t n = a.n;
+  double* k[7];
+  for (int j = 0; j < 7; ++j) k[j] = a.k[j];
+  double* utmp = a.utmp;
+  double* u = a.u;
+  double* unew = a.unew;
+  double t = a.t0, dt = a.dt, qold = 1e-4;
+  int64_t it = a.it0, ks = a.ks0, naccept = 0, nreject = 0, nf = 0;
+  int retcode = FVM_ODE_SUCCESS;
+  const int nblk = (int)((n + kRedThreadsE - 1) / kRedThreadsE);  // nonempty blocks of the emulated grid
+  while (t < a.tf && it < a.maxiters) {
+    ++it;
+    dt = fmin(dt, a.tf - t);
+    for (int j = 1; j < 7; ++j) {
+      double c[7];
+      c[0] = 1.0;
+      for (int i = 0; i < j; ++i) c[i + 1] = dt * a.A[j][i];
+      double* dst = j == 6 ? unew : utmp;
+      for (int64_t e = tid; e < n; e += kSmallThreads) {
+        double v = c[0] * u[e];
+        for (int i = 0; i < j; ++i) v = v + c[i + 1] * k[i][e];
+        dst[e] = v;
+      }
+      __syncthreads();
+      for (int64_t e = tid; e < n; e += kSmallThreads) k[j][e] = ell_row_apply<W, OFF>(a, e, dst);
+      ++nf;
+      __syncthreads();
+    }
+    // the error norm: per element r = (sum_j dt bt_j k_j) / (abstol + reltol max(|u|, |unew|)),
+    // summed as the 512 x 256 grid of the multi-kernel path sums it
+    double cb[7];
+    for (int j = 0; j < 7; ++j) cb[j] = dt * a.Bt[j];
+    for (int b0 = 0; b0 < nblk; b0 += kSmallThreads / kRedThreadsE) {
+      const int blk = b0 + tid / kRedThreadsE, lt = tid % kRedThreadsE;
+      const int64_t e = (int64_t)blk * kRedThreadsE + lt;
+      double acc = 0.0;
+      if (blk < nblk && e < n) {
+        double v = cb[0] * k[0][e];
+        for (int j = 1; j < 7; ++j) v = v + cb[j] * k[j][e];
+        const double sc = a.abstol + a.reltol * fmax(fabs(u[e]), fabs(unew[e]));
+        const double r = v / sc;
+        acc += r * r;
+      }
+      sh[tid] = acc;
+      __syncthreads();
+      for (int s = kRedThreadsE / 2; s > 0; s >>= 1) {
+        if (lt < s) sh[tid] += sh[tid + s];
+        __syncthreads();
+      }
+      if (lt == 0 && blk < nblk) part[blk] = sh[tid];
+      __syncthreads();
+    }
+    for (int p = nblk + tid; p < kRedBlocksE; p += kSmallThreads) part[p] = 0.0;
+    __syncthreads();
+    if (tid < kRedThreadsE) {
+      double acc = 0.0;
+      for (int i = tid; i < kRedBlocksE; i += kRedThreadsE) acc += part[i];
+      sh[tid] = acc;
+    }
+    __syncthreads();
+    for (int s = kRedThreadsE / 2; s > 0; s >>= 1) {
+      if (tid < s) sh[tid] += sh[tid + s];
+      __syncthreads();
+    }
+    const double EEst = sqrt(sh[0] / (double)n);
+    __syncthreads();  // every thread has read sh[0]
+    if (!isfinite(EEst)) {
+      retcode = FVM_ODE_UNSTABLE;
+      break;
+    }
+    if (EEst <= 1.0) {
+      const double tn = t + dt;
+      while (ks < a.n_save && a.save_t[ks] <= tn + 1e-12 * fmax(1.0, fabs(tn))) {
+        const double th = fmin(1.0, fmax(0.0, (a.save_t[ks] - t) / dt));
+        const double t2 = th * th, t3 = t2 * th;
+        const double h00 = 2 * t3 - 3 * t2 + 1;
+        const double h10 = t3 - 2 * t2 + th;
+        const double h01 = -2 * t3 + 3 * t2;
+        const double h11 = t3 - t2;
+        const double c0 = h00, c1 = h10 * dt, c2 = h01, c3 = h11 * dt;
+        double* out = a.saves + ks * n;
+        for (int64_t e = tid; e < n; e += kSmallThreads) {
+          double v = c0 * u[e];
+          v = v + c1 * k[0][e];
+          v = v + c2 * unew[e];
+          v = v + c3 * k[6][e];
+          out[e] = v;
+        }
+        ++ks;
+      }
+      double* sw = u;
+      u = unew;
+      unew = sw;
+      t = tn;
+      ++naccept;
+      if (a.fsal) {
+        double* k0 = k[0];
+        k[0] = k[6];
+        k[6] = k0;
+      } else {
+        __syncthreads();
+        for (int64_t e = tid; e < n; e += kSmallThreads) k[0][e] = ell_row_apply<W, OFF>(a, e, u);
+        ++nf;
+      }
+      double q = fvm::ctrl_pow(fmax(EEst, 1e-16), 7.0 / 50.0) / fvm::ctrl_pow(qold, 2.0 / 25.0);
+      q = fmin(5.0, fmax(0.1, q / 0.9));
+      dt = dt / q;
+      qold = fmax(EEst, 1e-4);
+    } else {
+      ++nreject;
+      const double q = fmin(5.0, fvm::ctrl_pow(fmax(EEst, 1e-16), 7.0 / 50.0) / 0.9);
+      dt = dt / q;
+    }
+    __syncthreads();
+  }
+  if (retcode == FVM_ODE_SUCCESS && !(t >= a.tf)) retcode = FVM_ODE_MAXITERS;
+  for (int64_t e = tid; e < n; e += kSmallThreads) a.u_out[e] = u[e];
+  if (tid == 0) {
+    a.stats[0] = t;
+    a.stats[1] = (double)naccept;
+    a.stats[2] = (double)nreject;
+    a.stats[3] = (double)nf;
+    a.stats[4] = (double)ks;
+    a.stats[5] = (double)retcode;
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -852,7 +1052,7 @@ int dupload(T** dst, const T* src, int64_t n) {
 void free_tmpl(fvm_template* t) {
   if (!t) return;
   void* p[] = {t->row_ptr, t->col, t->val, t->valM, t->valM32, t->b, t->fixed, t->vol, t->dinv,
-               t->wv, t->dinv32, t->wv32, t->work, t->part, t->sc, t->ell_col, t->ellM,
+               t->wv, t->dinv32, t->wv32, t->work, t->part, t->sc, t->ell_col, t->ell_off, t->ellM,
                t->ellM32, t->diagM, t->diagM32, t->ovf_rp, t->ovf_col, t->ovfM, t->ovfM32,
                t->ellA, t->diagA, t->ovfA};
   for (void* q : p)
@@ -899,6 +1099,20 @@ void by_width(int W, F&& f) {
   }
 }
 
+// f(width tag, column array) for the template's ELL copy: its width as a constant
+// and its columns as int32 columns or int16 offsets
+template <class P>
+using ci_t = std::remove_cv_t<std::remove_pointer_t<P>>;
+template <class F>
+void by_ell(const fvm_template* t, F&& f) {
+  by_width(t->ellW, [&](auto w) {
+    if (t->ell_off)
+      f(w, (const int16_t*)t->ell_off);
+    else
+      f(w, (const int32_t*)t->ell_col);
+  });
+}
+
 // One CG direction step (p, q = M p, partials of p.q) on this solve's form of M;
 // valM is the CSR value array of the precision V.
 template <class V>
@@ -909,9 +1123,9 @@ void launch_dir(const fvm_template* t, const V* valM, int k, int proj, double ir
                        t->col, valM, z, pold, pnew, q, t->sc, t->part);
     return;
   }
-  by_width(t->ellW, [&](auto w) {
-    hipLaunchKernelGGL((cg_dir_ell<V, decltype(w)::value>), grid(t), dim3(kRT), 0, s, t->n, k, proj, irt2, tabs2,
-                       t->ell_col, EllVals<V>::ell(t), EllVals<V>::diag(t), t->ovf_rp, t->ovf_col,
+  by_ell(t, [&](auto w, auto ec) {
+    hipLaunchKernelGGL((cg_dir_ell<V, decltype(w)::value, ci_t<decltype(ec)>>), grid(t), dim3(kRT), 0, s, t->n, k,
+                       proj, irt2, tabs2, ec, EllVals<V>::ell(t), EllVals<V>::diag(t), t->ovf_rp, t->ovf_col,
                        EllVals<V>::ovf(t), z, pold, pnew, q, t->sc, t->part);
   });
 }
@@ -924,9 +1138,9 @@ void launch_bi_spmv(const fvm_template* t, const double* x, double* y, const dou
                        t->part);
     return;
   }
-  by_width(t->ellW, [&](auto w) {
-    hipLaunchKernelGGL((bi_spmv_ell<MODE, decltype(w)::value>), grid(t), dim3(kRT), 0, s, t->n, t->ell_col, t->ellM,
-                       t->diagM, t->ovf_rp, t->ovf_col, t->ovfM, x, y, a, t->sc, t->part);
+  by_ell(t, [&](auto w, auto ec) {
+    hipLaunchKernelGGL((bi_spmv_ell<MODE, decltype(w)::value, ci_t<decltype(ec)>>), grid(t), dim3(kRT), 0, s, t->n,
+                       ec, t->ellM, t->diagM, t->ovf_rp, t->ovf_col, t->ovfM, x, y, a, t->sc, t->part);
   });
 }
 
@@ -934,16 +1148,17 @@ void launch_bi_spmv(const fvm_template* t, const double* x, double* y, const dou
 // copy when there is one, else on CSR.
 void spmv_any(const fvm_template* t, int which, const double* x, const double* b, double* y, hipStream_t s) {
   if (t->ellW > 0) {
-    by_width(t->ellW, [&](auto w) {
+    by_ell(t, [&](auto w, auto ec) {
       constexpr int W = decltype(w)::value;
+      using CI = ci_t<decltype(ec)>;
       if (which == 0)
-        hipLaunchKernelGGL((spmv_ell_kernel<double, W>), grid(t), dim3(kRT), 0, s, t->n, t->ell_col, t->ellA,
+        hipLaunchKernelGGL((spmv_ell_kernel<double, W, CI>), grid(t), dim3(kRT), 0, s, t->n, ec, t->ellA,
                            t->diagA, t->ovf_rp, t->ovf_col, t->ovfA, x, b, y);
       else if (which == 1)
-        hipLaunchKernelGGL((spmv_ell_kernel<double, W>), grid(t), dim3(kRT), 0, s, t->n, t->ell_col, t->ellM,
+        hipLaunchKernelGGL((spmv_ell_kernel<double, W, CI>), grid(t), dim3(kRT), 0, s, t->n, ec, t->ellM,
                            t->diagM, t->ovf_rp, t->ovf_col, t->ovfM, x, b, y);
       else
-        hipLaunchKernelGGL((spmv_ell_kernel<float, W>), grid(t), dim3(kRT), 0, s, t->n, t->ell_col, t->ellM32,
+        hipLaunchKernelGGL((spmv_ell_kernel<float, W, CI>), grid(t), dim3(kRT), 0, s, t->n, ec, t->ellM32,
                            t->diagM32, t->ovf_rp, t->ovf_col, t->ovfM32, x, b, y);
     });
     return;
@@ -994,11 +1209,28 @@ int build_ell(fvm_template* t, const std::vector<int32_t>& rp, const std::vector
     orp[i + 1] = (int32_t)ocol.size();
   }
   const int64_t no = (int64_t)ocol.size();
+  // the slots' columns as 16-bit offsets from the row when every one fits
+  bool fits16 = true;
+  for (int s = 0; s < W && fits16; ++s)
+    for (int64_t i = 0; i < N; ++i) {
+      const int64_t d = (int64_t)ecol[(size_t)s * N + i] - i;
+      if (d < -32768 || d > 32767) {
+        fits16 = false;
+        break;
+      }
+    }
   int32_t *d_esrc = nullptr, *d_dsrc = nullptr, *d_osrc = nullptr;
   int rc = FVM_OK;
   do {
-    if ((rc = dupload(&t->ell_col, ecol.data(), (int64_t)W * N)) || (rc = dupload(&t->ovf_rp, orp.data(), N + 1)) ||
-        (rc = dupload(&t->ovf_col, ocol.data(), no)))
+    if (fits16) {
+      std::vector<int16_t> eoff((size_t)W * N);
+      for (int s = 0; s < W; ++s)
+        for (int64_t i = 0; i < N; ++i) eoff[(size_t)s * N + i] = (int16_t)(ecol[(size_t)s * N + i] - i);
+      if ((rc = dupload(&t->ell_off, eoff.data(), (int64_t)W * N))) break;
+    } else if ((rc = dupload(&t->ell_col, ecol.data(), (int64_t)W * N))) {
+      break;
+    }
+    if ((rc = dupload(&t->ovf_rp, orp.data(), N + 1)) || (rc = dupload(&t->ovf_col, ocol.data(), no)))
       break;
     if ((rc = dalloc(&t->ellM, (int64_t)W * N)) || (rc = dalloc(&t->ellM32, (int64_t)W * N)) ||
         (rc = dalloc(&t->diagM, N)) || (rc = dalloc(&t->diagM32, N)) || (rc = dalloc(&t->ovfM, no)) ||
@@ -1298,10 +1530,11 @@ int fvm_template_destroy(fvm_template* t) {
   return FVM_OK;
 }
 
-int fvm_template_ell(const fvm_template* t, int32_t* width, int64_t* overflow) {
+int fvm_template_ell(const fvm_template* t, int32_t* width, int64_t* overflow, int32_t* index_bytes) {
   FVM_CHECK(t && width && overflow, "fvm_template_ell: null argument");
   *width = t->ellW;
   *overflow = t->ell_overflow;
+  if (index_bytes) *index_bytes = t->ell_off ? 2 : 4;
   return FVM_OK;
 }
 
@@ -1420,3 +1653,79 @@ int fvm_template_solve(fvm_template* t, const fvm_solve_opts* o, double* d_x, fv
 }
 
 }  // extern "C"
+
+namespace fvm {
+
+// The Tsit5 loop of fvm_ode_solve on the template's ELL copy of A in one launch
+// (tsit5_small_kernel), from time t0 with step dt, it0 steps and ks0 saves done,
+// k[0] = f(u) already computed; the final state goes to d_u.  Returns
+// FVM_ERR_STATE when the template has no ELL copy or n exceeds kSmallTsit5Max, so
+// that the caller keeps the multi-kernel loop.
+int template_tsit5_small(const fvm_template* t, const fvm_ode_opts* o, double* const k[7], double* utmp,
+                         double* u, double* unew, double* d_u, double t0, double dt, int64_t it0, int64_t ks0,
+                         const double A[7][6], const double C[7], const double Bt[7], fvm_ode_stats* st,
+                         hipStream_t s) {
+  if (!t || t->ellW <= 0 || o->n != t->n || o->n > kSmallTsit5Max) return FVM_ERR_STATE;
+  SmallTsit5 a;
+  a.n = o->n;
+  a.ecol = t->ell_col;
+  a.eoff = t->ell_off;
+  a.eval = t->ellA;
+  a.diag = t->diagA;
+  a.orp = t->ovf_rp;
+  a.ocol = t->ovf_col;
+  a.oval = t->ovfA;
+  a.b = t->b;
+  for (int j = 0; j < 7; ++j) a.k[j] = k[j];
+  a.utmp = utmp;
+  a.u = u;
+  a.unew = unew;
+  a.u_out = d_u;
+  a.t0 = t0;
+  a.tf = o->tf;
+  a.dt = dt;
+  a.abstol = o->abstol;
+  a.reltol = o->reltol;
+  a.maxiters = o->maxiters;
+  a.it0 = it0;
+  a.fsal = o->fsal;
+  a.n_save = o->n_save;
+  a.ks0 = ks0;
+  a.saves = o->d_saves;
+  for (int j = 0; j < 7; ++j) {
+    for (int i = 0; i < 6; ++i) a.A[j][i] = A[j][i];
+    a.C[j] = C[j];
+    a.Bt[j] = Bt[j];
+  }
+  double* dev = nullptr;  // save times, then 6 result slots
+  FVM_HIP(hipMalloc((void**)&dev, sizeof(double) * (size_t)(o->n_save + 6)));
+  a.save_t = dev;
+  a.stats = dev + o->n_save;
+  int rc = FVM_OK;
+  hipError_t e = hipSuccess;
+  if (o->n_save > 0)
+    e = hipMemcpyAsync(dev, o->save_t, sizeof(double) * (size_t)o->n_save, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    by_width(t->ellW, [&](auto w) {
+      if (t->ell_off)
+        hipLaunchKernelGGL((tsit5_small_kernel<decltype(w)::value, true>), dim3(1), dim3(kSmallThreads), 0, s, a);
+      else
+        hipLaunchKernelGGL((tsit5_small_kernel<decltype(w)::value, false>), dim3(1), dim3(kSmallThreads), 0, s, a);
+    });
+    e = hipGetLastError();
+  }
+  double r[6] = {0, 0, 0, 0, 0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(r, a.stats, sizeof r, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(dev);
+  if (e != hipSuccess) return hip_fail(e, "fvm_ode_solve (one-workgroup Tsit5)");
+  st->t = r[0];
+  st->naccept += (int64_t)r[1];
+  st->nreject += (int64_t)r[2];
+  st->nf += (int64_t)r[3];
+  st->nsaved = (int64_t)r[4];
+  st->retcode = (int32_t)r[5];
+  return rc;
+}
+
+}  // namespace fvm

@@ -120,11 +120,10 @@ int save(const fvm_ode_opts* o, int64_t k, const double* v, fvm_stream s) {
 int tsit5(const fvm_ode_opts* o, const Op& op, double* d_u, fvm_ode_stats* st, fvm_stream s) {
   const int64_t n = o->n;
   Work w;
-  FVM_HIP(hipMalloc((void**)&w.base, sizeof(double) * (size_t)(n > 0 ? n : 1) * 12));
+  FVM_HIP(hipMalloc((void**)&w.base, sizeof(double) * (size_t)(n > 0 ? n : 1) * 11));
   double* k[7];
   for (int j = 0; j < 7; ++j) k[j] = w.base + j * n;
-  double *utmp = w.base + 7 * n, *unew = w.base + 8 * n, *err = w.base + 9 * n, *interp = w.base + 10 * n,
-         *u = w.base + 11 * n;
+  double *utmp = w.base + 7 * n, *unew = w.base + 8 * n, *interp = w.base + 9 * n, *u = w.base + 10 * n;
   FVM_HIP(hipMemcpyAsync(u, d_u, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, (hipStream_t)s));
   double t = o->t0;
   const double tf = o->tf;
@@ -140,6 +139,13 @@ int tsit5(const fvm_ode_opts* o, const Op& op, double* d_u, fvm_ode_stats* st, f
     if ((rc = save(o, ks++, u, s))) return rc;
   int64_t it = 0;
   st->retcode = FVM_ODE_SUCCESS;
+  if (op.kind == FVM_OP_TEMPLATE && !op.dirichlet) {
+    // small templates: the whole loop in one workgroup (same arithmetic per element)
+    rc = fvm::template_tsit5_small((const fvm_template*)op.h, o, k, utmp, u, unew, d_u, t, dt, it, ks, kA, kC, kBt,
+                                   st, (hipStream_t)s);
+    if (rc != FVM_ERR_STATE) return rc;
+    rc = FVM_OK;
+  }
   while (t < tf && it < o->maxiters) {
     ++it;
     dt = std::min(dt, tf - t);
@@ -159,8 +165,7 @@ int tsit5(const fvm_ode_opts* o, const Op& op, double* d_u, fvm_ode_stats* st, f
     {
       double c[7];
       for (int j = 0; j < 7; ++j) c[j] = dt * kBt[j];
-      if ((rc = lc(n, err, 7, c, k, s)) || (rc = norm(n, err, u, unew, o->abstol, o->reltol, &EEst, s)))
-        return rc;
+      if ((rc = fvm_vec_lincomb_err_norm(n, 7, c, k, u, unew, o->abstol, o->reltol, &EEst, s))) return rc;
     }
     if (!std::isfinite(EEst)) {
       st->retcode = FVM_ODE_UNSTABLE;
@@ -189,13 +194,13 @@ int tsit5(const fvm_ode_opts* o, const Op& op, double* d_u, fvm_ode_stats* st, f
       // the Dirichlet update changed it, which a constant update can do only once
       if (o->fsal && (!op.dirichlet || st->naccept >= 2)) std::swap(k[0], k[6]);
       else if ((rc = op.rhs(u, k[0], t, s))) return rc;
-      double q = std::pow(std::max(EEst, 1e-16), 7.0 / 50.0) / std::pow(qold, 2.0 / 25.0);
+      double q = fvm::ctrl_pow(std::max(EEst, 1e-16), 7.0 / 50.0) / fvm::ctrl_pow(qold, 2.0 / 25.0);
       q = std::min(5.0, std::max(0.1, q / 0.9));
       dt = dt / q;
       qold = std::max(EEst, 1e-4);
     } else {
       ++st->nreject;
-      const double q = std::min(5.0, std::pow(std::max(EEst, 1e-16), 7.0 / 50.0) / 0.9);
+      const double q = std::min(5.0, fvm::ctrl_pow(std::max(EEst, 1e-16), 7.0 / 50.0) / 0.9);
       dt = dt / q;
     }
   }

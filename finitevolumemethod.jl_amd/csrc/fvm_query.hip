@@ -294,6 +294,8 @@ int fvm_compute_flux(fvm_ctx* c, int64_t n, const int32_t* d_i, const int32_t* d
                      double t, const double* d_dq, double* d_out, fvm_stream s) {
   (void)t;  // the traced flux forms do not depend on t
   FVM_CHECK(c && n >= 0, "fvm_compute_flux: bad argument");
+  FVM_CHECK(c->flux_kind != FVM_FLUX_JIT,
+            "fvm_compute_flux: not available for closures compiled at run time (FVM_FLUX_JIT)");
   if (n == 0) return FVM_OK;
   FVM_CHECK(d_i && d_j && d_u && d_out, "fvm_compute_flux: null argument");
   FVM_CHECK((c->flux_kind != FVM_FLUX_D_EDGE && c->flux_kind != FVM_FLUX_LINEAR) || d_dq,

@@ -21,759 +21,14 @@
 #include <vector>
 
 #include "fvm_internal.hpp"
+#include "fvm_jit.hpp"
+#include "fvm_rhs_kernels.hpp"
 
 using namespace fvm;
 
+using namespace fvm_k;
+
 namespace {
-
-// RHS, Jacobian-vector product, or one forward-Euler step of the ensemble path
-enum { M_RHS = 0, M_JVP = 1, M_EULER = 2 };
-
-struct KArgs {
-  int32_t n_tiles, n_fix;
-  int32_t n_bitems;  // boundary work items computed by the tile kernel's leading workgroups
-  int64_t T, B;
-  const int32_t* tile_tri_start;
-  const int32_t* tile_node_start;
-  const int32_t* tile_nodes;
-  const uint32_t* loc;
-  const int32_t* inc_start;
-  const uint16_t* inc;
-  const int32_t* part_idx;
-  const int32_t* fix_nodes;
-  const int32_t* fix_part_start;
-  const int32_t* fix_bedge_start;
-  const int32_t* fix_meta;  // [4][n_fix]: node, first slot, first boundary item, counts and code
-  const double* fix_cv;     // [n_fix]
-  const int32_t* fix_bedge;
-  const int32_t* fix_bvert_start;
-  const int32_t* fix_bvert;
-  const double* props;
-  const int32_t* tri;
-  const double* x;
-  const double* y;
-  const double* cv;
-  const int32_t* bedges;
-  const int32_t* badj;
-  const int8_t* code;  // 0 none, 1 Dirichlet, 2 Dudt, 3 not a vertex
-  const int32_t* fidx;
-  const int32_t* bneu;
-  double d0, d1, d2, d3;
-  const double* d_edge;   // [3][T] sorted
-  const double* d_bedge;  // [2*B] interleaved
-  const double* log_d;
-  double s0, s1, s2, s3, st;
-  const double* s_node;
-  CondTable C;
-  const double* u;
-  const double* v;  // JVP direction, or VJP cotangent
-  double* out;
-  double* out2;
-  double* partial;
-  double* partial2;
-  const int32_t* lpart;  // per local entry: partial slot, or -1 - code (exclusive node)
-  const double* lcv;     // per local entry: control volume
-  double* bflux;         // boundary terms per work item
-  double t;
-  const double* tile_dt;  // M_EULER: time step of each tile's nodes
-  const double* dtv;      // M_EULER: time step per node (fix-up nodes)
-};
-
-// How much of the triangle record the tile kernels rebuild from the vertex
-// coordinates instead of streaming it (the rebuilt values equal the stored ones
-// bit for bit: same operations as the geometry kernel):
-//   1: s7..s9 and the CV-edge midpoints (only the forms that need them;
-//      FVM_FLUX_D_EDGE reads neither);
-//   2: also s1..s6;
-//   3: the whole record (also the normals and lengths).
-constexpr int kGeo = FVM_GEO_LEVEL;
-__host__ __device__ constexpr bool loads_field(int fk, int f) {
-  return kGeo >= 3 ? false : (kGeo == 2 ? f >= P_NX : (f < 6 || f >= P_NX));
-}
-__host__ __device__ constexpr bool rebuilds(int fk) { return kGeo >= 2 || fk != FVM_FLUX_D_EDGE; }
-
-__device__ __forceinline__ double dpoly(const KArgs& a, double u) {
-  return a.d0 + u * (a.d1 + u * (a.d2 + u * a.d3));
-}
-__device__ __forceinline__ double dpoly_du(const KArgs& a, double u) {
-  return a.d1 + u * (2.0 * a.d2 + 3.0 * a.d3 * u);
-}
-
-// Forms whose D depends on u_s = alpha x_s + beta y_s + gamma: the cubic, or the
-// power law D = d0 + d1 |u|^d2 (porous-medium diffusivity, d2 > 0).
-__host__ __device__ constexpr bool d_of_u(int fk) { return fk == FVM_FLUX_D_POLY_U || fk == FVM_FLUX_D_POWER; }
-template <int FK>
-__device__ __forceinline__ double dfun(const KArgs& a, double u) {
-  if (FK == FVM_FLUX_D_POWER) return a.d0 + a.d1 * pow(fabs(u), a.d2);
-  return dpoly(a, u);
-}
-// dD/du; for the power law d1 d2 |u|^(d2 - 1) sign(u), taken as 0 at u = 0
-template <int FK>
-__device__ __forceinline__ double dfun_du(const KArgs& a, double u) {
-  if (FK == FVM_FLUX_D_POWER) {
-    const double au = fabs(u);
-    if (!(au > 0.0)) return 0.0;
-    const double d = a.d1 * a.d2 * pow(au, a.d2 - 1.0);
-    return u > 0.0 ? d : -d;
-  }
-  return dpoly_du(a, u);
-}
-
-// S(x, y, t, u) of the source term (source_contributions.jl:2-14)
-__device__ __forceinline__ double source(const KArgs& a, int g, double u) {
-  double S = a.s0 + u * (a.s1 + u * (a.s2 + u * a.s3));
-  if (a.s_node) S = a.s_node[g] + S;
-  if (a.st != 0.0) S = S + a.st * a.t;
-  return S;
-}
-__device__ __forceinline__ double source_du(const KArgs& a, double u) {
-  return a.s1 + u * (2.0 * a.s2 + 3.0 * a.s3 * u);
-}
-
-// The three CV-edge fluxes Q_e = (q . n) l of one triangle (get_fluxes,
-// triangle_contributions.jl:2-7), or their directional derivatives.
-template <int FK, int MODE>
-__device__ __forceinline__ void tri_fluxes(const KArgs& a, const double* P, double ua, double ub,
-                                           double uc, double va, double vb, double vc, double Da,
-                                           double Db, double Dc, const double* Ds, double* Q) {
-  const double al = P[0] * ua + P[1] * ub + P[2] * uc;
-  const double be = P[3] * ua + P[4] * ub + P[5] * uc;
-  double ga = 0.0, dal = 0.0, dbe = 0.0, dga = 0.0, da = 0.0, db = 0.0, dg = 0.0;
-  constexpr bool G = d_of_u(FK) || FK == FVM_FLUX_LINEAR;  // gamma enters q
-  if (G) ga = P[6] * ua + P[7] * ub + P[8] * uc;
-  if (MODE == M_JVP) {
-    dal = P[0] * va + P[1] * vb + P[2] * vc;
-    dbe = P[3] * va + P[4] * vb + P[5] * vc;
-    if (G) dga = P[6] * va + P[7] * vb + P[8] * vc;
-  }
-  if (FK == FVM_FLUX_D_NODAL) {
-    da = P[0] * Da + P[1] * Db + P[2] * Dc;
-    db = P[3] * Da + P[4] * Db + P[5] * Dc;
-    dg = P[6] * Da + P[7] * Db + P[8] * Dc;
-  }
-#pragma unroll
-  for (int e = 0; e < 3; ++e) {
-    if (FK == FVM_FLUX_LINEAR) {
-      // (q . n) l with q = A (alpha, beta, gamma): the coefficients come projected
-      const double* c = Ds + 3 * e;
-      Q[e] = MODE != M_JVP ? c[0] * al + c[1] * be + c[2] * ga : c[0] * dal + c[1] * dbe + c[2] * dga;
-      continue;
-    }
-    const double nx = P[P_NX + 2 * e], ny = P[P_NX + 1 + 2 * e], l = P[P_L + e];
-    double D, Dd = 0.0;
-    if (d_of_u(FK)) {
-      const double x = P[P_MX + 2 * e], y = P[P_MX + 1 + 2 * e];
-      const double us = al * x + be * y + ga;
-      D = dfun<FK>(a, us);
-      if (MODE == M_JVP) Dd = dfun_du<FK>(a, us) * (dal * x + dbe * y + dga);
-    } else if (FK == FVM_FLUX_D_NODAL) {
-      const double x = P[P_MX + 2 * e], y = P[P_MX + 1 + 2 * e];
-      D = da * x + db * y + dg;
-    } else {
-      D = Ds[e];
-    }
-    if (MODE != M_JVP) {
-      const double qx = -D * al, qy = -D * be;
-      const double qn = qx * nx + qy * ny;
-      Q[e] = qn * l;
-    } else {
-      const double qx = -(Dd * al + D * dal), qy = -(Dd * be + D * dbe);
-      Q[e] = (qx * nx + qy * ny) * l;
-    }
-  }
-}
-
-struct BGeo {
-  double nx, ny, mx[2], my[2], ell;
-};
-
-// get_boundary_cv_components (control_volumes.jl:41-56)
-__device__ __forceinline__ BGeo bgeo(const KArgs& a, int e) {
-  const int i = a.bedges[e], j = a.bedges[a.B + e];
-  const double px = a.x[i], py = a.y[i], qx = a.x[j], qy = a.y[j];
-  BGeo G;
-  const double lij = sqrt((qx - px) * (qx - px) + (qy - py) * (qy - py));
-  G.nx = (qy - py) / lij;
-  G.ny = -(qx - px) / lij;
-  const double mijx = (px + qx) / 2, mijy = (py + qy) / 2;
-  G.mx[0] = (px + mijx) / 2;
-  G.my[0] = (py + mijy) / 2;
-  G.mx[1] = (qx + mijx) / 2;
-  G.my[1] = (qy + mijy) / 2;
-  G.ell = sqrt((mijx - px) * (mijx - px) + (mijy - py) * (mijy - py));
-  return G;
-}
-
-// Flux through the boundary CV edge of endpoint ep of boundary edge e, times its
-// length (get_boundary_fluxes, boundary_edge_contributions.jl:2-48), or its
-// directional derivative.
-template <int FK, int MODE>
-__device__ double bedge_flux(const KArgs& a, int e, int ep) {
-  const BGeo G = bgeo(a, e);
-  const int64_t Tt = a.badj[e];
-  const int va = a.tri[Tt], vb = a.tri[a.T + Tt], vc = a.tri[2 * a.T + Tt];
-  double s[9];
-#pragma unroll
-  for (int f = 0; f < 9; ++f) s[f] = a.props[f * a.T + Tt];
-  const double ua = a.u[va], ub = a.u[vb], uc = a.u[vc];
-  const double al = s[0] * ua + s[1] * ub + s[2] * uc;
-  const double be = s[3] * ua + s[4] * ub + s[5] * uc;
-  const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
-  double dal = 0, dbe = 0, dga = 0;
-  if (MODE == M_JVP) {
-    const double wa = a.v[va], wb = a.v[vb], wc = a.v[vc];
-    dal = s[0] * wa + s[1] * wb + s[2] * wc;
-    dbe = s[3] * wa + s[4] * wb + s[5] * wc;
-    dga = s[6] * wa + s[7] * wb + s[8] * wc;
-  }
-  const double mx = ep ? G.mx[1] : G.mx[0], my = ep ? G.my[1] : G.my[0];
-  const int nf = a.bneu[e];
-  if (FK == FVM_FLUX_LINEAR && nf < 0) {
-    const double* c = a.d_bedge + 3 * (2 * e + ep);  // projected onto n, times the length
-    return MODE != M_JVP ? c[0] * al + c[1] * be + c[2] * ga : c[0] * dal + c[1] * dbe + c[2] * dga;
-  }
-  double qn;
-  if (nf < 0) {
-    double D, Dd = 0.0;
-    if (d_of_u(FK)) {
-      const double um = al * mx + be * my + ga;
-      D = dfun<FK>(a, um);
-      if (MODE == M_JVP) Dd = dfun_du<FK>(a, um) * (dal * mx + dbe * my + dga);
-    } else if (FK == FVM_FLUX_D_NODAL) {
-      const double Da = exp(a.log_d[va]), Db = exp(a.log_d[vb]), Dc = exp(a.log_d[vc]);
-      const double da = s[0] * Da + s[1] * Db + s[2] * Dc;
-      const double db = s[3] * Da + s[4] * Db + s[5] * Dc;
-      const double dg = s[6] * Da + s[7] * Db + s[8] * Dc;
-      D = da * mx + db * my + dg;
-    } else {
-      D = a.d_bedge[2 * e + ep];
-    }
-    if (MODE != M_JVP) {
-      const double qx = -D * al, qy = -D * be;
-      qn = qx * G.nx + qy * G.ny;
-    } else {
-      const double qx = -(Dd * al + D * dal), qy = -(Dd * be + D * dbe);
-      qn = qx * G.nx + qy * G.ny;
-    }
-  } else {
-    if (MODE != M_JVP)
-      qn = eval_cond(a.C, nf, mx, my, a.t, al * mx + be * my + ga,
-                     a.C.bedge_g ? a.C.bedge_g[2 * e + ep] : 0.0);
-    else
-      qn = a.C.cu[nf] * (dal * mx + dbe * my + dga);
-  }
-  return qn * G.ell;
-}
-
-// ---------------------------------------------------------------------------
-// RHS / JVP: tile kernel and fix-up kernel
-// ---------------------------------------------------------------------------
-// Local-node capacity of a tile processed by BLOCK threads: two node slots per
-// thread, at most 1024 (local ids are packed in 10 bits).
-__host__ __device__ constexpr int node_cap(int block) { return 2 * block < 1024 ? 2 * block : 1024; }
-
-// fvm_eqs_single_source_contribution! (source_contributions.jl:33-47) for the RHS,
-// its derivative for the JVP, or (M_EULER) the new state u + dt du followed by the
-// Dirichlet callback (dirichlet.jl:78-86) at t + dt; the node's code and control
-// volume are already in registers.
-template <int MODE>
-__device__ __forceinline__ double finish_pre(const KArgs& a, int g, double acc, double ug, double vg,
-                                             int c, double V, double dt) {
-  if (c == 0) {
-    if (MODE == M_RHS) return acc / V + source(a, g, ug);
-    if (MODE == M_EULER) return ug + dt * (acc / V + source(a, g, ug));
-    return acc / V + source_du(a, ug) * vg;
-  }
-  if (c == 2) {
-    const int f = a.fidx[g];
-    if (MODE == M_RHS)
-      return eval_cond(a.C, f, a.x[g], a.y[g], a.t, ug, a.C.node_g ? a.C.node_g[g] : 0.0);
-    if (MODE == M_EULER)
-      return ug + dt * eval_cond(a.C, f, a.x[g], a.y[g], a.t, ug, a.C.node_g ? a.C.node_g[g] : 0.0);
-    return a.C.cu[f] * vg;
-  }
-  if (MODE == M_EULER) {
-    if (c == 1)
-      return eval_cond(a.C, a.fidx[g], a.x[g], a.y[g], a.t + dt, ug, a.C.node_g ? a.C.node_g[g] : 0.0);
-    return ug;
-  }
-  return 0.0;
-}
-
-// One workgroup per tile.  All global loads of the tile (node ids, incidence
-// block, partial slots, the triangle records, then the per-node gathers that
-// depend on the node ids) are issued before the first wait, so the workgroup
-// exposes one memory latency; the rest works out of LDS and registers.
-template <int FK, int MODE, int BLOCK, int TPT>
-__global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
-  constexpr int TILE = BLOCK * TPT;
-  constexpr int MAXN = node_cap(BLOCK);
-  constexpr int NS = MAXN / BLOCK;
-  __shared__ double s_u[MAXN];
-  __shared__ double s_v[MODE == M_JVP ? MAXN : 1];
-  __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
-  constexpr bool RC = rebuilds(FK);  // part of the triangle record rebuilt from coordinates
-  // the node coordinates are dead once the fluxes are formed: the fluxes reuse their
-  // space (fewer LDS bytes per workgroup, more workgroups per CU)
-  constexpr int XY = RC ? 2 * MAXN : 0;
-  __shared__ double s_xyq[XY > 3 * TILE ? XY : 3 * TILE];
-  double* const s_x = s_xyq;
-  double* const s_y = s_xyq + MAXN;
-  double* const s_q = s_xyq;
-  __shared__ uint16_t s_inc[3 * TILE];
-  __shared__ int s_is[MAXN + 1];
-  const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
-  const int wid = xcd_remap(blockIdx.x, gridDim.x);
-  const int tid = threadIdx.x;
-  if (wid < nbw) {
-    // leading workgroups: the boundary-edge fluxes, one thread per (edge, endpoint)
-    const int k = wid * BLOCK + tid;
-    if (k < a.n_bitems) {
-      const int w = a.fix_bedge[k];
-      a.bflux[k] = bedge_flux<FK, MODE>(a, w >> 1, w & 1);
-    }
-    return;
-  }
-  const int tile = wid - nbw;
-  if (tile >= a.n_tiles) return;
-  const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
-  const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
-  const int ib0 = 3 * t0;  // a tile's incidence block is [3 t0, 3 (t0 + nt))
-  const double dt = MODE == M_EULER ? a.tile_dt[tile] : 0.0;
-
-  int g[NS], pi[NS], is[NS];
-#pragma unroll
-  for (int r = 0; r < NS; ++r) {
-    const int l = tid + r * BLOCK;
-    g[r] = 0;
-    pi[r] = -1;
-    is[r] = 0;
-    if (l < nn) {
-      g[r] = a.tile_nodes[n0 + l];
-      pi[r] = a.lpart[n0 + l];
-      is[r] = a.inc_start[n0 + l] - ib0;
-    }
-  }
-  uint16_t iv[3 * TPT];
-#pragma unroll
-  for (int k = 0; k < 3 * TPT; ++k) {
-    const int idx = tid + k * BLOCK;
-    iv[k] = idx < 3 * nt ? a.inc[ib0 + idx] : (uint16_t)0;
-  }
-  double P[TPT][P_COUNT];
-  double Ds[TPT][FK == FVM_FLUX_LINEAR ? 9 : 3];
-  uint32_t L[TPT];
-#pragma unroll
-  for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * BLOCK;
-    L[r] = 0;
-    if (lt < nt) {
-      const int64_t s = t0 + lt;
-      L[r] = a.loc[s];
-#pragma unroll
-      for (int f = 0; f < P_COUNT; ++f)
-        if (loads_field(FK, f)) P[r][f] = __builtin_nontemporal_load(a.props + f * a.T + s);
-      if (FK == FVM_FLUX_D_EDGE) {
-#pragma unroll
-        for (int e = 0; e < 3; ++e) Ds[r][e] = __builtin_nontemporal_load(a.d_edge + e * a.T + s);
-      }
-      if (FK == FVM_FLUX_LINEAR) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) Ds[r][k] = __builtin_nontemporal_load(a.d_edge + k * a.T + s);
-      }
-    }
-  }
-  double ug[NS], vg[NS], Vg[NS], lg[NS], xg[NS], yg[NS];
-  int cg[NS];
-#pragma unroll
-  for (int r = 0; r < NS; ++r) {
-    const int l = tid + r * BLOCK;
-    ug[r] = vg[r] = Vg[r] = lg[r] = xg[r] = yg[r] = 0.0;
-    cg[r] = 3;
-    if (l < nn) {
-      ug[r] = a.u[g[r]];
-      if (MODE == M_JVP) vg[r] = a.v[g[r]];
-      if (FK == FVM_FLUX_D_NODAL) lg[r] = a.log_d[g[r]];
-      if (RC) {
-        xg[r] = a.x[g[r]];
-        yg[r] = a.y[g[r]];
-      }
-      if (pi[r] < 0) {
-        cg[r] = -1 - pi[r];
-        Vg[r] = a.lcv[n0 + l];
-      }
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < NS; ++r) {
-    const int l = tid + r * BLOCK;
-    if (l < nn) {
-      s_u[l] = ug[r];
-      if (MODE == M_JVP) s_v[l] = vg[r];
-      if (FK == FVM_FLUX_D_NODAL) s_D[l] = exp(lg[r]);
-      if (RC) {
-        s_x[l] = xg[r];
-        s_y[l] = yg[r];
-      }
-      s_is[l] = is[r];
-    }
-  }
-  if (tid == 0) s_is[nn] = 3 * nt;
-#pragma unroll
-  for (int k = 0; k < 3 * TPT; ++k) {
-    const int idx = tid + k * BLOCK;
-    if (idx < 3 * nt) s_inc[idx] = iv[k];
-  }
-  __syncthreads();
-  double Q[TPT][3];
-#pragma unroll
-  for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * BLOCK;
-    Q[r][0] = Q[r][1] = Q[r][2] = 0.0;
-    if (lt < nt) {
-      const int ia = L[r] & 1023u, ib = (L[r] >> 10) & 1023u, ic = (L[r] >> 20) & 1023u;
-      if (RC) rebuild_record<kGeo>(s_x[ia], s_y[ia], s_x[ib], s_y[ib], s_x[ic], s_y[ic], P[r]);
-      tri_fluxes<FK, MODE>(a, P[r], s_u[ia], s_u[ib], s_u[ic],
-                           MODE == M_JVP ? s_v[ia] : 0.0, MODE == M_JVP ? s_v[ib] : 0.0,
-                           MODE == M_JVP ? s_v[ic] : 0.0,
-                           FK == FVM_FLUX_D_NODAL ? s_D[ia] : 0.0,
-                           FK == FVM_FLUX_D_NODAL ? s_D[ib] : 0.0,
-                           FK == FVM_FLUX_D_NODAL ? s_D[ic] : 0.0, Ds[r], Q[r]);
-    }
-  }
-  if (RC) __syncthreads();  // every thread has read the coordinates that s_q overwrites
-#pragma unroll
-  for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * BLOCK;
-    if (lt < nt) {
-      s_q[3 * lt] = Q[r][0];
-      s_q[3 * lt + 1] = Q[r][1];
-      s_q[3 * lt + 2] = Q[r][2];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < NS; ++r) {
-    const int l = tid + r * BLOCK;
-    if (l < nn) {
-      double acc = 0.0;
-      const int k1 = s_is[l + 1];
-      for (int k = s_is[l]; k < k1; ++k) {
-        const int w = s_inc[k];
-        const int lt = w >> 2, sl = w & 3;
-        // update_du! (triangle_contributions.jl:10-15): i += Q3 - Q1, j += Q1 - Q2, k += Q2 - Q3
-        acc = acc + s_q[3 * lt + (sl == 0 ? 2 : sl - 1)] - s_q[3 * lt + sl];
-      }
-      // the node's u (and v) are re-read from LDS and its volume from memory here rather
-      // than held in registers across the barriers: fewer VGPRs, more waves per SIMD
-      if (pi[r] >= 0)
-        a.partial[pi[r]] = acc;
-      else
-        a.out[g[r]] = finish_pre<MODE>(a, g[r], acc, s_u[l], MODE == M_JVP ? s_v[l] : 0.0, -1 - pi[r],
-                                       a.lcv[n0 + l], dt);
-    }
-  }
-}
-
-// Shared nodes: partial sums in tile order, then the boundary-edge fluxes (computed
-// by the tile kernel's leading workgroups), then the node rule.  The first four slots are loaded without a loop-carried
-// dependency (nearly every node has at most four).
-template <int FK, int MODE>
-__global__ __launch_bounds__(256) void fix_kernel(KArgs a) {
-  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (fi >= a.n_fix) return;
-  // the node's static data in fix order: one coalesced record, one gather (u)
-  const int n = a.n_fix;
-  const int g = a.fix_meta[fi], p0 = a.fix_meta[n + fi], b0 = a.fix_meta[2 * n + fi];
-  const uint32_t w = (uint32_t)a.fix_meta[3 * n + fi];
-  const int p1 = p0 + (int)(w >> 20), b1 = b0 + (int)((w >> 8) & 4095u), c = (int)(w & 255u);
-  const double V = a.fix_cv[fi];
-  const double ug = a.u[g];
-  const double vg = MODE == M_JVP ? a.v[g] : 0.0;
-  double pv[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) pv[k] = p0 + k < p1 ? a.partial[p0 + k] : 0.0;
-  double acc = 0.0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (p0 + k < p1) acc = acc + pv[k];
-  for (int p = p0 + 4; p < p1; ++p) acc = acc + a.partial[p];
-  for (int k = b0; k < b1; ++k) acc = acc - a.bflux[k];
-  a.out[g] = finish_pre<MODE>(a, g, acc, ug, vg, c, V, MODE == M_EULER ? a.dtv[g] : 0.0);
-}
-
-// ---------------------------------------------------------------------------
-// VJP
-// ---------------------------------------------------------------------------
-// w / V at nodes whose du carries the fluxes, 0 elsewhere.
-__device__ __forceinline__ double wt_of(const KArgs& a, int g) {
-  return a.code[g] == 0 ? a.v[g] / a.cv[g] : 0.0;
-}
-// Adjoint of the node rule: w_g dS/du (free nodes) or w_g da/du (Dudt nodes).
-__device__ __forceinline__ double node_adj(const KArgs& a, int g, double ug) {
-  const int c = a.code[g];
-  if (c == 0) return a.v[g] * source_du(a, ug);
-  if (c == 2) return a.v[g] * a.C.cu[a.fidx[g]];
-  return 0.0;
-}
-
-// Adjoint of one CV-edge flux Q = -l D (alpha nx + beta ny) with upstream Qbar:
-// accumulates alpha/beta/gamma adjoints and the adjoint of D.
-template <int FK>
-__device__ __forceinline__ void edge_adj(const KArgs& a, double Qbar, double x, double y, double nx,
-                                         double ny, double l, double al, double be, double ga,
-                                         double D_given, double& abar, double& bbar, double& gbar,
-                                         double& Dbar) {
-  const double gn = al * nx + be * ny;
-  double D, Du = 0.0;
-  if (d_of_u(FK)) {
-    const double us = al * x + be * y + ga;
-    D = dfun<FK>(a, us);
-    Du = dfun_du<FK>(a, us);
-  } else {
-    D = D_given;
-  }
-  abar += Qbar * (-l * (D * nx + Du * x * gn));
-  bbar += Qbar * (-l * (D * ny + Du * y * gn));
-  gbar += Qbar * (-l * Du * gn);
-  Dbar += Qbar * (-l * gn);
-}
-
-// Adjoint of both boundary CV-edge fluxes of boundary edge e with respect to the
-// vertex in slot `slot` of its triangle.
-template <int FK>
-__device__ void bedge_adj(const KArgs& a, int e, int slot, double& ubar, double& lbar) {
-  const BGeo G = bgeo(a, e);
-  const int64_t Tt = a.badj[e];
-  const int vtx[3] = {a.tri[Tt], a.tri[a.T + Tt], a.tri[2 * a.T + Tt]};
-  double s[9];
-#pragma unroll
-  for (int f = 0; f < 9; ++f) s[f] = a.props[f * a.T + Tt];
-  const double ua = a.u[vtx[0]], ub = a.u[vtx[1]], uc = a.u[vtx[2]];
-  const double al = s[0] * ua + s[1] * ub + s[2] * uc;
-  const double be = s[3] * ua + s[4] * ub + s[5] * uc;
-  const double ga = s[6] * ua + s[7] * ub + s[8] * uc;
-  double Dn[3] = {0, 0, 0}, da = 0, db = 0, dg = 0;
-  if (FK == FVM_FLUX_D_NODAL) {
-    for (int v = 0; v < 3; ++v) Dn[v] = exp(a.log_d[vtx[v]]);
-    da = s[0] * Dn[0] + s[1] * Dn[1] + s[2] * Dn[2];
-    db = s[3] * Dn[0] + s[4] * Dn[1] + s[5] * Dn[2];
-    dg = s[6] * Dn[0] + s[7] * Dn[1] + s[8] * Dn[2];
-  }
-  const int nf = a.bneu[e];
-  double abar = 0, bbar = 0, gbar = 0, lb = 0;
-  for (int ep = 0; ep < 2; ++ep) {
-    const int node = a.bedges[ep * a.B + e];
-    const double Qbar = -wt_of(a, node);  // du_node -= Q
-    const double mx = ep ? G.mx[1] : G.mx[0], my = ep ? G.my[1] : G.my[0];
-    if (nf < 0 && FK == FVM_FLUX_LINEAR) {
-      const double* c = a.d_bedge + 3 * (2 * e + ep);
-      abar += Qbar * c[0];
-      bbar += Qbar * c[1];
-      gbar += Qbar * c[2];
-    } else if (nf < 0) {
-      double Dg = 0, Dbar = 0;
-      if (FK == FVM_FLUX_D_NODAL) Dg = da * mx + db * my + dg;
-      if (FK == FVM_FLUX_D_EDGE) Dg = a.d_bedge[2 * e + ep];
-      edge_adj<FK>(a, Qbar, mx, my, G.nx, G.ny, G.ell, al, be, ga, Dg, abar, bbar, gbar, Dbar);
-      if (FK == FVM_FLUX_D_NODAL) lb += Dbar * (s[slot] * mx + s[3 + slot] * my + s[6 + slot]);
-    } else {
-      const double c = Qbar * G.ell * a.C.cu[nf];
-      abar += c * mx;
-      bbar += c * my;
-      gbar += c;
-    }
-  }
-  ubar += s[slot] * abar + s[3 + slot] * bbar + s[6 + slot] * gbar;
-  lbar += lb * Dn[slot];
-}
-
-template <int FK, int BLOCK, int TPT>
-__global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
-  constexpr int TILE = BLOCK * TPT;
-  constexpr int MAXN = node_cap(BLOCK);
-  constexpr int CW = FK == FVM_FLUX_D_NODAL ? 6 : 3;  // u-adjoint (+ log D adjoint) per vertex
-  __shared__ double s_u[MAXN];
-  constexpr bool RC = rebuilds(FK);  // part of the triangle record rebuilt from coordinates
-  // w~, D, x and y are dead once the per-vertex adjoints are formed: the adjoints
-  // reuse their space (fewer LDS bytes per workgroup, more workgroups per CU)
-  constexpr int ND = FK == FVM_FLUX_D_NODAL ? MAXN : 0;
-  constexpr int PRE = MAXN + ND + (RC ? 2 * MAXN : 0);
-  __shared__ double s_big[PRE > CW * TILE ? PRE : CW * TILE];
-  double* const s_w = s_big;
-  double* const s_D = s_big + MAXN;
-  double* const s_x = s_big + MAXN + ND;
-  double* const s_y = s_x + MAXN;
-  double* const s_c = s_big;
-  const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
-  const int wid = xcd_remap(blockIdx.x, gridDim.x);
-  const int tid = threadIdx.x;
-  if (wid < nbw) {
-    // leading workgroups: adjoints of the boundary-edge fluxes per (edge, triangle vertex)
-    const int k = wid * BLOCK + tid;
-    if (k < a.n_bitems) {
-      const int w = a.fix_bvert[k];
-      double ub = 0.0, lb = 0.0;
-      bedge_adj<FK>(a, w >> 2, w & 3, ub, lb);
-      a.bflux[2 * k] = ub;
-      a.bflux[2 * k + 1] = lb;
-    }
-    return;
-  }
-  const int tile = wid - nbw;
-  if (tile >= a.n_tiles) return;
-  const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
-  const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
-  double P[TPT][P_COUNT];
-  double Ds[TPT][FK == FVM_FLUX_LINEAR ? 9 : 3];
-  uint32_t L[TPT];
-#pragma unroll
-  for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * BLOCK;
-    L[r] = 0;
-    if (lt < nt) {
-      const int64_t s = t0 + lt;
-      L[r] = a.loc[s];
-#pragma unroll
-      for (int f = 0; f < P_COUNT; ++f)
-        if (loads_field(FK, f)) P[r][f] = __builtin_nontemporal_load(a.props + f * a.T + s);
-      if (FK == FVM_FLUX_D_EDGE) {
-#pragma unroll
-        for (int e = 0; e < 3; ++e) Ds[r][e] = __builtin_nontemporal_load(a.d_edge + e * a.T + s);
-      }
-      if (FK == FVM_FLUX_LINEAR) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) Ds[r][k] = __builtin_nontemporal_load(a.d_edge + k * a.T + s);
-      }
-    }
-  }
-  for (int l = tid; l < nn; l += BLOCK) {
-    const int g = a.tile_nodes[n0 + l];
-    s_u[l] = a.u[g];
-    s_w[l] = wt_of(a, g);
-    if (FK == FVM_FLUX_D_NODAL) s_D[l] = exp(a.log_d[g]);
-    if (RC) {
-      s_x[l] = a.x[g];
-      s_y[l] = a.y[g];
-    }
-  }
-  __syncthreads();
-  double cr[TPT][CW];
-#pragma unroll
-  for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * BLOCK;
-#pragma unroll
-    for (int v = 0; v < CW; ++v) cr[r][v] = 0.0;
-    if (lt < nt) {
-      const int id[3] = {(int)(L[r] & 1023u), (int)((L[r] >> 10) & 1023u), (int)((L[r] >> 20) & 1023u)};
-      if (RC) rebuild_record<kGeo>(s_x[id[0]], s_y[id[0]], s_x[id[1]], s_y[id[1]], s_x[id[2]], s_y[id[2]], P[r]);
-      const double* p = P[r];
-      const double ua = s_u[id[0]], ub = s_u[id[1]], uc = s_u[id[2]];
-      const double al = p[0] * ua + p[1] * ub + p[2] * uc;
-      const double be = p[3] * ua + p[4] * ub + p[5] * uc;
-      const double ga = d_of_u(FK) ? p[6] * ua + p[7] * ub + p[8] * uc : 0.0;
-      double Dn[3] = {0, 0, 0}, da = 0, db = 0, dg = 0;
-      if (FK == FVM_FLUX_D_NODAL) {
-        Dn[0] = s_D[id[0]];
-        Dn[1] = s_D[id[1]];
-        Dn[2] = s_D[id[2]];
-        da = p[0] * Dn[0] + p[1] * Dn[1] + p[2] * Dn[2];
-        db = p[3] * Dn[0] + p[4] * Dn[1] + p[5] * Dn[2];
-        dg = p[6] * Dn[0] + p[7] * Dn[1] + p[8] * Dn[2];
-      }
-      double abar = 0, bbar = 0, gbar = 0, lbar[3] = {0, 0, 0};
-#pragma unroll
-      for (int e = 0; e < 3; ++e) {
-        // du_e1 -= Q_e, du_e2 += Q_e  ->  Qbar = w~_e2 - w~_e1
-        const double Qbar = s_w[id[(e + 1) % 3]] - s_w[id[e]];
-        const double nx = p[P_NX + 2 * e], ny = p[P_NX + 1 + 2 * e], l = p[P_L + e];
-        double x = 0, y = 0, Dg = 0;
-        if (FK != FVM_FLUX_D_EDGE) {
-          x = p[P_MX + 2 * e];
-          y = p[P_MX + 1 + 2 * e];
-        }
-        if (FK == FVM_FLUX_D_NODAL) Dg = da * x + db * y + dg;
-        if (FK == FVM_FLUX_D_EDGE) Dg = Ds[r][e];
-        double Dbar = 0;
-        if (FK == FVM_FLUX_LINEAR) {
-          abar += Qbar * Ds[r][3 * e];
-          bbar += Qbar * Ds[r][3 * e + 1];
-          gbar += Qbar * Ds[r][3 * e + 2];
-        } else {
-          edge_adj<FK>(a, Qbar, x, y, nx, ny, l, al, be, ga, Dg, abar, bbar, gbar, Dbar);
-        }
-        if (FK == FVM_FLUX_D_NODAL) {
-#pragma unroll
-          for (int v = 0; v < 3; ++v) lbar[v] += Dbar * (p[v] * x + p[3 + v] * y + p[6 + v]);
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < 3; ++v) {
-        double ub = p[v] * abar + p[3 + v] * bbar;
-        if (d_of_u(FK) || FK == FVM_FLUX_LINEAR) ub = ub + p[6 + v] * gbar;  // gamma enters q
-        cr[r][v] = ub;
-        if (FK == FVM_FLUX_D_NODAL) cr[r][3 + v] = lbar[v] * Dn[v];
-      }
-    }
-  }
-  __syncthreads();  // every thread has read the weights / D / coordinates that s_c overwrites
-#pragma unroll
-  for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * BLOCK;
-    if (lt < nt) {
-#pragma unroll
-      for (int v = 0; v < CW; ++v) s_c[CW * lt + v] = cr[r][v];
-    }
-  }
-  __syncthreads();
-  for (int l = tid; l < nn; l += BLOCK) {
-    double acc = 0.0, accl = 0.0;
-    const int k0 = a.inc_start[n0 + l], k1 = a.inc_start[n0 + l + 1];
-    for (int k = k0; k < k1; ++k) {
-      const int w = a.inc[k];
-      acc += s_c[CW * (w >> 2) + (w & 3)];
-      if (FK == FVM_FLUX_D_NODAL) accl += s_c[CW * (w >> 2) + 3 + (w & 3)];
-    }
-    const int pi = a.part_idx[n0 + l];
-    if (pi >= 0) {
-      a.partial[pi] = acc;
-      a.partial2[pi] = accl;
-    } else {
-      const int g = a.tile_nodes[n0 + l];
-      a.out[g] = acc + node_adj(a, g, s_u[l]);
-      if (a.out2) a.out2[g] = accl;
-    }
-  }
-}
-
-template <int FK>
-__global__ __launch_bounds__(256) void vjp_fix_kernel(KArgs a) {
-  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (fi >= a.n_fix) return;
-  const int g = a.fix_nodes[fi];
-  double acc = 0.0, accl = 0.0;
-  for (int p = a.fix_part_start[fi]; p < a.fix_part_start[fi + 1]; ++p) {
-    acc += a.partial[p];
-    accl += a.partial2[p];
-  }
-  for (int k = a.fix_bvert_start[fi]; k < a.fix_bvert_start[fi + 1]; ++k) {
-    acc += a.bflux[2 * k];
-    accl += a.bflux[2 * k + 1];
-  }
-  a.out[g] = acc + node_adj(a, g, a.u[g]);
-  if (a.out2) a.out2[g] = accl;
-}
-
-__global__ __launch_bounds__(256) void dirichlet_kernel(int n, const int32_t* __restrict__ nodes,
-                                                        const int32_t* __restrict__ fidx,
-                                                        const double* __restrict__ X,
-                                                        const double* __restrict__ Y, CondTable C,
-                                                        double t, double* __restrict__ u) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int g = nodes[k];
-  u[g] = eval_cond(C, fidx[g], X[g], Y[g], t, u[g], C.node_g ? C.node_g[g] : 0.0);
-}
 
 KArgs make_args(const fvm_ctx* c) {
   const fvm_mesh* m = c->m;
@@ -864,9 +119,20 @@ int launch_fk(const KArgs& a, int tile_tris, hipStream_t s) {
   return FVM_OK;
 }
 
+// FVM_FLUX_JIT: the same tile / fix-up pair from the run-time compiled module
+int launch_jit(const fvm_ctx* c, const KArgs& a, int mode, hipStream_t s) {
+  const JitKernels* j = c->jit;
+  FVM_CHECK(mode == M_RHS || mode == M_JVP, "closures compiled at run time: RHS and JVP only");
+  int rc = jit_launch(mode == M_RHS ? j->tile_rhs : j->tile_jvp, tile_grid(a, j->block).x, j->block, &a, s);
+  if (rc == FVM_OK && a.n_fix > 0)
+    rc = jit_launch(mode == M_RHS ? j->fix_rhs : j->fix_jvp, (unsigned)((a.n_fix + 255) / 256), 256, &a, s);
+  return rc;
+}
+
 template <int MODE>
 int launch(const fvm_ctx* c, const KArgs& a, hipStream_t s) {
   switch (c->flux_kind) {
+    case FVM_FLUX_JIT: return launch_jit(c, a, MODE, s);
     case FVM_FLUX_D_POLY_U: return launch_fk<FVM_FLUX_D_POLY_U, MODE>(a, c->m->tile_tris, s);
     case FVM_FLUX_D_NODAL: return launch_fk<FVM_FLUX_D_NODAL, MODE>(a, c->m->tile_tris, s);
     case FVM_FLUX_LINEAR: return launch_fk<FVM_FLUX_LINEAR, MODE>(a, c->m->tile_tris, s);
@@ -973,8 +239,10 @@ void free_ctx(fvm_ctx* c) {
 
 int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
   const int64_t N = m->N, T = m->T, B = m->B;
-  FVM_CHECK(ph->flux_kind >= FVM_FLUX_D_POLY_U && ph->flux_kind <= FVM_FLUX_D_POWER,
+  FVM_CHECK(ph->flux_kind >= FVM_FLUX_D_POLY_U && ph->flux_kind <= FVM_FLUX_JIT,
             "fvm_ctx_create: unknown flux_kind");
+  FVM_CHECK(ph->flux_kind != FVM_FLUX_JIT || ph->jit_source,
+            "fvm_ctx_create: FVM_FLUX_JIT needs jit_source");
   FVM_CHECK(ph->flux_kind != FVM_FLUX_D_POWER || ph->dpoly[2] > 0,
             "fvm_ctx_create: FVM_FLUX_D_POWER needs a positive exponent dpoly[2]");
   FVM_CHECK(ph->node_kind, "fvm_ctx_create: node_kind is required");
@@ -988,6 +256,10 @@ int build_ctx(const fvm_mesh* m, const fvm_physics_desc* ph, fvm_ctx* c) {
   if (ph->flux_kind == FVM_FLUX_D_NODAL) FVM_CHECK(ph->log_d, "fvm_ctx_create: log_d required");
   c->m = m;
   c->flux_kind = ph->flux_kind;
+  if (ph->flux_kind == FVM_FLUX_JIT) {
+    int jrc = jit_kernels(ph->jit_source, m->tile_tris, &c->jit);
+    if (jrc != FVM_OK) return jrc;
+  }
   for (int k = 0; k < 4; ++k) {
     c->dpoly[k] = ph->dpoly[k];
     c->spoly[k] = ph->spoly[k];
@@ -1158,6 +430,7 @@ int fvm_rhs_timed(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_str
   int rc = FVM_OK;
   const int tt = c->m->tile_tris;
   switch (c->flux_kind) {
+    case FVM_FLUX_JIT: rc = jit_launch(c->jit->tile_rhs, tile_grid(a, c->jit->block).x, c->jit->block, &a, st); break;
     case FVM_FLUX_D_POLY_U: launch_tiles<FVM_FLUX_D_POLY_U, M_RHS>(a, tt, st); break;
     case FVM_FLUX_D_NODAL: launch_tiles<FVM_FLUX_D_NODAL, M_RHS>(a, tt, st); break;
     case FVM_FLUX_LINEAR: launch_tiles<FVM_FLUX_LINEAR, M_RHS>(a, tt, st); break;
@@ -1166,6 +439,10 @@ int fvm_rhs_timed(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_str
   }
   FVM_HIP(hipEventRecord(ev[1], st));
   switch (c->flux_kind) {
+    case FVM_FLUX_JIT:
+      if (rc == FVM_OK && a.n_fix > 0)
+        rc = jit_launch(c->jit->fix_rhs, (unsigned)((a.n_fix + 255) / 256), 256, &a, st);
+      break;
     case FVM_FLUX_D_POLY_U: launch_fix<FVM_FLUX_D_POLY_U, M_RHS>(a, st); break;
     case FVM_FLUX_D_NODAL: launch_fix<FVM_FLUX_D_NODAL, M_RHS>(a, st); break;
     case FVM_FLUX_LINEAR: launch_fix<FVM_FLUX_LINEAR, M_RHS>(a, st); break;
@@ -1208,6 +485,12 @@ int fvm_vjp(fvm_ctx* c, const double* d_u, const double* d_w, double* d_ubar, do
   a.n_bitems = c->m->n_bvert_items;
   hipStream_t st = (hipStream_t)s;
   switch (c->flux_kind) {
+    case FVM_FLUX_JIT: {
+      const JitKernels* j = c->jit;
+      int rc = jit_launch(j->vjp_tile, tile_grid(a, j->block).x, j->block, &a, st);
+      if (rc == FVM_OK && a.n_fix > 0) rc = jit_launch(j->vjp_fix, (unsigned)((a.n_fix + 255) / 256), 256, &a, st);
+      return rc;
+    }
     case FVM_FLUX_D_POLY_U: return launch_vjp_fk<FVM_FLUX_D_POLY_U>(a, c->m->tile_tris, st);
     case FVM_FLUX_D_NODAL: return launch_vjp_fk<FVM_FLUX_D_NODAL>(a, c->m->tile_tris, st);
     case FVM_FLUX_LINEAR: return launch_vjp_fk<FVM_FLUX_LINEAR>(a, c->m->tile_tris, st);

@@ -18,12 +18,77 @@ struct Terms {
   const double* x[kMaxTerms];
 };
 
-__global__ __launch_bounds__(256) void lincomb_kernel(int64_t n, double* __restrict__ out, Terms t) {
+// out[i] = c_0 x_0[i] + c_1 x_1[i] + ..., the terms added left to right.  out may
+// alias an input: each element is read and written by the same thread.
+template <int NT>
+__device__ __forceinline__ double comb(const Terms& t, const double* xv) {
+  double v = t.c[0] * xv[0];
+#pragma unroll
+  for (int k = 1; k < NT; ++k) v = v + t.c[k] * xv[k];
+  return v;
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void lincomb_kernel(int64_t n, double* out, Terms t) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    double v = t.c[0] * t.x[0][i];
-    for (int k = 1; k < t.n; ++k) v = v + t.c[k] * t.x[k][i];
-    out[i] = v;
+    double xv[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) xv[k] = t.x[k][i];
+    out[i] = comb<NT>(t, xv);
+  }
+}
+
+// The same combination over pairs of elements with 16-byte loads and stores (every
+// vector 16-byte aligned), two pairs per thread and iteration with all loads issued
+// before the first store; the last element of an odd n is done by thread 0.
+template <int NT>
+__global__ __launch_bounds__(256) void lincomb2_kernel(int64_t n, double* out, Terms t) {
+  const int64_t np = n >> 1;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += 2 * stride) {
+    const int64_t p2 = p + stride;
+    const bool two = p2 < np;
+    double2 a[NT], b[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      a[k] = reinterpret_cast<const double2*>(t.x[k])[p];
+      if (two) b[k] = reinterpret_cast<const double2*>(t.x[k])[p2];
+    }
+    double xa[NT], ya[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+      xa[k] = a[k].x;
+      ya[k] = a[k].y;
+    }
+    reinterpret_cast<double2*>(out)[p] = make_double2(comb<NT>(t, xa), comb<NT>(t, ya));
+    if (two) {
+#pragma unroll
+      for (int k = 0; k < NT; ++k) {
+        xa[k] = b[k].x;
+        ya[k] = b[k].y;
+      }
+      reinterpret_cast<double2*>(out)[p2] = make_double2(comb<NT>(t, xa), comb<NT>(t, ya));
+    }
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    double xv[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) xv[k] = t.x[k][n - 1];
+    out[n - 1] = comb<NT>(t, xv);
+  }
+}
+
+template <int NT>
+void launch_lincomb(int64_t n, double* out, const Terms& t, bool aligned, hipStream_t s) {
+  if (aligned) {
+    int64_t blocks = ((n >> 1) + 511) / 512;  // two pairs per thread
+    blocks = blocks < 1 ? 1 : (blocks > 8192 ? 8192 : blocks);
+    hipLaunchKernelGGL(lincomb2_kernel<NT>, dim3((unsigned)blocks), dim3(256), 0, s, n, out, t);
+  } else {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(lincomb_kernel<NT>, dim3((unsigned)blocks), dim3(256), 0, s, n, out, t);
   }
 }
 
@@ -54,6 +119,35 @@ __global__ __launch_bounds__(kRedThreads) void reduce_stage1(int kind, int64_t n
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     acc += red_term(kind, x, y, a, b, abstol, reltol, i);
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// The error estimate of a Runge-Kutta step without storing the error vector: the
+// kind-1 reduction of err = sum_k c_k x_k, each err_i formed as lincomb forms it and
+// summed in reduce_stage1's order, so the result equals lincomb followed by the
+// reduction bit for bit.
+template <int NT>
+__global__ __launch_bounds__(kRedThreads) void lc_err_stage1(int64_t n, Terms t, const double* a,
+                                                             const double* b, double abstol, double reltol,
+                                                             double* part) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double xv[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) xv[k] = t.x[k][i];
+    const double e = comb<NT>(t, xv);
+    const double sc = abstol + reltol * fmax(fabs(a[i]), fabs(b[i]));
+    const double r = e / sc;
+    acc += r * r;
+  }
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int s = kRedThreads / 2; s > 0; s >>= 1) {
@@ -169,7 +263,7 @@ double* g_ascratch = nullptr;  // Arnoldi step results: 2 kMaxDots coefficients 
 double* g_scratch = nullptr;   // kRedBlocks partials + 1 result
 double* g_mscratch = nullptr;  // kMaxDots * kRedBlocks partials + kMaxDots results
 
-int scratch(double** out) {
+int red_scratch(double** out) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_scratch) FVM_HIP(hipMalloc((void**)&g_scratch, sizeof(double) * (kRedBlocks + 8)));
   *out = g_scratch;
@@ -186,7 +280,7 @@ int mscratch(double** out) {
 int reduce(int kind, int64_t n, const double* x, const double* y, const double* a, const double* b,
            double abstol, double reltol, double* result, hipStream_t s) {
   double* buf;
-  int rc = scratch(&buf);
+  int rc = red_scratch(&buf);
   if (rc) return rc;
   std::lock_guard<std::mutex> use(g_use);
   hipLaunchKernelGGL(reduce_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, s, kind, n, x, y, a, b,
@@ -209,7 +303,7 @@ int cgs2_norm(int64_t n, int k, const double* const* xs, double* w, double* h_ho
   if (k < 1 || k > kMaxDots) return FVM_ERR_ARG;
   double *buf, *red;
   int rc = mscratch(&buf);
-  if (rc || (rc = scratch(&red))) return rc;
+  if (rc || (rc = red_scratch(&red))) return rc;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_ascratch) FVM_HIP(hipMalloc((void**)&g_ascratch, sizeof(double) * (2 * kMaxDots + 8)));
@@ -223,8 +317,9 @@ int cgs2_norm(int64_t n, int k, const double* const* xs, double* w, double* h_ho
   }
   int64_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
+  // results packed as [pass 0 coefficients | pass 1 coefficients | |w|^2]: one copy back
   for (int pass = 0; pass < 2; ++pass) {
-    double* res = g_ascratch + pass * kMaxDots;
+    double* res = g_ascratch + pass * k;
     for (int first = 0; first < k; first += kDotChunk) {
       DotChunk c;
       c.first = first;
@@ -237,11 +332,9 @@ int cgs2_norm(int64_t n, int k, const double* const* xs, double* w, double* h_ho
   }
   hipLaunchKernelGGL(reduce_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, s, 0, n, w, w, nullptr, nullptr,
                      0.0, 0.0, red);
-  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kRedThreads), 0, s, red, g_ascratch + 2 * kMaxDots);
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kRedThreads), 0, s, red, g_ascratch + 2 * k);
   FVM_HIP(hipGetLastError());
-  FVM_HIP(hipMemcpyAsync(h_host, g_ascratch, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-  FVM_HIP(hipMemcpyAsync(h_host + k, g_ascratch + kMaxDots, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-  FVM_HIP(hipMemcpyAsync(h_host + 2 * k, g_ascratch + 2 * kMaxDots, sizeof(double), hipMemcpyDeviceToHost, s));
+  FVM_HIP(hipMemcpyAsync(h_host, g_ascratch, sizeof(double) * (2 * k + 1), hipMemcpyDeviceToHost, s));
   FVM_HIP(hipStreamSynchronize(s));
   return FVM_OK;
 }
@@ -262,10 +355,61 @@ int fvm_vec_lincomb(int64_t n, double* d_out, int nterms, const double* coefs,
     t.c[k] = coefs[k];
     t.x[k] = d_xs[k];
   }
-  int64_t blocks = (n + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(lincomb_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)s, n, d_out, t);
+  bool aligned = ((uintptr_t)d_out & 15) == 0;
+  for (int k = 0; k < nterms; ++k) aligned = aligned && ((uintptr_t)d_xs[k] & 15) == 0;
+  const hipStream_t st = (hipStream_t)s;
+  switch (nterms) {
+    case 1: launch_lincomb<1>(n, d_out, t, aligned, st); break;
+    case 2: launch_lincomb<2>(n, d_out, t, aligned, st); break;
+    case 3: launch_lincomb<3>(n, d_out, t, aligned, st); break;
+    case 4: launch_lincomb<4>(n, d_out, t, aligned, st); break;
+    case 5: launch_lincomb<5>(n, d_out, t, aligned, st); break;
+    case 6: launch_lincomb<6>(n, d_out, t, aligned, st); break;
+    case 7: launch_lincomb<7>(n, d_out, t, aligned, st); break;
+    default: launch_lincomb<8>(n, d_out, t, aligned, st); break;
+  }
   FVM_HIP(hipGetLastError());
+  return FVM_OK;
+}
+
+int fvm_vec_lincomb_err_norm(int64_t n, int nterms, const double* coefs, const double* const* d_xs,
+                             const double* d_u0, const double* d_u1, double abstol, double reltol,
+                             double* result, fvm_stream s) {
+  FVM_CHECK(coefs && d_xs && d_u0 && d_u1 && result, "fvm_vec_lincomb_err_norm: null argument");
+  FVM_CHECK(nterms >= 1 && nterms <= kMaxTerms, "fvm_vec_lincomb_err_norm: 1..8 terms");
+  if (n <= 0) {
+    *result = 0.0;
+    return FVM_OK;
+  }
+  Terms t;
+  t.n = nterms;
+  for (int k = 0; k < nterms; ++k) {
+    FVM_CHECK(d_xs[k], "fvm_vec_lincomb_err_norm: null input vector");
+    t.c[k] = coefs[k];
+    t.x[k] = d_xs[k];
+  }
+  double* buf;
+  int rc = red_scratch(&buf);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> use(g_use);
+  const hipStream_t st = (hipStream_t)s;
+  const dim3 g(kRedBlocks), b(kRedThreads);
+  switch (nterms) {
+    case 1: hipLaunchKernelGGL(lc_err_stage1<1>, g, b, 0, st, n, t, d_u0, d_u1, abstol, reltol, buf); break;
+    case 2: hipLaunchKernelGGL(lc_err_stage1<2>, g, b, 0, st, n, t, d_u0, d_u1, abstol, reltol, buf); break;
+    case 3: hipLaunchKernelGGL(lc_err_stage1<3>, g, b, 0, st, n, t, d_u0, d_u1, abstol, reltol, buf); break;
+    case 4: hipLaunchKernelGGL(lc_err_stage1<4>, g, b, 0, st, n, t, d_u0, d_u1, abstol, reltol, buf); break;
+    case 5: hipLaunchKernelGGL(lc_err_stage1<5>, g, b, 0, st, n, t, d_u0, d_u1, abstol, reltol, buf); break;
+    case 6: hipLaunchKernelGGL(lc_err_stage1<6>, g, b, 0, st, n, t, d_u0, d_u1, abstol, reltol, buf); break;
+    case 7: hipLaunchKernelGGL(lc_err_stage1<7>, g, b, 0, st, n, t, d_u0, d_u1, abstol, reltol, buf); break;
+    default: hipLaunchKernelGGL(lc_err_stage1<8>, g, b, 0, st, n, t, d_u0, d_u1, abstol, reltol, buf); break;
+  }
+  hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(kRedThreads), 0, st, buf, buf + kRedBlocks);
+  FVM_HIP(hipGetLastError());
+  double sum = 0.0;
+  FVM_HIP(hipMemcpyAsync(&sum, buf + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, st));
+  FVM_HIP(hipStreamSynchronize(st));
+  *result = sqrt(sum / (double)n);
   return FVM_OK;
 }
 

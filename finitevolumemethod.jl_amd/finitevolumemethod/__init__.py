@@ -20,6 +20,7 @@ from .mesh import Triangulation, jittered_rectangle, triangulate, triangulate_re
 from .problem import FVMProblem, SteadyFVMProblem, construct_flux_function  # noqa: F401
 from .system import FVMSystem, InvalidFluxError, is_system  # noqa: F401
 from .queries import compute_flux, find_triangle, pl_interpolate  # noqa: F401
+from .rules import NoTangent, Tangent, frule, rrule  # noqa: F401
 from .solve import SSPRK3, TRBDF2, Euler, ODESolution, Tsit5, solve  # noqa: F401
 from .templates import (AbstractFVMTemplate, BiCGStab, CG, DiffusionEquation,  # noqa: F401
                         LaplacesEquation, LinearReactionDiffusionEquation, LinearSolution,

@@ -46,7 +46,7 @@ class PhysicsDesc(ctypes.Structure):
                 ("node_kind", _i8p), ("node_fidx", _i32p), ("bedge_neumann_fidx", _i32p),
                 ("n_functions", ctypes.c_int32), ("fn_c0", _dp), ("fn_cu", _dp),
                 ("fn_ct", _dp), ("fn_cx", _dp), ("fn_cy", _dp), ("node_g", _dp),
-                ("bedge_g", _dp), ("fn_lam", _dp)]
+                ("bedge_g", _dp), ("fn_lam", _dp), ("jit_source", ctypes.c_char_p)]
 
 
 class _Tiles(ctypes.Structure):
@@ -122,7 +122,7 @@ class SystemDesc(ctypes.Structure):
                 ("node_g", _dp), ("bedge_g", _dp)]
 
 
-FLUX_D_POLY_U, FLUX_D_NODAL, FLUX_D_EDGE, FLUX_LINEAR, FLUX_D_POWER = 0, 1, 2, 3, 4
+FLUX_D_POLY_U, FLUX_D_NODAL, FLUX_D_EDGE, FLUX_LINEAR, FLUX_D_POWER, FLUX_JIT = 0, 1, 2, 3, 4, 5
 TEMPLATE_POISSON, TEMPLATE_MEAN_EXIT_TIME, TEMPLATE_DIFFUSION = 0, 1, 2
 SOLVE_CG, SOLVE_CG_MIXED, SOLVE_BICGSTAB, SOLVE_CG_RELIABLE = 0, 1, 2, 3
 NODE_NONE, NODE_DIRICHLET, NODE_DUDT = 0, 1, 2
@@ -201,7 +201,7 @@ HIP_API = {
     "fvm_template_create": [_vp, ctypes.POINTER(TemplateDesc), ctypes.POINTER(_vp)],
     "fvm_template_destroy": [_vp],
     "fvm_template_info": [_vp, _i64p, _i64p],
-    "fvm_template_ell": [_vp, ctypes.POINTER(ctypes.c_int32), _i64p],
+    "fvm_template_ell": [_vp, ctypes.POINTER(ctypes.c_int32), _i64p, ctypes.POINTER(ctypes.c_int32)],
     "fvm_template_download": [_vp, _i32p, _i32p, _dp, _dp],
     "fvm_template_apply": [_vp, _vp, _vp, _vp],
     "fvm_template_spmv": [_vp, ctypes.c_int32, _vp, _vp, _vp],
@@ -231,6 +231,10 @@ HIP_API = {
     "fvm_vec_dot": [ctypes.c_int64, _vp, _vp, _dp, _vp],
     "fvm_vec_mdot": [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_vp), _vp, _dp, _vp],
     "fvm_vec_maxpy": [ctypes.c_int64, _vp, ctypes.c_int32, _dp, ctypes.POINTER(_vp), _vp],
+    "fvm_trbdf2_affine": [_vp, _vp, _vp, _vp, _vp],
+    "fvm_jit_check": [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_size_t)],
+    "fvm_vec_lincomb_err_norm": [ctypes.c_int64, ctypes.c_int, _dp, ctypes.POINTER(_vp), _vp, _vp,
+                                 ctypes.c_double, ctypes.c_double, _dp, _vp],
     "fvm_vec_err_norm": [ctypes.c_int64, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _dp,
                          _vp],
 }
@@ -409,6 +413,17 @@ def err_norm(err, u0, u1, abstol, reltol, stream=None):
     r = ctypes.c_double(0)
     check(hip_lib().fvm_vec_err_norm(err.n, err.ptr, u0.ptr, u1.ptr, abstol, reltol,
                                      ctypes.byref(r), stream), "fvm_vec_err_norm")
+    return r.value
+
+
+def lincomb_err_norm(coefs, xs, u0, u1, abstol, reltol, stream=None):
+    """err_norm(sum coefs[k] * xs[k], u0, u1, abstol, reltol) without storing the sum."""
+    n = len(xs)
+    c = (ctypes.c_double * n)(*[float(v) for v in coefs])
+    p = (_vp * n)(*[x.ptr for x in xs])
+    r = ctypes.c_double(0)
+    check(hip_lib().fvm_vec_lincomb_err_norm(u0.n, n, c, p, u0.ptr, u1.ptr, abstol, reltol,
+                                             ctypes.byref(r), stream), "fvm_vec_lincomb_err_norm")
     return r.value
 
 

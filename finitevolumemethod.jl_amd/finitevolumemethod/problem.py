@@ -155,6 +155,11 @@ def physics_desc(phys_kwargs):
     d.fn_lam = _lib.ptr(arr(fn[5]) if len(fn) > 5 else None)
     d.node_g = _lib.ptr(arr(phys_kwargs.get("node_g")))
     d.bedge_g = _lib.ptr(arr(phys_kwargs.get("bedge_g")))
+    src = phys_kwargs.get("jit_source")
+    if src is not None:
+        b = ctypes.create_string_buffer(src.encode())
+        keep.append(b)
+        d.jit_source = ctypes.cast(b, ctypes.c_char_p)
     return d, keep
 
 
@@ -247,12 +252,33 @@ def conditions_physics(conds, tri):
                 bedge_g=bedge_g if any_bedge_g else None)
 
 
+def _jit_physics(prob):
+    """FVM_FLUX_JIT physics: the diffusion and source closures compiled for the device
+    (jit.py), for closures outside the traced forms."""
+    from . import jit
+
+    D, Dp = prob.diffusion_function, prob.diffusion_parameters
+    S, Sp = prob.source_function, prob.source_parameters
+    return dict(flux_kind=_lib.FLUX_JIT,
+                jit_source=jit.source_of(lambda x, y, t, u: D(x, y, t, u, Dp),
+                                         lambda x, y, t, u: S(x, y, t, u, Sp)))
+
+
 def physics_of(prob):
-    """Trace an FVMProblem's closures onto the device forms (see forms.py)."""
+    """Trace an FVMProblem's closures onto the device forms (see forms.py); a diffusion
+    or source closure outside them is compiled for the device instead (jit.py)."""
     mesh = prob.mesh
     tri = mesh.triangulation
     ph = conditions_physics(prob.conditions, tri)
     D = prob.diffusion_function
+    if (prob.user_flux_function is None and D is not None and not isinstance(D, forms.NodalDiffusion)):
+        Dp, S, Sp = prob.diffusion_parameters, prob.source_function, prob.source_parameters
+        try:
+            forms.classify_diffusion(lambda x, y, t, u: D(x, y, t, u, Dp))
+            forms.classify_source(lambda x, y, t, u: S(x, y, t, u, Sp), tri.x, tri.y)
+        except forms.UnsupportedClosure:
+            ph.update(_jit_physics(prob))
+            return ph
     if prob.user_flux_function is None and isinstance(D, forms.NodalDiffusion):
         ph.update(flux_kind=_lib.FLUX_D_NODAL, log_d=D.log_d)
     else:

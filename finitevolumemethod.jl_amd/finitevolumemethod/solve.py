@@ -74,6 +74,7 @@ class TRBDF2:
     krylov_rtol: float = 1e-3
     krylov_restart: int = 30
     smooth_est: bool = True
+    device_loop: bool = True       # small affine problems: the whole integration in one launch (implicit.py)
 
 
 @dataclass
@@ -88,6 +89,39 @@ class ODESolution:
 
     def __len__(self):
         return len(self.u)
+
+
+def ctrl_log(x):
+    """log(x) as the library's step controllers compute it (fvm_ctrl.hpp, the same
+    operations in the same order): x = m 2^e, m in [sqrt(1/2), sqrt(2)),
+    s = (m - 1) / (m + 1), log m = 2 (s + s^3 / 3 + ... + s^25 / 25)."""
+    m, e = math.frexp(x)
+    if m < 0.70710678118654752:
+        m = m * 2.0
+        e = e - 1
+    s = (m - 1.0) / (m + 1.0)
+    s2 = s * s
+    p = 1.0 / 25.0
+    for k in range(23, 0, -2):
+        p = p * s2 + 1.0 / float(k)
+    return 2.0 * s * p + float(e) * 0.69314718055994531
+
+
+def ctrl_exp(y):
+    """exp(y) as fvm_ctrl.hpp computes it: y = k ln 2 + r, Taylor to r^20 / 20!, times 2^k."""
+    k = math.floor(y / 0.69314718055994531 + 0.5)
+    r = y - k * 0.69314718055994531
+    p = 1.0
+    for i in range(20, 0, -1):
+        p = 1.0 + p * r / float(i)
+    return math.ldexp(p, int(k))
+
+
+def ctrl_pow(x, p):
+    """x^p of the step-size controllers: the library's device and host loops and this
+    package's host loop use the same operations, so they pick the same steps bit for
+    bit (libm's and the device library's pow differ in the last place)."""
+    return ctrl_exp(p * ctrl_log(x))
 
 
 class _Saver:
@@ -166,6 +200,11 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
     if not isinstance(prob, FVMProblem):
         raise TypeError(f"solve: unsupported problem type {type(prob).__name__}")
     ctx = prob.device_context()
+    if isinstance(alg, TRBDF2):
+        from . import implicit
+
+        if implicit.eligible(prob, alg, saveat, callback):
+            return implicit.trbdf2_affine(prob, alg, saveat, abstol, reltol, dt, maxiters, save_start)
     rhs = lambda u, du, t: ctx.rhs(u, du, t)  # noqa: E731
     has_dir = prob.conditions.has_dirichlet_nodes()
     cb = (lambda u, t: ctx.apply_dirichlet(u, t)) if has_dir else None
@@ -259,7 +298,6 @@ def _integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, d
         k = [_lib.DeviceArray(n) for _ in range(7)]
         utmp = _lib.DeviceArray(n)
         unew = _lib.DeviceArray(n)
-        err = _lib.DeviceArray(n)
         interp = _lib.DeviceArray(n)
         f(u, k[0], t)
         if dt is None:
@@ -279,8 +317,7 @@ def _integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, d
                 dst = unew if s == 6 else utmp
                 _lib.lincomb(dst, coefs, xs)
                 f(dst, k[s], t + _TSIT5_C[s] * dt)
-            _lib.lincomb(err, [dt * b for b in _TSIT5_BTILDE], k)
-            EEst = _lib.err_norm(err, u, unew, abstol, reltol)
+            EEst = _lib.lincomb_err_norm([dt * b for b in _TSIT5_BTILDE], k, u, unew, abstol, reltol)
             if not np.isfinite(EEst):
                 sol.retcode = "Unstable"
                 break
@@ -307,13 +344,13 @@ def _integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, d
                     k[0], k[6] = k[6], k[0]  # first same as last
                 else:
                     f(u, k[0], t)
-                q = max(EEst, 1e-16) ** (7 / 50) / qold ** (2 / 25)
+                q = ctrl_pow(max(EEst, 1e-16), 7 / 50) / ctrl_pow(qold, 2 / 25)
                 q = min(5.0, max(0.1, q / 0.9))
                 dt = dt / q
                 qold = max(EEst, 1e-4)
             else:
                 nreject += 1
-                q = min(5.0, max(EEst, 1e-16) ** (7 / 50) / 0.9)
+                q = min(5.0, ctrl_pow(max(EEst, 1e-16), 7 / 50) / 0.9)
                 dt = dt / q
         sol.retcode = "Success" if t >= tf else (sol.retcode if sol.retcode != "Default" else "MaxIters")
     else:
@@ -417,7 +454,7 @@ def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, 
         starting from the previous solve's; returns False when Newton diverges or
         does not converge in max_newton iterations."""
         A = W_apply(ubase, tbase, h)
-        eta = max(nl["eta"], 1e-16) ** 0.8
+        eta = ctrl_pow(max(nl["eta"], 1e-16), 0.8)
         prev = None
         for _ in range(alg.max_newton):
             f(z, fz, ts)
@@ -513,7 +550,7 @@ def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, 
                 fn.copy_from(f1)
         else:
             stats["nreject"] += 1
-        q = 0.9 * max(EEst, 1e-10) ** (-1.0 / 3.0)
+        q = 0.9 * ctrl_pow(max(EEst, 1e-10), -1.0 / 3.0)
         dt = h * min(5.0, max(0.2, q))
     if sol.retcode == "Default":
         sol.retcode = "Success" if t >= tf else "MaxIters"

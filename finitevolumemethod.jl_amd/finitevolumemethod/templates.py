@@ -182,12 +182,13 @@ class DeviceTemplate:
         _lib.check(_lib.hip_lib().fvm_template_spmv(self._handle, int(which), x.ptr, y.ptr, stream),
                    "fvm_template_spmv")
 
-    def ell_info(self):
-        """(slots per row of the ELL copy of the operator, off-diagonals in its overflow list)."""
-        w, o = ctypes.c_int32(), ctypes.c_int64()
-        _lib.check(_lib.hip_lib().fvm_template_ell(self._handle, ctypes.byref(w), ctypes.byref(o)),
-                   "fvm_template_ell")
-        return w.value, o.value
+    def ell_info(self, index_bytes=False):
+        """(slots per row of the ELL copy of the operator, off-diagonals in its overflow
+        list[, bytes of each slot's column index: 2 for 16-bit offsets, 4 for columns])."""
+        w, o, ib = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
+        _lib.check(_lib.hip_lib().fvm_template_ell(self._handle, ctypes.byref(w), ctypes.byref(o),
+                                                   ctypes.byref(ib)), "fvm_template_ell")
+        return (w.value, o.value, ib.value) if index_bytes else (w.value, o.value)
 
     def residual(self, x, stream=None):
         r = ctypes.c_double()

@@ -111,8 +111,12 @@ enum {
                             a's given at every CV-edge midpoint and boundary half-edge point
                             (independent of t and u): advection-diffusion, anisotropic
                             diffusion, any flux linear in u and its gradient */
-  FVM_FLUX_D_POWER = 4   /* q = -D(u_s)(alpha, beta), D = dpoly[0] + dpoly[1] |u_s|^dpoly[2],
+  FVM_FLUX_D_POWER = 4,  /* q = -D(u_s)(alpha, beta), D = dpoly[0] + dpoly[1] |u_s|^dpoly[2],
                             dpoly[2] > 0 (porous-medium diffusivity) */
+  FVM_FLUX_JIT = 5       /* q = -D(x_s, y_s, t, u_s)(alpha, beta) and the source S(x, y, t, u) of
+                            any closure (problem.jl:118-163, 425-440), given as HIP source in
+                            jit_source and compiled with hipRTC at fvm_ctx_create; spoly, st
+                            and s_node are then unused */
 };
 enum { FVM_NODE_NONE = 0, FVM_NODE_DIRICHLET = 1, FVM_NODE_DUDT = 2 };
 
@@ -140,9 +144,22 @@ typedef struct fvm_physics_desc {
   const double* node_g;         /* [n_points] g at Dirichlet / Dudt nodes, or NULL (0) */
   const double* bedge_g;        /* [2*n_boundary_edges] g at Neumann half-edge points, or NULL */
   const double* fn_lam;         /* [n_functions] exp rate of the g term, or NULL (0) */
+  const char* jit_source;       /* FVM_FLUX_JIT: HIP source defining, for the point (x, y), time t
+                                   and solution value u,
+                                     __device__ double fvm_jit_D(double x, double y, double t, double u);
+                                     __device__ double fvm_jit_dD(double x, double y, double t, double u);
+                                     __device__ double fvm_jit_S(double x, double y, double t, double u);
+                                     __device__ double fvm_jit_dS(double x, double y, double t, double u);
+                                   (dD, dS: derivatives in u); NULL otherwise */
 } fvm_physics_desc;
 
 int fvm_ctx_create(const fvm_mesh* m, const fvm_physics_desc* phys, fvm_ctx** out);
+/* Compiles the FVM_FLUX_JIT kernels of jit_source for gfx950 (tiles of tile_triangles,
+ * 0 = the default 512) without loading them: no GPU needed.  Returns FVM_ERR_ARG with
+ * the compiler's diagnostics in fvm_last_error when the source does not compile;
+ * code_bytes (may be NULL) receives the code object's size.  Julia compiles a closure
+ * into fvm_eqs! on first call (problem.jl:425-440); this is that step, checked ahead. */
+int fvm_jit_check(const char* jit_source, int32_t tile_triangles, size_t* code_bytes);
 int fvm_ctx_destroy(fvm_ctx* c);
 /* FVM_FLUX_D_NODAL: use the device field d_log_d [n_points] from now on. */
 int fvm_ctx_set_log_d(fvm_ctx* c, const double* d_log_d);
@@ -234,10 +251,11 @@ int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_templ
 int fvm_template_destroy(fvm_template* t);
 int fvm_template_info(const fvm_template* t, int64_t* n, int64_t* nnz);
 /* The ELL copy of the operator used by the Krylov products and fvm_template_apply /
- * fvm_template_spmv: slots per row (0: none, CSR throughout) and the number of
- * off-diagonals kept in the overflow list.  No counterpart in the reference (its
- * operator is a SparseMatrixCSC handed to KLU). */
-int fvm_template_ell(const fvm_template* t, int32_t* width, int64_t* overflow);
+ * fvm_template_spmv: slots per row (0: none, CSR throughout), the number of
+ * off-diagonals kept in the overflow list and the bytes of each slot's column index
+ * (2: offsets from the row, when every one fits 16 bits; 4: columns).  No counterpart
+ * in the reference (its operator is a SparseMatrixCSC handed to KLU). */
+int fvm_template_ell(const fvm_template* t, int32_t* width, int64_t* overflow, int32_t* index_bytes);
 /* the assembled A (CSR, 0-based, int32) and b; any pointer may be NULL */
 int fvm_template_download(const fvm_template* t, int32_t* row_ptr, int32_t* col, double* val,
                           double* b);
@@ -417,6 +435,62 @@ int fvm_vec_maxpy(int64_t n, double* d_y, int32_t k, const double* coefs, const 
  * error norm of OrdinaryDiffEq's adaptive step control. */
 int fvm_vec_err_norm(int64_t n, const double* d_err, const double* d_u0, const double* d_u1,
                      double abstol, double reltol, double* result, fvm_stream s);
+/* fvm_vec_err_norm of err = sum_k coefs[k] * d_xs[k] (1 <= nterms <= 8) without
+ * storing err: the Runge-Kutta error estimate (the b - bhat weights times the
+ * stages) and its norm in one pass; equal bit for bit to fvm_vec_lincomb followed
+ * by fvm_vec_err_norm. */
+int fvm_vec_lincomb_err_norm(int64_t n, int nterms, const double* coefs, const double* const* d_xs,
+                             const double* d_u0, const double* d_u1, double abstol, double reltol,
+                             double* result, fvm_stream s);
+
+/* ---- TR-BDF2 for small affine problems (solve.jl:167-183 with TRBDF2) -------
+ * f(u) = J u + c with J constant (linear fluxes, sources affine in u, conditions
+ * constant in t): the whole TR-BDF2 integration (the tutorials' stiff integrator,
+ * there with a KLU factorisation of J) in one workgroup: Newton with GMRES on
+ * W = I - d h J per stage, the W^-1-filtered error estimate, the step controller,
+ * the Dirichlet update after every accepted step (constant values) and saveat.
+ * J comes in ELL form (width columns per row, slot-major [width][n], padding slots
+ * with the row's own column and a zero value, further entries in an overflow CSR).
+ * Small n only (<= 65536): larger problems use the host-driven integrator. */
+typedef struct fvm_affine_op {
+  int64_t n;
+  int32_t width;
+  const int32_t* d_col;      /* [width * n] */
+  const double* d_val;       /* [width * n] */
+  const double* d_diag;      /* [n] J_ii */
+  const int32_t* d_ovf_rp;   /* [n + 1] */
+  const int32_t* d_ovf_col;  /* [ovf_rp[n]] (may be NULL when empty) */
+  const double* d_ovf_val;
+  const double* d_c;         /* [n] f(0) */
+  int32_t n_dirichlet;
+  const int32_t* d_dir_nodes; /* [n_dirichlet] */
+  const double* d_dir_values; /* [n_dirichlet] */
+} fvm_affine_op;
+
+typedef struct fvm_trbdf2_opts {
+  double t0, tf, dt;          /* dt: the first step (> 0) */
+  double abstol, reltol;
+  int64_t maxiters;
+  double newton_kappa;        /* Newton stops when theta / (1 - theta) |dz| <= kappa */
+  int32_t max_newton;
+  int32_t krylov_restart;     /* 1..30 */
+  double krylov_rtol;
+  int32_t smooth_est;         /* filter the error estimate through W^-1 */
+  int64_t n_save;
+  const double* save_t;       /* host [n_save], ascending, > t0 */
+  double* d_saves;            /* device [n_save * n] */
+} fvm_trbdf2_opts;
+
+typedef struct fvm_trbdf2_stats {
+  int64_t nf, njvp, naccept, nreject, nnewton, nkrylov, nnewton_fail, nsaved;
+  int32_t retcode;            /* 0 Success, 1 MaxIters, 2 Unstable, 3 ConvergenceFailure */
+  int32_t reserved;
+  double t;
+} fvm_trbdf2_stats;
+
+/* d_u [n]: the initial state in, the state at st->t out.  Blocks until done. */
+int fvm_trbdf2_affine(const fvm_affine_op* op, const fvm_trbdf2_opts* o, double* d_u, fvm_trbdf2_stats* st,
+                      fvm_stream s);
 
 /* ---- explicit time integration (solve(::FVMProblem), solve.jl:133-220) -----
  * The whole adaptive or fixed-step integration of du/dt = f(u, t) in one call, f one

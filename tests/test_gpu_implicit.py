@@ -97,3 +97,54 @@ def test_trbdf2_takes_large_steps_on_stiff_problems(gpu):
     # measured: 75 TR-BDF2 steps against 712 Tsit5 steps (the start is resolved by both)
     assert imp.stats["naccept"] * 5 < exp_.stats["naccept"], (imp.stats, exp_.stats)
     assert np.max(np.abs(imp.u[-1] - exp_.u[-1])) <= 0.05
+
+
+def test_affine_problems_run_in_one_launch_and_match_the_host_integrator(gpu):
+    """f(u) = J u + c (constant D, Neumann flux or Dirichlet values constant): the
+    Jacobian extracted from fvm_jvp probes equals the JVP, the one-workgroup TR-BDF2
+    (fvm_trbdf2_affine) follows the host integrator step for step to the Krylov and
+    Newton tolerances, and nonlinear or time-dependent problems stay on the host."""
+    from finitevolumemethod import _lib, implicit
+
+    tri, prob, orc, u0 = plate()
+    assert implicit.is_affine(prob)
+    A = implicit.affine_operator(prob)
+    rp, col, val = A.csr
+    import scipy.sparse as sp
+    J = sp.csr_matrix((val, col, rp), shape=(A.n, A.n))
+    rng = np.random.default_rng(3)
+    ctx = prob.device_context()
+    for _ in range(2):
+        v = rng.standard_normal(A.n)
+        jv = _lib.DeviceArray(A.n)
+        ctx.jvp(_lib.DeviceArray.from_host(u0), _lib.DeviceArray.from_host(v), jv, 0.0)
+        ref = jv.to_host()
+        assert np.max(np.abs(J @ v - ref)) <= 1e-13 * np.max(np.abs(ref))
+        f = FVM.fvm_eqs(np.zeros(A.n), v, prob, 0.0)
+        assert np.max(np.abs(J @ v + A.d_c.to_host() - f)) <= 1e-13 * np.max(np.abs(f))
+    a = FVM.solve(prob, FVM.TRBDF2(), saveat=0.05)
+    b = FVM.solve(prob, FVM.TRBDF2(device_loop=False), saveat=0.05)
+    assert a.stats.get("device_loop") and not b.stats.get("device_loop")
+    assert a.retcode == b.retcode == "Success"
+    assert abs(a.stats["naccept"] - b.stats["naccept"]) <= 2, (a.stats, b.stats)
+    for x, y in zip(a.u, b.u):
+        assert np.max(np.abs(x - y)) <= 1e-5 * 50
+    # a Neumann problem (the 100 x 100 walkthrough's kind) at a smaller size
+    L = 320.0
+    tri = FVM.triangulate_rectangle(0.0, L, 0.0, L, 40, 40, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    prob = FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 2.0 + 0.0 * u, FVM.Neumann),
+                          diffusion_function=lambda x, y, t, u, p: 2.0 + 0.0 * u,
+                          initial_condition=np.where((0.4 * L <= tri.y) & (tri.y <= 0.6 * L), 1.0, 0.0),
+                          final_time=500.0)
+    a = FVM.solve(prob, FVM.TRBDF2(), saveat=100.0)
+    b = FVM.solve(prob, FVM.TRBDF2(device_loop=False), saveat=100.0)
+    assert a.stats.get("device_loop") and a.retcode == b.retcode == "Success"
+    for x, y in zip(a.u, b.u):
+        assert np.max(np.abs(x - y)) <= 1e-4 * max(1.0, np.max(np.abs(y)))
+    # mass grows by the constant inflow 2 L per unit time through the whole boundary
+    V = mesh.cv_volumes
+    for t, u in zip(a.t, a.u):
+        assert abs(V @ u - (V @ a.u[0] + 2.0 * 4 * L * t)) <= 1e-6 * (V @ u)
+    prob_nl, _, _ = nonlinear_problem(nx=10, ny=9)
+    assert not implicit.is_affine(prob_nl)

@@ -113,6 +113,26 @@ def test_diffusion_template_matches_host_loop(gpu):
     b = integrate(lambda u, du, t: p.apply(u, du), prob._u0[:p.n], 0.0, 0.5, FVM.Tsit5(), saveat=0.05,
                   fsal=True)
     b.u = [np.concatenate([s, [1.0]]) for s in b.u]
+    # a template this small runs its whole loop in one workgroup: the same arithmetic
+    # per element, the error norm summed in the same order and the controller's power
+    # the host's (fvm_ctrl.hpp): the same states bit for bit
+    assert_same(a, b)
+
+
+def test_large_diffusion_template_matches_host_loop_bit_for_bit(gpu):
+    """Above the one-workgroup size (65,536 unknowns) the library loop is the
+    multi-kernel one: every saved state, time and counter equals the host loop's."""
+    tri = FVM.triangulate_rectangle(0.0, 2.0, 0.0, 2.0, 280, 280, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0, FVM.Dirichlet)
+    u0 = np.where(tri.y <= 1.0, 50.0, 0.0)
+    prob = FVM.DiffusionEquation(mesh, BCs, diffusion_function=lambda x, y, p: 1 / 9,
+                                 initial_condition=u0, final_time=2e-4)
+    a = FVM.solve(prob, FVM.Tsit5(), saveat=1e-4)
+    p = prob.problem
+    b = integrate(lambda u, du, t: p.apply(u, du), prob._u0[:p.n], 0.0, 2e-4, FVM.Tsit5(), saveat=1e-4,
+                  fsal=True)
+    b.u = [np.concatenate([s, [1.0]]) for s in b.u]
     assert_same(a, b)
 
 

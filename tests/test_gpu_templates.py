@@ -375,9 +375,10 @@ def test_ell_and_csr_forms_of_the_krylov_operator_agree(gpu, mesh, alg):
     BCs = FVM.BoundaryConditions(m, lambda x, y, t, u, p: 0.0 * x, FVM.Dirichlet)
     prob = FVM.PoissonsEquation(m, BCs, diffusion_function=lambda x, y, p: 1.0 + 0.5 * np.sin(3 * x) ** 2,
                                 source_function=lambda x, y, p: np.cos(2 * x) - y)
-    W, n_over = prob.problem.ell_info()
+    W, n_over, ib = prob.problem.ell_info(index_bytes=True)
     assert 4 <= W <= 10
     assert (n_over == 40 - W) if mesh == "wheel" else (W == 8 and n_over == 0)
+    assert ib == 2  # every column within 2^15 of its row: 16-bit offsets
     direct = spla.spsolve(prob.A.tocsc(), prob.b)
     sols = {f: FVM.solve(prob, alg, rtol=1e-11, spmv_format=f) for f in ("ell", "csr")}
     for f, s in sols.items():
@@ -427,3 +428,25 @@ def test_template_solve_forwards_callback_and_options(gpu):
     with pytest.raises(TypeError):
         FVM.solve(pois, callback=lambda u, t: None)
     assert FVM.solve(pois, reltol=1e-12).resid <= 1e-12
+
+
+def test_operator_with_columns_beyond_16_bit_offsets(gpu):
+    """Nodes numbered in a random order put neighbours far apart: the ELL copy then
+    keeps 32-bit columns, and the solvers give the same solution as SuperLU."""
+    base = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 200, 190, jitter=0.25, seed=31)
+    rng = np.random.default_rng(31)
+    perm = rng.permutation(base.num_points())  # new id of each old node
+    inv = np.argsort(perm)
+    tri = FVM.Triangulation(base.x[inv], base.y[inv], perm[base.triangles], perm[base.boundary_edges],
+                            base.boundary_segments)
+    m = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(m, lambda x, y, t, u, p: 0.0 * x, FVM.Dirichlet)
+    prob = FVM.PoissonsEquation(m, BCs, diffusion_function=lambda x, y, p: 1.0 + 0.5 * np.sin(3 * x) ** 2,
+                                source_function=lambda x, y, p: np.cos(2 * x) - y)
+    W, n_over, ib = prob.problem.ell_info(index_bytes=True)
+    assert ib == 4
+    direct = spla.spsolve(prob.A.tocsc(), prob.b)
+    for alg in (TM.CG(), TM.MixedCG(), TM.BiCGStab()):
+        s = FVM.solve(prob, alg, rtol=1e-11)
+        assert s.retcode == "Success", s.stats
+        assert np.linalg.norm(s.u - direct) <= 1e-8 * np.linalg.norm(direct)

@@ -90,3 +90,55 @@ def test_reductions_from_several_host_threads(gpu):
         _lib.check(L.fvm_stream_destroy(h), "fvm_stream_destroy")
     for k in range(4):
         assert out[k][0] == ref[k] and out[k][1] == refn[k]
+
+
+class _View:
+    """n doubles of a device buffer starting at element off (8-byte aligned only when
+    off is odd), for the unaligned path of the vector kernels."""
+
+    def __init__(self, arr, off, n):
+        self.arr, self.n, self.ptr = arr, n, arr.ptr + 8 * off
+
+    def to_host(self):
+        return self.arr.to_host()[(self.ptr - self.arr.ptr) // 8:][: self.n]
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 1000, 123457, 300002])
+@pytest.mark.parametrize("off", [0, 1], ids=["aligned", "offset"])
+def test_lincomb_equals_left_to_right_sum(gpu, n, off):
+    """Every term count, 16-byte aligned and odd-offset vectors, even and odd n:
+    out = c0 x0 + c1 x1 + ... summed left to right, which NumPy's elementwise
+    operations reproduce exactly (separately rounded products and sums)."""
+    rng = np.random.default_rng(n + 7 * off)
+    for nt in range(1, 9):
+        X = rng.standard_normal((nt, n))
+        c = rng.standard_normal(nt)
+        bufs = [_lib.DeviceArray.from_host(np.concatenate([np.zeros(off), x, [0.0]])) for x in X]
+        xs = [_View(b, off, n) for b in bufs]
+        out_buf = _lib.DeviceArray(n + off + 1)
+        out = _View(out_buf, off, n)
+        _lib.lincomb(out, c, xs)
+        ref = c[0] * X[0]
+        for k in range(1, nt):
+            ref = ref + c[k] * X[k]
+        assert np.array_equal(out.to_host(), ref), nt
+        # out aliasing the first input, as the integrators use it
+        _lib.lincomb(xs[0], c, xs)
+        assert np.array_equal(xs[0].to_host(), ref), nt
+
+
+@pytest.mark.parametrize("n", [1, 1001, 300001])
+def test_fused_error_norm_equals_lincomb_then_norm(gpu, n):
+    rng = np.random.default_rng(n)
+    k = [_lib.DeviceArray.from_host(rng.standard_normal(n)) for _ in range(7)]
+    u0 = _lib.DeviceArray.from_host(rng.standard_normal(n))
+    u1 = _lib.DeviceArray.from_host(rng.standard_normal(n))
+    c = rng.standard_normal(7) * 1e-3
+    err = _lib.DeviceArray(n)
+    _lib.lincomb(err, c, k)
+    ref = _lib.err_norm(err, u0, u1, 1e-6, 1e-3)
+    got = _lib.lincomb_err_norm(c, k, u0, u1, 1e-6, 1e-3)
+    assert got == ref
+    e = err.to_host()
+    sc = 1e-6 + 1e-3 * np.maximum(np.abs(u0.to_host()), np.abs(u1.to_host()))
+    assert abs(got - np.sqrt(np.mean((e / sc) ** 2))) <= 1e-13 * got

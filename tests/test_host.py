@@ -273,3 +273,70 @@ def test_c_abi_library_exports_every_declared_symbol():
     missing = [s for s in sorted(declared) if not hasattr(lib, s)]
     assert not missing, missing
     assert declared <= set(_lib.HIP_API), sorted(declared - set(_lib.HIP_API))
+
+
+def test_rkc_coefficients_stability_and_order():
+    """The RKC(s) recurrence (adjoint.rkc_coefficients) applied to y' = z y: |R(z)| <= 1
+    on its stability interval [-beta(s), 0] with beta(s) / s^2 -> 0.653 (damping 2/13),
+    second order (local error ~ z^3), R(z) the s-th Chebyshev-based stability
+    polynomial; rkc_stages picks the fewest stages covering the requested interval."""
+    from finitevolumemethod.adjoint import rkc_coefficients, rkc_stages
+
+    def R(z, c):
+        Y0, F0 = 1.0, z
+        Ym2, Ym1 = Y0, Y0 + c["mut1"] * F0
+        for j in range(2, c["s"] + 1):
+            Y = ((1 - c["mu"][j] - c["nu"][j]) * Y0 + c["mu"][j] * Ym1 + c["nu"][j] * Ym2
+                 + c["mut"][j] * z * Ym1 + c["gt"][j] * F0)
+            Ym2, Ym1 = Ym1, Y
+        return Ym1
+
+    for s in (2, 3, 7, 40, 150):
+        c = rkc_coefficients(s)
+        zs = -np.linspace(0.0, c["beta"], 4001)
+        assert max(abs(R(z, c)) for z in zs) <= 1.0 + 1e-12
+        # strong damping (eps = 10): stable on its interval, and with many stages the
+        # stiff part of it is damped to <= 0.25 per step
+        c10 = rkc_coefficients(s, 10.0)
+        zs10 = -np.linspace(0.05 * c10["beta"], c10["beta"], 4001)
+        r10 = max(abs(R(z, c10)) for z in zs10)
+        assert r10 <= 1.0 and (s < 40 or r10 <= 0.25)
+        assert abs(R(-1.5 * c["beta"], c)) > 1.0  # the interval ends near beta
+        if s >= 40:
+            assert abs(c["beta"] / s ** 2 - 0.653) < 2e-3
+        e1, e2 = (abs(R(-h, c) - np.exp(-h)) for h in (1e-2, 5e-3))
+        assert abs(np.log2(e1 / e2) - 3.0) < 0.05
+        assert c["c"][s] == pytest.approx(1.0, abs=1e-12)
+    for target in (1.0, 50.0, 5000.0):
+        s = rkc_stages(target, margin=1.0)
+        assert rkc_coefficients(s)["beta"] >= target
+        assert s == 2 or rkc_coefficients(s - 1)["beta"] < target
+
+
+def test_controller_power_is_the_same_in_python_and_cpp():
+    """The step controllers' x^p (fvm_ctrl.hpp, restated in solve.ctrl_pow) built
+    from operations that round the same everywhere: the C++ function (compiled for
+    the host here; the device build uses the same text) and the Python one agree bit
+    for bit, and both are within 1e-14 of x^p."""
+    import subprocess
+    import tempfile
+
+    from finitevolumemethod.solve import ctrl_pow
+
+    src = ('#include <cmath>\n#include "fvm_ctrl.hpp"\n'
+           'extern "C" double cpow(double x, double p) { return fvm::ctrl_pow(x, p); }\n')
+    d = tempfile.mkdtemp()
+    open(os.path.join(d, "c.cpp"), "w").write(src)
+    so = os.path.join(d, "c.so")
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-shared", "-fPIC", "-D__host__=",
+                    "-D__device__=", "-D__HIPCC_RTC__", "-I", os.path.join(ROOT, "finitevolumemethod.jl_amd", "csrc"),
+                    os.path.join(d, "c.cpp"), "-o", so], check=True)
+    f = ctypes.CDLL(so).cpow
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_double, ctypes.c_double]
+    rng = np.random.default_rng(5)
+    for x in 10.0 ** rng.uniform(-16, 3, 3000):
+        for p in (7 / 50, 2 / 25, 0.8, -1.0 / 3.0):
+            a, b = ctrl_pow(float(x), p), f(float(x), p)
+            assert a == b, (x, p, a, b)
+            assert abs(a - float(x) ** p) <= 1e-14 * float(x) ** p
