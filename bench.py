@@ -319,14 +319,18 @@ def published_walkthroughs():
     15.442 ms with KLU, and as a SteadyFVMProblem: 189.406 ms)."""
     import finitevolumemethod as FVM
 
-    def best(fn):
+    def best(fn, stats=False):
         fn()
         ts = []
         for _ in range(3):
             t0 = time.perf_counter()
-            fn()
+            sol = fn()
             ts.append(time.perf_counter() - t0)
-        return round(1e3 * min(ts), 2)
+        ms = round(1e3 * min(ts), 2)
+        if not stats:
+            return ms
+        keep = ("naccept", "nreject", "nnewton", "nkrylov", "device_loop", "jacobian_colours")
+        return ms, {k: v for k, v in getattr(sol, "stats", {}).items() if k in keep}
 
     out = []
     tri = FVM.triangulate_rectangle(0.0, 2.0, 0.0, 2.0, 50, 50, single_boundary=True)
@@ -340,9 +344,11 @@ def published_walkthroughs():
     plate_fvm = FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 0.0 * u, FVM.Dirichlet),
                                diffusion_function=lambda x, y, t, u, p: 1 / 9 + 0.0 * u,
                                initial_condition=np.where(tri.y <= 1.0, 50.0, 0.0), final_time=0.5)
-    out.append(dict(workload="the plate as an FVMProblem, TRBDF2 (ours: matrix-free Newton-GMRES on fvm_jvp; "
-                             "published: KLU on the sparse Jacobian), saveat 0.05",
-                    ms=best(lambda: FVM.solve(plate_fvm, FVM.TRBDF2(), saveat=0.05)), published_ms=49.237,
+    ms, st = best(lambda: FVM.solve(plate_fvm, FVM.TRBDF2(), saveat=0.05), stats=True)
+    out.append(dict(workload="the plate as an FVMProblem, TRBDF2 (ours: the affine problem's Jacobian from "
+                             "coloured fvm_jvp probes, then the whole integration with Newton-GMRES in one "
+                             "workgroup; published: KLU on the sparse Jacobian), saveat 0.05",
+                    ms=ms, published_ms=49.237, stats=st,
                     source="docs/src/literate_wyos/diffusion_equations.jl:336-340"))
     L = 320.0
     tri = FVM.triangulate_rectangle(0.0, L, 0.0, L, 100, 100, single_boundary=True)
@@ -358,9 +364,10 @@ def published_walkthroughs():
                              diffusion_function=lambda x, y, t, u, p: 2.0 + 0.0 * u,
                              initial_condition=np.where((0.4 * L <= tri.y) & (tri.y <= 0.6 * L), 1.0, 0.0),
                              final_time=500.0)
-    out.append(dict(workload="the same as an FVMProblem, stiff (ours: TRBDF2, matrix-free Newton-GMRES; "
+    ms, st = best(lambda: FVM.solve(big_fvm, FVM.TRBDF2(), saveat=100.0), stats=True)
+    out.append(dict(workload="the same as an FVMProblem, stiff (ours: TRBDF2 in one workgroup as above; "
                              "published: CVODE_BDF with GMRES), saveat 100",
-                    ms=best(lambda: FVM.solve(big_fvm, FVM.TRBDF2(), saveat=100.0)), published_ms=94.839,
+                    ms=ms, published_ms=94.839, stats=st,
                     source="docs/src/literate_wyos/diffusion_equations.jl:451-455"))
     tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 100, 100, single_boundary=True)
     mesh = FVM.FVMGeometry(tri)

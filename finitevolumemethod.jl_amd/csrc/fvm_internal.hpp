@@ -21,8 +21,10 @@
 // traffic is matched to the build that produced it (2: fix-up records in fix order;
 // 3: tile kernels' fluxes share LDS with the node coordinates; 4: also in the VJP;
 // 5: node data re-read after the barriers; 6: fluxes stored (Q3, Q1, Q2, Q3) per
-// triangle and node sums taken four incidences per round, in the VJP too).
-#define FVM_RHS_REV 6
+// triangle and node sums taken four incidences per round, in the VJP too; 7: the
+// four flux copies stored as rows of TILE + 1 doubles, so the stores are free of LDS
+// bank conflicts).
+#define FVM_RHS_REV 7
 #ifndef FVM_GEO_LEVEL
 #define FVM_GEO_LEVEL 4
 #endif

@@ -850,7 +850,10 @@ __global__ __launch_bounds__(kRT) void spmv_ell_kernel(int64_t n, const CI* __re
 // of its 512-workgroup grid, emulated here block by block; the step controller's
 // power is fvm_ctrl.hpp's, the host loops' own: the same states bit for bit.
 constexpr int kSmallThreads = 1024;
-constexpr int64_t kSmallTsit5Max = 65536;  // one workgroup pays off below this many unknowns
+// One workgroup pays off below this many unknowns: a step costs about 16 us + 17 ns
+// per unknown on one CU (2,500 unknowns: 60 us; 10,000: 190 us) against about 80 us
+// for the launches of the multi-kernel step on the whole chip.
+constexpr int64_t kSmallTsit5Max = 3600;
 constexpr int kRedBlocksE = 512, kRedThreadsE = 256;  // the grid whose summation order is reproduced
 
 struct SmallTsit5 {

@@ -312,14 +312,18 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
   __shared__ double s_D[FK == FVM_FLUX_D_NODAL ? MAXN : 1];
   constexpr bool RC = rebuilds(FK);  // part of the triangle record rebuilt from coordinates
   // the node coordinates are dead once the fluxes are formed: the fluxes reuse their
-  // space (fewer LDS bytes per workgroup, more workgroups per CU).  Each triangle's
-  // fluxes are stored as (Q3, Q1, Q2, Q3), so the incidence entry w = 4 lt + slot
-  // addresses its vertex's two terms directly: s_q[w] is the flux added and s_q[w + 1]
-  // the flux subtracted (i: Q3 - Q1, j: Q1 - Q2, k: Q2 - Q3).
-  // Two zeros after the fluxes serve the padding entries of the incidence rounds.
+  // space (fewer LDS bytes per workgroup, more workgroups per CU).  The fluxes are
+  // four rows of R = TILE + 1 doubles, (Q3, Q1, Q2, Q3) of triangle lt in column lt,
+  // so that a thread's four stores go to consecutive addresses across the wave (no
+  // bank conflicts), and the incidence entry of (lt, slot), restaged as
+  // w = slot R + lt, addresses its vertex's two terms directly: s_q[w] is the flux
+  // added and s_q[w + R] the flux subtracted (i: Q3 - Q1, j: Q1 - Q2, k: Q2 - Q3).
+  // Column TILE of rows 0 and 1 holds zeros for the padding entries of the rounds.
+  constexpr int R = TILE + 1;
   constexpr int XY = RC ? 2 * MAXN : 0;
-  static_assert(XY <= 4 * TILE, "the coordinates fit below the padding zeros");
-  __shared__ double s_xyq[4 * TILE + 2];
+  static_assert(XY <= 4 * R, "the coordinates fit in the flux rows");
+  static_assert(3 * R <= 65535, "restaged entries fit 16 bits");
+  __shared__ double s_xyq[4 * R];
   double* const s_x = s_xyq;
   double* const s_y = s_xyq + MAXN;
   double* const s_q = s_xyq;
@@ -421,15 +425,11 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
       s_is[l] = is[r];
     }
   }
-  if (tid == 0) {
-    s_is[nn] = 3 * nt;
-    s_q[4 * TILE] = 0.0;
-    s_q[4 * TILE + 1] = 0.0;
-  }
+  if (tid == 0) s_is[nn] = 3 * nt;
 #pragma unroll
   for (int k = 0; k < 3 * TPT; ++k) {
     const int idx = tid + k * BLOCK;
-    if (idx < 3 * nt) s_inc[idx] = iv[k];
+    if (idx < 3 * nt) s_inc[idx] = (uint16_t)((iv[k] & 3) * R + (iv[k] >> 2));
   }
   __syncthreads();
   double Q[TPT][3];
@@ -453,11 +453,15 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
   for (int r = 0; r < TPT; ++r) {
     const int lt = tid + r * BLOCK;
     if (lt < nt) {
-      s_q[4 * lt] = Q[r][2];
-      s_q[4 * lt + 1] = Q[r][0];
-      s_q[4 * lt + 2] = Q[r][1];
-      s_q[4 * lt + 3] = Q[r][2];
+      s_q[lt] = Q[r][2];
+      s_q[R + lt] = Q[r][0];
+      s_q[2 * R + lt] = Q[r][1];
+      s_q[3 * R + lt] = Q[r][2];
     }
+  }
+  if (tid == 0) {
+    s_q[TILE] = 0.0;
+    s_q[R + TILE] = 0.0;
   }
   __syncthreads();
 #pragma unroll
@@ -474,13 +478,13 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int e = s_inc[k + j];
-          w[j] = k + j < k1 ? e : 4 * TILE;
+          w[j] = k + j < k1 ? e : TILE;
         }
         double qa[4], qb[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           qa[j] = s_q[w[j]];
-          qb[j] = s_q[w[j] + 1];
+          qb[j] = s_q[w[j] + R];
         }
         // update_du! (triangle_contributions.jl:10-15): i += Q3 - Q1, j += Q1 - Q2, k += Q2 - Q3
 #pragma unroll
