@@ -294,6 +294,33 @@ int reduce(int kind, int64_t n, const double* x, const double* y, const double* 
 
 }  // namespace
 
+// Scratch of fvm_vec_mdot_noalloc: one partial per (vector, workgroup), kMaxDots
+// vectors (256.5 KiB per device, held until the process ends).  Host code reaches it
+// only through the address hipGetSymbolAddress returns on each device.
+static __device__ __align__(256) double scratch[(size_t)kMaxDots * kRedBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_na_mu;
+double* g_na_ptr[kMaxDevices] = {};  // scratch's device address, per device
+
+// The current device's copy of scratch, looked up once per device.
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  FVM_HIP(hipGetDevice(&dev));
+  FVM_CHECK(dev >= 0 && dev < kMaxDevices, "fvm_vec_mdot_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    FVM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FVM_CHECK(p, "fvm_vec_mdot_noalloc: no device address for the scratch array");
+    g_na_ptr[dev] = (double*)p;
+  }
+  *out = g_na_ptr[dev];
+  return FVM_OK;
+}
+}  // namespace
+
 namespace fvm {
 
 // Classical Gram-Schmidt applied twice to w against xs[0..k) and |w|^2, with every
@@ -435,6 +462,30 @@ int fvm_vec_mdot(int64_t n, int32_t k, const double* const* d_xs, const double* 
   FVM_HIP(hipGetLastError());
   FVM_HIP(hipMemcpyAsync(results, res, sizeof(double) * (size_t)k, hipMemcpyDeviceToHost, st));
   FVM_HIP(hipStreamSynchronize(st));
+  return FVM_OK;
+}
+
+int fvm_vec_mdot_noalloc(int64_t n, int32_t k, const double* const* d_xs, const double* d_y, double* d_results,
+                         fvm_stream s) {
+  FVM_CHECK(d_xs && d_y && d_results, "fvm_vec_mdot_noalloc: null argument");
+  FVM_CHECK(k >= 1 && k <= kMaxDots, "fvm_vec_mdot_noalloc: k must be 1..64 (the scratch holds 64 vectors' partials)");
+  FVM_CHECK(n >= 0, "fvm_vec_mdot_noalloc: negative length");
+  for (int j = 0; j < k; ++j) FVM_CHECK(d_xs[j], "fvm_vec_mdot_noalloc: null input vector");
+  double* buf;
+  int rc = noalloc_scratch(&buf);
+  if (rc) return rc;
+  // the same kernels, grid and summation order as fvm_vec_mdot: every partial is
+  // rewritten by this call, so nothing needs clearing
+  hipStream_t st = (hipStream_t)s;
+  for (int first = 0; first < k; first += kDotChunk) {
+    DotChunk c;
+    c.first = first;
+    c.k = k - first < kDotChunk ? k - first : kDotChunk;
+    for (int j = 0; j < kDotChunk; ++j) c.x[j] = j < c.k ? d_xs[first + j] : d_xs[first];
+    hipLaunchKernelGGL(mdot_stage1, dim3(kRedBlocks), dim3(kRedThreads), 0, st, n, c, d_y, buf);
+  }
+  hipLaunchKernelGGL(mdot_stage2, dim3((unsigned)k), dim3(kRedThreads), 0, st, buf, d_results);
+  FVM_HIP(hipGetLastError());
   return FVM_OK;
 }
 

@@ -230,6 +230,7 @@ HIP_API = {
     "fvm_vec_mul": [ctypes.c_int64, _vp, _vp, _vp, _vp],
     "fvm_vec_dot": [ctypes.c_int64, _vp, _vp, _dp, _vp],
     "fvm_vec_mdot": [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_vp), _vp, _dp, _vp],
+    "fvm_vec_mdot_noalloc": [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_vp), _vp, _vp, _vp],
     "fvm_vec_maxpy": [ctypes.c_int64, _vp, ctypes.c_int32, _dp, ctypes.POINTER(_vp), _vp],
     "fvm_trbdf2_affine": [_vp, _vp, _vp, _vp, _vp],
     "fvm_jit_check": [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_size_t)],

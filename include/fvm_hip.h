@@ -428,6 +428,16 @@ int fvm_vec_dot(int64_t n, const double* d_x, const double* d_y, double* result,
  * the Gram-Schmidt step of the Krylov solvers). */
 int fvm_vec_mdot(int64_t n, int32_t k, const double* const* d_xs, const double* d_y, double* results,
                  fvm_stream s);
+/* fvm_vec_mdot without allocating: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The k dot products (1 <= k <= 64, more is an
+ * error) go to the caller's device array d_results, equal bit for bit to
+ * fvm_vec_mdot's; the partial sums live in a static per-device array of the library
+ * (64 x 512 doubles = 256 KiB per GPU, held until the process ends, every partial
+ * rewritten by each call).  Asynchronous on s; calls that overlap on one GPU race
+ * on that array. */
+int fvm_vec_mdot_noalloc(int64_t n, int32_t k, const double* const* d_xs, const double* d_y, double* d_results,
+                         fvm_stream s);
 /* d_y += sum_j coefs[j] * d_xs[j], k <= 64 (no d_xs[j] may alias d_y). */
 int fvm_vec_maxpy(int64_t n, double* d_y, int32_t k, const double* coefs, const double* const* d_xs,
                   fvm_stream s);
