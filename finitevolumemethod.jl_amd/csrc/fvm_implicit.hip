@@ -58,87 +58,149 @@ enum { V_U, V_FN, V_Z, V_FG, V_U1, V_F1, V_R, V_G, V_DZ, V_EST, V_TMP, V_INTERP,
 
 struct Shared {
   double red[kWaves][kChunk];
-  double out[kChunk];
+  double sums[kChunk];                 // block_sums' results
+  double hp[2][kMaxRestart + 1];       // the two Gram-Schmidt passes' projections
   double H[kMaxRestart + 1][kMaxRestart];
   double g[kMaxRestart + 1], cs[kMaxRestart], sn[kMaxRestart], y[kMaxRestart];
 };
 
 __device__ __forceinline__ void bar() { __syncthreads(); }
 
-// sums of kChunk per-thread values over the workgroup, in a fixed order (waves by
-// shuffles, then the waves in order); every thread gets the results
-__device__ void block_sums(Shared& S, double* v, int k) {
+// Elementwise passes take kU elements per thread per round, every load of a round
+// issued before the first store; each element's arithmetic and each thread's
+// accumulation order (ascending i) are those of a plain loop over i.  One element per
+// round measured fastest (plate 17.6 ms against 18.7 ms with four): four per round
+// doubles the kernel's code to 115 KB, past the 64 KB instruction cache its one CU has.
+constexpr int kU = 1;
+constexpr int kUD = 1;  // the dot sweeps: kUD elements x kChunk basis vectors in flight
+
+template <class L, class St>
+__device__ __forceinline__ void vpass(int64_t n, L load, St store) {
+  for (int64_t i0 = threadIdx.x; i0 < n; i0 += kU * kT) {
+    double v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t i = i0 + u * kT;
+      v[u] = i < n ? load(i) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t i = i0 + u * kT;
+      if (i < n) store(i, v[u]);
+    }
+  }
+}
+
+// sums of k <= kChunk per-thread values over the workgroup, in a fixed order (waves
+// by shuffles, then the waves in order), left in dst[0..k) (LDS) for every thread to
+// read after the call
+__device__ __forceinline__ void block_sums(Shared& S, const double* v, int k, double* dst) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = 0; j < k; ++j) {
-    double x = v[j];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if (lane == 0) S.red[wave][j] = x;
+#pragma unroll
+  for (int j = 0; j < kChunk; ++j) {
+    if (j < k) {
+      double x = v[j];
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+      if (lane == 0) S.red[wave][j] = x;
+    }
   }
   bar();
   if ((int)threadIdx.x < k) {
     double x = 0.0;
     for (int w = 0; w < kWaves; ++w) x += S.red[w][threadIdx.x];
-    S.out[threadIdx.x] = x;
+    dst[threadIdx.x] = x;
   }
-  bar();
-  for (int j = 0; j < k; ++j) v[j] = S.out[j];
   bar();
 }
 
-__device__ double block_sum(Shared& S, double x) {
+__device__ __forceinline__ double block_sum(Shared& S, double x) {
   double v[kChunk] = {x};
-  block_sums(S, v, 1);
-  return v[0];
+  block_sums(S, v, 1, S.sums);
+  return S.sums[0];
+}
+
+// row i of J x (+ c when affine); W > 0 is the ELL width known at compile time,
+// W == 0 reads it from the operator
+template <int W>
+__device__ __forceinline__ double j_row(const Trb& a, const double* x, int64_t i, bool affine) {
+  const int64_t n = a.n;
+  double acc = 0.0;
+  if (W > 0) {
+    int c[W > 0 ? W : 1];
+    double v[W > 0 ? W : 1];
+#pragma unroll
+    for (int s = 0; s < W; ++s) {
+      c[s] = a.col[s * n + i];
+      v[s] = a.val[s * n + i];
+    }
+#pragma unroll
+    for (int s = 0; s < W; ++s) acc += v[s] * x[c[s]];
+  } else {
+    for (int s = 0; s < a.W; ++s) acc += a.val[s * n + i] * x[a.col[s * n + i]];
+  }
+  for (int k = a.orp[i]; k < a.orp[i + 1]; ++k) acc += a.oval[k] * x[a.ocol[k]];
+  acc += a.diag[i] * x[i];
+  return affine ? acc + a.c[i] : acc;
 }
 
 // y = J x (+ c when affine)
-__device__ void apply_J(const Trb& a, const double* x, double* y, bool affine) {
-  const int64_t n = a.n;
-  for (int64_t i = threadIdx.x; i < n; i += kT) {
-    double acc = 0.0;
-    for (int s = 0; s < a.W; ++s) acc += a.val[s * n + i] * x[a.col[s * n + i]];
-    for (int k = a.orp[i]; k < a.orp[i + 1]; ++k) acc += a.oval[k] * x[a.ocol[k]];
-    acc += a.diag[i] * x[i];
-    y[i] = affine ? acc + a.c[i] : acc;
-  }
+template <int W>
+__device__ __forceinline__ void apply_J(const Trb& a, const double* x, double* y, bool affine) {
+  vpass(a.n, [&](int64_t i) { return j_row<W>(a, x, i, affine); }, [&](int64_t i, double v) { y[i] = v; });
   bar();
 }
 
 // sqrt(mean((e / (abstol + reltol max(|p|, |q|)))^2)), the integrators' error norm
-__device__ double err_norm(Shared& S, const Trb& a, const double* e, const double* p, const double* q) {
+__device__ __forceinline__ double err_norm(Shared& S, const Trb& a, const double* e, const double* p, const double* q) {
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < a.n; i += kT) {
-    const double r = e[i] / (a.abstol + a.reltol * fmax(fabs(p[i]), fabs(q[i])));
-    acc += r * r;
-  }
+  vpass(
+      a.n,
+      [&](int64_t i) {
+        const double r = e[i] / (a.abstol + a.reltol * fmax(fabs(p[i]), fabs(q[i])));
+        return r * r;
+      },
+      [&](int64_t, double v) { acc += v; });
   return sqrt(block_sum(S, acc) / (double)a.n);
 }
 
 // x = W^-1 b approximately, W = I - dh J, restarted GMRES(m) from x = 0 (solve.py's
 // gmres_device: stop when |r| <= rtol |b|); returns the Arnoldi steps taken
-__device__ int gmres(Shared& S, const Trb& a, const double* b, double* x, double dh, int64_t maxiter) {
+template <int W>
+__device__ __forceinline__ int gmres(Shared& S, const Trb& a, const double* b, double* x, double dh, int64_t maxiter) {
   const int64_t n = a.n;
   const int m = a.restart;
   double* w = a.v[V_W];
   double* r = a.v[V_SPARE];
-  for (int64_t i = threadIdx.x; i < n; i += kT) x[i] = 0.0;
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kT) acc += b[i] * b[i];
+  vpass(n, [&](int64_t i) { return b[i]; }, [&](int64_t i, double v) {
+    x[i] = 0.0;
+    acc += v * v;
+  });
   const double bnorm = sqrt(block_sum(S, acc));
   if (bnorm == 0.0) return 0;
   int total = 0;
+  bool x_zero = true;
   while (total < maxiter) {
-    // r = b - W x
-    apply_J(a, x, w, false);
-    acc = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += kT) {
-      const double wx = x[i] - dh * w[i];
-      r[i] = b[i] - wx;
-      acc += r[i] * r[i];
+    // r = b - W x (r = b while x is still zero)
+    double beta = bnorm;
+    if (!x_zero) {
+      apply_J<W>(a, x, w, false);
+      acc = 0.0;
+      vpass(
+          n,
+          [&](int64_t i) {
+            const double wx = x[i] - dh * w[i];
+            return b[i] - wx;
+          },
+          [&](int64_t i, double v) {
+            r[i] = v;
+            acc += v * v;
+          });
+      beta = sqrt(block_sum(S, acc));
+      if (beta <= a.krtol * bnorm) break;
     }
-    const double beta = sqrt(block_sum(S, acc));
-    if (beta <= a.krtol * bnorm) break;
-    for (int64_t i = threadIdx.x; i < n; i += kT) a.basis[i] = r[i] / beta;
+    const double* rr = x_zero ? b : r;
+    vpass(n, [&](int64_t i) { return rr[i] / beta; }, [&](int64_t i, double v) { a.basis[i] = v; });
     if (threadIdx.x == 0) {
       for (int j = 0; j <= m; ++j) S.g[j] = 0.0;
       S.g[0] = beta;
@@ -147,40 +209,70 @@ __device__ int gmres(Shared& S, const Trb& a, const double* b, double* x, double
     int kd = 0;
     for (int j = 0; j < m; ++j) {
       const double* vj = a.basis + j * n;
-      apply_J(a, vj, w, false);
-      for (int64_t i = threadIdx.x; i < n; i += kT) w[i] = vj[i] - dh * w[i];
+      vpass(n, [&](int64_t i) { return vj[i] - dh * j_row<W>(a, vj, i, false); },
+            [&](int64_t i, double v) { w[i] = v; });
       bar();
       // classical Gram-Schmidt applied twice: each pass forms every dot product with
-      // the same w (kChunk per sweep), then subtracts them all in one sweep
-      double h[kMaxRestart + 1], hp[kMaxRestart + 1];
-      for (int q = 0; q <= j; ++q) h[q] = 0.0;
+      // the same w (kChunk per sweep, reduced into S.hp[pass]), then subtracts them
+      // all in one sweep
       for (int pass = 0; pass < 2; ++pass) {
+        double* hp = S.hp[pass];
         for (int q0 = 0; q0 <= j; q0 += kChunk) {
           const int k = j + 1 - q0 < kChunk ? j + 1 - q0 : kChunk;
           double dd[kChunk];
-          for (int q = 0; q < k; ++q) dd[q] = 0.0;
-          for (int64_t i = threadIdx.x; i < n; i += kT) {
-            const double wi = w[i];
-            for (int q = 0; q < k; ++q) dd[q] += a.basis[(q0 + q) * n + i] * wi;
+#pragma unroll
+          for (int q = 0; q < kChunk; ++q) dd[q] = 0.0;
+          for (int64_t i0 = threadIdx.x; i0 < n; i0 += kUD * kT) {
+            double wv[kUD], bv[kUD][kChunk];
+#pragma unroll
+            for (int u = 0; u < kUD; ++u) {
+              const int64_t i = i0 + u * kT;
+              wv[u] = i < n ? w[i] : 0.0;
+#pragma unroll
+              for (int q = 0; q < kChunk; ++q) bv[u][q] = (q < k && i < n) ? a.basis[(q0 + q) * n + i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kUD; ++u) {
+              if (i0 + u * kT < n) {
+#pragma unroll
+                for (int q = 0; q < kChunk; ++q)
+                  if (q < k) dd[q] += bv[u][q] * wv[u];
+              }
+            }
           }
-          block_sums(S, dd, k);
-          for (int q = 0; q < k; ++q) hp[q0 + q] = dd[q];
+          block_sums(S, dd, k, hp + q0);
         }
-        for (int64_t i = threadIdx.x; i < n; i += kT) {
-          double wi = w[i];
-          for (int q = 0; q <= j; ++q) wi = wi - hp[q] * a.basis[q * n + i];
-          w[i] = wi;
+        for (int64_t i0 = threadIdx.x; i0 < n; i0 += kU * kT) {
+          double wv[kU];
+#pragma unroll
+          for (int u = 0; u < kU; ++u) {
+            const int64_t i = i0 + u * kT;
+            wv[u] = i < n ? w[i] : 0.0;
+          }
+          for (int q = 0; q <= j; ++q) {
+            const double h = hp[q];
+            const double* bq = a.basis + q * n;
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+              const int64_t i = i0 + u * kT;
+              if (i < n) wv[u] = wv[u] - h * bq[i];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kU; ++u) {
+            const int64_t i = i0 + u * kT;
+            if (i < n) w[i] = wv[u];
+          }
         }
-        for (int q = 0; q <= j; ++q) h[q] += hp[q];
         bar();
       }
       acc = 0.0;
-      for (int64_t i = threadIdx.x; i < n; i += kT) acc += w[i] * w[i];
+      vpass(n, [&](int64_t i) { return w[i]; }, [&](int64_t, double v) { acc += v * v; });
       const double hn = sqrt(block_sum(S, acc));
       if (hn > 0.0)
-        for (int64_t i = threadIdx.x; i < n; i += kT) a.basis[(j + 1) * n + i] = w[i] / hn;
+        vpass(n, [&](int64_t i) { return w[i] / hn; }, [&](int64_t i, double v) { a.basis[(j + 1) * n + i] = v; });
       if (threadIdx.x == 0) {
-        for (int q = 0; q <= j; ++q) S.H[q][j] = h[q];
+        for (int q = 0; q <= j; ++q) S.H[q][j] = S.hp[0][q] + S.hp[1][q];
         S.H[j + 1][j] = hn;
         for (int q = 0; q < j; ++q) {  // previous rotations
           const double p = S.H[q][j], o = S.H[q + 1][j];
@@ -208,12 +300,30 @@ __device__ int gmres(Shared& S, const Trb& a, const double* b, double* x, double
       }
     }
     bar();
-    for (int64_t i = threadIdx.x; i < n; i += kT) {
-      double xi = x[i];
-      for (int q = 0; q < kd; ++q) xi = xi + S.y[q] * a.basis[q * n + i];
-      x[i] = xi;
+    for (int64_t i0 = threadIdx.x; i0 < n; i0 += kU * kT) {
+      double xv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t i = i0 + u * kT;
+        xv[u] = i < n ? x[i] : 0.0;
+      }
+      for (int q = 0; q < kd; ++q) {
+        const double yq = S.y[q];
+        const double* bq = a.basis + q * n;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int64_t i = i0 + u * kT;
+          if (i < n) xv[u] = xv[u] + yq * bq[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t i = i0 + u * kT;
+        if (i < n) x[i] = xv[u];
+      }
     }
     bar();
+    x_zero = false;
     if (fabs(S.g[kd]) <= a.krtol * bnorm) break;
   }
   return total;
@@ -221,10 +331,11 @@ __device__ int gmres(Shared& S, const Trb& a, const double* b, double* x, double
 
 __device__ __forceinline__ void lincomb2(const Trb& a, double* out, double p, const double* x, double q,
                                          const double* y) {
-  for (int64_t i = threadIdx.x; i < a.n; i += kT) out[i] = p * x[i] + q * y[i];
+  vpass(a.n, [&](int64_t i) { return p * x[i] + q * y[i]; }, [&](int64_t i, double v) { out[i] = v; });
   bar();
 }
 
+template <int W>
 __global__ __launch_bounds__(kT) void trbdf2_affine_kernel(Trb a) {
   __shared__ Shared S;
   const int64_t n = a.n;
@@ -236,7 +347,7 @@ __global__ __launch_bounds__(kT) void trbdf2_affine_kernel(Trb a) {
   int64_t nf = 0, njvp = 0, naccept = 0, nreject = 0, nnewton = 0, nkrylov = 0, nfail = 0, ks = 0, it = 0;
   int retcode = 0;  // FVM_ODE_SUCCESS
   double t = a.t0, dt = a.dt, eta_keep = 1.0;
-  apply_J(a, u, fn, true);
+  apply_J<W>(a, u, fn, true);
   ++nf;
   // Newton on z - d h f(z) = rhs from the predictor in z; fz <- f(z); false on failure
   auto newton = [&](double* zz, const double* rhs, double h, double* fz) -> bool {
@@ -244,14 +355,14 @@ __global__ __launch_bounds__(kT) void trbdf2_affine_kernel(Trb a) {
     double eta = fvm::ctrl_pow(fmax(eta_keep, 1e-16), 0.8), prev = -1.0;
     bool ok = false;
     for (int itn = 0; itn < a.max_newton; ++itn) {
-      apply_J(a, zz, fz, true);
+      apply_J<W>(a, zz, fz, true);
       ++nf;
-      for (int64_t i = threadIdx.x; i < n; i += kT) G[i] = -1.0 * zz[i] + dh * fz[i] + rhs[i];
+      vpass(n, [&](int64_t i) { return -1.0 * zz[i] + dh * fz[i] + rhs[i]; }, [&](int64_t i, double v) { G[i] = v; });
       bar();
-      const int ki = gmres(S, a, G, dz, dh, 20 * (int64_t)a.restart);
+      const int ki = gmres<W>(S, a, G, dz, dh, 20 * (int64_t)a.restart);
       nkrylov += ki;
       njvp += ki + 1;
-      for (int64_t i = threadIdx.x; i < n; i += kT) zz[i] = zz[i] + dz[i];
+      vpass(n, [&](int64_t i) { return zz[i] + dz[i]; }, [&](int64_t i, double v) { zz[i] = v; });
       bar();
       ++nnewton;
       const double ndz = err_norm(S, a, dz, u, zz);
@@ -269,7 +380,7 @@ __global__ __launch_bounds__(kT) void trbdf2_affine_kernel(Trb a) {
     }
     if (!ok) return false;
     eta_keep = eta;
-    apply_J(a, zz, fz, true);
+    apply_J<W>(a, zz, fz, true);
     ++nf;
     return true;
   };
@@ -298,17 +409,19 @@ __global__ __launch_bounds__(kT) void trbdf2_affine_kernel(Trb a) {
     }
     {
       const double c0 = TK * h / kG, c1 = -TK * h / (kG * (1 - kG)), c2 = TK * h / (1 - kG);
-      for (int64_t i = threadIdx.x; i < n; i += kT) {
-        double v = c0 * fn[i];
-        v = v + c1 * fg[i];
-        v = v + c2 * f1[i];
-        est[i] = v;
-      }
+      vpass(
+          n,
+          [&](int64_t i) {
+            double v = c0 * fn[i];
+            v = v + c1 * fg[i];
+            return v + c2 * f1[i];
+          },
+          [&](int64_t i, double v) { est[i] = v; });
       bar();
     }
     const double* e = est;
     if (a.smooth) {
-      const int ki = gmres(S, a, est, tmp, d * h, 20 * (int64_t)a.restart);
+      const int ki = gmres<W>(S, a, est, tmp, d * h, 20 * (int64_t)a.restart);
       nkrylov += ki;
       njvp += ki + 1;
       e = tmp;
@@ -326,27 +439,29 @@ __global__ __launch_bounds__(kT) void trbdf2_affine_kernel(Trb a) {
         const double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + th, h01 = -2 * t3 + 3 * t2, h11 = t3 - t2;
         const double q0 = h00, q1 = h10 * h, q2 = h01, q3 = h11 * h;
         double* out = a.saves + ks * n;
-        for (int64_t i = threadIdx.x; i < n; i += kT) {
-          double v = q0 * u[i];
-          v = v + q1 * fn[i];
-          v = v + q2 * u1[i];
-          v = v + q3 * f1[i];
-          out[i] = v;
-        }
+        vpass(
+            n,
+            [&](int64_t i) {
+              double v = q0 * u[i];
+              v = v + q1 * fn[i];
+              v = v + q2 * u1[i];
+              return v + q3 * f1[i];
+            },
+            [&](int64_t i, double v) { out[i] = v; });
         ++ks;
       }
       bar();
-      for (int64_t i = threadIdx.x; i < n; i += kT) u[i] = u1[i];
+      vpass(n, [&](int64_t i) { return u1[i]; }, [&](int64_t i, double v) { u[i] = v; });
       bar();
       t = tn;
       ++naccept;
       if (a.n_dir > 0) {  // the Dirichlet callback (values constant), then f(u) afresh
         for (int k = threadIdx.x; k < a.n_dir; k += kT) u[a.dir_idx[k]] = a.dir_val[k];
         bar();
-        apply_J(a, u, fn, true);
+        apply_J<W>(a, u, fn, true);
         ++nf;
       } else {
-        for (int64_t i = threadIdx.x; i < n; i += kT) fn[i] = f1[i];
+        vpass(n, [&](int64_t i) { return f1[i]; }, [&](int64_t i, double v) { fn[i] = v; });
         bar();
       }
     } else {
@@ -429,7 +544,18 @@ extern "C" int fvm_trbdf2_affine(const fvm_affine_op* op, const fvm_trbdf2_opts*
   if (e == hipSuccess && o->n_save > 0)
     e = hipMemcpyAsync(tail, o->save_t, sizeof(double) * (size_t)o->n_save, hipMemcpyHostToDevice, st_);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(trbdf2_affine_kernel, dim3(1), dim3(kT), 0, st_, a);
+    // the ELL widths of triangle meshes as compile-time constants; others at run time
+    switch (a.W) {
+#define FVM_TRB_W(w)                                                               \
+  case w:                                                                          \
+    hipLaunchKernelGGL(trbdf2_affine_kernel<w>, dim3(1), dim3(kT), 0, st_, a); \
+    break;
+      FVM_TRB_W(4) FVM_TRB_W(5) FVM_TRB_W(6) FVM_TRB_W(7) FVM_TRB_W(8) FVM_TRB_W(9) FVM_TRB_W(10)
+      FVM_TRB_W(11) FVM_TRB_W(12)
+#undef FVM_TRB_W
+      default:
+        hipLaunchKernelGGL(trbdf2_affine_kernel<0>, dim3(1), dim3(kT), 0, st_, a);
+    }
     e = hipGetLastError();
   }
   double r[10] = {0};

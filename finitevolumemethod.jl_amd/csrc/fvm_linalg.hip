@@ -902,6 +902,25 @@ __device__ __forceinline__ double ell_row_apply(const SmallTsit5& a, int64_t i, 
   return acc + a.b[i];
 }
 
+// The pairwise tree of a 256-thread reduction, sh[t] += sh[t + s] for s = 128, ..., 1,
+// for each group of 256 threads of the workgroup at once: the two levels that pair
+// waves go through LDS, the six inside a wave through shuffles (lane l adds lane
+// l + s, the same pairs in the same order).  Thread 0 of each group returns its sum.
+__device__ __forceinline__ double tree256(double* sh, double x) {
+  const int tid = threadIdx.x, lt = tid % kRedThreadsE;
+  sh[tid] = x;
+  __syncthreads();
+  if (lt < 128) sh[tid] = sh[tid] + sh[tid + 128];
+  __syncthreads();
+  double v = 0.0;
+  if (lt < 64) {
+    v = sh[tid] + sh[tid + 64];
+    for (int s = 32; s > 0; s >>= 1) v += __shfl_down(v, s, 64);
+  }
+  __syncthreads();  // sh is free again
+  return v;
+}
+
 template <int W, bool OFF>
 __global__ __launch_bounds__(kSmallThreads) void tsit5_small_kernel(SmallTsit5 a) {
   __shared__ double sh[kSmallThreads];
@@ -950,27 +969,17 @@ __global__ __launch_bounds__(kSmallThreads) void tsit5_small_kernel(SmallTsit5 a
         const double r = v / sc;
         acc += r * r;
       }
-      sh[tid] = acc;
-      __syncthreads();
-      for (int s = kRedThreadsE / 2; s > 0; s >>= 1) {
-        if (lt < s) sh[tid] += sh[tid + s];
-        __syncthreads();
-      }
-      if (lt == 0 && blk < nblk) part[blk] = sh[tid];
-      __syncthreads();
+      acc = tree256(sh, acc);
+      if (lt == 0 && blk < nblk) part[blk] = acc;
     }
     for (int p = nblk + tid; p < kRedBlocksE; p += kSmallThreads) part[p] = 0.0;
     __syncthreads();
-    if (tid < kRedThreadsE) {
-      double acc = 0.0;
+    double acc = 0.0;
+    if (tid < kRedThreadsE)
       for (int i = tid; i < kRedBlocksE; i += kRedThreadsE) acc += part[i];
-      sh[tid] = acc;
-    }
+    acc = tree256(sh, acc);
+    if (tid == 0) sh[0] = acc;
     __syncthreads();
-    for (int s = kRedThreadsE / 2; s > 0; s >>= 1) {
-      if (tid < s) sh[tid] += sh[tid + s];
-      __syncthreads();
-    }
     const double EEst = sqrt(sh[0] / (double)n);
     __syncthreads();  // every thread has read sh[0]
     if (!isfinite(EEst)) {

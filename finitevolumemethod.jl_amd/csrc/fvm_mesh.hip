@@ -205,6 +205,11 @@ int build(const fvm_mesh_desc* d, const int32_t* tri_group, fvm_mesh* m) {
   m->tile_tris = tile_tris;
   m->max_nodes = max_nodes;
   const int64_t N = m->N, T = m->T, B = m->B;
+  // the RHS tile kernel addresses these arrays with 32-bit byte offsets
+  constexpr int64_t kOff = (int64_t)1 << 32;
+  FVM_CHECK(8 * N < kOff && 8 * (int64_t)m->n_local < kOff && 6 * T < kOff,
+            "fvm_mesh_create: mesh too large (the node and tile-node arrays must stay below 4 GiB: "
+            "at most 536,870,911 points and tile-node slots)");
   int rc;
 #define UP(dst, src, n) \
   if ((rc = upload(&(dst), src, n)) != FVM_OK) return rc

@@ -298,6 +298,20 @@ __device__ __forceinline__ double finish_pre(const KArgs& a, int g, double acc, 
   return 0.0;
 }
 
+// Element i of an array the whole launch shares, addressed by a 32-bit byte offset:
+// the access takes its base from scalar registers and the offset from one VGPR, with
+// no 64-bit address arithmetic per lane.  fvm_mesh_create keeps the node, tile-node
+// and incidence arrays below 4 GiB.
+template <class T>
+__device__ __forceinline__ T ld32(const T* base, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T* base, uint32_t i, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
+}
+__device__ __forceinline__ uint32_t umin(uint32_t x, uint32_t y) { return x < y ? x : y; }
+
 // One workgroup per tile.  All global loads of the tile (node ids, incidence
 // block, partial slots, the triangle records, then the per-node gathers that
 // depend on the node ids) are issued before the first wait, so the workgroup
@@ -348,35 +362,30 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
   const int ib0 = 3 * t0;  // a tile's incidence block is [3 t0, 3 (t0 + nt))
   const double dt = MODE == M_EULER ? a.tile_dt[tile] : 0.0;
 
+  // Every lane loads: lanes past the tile's nodes, entries or triangles read the last
+  // valid one (their values are never used), so the loads need no branches, and each
+  // takes a 32-bit offset from a base in scalar registers.
   int g[NS], pi[NS], is[NS];
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
-    const int l = tid + r * BLOCK;
-    g[r] = 0;
-    pi[r] = -1;
-    is[r] = 0;
-    if (l < nn) {
-      g[r] = a.tile_nodes[n0 + l];
-      pi[r] = a.lpart[n0 + l];
-      is[r] = a.inc_start[n0 + l] - ib0;
-    }
+    const uint32_t li = (uint32_t)n0 + umin((uint32_t)(tid + r * BLOCK), (uint32_t)nn - 1u);
+    g[r] = ld32(a.tile_nodes, li);
+    pi[r] = ld32(a.lpart, li);
+    is[r] = ld32(a.inc_start, li);
   }
   uint16_t iv[3 * TPT];
 #pragma unroll
-  for (int k = 0; k < 3 * TPT; ++k) {
-    const int idx = tid + k * BLOCK;
-    iv[k] = idx < 3 * nt ? a.inc[ib0 + idx] : (uint16_t)0;
-  }
+  for (int k = 0; k < 3 * TPT; ++k)
+    iv[k] = ld32(a.inc, (uint32_t)ib0 + umin((uint32_t)(tid + k * BLOCK), 3u * (uint32_t)nt - 1u));
   double P[TPT][P_COUNT];
   double Ds[TPT][FK == FVM_FLUX_LINEAR ? 9 : 3];
   uint32_t L[TPT];
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
     const int lt = tid + r * BLOCK;
-    L[r] = 0;
+    L[r] = ld32(a.loc, (uint32_t)t0 + umin((uint32_t)lt, (uint32_t)nt - 1u));
     if (lt < nt) {
       const int64_t s = t0 + lt;
-      L[r] = a.loc[s];
 #pragma unroll
       for (int f = 0; f < P_COUNT; ++f)
         if (loads_field(FK, f)) P[r][f] = __builtin_nontemporal_load(a.props + f * a.T + s);
@@ -390,26 +399,15 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
       }
     }
   }
-  double ug[NS], vg[NS], Vg[NS], lg[NS], xg[NS], yg[NS];
-  int cg[NS];
+  double ug[NS], vg[NS], lg[NS], xg[NS], yg[NS];
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
-    const int l = tid + r * BLOCK;
-    ug[r] = vg[r] = Vg[r] = lg[r] = xg[r] = yg[r] = 0.0;
-    cg[r] = 3;
-    if (l < nn) {
-      ug[r] = a.u[g[r]];
-      if (MODE == M_JVP) vg[r] = a.v[g[r]];
-      if (FK == FVM_FLUX_D_NODAL) lg[r] = a.log_d[g[r]];
-      if (RC) {
-        xg[r] = a.x[g[r]];
-        yg[r] = a.y[g[r]];
-      }
-      if (pi[r] < 0) {
-        cg[r] = -1 - pi[r];
-        Vg[r] = a.lcv[n0 + l];
-      }
-    }
+    const uint32_t gi = (uint32_t)g[r];
+    ug[r] = ld32(a.u, gi);
+    vg[r] = MODE == M_JVP ? ld32(a.v, gi) : 0.0;
+    lg[r] = FK == FVM_FLUX_D_NODAL ? ld32(a.log_d, gi) : 0.0;
+    xg[r] = RC ? ld32(a.x, gi) : 0.0;
+    yg[r] = RC ? ld32(a.y, gi) : 0.0;
   }
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
@@ -422,7 +420,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
         s_x[l] = xg[r];
         s_y[l] = yg[r];
       }
-      s_is[l] = is[r];
+      s_is[l] = is[r] - ib0;
     }
   }
   if (tid == 0) s_is[nn] = 3 * nt;
@@ -493,10 +491,11 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
       // the node's u (and v) are re-read from LDS and its volume from memory here rather
       // than held in registers across the barriers: fewer VGPRs, more waves per SIMD
       if (pi[r] >= 0)
-        a.partial[pi[r]] = acc;
+        st32(a.partial, (uint32_t)pi[r], acc);
       else
-        a.out[g[r]] = finish_pre<FK, MODE>(a, g[r], acc, s_u[l], MODE == M_JVP ? s_v[l] : 0.0, -1 - pi[r],
-                                       a.lcv[n0 + l], dt);
+        st32(a.out, (uint32_t)g[r],
+             finish_pre<FK, MODE>(a, g[r], acc, s_u[l], MODE == M_JVP ? s_v[l] : 0.0, -1 - pi[r],
+                                  ld32(a.lcv, (uint32_t)(n0 + l)), dt));
     }
   }
 }
