@@ -88,8 +88,11 @@ def copy_bandwidth(n=1 << 27, reps=20):
     return 2 * 8 * n * reps / (ms * 1e-3)
 
 
-def vjp_ratio(n_side, reps=20):
-    """Time fvm_rhs and fvm_vjp (u and log D) on the nodal-log-D workload."""
+def vjp_ratio(n_side, reps=20, cold_reps=10):
+    """Time fvm_rhs and fvm_vjp (u and log D) on the nodal-log-D workload: warm
+    (back-to-back calls, the ~0.45 GB working set partly resident in the 256 MB MALL)
+    and cold (each call timed alone after writing a 1 GiB buffer, which evicts the MALL
+    and the L2s)."""
     from finitevolumemethod import _lib
     from finitevolumemethod import workloads as WL
 
@@ -109,8 +112,23 @@ def vjp_ratio(n_side, reps=20):
     for _ in range(reps):
         ctx.vjp(d_u, d_w, d_ub, d_lb)
     t_vjp = tm.stop() / reps
+    flush = _lib.DeviceArray(1 << 27)
+
+    def cold(fn):
+        ts = []
+        for _ in range(cold_reps):
+            flush.fill_zero()
+            tm.start()
+            fn()
+            ts.append(tm.stop())
+        return float(np.mean(ts))
+
+    c_rhs = cold(lambda: ctx.rhs(d_u, d_du))
+    c_vjp = cold(lambda: ctx.vjp(d_u, d_w, d_ub, d_lb))
+    del flush
     return dict(nodes=int(tri.num_points()), triangles=int(len(tri.triangles)),
-                rhs_ms=round(t_rhs, 4), vjp_ms=round(t_vjp, 4), ratio=round(t_vjp / t_rhs, 3))
+                rhs_ms=round(t_rhs, 4), vjp_ms=round(t_vjp, 4), ratio=round(t_vjp / t_rhs, 3),
+                cold_rhs_ms=round(c_rhs, 4), cold_vjp_ms=round(c_vjp, 4), cold_ratio=round(c_vjp / c_rhs, 3))
 
 
 def fit_bench(n_side, iters=50, tf=0.1, nsteps=10):
@@ -645,6 +663,15 @@ def main():
                 "hbm": {"achieved": round(traffic / t_s / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                         "frac": round(traffic / t_s / HBM_PEAK, 4),
                         "frac_of_copy_kernel": round(traffic / t_s / copy_gbps, 4)}}
+        # the dominant kernel alone, over its own HIP-event duration
+        tk = cnt["tile_kernel"]
+        t_tk = ms_tiles * 1e-3
+        roof["tile_kernel"] = {
+            "ms": round(ms_tiles, 5), "valu_insts": int(tk["valu_insts"]),
+            "achieved": round(tk["valu_insts"] * 128 / t_tk / 1e12, 3), "unit": "TFLOP/s",
+            "frac": round(tk["valu_insts"] * 128 / t_tk / VALU_PEAK, 4),
+            "hbm_GBps": round(tk["hbm_bytes"] / t_tk / 1e9, 1), "hbm_frac": round(tk["hbm_bytes"] / t_tk / HBM_PEAK, 4),
+            "valu_busy_rocprof": tk["valu_busy"]}
     else:
         traffic = None
         roof = {"bound": "valu", "achieved": None, "peak": VALU_PEAK / 1e12, "unit": "TFLOP/s", "frac": None,
