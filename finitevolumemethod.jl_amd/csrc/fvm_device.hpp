@@ -14,6 +14,20 @@ namespace fvm {
 // over them, each XCD with its own L2.
 constexpr int kXCD = 8;
 
+// Element i of an array the whole launch shares, addressed by a 32-bit byte offset:
+// the access takes its base from scalar registers and the offset from one VGPR, with
+// no 64-bit address arithmetic per lane.  fvm_mesh_create (and fvm_system_create for
+// a system's state) keep the arrays the tile kernels address this way below 4 GiB.
+template <class T>
+__device__ __forceinline__ T ld32(const T* base, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T* base, uint32_t i, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
+}
+__device__ __forceinline__ uint32_t umin(uint32_t x, uint32_t y) { return x < y ? x : y; }
+
 // Maps workgroup id to a work index so that each XCD processes one contiguous
 // range of work items (neighbouring tiles share nodes, hence L2 lines).  The
 // grid must hold kXCD * ceil(n / kXCD) workgroups; ids >= n are idle.

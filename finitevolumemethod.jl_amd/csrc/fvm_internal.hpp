@@ -23,8 +23,9 @@
 // 5: node data re-read after the barriers; 6: fluxes stored (Q3, Q1, Q2, Q3) per
 // triangle and node sums taken four incidences per round, in the VJP too; 7: the
 // four flux copies stored as rows of TILE + 1 doubles, so the stores are free of LDS
-// bank conflicts; 8: the tile kernel's loads branch-free, at 32-bit offsets).
-#define FVM_RHS_REV 8
+// bank conflicts; 8: the tile kernel's loads branch-free, at 32-bit offsets; 9: the
+// VJP tile kernel's too, with its incidence block staged in LDS).
+#define FVM_RHS_REV 9
 #ifndef FVM_GEO_LEVEL
 #define FVM_GEO_LEVEL 4
 #endif

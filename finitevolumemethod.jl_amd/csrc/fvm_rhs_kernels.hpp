@@ -298,20 +298,6 @@ __device__ __forceinline__ double finish_pre(const KArgs& a, int g, double acc, 
   return 0.0;
 }
 
-// Element i of an array the whole launch shares, addressed by a 32-bit byte offset:
-// the access takes its base from scalar registers and the offset from one VGPR, with
-// no 64-bit address arithmetic per lane.  fvm_mesh_create keeps the node, tile-node
-// and incidence arrays below 4 GiB.
-template <class T>
-__device__ __forceinline__ T ld32(const T* base, uint32_t i) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
-}
-template <class T>
-__device__ __forceinline__ void st32(T* base, uint32_t i, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T)) = v;
-}
-__device__ __forceinline__ uint32_t umin(uint32_t x, uint32_t y) { return x < y ? x : y; }
-
 // One workgroup per tile.  All global loads of the tile (node ids, incidence
 // block, partial slots, the triangle records, then the per-node gathers that
 // depend on the node ids) are issued before the first wait, so the workgroup
@@ -619,20 +605,24 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
   constexpr int TILE = BLOCK * TPT;
   constexpr int MAXN = node_cap(BLOCK);
   constexpr int CW = FK == FVM_FLUX_D_NODAL ? 6 : 3;  // u-adjoint (+ log D adjoint) per vertex
-  __shared__ double s_u[MAXN];
   constexpr bool RC = rebuilds(FK);  // part of the triangle record rebuilt from coordinates
-  // w~, D, x and y are dead once the per-vertex adjoints are formed: the adjoints
-  // reuse their space (fewer LDS bytes per workgroup, more workgroups per CU)
+  // w~, D, x, y and u are dead once the per-vertex adjoints are formed: the adjoints
+  // reuse their space (fewer LDS bytes per workgroup, more workgroups per CU; the
+  // node rule reads u again from memory)
   constexpr int ND = FK == FVM_FLUX_D_NODAL ? MAXN : 0;
-  constexpr int PRE = MAXN + ND + (RC ? 2 * MAXN : 0);
+  constexpr int PRE = 2 * MAXN + ND + (RC ? 2 * MAXN : 0);
   // + CW zeros after the per-vertex adjoints: the target of the padding entries of
   // the incidence rounds
   __shared__ double s_big[PRE > CW * TILE + CW ? PRE : CW * TILE + CW];
   double* const s_w = s_big;
-  double* const s_D = s_big + MAXN;
-  double* const s_x = s_big + MAXN + ND;
+  double* const s_u = s_big + MAXN;
+  double* const s_D = s_big + 2 * MAXN;
+  double* const s_x = s_big + 2 * MAXN + ND;
   double* const s_y = s_x + MAXN;
   double* const s_c = s_big;
+  __shared__ uint16_t s_inc[3 * TILE];
+  __shared__ int s_is[MAXN + 1];
+  constexpr int NS = MAXN / BLOCK;
   const int nbw = (a.n_bitems + BLOCK - 1) / BLOCK;
   const int wid = xcd_remap(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
@@ -652,16 +642,30 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
   if (tile >= a.n_tiles) return;
   const int t0 = a.tile_tri_start[tile], nt = a.tile_tri_start[tile + 1] - t0;
   const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
+  const int ib0 = 3 * t0;  // the tile's incidence block is [3 t0, 3 (t0 + nt))
+  // every load of the tile issued before the first wait, branch-free (lanes past the
+  // tile's nodes, entries or triangles read the last valid one and never use it), at
+  // 32-bit offsets from bases in scalar registers
+  int g[NS], is[NS];
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const uint32_t li = (uint32_t)n0 + umin((uint32_t)(tid + r * BLOCK), (uint32_t)nn - 1u);
+    g[r] = ld32(a.tile_nodes, li);
+    is[r] = ld32(a.inc_start, li);
+  }
+  uint16_t iv[3 * TPT];
+#pragma unroll
+  for (int k = 0; k < 3 * TPT; ++k)
+    iv[k] = ld32(a.inc, (uint32_t)ib0 + umin((uint32_t)(tid + k * BLOCK), 3u * (uint32_t)nt - 1u));
   double P[TPT][P_COUNT];
   double Ds[TPT][FK == FVM_FLUX_LINEAR ? 9 : 3];
   uint32_t L[TPT];
 #pragma unroll
   for (int r = 0; r < TPT; ++r) {
     const int lt = tid + r * BLOCK;
-    L[r] = 0;
+    L[r] = ld32(a.loc, (uint32_t)t0 + umin((uint32_t)lt, (uint32_t)nt - 1u));
     if (lt < nt) {
       const int64_t s = t0 + lt;
-      L[r] = a.loc[s];
 #pragma unroll
       for (int f = 0; f < P_COUNT; ++f)
         if (loads_field(FK, f)) P[r][f] = __builtin_nontemporal_load(a.props + f * a.T + s);
@@ -675,16 +679,37 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
       }
     }
   }
-  for (int l = tid; l < nn; l += BLOCK) {
-    const int g = a.tile_nodes[n0 + l];
-    s_u[l] = a.u[g];
-    s_w[l] = wt_of(a, g);
-    if (FK == FVM_FLUX_D_NODAL) s_D[l] = exp(a.log_d[g]);
-    if (RC) {
-      s_x[l] = a.x[g];
-      s_y[l] = a.y[g];
+  double ug[NS], vg[NS], cvg[NS], lg[NS], xg[NS], yg[NS];
+  int8_t cg[NS];
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const uint32_t gi = (uint32_t)g[r];
+    ug[r] = ld32(a.u, gi);
+    vg[r] = ld32(a.v, gi);
+    cvg[r] = ld32(a.cv, gi);
+    cg[r] = ld32(a.code, gi);
+    lg[r] = FK == FVM_FLUX_D_NODAL ? ld32(a.log_d, gi) : 0.0;
+    xg[r] = RC ? ld32(a.x, gi) : 0.0;
+    yg[r] = RC ? ld32(a.y, gi) : 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const int l = tid + r * BLOCK;
+    if (l < nn) {
+      s_u[l] = ug[r];
+      s_w[l] = cg[r] == 0 ? vg[r] / cvg[r] : 0.0;  // wt_of
+      if (FK == FVM_FLUX_D_NODAL) s_D[l] = exp(lg[r]);
+      if (RC) {
+        s_x[l] = xg[r];
+        s_y[l] = yg[r];
+      }
+      s_is[l] = is[r] - ib0;
     }
   }
+  if (tid == 0) s_is[nn] = 3 * nt;
+#pragma unroll
+  for (int k = 0; k < 3 * TPT; ++k)
+    if (tid + k * BLOCK < 3 * nt) s_inc[tid + k * BLOCK] = iv[k];
   __syncthreads();
   double cr[TPT][CW];
 #pragma unroll
@@ -755,17 +780,20 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
   }
   if (tid < CW) s_c[CW * TILE + tid] = 0.0;
   __syncthreads();
-  for (int l = tid; l < nn; l += BLOCK) {
+#pragma unroll
+  for (int r = 0; r < NS; ++r) {
+    const int l = tid + r * BLOCK;
+    if (l >= nn) continue;
     double acc = 0.0, accl = 0.0;
-    const int k0 = a.inc_start[n0 + l], k1 = a.inc_start[n0 + l + 1];
-    // four incidences per round: their entries (loads clamped into the node's list),
+    const int k1 = s_is[l + 1];
+    // four incidences per round: their entries (reads clamped into the node's list),
     // then the sums in list order; entries past the list address the zeros, whose
     // sum leaves the accumulators unchanged bit for bit
-    for (int k = k0; k < k1; k += 4) {
+    for (int k = s_is[l]; k < k1; k += 4) {
       int w[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int e = a.inc[k + j < k1 ? k + j : k1 - 1];
+        const int e = s_inc[k + j < k1 ? k + j : k1 - 1];
         w[j] = k + j < k1 ? e : TILE << 2;
       }
 #pragma unroll
@@ -774,14 +802,16 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
         if (FK == FVM_FLUX_D_NODAL) accl += s_c[CW * (w[j] >> 2) + 3 + (w[j] & 3)];
       }
     }
-    const int pi = a.part_idx[n0 + l];
+    // the node's slot, id and u are read again here rather than held across the
+    // barriers: fewer VGPRs, more waves per SIMD
+    const int pi = ld32(a.part_idx, (uint32_t)(n0 + l));
     if (pi >= 0) {
-      a.partial[pi] = acc;
-      a.partial2[pi] = accl;
+      st32(a.partial, (uint32_t)pi, acc);
+      st32(a.partial2, (uint32_t)pi, accl);
     } else {
-      const int g = a.tile_nodes[n0 + l];
-      a.out[g] = acc + node_adj<FK>(a, g, s_u[l]);
-      if (a.out2) a.out2[g] = accl;
+      const uint32_t gl = (uint32_t)ld32(a.tile_nodes, (uint32_t)(n0 + l));
+      st32(a.out, gl, acc + node_adj<FK>(a, (int)gl, ld32(a.u, gl)));
+      if (a.out2) st32(a.out2, gl, accl);
     }
   }
 }

@@ -257,51 +257,38 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
   const int n0 = a.tile_node_start[tile], nn = a.tile_node_start[tile + 1] - n0;
   const int ib0 = 3 * t0;
   const double* src = JV ? a.v : a.u;
-  // every global load of the tile before the first LDS store
+  // every global load of the tile before the first LDS store, branch-free (lanes
+  // past the tile's nodes, entries or triangles read the last valid one and never
+  // use it), at 32-bit offsets from bases in scalar registers
   int g[NS], pi[NS], is[NS];
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
-    const int l = tid + r * BLOCK;
-    g[r] = 0;
-    pi[r] = -1;
-    is[r] = 0;
-    if (l < nn) {
-      g[r] = a.tile_nodes[n0 + l];
-      pi[r] = a.lpart[n0 + l];
-      is[r] = a.inc_start[n0 + l] - ib0;
-    }
+    const uint32_t li = (uint32_t)n0 + umin((uint32_t)(tid + r * BLOCK), (uint32_t)nn - 1u);
+    g[r] = ld32(a.tile_nodes, li);
+    pi[r] = ld32(a.lpart, li);
+    is[r] = ld32(a.inc_start, li);
   }
   uint16_t iv[3 * TPT];
 #pragma unroll
-  for (int k = 0; k < 3 * TPT; ++k) {
-    const int idx = tid + k * BLOCK;
-    iv[k] = idx < 3 * nt ? a.inc[ib0 + idx] : (uint16_t)0;
-  }
+  for (int k = 0; k < 3 * TPT; ++k)
+    iv[k] = ld32(a.inc, (uint32_t)ib0 + umin((uint32_t)(tid + k * BLOCK), 3u * (uint32_t)nt - 1u));
   uint32_t L[TPT];
 #pragma unroll
-  for (int r = 0; r < TPT; ++r) {
-    const int lt = tid + r * BLOCK;
-    L[r] = lt < nt ? a.loc[t0 + lt] : 0u;
-  }
+  for (int r = 0; r < TPT; ++r) L[r] = ld32(a.loc, (uint32_t)t0 + umin((uint32_t)(tid + r * BLOCK), (uint32_t)nt - 1u));
   double xg[NS], yg[NS], ug[NS][NV];
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
-    const int l = tid + r * BLOCK;
-    xg[r] = yg[r] = 0.0;
+    const uint32_t gi = (uint32_t)g[r];
+    xg[r] = ld32(a.x, gi);
+    yg[r] = ld32(a.y, gi);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) ug[r][v] = 0.0;
-    if (l < nn) {
-      xg[r] = a.x[g[r]];
-      yg[r] = a.y[g[r]];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) ug[r][v] = src[v + NV * (int64_t)g[r]];
-    }
+    for (int v = 0; v < NV; ++v) ug[r][v] = ld32(src, (uint32_t)v + (uint32_t)NV * gi);
   }
 #pragma unroll
   for (int r = 0; r < NS; ++r) {
     const int l = tid + r * BLOCK;
     if (l < nn) {
-      s_is[l] = is[r];
+      s_is[l] = is[r] - ib0;
       s_x[l] = xg[r];
       s_y[l] = yg[r];
 #pragma unroll
@@ -573,6 +560,8 @@ int fvm_system_create(const fvm_mesh* m, const fvm_system_desc* d, fvm_system** 
   *out = nullptr;
   const int nv = d->n_vars;
   FVM_CHECK(nv >= 1 && nv <= kMaxVars, "fvm_system_create: n_vars must lie in [1, 4]");
+  FVM_CHECK(8 * (int64_t)nv * m->N < ((int64_t)1 << 32),
+            "fvm_system_create: the state (n_vars x n_points doubles) must stay below 4 GiB");
   FVM_CHECK(d->flux_k, "fvm_system_create: flux_k required");
   FVM_CHECK(d->n_terms >= 0 && d->n_terms <= kMaxTerms, "fvm_system_create: at most 24 source terms");
   FVM_CHECK(d->n_terms == 0 || (d->term_var && d->term_exp && d->term_coef),
