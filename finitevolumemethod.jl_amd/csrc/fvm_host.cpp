@@ -448,4 +448,30 @@ int fvmh_csr_pattern(int64_t n_points, int64_t n_tri, const int32_t* tri, int64_
   return 0;
 }
 
+int fvmh_colour_d2(int64_t n, const int64_t* row_ptr, const int32_t* col, int32_t* colour) {
+  // colours are taken in node order; a node's forbidden set is every colour within
+  // distance two, marked in a stamp array (no clearing between nodes)
+  std::fill(colour, colour + n, -1);
+  std::vector<int64_t> stamp;
+  int32_t n_colours = 0;
+  for (int64_t j = 0; j < n; ++j) {
+    for (int64_t a = row_ptr[j]; a < row_ptr[j + 1]; ++a) {
+      const int64_t i = col[a];
+      if (i < 0 || i >= n) return fail("fvmh_colour_d2: column index out of range");
+      for (int64_t b = row_ptr[i]; b < row_ptr[i + 1]; ++b) {
+        const int32_t c = colour[col[b]];
+        if (c >= 0) {
+          if ((int64_t)stamp.size() <= c) stamp.resize(c + 1, -1);
+          stamp[c] = j;
+        }
+      }
+    }
+    int32_t c = 0;
+    while (c < (int32_t)stamp.size() && stamp[c] == j) ++c;
+    colour[j] = c;
+    if (c + 1 > n_colours) n_colours = c + 1;
+  }
+  return n_colours;
+}
+
 }  // extern "C"

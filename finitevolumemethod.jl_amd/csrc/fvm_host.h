@@ -115,6 +115,12 @@ void fvmh_tiles_free(fvmh_tiles* t);
 int fvmh_csr_pattern(int64_t n_points, int64_t n_tri, const int32_t* tri,
                      int64_t* row_ptr, int32_t* col, int64_t* nnz_out);
 
+// Greedy distance-2 colouring of a symmetric CSR pattern (row i holds i): columns
+// that share a row get different colours, so one JVP probe per colour recovers the
+// matrix (sparse Jacobian extraction).  colour[n]; returns the number of colours, or
+// -1 on an index out of range.
+int fvmh_colour_d2(int64_t n, const int64_t* row_ptr, const int32_t* col, int32_t* colour);
+
 // Last error message of this library (thread-local).
 const char* fvmh_last_error(void);
 

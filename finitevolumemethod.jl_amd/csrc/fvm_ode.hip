@@ -286,6 +286,30 @@ int fvm_arnoldi_step(int32_t op_kind, void* op, int64_t n, const double* d_u, do
   return fvm::cgs2_norm(n, j + 1, d_V, d_w, h, (hipStream_t)s);
 }
 
+int fvm_arnoldi_step_scaled(int32_t op_kind, void* op, int64_t n, const double* d_u, double t, double a,
+                            double b, const double* d_scale, double* d_tmp, int32_t j, double* const* d_V,
+                            double* d_w, double* h, fvm_stream s) {
+  FVM_CHECK(op && d_u && d_scale && d_tmp && d_V && d_w && h, "fvm_arnoldi_step_scaled: null argument");
+  FVM_CHECK(op_kind >= FVM_OP_PROBLEM && op_kind <= FVM_OP_TEMPLATE,
+            "fvm_arnoldi_step_scaled: unknown operator kind");
+  FVM_CHECK(j >= 0 && j < 64 && n > 0, "fvm_arnoldi_step_scaled: 0 <= j < 64 and n > 0 required");
+  for (int i = 0; i <= j; ++i)
+    FVM_CHECK(d_V[i] && d_V[i] != d_w && d_V[i] != d_tmp, "fvm_arnoldi_step_scaled: null or aliased basis vector");
+  FVM_CHECK(d_tmp != d_w, "fvm_arnoldi_step_scaled: d_tmp and d_w must differ");
+  // z = scale * V[j] (the right preconditioner), then w = a z + b J z
+  int rc = fvm_vec_mul(n, d_tmp, d_scale, d_V[j], s);
+  if (rc) return rc;
+  int64_t nf = 0;
+  Op o{op_kind, op, false, &nf};
+  if ((rc = o.jvp(d_u, d_tmp, d_w, t, s))) return rc;
+  if (!(a == 0.0 && b == 1.0)) {
+    const double c[2] = {a, b};
+    const double* x[2] = {d_tmp, d_w};
+    if ((rc = lc(n, d_w, 2, c, x, s))) return rc;
+  }
+  return fvm::cgs2_norm(n, j + 1, d_V, d_w, h, (hipStream_t)s);
+}
+
 int fvm_ode_solve(const fvm_ode_opts* o, double* d_u, fvm_ode_stats* st, fvm_stream s) {
   FVM_CHECK(o && d_u && st && o->op, "fvm_ode_solve: null argument");
   FVM_CHECK(o->op_kind >= FVM_OP_PROBLEM && o->op_kind <= FVM_OP_TEMPLATE, "fvm_ode_solve: unknown operator kind");

@@ -149,6 +149,7 @@ def _setup_host(L):
     L.fvmh_tiles_free.argtypes = [ctypes.POINTER(_Tiles)]
     L.fvmh_tiles_free.restype = None
     L.fvmh_csr_pattern.argtypes = [ctypes.c_int64, ctypes.c_int64, _i32p, _i64p, _i32p, _i64p]
+    L.fvmh_colour_d2.argtypes = [ctypes.c_int64, _i64p, _i32p, _i32p]
 
 
 def host_lib():
@@ -233,6 +234,8 @@ HIP_API = {
     "fvm_vec_mdot_noalloc": [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_vp), _vp, _vp, _vp],
     "fvm_vec_maxpy": [ctypes.c_int64, _vp, ctypes.c_int32, _dp, ctypes.POINTER(_vp), _vp],
     "fvm_trbdf2_affine": [_vp, _vp, _vp, _vp, _vp],
+    "fvm_arnoldi_step_scaled": [ctypes.c_int32, _vp, ctypes.c_int64, _vp, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, _vp, _vp, ctypes.c_int32, ctypes.POINTER(_vp), _vp, _dp, _vp],
     "fvm_jit_check": [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_size_t)],
     "fvm_vec_lincomb_err_norm": [ctypes.c_int64, ctypes.c_int, _dp, ctypes.POINTER(_vp), _vp, _vp,
                                  ctypes.c_double, ctypes.c_double, _dp, _vp],

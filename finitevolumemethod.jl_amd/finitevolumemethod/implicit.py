@@ -75,18 +75,15 @@ def is_affine(prob):
 
 def _colouring(rp, col, n):
     """Greedy distance-2 colouring of a symmetric pattern: columns that share a row
-    get different colours."""
-    colour = np.full(n, -1, np.int64)
-    for j in range(n):
-        nb = col[rp[j]:rp[j + 1]]
-        seen = set()
-        for i in nb:
-            seen.update(colour[col[rp[i]:rp[i + 1]]].tolist())
-        c = 0
-        while c in seen:
-            c += 1
-        colour[j] = c
-    return colour
+    get different colours (fvmh_colour_d2 in the host library)."""
+    rp = np.ascontiguousarray(rp, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    colour = np.empty(n, np.int32)
+    nc = _lib.host_lib().fvmh_colour_d2(n, _lib.ptr(rp, _lib._i64p), _lib.ptr(col, _lib._i32p),
+                                        _lib.ptr(colour, _lib._i32p))
+    if nc < 0:
+        raise ValueError(_lib.host_lib().fvmh_last_error().decode())
+    return colour.astype(np.int64)
 
 
 class AffineOperator:

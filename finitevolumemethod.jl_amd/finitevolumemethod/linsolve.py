@@ -26,25 +26,35 @@ def _workspace(n, restart):
 def library_arnoldi(op_kind, handle, u, t, a, b):
     """The Arnoldi step of gmres_device for A = a I + b J(u, t), J the Jacobian of a
     library operator (fvm_arnoldi_step: the product and both Gram-Schmidt passes
-    with one host synchronisation; the same kernels as the step it replaces)."""
+    with one host synchronisation; the same kernels as the step it replaces).  With a
+    right preconditioner diag(scale), gmres_device passes scale and a work vector and
+    the step is fvm_arnoldi_step_scaled, A diag(scale) V[j]."""
     L = _lib.hip_lib()
     buf = np.empty(2 * 64 + 1)
 
-    def step(V, j, w):
+    def step(V, j, w, scale=None, tmp=None):
         k = j + 1
         ptrs = (_lib._vp * k)(*[v.ptr for v in V[:k]])
-        _lib.check(L.fvm_arnoldi_step(int(op_kind), handle, w.n, u.ptr, float(t), float(a), float(b), j, ptrs,
-                                      w.ptr, _lib.ptr(buf), None), "fvm_arnoldi_step")
+        if scale is None:
+            _lib.check(L.fvm_arnoldi_step(int(op_kind), handle, w.n, u.ptr, float(t), float(a), float(b), j, ptrs,
+                                          w.ptr, _lib.ptr(buf), None), "fvm_arnoldi_step")
+        else:
+            _lib.check(L.fvm_arnoldi_step_scaled(int(op_kind), handle, w.n, u.ptr, float(t), float(a), float(b),
+                                                 scale.ptr, tmp.ptr, j, ptrs, w.ptr, _lib.ptr(buf), None),
+                       "fvm_arnoldi_step_scaled")
         return buf[:k].copy(), buf[k:2 * k].copy(), float(buf[2 * k])
 
     return step
 
 
-def gmres_device(apply_A, b, x, rtol=1e-6, restart=50, maxiter=1000, arnoldi=None):
+def gmres_device(apply_A, b, x, rtol=1e-6, restart=50, maxiter=1000, arnoldi=None, precond=None):
     """Restarted GMRES(m) (Saad & Schultz 1986) for A x = b with x0 = 0; the
     operator is a callable apply_A(v, out) on device vectors; m <= 63.  arnoldi
     (library_arnoldi) may do the Arnoldi step for the same A in one library call.
-    Returns the number of iterations."""
+    precond (a device vector s) right-preconditions with diag(s): the iteration runs
+    on A diag(s) and x = diag(s) y, so the residual it minimises is A's own (Saad,
+    Iterative Methods for Sparse Linear Systems, 2nd ed., 9.3.2).  Returns the number
+    of iterations."""
     if not 1 <= restart <= 63:
         raise ValueError("gmres_device: restart must lie in [1, 63]")
     n = b.n
@@ -69,9 +79,13 @@ def gmres_device(apply_A, b, x, rtol=1e-6, restart=50, maxiter=1000, arnoldi=Non
         k_done = 0
         for j in range(restart):
             if arnoldi is not None:
-                h, h2, ww = arnoldi(V, j, w)
+                h, h2, ww = arnoldi(V, j, w) if precond is None else arnoldi(V, j, w, precond, r)
             else:
-                apply_A(V[j], w)
+                if precond is None:
+                    apply_A(V[j], w)
+                else:
+                    _lib.mul(r, precond, V[j])  # r is free until the next restart
+                    apply_A(r, w)
                 # classical Gram-Schmidt applied twice (CGS2: as stable as modified
                 # Gram-Schmidt, Giraud et al. 2005), each pass one fused multi-dot and one
                 # multi-axpy: two host synchronisations instead of j + 1
@@ -100,7 +114,13 @@ def gmres_device(apply_A, b, x, rtol=1e-6, restart=50, maxiter=1000, arnoldi=Non
                 break
         y = np.linalg.solve(np.triu(H[:k_done, :k_done]), g[:k_done]) if k_done else np.zeros(0)
         if k_done:
-            _lib.maxpy(x, y, V[:k_done])
+            if precond is None:
+                _lib.maxpy(x, y, V[:k_done])
+            else:
+                r.fill_zero()
+                _lib.maxpy(r, y, V[:k_done])
+                _lib.mul(r, precond, r)
+                _lib.lincomb(x, (1.0, 1.0), (x, r))
         if abs(g[k_done]) <= rtol * bnorm:
             break
     return total

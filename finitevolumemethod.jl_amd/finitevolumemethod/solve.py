@@ -75,6 +75,7 @@ class TRBDF2:
     krylov_restart: int = 30
     smooth_est: bool = True
     device_loop: bool = True       # small affine problems: the whole integration in one launch (implicit.py)
+    precond: str = "jacobi"        # right-precondition GMRES with diag(W) (FVMProblem), or "none"
 
 
 @dataclass
@@ -208,12 +209,45 @@ def solve(prob, alg=None, *, saveat=None, abstol=None, reltol=None, dt=None, max
     rhs = lambda u, du, t: ctx.rhs(u, du, t)  # noqa: E731
     has_dir = prob.conditions.has_dirichlet_nodes()
     cb = (lambda u, t: ctx.apply_dirichlet(u, t)) if has_dir else None
+    extra = {}
+    if isinstance(alg, TRBDF2) and alg.precond == "jacobi":
+        extra["jac_diag"] = jacobian_diagonal(prob)
+    elif isinstance(alg, TRBDF2) and alg.precond != "none":
+        raise ValueError(f"TRBDF2: unknown precond {alg.precond!r} (\"jacobi\" or \"none\")")
     return integrate(rhs, prob.initial_condition, prob.initial_time, prob.final_time,
                      alg if alg is not None else Tsit5(), saveat=saveat, abstol=abstol,
                      reltol=reltol, dt=dt, maxiters=maxiters, save_start=save_start,
                      step_callback=cb, user_callback=callback,
                      jvp=lambda u, v, out, t: ctx.jvp(u, v, out, t),
-                     op=(_lib.OP_PROBLEM, ctx.handle, has_dir), fsal=fsal_ok(prob))
+                     op=(_lib.OP_PROBLEM, ctx.handle, has_dir), fsal=fsal_ok(prob), **extra)
+
+
+def jacobian_diagonal(prob):
+    """diag(u, t, out): out = the diagonal of fvm_eqs!'s Jacobian at (u, t), from one
+    fvm_jvp per colour of a distance-2 colouring of jacobian_sparsity's pattern
+    (columns of one colour share no row, so J p_c holds each of their diagonal entries
+    in its own row: out = sum_c p_c * J p_c).  The colouring and the probe vectors
+    are kept on the problem."""
+    from .equations import csr_pattern
+    from .implicit import _colouring
+
+    ctx = prob.device_context()
+    probes = getattr(prob, "_diag_probes", None)
+    if probes is None:
+        n = prob.mesh.triangulation.num_points()
+        rp, col = csr_pattern(prob.mesh.triangulation)
+        colour = _colouring(rp, col, n)
+        probes = [_lib.DeviceArray.from_host((colour == c).astype(np.float64)) for c in range(int(colour.max()) + 1)]
+        prob._diag_probes = probes
+    jv = _lib.DeviceArray(probes[0].n)
+
+    def diag(u, t, out):
+        out.fill_zero()
+        for p in probes:
+            ctx.jvp(u, p, jv, t)
+            _lib.mul(jv, p, jv)
+            _lib.lincomb(out, (1.0, 1.0), (out, jv))
+    return diag
 
 
 def fsal_ok(prob):
@@ -252,7 +286,7 @@ def integrate(rhs, u0, t0, tf, alg, *, saveat=None, save_start=True, **kwargs):
 
 def _integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, dt=None,
                maxiters=10**7, save_start=True, step_callback=None, user_callback=None, jvp=None,
-               op=None, fsal=False):
+               op=None, fsal=False, jac_diag=None):
     """Integrate du/dt = rhs(u, t) on the device from t0 to tf.  Implicit methods
     need jvp(u, v, out, t) = J(u) v.
 
@@ -274,7 +308,8 @@ def _integrate(rhs, u0, t0, tf, alg, *, saveat=None, abstol=1e-6, reltol=1e-3, d
             raise TypeError("TRBDF2 needs the Jacobian-vector product of the right-hand side")
         return _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, saveat=saveat, abstol=abstol,
                                  reltol=reltol, dt=dt, maxiters=maxiters, save_start=save_start,
-                                 step_callback=step_callback, user_callback=user_callback, op=op)
+                                 step_callback=step_callback, user_callback=user_callback, op=op,
+                                 jac_diag=jac_diag)
     n = len(u0)
     u = _lib.DeviceArray.from_host(np.asarray(u0, np.float64))
     sol = ODESolution()
@@ -410,7 +445,10 @@ _TRB_K = (3 * _TRB_G ** 2 - 4 * _TRB_G + 2) / (6 * (2.0 - _TRB_G))
 
 
 def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, maxiters,
-                      save_start, step_callback, user_callback, op=None):
+                      save_start, step_callback, user_callback, op=None, jac_diag=None):
+    """jac_diag(u, t, out): the diagonal of J(u, t) on the device; with it, GMRES is
+    right-preconditioned by diag(W)^-1, W = I - d h J frozen at the step's start (one
+    diagonal per step, its inverse per step size)."""
     from .linsolve import gmres_device, library_arnoldi
 
     n = len(u0)
@@ -441,10 +479,28 @@ def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, 
             return None
         step = library_arnoldi(op[0], op[1], ubase, tbase, 1.0, -_TRB_D * h)
 
-        def counted(V, j, w):
+        def counted(V, j, w, *pre):
             stats["njvp"] += 1
-            return step(V, j, w)
+            return step(V, j, w, *pre)
         return counted
+
+    # Jacobi preconditioner: diag(J) at the step's start, downloaded once per step;
+    # 1 / diag(W) for each step size tried from it
+    pre = dict(t=None, d=None, h=None)
+    S = _lib.DeviceArray(n) if jac_diag is not None else None
+
+    def precond(ubase, tbase, h):
+        if jac_diag is None:
+            return None
+        if pre["t"] != tbase or pre["d"] is None:
+            jac_diag(ubase, tbase, V["tmp"])
+            pre["d"], pre["t"], pre["h"] = V["tmp"].to_host(), tbase, None
+        if pre["h"] != h:
+            w = 1.0 - _TRB_D * h * pre["d"]
+            s = np.where(np.abs(w) > 1e-12, 1.0 / np.where(w == 0.0, 1.0, w), 1.0)
+            S.upload(s)
+            pre["h"] = h
+        return S
 
     nl = dict(eta=1.0)  # Newton's contraction estimate theta / (1 - theta), kept between solves
 
@@ -461,7 +517,8 @@ def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, 
             _lib.lincomb(V["G"], (-1.0, _TRB_D * h, 1.0), (z, fz, r))  # -G = r + d h f(z) - z
             stats["nkrylov"] += gmres_device(A, V["G"], V["dz"], rtol=alg.krylov_rtol,
                                              restart=alg.krylov_restart, maxiter=20 * alg.krylov_restart,
-                                             arnoldi=W_arnoldi(ubase, tbase, h))
+                                             arnoldi=W_arnoldi(ubase, tbase, h),
+                                             precond=precond(ubase, tbase, h))
             _lib.lincomb(z, (1.0, 1.0), (z, V["dz"]))
             stats["nnewton"] += 1
             ndz = _lib.err_norm(V["dz"], ubase, z, abstol, reltol)
@@ -520,7 +577,7 @@ def _integrate_trbdf2(rhs, jvp, u0, t0, tf, alg, *, saveat, abstol, reltol, dt, 
         if alg.smooth_est:  # W^-1 est (Hosea & Shampine): keeps stiff components from inflating it
             stats["nkrylov"] += gmres_device(W_apply(u, t, h), est, V["tmp"], rtol=alg.krylov_rtol,
                                              restart=alg.krylov_restart, maxiter=20 * alg.krylov_restart,
-                                             arnoldi=W_arnoldi(u, t, h))
+                                             arnoldi=W_arnoldi(u, t, h), precond=precond(u, t, h))
             est = V["tmp"]
         EEst = _lib.err_norm(est, u, u1, abstol, reltol)
         if not math.isfinite(EEst):

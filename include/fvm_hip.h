@@ -566,6 +566,15 @@ int fvm_ode_solve(const fvm_ode_opts* o, double* d_u, fvm_ode_stats* st, fvm_str
  * or a Krylov method from LinearSolve (solve.jl:167-220).  Blocks until done. */
 int fvm_arnoldi_step(int32_t op_kind, void* op, int64_t n, const double* d_u, double t, double a, double b,
                      int32_t j, double* const* d_V, double* d_w, double* h, fvm_stream s);
+/* fvm_arnoldi_step for the right-preconditioned operator A diag(d_scale): w =
+ * (a I + b J)(d_scale * d_V[j]), the product's input staged in d_tmp (n doubles, not
+ * d_w and not a basis vector).  With d_scale the inverse diagonal of A this is the
+ * Jacobi-preconditioned GMRES of TR-BDF2's and SteadyFVMProblem's Newton steps,
+ * where the reference factorises the Jacobian with KLU (solve.jl:50-77, 167-183).
+ * Blocks until done. */
+int fvm_arnoldi_step_scaled(int32_t op_kind, void* op, int64_t n, const double* d_u, double t, double a,
+                            double b, const double* d_scale, double* d_tmp, int32_t j, double* const* d_V,
+                            double* d_w, double* h, fvm_stream s);
 
 #ifdef __cplusplus
 }

@@ -340,3 +340,24 @@ def test_controller_power_is_the_same_in_python_and_cpp():
             a, b = ctrl_pow(float(x), p), f(float(x), p)
             assert a == b, (x, p, a, b)
             assert abs(a - float(x) ** p) <= 1e-14 * float(x) ** p
+
+
+def test_distance_two_colouring_separates_columns_of_every_row():
+    """fvmh_colour_d2 (the colouring behind the Jacobian probes): in every row of
+    jacobian_sparsity's pattern the columns carry distinct colours, on a jittered mesh
+    and on a triangulate_rectangle grid, with at most 16 colours; a bad column index
+    is rejected."""
+    from finitevolumemethod.equations import csr_pattern
+    from finitevolumemethod.implicit import _colouring
+
+    for tri in (FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 31, 27, jitter=0.25, seed=4),
+                FVM.triangulate_rectangle(0.0, 2.0, 0.0, 1.0, 40, 21, single_boundary=True)):
+        rp, col = csr_pattern(tri)
+        n = tri.num_points()
+        c = _colouring(rp, col, n)
+        assert c.min() == 0 and c.max() < 16
+        for i in range(n):
+            cs = c[col[rp[i]:rp[i + 1]]]
+            assert len(np.unique(cs)) == len(cs), i
+    with pytest.raises(ValueError):
+        _colouring(np.array([0, 2, 3]), np.array([0, 5, 1], np.int32), 2)
