@@ -266,14 +266,17 @@ def template_solve(n_side, rtol=1e-10, direct=True):
     x = _lib.DeviceArray.from_host(sols["cg_fp64"])
     y = _lib.DeviceArray(N)
     W, n_over, ib = dt.ell_info(index_bytes=True)
-    out["operator_form"] = dict(ell_slots=W, overflow_entries=n_over, column_index_bytes=ib)
-    # bytes per product on the ELL copy: W (column, value) slots, the diagonal, the
-    # overflow start, x gathered once and y written once per row, and the overflow
-    # entries (column, value, gather)
-    rows = N
+    n_off = int(dt.nnz) - N
+    n_slices = (N + 63) // 64
+    out["operator_form"] = dict(layout="sliced ELL: 64-row slices (one wave), rows ordered by length inside "
+                                       "a slice, slot s holding the s-th off-diagonal of the rows that have one",
+                                unrolled_slots=W, off_diagonal_entries=n_off, entries_past_unrolled_slots=n_over,
+                                column_index_bytes=ib)
+    # bytes per product on the sliced ELL copy: each off-diagonal's (column, value),
+    # per row the diagonal, the lane's row byte, x gathered once and y written once,
+    # per slice its W + 1 slot starts and remainder pointer
     for which, key, s in ((1, "spmv_fp64", 8), (2, "spmv_fp32", 4)):
-        per_row = W * (ib + s) + s + 4 + 8 + 8
-        per_over = 4 + s + 8
+        nbytes = n_off * (ib + s) + N * (s + 1 + 8 + 8) + n_slices * 4 * (W + 2)
         for _ in range(3):
             dt.spmv(which, x, y)
         tm = _lib.Timer()
@@ -282,8 +285,7 @@ def template_solve(n_side, rtol=1e-10, direct=True):
         for _ in range(reps):
             dt.spmv(which, x, y)
         ms = tm.stop() / reps
-        out[key] = dict(us=round(ms * 1e3, 2),
-                        GBps=round((per_row * rows + per_over * n_over) / (ms * 1e-3) / 1e9, 1))
+        out[key] = dict(us=round(ms * 1e3, 2), GBps=round(nbytes / (ms * 1e-3) / 1e9, 1))
     if direct:
         import scipy.sparse.linalg as spla
 

@@ -27,9 +27,10 @@
 //                       recomputed in FP64 whenever it has dropped by inner_rtol
 //                       (default 0.1), keeping the search direction.
 //   FVM_SOLVE_BICGSTAB  FP64 BiCGStab on M with Jacobi right preconditioning.
-// Every reduction finishes in the last workgroup to arrive, in a fixed order, so
-// solver scalars stay on the device and results are bitwise reproducible; the
-// host reads the scalars only every check_every iterations.
+// Each reduction leaves one partial per workgroup, and every workgroup of the next
+// kernel re-adds all of them in the same fixed order, so solver scalars stay on the
+// device and results are bitwise reproducible; the host reads the scalars only
+// every check_every iterations.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -47,7 +48,8 @@ constexpr int kMaxG = 1024;              // workgroups of the solver kernels (fi
 constexpr int kLPR = 8;                  // lanes per CSR row in the SpMV kernels
 constexpr int kRowsPerBlock = kRT / kLPR;
 constexpr int kRPG = FVM_SPMV_ROWS;      // rows per lane group per step of the SpMV loops
-constexpr int kEllMin = 4, kEllMax = 10; // ELL slots per row: the fewest covering 99 % of the rows
+constexpr int kEllMin = 4, kEllMax = 10; // unrolled slots of the sliced ELL form: the longest row's, in 4..10
+constexpr int kSlice = 64;               // rows per slice of the sliced ELL form: one wave
 
 // solver scalars (device doubles)
 // (S_RZ, S_ALPHA, S_RHO and S_OMEGA take two slots each: value of an even / odd iteration)
@@ -80,26 +82,30 @@ struct fvm_template {
   double* part = nullptr;    // [PT_COUNT * kMaxG] reduction partials
   double* sc = nullptr;      // [S_COUNT] solver scalars
   int G = 1;
-  // ELL copy of M for the CG direction kernel (ellW = 0: not built): ellW
-  // off-diagonal slots per row stored slot-major, [s][n], padded with the row's own
-  // index and a zero value; further off-diagonals of a row in ovf_rp / ovf_col; the
-  // diagonal slot's value (FP32: the row sum of the difference form) in diagM*.
+  // Sliced ELL copy of M for the Krylov products (ellW = 0: not built).  The rows
+  // go in slices of kSlice (one wave each); inside a slice they are ordered by
+  // their number of off-diagonals, longest first (sl_perm: the row, relative to the
+  // slice, of each lane), and slot s of a slice holds the s-th off-diagonal (CSR
+  // order) of every row that has one: lanes 0 .. cnt_s - 1, stored contiguously,
+  // so no slot is padding.  sl_off[(ellW + 1) b + s], s = 0 .. ellW, is the first
+  // entry of slot s of slice b; slices with rows longer than ellW list the ends of
+  // slots ellW, ellW + 1, ... in sl_rem_off[sl_rem_ptr[b] .. sl_rem_ptr[b + 1]).
+  // The diagonal (FP32: the row sum of the difference form) in diagM*, row order.
   int ellW = 0;
-  int64_t ell_overflow = 0;
-  bool use_ell = false;      // this solve's choice (fvm_solve_opts.spmv_format)
-  int32_t* ell_col = nullptr;  // column of each slot, or (ell_off) its offset from the row
+  int64_t ell_overflow = 0;    // entries in slots ellW and beyond
+  bool use_ell = false;        // this solve's choice (fvm_solve_opts.spmv_format)
+  int32_t* sl_off = nullptr;
+  uint8_t* sl_perm = nullptr;
+  int32_t* sl_rem_ptr = nullptr;
+  int32_t* sl_rem_off = nullptr;
+  int32_t* ell_col = nullptr;  // column of each entry, or (ell_off) its offset from the row
   int16_t* ell_off = nullptr;  // used instead of ell_col when every |column - row| < 2^15
   double* ellM = nullptr;
   float* ellM32 = nullptr;
   double* diagM = nullptr;
   float* diagM32 = nullptr;
-  int32_t* ovf_rp = nullptr;
-  int32_t* ovf_col = nullptr;
-  double* ovfM = nullptr;
-  float* ovfM32 = nullptr;
-  double* ellA = nullptr;    // the same slots of A itself (fvm_template_apply / _spmv)
+  double* ellA = nullptr;      // the same entries of A itself (fvm_template_apply / _spmv)
   double* diagA = nullptr;
-  double* ovfA = nullptr;
 };
 
 namespace {
@@ -497,47 +503,87 @@ __global__ __launch_bounds__(kRT) void cg_dir_spmv(int64_t n, int k, int proj, d
   store_partial(pq, part + PT_PQ * kMaxG);
 }
 
-// The column of an ELL slot: stored as is (int32) or as its offset from the row
-// (int16, when the whole operator allows it: two bytes fewer per slot).
+// The column of an ELL entry: stored as is (int32) or as its offset from the row
+// (int16, when the whole operator allows it: two bytes fewer per entry).
 __device__ __forceinline__ int ell_column(int32_t c, int64_t) { return c; }
 __device__ __forceinline__ int ell_column(int16_t d, int64_t i) { return (int)i + (int)d; }
 
-// Rows of the ELL copy of M, one per thread: the W column / value pairs of a row
-// are coalesced loads issued together, then the W gathers, so a row costs two
-// dependent loads instead of the CSR form's three (row start, entries, gathers)
-// and every lane works.  body(i, g(i), y_i) with y_i = the off-diagonal terms in
-// CSR order (v (g(c) - g(i)) in the difference form, v g(c) otherwise), then the
-// row's overflow entries, then diag_i g(i).
+// The index arrays of the sliced ELL form (fvm_template: sl_off, sl_perm,
+// sl_rem_ptr, sl_rem_off).
+struct SellIdx {
+  const int32_t* off;
+  const uint8_t* perm;
+  const int32_t* rem_ptr;
+  const int32_t* rem_off;
+};
+
+// One slice of the sliced ELL form, taken by one wave: lane l computes row
+// i = 64 b + perm[64 b + l], its W first slots' column / value loads issued together
+// (slot s's entries of the lanes that have one are contiguous: coalesced, and no
+// entry is padding), then their gathers; rows longer than W continue slot by slot.
+// body(i, g(i), y_i) with y_i the off-diagonal terms in CSR order (v (g(c) - g(i))
+// in the difference form, v g(c) otherwise), then diag_i g(i).  The loads and
+// gathers are not predicated, so that all of them are in flight together: a lane
+// past its row's last slot reads the entry arrays' kSlice-entry tail or another
+// lane's entry and uses column i and value 0 instead (an exact zero term, as in a
+// padded ELL row).  The slot starts are wave-uniform: scalar loads.
 template <bool DIFF, int W, class V, class CI, class Gather, class Body>
-__device__ __forceinline__ void ell_rows(int64_t n, const CI* __restrict__ ecol, const V* __restrict__ eval,
-                                         const V* __restrict__ diag, const int32_t* __restrict__ orp,
-                                         const int32_t* __restrict__ ocol, const V* __restrict__ oval, Gather g,
-                                         Body body) {
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
-    int c[W];
-    V v[W];
+__device__ __forceinline__ void sell_slice(int64_t n, int64_t b, const SellIdx& S, const CI* __restrict__ ecol,
+                                           const V* __restrict__ eval, const V* __restrict__ diag, Gather g,
+                                           Body body) {
+  using cint = const __attribute__((address_space(4))) int32_t;
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int64_t r0 = b * kSlice;
+  int o[W + 1];
 #pragma unroll
-    for (int s = 0; s < W; ++s) {
-      c[s] = ell_column(ecol[s * n + i], i);
-      v[s] = eval[s * n + i];
-    }
-    const int o0 = orp[i], o1 = orp[i + 1];
-    const V gi = g((int)i);
-    V acc = 0;
+  for (int s = 0; s <= W; ++s) o[s] = ((cint*)S.off)[(W + 1) * b + s];
+  const int q0 = ((cint*)S.rem_ptr)[b], q1 = ((cint*)S.rem_ptr)[b + 1];
+  if (lane >= n - r0) return;
+  const int64_t i = r0 + S.perm[r0 + lane];
+  int c[W];
+  V v[W];
 #pragma unroll
-    for (int s = 0; s < W; ++s) acc += v[s] * (DIFF ? g(c[s]) - gi : g(c[s]));
-    for (int kk = o0; kk < o1; ++kk) acc += oval[kk] * (DIFF ? g(ocol[kk]) - gi : g(ocol[kk]));
-    acc += diag[i] * gi;
-    body(i, gi, acc);
+  for (int s = 0; s < W; ++s) {
+    const int e = o[s] + lane;
+    const bool on = e < o[s + 1];
+    const int cs = ell_column(ecol[e], i);
+    const V vs = eval[e];
+    c[s] = on ? cs : (int)i;
+    v[s] = on ? vs : (V)0;
   }
+  const V gi = g((int)i);
+  V acc = 0;
+#pragma unroll
+  for (int s = 0; s < W; ++s) acc += v[s] * (DIFF ? g(c[s]) - gi : g(c[s]));
+  for (int q = q0, st = o[W]; q < q1; ++q) {  // slots W, W + 1, ... (rows longer than W)
+    const int en = ((cint*)S.rem_off)[q], e = st + lane;
+    if (e < en) {
+      const int cc = ell_column(ecol[e], i);
+      acc += eval[e] * (DIFF ? g(cc) - gi : g(cc));
+    }
+    st = en;
+  }
+  acc += diag[i] * gi;
+  body(i, gi, acc);
 }
 
-// The CG direction iteration on the ELL copy of M.
+// Every slice of the sliced ELL form, one wave per slice, grid-stride.
+template <bool DIFF, int W, class V, class CI, class Gather, class Body>
+__device__ __forceinline__ void sell_rows(int64_t n, const SellIdx& S, const CI* __restrict__ ecol,
+                                          const V* __restrict__ eval, const V* __restrict__ diag, Gather g,
+                                          Body body) {
+  constexpr int kWaves = kRT / kSlice;
+  const int64_t ns = (n + kSlice - 1) / kSlice;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kSlice);
+  for (int64_t b = (int64_t)blockIdx.x * kWaves + w; b < ns; b += (int64_t)gridDim.x * kWaves)
+    sell_slice<DIFF, W>(n, b, S, ecol, eval, diag, g, body);
+}
+
+// The CG direction iteration on the sliced ELL copy of M.
 template <class V, int W, class CI>
 __global__ __launch_bounds__(kRT) void cg_dir_ell(int64_t n, int k, int proj, double irt2, double tabs2,
-                                                  const CI* __restrict__ ecol, const V* __restrict__ eval,
-                                                  const V* __restrict__ diag, const int32_t* __restrict__ orp,
-                                                  const int32_t* __restrict__ ocol, const V* __restrict__ oval,
+                                                  SellIdx S, const CI* __restrict__ ecol,
+                                                  const V* __restrict__ eval, const V* __restrict__ diag,
                                                   const V* __restrict__ z, const V* __restrict__ pold,
                                                   V* __restrict__ pnew, V* __restrict__ q, double* sc,
                                                   double* part) {
@@ -545,7 +591,7 @@ __global__ __launch_bounds__(kRT) void cg_dir_ell(int64_t n, int k, int proj, do
   if (!cg_dir_coeffs<V>(k, proj, irt2, tabs2, sc, part, &va, &vb)) return;
   double pq = 0.0;
   auto dir = [&](int c) { return va * z[c] + vb * pold[c]; };
-  ell_rows<kDiffForm<V>, W>(n, ecol, eval, diag, orp, ocol, oval, dir, [&](int64_t i, V pi, V qi) {
+  sell_rows<kDiffForm<V>, W>(n, S, ecol, eval, diag, dir, [&](int64_t i, V pi, V qi) {
     pnew[i] = pi;
     q[i] = qi;
     pq += (double)pi * (double)qi;
@@ -690,17 +736,15 @@ __global__ __launch_bounds__(kRT) void bi_spmv(int64_t n, const int32_t* __restr
   if (MODE == 1) store_partial(yy, part + PT_TT * kMaxG);
 }
 
-// bi_spmv on the ELL copy of M (FP64).
+// bi_spmv on the sliced ELL copy of M (FP64).
 template <int MODE, int W, class CI>
-__global__ __launch_bounds__(kRT) void bi_spmv_ell(int64_t n, const CI* __restrict__ ecol,
+__global__ __launch_bounds__(kRT) void bi_spmv_ell(int64_t n, SellIdx S, const CI* __restrict__ ecol,
                                                    const double* __restrict__ eval, const double* __restrict__ diag,
-                                                   const int32_t* __restrict__ orp, const int32_t* __restrict__ ocol,
-                                                   const double* __restrict__ oval, const double* __restrict__ x,
-                                                   double* __restrict__ y, const double* __restrict__ a,
-                                                   const double* sc, double* part) {
+                                                   const double* __restrict__ x, double* __restrict__ y,
+                                                   const double* __restrict__ a, const double* sc, double* part) {
   if (wg_done(sc)) return;
   double ay = 0.0, yy = 0.0;
-  ell_rows<false, W>(n, ecol, eval, diag, orp, ocol, oval, [&](int c) { return x[c]; },
+  sell_rows<false, W>(n, S, ecol, eval, diag, [&](int c) { return x[c]; },
                      [&](int64_t i, double, double yi) {
                        y[i] = yi;
                        ay += a[i] * yi;
@@ -825,15 +869,13 @@ __global__ __launch_bounds__(kRT) void spmv_kernel(int64_t n, const int32_t* __r
   });
 }
 
-// y = A x (+ b) on an ELL copy (precision V, the difference form for FP32)
+// y = A x (+ b) on a sliced ELL copy (precision V, the difference form for FP32)
 template <class V, int W, class CI>
-__global__ __launch_bounds__(kRT) void spmv_ell_kernel(int64_t n, const CI* __restrict__ ecol,
+__global__ __launch_bounds__(kRT) void spmv_ell_kernel(int64_t n, SellIdx S, const CI* __restrict__ ecol,
                                                        const V* __restrict__ eval, const V* __restrict__ diag,
-                                                       const int32_t* __restrict__ orp,
-                                                       const int32_t* __restrict__ ocol, const V* __restrict__ oval,
                                                        const double* __restrict__ x, const double* __restrict__ b,
                                                        double* __restrict__ y) {
-  ell_rows<kDiffForm<V>, W>(n, ecol, eval, diag, orp, ocol, oval, [&](int c) { return (V)x[c]; },
+  sell_rows<kDiffForm<V>, W>(n, S, ecol, eval, diag, [&](int c) { return (V)x[c]; },
                             [&](int64_t i, V, V yi) { y[i] = b ? (double)yi + b[i] : (double)yi; });
 }
 
@@ -845,7 +887,7 @@ __global__ __launch_bounds__(kRT) void spmv_ell_kernel(int64_t n, const CI* __re
 // on the ELL copy of A, error norm, step control, saveat, first same as last) with
 // barriers between the phases, so the integration is one launch instead of ~13
 // launches and a host synchronisation per step.  Each element's arithmetic is the
-// multi-kernel path's: the same row products (ell_rows' order), the same linear
+// multi-kernel path's: the same row products (sell_slice's order), the same linear
 // combinations (terms left to right), and the error norm's sum in the fixed order
 // of its 512-workgroup grid, emulated here block by block; the step controller's
 // power is fvm_ctrl.hpp's, the host loops' own: the same states bit for bit.
@@ -858,13 +900,11 @@ constexpr int kRedBlocksE = 512, kRedThreadsE = 256;  // the grid whose summatio
 
 struct SmallTsit5 {
   int64_t n;
+  SellIdx S;            // the sliced ELL copy of A
   const int32_t* ecol;  // one of ecol / eoff
   const int16_t* eoff;
   const double* eval;
   const double* diag;
-  const int32_t* orp;
-  const int32_t* ocol;
-  const double* oval;
   const double* b;
   double* k[7];  // the stages (k[0] = f(u) on entry)
   double* utmp;
@@ -882,24 +922,20 @@ struct SmallTsit5 {
   double A[7][6], C[7], Bt[7];
 };
 
+// out = A x + b over the state, the workgroup's 16 waves taking the slices of the
+// sliced ELL copy in turn.
 template <int W, bool OFF>
-__device__ __forceinline__ double ell_row_apply(const SmallTsit5& a, int64_t i, const double* x) {
-  const int64_t n = a.n;
-  int c[W];
-  double v[W];
-#pragma unroll
-  for (int s = 0; s < W; ++s) {
-    c[s] = OFF ? ell_column(a.eoff[s * n + i], i) : a.ecol[s * n + i];
-    v[s] = a.eval[s * n + i];
+__device__ __forceinline__ void small_apply(const SmallTsit5& a, const double* x, double* out) {
+  const int64_t ns = (a.n + kSlice - 1) / kSlice;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kSlice);
+  auto g = [&](int c) { return x[c]; };
+  auto body = [&](int64_t i, double, double acc) { out[i] = acc + a.b[i]; };
+  for (int64_t b = w; b < ns; b += kSmallThreads / kSlice) {
+    if (OFF)
+      sell_slice<false, W>(a.n, b, a.S, a.eoff, a.eval, a.diag, g, body);
+    else
+      sell_slice<false, W>(a.n, b, a.S, a.ecol, a.eval, a.diag, g, body);
   }
-  const int o0 = a.orp[i], o1 = a.orp[i + 1];
-  const double gi = x[i];
-  double acc = 0;
-#pragma unroll
-  for (int s = 0; s < W; ++s) acc += v[s] * x[c[s]];
-  for (int kk = o0; kk < o1; ++kk) acc += a.oval[kk] * x[a.ocol[kk]];
-  acc += a.diag[i] * gi;
-  return acc + a.b[i];
 }
 
 // The pairwise tree of a 256-thread reduction, sh[t] += sh[t + s] for s = 128, ..., 1,
@@ -950,7 +986,7 @@ __global__ __launch_bounds__(kSmallThreads) void tsit5_small_kernel(SmallTsit5 a
         dst[e] = v;
       }
       __syncthreads();
-      for (int64_t e = tid; e < n; e += kSmallThreads) k[j][e] = ell_row_apply<W, OFF>(a, e, dst);
+      small_apply<W, OFF>(a, dst, k[j]);
       ++nf;
       __syncthreads();
     }
@@ -1017,7 +1053,7 @@ __global__ __launch_bounds__(kSmallThreads) void tsit5_small_kernel(SmallTsit5 a
         k[6] = k0;
       } else {
         __syncthreads();
-        for (int64_t e = tid; e < n; e += kSmallThreads) k[0][e] = ell_row_apply<W, OFF>(a, e, u);
+        small_apply<W, OFF>(a, u, k[0]);
         ++nf;
       }
       double q = fvm::ctrl_pow(fmax(EEst, 1e-16), 7.0 / 50.0) / fvm::ctrl_pow(qold, 2.0 / 25.0);
@@ -1064,9 +1100,9 @@ int dupload(T** dst, const T* src, int64_t n) {
 void free_tmpl(fvm_template* t) {
   if (!t) return;
   void* p[] = {t->row_ptr, t->col, t->val, t->valM, t->valM32, t->b, t->fixed, t->vol, t->dinv,
-               t->wv, t->dinv32, t->wv32, t->work, t->part, t->sc, t->ell_col, t->ell_off, t->ellM,
-               t->ellM32, t->diagM, t->diagM32, t->ovf_rp, t->ovf_col, t->ovfM, t->ovfM32,
-               t->ellA, t->diagA, t->ovfA};
+               t->wv, t->dinv32, t->wv32, t->work, t->part, t->sc, t->sl_off, t->sl_perm, t->sl_rem_ptr,
+               t->sl_rem_off, t->ell_col, t->ell_off, t->ellM, t->ellM32, t->diagM, t->diagM32, t->ellA,
+               t->diagA};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete t;
@@ -1087,16 +1123,16 @@ template <>
 struct EllVals<double> {
   static const double* ell(const fvm_template* t) { return t->ellM; }
   static const double* diag(const fvm_template* t) { return t->diagM; }
-  static const double* ovf(const fvm_template* t) { return t->ovfM; }
 };
 template <>
 struct EllVals<float> {
   static const float* ell(const fvm_template* t) { return t->ellM32; }
   static const float* diag(const fvm_template* t) { return t->diagM32; }
-  static const float* ovf(const fvm_template* t) { return t->ovfM32; }
 };
 
-// f(std::integral_constant<int, W>) for the template's ELL width
+inline SellIdx sell_idx(const fvm_template* t) { return SellIdx{t->sl_off, t->sl_perm, t->sl_rem_ptr, t->sl_rem_off}; }
+
+// f(std::integral_constant<int, W>) for the template's unrolled ELL slots
 template <class F>
 void by_width(int W, F&& f) {
   static_assert(kEllMin == 4 && kEllMax == 10, "one case per ELL width");
@@ -1111,8 +1147,8 @@ void by_width(int W, F&& f) {
   }
 }
 
-// f(width tag, column array) for the template's ELL copy: its width as a constant
-// and its columns as int32 columns or int16 offsets
+// f(width tag, column array) for the template's sliced ELL copy: its unrolled slots
+// as a constant and its columns as int32 columns or int16 offsets
 template <class P>
 using ci_t = std::remove_cv_t<std::remove_pointer_t<P>>;
 template <class F>
@@ -1137,8 +1173,8 @@ void launch_dir(const fvm_template* t, const V* valM, int k, int proj, double ir
   }
   by_ell(t, [&](auto w, auto ec) {
     hipLaunchKernelGGL((cg_dir_ell<V, decltype(w)::value, ci_t<decltype(ec)>>), grid(t), dim3(kRT), 0, s, t->n, k,
-                       proj, irt2, tabs2, ec, EllVals<V>::ell(t), EllVals<V>::diag(t), t->ovf_rp, t->ovf_col,
-                       EllVals<V>::ovf(t), z, pold, pnew, q, t->sc, t->part);
+                       proj, irt2, tabs2, sell_idx(t), ec, EllVals<V>::ell(t), EllVals<V>::diag(t), z, pold, pnew,
+                       q, t->sc, t->part);
   });
 }
 
@@ -1152,7 +1188,7 @@ void launch_bi_spmv(const fvm_template* t, const double* x, double* y, const dou
   }
   by_ell(t, [&](auto w, auto ec) {
     hipLaunchKernelGGL((bi_spmv_ell<MODE, decltype(w)::value, ci_t<decltype(ec)>>), grid(t), dim3(kRT), 0, s, t->n,
-                       ec, t->ellM, t->diagM, t->ovf_rp, t->ovf_col, t->ovfM, x, y, a, t->sc, t->part);
+                       sell_idx(t), ec, t->ellM, t->diagM, x, y, a, t->sc, t->part);
   });
 }
 
@@ -1163,15 +1199,16 @@ void spmv_any(const fvm_template* t, int which, const double* x, const double* b
     by_ell(t, [&](auto w, auto ec) {
       constexpr int W = decltype(w)::value;
       using CI = ci_t<decltype(ec)>;
+      const SellIdx S = sell_idx(t);
       if (which == 0)
-        hipLaunchKernelGGL((spmv_ell_kernel<double, W, CI>), grid(t), dim3(kRT), 0, s, t->n, ec, t->ellA,
-                           t->diagA, t->ovf_rp, t->ovf_col, t->ovfA, x, b, y);
+        hipLaunchKernelGGL((spmv_ell_kernel<double, W, CI>), grid(t), dim3(kRT), 0, s, t->n, S, ec, t->ellA,
+                           t->diagA, x, b, y);
       else if (which == 1)
-        hipLaunchKernelGGL((spmv_ell_kernel<double, W, CI>), grid(t), dim3(kRT), 0, s, t->n, ec, t->ellM,
-                           t->diagM, t->ovf_rp, t->ovf_col, t->ovfM, x, b, y);
+        hipLaunchKernelGGL((spmv_ell_kernel<double, W, CI>), grid(t), dim3(kRT), 0, s, t->n, S, ec, t->ellM,
+                           t->diagM, x, b, y);
       else
-        hipLaunchKernelGGL((spmv_ell_kernel<float, W, CI>), grid(t), dim3(kRT), 0, s, t->n, ec, t->ellM32,
-                           t->diagM32, t->ovf_rp, t->ovf_col, t->ovfM32, x, b, y);
+        hipLaunchKernelGGL((spmv_ell_kernel<float, W, CI>), grid(t), dim3(kRT), 0, s, t->n, S, ec, t->ellM32,
+                           t->diagM32, x, b, y);
     });
     return;
   }
@@ -1184,96 +1221,101 @@ void spmv_any(const fvm_template* t, int which, const double* x, const double* b
                        x, b, y);
 }
 
-// The ELL copy of M (values gathered from valM / valM32 after their assembly).
+// The sliced ELL copy of M and A (values gathered from val / valM / valM32 after
+// their assembly; layout at fvm_template).
 int build_ell(fvm_template* t, const std::vector<int32_t>& rp, const std::vector<int32_t>& col) {
-  const int64_t N = t->n;
-  std::vector<int64_t> over(kEllMax + 2, 0);  // over[w]: rows with more than w off-diagonals
+  const int64_t N = t->n, ns = (N + kSlice - 1) / kSlice;
+  std::vector<int32_t> deg(N, 0), dsrc(N, -1);
+  int dmax = 0;
   for (int64_t i = 0; i < N; ++i) {
-    int off = 0;
-    for (int k = rp[i]; k < rp[i + 1]; ++k) off += col[k] != i;
-    for (int w = kEllMin; w <= kEllMax; ++w) over[w] += off > w;
-  }
-  int W = kEllMax;
-  for (int w = kEllMin; w <= kEllMax; ++w)
-    if (over[w] * 100 <= N) {
-      W = w;
-      break;
-    }
-  std::vector<int32_t> ecol((size_t)W * N), esrc((size_t)W * N), dsrc(N, -1), orp(N + 1, 0), ocol, osrc;
-  for (int64_t i = 0; i < N; ++i) {
-    int sl = 0;
     for (int k = rp[i]; k < rp[i + 1]; ++k) {
-      if (col[k] == i) {
+      if (col[k] == i)
         dsrc[i] = k;
-      } else if (sl < W) {
-        ecol[(size_t)sl * N + i] = col[k];
-        esrc[(size_t)sl * N + i] = k;
-        ++sl;
-      } else {
-        ocol.push_back(col[k]);
-        osrc.push_back(k);
-      }
+      else
+        ++deg[i];
     }
-    for (; sl < W; ++sl) {
-      ecol[(size_t)sl * N + i] = (int32_t)i;
-      esrc[(size_t)sl * N + i] = -1;
-    }
-    orp[i + 1] = (int32_t)ocol.size();
+    dmax = std::max(dmax, deg[i]);
   }
-  const int64_t no = (int64_t)ocol.size();
-  // the slots' columns as 16-bit offsets from the row when every one fits
+  const int W = std::min(kEllMax, std::max(kEllMin, dmax));
+  const int64_t n_off = (int64_t)rp[N] - std::count_if(dsrc.begin(), dsrc.end(), [](int32_t k) { return k >= 0; });
+  FVM_CHECK(n_off < ((int64_t)1 << 31) - kSlice, "fvm_template_create: too many off-diagonal entries for the ELL copy");
+  std::vector<int32_t> off((size_t)(W + 1) * ns), rem_ptr(ns + 1, 0), rem_off, ecol, esrc, erow;
+  std::vector<uint8_t> perm(N);
+  ecol.reserve(n_off);
+  esrc.reserve(n_off);
+  erow.reserve(n_off);
   bool fits16 = true;
-  for (int s = 0; s < W && fits16; ++s)
-    for (int64_t i = 0; i < N; ++i) {
-      const int64_t d = (int64_t)ecol[(size_t)s * N + i] - i;
-      if (d < -32768 || d > 32767) {
-        fits16 = false;
-        break;
-      }
+  int64_t over = 0;
+  int lanes[kSlice], cur[kSlice];
+  for (int64_t b = 0; b < ns; ++b) {
+    const int64_t r0 = b * kSlice;
+    const int nr = (int)std::min<int64_t>(kSlice, N - r0);
+    for (int l = 0; l < nr; ++l) lanes[l] = l;
+    std::stable_sort(lanes, lanes + nr, [&](int x, int y) { return deg[r0 + x] > deg[r0 + y]; });
+    for (int l = 0; l < nr; ++l) {
+      perm[r0 + l] = (uint8_t)lanes[l];
+      cur[l] = rp[r0 + lanes[l]];
     }
-  int32_t *d_esrc = nullptr, *d_dsrc = nullptr, *d_osrc = nullptr;
+    const int smax = nr ? deg[r0 + lanes[0]] : 0;
+    for (int sl = 0; sl < smax; ++sl) {
+      if (sl <= W) off[(size_t)(W + 1) * b + sl] = (int32_t)ecol.size();
+      for (int l = 0; l < nr && deg[r0 + lanes[l]] > sl; ++l) {
+        const int64_t row = r0 + lanes[l];
+        int k = cur[l];
+        while (col[k] == row) ++k;  // the diagonal is not a slot
+        cur[l] = k + 1;
+        const int64_t d = (int64_t)col[k] - row;
+        fits16 = fits16 && d >= -32768 && d <= 32767;
+        ecol.push_back(col[k]);
+        esrc.push_back(k);
+        erow.push_back((int32_t)row);
+        over += sl >= W;
+      }
+      if (sl >= W) rem_off.push_back((int32_t)ecol.size());
+    }
+    for (int sl = smax; sl <= W; ++sl) off[(size_t)(W + 1) * b + sl] = (int32_t)ecol.size();
+    rem_ptr[b + 1] = (int32_t)rem_off.size();
+  }
+  const int64_t ne = (int64_t)ecol.size();
+  // a kSlice-entry tail: sell_slice reads up to 63 entries past a slot's last one
+  ecol.resize(ne + kSlice, 0);
+  erow.resize(ne + kSlice, 0);
+  esrc.resize(ne + kSlice, -1);
+  int32_t *d_esrc = nullptr, *d_dsrc = nullptr;
   int rc = FVM_OK;
   do {
+    if ((rc = dupload(&t->sl_off, off.data(), (int64_t)off.size())) ||
+        (rc = dupload(&t->sl_perm, perm.data(), N)) || (rc = dupload(&t->sl_rem_ptr, rem_ptr.data(), ns + 1)) ||
+        (rc = dupload(&t->sl_rem_off, rem_off.data(), (int64_t)rem_off.size())))
+      break;
     if (fits16) {
-      std::vector<int16_t> eoff((size_t)W * N);
-      for (int s = 0; s < W; ++s)
-        for (int64_t i = 0; i < N; ++i) eoff[(size_t)s * N + i] = (int16_t)(ecol[(size_t)s * N + i] - i);
-      if ((rc = dupload(&t->ell_off, eoff.data(), (int64_t)W * N))) break;
-    } else if ((rc = dupload(&t->ell_col, ecol.data(), (int64_t)W * N))) {
+      std::vector<int16_t> eoff(ne + kSlice);
+      for (int64_t e = 0; e < ne + kSlice; ++e) eoff[e] = (int16_t)(ecol[e] - erow[e]);
+      if ((rc = dupload(&t->ell_off, eoff.data(), ne + kSlice))) break;
+    } else if ((rc = dupload(&t->ell_col, ecol.data(), ne + kSlice))) {
       break;
     }
-    if ((rc = dupload(&t->ovf_rp, orp.data(), N + 1)) || (rc = dupload(&t->ovf_col, ocol.data(), no)))
+    const int64_t np = ne + kSlice;
+    if ((rc = dalloc(&t->ellM, np)) || (rc = dalloc(&t->ellM32, np)) || (rc = dalloc(&t->diagM, N)) ||
+        (rc = dalloc(&t->diagM32, N)) || (rc = dalloc(&t->ellA, np)) || (rc = dalloc(&t->diagA, N)))
       break;
-    if ((rc = dalloc(&t->ellM, (int64_t)W * N)) || (rc = dalloc(&t->ellM32, (int64_t)W * N)) ||
-        (rc = dalloc(&t->diagM, N)) || (rc = dalloc(&t->diagM32, N)) || (rc = dalloc(&t->ovfM, no)) ||
-        (rc = dalloc(&t->ovfM32, no)))
-      break;
-    if ((rc = dalloc(&t->ellA, (int64_t)W * N)) || (rc = dalloc(&t->diagA, N)) || (rc = dalloc(&t->ovfA, no)))
-      break;
-    if ((rc = dupload(&d_esrc, esrc.data(), (int64_t)W * N)) || (rc = dupload(&d_dsrc, dsrc.data(), N)) ||
-        (rc = dupload(&d_osrc, osrc.data(), no)))
-      break;
+    if ((rc = dupload(&d_esrc, esrc.data(), np)) || (rc = dupload(&d_dsrc, dsrc.data(), N))) break;
     auto g = [](int64_t m) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((m + 255) / 256, 4096))); };
-    hipLaunchKernelGGL(gather_vals, g((int64_t)W * N), dim3(256), 0, 0, (int64_t)W * N, d_esrc, t->valM, t->valM32,
-                       t->ellM, t->ellM32);
+    hipLaunchKernelGGL(gather_vals, g(np), dim3(256), 0, 0, np, d_esrc, t->valM, t->valM32, t->ellM, t->ellM32);
+    hipLaunchKernelGGL(gather_vals, g(np), dim3(256), 0, 0, np, d_esrc, t->val, (const float*)nullptr, t->ellA,
+                       (float*)nullptr);
     hipLaunchKernelGGL(gather_vals, g(N), dim3(256), 0, 0, N, d_dsrc, t->valM, t->valM32, t->diagM, t->diagM32);
-    if (no) hipLaunchKernelGGL(gather_vals, g(no), dim3(256), 0, 0, no, d_osrc, t->valM, t->valM32, t->ovfM, t->ovfM32);
-    hipLaunchKernelGGL(gather_vals, g((int64_t)W * N), dim3(256), 0, 0, (int64_t)W * N, d_esrc, t->val,
-                       (const float*)nullptr, t->ellA, (float*)nullptr);
     hipLaunchKernelGGL(gather_vals, g(N), dim3(256), 0, 0, N, d_dsrc, t->val, (const float*)nullptr, t->diagA,
                        (float*)nullptr);
-    if (no)
-      hipLaunchKernelGGL(gather_vals, g(no), dim3(256), 0, 0, no, d_osrc, t->val, (const float*)nullptr, t->ovfA,
-                         (float*)nullptr);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) rc = hip_fail(e, "fvm_template_create: ELL copy");
   } while (false);
-  for (void* p : {(void*)d_esrc, (void*)d_dsrc, (void*)d_osrc})
+  for (void* p : {(void*)d_esrc, (void*)d_dsrc})
     if (p) (void)hipFree(p);
   if (rc == FVM_OK) {
     t->ellW = W;
-    t->ell_overflow = no;
+    t->ell_overflow = over;
   }
   return rc;
 }
@@ -1680,13 +1722,11 @@ int template_tsit5_small(const fvm_template* t, const fvm_ode_opts* o, double* c
   if (!t || t->ellW <= 0 || o->n != t->n || o->n > kSmallTsit5Max) return FVM_ERR_STATE;
   SmallTsit5 a;
   a.n = o->n;
+  a.S = sell_idx(t);
   a.ecol = t->ell_col;
   a.eoff = t->ell_off;
   a.eval = t->ellA;
   a.diag = t->diagA;
-  a.orp = t->ovf_rp;
-  a.ocol = t->ovf_col;
-  a.oval = t->ovfA;
   a.b = t->b;
   for (int j = 0; j < 7; ++j) a.k[j] = k[j];
   a.utmp = utmp;

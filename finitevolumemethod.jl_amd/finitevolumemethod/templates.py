@@ -183,8 +183,9 @@ class DeviceTemplate:
                    "fvm_template_spmv")
 
     def ell_info(self, index_bytes=False):
-        """(slots per row of the ELL copy of the operator, off-diagonals in its overflow
-        list[, bytes of each slot's column index: 2 for 16-bit offsets, 4 for columns])."""
+        """(unrolled slots of the sliced ELL copy of the operator (its longest row's
+        off-diagonals, 4..10), off-diagonals past those slots[, bytes of each entry's
+        column index: 2 for 16-bit offsets, 4 for columns])."""
         w, o, ib = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
         _lib.check(_lib.hip_lib().fvm_template_ell(self._handle, ctypes.byref(w), ctypes.byref(o),
                                                    ctypes.byref(ib)), "fvm_template_ell")

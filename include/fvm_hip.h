@@ -250,11 +250,13 @@ typedef struct fvm_template_desc {
 int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_template** out);
 int fvm_template_destroy(fvm_template* t);
 int fvm_template_info(const fvm_template* t, int64_t* n, int64_t* nnz);
-/* The ELL copy of the operator used by the Krylov products and fvm_template_apply /
- * fvm_template_spmv: slots per row (0: none, CSR throughout), the number of
- * off-diagonals kept in the overflow list and the bytes of each slot's column index
- * (2: offsets from the row, when every one fits 16 bits; 4: columns).  No counterpart
- * in the reference (its operator is a SparseMatrixCSC handed to KLU). */
+/* The sliced ELL copy of the operator used by the Krylov products and
+ * fvm_template_apply / fvm_template_spmv (64-row slices, rows ordered by length inside
+ * a slice, no padding): its unrolled slots, the longest row's off-diagonal count
+ * clamped to 4..10 (0: no copy, CSR throughout), the number of off-diagonals past
+ * those slots and the bytes of each entry's column index (2: offsets from the row,
+ * when every one fits 16 bits; 4: columns).  No counterpart in the reference (its
+ * operator is a SparseMatrixCSC handed to KLU). */
 int fvm_template_ell(const fvm_template* t, int32_t* width, int64_t* overflow, int32_t* index_bytes);
 /* the assembled A (CSR, 0-based, int32) and b; any pointer may be NULL */
 int fvm_template_download(const fvm_template* t, int32_t* row_ptr, int32_t* col, double* val,

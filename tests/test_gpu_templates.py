@@ -361,8 +361,9 @@ def wheel_triangulation(k=40):
 @pytest.mark.parametrize("mesh", ["jittered", "wheel"])
 @pytest.mark.parametrize("alg", ALGS, ids=IDS)
 def test_ell_and_csr_forms_of_the_krylov_operator_agree(gpu, mesh, alg):
-    """The Krylov products on the ELL copy of M (one thread per row, overflow list
-    for long rows) and on CSR: both converge to the direct solution of the same
+    """The Krylov products on the sliced ELL copy of M (a wave per 64-row slice, rows
+    longer than the unrolled slots continued slot by slot) and on CSR: both
+    converge to the direct solution of the same
     A u = b (1e-8 relative), and their iteration counts agree within a few percent
     (the two forms round the products' sums differently)."""
     if mesh == "wheel":
@@ -389,6 +390,58 @@ def test_ell_and_csr_forms_of_the_krylov_operator_agree(gpu, mesh, alg):
     # the default form is the ELL one, and both are deterministic
     again = FVM.solve(prob, alg, rtol=1e-11)
     assert np.array_equal(again.u, sols["ell"].u)
+
+
+def _permuted(base, seed):
+    """The same triangulation with its nodes numbered in a random order."""
+    perm = np.random.default_rng(seed).permutation(base.num_points())  # new id of each old node
+    inv = np.argsort(perm)
+    return FVM.Triangulation(base.x[inv], base.y[inv], perm[base.triangles], perm[base.boundary_edges],
+                             base.boundary_segments)
+
+
+@pytest.mark.parametrize("mesh", ["jittered", "wheel", "permuted"])
+def test_sliced_ell_product_sums_each_row_in_csr_order(gpu, mesh):
+    """y = A x on the sliced ELL copy (rows ordered by length inside each 64-row
+    slice, no padding, rows longer than the unrolled slots continued slot by slot;
+    16-bit column offsets, or 32-bit columns for the permuted numbering) adds each
+    row's off-diagonal terms in CSR order and the diagonal term last: bit for bit
+    the sum formed here in that order.  Meshes: 7,200 nodes (a partial last
+    slice), a wheel whose hub has 40 neighbours, 38,000 nodes randomly renumbered."""
+    if mesh == "wheel":
+        tri = wheel_triangulation()
+    elif mesh == "permuted":
+        tri = _permuted(FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 200, 190, jitter=0.25, seed=3), 3)
+    else:
+        tri = FVM.jittered_rectangle(0.0, 1.0, 0.0, 1.0, 90, 80, jitter=0.25, seed=21)
+    m = FVM.FVMGeometry(tri)
+    BCs = FVM.BoundaryConditions(m, lambda x, y, t, u, p: 0.0 * x, FVM.Dirichlet)
+    prob = FVM.PoissonsEquation(m, BCs, diffusion_function=lambda x, y, p: 1.0 + 0.5 * np.sin(3 * x) ** 2,
+                                source_function=lambda x, y, p: np.cos(2 * x) - y)
+    W, n_over, ib = prob.problem.ell_info(index_bytes=True)
+    assert ib == (4 if mesh == "permuted" else 2)
+    rp, col, val, _ = prob.problem.download()
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    on_diag = col == rows
+    assert np.array_equal(np.bincount(rows[on_diag], minlength=n), np.ones(n))
+    pos = np.arange(len(col)) - rp[rows]
+    dpos = np.zeros(n, np.int64)
+    dpos[rows[on_diag]] = pos[on_diag]
+    slot = pos - (pos > dpos[rows])  # CSR position among the row's off-diagonals
+    deg = np.bincount(rows[~on_diag], minlength=n)
+    assert W == min(10, max(4, deg.max())) and n_over == np.maximum(deg - W, 0).sum()
+    x = np.random.default_rng(5).uniform(-1.0, 1.0, n)
+    acc = np.zeros(n)
+    for s in range(deg.max()):
+        sel = ~on_diag & (slot == s)
+        term = np.zeros(n)
+        term[rows[sel]] = val[sel] * x[col[sel]]
+        acc = acc + term
+    acc = acc + val[on_diag][np.argsort(rows[on_diag])] * x
+    y = FVM.DeviceArray(n)
+    prob.problem.spmv(0, FVM.DeviceArray.from_host(x), y)
+    assert np.array_equal(y.to_host(), acc)
 
 
 def test_unknown_spmv_format_is_rejected(gpu):
