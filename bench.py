@@ -364,7 +364,11 @@ def published_walkthroughs():
     78.761 ms), with the FVMProblem forms of both (:336-340, TRBDF2 with KLU: 49.237 ms;
     :451-455, CVODE_BDF with GMRES: 94.839 ms); poissons_equation.jl:92-95, 123-130, 168-180 (laplace u = -sin(pi x) sin(pi y) on
     a 100 x 100 grid of [0, 1]^2, u = 0 on the boundary, as the PoissonsEquation template:
-    15.442 ms with KLU, and as a SteadyFVMProblem: 189.406 ms)."""
+    15.442 ms with KLU, and as a SteadyFVMProblem: 189.406 ms); laplaces_equation.jl:161-222
+    (D = (x + 1)(y + 2) on a 100 x 100 grid of [0, 5]^2: 15.368 ms with KLU, 495.417 ms as a
+    SteadyFVMProblem); linear_reaction_diffusion_equations.jl:82-186 (150 x 150 grid of
+    [0, 1]^2 to t = 8, saveat 2: 48.360 ms for the template and 163.686 ms for the
+    FVMProblem, both with CVODE_BDF and GMRES)."""
     import finitevolumemethod as FVM
 
     def best(fn, stats=False):
@@ -452,6 +456,56 @@ def published_walkthroughs():
     out.append(dict(workload="the same Poisson problem as a SteadyFVMProblem, device Newton-GMRES "
                     "(published: DynamicSS(TRBDF2(KLU)))", ms=best(lambda: FVM.solve(steady)), published_ms=189.406,
                     source="docs/src/literate_wyos/poissons_equation.jl:176-180"))
+    # laplaces_equation.jl:161-222: D = (x + 1)(y + 2) on [0, 5]^2, u = 0 / 5 on x = 0 / 5,
+    # zero flux on y = 0, 5
+    tri = FVM.triangulate_rectangle(0.0, 5.0, 0.0, 5.0, 100, 100, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    zero_f = lambda x, y, t, u, p: 0.0 * x  # noqa: E731
+    five_f = lambda x, y, t, u, p: 5.0 + 0.0 * x  # noqa: E731
+    BCs = FVM.BoundaryConditions(mesh, (zero_f, five_f, zero_f, zero_f),
+                                 (FVM.Neumann, FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet))
+    lap = FVM.LaplacesEquation(mesh, BCs, diffusion_function=lambda x, y, p: (x + 1) * (y + 2))
+    out.append(dict(workload="LaplacesEquation 100x100 on [0,5]^2, D=(x+1)(y+2), Dirichlet / Neumann sides, "
+                             "device CG to 1e-10 (published: KLU)", ms=best(lambda: FVM.solve(lap)),
+                    published_ms=15.368, source="docs/src/literate_wyos/laplaces_equation.jl:161-215"))
+    lap_fvm = FVM.SteadyFVMProblem(FVM.FVMProblem(
+        mesh, BCs, diffusion_function=lambda x, y, t, u, p: (x + 1) * (y + 2) + 0.0 * u,
+        initial_condition=np.zeros(tri.num_points()), final_time=np.inf))
+    ms, _ = best(lambda: FVM.solve(lap_fvm), stats=True)
+    st = FVM.solve(lap_fvm).stats
+    out.append(dict(workload="the same as a SteadyFVMProblem, device Newton-GMRES with Jacobi preconditioning "
+                             "(published: DynamicSS(TRBDF2(KLU)))", ms=ms, published_ms=495.417,
+                    stats=dict(newton_iters=st["newton_iters"], nkrylov=st["nkrylov"]),
+                    source="docs/src/literate_wyos/laplaces_equation.jl:194-222"))
+    # linear_reaction_diffusion_equations.jl:82-184: dT/dt = div[1e-3 x^2 y grad T] + (x-1)(y-1) T
+    # on a 150 x 150 grid of [0, 1]^2, grad T . n = 1, T0 = x^2 + y^2, to t = 8, saveat 2
+    tri = FVM.triangulate_rectangle(0.0, 1.0, 0.0, 1.0, 150, 150, single_boundary=True)
+    mesh = FVM.FVMGeometry(tri)
+    Dfun = lambda x, y, p: 1.0e-3 * x ** 2 * y  # noqa: E731
+    Sfun = lambda x, y, p: (x - 1) * (y - 1)  # noqa: E731
+    ic = tri.x ** 2 + tri.y ** 2
+    lrd = FVM.LinearReactionDiffusionEquation(
+        mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: 1.0 + 0.0 * x, FVM.Neumann),
+        diffusion_function=Dfun, source_function=Sfun, initial_condition=ic, final_time=8.0)
+    out.append(dict(workload="LinearReactionDiffusionEquation 150x150 on [0,1]^2, D=1e-3 x^2 y, f=(x-1)(y-1), "
+                             "Neumann, to t=8, saveat 2 (ours: Tsit5, the tutorial's own solve; published: "
+                             "CVODE_BDF with GMRES)", ms=best(lambda: FVM.solve(lrd, FVM.Tsit5(), saveat=2.0)),
+                    published_ms=48.360, source="docs/src/literate_wyos/linear_reaction_diffusion_equations.jl:172-178"))
+
+    def make_lrd():
+        # the Neumann condition as q . n = -D (x, y) (:119-123)
+        return FVM.FVMProblem(mesh, FVM.BoundaryConditions(mesh, lambda x, y, t, u, p: -Dfun(x, y, None) + 0.0 * u,
+                                                           FVM.Neumann),
+                              diffusion_function=lambda x, y, t, u, p: Dfun(x, y, None) + 0.0 * u,
+                              source_function=lambda x, y, t, u, p: Sfun(x, y, None) * u,
+                              initial_condition=ic, final_time=8.0)
+
+    lrd_fvm = make_lrd()
+    ms, st = best(lambda: FVM.solve(lrd_fvm, FVM.TRBDF2(), saveat=2.0), stats=True)
+    out.append(dict(workload="the same as an FVMProblem, stiff (ours: TRBDF2 in one workgroup; published: "
+                             "CVODE_BDF with GMRES), saveat 2", ms=ms, first_solve_ms=first(make_lrd, 2.0),
+                    published_ms=163.686, stats=st,
+                    source="docs/src/literate_wyos/linear_reaction_diffusion_equations.jl:180-186"))
     log(f"published walkthroughs {out}")
     return out
 

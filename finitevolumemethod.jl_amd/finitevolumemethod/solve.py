@@ -672,13 +672,20 @@ def _initial_dt(f, u0, f0, t0, tf, abstol, reltol, n, exponent=1 / 5):
     return min(100 * dt0, dt1, tf - t0)
 
 
-def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kwargs):
+def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, precond="jacobi", **kwargs):
     """SteadyFVMProblem: Newton's method on fvm_eqs!(du, u) = 0 with the device
     Jacobian-vector product (fvm_jvp, fvm_system_jvp) inside restarted GMRES.  Dirichlet nodes are
     set by the Dirichlet functions first and stay fixed (their du is 0).  Stops when
-    |F| <= abstol + reltol |F0| and the last correction is below sqrt(reltol) (1 + |u|)."""
+    |F| <= abstol + reltol |F0| and the last correction is below sqrt(reltol) (1 + |u|).
+    precond="jacobi" (FVMProblem): GMRES right-preconditioned with diag(J)^-1, the
+    diagonal from coloured fvm_jvp probes (jacobian_diagonal), taken once for a problem
+    affine in u and at every Newton iterate otherwise (rows with a zero diagonal, the
+    Dirichlet nodes', scaled by 1); "none": plain GMRES."""
+    from .implicit import is_affine
     from .linsolve import gmres_device, library_arnoldi
 
+    if precond not in ("jacobi", "none"):
+        raise ValueError(f"solve: unknown precond {precond!r} (jacobi or none)")
     p = prob.problem
     ctx = p.device_context()
     sys = isinstance(p, FVMSystem)
@@ -696,6 +703,10 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
     it = 0
     rn = r0
     step = math.inf
+    nkrylov = 0
+    jdiag = jacobian_diagonal(p) if precond == "jacobi" and not sys else None
+    S = _lib.DeviceArray(n) if jdiag is not None else None
+    fixed_J = jdiag is not None and is_affine(p)
 
     def converged():
         # the residual has dropped, and the last Newton correction is small next to u
@@ -708,9 +719,14 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
         if converged():
             break
         it += 1
+        if jdiag is not None and (it == 1 or not fixed_J):
+            jdiag(u, t, du)
+            d = du.to_host()
+            S.upload(np.where(d != 0.0, 1.0 / np.where(d == 0.0, 1.0, d), 1.0))
         jv = lambda v, out: ctx.jvp(u, v, out, t)  # noqa: E731
-        gmres_device(jv, F, du, rtol=1e-3, restart=50, maxiter=2000,
-                     arnoldi=library_arnoldi(_lib.OP_SYSTEM if sys else _lib.OP_PROBLEM, ctx.handle, u, t, 0.0, 1.0))
+        nkrylov += gmres_device(jv, F, du, rtol=1e-3, restart=50, maxiter=2000, precond=S,
+                                arnoldi=library_arnoldi(_lib.OP_SYSTEM if sys else _lib.OP_PROBLEM, ctx.handle, u, t,
+                                                        0.0, 1.0))
         _lib.lincomb(u, (1.0, -1.0), (u, du))
         step = math.sqrt(_lib.dot(du, du))
         ctx.rhs(u, F, t)
@@ -718,5 +734,6 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, **kw
     sol.t = [math.inf]
     sol.u = [p.unflat(u.to_host()) if sys else u.to_host()]
     sol.retcode = "Success" if converged() else "MaxIters"
-    sol.stats = dict(newton_iters=it, residual=rn, initial_residual=r0, last_step=step)
+    sol.stats = dict(newton_iters=it, residual=rn, initial_residual=r0, last_step=step, nkrylov=nkrylov,
+                     precond=precond if jdiag is not None else "none")
     return sol

@@ -176,3 +176,33 @@ def test_poisson_template_equals_steady_problem(gpu):
     print("poisson template vs steady", np.linalg.norm(u1 - u2) / np.linalg.norm(u1), fvm_sol.stats)
     assert np.linalg.norm(u1 - exact) <= 1e-4 * np.linalg.norm(exact)
     assert np.linalg.norm(u2 - u1) <= 1e-4 * np.linalg.norm(u1)
+
+
+def test_laplace_walkthrough_template_and_steady_problem(gpu):
+    """laplaces_equation.jl:148-222: div[(x + 1)(y + 2) grad u] = 0 on [0, 5]^2 (100 x 100
+    grid, four boundary segments), u = 0 at x = 0, u = 5 at x = 5, zero flux on y = 0, 5;
+    exact u = 5 log(1 + x) / log 6.  The LaplacesEquation template (device CG) and the
+    SteadyFVMProblem (Newton-GMRES on fvm_jvp, Jacobi-preconditioned and plain) match
+    it to the tutorial's rtol 1e-3 (:194) and each other to the solvers' tolerances;
+    the preconditioner cuts the Krylov iterations (D varies 42-fold over the domain)."""
+    tri = FVM.triangulate_rectangle(0.0, 5.0, 0.0, 5.0, 100, 100, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    zero = lambda x, y, t, u, p: 0.0 * x  # noqa: E731
+    five = lambda x, y, t, u, p: 5.0 + 0.0 * x  # noqa: E731
+    BCs = FVM.BoundaryConditions(mesh, (zero, five, zero, zero),
+                                 (FVM.Neumann, FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet))
+    exact = 5.0 * np.log1p(tri.x) / np.log(6.0)
+    lap = FVM.LaplacesEquation(mesh, BCs, diffusion_function=lambda x, y, p: (x + 1) * (y + 2))
+    s_t = FVM.solve(lap)
+    assert np.linalg.norm(s_t.u - exact) <= 1e-3 * np.linalg.norm(exact)
+    prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: (x + 1) * (y + 2) + 0.0 * u,
+                          initial_condition=np.zeros(tri.num_points()), final_time=np.inf)
+    sols = {pc: FVM.solve(FVM.SteadyFVMProblem(prob), precond=pc) for pc in ("jacobi", "none")}
+    for pc, s in sols.items():
+        assert s.retcode == "Success" and s.stats["precond"] == pc, s.stats
+        u = s.u[-1]
+        assert np.linalg.norm(u - exact) <= 1e-3 * np.linalg.norm(exact), pc
+        assert np.linalg.norm(u - s_t.u) <= 1e-6 * np.linalg.norm(s_t.u), pc
+    assert sols["jacobi"].stats["nkrylov"] < sols["none"].stats["nkrylov"], {k: v.stats for k, v in sols.items()}
+    with pytest.raises(ValueError):
+        FVM.solve(FVM.SteadyFVMProblem(prob), precond="ilu")
