@@ -1579,6 +1579,72 @@ int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_templ
   return FVM_OK;
 }
 
+int fvm_template_create_csr(const fvm_mesh* m, int64_t n, const int32_t* row_ptr, const int32_t* col,
+                            const double* val, const double* b, const uint8_t* fixed, fvm_template** out) {
+  FVM_CHECK(m && row_ptr && col && val && b && fixed && out, "fvm_template_create_csr: null argument");
+  *out = nullptr;
+  FVM_CHECK(n == m->N && n > 0, "fvm_template_create_csr: n must be the mesh's number of points");
+  FVM_CHECK(row_ptr[0] == 0, "fvm_template_create_csr: row_ptr[0] must be 0");
+  for (int64_t i = 0; i < n; ++i)
+    FVM_CHECK(row_ptr[i + 1] >= row_ptr[i], "fvm_template_create_csr: row_ptr must not decrease");
+  const int64_t nnz = row_ptr[n];
+  std::vector<int32_t> rp(row_ptr, row_ptr + n + 1), cl(col, col + nnz);
+  std::vector<double> v(val, val + nnz);
+  std::vector<uint8_t> fx(fixed, fixed + n);
+  for (int64_t i = 0; i < n; ++i) {
+    bool diag = false;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      FVM_CHECK(cl[k] >= 0 && cl[k] < n, "fvm_template_create_csr: column out of range");
+      diag = diag || cl[k] == i;
+      if (fx[i]) v[k] = cl[k] == i ? 1.0 : 0.0;  // a fixed row is x_i = b_i
+    }
+    FVM_CHECK(diag, "fvm_template_create_csr: every row must hold its diagonal");
+    fx[i] = fx[i] != 0;
+  }
+  fvm_template* t = new fvm_template();
+  t->m = m;
+  t->kind = FVM_TEMPLATE_POISSON;  // a linear system A x = b
+  t->n = n;
+  t->nnz = nnz;
+  t->G = (int)std::max<int64_t>(1, std::min<int64_t>((n + kRT - 1) / kRT, kMaxG));
+  int rc = FVM_OK;
+  do {
+    if ((rc = dupload(&t->row_ptr, rp.data(), n + 1)) || (rc = dupload(&t->col, cl.data(), nnz)) ||
+        (rc = dupload(&t->val, v.data(), nnz)) || (rc = dupload(&t->b, b, n)) ||
+        (rc = dupload(&t->fixed, fx.data(), n)))
+      break;
+    if ((rc = dalloc(&t->valM, nnz)) || (rc = dalloc(&t->valM32, nnz)) || (rc = dalloc(&t->vol, n))) break;
+    if ((rc = dalloc(&t->dinv, n)) || (rc = dalloc(&t->wv, n)) || (rc = dalloc(&t->dinv32, n)) ||
+        (rc = dalloc(&t->wv32, n)))
+      break;
+    if ((rc = dalloc(&t->work, 10 * n)) || (rc = dalloc(&t->part, PT_COUNT * kMaxG)) ||
+        (rc = dalloc(&t->sc, S_COUNT)))
+      break;
+    hipError_t e = hipMemset(t->sc, 0, sizeof(double) * S_COUNT);
+    if (e == hipSuccess) e = hipMemcpy(t->vol, m->cv, sizeof(double) * n, hipMemcpyDeviceToDevice);
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(scaled_kernel, g, dim3(256), 0, 0, n, t->row_ptr, t->col, t->val, t->fixed, t->vol,
+                         t->valM, t->valM32, t->dinv, t->dinv32, t->wv, t->wv32);
+      hipLaunchKernelGGL(norm2_kernel, grid(t), dim3(kRT), 0, 0, n, t->b, t->part);
+      hipLaunchKernelGGL(finish_sum, dim3(1), dim3(kRT), 0, 0, t->G, t->part, t->sc, (int)S_B2);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "fvm_template_create_csr");
+      break;
+    }
+    rc = build_ell(t, rp, cl);
+  } while (false);
+  if (rc != FVM_OK) {
+    free_tmpl(t);
+    return rc;
+  }
+  *out = t;
+  return FVM_OK;
+}
+
 int fvm_template_destroy(fvm_template* t) {
   free_tmpl(t);
   return FVM_OK;

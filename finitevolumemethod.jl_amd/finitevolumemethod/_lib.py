@@ -200,6 +200,8 @@ HIP_API = {
     "fvm_jvp": [_vp, _vp, _vp, _vp, ctypes.c_double, _vp],
     "fvm_vjp": [_vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp],
     "fvm_template_create": [_vp, ctypes.POINTER(TemplateDesc), ctypes.POINTER(_vp)],
+    "fvm_template_create_csr": [_vp, ctypes.c_int64, _i32p, _i32p, _dp, _dp, ctypes.POINTER(ctypes.c_uint8),
+                                ctypes.POINTER(_vp)],
     "fvm_template_destroy": [_vp],
     "fvm_template_info": [_vp, _i64p, _i64p],
     "fvm_template_ell": [_vp, ctypes.POINTER(ctypes.c_int32), _i64p, ctypes.POINTER(ctypes.c_int32)],

@@ -672,7 +672,48 @@ def _initial_dt(f, u0, f0, t0, tf, abstol, reltol, n, exponent=1 / 5):
     return min(100 * dt0, dt1, tf - t0)
 
 
-def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, precond="jacobi", **kwargs):
+def _steady_affine(p, u, abstol, reltol, r0):
+    """F(u) = J u + c = 0 for an FVMProblem affine in u, as one linear system on the
+    device: J from coloured fvm_jvp probes (implicit.affine_operator, kept on the
+    problem), A = J with the rows whose F is identically zero (Dirichlet nodes, points
+    that are not vertices) made x_i = u_i, b = -c, solved by the template solvers'
+    mixed-precision CG (fvm_template_create_csr, fvm_template_solve) to
+    |b - A x| = |F(x)| <= abstol + reltol |F(u0)|.  Returns (x, stats), or None when the
+    solve stopped short (CG needs -diag(V) A close to symmetric positive definite;
+    the caller then runs Newton-GMRES from u)."""
+    from .implicit import affine_operator
+    from .templates import DeviceTemplate, MixedCG
+
+    A = affine_operator(p)
+    rp, col, val = A.csr
+    if getattr(A, "_fixed", None) is None:
+        n = len(rp) - 1
+        c = A.d_c.to_host()
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        fixed = np.bincount(rows, weights=np.abs(val), minlength=n) == 0
+        fixed &= c == 0.0
+        fixed[A.dir_nodes] = True
+        A._fixed, A._minus_c = fixed, -c
+    fixed = A._fixed
+    b = np.where(fixed, u.to_host(), A._minus_c)
+    key = (id(A), b.tobytes())
+    cached = getattr(p, "_steady_linear", None)
+    if cached is None or cached[0] != key:
+        cached = (key, DeviceTemplate.from_csr(p.mesh, rp, col, val, b, fixed))
+        p._steady_linear = cached
+    T = cached[1]
+    bn = float(np.linalg.norm(b))
+    rtol = max((abstol + reltol * r0) / bn, 1e-15) if bn > 0 else 1e-15
+    x = _lib.DeviceArray(len(b))
+    _lib.lincomb(x, (1.0,), (u,))
+    x, st = T.solve(MixedCG(), rtol=min(rtol, 1e-2), x=x)
+    if not st["converged"]:
+        return None
+    return x, st
+
+
+def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, precond="jacobi", linear="auto",
+                 **kwargs):
     """SteadyFVMProblem: Newton's method on fvm_eqs!(du, u) = 0 with the device
     Jacobian-vector product (fvm_jvp, fvm_system_jvp) inside restarted GMRES.  Dirichlet nodes are
     set by the Dirichlet functions first and stay fixed (their du is 0).  Stops when
@@ -680,12 +721,17 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, prec
     precond="jacobi" (FVMProblem): GMRES right-preconditioned with diag(J)^-1, the
     diagonal from coloured fvm_jvp probes (jacobian_diagonal), taken once for a problem
     affine in u and at every Newton iterate otherwise (rows with a zero diagonal, the
-    Dirichlet nodes', scaled by 1); "none": plain GMRES."""
+    Dirichlet nodes', scaled by 1); "none": plain GMRES.  linear="auto": an FVMProblem
+    affine in u is first solved as one linear system with the template solvers
+    (_steady_affine), Newton-GMRES only finishing what that leaves; "newton": Newton-GMRES
+    throughout."""
     from .implicit import is_affine
     from .linsolve import gmres_device, library_arnoldi
 
     if precond not in ("jacobi", "none"):
         raise ValueError(f"solve: unknown precond {precond!r} (jacobi or none)")
+    if linear not in ("auto", "newton"):
+        raise ValueError(f"solve: unknown linear {linear!r} (auto or newton)")
     p = prob.problem
     ctx = p.device_context()
     sys = isinstance(p, FVMSystem)
@@ -704,13 +750,20 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, prec
     rn = r0
     step = math.inf
     nkrylov = 0
-    jdiag = jacobian_diagonal(p) if precond == "jacobi" and not sys else None
-    S = _lib.DeviceArray(n) if jdiag is not None else None
-    fixed_J = jdiag is not None and is_affine(p)
+    affine = not sys and is_affine(p)
+    lin = None
+    if affine and linear == "auto" and rn > abstol:
+        lin = _steady_affine(p, u, abstol, reltol, r0)
+        if lin is not None:
+            u = lin[0]
+            ctx.rhs(u, F, t)
+            rn = math.sqrt(_lib.dot(F, F))
+    jdiag = S = None  # the Jacobi preconditioner, made when Newton-GMRES first runs
 
     def converged():
         # the residual has dropped, and the last Newton correction is small next to u
-        # (GMRES solves each correction to 1e-3 only)
+        # (GMRES solves each correction to 1e-3 only; a linear solve to the tolerance
+        # needs no correction)
         if rn > abstol + reltol * r0:
             return False
         return it == 0 or step <= math.sqrt(reltol) * (1.0 + math.sqrt(_lib.dot(u, u)))
@@ -719,7 +772,9 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, prec
         if converged():
             break
         it += 1
-        if jdiag is not None and (it == 1 or not fixed_J):
+        if it == 1 and precond == "jacobi" and not sys:
+            jdiag, S = jacobian_diagonal(p), _lib.DeviceArray(n)
+        if jdiag is not None and (it == 1 or not affine):
             jdiag(u, t, du)
             d = du.to_host()
             S.upload(np.where(d != 0.0, 1.0 / np.where(d == 0.0, 1.0, d), 1.0))
@@ -736,4 +791,7 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, prec
     sol.retcode = "Success" if converged() else "MaxIters"
     sol.stats = dict(newton_iters=it, residual=rn, initial_residual=r0, last_step=step, nkrylov=nkrylov,
                      precond=precond if jdiag is not None else "none")
+    if lin is not None:
+        sol.stats["linear_solve"] = dict(method="MixedCG on the extracted Jacobian",
+                                         iterations=lin[1]["iterations"], outer=lin[1]["outer"])
     return sol

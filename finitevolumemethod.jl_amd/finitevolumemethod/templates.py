@@ -157,6 +157,31 @@ class DeviceTemplate:
         self.nnz = nnz.value
         self._csr = None
 
+    @classmethod
+    def from_csr(cls, mesh, row_ptr, col, val, b, fixed):
+        """A x = b from host CSR arrays (fvm_template_create_csr): rows with fixed != 0
+        are x_i = b_i; the solvers and products of an assembled template apply."""
+        self = cls.__new__(cls)
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        cl = np.ascontiguousarray(col, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        bb = np.ascontiguousarray(b, np.float64)
+        fx = np.ascontiguousarray(fixed, np.uint8)
+        n = len(rp) - 1
+        h = ctypes.c_void_p()
+        _lib.check(_lib.hip_lib().fvm_template_create_csr(
+            mesh.handle, n, _lib.ptr(rp, _lib._i32p), _lib.ptr(cl, _lib._i32p), _lib.ptr(v), _lib.ptr(bb),
+            fx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), ctypes.byref(h)), "fvm_template_create_csr")
+        self._handle = h.value
+        self._mesh = mesh
+        self.kind = _lib.TEMPLATE_POISSON
+        self.n = n
+        self.node_kind = np.where(fx != 0, _lib.NODE_DIRICHLET, 0).astype(np.int8)
+        self.cond_val = np.where(fx != 0, bb, 0.0)
+        self.nnz = int(rp[-1])
+        self._csr = None
+        return self
+
     @property
     def handle(self):
         return self._handle

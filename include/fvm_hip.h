@@ -248,6 +248,17 @@ typedef struct fvm_template_desc {
 } fvm_template_desc;
 
 int fvm_template_create(const fvm_mesh* m, const fvm_template_desc* d, fvm_template** out);
+/* A linear system A x = b handed in as host CSR (0-based, int32 columns, every row
+ * holding its diagonal; rows with fixed[i] != 0 are taken as x_i = b_i), set up for
+ * fvm_template_solve and the products exactly like an assembled PoissonsEquation (the
+ * mesh gives the control-volume areas of the M = -diag(V) A scaling).  For the
+ * SteadyFVMProblem of an fvm_eqs! that is affine in u: A = its Jacobian (from
+ * fvm_jvp probes), b = -f(0), the Dirichlet nodes fixed -- the LinearProblem the
+ * reference builds for the same equation with a template (poissons_equation.jl:58-88,
+ * abstract_templates.jl:302-309), solved on the device instead of by Newton steps
+ * (solve.jl:185-195, 209-220 hand SteadyFVMProblems to NonlinearSolve). */
+int fvm_template_create_csr(const fvm_mesh* m, int64_t n, const int32_t* row_ptr, const int32_t* col,
+                            const double* val, const double* b, const uint8_t* fixed, fvm_template** out);
 int fvm_template_destroy(fvm_template* t);
 int fvm_template_info(const fvm_template* t, int64_t* n, int64_t* nnz);
 /* The sliced ELL copy of the operator used by the Krylov products and

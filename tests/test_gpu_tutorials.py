@@ -184,7 +184,10 @@ def test_laplace_walkthrough_template_and_steady_problem(gpu):
     exact u = 5 log(1 + x) / log 6.  The LaplacesEquation template (device CG) and the
     SteadyFVMProblem (Newton-GMRES on fvm_jvp, Jacobi-preconditioned and plain) match
     it to the tutorial's rtol 1e-3 (:194) and each other to the solvers' tolerances;
-    the preconditioner cuts the Krylov iterations (D varies 42-fold over the domain)."""
+    the preconditioner cuts the Krylov iterations (D varies 42-fold over the domain).
+    By default the affine steady problem is one linear solve on its extracted Jacobian
+    (no Newton step) to the steady tolerances (|F| <= 1e-8 + 1e-8 |F0|), which agrees
+    with the template's solution to 1e-6 like the Newton runs."""
     tri = FVM.triangulate_rectangle(0.0, 5.0, 0.0, 5.0, 100, 100, single_boundary=False)
     mesh = FVM.FVMGeometry(tri)
     zero = lambda x, y, t, u, p: 0.0 * x  # noqa: E731
@@ -197,12 +200,18 @@ def test_laplace_walkthrough_template_and_steady_problem(gpu):
     assert np.linalg.norm(s_t.u - exact) <= 1e-3 * np.linalg.norm(exact)
     prob = FVM.FVMProblem(mesh, BCs, diffusion_function=lambda x, y, t, u, p: (x + 1) * (y + 2) + 0.0 * u,
                           initial_condition=np.zeros(tri.num_points()), final_time=np.inf)
-    sols = {pc: FVM.solve(FVM.SteadyFVMProblem(prob), precond=pc) for pc in ("jacobi", "none")}
+    sols = {pc: FVM.solve(FVM.SteadyFVMProblem(prob), precond=pc, linear="newton") for pc in ("jacobi", "none")}
     for pc, s in sols.items():
         assert s.retcode == "Success" and s.stats["precond"] == pc, s.stats
         u = s.u[-1]
         assert np.linalg.norm(u - exact) <= 1e-3 * np.linalg.norm(exact), pc
         assert np.linalg.norm(u - s_t.u) <= 1e-6 * np.linalg.norm(s_t.u), pc
     assert sols["jacobi"].stats["nkrylov"] < sols["none"].stats["nkrylov"], {k: v.stats for k, v in sols.items()}
+    lin = FVM.solve(FVM.SteadyFVMProblem(prob))
+    assert lin.retcode == "Success" and lin.stats["newton_iters"] == 0 and "linear_solve" in lin.stats, lin.stats
+    assert np.linalg.norm(lin.u[-1] - exact) <= 1e-3 * np.linalg.norm(exact)
+    assert np.linalg.norm(lin.u[-1] - s_t.u) <= 1e-6 * np.linalg.norm(s_t.u)
     with pytest.raises(ValueError):
         FVM.solve(FVM.SteadyFVMProblem(prob), precond="ilu")
+    with pytest.raises(ValueError):
+        FVM.solve(FVM.SteadyFVMProblem(prob), linear="lu")
