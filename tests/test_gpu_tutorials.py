@@ -215,3 +215,46 @@ def test_laplace_walkthrough_template_and_steady_problem(gpu):
         FVM.solve(FVM.SteadyFVMProblem(prob), precond="ilu")
     with pytest.raises(ValueError):
         FVM.solve(FVM.SteadyFVMProblem(prob), linear="lu")
+
+
+def test_capacitor_walkthrough_template_and_steady_problem(gpu):
+    """poissons_equation.jl:185-382, the parallel-plate capacitor on [0, 10]^2 (here a
+    51 x 51 grid on which the plates are grid lines): internal Dirichlet nodes V = -1 on
+    2 <= x <= 8, y = 3 and V = 1 on y = 7, eps = 1 + (eps0 - 1)(1 + erf(r / 4)) / 2 and the
+    source -Q exp(-r^2 / 2) / (2 pi eps0) of the distance r to the plates, Dirichlet 0 at
+    the bottom and top, zero flux at the sides.  The PoissonsEquation template and the
+    SteadyFVMProblem (one linear solve on the extracted Jacobian by default, Newton-GMRES
+    on request) agree to 1e-6 and hold the plate and boundary values exactly."""
+    from scipy.special import erf
+
+    def dist(x, y):
+        dx = np.asarray(x) - np.clip(x, 2.0, 8.0)
+        return np.minimum(np.hypot(dx, np.asarray(y) - 3.0), np.hypot(dx, np.asarray(y) - 7.0))
+
+    eps0, Q = 8.8541878128e-12, 1.0e-6
+    diel = lambda x, y, p: 1 + (eps0 - 1) * (1 + erf(dist(x, y) / 4.0)) / 2  # noqa: E731
+    src = lambda x, y, p: -(Q / (2 * np.pi) * np.exp(-dist(x, y) ** 2 / 2)) / eps0  # noqa: E731
+    tri = FVM.triangulate_rectangle(0.0, 10.0, 0.0, 10.0, 51, 51, single_boundary=False)
+    mesh = FVM.FVMGeometry(tri)
+    zero = lambda x, y, t, u, p: 0.0 * x  # noqa: E731
+    BCs = FVM.BoundaryConditions(mesh, (zero, zero, zero, zero), (FVM.Dirichlet, FVM.Neumann, FVM.Dirichlet, FVM.Neumann))
+    on = (tri.x >= 2.0 - 1e-9) & (tri.x <= 8.0 + 1e-9)
+    lower = np.nonzero(on & (np.abs(tri.y - 3.0) < 1e-9))[0]
+    upper = np.nonzero(on & (np.abs(tri.y - 7.0) < 1e-9))[0]
+    assert len(lower) == len(upper) == 31
+    ICs = FVM.InternalConditions((lambda x, y, t, u, p: -1.0 + 0.0 * x, lambda x, y, t, u, p: 1.0 + 0.0 * x),
+                                 dirichlet_nodes={**{int(i): 0 for i in lower}, **{int(i): 1 for i in upper}})
+    tmpl = FVM.solve(FVM.PoissonsEquation(mesh, BCs, ICs, diffusion_function=diel, source_function=src))
+    prob = FVM.FVMProblem(mesh, BCs, ICs, diffusion_function=lambda x, y, t, u, p: diel(x, y, None) + 0.0 * u,
+                          source_function=lambda x, y, t, u, p: -src(x, y, None) + 0.0 * u,
+                          initial_condition=np.zeros(tri.num_points()), final_time=np.inf)
+    for linear in ("auto", "newton"):
+        s = FVM.solve(FVM.SteadyFVMProblem(prob), linear=linear)
+        assert s.retcode == "Success", (linear, s.stats)
+        assert ("linear_solve" in s.stats) == (linear == "auto"), s.stats
+        u = s.u[-1]
+        assert np.linalg.norm(u - tmpl.u) <= 1e-6 * np.linalg.norm(tmpl.u), linear
+        assert np.all(u[lower] == -1.0) and np.all(u[upper] == 1.0)
+        bottom_top = np.nonzero((tri.y == 0.0) | (tri.y == 10.0))[0]
+        assert np.all(u[bottom_top] == 0.0)
+    assert np.all(tmpl.u[lower] == -1.0) and np.all(tmpl.u[upper] == 1.0)
