@@ -678,11 +678,12 @@ def _steady_affine(p, u, abstol, reltol, r0):
     problem), A = J with the rows whose F is identically zero (Dirichlet nodes, points
     that are not vertices) made x_i = u_i, b = -c, solved by the template solvers'
     mixed-precision CG (fvm_template_create_csr, fvm_template_solve) to
-    |b - A x| = |F(x)| <= abstol + reltol |F(u0)|.  Returns (x, stats), or None when the
-    solve stopped short (CG needs -diag(V) A close to symmetric positive definite;
-    the caller then runs Newton-GMRES from u)."""
+    |b - A x| = |F(x)| <= abstol + reltol |F(u0)|, or, when CG stops short (it needs
+    -diag(V) A close to symmetric positive definite, which a reaction term can spoil),
+    by their BiCGStab.  Returns (x, stats, method), or None when neither converged (the
+    caller then runs Newton-GMRES from u)."""
     from .implicit import affine_operator
-    from .templates import DeviceTemplate, MixedCG
+    from .templates import BiCGStab, DeviceTemplate, MixedCG
 
     A = affine_operator(p)
     rp, col, val = A.csr
@@ -704,12 +705,13 @@ def _steady_affine(p, u, abstol, reltol, r0):
     T = cached[1]
     bn = float(np.linalg.norm(b))
     rtol = max((abstol + reltol * r0) / bn, 1e-15) if bn > 0 else 1e-15
-    x = _lib.DeviceArray(len(b))
-    _lib.lincomb(x, (1.0,), (u,))
-    x, st = T.solve(MixedCG(), rtol=min(rtol, 1e-2), x=x)
-    if not st["converged"]:
-        return None
-    return x, st
+    for alg in (MixedCG(), BiCGStab()):
+        x = _lib.DeviceArray(len(b))
+        _lib.lincomb(x, (1.0,), (u,))
+        x, st = T.solve(alg, rtol=min(rtol, 1e-2), x=x, maxiter=20 * len(b) + 1000)
+        if st["converged"]:
+            return x, st, f"{type(alg).__name__} on the extracted Jacobian"
+    return None
 
 
 def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, precond="jacobi", linear="auto",
@@ -792,6 +794,5 @@ def solve_steady(prob, alg=None, *, abstol=1e-8, reltol=1e-8, maxiters=200, prec
     sol.stats = dict(newton_iters=it, residual=rn, initial_residual=r0, last_step=step, nkrylov=nkrylov,
                      precond=precond if jdiag is not None else "none")
     if lin is not None:
-        sol.stats["linear_solve"] = dict(method="MixedCG on the extracted Jacobian",
-                                         iterations=lin[1]["iterations"], outer=lin[1]["outer"])
+        sol.stats["linear_solve"] = dict(method=lin[2], iterations=lin[1]["iterations"], outer=lin[1]["outer"])
     return sol
