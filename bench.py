@@ -370,7 +370,9 @@ def published_walkthroughs():
     [0, 1]^2 to t = 8, saveat 2: 48.360 ms for the template and 163.686 ms for the
     FVMProblem, both with CVODE_BDF and GMRES); poissons_equation.jl:185-382 (the
     parallel-plate capacitor: 9.061 ms with KLU, 329.327 ms as a SteadyFVMProblem), on a
-    grid of comparable size instead of the reference's Delaunay-refined mesh."""
+    grid of comparable size instead of the reference's Delaunay-refined mesh;
+    mean_exit_time.jl:89-232 (the perturbed annulus: 2.559 ms with KLU, 221.851 ms as a
+    SteadyFVMProblem with KenCarp47), on a ring mesh of the same domain."""
     import finitevolumemethod as FVM
 
     def best(fn, stats=False):
@@ -520,6 +522,26 @@ def published_walkthroughs():
                     stats=dict(newton_iters=st["newton_iters"], linear_solve=st.get("linear_solve")),
                     mesh_note="comparable mesh, not the reference's",
                     source="docs/src/literate_wyos/poissons_equation.jl:352-382"))
+    # mean_exit_time.jl:89-232: the annulus 1 <= r <= 3 with a perturbed interface between
+    # D = 6.25e-4 and 6.25e-5, T = 0 on a short outer arc, the hole and two point holes, zero
+    # flux on the rest (workloads.met_problems: rings and the interface points, Delaunay,
+    # instead of the reference's constrained refinement to max_area 1e-3 of the area)
+    from finitevolumemethod import workloads as WL
+
+    tri_m, met, met_steady = WL.met_problems()
+    out.append(dict(workload="MeanExitTimeProblem on the perturbed annulus with point holes (2813-point ring mesh; "
+                             "the reference refines a constrained Delaunay mesh), device CG to 1e-10 "
+                             "(published: KLU)", ms=best(lambda: FVM.solve(met)), published_ms=2.559,
+                    mesh_note="comparable mesh, not the reference's", nodes=int(tri_m.num_points()),
+                    source="docs/src/literate_wyos/mean_exit_time.jl:89-224"))
+    ms = best(lambda: FVM.solve(met_steady))
+    st = FVM.solve(met_steady).stats
+    out.append(dict(workload="the same as a SteadyFVMProblem (ours: one device linear solve on the Jacobian; "
+                             "published: DynamicSS(KenCarp47(KLU)))", ms=ms,
+                    first_solve_ms=first_steady(lambda: WL.met_problems()[2]), published_ms=221.851,
+                    stats=dict(newton_iters=st["newton_iters"], linear_solve=st.get("linear_solve")),
+                    mesh_note="comparable mesh, not the reference's",
+                    source="docs/src/literate_wyos/mean_exit_time.jl:160-232"))
     # laplaces_equation.jl:161-222: D = (x + 1)(y + 2) on [0, 5]^2, u = 0 / 5 on x = 0 / 5,
     # zero flux on y = 0, 5
     tri = FVM.triangulate_rectangle(0.0, 5.0, 0.0, 5.0, 100, 100, single_boundary=False)

@@ -337,11 +337,27 @@ def _power_law(fn):
         return None
 
 
-def classify_diffusion(fn):
+def _poly_holds(fn, poly, x, y):
+    """Whether D(x, y, 0, u) = d0 + d1 u + d2 u^2 + d3 u^3 at every point (x, y), for a
+    few values of u, to rounding: a trace at the probe points cannot see x, y
+    dependence that is constant there (a D that changes only in part of the domain)."""
+    for uk in (0.0, 0.75, -1.25, 2.5):
+        v = np.broadcast_to(np.asarray(fn(x, y, 0.0, uk), np.float64), x.shape)
+        terms = np.abs([poly[0], poly[1] * uk, poly[2] * uk ** 2, poly[3] * uk ** 3])
+        ref = poly[0] + uk * (poly[1] + uk * (poly[2] + uk * poly[3]))
+        if not np.all(np.abs(v - ref) <= 1e-13 * (terms.sum() + np.abs(v))):
+            return False
+    return True
+
+
+def classify_diffusion(fn, x=None, y=None):
     """Trace D(x, y, t, u, p) (already bound to p).  Returns ('poly', [d0..d3]) for
     a constant-coefficient cubic in u, ('power', (d0, c, p)) for d0 + c |u|^p, or
     ('xy', None) when D depends on (x, y) only (the caller evaluates it at the
-    CV-edge points)."""
+    CV-edge points).  The trace runs at fixed probe points; given the mesh's points
+    (x, y), a constant-coefficient result is checked there too, and D found to vary
+    over them is ('xy', None) if it does not depend on u, else unsupported (a
+    closure mixing (x, y) and u goes to the compiled path)."""
     probe_x = np.array([0.1234, 0.7421, 0.3317, 0.9012])
     probe_y = np.array([0.5531, 0.0713, 0.8822, 0.2417])
     try:
@@ -367,6 +383,10 @@ def classify_diffusion(fn):
         else:
             poly[ku] = c
     if depends_xy:
+        if any(poly[1:]):
+            raise UnsupportedClosure("diffusion mixing (x, y) and u dependence is not supported")
+        return "xy", None
+    if x is not None and len(x) and not _poly_holds(fn, poly, np.asarray(x, np.float64), np.asarray(y, np.float64)):
         if any(poly[1:]):
             raise UnsupportedClosure("diffusion mixing (x, y) and u dependence is not supported")
         return "xy", None

@@ -274,7 +274,7 @@ def physics_of(prob):
     if (prob.user_flux_function is None and D is not None and not isinstance(D, forms.NodalDiffusion)):
         Dp, S, Sp = prob.diffusion_parameters, prob.source_function, prob.source_parameters
         try:
-            forms.classify_diffusion(lambda x, y, t, u: D(x, y, t, u, Dp))
+            forms.classify_diffusion(lambda x, y, t, u: D(x, y, t, u, Dp), tri.x, tri.y)
             forms.classify_source(lambda x, y, t, u: S(x, y, t, u, Sp), tri.x, tri.y)
         except forms.UnsupportedClosure:
             ph.update(_jit_physics(prob))
@@ -292,7 +292,7 @@ def physics_of(prob):
                 raise ValueError("FVMProblem needs a diffusion_function or a flux_function")
             Dp = prob.diffusion_parameters
             Dfun = lambda x, y, t, u: D(x, y, t, u, Dp)  # noqa: E731
-            kind, poly = forms.classify_diffusion(Dfun)
+            kind, poly = forms.classify_diffusion(Dfun, tri.x, tri.y)
         if kind == "poly":
             ph.update(flux_kind=_lib.FLUX_D_POLY_U, dpoly=poly)
         elif kind == "power":

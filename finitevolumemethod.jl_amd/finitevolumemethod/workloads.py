@@ -105,3 +105,86 @@ def ensemble_members(P=256, seed=SEED + 5, final_time=1.0):
                                 initial_condition=u0, final_time=final_time))
         Ds.append(D)
     return probs, Ds
+
+
+def met_interface_radius(theta):
+    """R1(theta) = 2 (1 + 0.05 (sin 3 theta + cos 5 theta)): the interface between the two
+    diffusivities of the mean-exit-time walkthrough (mean_exit_time.jl:89-93)."""
+    return 2.0 * (1.0 + 0.05 * (np.sin(3 * theta) + np.cos(5 * theta)))
+
+
+def met_annulus_triangulation(h=0.1):
+    """The mean-exit-time walkthrough's domain (mean_exit_time.jl:89-119), the annulus
+    1 <= r <= 3, meshed here without DelaunayTriangulation's constrained refinement:
+    the two point holes (-2, 0) and (0, 2.95) as points 0 and 1; 249 points on the
+    interface r = R1(theta) (the reference's LinRange(0, 2 pi, 250) less its last);
+    rings of points about h apart (ring points closer than h / 3 to those dropped), the
+    outer ring holding the arc ends theta = +-0.25; Delaunay (Qhull), the triangles of
+    the hole removed.  Boundary segments in the reference's order: 0 the outer arc
+    0.25 <= theta <= 2 pi - 0.25, 1 the outer arc |theta| < 0.25, 2 the hole."""
+    from scipy.spatial import cKDTree
+
+    from .mesh import Triangulation, triangulate
+
+    fixed = [(-2.0, 0.0), (0.0, 2.95)]
+    th = np.linspace(0.0, 2 * np.pi, 250)[:-1]
+    R = met_interface_radius(th)
+    fixed += list(zip(R * np.cos(th), R * np.sin(th)))
+    F = np.array(fixed)
+    ring = []
+    nr = int(round(2.0 / h))
+    for k in range(nr + 1):
+        r = 1.0 + 2.0 * k / nr
+        if k == nr:
+            m1 = max(2, int(round(0.5 * r / h)))
+            m2 = max(4, int(round((2 * np.pi - 0.5) * r / h)))
+            a = np.concatenate([np.linspace(-0.25, 0.25, m1 + 1), np.linspace(0.25, 2 * np.pi - 0.25, m2 + 1)[1:-1]])
+        else:
+            m = max(12, int(round(2 * np.pi * r / h)))
+            a = 2 * np.pi * np.arange(m) / m + 0.37 * k
+        ring += list(zip(r * np.cos(a), r * np.sin(a)))
+    Rg = np.array(ring)
+    rr = np.hypot(Rg[:, 0], Rg[:, 1])
+    keep = (cKDTree(F).query(Rg)[0] >= h / 3) | (rr <= 1.0 + 1e-12) | (rr >= 3.0 - 1e-12)
+    P = np.concatenate([F, Rg[keep]])
+    full = triangulate(P[:, 0], P[:, 1])
+    T = full.triangles
+    T = T[np.hypot(P[T, 0].mean(1), P[T, 1].mean(1)) > 1.0]
+    count = {}
+    for a, b, c in T:
+        for e in ((a, b), (b, c), (c, a)):
+            count[e] = count.get(e, 0) + 1
+    bedges = np.array(sorted(e for e in count if (e[1], e[0]) not in count), np.int32)
+    mx, my = P[bedges, 0].mean(1), P[bedges, 1].mean(1)
+    seg = np.where(np.hypot(mx, my) < 2.0, 2, np.where(np.abs(np.arctan2(my, mx)) < 0.25, 1, 0)).astype(np.int32)
+    return Triangulation(P[:, 0], P[:, 1], T, bedges, seg)
+
+
+def met_problems(h=0.1):
+    """(tri, template, steady): the mean-exit-time walkthrough (mean_exit_time.jl:89-180)
+    as the MeanExitTimeProblem template and as the SteadyFVMProblem of D grad^2 T = -1:
+    D = 6.25e-4 inside the interface and 6.25e-5 outside, zero flux on the long outer
+    arc, T = 0 on the short outer arc, the hole and the two point holes, the initial
+    guess the radially symmetric T_exact (:163-170)."""
+    from .conditions import InternalConditions, Neumann
+    from .problem import SteadyFVMProblem
+    from .templates import MeanExitTimeProblem
+
+    tri = met_annulus_triangulation(h)
+    mesh = FVMGeometry(tri)
+    zero = lambda x, y, t, u, p: 0.0 * x  # noqa: E731
+    BCs = BoundaryConditions(mesh, (zero, zero, zero), (Neumann, Dirichlet, Dirichlet))
+    ICs = InternalConditions(zero, dirichlet_nodes={0: 0, 1: 0})
+    D1, D2 = 6.25e-4, 6.25e-5
+
+    def D(x, y, p=None):
+        return np.where(np.hypot(x, y) < met_interface_radius(np.arctan2(y, x)), D1, D2)
+
+    met = MeanExitTimeProblem(mesh, BCs, ICs, diffusion_function=D)
+    r = np.hypot(tri.x, tri.y)
+    R1, R2 = 2.0, 3.0
+    guess = np.where(r < R1, (R1 ** 2 - r ** 2) / (4 * D1) + (R2 ** 2 - R1 ** 2) / (4 * D2), (R2 ** 2 - r ** 2) / (4 * D2))
+    steady = SteadyFVMProblem(FVMProblem(mesh, BCs, ICs, diffusion_function=lambda x, y, t, u, p: D(x, y) + 0.0 * u,
+                                         source_function=lambda x, y, t, u, p: 1.0 + 0.0 * u,
+                                         initial_condition=guess, final_time=np.inf))
+    return tri, met, steady

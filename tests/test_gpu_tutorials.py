@@ -262,3 +262,26 @@ def test_capacitor_walkthrough_template_and_steady_problem(gpu):
         bottom_top = np.nonzero((tri.y == 0.0) | (tri.y == 10.0))[0]
         assert np.all(u[bottom_top] == 0.0)
     assert np.all(tmpl.u[lower] == -1.0) and np.all(tmpl.u[upper] == 1.0)
+
+
+def test_mean_exit_time_walkthrough_template_and_steady_problem(gpu):
+    """mean_exit_time.jl:89-215 on a ring mesh of its domain (workloads.met_problems):
+    the MeanExitTimeProblem template and the SteadyFVMProblem of D grad^2 T = -1 agree
+    (the reference asserts rtol 1e-2 at :206, here 1e-6), T = 0 at the absorbing
+    boundaries and the two point holes, T > 0 elsewhere, at the scale of the reference's
+    figure."""
+    from finitevolumemethod import workloads as WL
+
+    tri, met, steady = WL.met_problems()
+    a = FVM.solve(met)
+    s = FVM.solve(steady)
+    assert s.retcode == "Success" and "linear_solve" in s.stats, s.stats
+    T = s.u[-1]
+    assert np.linalg.norm(T - a.u) <= 1e-6 * np.linalg.norm(a.u)
+    assert T[0] == 0.0 and T[1] == 0.0
+    seg = tri.boundary_segments
+    absorbing = np.unique(tri.boundary_edges[seg >= 1])
+    assert np.all(T[absorbing] == 0.0)
+    free = np.setdiff1d(np.arange(tri.num_points()), np.concatenate([absorbing, [0, 1]]))
+    assert np.all(T[free] > 0.0)
+    assert 1e3 < T.max() < 1e5  # the scale of the reference's figure (contours 0:1000:15000, :147)

@@ -194,6 +194,25 @@ def test_condition_merging_matches_restatement():
         FVM.BoundaryConditions(mesh, (f,) * 3, types[:3])
 
 
+def test_diffusion_varying_only_away_from_the_probe_points_is_not_taken_as_constant():
+    """The trace runs at fixed probe points inside [0, 1]^2; a D(x, y) that is constant
+    there but not over the mesh (the mean-exit-time walkthrough's D, 6.25e-4 inside
+    r = R1(theta) ~ 2 and 6.25e-5 outside) is checked at the mesh's points: it becomes the
+    evaluated-per-edge form, and a D mixing such (x, y) dependence with u is left to the
+    compiled path.  Constant-coefficient closures still pass the check."""
+    th = np.linspace(0.0, 2 * np.pi, 200)
+    r = np.linspace(1.0, 3.0, 200)
+    xs, ys = r * np.cos(th), r * np.sin(th)
+    piecewise = lambda x, y, t, u: np.where(np.hypot(x, y) < 2.0, 6.25e-4, 6.25e-5)  # noqa: E731
+    assert forms.classify_diffusion(piecewise) == ("poly", [6.25e-4, 0.0, 0.0, 0.0])  # the probes alone
+    assert forms.classify_diffusion(piecewise, xs, ys) == ("xy", None)
+    with pytest.raises(forms.UnsupportedClosure):
+        forms.classify_diffusion(lambda x, y, t, u: np.where(np.hypot(x, y) < 2.0, 1.0, 2.0) * u, xs, ys)
+    assert forms.classify_diffusion(lambda x, y, t, u: 0.1 + u ** 2, xs, ys) == ("poly", [0.1, 0.0, 1.0, 0.0])
+    assert forms.classify_diffusion(lambda x, y, t, u: 1 / 9 + 0.0 * u, xs, ys) == ("poly", [1 / 9, 0.0, 0.0, 0.0])
+    assert forms.classify_diffusion(lambda x, y, t, u: 0.5 * u ** 3 - u + 2.0, xs, ys) == ("poly", [2.0, -1.0, 0.0, 0.5])
+
+
 def test_closure_tracing_forms():
     assert forms.classify_diffusion(lambda x, y, t, u: 0.1 + u ** 2) == ("poly", [0.1, 0.0, 1.0, 0.0])
     assert forms.classify_diffusion(lambda x, y, t, u: 1 / 9) == ("poly", [1 / 9, 0.0, 0.0, 0.0])
