@@ -293,6 +293,14 @@ __global__ __launch_bounds__(256) void scaled_kernel(int64_t N, const int32_t* _
   wv32[r] = (float)w;
 }
 
+// The rows of a solver kernel's workgroup: with a grid that is a multiple of the XCD
+// count, each XCD takes contiguous ranges of rows (xcd_remap), the same in every
+// solver kernel, so that a row's neighbours, gathered by nearby rows, and the vectors
+// the previous kernel wrote for them tend to be in that XCD's own L2.
+__device__ __forceinline__ int64_t solver_wid() {
+  return gridDim.x % kXCD == 0 ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : (int64_t)blockIdx.x;
+}
+
 // ----------------------------------------------------------------------------
 // reductions
 // ----------------------------------------------------------------------------
@@ -423,7 +431,7 @@ __global__ __launch_bounds__(kRT) void cg_init(int64_t n, const V* __restrict__ 
                                                V* __restrict__ p, const V* __restrict__ keep, double* sc,
                                                double* part) {
   double rz = 0.0, wr = 0.0, rk = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+  for (int64_t i = solver_wid() * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     const V ri = r[i];
     const V zi = dinv[i] * ri;
     z[i] = zi;
@@ -575,7 +583,7 @@ __device__ __forceinline__ void sell_rows(int64_t n, const SellIdx& S, const CI*
   constexpr int kWaves = kRT / kSlice;
   const int64_t ns = (n + kSlice - 1) / kSlice;
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kSlice);
-  for (int64_t b = (int64_t)blockIdx.x * kWaves + w; b < ns; b += (int64_t)gridDim.x * kWaves)
+  for (int64_t b = solver_wid() * kWaves + w; b < ns; b += (int64_t)gridDim.x * kWaves)
     sell_slice<DIFF, W>(n, b, S, ecol, eval, diag, g, body);
 }
 
@@ -631,7 +639,7 @@ __global__ __launch_bounds__(kRT) void cg_update(int64_t n, int k, V* __restrict
   }
   const V alpha = (V)(sc[S_RZ + (k & 1)] / pq);
   double rz = 0.0, wr = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+  for (int64_t i = solver_wid() * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     if (x64) x64[i] = x64[i] + (double)alpha * (double)p[i];
     else d[i] = d[i] + alpha * p[i];
     const V ri = r[i] - alpha * q[i];
@@ -656,7 +664,7 @@ __global__ __launch_bounds__(kRT) void bi_init(int64_t n, const double* __restri
                                                double* __restrict__ rhat, double* __restrict__ p,
                                                double* __restrict__ v, double* sc, double* part) {
   double rho = 0.0, wr = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+  for (int64_t i = solver_wid() * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     const double ri = r[i];
     rhat[i] = ri;
     d[i] = 0.0;
@@ -710,7 +718,7 @@ __global__ __launch_bounds__(kRT) void bi_dir(int64_t n, int k, double tabs2, co
     sc[S_RHO + (k & 1)] = rho;
     sc[S_WR] = wr;
   }
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+  for (int64_t i = solver_wid() * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     const double pi = r[i] + beta * (p[i] - omega * v[i]);
     p[i] = pi;
     phat[i] = dinv[i] * pi;
@@ -770,7 +778,7 @@ __global__ __launch_bounds__(kRT) void bi_s(int64_t n, int k, double* __restrict
   }
   const double alpha = sc[S_RHO + (k & 1)] / rv;
   if (blockIdx.x == 0 && threadIdx.x == 0) sc[S_ALPHA + (k & 1)] = alpha;
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+  for (int64_t i = solver_wid() * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     const double si = r[i] - alpha * v[i];
     r[i] = si;
     shat[i] = dinv[i] * si;
@@ -794,7 +802,7 @@ __global__ __launch_bounds__(kRT) void bi_update(int64_t n, int k, double* __res
     sc[S_ITER] = (double)(k + 1);
   }
   double rho = 0.0, wr = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+  for (int64_t i = solver_wid() * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     d[i] = d[i] + alpha * phat[i] + omega * shat[i];
     const double ri = r[i] - omega * t[i];
     r[i] = ri;
@@ -830,7 +838,7 @@ __global__ __launch_bounds__(kRT) void true_residual(int64_t n, const int32_t* _
 
 __global__ __launch_bounds__(kRT) void norm2_kernel(int64_t n, const double* __restrict__ x, double* part) {
   double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT)
+  for (int64_t i = solver_wid() * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT)
     acc += x[i] * x[i];
   store_partial(acc, part + PT_RES * kMaxG);
 }
@@ -846,7 +854,7 @@ __global__ __launch_bounds__(kRT) void finish_sum(int G, const double* __restric
 // x = 0 (unless keep) with x_i = b_i on fixed rows
 __global__ __launch_bounds__(kRT) void init_x(int64_t n, const uint8_t* __restrict__ fixed,
                                               const double* __restrict__ b, double* __restrict__ x, int keep) {
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
+  for (int64_t i = solver_wid() * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT) {
     if (fixed[i]) x[i] = b[i];
     else if (!keep) x[i] = 0.0;
   }
@@ -854,7 +862,7 @@ __global__ __launch_bounds__(kRT) void init_x(int64_t n, const uint8_t* __restri
 
 template <class V>
 __global__ __launch_bounds__(kRT) void add_correction(int64_t n, const V* __restrict__ d, double* __restrict__ x) {
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT)
+  for (int64_t i = solver_wid() * kRT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kRT)
     x[i] = x[i] + (double)d[i];
 }
 
