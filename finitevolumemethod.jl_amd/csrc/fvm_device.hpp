@@ -35,6 +35,12 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   int per = nblocks / kXCD;
   return (bid % kXCD) * per + bid / kXCD;
 }
+// The fix-up kernels' workgroup -> range of fix nodes: XCD-contiguous when the grid is
+// a multiple of kXCD (the fix nodes go in the order of their first tile, so each XCD
+// then reads the partial sums its own tiles wrote), the identity otherwise.
+__device__ __forceinline__ int fix_wid() {
+  return gridDim.x % kXCD == 0 ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+}
 // Per-triangle record in the device SoA: field f of sorted triangle s lives at
 // props[f * T + s].  Fields follow TriangleProperties (geometry.jl:21-26).
 enum : int {

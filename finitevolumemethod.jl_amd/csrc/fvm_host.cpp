@@ -2,6 +2,7 @@
 #include "fvm_host.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -266,6 +267,7 @@ int fvmh_tiles_build_grouped(int64_t n_points, const double* x, const double* y,
   // 2. Greedy tiles along the curve.
   std::vector<int32_t> stamp(n_points, -1), tile_start;
   std::vector<int32_t> node_tiles(n_points, 0);
+  std::vector<int32_t> first_tile(n_points, INT32_MAX);  // the first (lowest) tile holding each node
   {
     int32_t tile = 0, count = 0, nodes = 0;
     tile_start.push_back(0);
@@ -282,7 +284,10 @@ int fvmh_tiles_build_grouped(int64_t n_points, const double* x, const double* y,
       }
       for (int v = 0; v < 3; ++v) {
         int32_t g = vert(s, v);
-        if (stamp[g] != tile) { stamp[g] = tile; ++node_tiles[g]; }
+        if (stamp[g] != tile) {
+          stamp[g] = tile;
+          if (++node_tiles[g] == 1) first_tile[g] = tile;
+        }
       }
       nodes += add;
       ++count;
@@ -307,11 +312,17 @@ int fvmh_tiles_build_grouped(int64_t n_points, const double* x, const double* y,
   for (int64_t e = 0; e < 2 * n_bedges; ++e) on_bnd[bedges[e]] = 1;
   for (int64_t e = 0; e < n_bedges; ++e)
     for (int v = 0; v < 3; ++v) on_bnd[tri[3 * (int64_t)badj[e] + v]] = 1;
-  // fix nodes: shared between tiles, on the boundary, or not a vertex
+  // fix nodes: shared between tiles, on the boundary, or not a vertex; ordered by their
+  // first tile (then by index), so that the fix-up, whose workgroups take contiguous
+  // ranges of fix nodes per XCD as the tile kernel's take contiguous ranges of tiles,
+  // reads partial sums the same XCD's tiles wrote
   std::vector<int32_t> fix_of(n_points, -1);
   std::vector<int32_t> fix_nodes;
   for (int64_t g = 0; g < n_points; ++g)
-    if (node_tiles[g] != 1 || on_bnd[g]) { fix_of[g] = (int32_t)fix_nodes.size(); fix_nodes.push_back((int32_t)g); }
+    if (node_tiles[g] != 1 || on_bnd[g]) fix_nodes.push_back((int32_t)g);
+  std::stable_sort(fix_nodes.begin(), fix_nodes.end(),
+                   [&](int32_t a, int32_t b) { return first_tile[a] < first_tile[b]; });
+  for (size_t f = 0; f < fix_nodes.size(); ++f) fix_of[fix_nodes[f]] = (int32_t)f;
   T->n_fix = (int32_t)fix_nodes.size();
   T->fix_nodes = xalloc<int32_t>(T->n_fix);
   std::copy(fix_nodes.begin(), fix_nodes.end(), T->fix_nodes);

@@ -24,8 +24,10 @@
 // triangle and node sums taken four incidences per round, in the VJP too; 7: the
 // four flux copies stored as rows of TILE + 1 doubles, so the stores are free of LDS
 // bank conflicts; 8: the tile kernel's loads branch-free, at 32-bit offsets; 9: the
-// VJP tile kernel's too, with its incidence block staged in LDS).
-#define FVM_RHS_REV 9
+// VJP tile kernel's too, with its incidence block staged in LDS; 10: fix nodes in the
+// order of their first tile, so that a tile's partial sums are contiguous, and the
+// fix-up workgroups XCD-contiguous).
+#define FVM_RHS_REV 10
 #ifndef FVM_GEO_LEVEL
 #define FVM_GEO_LEVEL 4
 #endif
@@ -63,6 +65,8 @@ int euler_step(fvm_ctx* c, double* d_u, const double* tile_dt, const double* dtv
   } while (0)
 
 inline int xcd_grid(int n) { return kXCD * ((n + kXCD - 1) / kXCD); }
+// workgroups of 256 threads of the fix-up kernels over n_fix nodes (fix_wid)
+inline int fix_grid(int n_fix) { return xcd_grid((n_fix + 255) / 256); }
 
 // Runs launch(d_flag) on stream st with a cleared device flag, waits, and returns
 // FVM_ERR_ARG ("<what>: a queried edge or triangle is not in the mesh") when a

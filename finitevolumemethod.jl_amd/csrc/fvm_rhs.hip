@@ -107,7 +107,7 @@ void launch_tiles(const KArgs& a, int tile_tris, hipStream_t s) {
 template <int FK, int MODE>
 void launch_fix(const KArgs& a, hipStream_t s) {
   if (a.n_fix > 0)
-    hipLaunchKernelGGL((fix_kernel<FK, MODE>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL((fix_kernel<FK, MODE>), dim3((unsigned)fix_grid(a.n_fix)), dim3(256), 0,
                        s, a);
 }
 
@@ -125,7 +125,7 @@ int launch_jit(const fvm_ctx* c, const KArgs& a, int mode, hipStream_t s) {
   FVM_CHECK(mode == M_RHS || mode == M_JVP, "closures compiled at run time: RHS and JVP only");
   int rc = jit_launch(mode == M_RHS ? j->tile_rhs : j->tile_jvp, tile_grid(a, j->block).x, j->block, &a, s);
   if (rc == FVM_OK && a.n_fix > 0)
-    rc = jit_launch(mode == M_RHS ? j->fix_rhs : j->fix_jvp, (unsigned)((a.n_fix + 255) / 256), 256, &a, s);
+    rc = jit_launch(mode == M_RHS ? j->fix_rhs : j->fix_jvp, (unsigned)fix_grid(a.n_fix), 256, &a, s);
   return rc;
 }
 
@@ -150,7 +150,7 @@ int launch_vjp_fk(const KArgs& a, int tile_tris, hipStream_t s) {
   else
     hipLaunchKernelGGL((vjp_tile_kernel<FK, 512, 2>), tile_grid(a, 512), dim3(512), 0, s, a);
   if (a.n_fix > 0)
-    hipLaunchKernelGGL((vjp_fix_kernel<FK>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL((vjp_fix_kernel<FK>), dim3((unsigned)fix_grid(a.n_fix)), dim3(256), 0, s,
                        a);
   FVM_HIP(hipGetLastError());
   return FVM_OK;
@@ -441,7 +441,7 @@ int fvm_rhs_timed(fvm_ctx* c, const double* d_u, double* d_du, double t, fvm_str
   switch (c->flux_kind) {
     case FVM_FLUX_JIT:
       if (rc == FVM_OK && a.n_fix > 0)
-        rc = jit_launch(c->jit->fix_rhs, (unsigned)((a.n_fix + 255) / 256), 256, &a, st);
+        rc = jit_launch(c->jit->fix_rhs, (unsigned)fix_grid(a.n_fix), 256, &a, st);
       break;
     case FVM_FLUX_D_POLY_U: launch_fix<FVM_FLUX_D_POLY_U, M_RHS>(a, st); break;
     case FVM_FLUX_D_NODAL: launch_fix<FVM_FLUX_D_NODAL, M_RHS>(a, st); break;
@@ -488,7 +488,7 @@ int fvm_vjp(fvm_ctx* c, const double* d_u, const double* d_w, double* d_ubar, do
     case FVM_FLUX_JIT: {
       const JitKernels* j = c->jit;
       int rc = jit_launch(j->vjp_tile, tile_grid(a, j->block).x, j->block, &a, st);
-      if (rc == FVM_OK && a.n_fix > 0) rc = jit_launch(j->vjp_fix, (unsigned)((a.n_fix + 255) / 256), 256, &a, st);
+      if (rc == FVM_OK && a.n_fix > 0) rc = jit_launch(j->vjp_fix, (unsigned)fix_grid(a.n_fix), 256, &a, st);
       return rc;
     }
     case FVM_FLUX_D_POLY_U: return launch_vjp_fk<FVM_FLUX_D_POLY_U>(a, c->m->tile_tris, st);

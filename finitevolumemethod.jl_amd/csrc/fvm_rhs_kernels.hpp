@@ -491,7 +491,7 @@ __global__ __launch_bounds__(BLOCK) void tile_kernel(KArgs a) {
 // dependency (nearly every node has at most four).
 template <int FK, int MODE>
 __global__ __launch_bounds__(256) void fix_kernel(KArgs a) {
-  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int fi = fix_wid() * blockDim.x + threadIdx.x;
   if (fi >= a.n_fix) return;
   // the node's static data in fix order: one coalesced record, one gather (u)
   const int n = a.n_fix;
@@ -818,7 +818,7 @@ __global__ __launch_bounds__(BLOCK) void vjp_tile_kernel(KArgs a) {
 
 template <int FK>
 __global__ __launch_bounds__(256) void vjp_fix_kernel(KArgs a) {
-  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int fi = fix_wid() * blockDim.x + threadIdx.x;
   if (fi >= a.n_fix) return;
   const int g = a.fix_nodes[fi];
   double acc = 0.0, accl = 0.0;

@@ -396,7 +396,7 @@ __global__ __launch_bounds__(BLOCK) void sys_tile_kernel(SysArgs a) {
 
 template <int NV, int JV>
 __global__ __launch_bounds__(256) void sys_fix_kernel(SysArgs a) {
-  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int fi = fix_wid() * blockDim.x + threadIdx.x;
   if (fi >= a.n_fix) return;
   // the node's static data in fix order: one coalesced record, gathers of u only
   const int n = a.n_fix;
@@ -546,7 +546,7 @@ int launch_sys(const fvm_system* s, const SysArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((sys_tile_kernel<NV, JV, 512, 2>), dim3((unsigned)xcd_grid(a.n_tiles + nb512)), dim3(512), 0,
                        st, a);
   if (a.n_fix > 0)
-    hipLaunchKernelGGL((sys_fix_kernel<NV, JV>), dim3((unsigned)((a.n_fix + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((sys_fix_kernel<NV, JV>), dim3((unsigned)fix_grid(a.n_fix)), dim3(256), 0, st, a);
   FVM_HIP(hipGetLastError());
   return FVM_OK;
 }
